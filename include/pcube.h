@@ -1,0 +1,140 @@
+/*
+ * libpcube: MI355X (gfx950) engine for the p=619 doubly-focused pseudocube
+ * sieve of AlexanderEastwood/psuedocubes, behind a plain C ABI.
+ *
+ * Conventions
+ *   - Device buffers are passed as uint64_t addresses obtained from
+ *     pcube_alloc (or offsets into them); host buffers as ordinary pointers.
+ *   - Every call returns a status code; pcube_last_error() describes the most
+ *     recent failure.  The Python layer maps the codes to the reference's
+ *     exceptions: OVERFLOW -> OverflowError, INVALID -> ValueError,
+ *     UNDERFLOW/RANGE -> ArithmeticError, NOMEM -> MemoryError.
+ *   - All work runs on one library-owned HIP stream of default priority; each
+ *     entry point returns after its results are on the host unless it says
+ *     otherwise.  One host thread drives the library (calls are serialised).
+ *   - 128-bit quantities are split into (low, high) uint64 limbs.
+ */
+#ifndef PCUBE_H
+#define PCUBE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    PCUBE_OK = 0,
+    PCUBE_OVERFLOW = 1,   /* output or focus buffer overflow                 */
+    PCUBE_INVALID = 2,    /* invalid arguments or geometry                    */
+    PCUBE_UNDERFLOW = 3,  /* unsigned underflow flagged while building x      */
+    PCUBE_NOMEM = 4,      /* device memory cap exceeded                        */
+    PCUBE_DEVICE = 5,     /* HIP runtime failure                               */
+    PCUBE_RANGE = 6       /* an emitted x fell outside (lo, hi]               */
+};
+
+const char *pcube_last_error(void);
+
+/* --- runtime: replaces pseudocube/gpu.py:38-67 (Runtime) ----------------- */
+
+/* Number of visible devices (replaces runtime.cp.cuda.runtime.getDeviceCount(),
+ * tools/reproduce.py:36). */
+int pcube_device_count(int *count);
+
+/* Select the device, create the stream, and fix the memory cap as
+ * min(memory_cap, total*fraction, free - 256 MiB) (pseudocube/gpu.py:42-45).
+ * cap/free/total may be NULL. */
+int pcube_init(int device, uint64_t memory_cap, double fraction, uint64_t *cap, uint64_t *free_bytes,
+               uint64_t *total_bytes);
+
+/* Reserved and peak device bytes of the capped pool (pseudocube/gpu.py:49-52). */
+int pcube_memory(uint64_t *reserved, uint64_t *peak, uint64_t *cap);
+
+/* Device name and compute-unit count. */
+int pcube_device_info(char *name, size_t name_len, int *compute_units);
+
+/* Kernel attributes for the compile record (pseudocube/gpu.py:54-61).
+ * index in [0, count); returns INVALID past the end. */
+int pcube_kernel_info(int index, char *name, size_t name_len, int *registers, int *local_bytes,
+                      int *shared_bytes);
+
+/* --- device buffers (the capped pool behind every array) ------------------ */
+int pcube_alloc(uint64_t bytes, uint64_t *address);
+int pcube_free(uint64_t address);
+int pcube_upload(uint64_t dst, const void *src, uint64_t bytes);
+int pcube_download(void *dst, uint64_t src, uint64_t bytes);
+
+/* --- focus wheels: replaces pseudocube/gpu.py:70-114 (GPUFocus) ----------- */
+
+/* Build the two sorted half arrays C and D of one focus on the device
+ * (partial CRT sums, pseudocube/gpu.py:82-92) plus a bucket directory of C.
+ * terms: concatenated CRT terms of the C rings then the D rings;
+ * sizes: ring sizes, nrings_c for C followed by nrings_d for D. */
+int pcube_focus_create(uint64_t modulus, const uint64_t *terms, const uint32_t *sizes, uint32_t nrings_c,
+                       uint32_t nrings_d, int *focus, uint64_t *nc, uint64_t *nd);
+int pcube_focus_arrays(int focus, uint64_t *c_address, uint64_t *d_address);
+int pcube_focus_destroy(int focus);
+
+/* Sorted focus coordinates in [low, high) written to the device buffer out
+ * (capacity cap), count in *count.  Same codes as the reference's CPU
+ * producer (review/focus_window_cpu.cpp:4-6): 0 ok, 1 overflow (*count is the
+ * full size, > cap), 2 invalid.  Replaces GPUFocus.window (pseudocube/gpu.py:94-114). */
+int pcube_focus_window(int focus, uint64_t low, uint64_t high, uint64_t out, uint64_t cap, uint64_t *count);
+
+/* np.searchsorted(sorted, queries, 'left') on the device; host queries/out
+ * (pseudocube/batch_experiment.py:41-42). */
+int pcube_searchsorted(uint64_t sorted, uint64_t n, const uint64_t *queries, uint64_t nq, uint64_t *out);
+
+/* --- matcher: replaces pseudocube/batch_experiment.py:79-254 -------------- */
+
+/* Residual moduli in filter order (cubic ascending, then the rest,
+ * pseudocube/gpu.py:134); at most 128. */
+int pcube_matcher_create(uint64_t A, uint64_t B, uint64_t lo0, uint64_t lo1, uint64_t hi0, uint64_t hi1,
+                         const uint32_t *moduli, uint32_t nres, int *matcher);
+int pcube_matcher_destroy(int matcher);
+
+/* Match one batch of up to 64 subtiles (or one subtile with ntiles = 1).
+ *   us/vs        device addresses of sorted u and v coordinates
+ *   v_first      tile_id = (v - v_first) / tile_width must be < ntiles
+ *   counters     host, ntiles * (ng + 3) uint64: [valid, group_1..group_ng,
+ *                emitted, underflow] per subtile, ng = ceil(nres / 4)
+ *                (pseudocube/batch_experiment.py:207-234)
+ *   values       host, min(emitted, cap) (low, high) limb pairs per subtile,
+ *                concatenated in subtile order; values_cap pairs available
+ *   engine       0 = residue-bitset engine, 1 = direct per-position engine
+ *   seconds      host, 2 doubles: prefilter build, match kernel (may be NULL)
+ * Returns OK, OVERFLOW (a subtile emitted more than cap; counters valid),
+ * UNDERFLOW, RANGE, INVALID. */
+int pcube_match(int matcher, uint64_t us, uint64_t nu, uint64_t vs, uint64_t nv, uint64_t v_first,
+                uint64_t tile_width, uint32_t ntiles, uint64_t cap, int engine, uint64_t *counters,
+                uint64_t *values, uint64_t values_cap, double *seconds);
+
+/* --- lease pipeline: replaces pseudocube/optimized.py:81-90 ---------------- */
+
+/* Run every compute batch of a lease with one lease-wide window per focus
+ * and no host synchronisation between batches.
+ *   tiles        host, ntiles_total * 4 uint64: (v_first, v_end, u_min, u_end)
+ *                per subtile, exactly tile_ranges() of each batch, batches
+ *                consecutive; batch_tiles[b] subtiles belong to batch b
+ *   bounds       host out, ntiles_total * 4 uint64: (a, b, c, d) subtile
+ *                bounds into the batch's u and v windows
+ *   batch_sizes  host out, nbatches * 2 uint64: (nu, nv) per batch
+ *   counters     host out, ntiles_total * (ng + 3)
+ *   values       host out as in pcube_match, all batches concatenated
+ *   seconds      host out, 4 doubles: windows, prefilter, match, total
+ * Returns OK or the first failing code (OVERFLOW for any window or subtile
+ * overflow, after which counters describe what happened). */
+int pcube_lease(int matcher, int focus_u, int focus_v, uint32_t nbatches, const uint32_t *batch_tiles,
+                const uint64_t *tiles, uint64_t tile_width, uint64_t cap, uint64_t window_cap, uint64_t *bounds,
+                uint64_t *batch_sizes, uint64_t *counters, uint64_t *values, uint64_t values_cap,
+                double *seconds);
+
+/* --- self test: replaces the `arithmetic` kernel (kernels/pseudocube.cuh:6-11) */
+/* out[3i..3i+2] = (low, high, borrow) of in[4i]*in[4i+1] - in[4i+2]*in[4i+3]; host arrays. */
+int pcube_arithmetic(const uint64_t *in, uint64_t *out, uint64_t count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

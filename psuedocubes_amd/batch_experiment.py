@@ -1,0 +1,191 @@
+"""Batched subtile engines on MI355X: geometry, the residue-bitset matcher, comparison.
+
+Drop-in for the reference's `pseudocube/batch_experiment.py` (`Prepared`
+16-24, `tile_ranges` 27-32, `prepare_arrays` 35-48, `prepare` 51-57,
+`FusedMatcher` 79-167, `MaskedMatcher` 170-254, `compare` 257-263).
+`MaskedMatcher` runs libpcube's bitset kernel; `FusedMatcher` runs the direct
+per-position kernel with the same per-subtile counters.  Both produce the same
+row, key for key.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+import time
+from typing import Any
+
+import numpy as np
+
+from . import native
+from .accounting import digest
+from .focus import Config, Tile
+from .gpu import DeviceArray, GPUFocus, MatchEngine, Runtime
+from .predicate import is_perfect_cube, local3, verify
+
+
+@dataclass
+class Prepared:
+    first: int
+    end: int
+    width: int
+    us: Any
+    vs: Any
+    bounds: list[tuple[int, int, int, int]]
+    prep_seconds: float
+
+
+def tile_ranges(config: Config, first: int, end: int, width: int) -> list[tuple[int, int, int, int]]:
+    """(v_first, v_end, u_min, u_end) for each subtile of [first, end) (`pseudocube/batch_experiment.py:27-32`)."""
+    Tile(first, end).validate(config)
+    if width <= 0 or (end - first + width - 1) // width > 64:
+        raise ValueError('experiment allows at most 64 bounded subtiles')
+    out = []
+    for v in range(first, end, width):
+        stop = min(v + width, end)
+        out.append((v, stop, config.u_interval(v)[0], config.u_interval(stop - 1)[1]))
+    return out
+
+
+def searchsorted(array: DeviceArray, queries: list[int]) -> list[int]:
+    q = np.array(queries, dtype=np.uint64)
+    out = np.zeros(q.size, dtype=np.uint64)
+    native.call('pcube_searchsorted', array.address, array.size, native.ptr64(q), q.size, native.ptr64(out))
+    return [int(x) for x in out]
+
+
+def prepare_arrays(runtime: Runtime, config: Config, first: int, end: int, width: int,
+                   us: DeviceArray, vs: DeviceArray, started: float) -> Prepared:
+    ranges = tile_ranges(config, first, end, width)
+    ub = searchsorted(us, [bound for _, _, lo, hi in ranges for bound in (lo, hi)])
+    vb = searchsorted(vs, [bound for lo, hi, _, _ in ranges for bound in (lo, hi)])
+    bounds = [(ub[2 * i], ub[2 * i + 1], vb[2 * i], vb[2 * i + 1]) for i in range(len(ranges))]
+    for a, b, c, d in bounds:
+        if (b - a) * (d - c) > 50_000_000:
+            raise OverflowError('per-subtile candidate opportunity limit exceeded')
+    runtime.check()
+    return Prepared(first, end, width, us, vs, bounds, time.perf_counter() - started)
+
+
+def prepare(runtime: Runtime, config: Config, af: GPUFocus, bf: GPUFocus,
+            first: int, end: int, width: int = 100000000) -> Prepared:
+    started = time.perf_counter()
+    ranges = tile_ranges(config, first, end, width)
+    us = af.window(ranges[0][2], ranges[-1][3])
+    vs = bf.window(first, end)
+    return prepare_arrays(runtime, config, first, end, width, us, vs, started)
+
+
+def batch_row(config: Config, filter_groups: list[list[int]], ranges: list[tuple[int, int, int, int]],
+              bounds: list[tuple[int, int, int, int]], nv: int, counters: np.ndarray, per_tile: list[list[int]],
+              output_cap: int, timings: dict[str, float], started: float) -> dict[str, Any]:
+    """Host collection and verification of one batch (`pseudocube/batch_experiment.py:121-167`).
+
+    Raises ArithmeticError on underflow, duplicates, a value failing the
+    exact predicate or landing in the wrong subtile; OverflowError when a
+    subtile emitted more than output_cap values.
+    """
+    ng = len(filter_groups)
+    transfer_started = time.perf_counter()
+    if np.any(counters[:, ng + 2]):
+        raise ArithmeticError('unsigned underflow')
+    if np.any(counters[:, ng + 1] > output_cap):
+        raise OverflowError('fused tile output overflow; no truncated coverage')
+    all_values: list[int] = []
+    for values in per_tile:
+        all_values.extend(sorted(values))
+    transfer_seconds = time.perf_counter() - transfer_started + timings.get('transfer_seconds', 0.0)
+    verification_started = time.perf_counter()
+    if len(all_values) != len(set(all_values)):
+        raise ArithmeticError('duplicate canonical output')
+    if any(not config.lo < n <= config.hi or not local3(n, config.p) for n in all_values):
+        raise ArithmeticError('GPU/CPU disagreement')
+    hits = [verify(n, config.p) for n in all_values if not is_perfect_cube(n)]
+    if any(not row['valid_pseudocube'] for row in hits):
+        raise ArithmeticError('independent witness verification failed')
+    scopes = []
+    position = 0
+    for row, (first, end, _, _) in enumerate(ranges):
+        number = int(counters[row, ng + 1])
+        values = sorted(all_values[position:position + number])
+        if any(not first <= config.canonical(n)[1] < end for n in values):
+            raise ArithmeticError('output assigned to wrong subtile')
+        scopes.append(dict(first=str(first), end=str(end), valid_focused_candidates=int(counters[row, 0]),
+                           group_survivors=[int(n) for n in counters[row, 1:ng + 1]], local_survivors=number,
+                           output_sha256=digest([str(n) for n in values])))
+        position += number
+    verification_seconds = time.perf_counter() - verification_started
+    groups = [int(n) for n in counters[:, 1:ng + 1].sum(axis=0)]
+    return dict(status='PASS', host_verified=True, focus_u=sum(b - a for a, b, _, _ in bounds), focus_v=int(nv),
+                cartesian_opportunities=sum((b - a) * (d - c) for a, b, c, d in bounds),
+                valid_focused_candidates=sum(int(n) for n in counters[:, 0]),
+                genuine_cubes=len(all_values) - len(hits), local_survivors=len(all_values), cube_checks=len(all_values),
+                kernel_seconds=timings.get('kernel_seconds', 0.0), transfer_seconds=transfer_seconds,
+                verification_seconds=verification_seconds,
+                focus_enumeration_seconds=timings.get('focus_enumeration_seconds', 0.0), subtile_scopes=scopes,
+                filter_groups=filter_groups, group_survivors=groups, all_values=[str(n) for n in all_values],
+                cpu_confirmed_hits=hits, overflow_splits=[],
+                elapsed_seconds=time.perf_counter() - started + timings.get('focus_enumeration_seconds', 0.0),
+                claim='Isolated validation only; no production coverage.')
+
+
+class FusedMatcher:
+    """Per-subtile counters over a whole batch, direct per-position engine (`pseudocube/batch_experiment.py:79-167`)."""
+    engine_id = 1
+    layout = 'fused-batch'
+    kernel_name = 'match_direct'
+
+    def __init__(self, runtime: Runtime, config: Config) -> None:
+        self.runtime, self.config = runtime, config
+        self.engine = MatchEngine(runtime, config)
+        self.filter_groups = self.engine.filter_groups
+        self.kernel = runtime.compile(self.kernel_name)
+        self.sm_count = runtime.compute_units
+        self.prefilter_seconds = 0.0
+
+    def run(self, batch: Prepared, output_cap: int = 8192) -> dict[str, Any]:
+        started = time.perf_counter()
+        ranges = tile_ranges(self.config, batch.first, batch.end, batch.width)
+        if len(ranges) != len(batch.bounds) or not 0 < output_cap <= 8192:
+            raise ValueError('invalid prepared geometry or output capacity')
+        counters, per_tile, seconds = self.engine.run(batch.us, batch.vs, batch.first, batch.width, len(ranges),
+                                                      output_cap, self.engine_id, overflow_ok=True)
+        self.prefilter_seconds = seconds[0]
+        timings = dict(kernel_seconds=seconds[1], focus_enumeration_seconds=batch.prep_seconds)
+        row = batch_row(self.config, self.filter_groups, ranges, batch.bounds, int(batch.vs.size), counters,
+                        per_tile, output_cap, timings, started)
+        row['kernel_registers'] = int(self.kernel['registers'])
+        row['kernel_local_bytes'] = int(self.kernel['local_bytes'])
+        row['matching_layout'] = self.layout
+        return row
+
+
+class MaskedMatcher(FusedMatcher):
+    """Residue-bitset engine (`pseudocube/batch_experiment.py:170-254`).
+
+    The first four residual filters are intersected as 64-bit residue
+    bitsets before any 128-bit candidate is built.
+    """
+    engine_id = 0
+    layout = 'residue-bitset-batch'
+    kernel_name = 'match'
+
+    def __init__(self, runtime: Runtime, config: Config) -> None:
+        super().__init__(runtime, config)
+        if not self.engine.order:
+            raise ValueError('bitset experiment requires at least one residual filter')
+        runtime.compile('mask_build')
+
+    def run(self, batch: Prepared, output_cap: int = 8192) -> dict[str, Any]:
+        result = super().run(batch, output_cap)
+        result['prefilter_build_seconds'] = self.prefilter_seconds
+        return result
+
+
+def compare(reference: dict[str, Any], actual: dict[str, Any]) -> None:
+    """Raise unless two batch rows agree on every compared field (`pseudocube/batch_experiment.py:257-263`)."""
+    for key in ('focus_u', 'focus_v', 'cartesian_opportunities', 'valid_focused_candidates', 'genuine_cubes',
+                'local_survivors', 'cube_checks', 'filter_groups', 'group_survivors', 'subtile_scopes',
+                'cpu_confirmed_hits'):
+        if reference[key] != actual[key]:
+            raise ArithmeticError('batch/reference mismatch: ' + key)
+    if sorted(reference['all_values']) != sorted(actual['all_values']):
+        raise ArithmeticError('complete output mismatch')

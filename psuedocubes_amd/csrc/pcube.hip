@@ -1,0 +1,902 @@
+// libpcube: host side of the C ABI declared in include/pcube.h.
+//
+// One process drives one device through one default-priority stream.  Device
+// memory comes from a capped, size-class caching pool so that per-batch
+// buffers do not call hipMalloc in steady state.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/pcube.h"
+#include "pcube_device.h"
+
+using namespace pcube;
+
+namespace {
+
+std::recursive_mutex g_mu;
+std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(call)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(PCUBE_DEVICE, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                                 \
+    } while (0)
+
+#define TRY(expr)                  \
+    do {                           \
+        int rc_ = (expr);          \
+        if (rc_ != PCUBE_OK) return rc_; \
+    } while (0)
+
+struct Context {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    uint64_t cap = 0, reserved = 0, peak = 0;
+    std::unordered_map<void*, uint64_t> live;      // address -> class size
+    std::multimap<uint64_t, void*> cached;          // class size -> free block
+    int compute_units = 0;
+    char name[256] = {0};
+} g;
+
+uint64_t size_class(uint64_t bytes) {
+    if (bytes == 0) bytes = 1;
+    if (bytes <= (1ull << 20)) {
+        uint64_t c = 256;
+        while (c < bytes) c <<= 1;
+        return c;
+    }
+    const uint64_t mb = 1ull << 20;
+    return (bytes + mb - 1) / mb * mb;
+}
+
+void release_cache() {
+    for (auto& kv : g.cached) {
+        (void)hipFree(kv.second);
+        g.reserved -= kv.first;
+    }
+    g.cached.clear();
+}
+
+int dev_alloc(uint64_t bytes, void** out) {
+    if (g.device < 0) return fail(PCUBE_INVALID, "pcube_init has not been called");
+    const uint64_t c = size_class(bytes);
+    auto it = g.cached.find(c);
+    if (it != g.cached.end()) {
+        *out = it->second;
+        g.cached.erase(it);
+        g.live[*out] = c;
+        return PCUBE_OK;
+    }
+    if (g.reserved + c > g.cap) release_cache();
+    if (g.reserved + c > g.cap)
+        return fail(PCUBE_NOMEM, "device memory cap exceeded: %llu reserved + %llu requested > cap %llu",
+                    (unsigned long long)g.reserved, (unsigned long long)c, (unsigned long long)g.cap);
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, c);
+    if (e != hipSuccess) {
+        release_cache();
+        e = hipMalloc(&p, c);
+        if (e != hipSuccess) return fail(PCUBE_NOMEM, "hipMalloc(%llu): %s", (unsigned long long)c, hipGetErrorString(e));
+    }
+    g.reserved += c;
+    g.peak = std::max(g.peak, g.reserved);
+    g.live[p] = c;
+    *out = p;
+    return PCUBE_OK;
+}
+
+int dev_free(void* p) {
+    if (!p) return PCUBE_OK;
+    auto it = g.live.find(p);
+    if (it == g.live.end()) return fail(PCUBE_INVALID, "free of an address the pool does not own");
+    g.cached.emplace(it->second, p);
+    g.live.erase(it);
+    return PCUBE_OK;
+}
+
+// A grow-only device buffer owned by a library object.
+struct Buffer {
+    void* p = nullptr;
+    uint64_t bytes = 0;
+    int ensure(uint64_t need) {
+        if (need <= bytes && p) return PCUBE_OK;
+        if (p) dev_free(p);
+        p = nullptr;
+        bytes = 0;
+        TRY(dev_alloc(need, &p));
+        bytes = need;
+        return PCUBE_OK;
+    }
+    void release() {
+        if (p) dev_free(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T>
+    T* as() const { return (T*)p; }
+};
+
+uint32_t blocks_for(uint64_t threads, uint32_t per_block) {
+    return (uint32_t)std::max<uint64_t>(1, (threads + per_block - 1) / per_block);
+}
+
+// ---------------------------------------------------------------------------
+// Scan and bucket sort helpers
+
+struct ScanWork {
+    Buffer partial, partial_off, partial_total;
+};
+
+// out[0..n) = exclusive scan of in, out[n] = total.  n <= 4096 * 4096.
+int scan(const uint32_t* in, uint32_t* out, uint64_t n, ScanWork& w) {
+    const uint64_t nblk = (n + kScanSegment - 1) / kScanSegment;
+    if (nblk == 0 || nblk > (uint64_t)kScanSegment) return fail(PCUBE_INVALID, "scan size %llu unsupported", (unsigned long long)n);
+    TRY(w.partial.ensure(nblk * 4));
+    TRY(w.partial_off.ensure(nblk * 4 + 4));
+    TRY(w.partial_total.ensure(4));
+    hipLaunchKernelGGL(k_scan_local, dim3((uint32_t)nblk), dim3(kScanThreads), 0, g.stream, in, out, n,
+                       w.partial.as<uint32_t>());
+    if (nblk == 1) {
+        HIP_TRY(hipMemcpyAsync(out + n, w.partial.as<uint32_t>(), 4, hipMemcpyDeviceToDevice, g.stream));
+        return PCUBE_OK;
+    }
+    hipLaunchKernelGGL(k_scan_local, dim3(1), dim3(kScanThreads), 0, g.stream, w.partial.as<uint32_t>(),
+                       w.partial_off.as<uint32_t>(), nblk, w.partial_total.as<uint32_t>());
+    hipLaunchKernelGGL(k_scan_add, dim3(blocks_for(n, 256)), dim3(256), 0, g.stream, out, n,
+                       w.partial_off.as<uint32_t>());
+    HIP_TRY(hipMemcpyAsync(out + n, w.partial_total.as<uint32_t>(), 4, hipMemcpyDeviceToDevice, g.stream));
+    return PCUBE_OK;
+}
+
+int fill32(uint32_t* p, uint64_t n, uint32_t v) {
+    if (!n) return PCUBE_OK;
+    hipLaunchKernelGGL(k_fill_u32, dim3(std::min<uint64_t>(blocks_for(n, 256), 4096)), dim3(256), 0, g.stream, p, n, v);
+    return PCUBE_OK;
+}
+int fill64(uint64_t* p, uint64_t n, uint64_t v) {
+    if (!n) return PCUBE_OK;
+    hipLaunchKernelGGL(k_fill_u64, dim3(std::min<uint64_t>(blocks_for(n, 256), 4096)), dim3(256), 0, g.stream, p, n, v);
+    return PCUBE_OK;
+}
+
+// Smallest shift with ((span - 1) >> shift) + 1 <= target buckets.
+uint32_t bucket_shift(uint64_t span, uint64_t target) {
+    if (target < 1) target = 1;
+    uint32_t s = 0;
+    while (s < 63 && ((span - 1) >> s) + 1 > target) ++s;
+    return s;
+}
+
+struct BucketWork {
+    Buffer hist, offsets;
+    ScanWork scan;
+};
+
+// Sort n device values lying in [low, low + span) into out.
+int bucket_sort(const uint64_t* in, uint64_t n, uint64_t low, uint64_t span, uint64_t* out, BucketWork& w) {
+    if (!n) return PCUBE_OK;
+    uint64_t target = 1;
+    while (target * 2 <= n / 2 && target < (1ull << 22)) target *= 2;
+    const uint32_t shift = bucket_shift(span, target);
+    const uint64_t nb = ((span - 1) >> shift) + 1;
+    TRY(w.hist.ensure(nb * 8));
+    TRY(w.offsets.ensure(nb * 4 + 4));
+    uint32_t* hist = w.hist.as<uint32_t>();
+    uint32_t* cursor = hist + nb;
+    TRY(fill32(hist, 2 * nb, 0));
+    hipLaunchKernelGGL(k_bucket_count, dim3(blocks_for(n, 256)), dim3(256), 0, g.stream, in, n, low, shift, hist);
+    TRY(scan(hist, w.offsets.as<uint32_t>(), nb, w.scan));
+    hipLaunchKernelGGL(k_bucket_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, g.stream, in, n, low, shift,
+                       w.offsets.as<uint32_t>(), cursor, out);
+    hipLaunchKernelGGL(k_bucket_sort, dim3(blocks_for(nb, 256)), dim3(256), 0, g.stream, out,
+                       w.offsets.as<uint32_t>(), nb, n);
+    return PCUBE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Focus objects
+
+struct Focus {
+    uint64_t M = 0, nc = 0, nd = 0, nbk = 0;
+    uint32_t dshift = 0;
+    Buffer C, D, dir;
+    BucketWork work;        // window buckets and scan scratch
+    Buffer total;           // device copy of the last window size (uint32 at offsets[nb])
+    ~Focus() {
+        C.release(); D.release(); dir.release();
+        work.hist.release(); work.offsets.release(); total.release();
+        work.scan.partial.release(); work.scan.partial_off.release(); work.scan.partial_total.release();
+    }
+};
+std::vector<std::unique_ptr<Focus>> g_focus;
+
+Focus* focus_at(int id) {
+    if (id < 0 || id >= (int)g_focus.size() || !g_focus[id]) return nullptr;
+    return g_focus[id].get();
+}
+
+int build_half(uint64_t M, const uint64_t* terms, const uint32_t* sizes, uint32_t nrings, Buffer& out, uint64_t* n,
+               BucketWork& work) {
+    uint64_t count = 1, nterms = 0;
+    for (uint32_t k = 0; k < nrings; ++k) {
+        if (sizes[k] == 0) return fail(PCUBE_INVALID, "empty ring");
+        count *= sizes[k];
+        nterms += sizes[k];
+        if (count > (1ull << 31)) return fail(PCUBE_INVALID, "focus half too large");
+    }
+    *n = count;
+    Buffer dterms, dsizes, unsorted;
+    TRY(dterms.ensure(std::max<uint64_t>(8, nterms * 8)));
+    TRY(dsizes.ensure(std::max<uint64_t>(4, nrings * 4)));
+    TRY(unsorted.ensure(count * 8));
+    TRY(out.ensure(count * 8));
+    if (nterms) HIP_TRY(hipMemcpyAsync(dterms.p, terms, nterms * 8, hipMemcpyHostToDevice, g.stream));
+    if (nrings) HIP_TRY(hipMemcpyAsync(dsizes.p, sizes, nrings * 4, hipMemcpyHostToDevice, g.stream));
+    hipLaunchKernelGGL(k_partial, dim3(blocks_for(count, 256)), dim3(256), 0, g.stream, dterms.as<uint64_t>(),
+                       dsizes.as<uint32_t>(), nrings, M, count, unsorted.as<uint64_t>());
+    TRY(bucket_sort(unsorted.as<uint64_t>(), count, 0, M, out.as<uint64_t>(), work));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    dterms.release(); dsizes.release(); unsorted.release();
+    return PCUBE_OK;
+}
+
+// Enqueue a window [low, high) of focus f into out (capacity cap).  The
+// device total is left at f.work.offsets[nb]; *nb_out receives nb.
+int window_enqueue(Focus& f, uint64_t low, uint64_t high, uint64_t* out, uint64_t cap, uint64_t* nb_out) {
+    const uint64_t per0 = low / f.M, per1 = (high - 1) / f.M, nper = per1 - per0 + 1;
+    if (nper > (1u << 20)) return fail(PCUBE_INVALID, "window spans too many periods");
+    const uint64_t span = high - low;
+    const double expected = (double)span / (double)f.M * (double)f.nc * (double)f.nd;
+    uint64_t target = 1;
+    while ((double)(target * 2) <= expected / 2 && target < (1ull << 22)) target *= 2;
+    const uint32_t bshift = bucket_shift(span, target);
+    const uint64_t nb = ((span - 1) >> bshift) + 1;
+    TRY(f.work.hist.ensure(nb * 8));
+    TRY(f.work.offsets.ensure(nb * 4 + 4));
+    uint32_t* hist = f.work.hist.as<uint32_t>();
+    uint32_t* cursor = hist + nb;
+    uint32_t* offsets = f.work.offsets.as<uint32_t>();
+    TRY(fill32(hist, 2 * nb, 0));
+    WindowArgs w;
+    w.C = f.C.as<uint64_t>();
+    w.D = f.D.as<uint64_t>();
+    w.dir = f.dir.as<uint32_t>();
+    w.nc = f.nc; w.nd = f.nd; w.nbk = f.nbk; w.M = f.M;
+    w.low = low; w.high = high; w.per0 = per0; w.cap = cap;
+    w.dshift = f.dshift; w.nper = (uint32_t)nper;
+    w.nchunks = (uint32_t)((f.nd + kChunk - 1) / kChunk);
+    w.bshift = bshift;
+    const uint64_t threads = (uint64_t)w.nper * w.nchunks;
+    hipLaunchKernelGGL(k_window<false>, dim3(blocks_for(threads, 256)), dim3(256), 0, g.stream, w, hist,
+                       (const uint32_t*)offsets, cursor, out);
+    TRY(scan(hist, offsets, nb, f.work.scan));
+    hipLaunchKernelGGL(k_window<true>, dim3(blocks_for(threads, 256)), dim3(256), 0, g.stream, w, hist,
+                       (const uint32_t*)offsets, cursor, out);
+    hipLaunchKernelGGL(k_bucket_sort, dim3(blocks_for(nb, 256)), dim3(256), 0, g.stream, out, offsets, nb, cap);
+    *nb_out = nb;
+    return PCUBE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Matchers
+
+struct Matcher {
+    uint64_t A = 0, B = 0;
+    U128 lo{0, 0}, hi{0, 0};
+    uint32_t nres = 0, nbits = 0, ncache = 0, ng = 0, allow_words = 0;
+    uint32_t row_base[kMaxBitsetFilters] = {0, 0, 0, 0};
+    uint32_t mask_rows = 0;
+    std::vector<Modulus> mods;
+    Buffer d_mods, d_allow;
+    Buffer rows, masks, rho, counters, list, list_count, error;
+    ~Matcher() {
+        d_mods.release(); d_allow.release(); rows.release(); masks.release(); rho.release();
+        counters.release(); list.release(); list_count.release(); error.release();
+    }
+};
+std::vector<std::unique_ptr<Matcher>> g_matchers;
+
+Matcher* matcher_at(int id) {
+    if (id < 0 || id >= (int)g_matchers.size() || !g_matchers[id]) return nullptr;
+    return g_matchers[id].get();
+}
+
+bool cubic_residue(uint32_t q, uint32_t a) {
+    for (uint32_t t = 1; t < q; ++t)
+        if ((uint64_t)t * t % q * t % q == a) return true;
+    return false;
+}
+bool allowed_residue(uint32_t q, uint32_t a) {
+    if (q == 9) return a == 1 || a == 8;
+    if (a == 0) return false;
+    if (q % 3 != 1) return true;
+    return cubic_residue(q, a);
+}
+
+// Enqueue the kernels of one batch; counters must be zeroed by the caller.
+int match_enqueue(Matcher& mt, const uint64_t* us, uint64_t nu, const uint64_t* vs, uint64_t nv, uint64_t v_first,
+                  uint64_t tile_width, uint32_t ntiles, uint64_t cap, int engine, uint64_t* counters,
+                  uint64_t tile_base, uint64_t list_cap, hipEvent_t ev_pre, hipEvent_t ev_match) {
+    if (nu >= (1ull << 32) - 64 || nv >= (1ull << 32)) return fail(PCUBE_INVALID, "batch too large");
+    const uint32_t words = (uint32_t)((nu + 63) / 64);
+    TRY(mt.rows.ensure(std::max<uint64_t>(1, nv) * sizeof(Row)));
+    if (engine == 0) {
+        TRY(mt.masks.ensure(std::max<uint64_t>(8, (uint64_t)mt.mask_rows * std::max(1u, words) * 8)));
+        TRY(mt.rho.ensure(std::max<uint64_t>(8, nu * mt.ncache * 2)));
+    }
+    MatchArgs m;
+    memset(&m, 0, sizeof m);
+    m.us = us; m.vs = vs;
+    m.rows = mt.rows.as<Row>();
+    m.masks = mt.masks.as<uint64_t>();
+    m.rho = mt.rho.as<uint16_t>();
+    m.mods = mt.d_mods.as<Modulus>();
+    m.allow = mt.d_allow.as<uint32_t>();
+    m.counters = counters;
+    m.list = mt.list.as<uint64_t>();
+    m.list_count = mt.list_count.as<uint64_t>();
+    m.error = mt.error.as<uint32_t>();
+    m.A = mt.A; m.B = mt.B; m.v_first = v_first; m.tile_width = tile_width; m.cap = cap;
+    m.list_cap = list_cap; m.tile_base = tile_base;
+    m.lo = mt.lo; m.hi = mt.hi;
+    m.nu = (uint32_t)nu; m.nv = (uint32_t)nv; m.words = words;
+    m.nres = mt.nres; m.nbits = engine == 0 ? mt.nbits : 0; m.ncache = engine == 0 ? mt.ncache : 0;
+    m.ng = mt.ng; m.ntiles = ntiles; m.allow_words = mt.allow_words;
+    for (int k = 0; k < kMaxBitsetFilters; ++k) m.row_base[k] = mt.row_base[k];
+    if (nv == 0) return PCUBE_OK;
+    // k_rows always computes v mod q for the bitset filters of the matcher
+    MatchArgs mr = m;
+    mr.nbits = mt.nbits;
+    hipLaunchKernelGGL(k_rows, dim3(blocks_for(nv, 256)), dim3(256), 0, g.stream, mr);
+    if (engine == 0 && nu) {
+        uint32_t chunk_base[kMaxBitsetFilters + 1] = {0, 0, 0, 0, 0};
+        for (uint32_t k = 0; k < kMaxBitsetFilters; ++k)
+            chunk_base[k + 1] = chunk_base[k] + (k < mt.nbits ? (mt.mods[k].q + kWave - 1) / kWave : 0);
+        const uint32_t chunks = chunk_base[mt.nbits];
+        const uint64_t waves = (uint64_t)chunks * words;
+        hipLaunchKernelGGL(k_masks, dim3(blocks_for(waves * kWave, 256)), dim3(256), 0, g.stream, m, chunks,
+                           chunk_base[1], mt.nbits > 2 ? chunk_base[2] : 0xffffffffu,
+                           mt.nbits > 3 ? chunk_base[3] : 0xffffffffu);
+        if (mt.ncache)
+            hipLaunchKernelGGL(k_rho, dim3(blocks_for(nu * mt.ncache, 256)), dim3(256), 0, g.stream, m);
+    }
+    if (ev_pre) HIP_TRY(hipEventRecord(ev_pre, g.stream));
+    const uint64_t rows_per_wave = std::max<uint64_t>(1, (nv + 8191) / 8192);
+    m.rows_per_wave = (uint32_t)rows_per_wave;
+    const uint64_t waves = (nv + rows_per_wave - 1) / rows_per_wave;
+    uint64_t blocks = (waves + kMatchWaves - 1) / kMatchWaves;
+    if (engine == 0) {
+        blocks = (blocks + 7) / 8 * 8;
+        hipLaunchKernelGGL(k_match, dim3((uint32_t)blocks), dim3(kMatchThreads), 0, g.stream, m);
+    } else {
+        hipLaunchKernelGGL(k_match_direct, dim3((uint32_t)blocks), dim3(kMatchThreads), 0, g.stream, m);
+    }
+    if (ev_match) HIP_TRY(hipEventRecord(ev_match, g.stream));
+    return PCUBE_OK;
+}
+
+// Copy the tagged value list to the host grouped by tile, in tile order.
+int collect_values(Matcher& mt, uint64_t ntiles_total, const std::vector<uint64_t>& host_counters, uint64_t cap,
+                   uint64_t list_cap, uint64_t* values, uint64_t values_cap, int* status) {
+    uint64_t listed = 0;
+    HIP_TRY(hipMemcpyAsync(&listed, mt.list_count.p, 8, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    uint64_t expected = 0;
+    std::vector<uint64_t> offset(ntiles_total + 1, 0);
+    for (uint64_t t = 0; t < ntiles_total; ++t) {
+        const uint64_t emitted = host_counters[t * (mt.ng + 3) + mt.ng + 1];
+        offset[t + 1] = offset[t] + std::min(emitted, cap);
+    }
+    expected = offset[ntiles_total];
+    if (listed != expected || listed > list_cap) {
+        *status = PCUBE_OVERFLOW;
+        fail(PCUBE_OVERFLOW, "survivor list holds %llu values, counters say %llu (list capacity %llu)",
+             (unsigned long long)listed, (unsigned long long)expected, (unsigned long long)list_cap);
+        return PCUBE_OK;
+    }
+    if (expected > values_cap) return fail(PCUBE_OVERFLOW, "host value buffer too small");
+    if (!listed) return PCUBE_OK;
+    std::vector<uint64_t> triples(listed * 3);
+    HIP_TRY(hipMemcpyAsync(triples.data(), mt.list.p, listed * 24, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    std::vector<uint64_t> cursor(offset.begin(), offset.end() - 1);
+    for (uint64_t e = 0; e < listed; ++e) {
+        const uint64_t t = triples[3 * e];
+        if (t >= ntiles_total || cursor[t] >= offset[t + 1]) return fail(PCUBE_INVALID, "corrupt survivor list");
+        values[2 * cursor[t]] = triples[3 * e + 1];
+        values[2 * cursor[t] + 1] = triples[3 * e + 2];
+        ++cursor[t];
+    }
+    return PCUBE_OK;
+}
+
+// Translate counters and error bits into the status code.
+int batch_status(const Matcher& mt, const std::vector<uint64_t>& c, uint64_t ntiles, uint64_t cap, uint32_t err) {
+    if (err & 1u) return fail(PCUBE_INVALID, "a v coordinate lies outside the subtiles");
+    for (uint64_t t = 0; t < ntiles; ++t)
+        if (c[t * (mt.ng + 3) + mt.ng + 2]) return fail(PCUBE_UNDERFLOW, "unsigned underflow flagged");
+    if (err & 2u) return fail(PCUBE_RANGE, "an emitted value lies outside (lo, hi]");
+    for (uint64_t t = 0; t < ntiles; ++t)
+        if (c[t * (mt.ng + 3) + mt.ng + 1] > cap) return fail(PCUBE_OVERFLOW, "output overflow in subtile %llu", (unsigned long long)t);
+    return PCUBE_OK;
+}
+
+struct Kernel {
+    const char* name;
+    const void* fn;
+};
+const Kernel kKernels[] = {
+    {"partial", (const void*)k_partial},
+    {"window_count", (const void*)k_window<false>},
+    {"window_emit", (const void*)k_window<true>},
+    {"bucket_sort", (const void*)k_bucket_sort},
+    {"scan_local", (const void*)k_scan_local},
+    {"rows", (const void*)k_rows},
+    {"mask_build", (const void*)k_masks},
+    {"residues", (const void*)k_rho},
+    {"match", (const void*)k_match},
+    {"match_direct", (const void*)k_match_direct},
+    {"arithmetic", (const void*)k_arithmetic},
+};
+
+double now_seconds() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+
+extern "C" {
+
+const char* pcube_last_error(void) { return g_err.c_str(); }
+
+int pcube_device_count(int* count) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) n = 0;
+    *count = n;
+    return PCUBE_OK;
+}
+
+int pcube_init(int device, uint64_t memory_cap, double fraction, uint64_t* cap, uint64_t* free_bytes,
+               uint64_t* total_bytes) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    if (g.device >= 0 && g.device != device) return fail(PCUBE_INVALID, "already initialised on device %d", g.device);
+    if (g.device < 0) {
+        HIP_TRY(hipSetDevice(device));
+        HIP_TRY(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+        hipDeviceProp_t prop;
+        HIP_TRY(hipGetDeviceProperties(&prop, device));
+        g.compute_units = prop.multiProcessorCount;
+        snprintf(g.name, sizeof g.name, "%s (%s)", prop.name, prop.gcnArchName);
+        g.device = device;
+    }
+    size_t fr = 0, tot = 0;
+    HIP_TRY(hipMemGetInfo(&fr, &tot));
+    const double frac_cap = (double)tot * fraction;
+    const int64_t headroom = (int64_t)fr - (int64_t)(256ull << 20) + (int64_t)g.reserved;
+    int64_t c = (int64_t)std::min<double>((double)memory_cap, frac_cap);
+    c = std::min<int64_t>(c, headroom);
+    if (c <= 0) return fail(PCUBE_NOMEM, "no permitted device memory");
+    g.cap = (uint64_t)c;
+    if (cap) *cap = g.cap;
+    if (free_bytes) *free_bytes = fr;
+    if (total_bytes) *total_bytes = tot;
+    return PCUBE_OK;
+}
+
+int pcube_memory(uint64_t* reserved, uint64_t* peak, uint64_t* cap) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    if (reserved) *reserved = g.reserved;
+    if (peak) *peak = g.peak;
+    if (cap) *cap = g.cap;
+    return PCUBE_OK;
+}
+
+int pcube_device_info(char* name, size_t name_len, int* compute_units) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    if (g.device < 0) return fail(PCUBE_INVALID, "pcube_init has not been called");
+    if (name && name_len) snprintf(name, name_len, "%s", g.name);
+    if (compute_units) *compute_units = g.compute_units;
+    return PCUBE_OK;
+}
+
+int pcube_kernel_info(int index, char* name, size_t name_len, int* registers, int* local_bytes, int* shared_bytes) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    const int n = (int)(sizeof kKernels / sizeof kKernels[0]);
+    if (index < 0 || index >= n) return fail(PCUBE_INVALID, "kernel index out of range");
+    if (name && name_len) snprintf(name, name_len, "%s", kKernels[index].name);
+    hipFuncAttributes attr;
+    memset(&attr, 0, sizeof attr);
+    HIP_TRY(hipFuncGetAttributes(&attr, kKernels[index].fn));
+    if (registers) *registers = attr.numRegs;
+    if (local_bytes) *local_bytes = (int)attr.localSizeBytes;
+    if (shared_bytes) *shared_bytes = (int)attr.sharedSizeBytes;
+    return PCUBE_OK;
+}
+
+int pcube_alloc(uint64_t bytes, uint64_t* address) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    void* p = nullptr;
+    TRY(dev_alloc(bytes, &p));
+    *address = (uint64_t)p;
+    return PCUBE_OK;
+}
+
+int pcube_free(uint64_t address) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    if (g.stream) HIP_TRY(hipStreamSynchronize(g.stream));
+    return dev_free((void*)address);
+}
+
+int pcube_upload(uint64_t dst, const void* src, uint64_t bytes) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    if (!bytes) return PCUBE_OK;
+    HIP_TRY(hipMemcpyAsync((void*)dst, src, bytes, hipMemcpyHostToDevice, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    return PCUBE_OK;
+}
+
+int pcube_download(void* dst, uint64_t src, uint64_t bytes) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    if (!bytes) return PCUBE_OK;
+    HIP_TRY(hipMemcpyAsync(dst, (const void*)src, bytes, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    return PCUBE_OK;
+}
+
+int pcube_focus_create(uint64_t modulus, const uint64_t* terms, const uint32_t* sizes, uint32_t nrings_c,
+                       uint32_t nrings_d, int* focus, uint64_t* nc, uint64_t* nd) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    if (g.device < 0) return fail(PCUBE_INVALID, "pcube_init has not been called");
+    if (modulus == 0 || modulus >= (1ull << 63)) return fail(PCUBE_INVALID, "focus modulus must lie in [1, 2^63)");
+    auto f = std::make_unique<Focus>();
+    f->M = modulus;
+    uint64_t cterms = 0;
+    for (uint32_t k = 0; k < nrings_c; ++k) cterms += sizes[k];
+    TRY(build_half(modulus, terms, sizes, nrings_c, f->C, &f->nc, f->work));
+    TRY(build_half(modulus, terms + cterms, sizes + nrings_c, nrings_d, f->D, &f->nd, f->work));
+    // bucket directory over C: about one entry per bucket
+    uint32_t log_nc = 0;
+    while ((2ull << log_nc) <= f->nc) ++log_nc;
+    uint32_t bits = 0;
+    while (bits < 64 && ((modulus - 1) >> bits) != 0) ++bits;
+    f->dshift = bits > log_nc ? bits - log_nc : 0;
+    f->nbk = ((modulus - 1) >> f->dshift) + 1;
+    TRY(f->dir.ensure((f->nbk + 1) * 4));
+    hipLaunchKernelGGL(k_directory, dim3(blocks_for(f->nbk + 1, 256)), dim3(256), 0, g.stream, f->C.as<uint64_t>(),
+                       f->nc, f->dshift, f->nbk, f->dir.as<uint32_t>());
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    *nc = f->nc;
+    *nd = f->nd;
+    g_focus.push_back(std::move(f));
+    *focus = (int)g_focus.size() - 1;
+    return PCUBE_OK;
+}
+
+int pcube_focus_arrays(int focus, uint64_t* c_address, uint64_t* d_address) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Focus* f = focus_at(focus);
+    if (!f) return fail(PCUBE_INVALID, "unknown focus");
+    *c_address = (uint64_t)f->C.p;
+    *d_address = (uint64_t)f->D.p;
+    return PCUBE_OK;
+}
+
+int pcube_focus_destroy(int focus) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    if (!focus_at(focus)) return fail(PCUBE_INVALID, "unknown focus");
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    g_focus[focus].reset();
+    return PCUBE_OK;
+}
+
+int pcube_focus_window(int focus, uint64_t low, uint64_t high, uint64_t out, uint64_t cap, uint64_t* count) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    *count = 0;
+    Focus* f = focus_at(focus);
+    if (!f || low > high) return fail(PCUBE_INVALID, "invalid focus window");
+    if (low == high) return PCUBE_OK;
+    uint64_t nb = 0;
+    TRY(window_enqueue(*f, low, high, (uint64_t*)out, cap, &nb));
+    uint32_t total = 0;
+    HIP_TRY(hipMemcpyAsync(&total, f->work.offsets.as<uint32_t>() + nb, 4, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    *count = total;
+    if (total > cap) return fail(PCUBE_OVERFLOW, "focus buffer overflow: %u coordinates, capacity %llu", total, (unsigned long long)cap);
+    return PCUBE_OK;
+}
+
+int pcube_searchsorted(uint64_t sorted, uint64_t n, const uint64_t* queries, uint64_t nq, uint64_t* out) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    if (!nq) return PCUBE_OK;
+    void *dq = nullptr, *dout = nullptr;
+    TRY(dev_alloc(nq * 8, &dq));
+    int rc = dev_alloc(nq * 8, &dout);
+    if (rc) { dev_free(dq); return rc; }
+    HIP_TRY(hipMemcpyAsync(dq, queries, nq * 8, hipMemcpyHostToDevice, g.stream));
+    hipLaunchKernelGGL(k_searchsorted, dim3(blocks_for(nq, 256)), dim3(256), 0, g.stream, (const uint64_t*)sorted, n,
+                       (const uint64_t*)dq, nq, (uint64_t*)dout);
+    HIP_TRY(hipMemcpyAsync(out, dout, nq * 8, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    dev_free(dq);
+    dev_free(dout);
+    return PCUBE_OK;
+}
+
+int pcube_matcher_create(uint64_t A, uint64_t B, uint64_t lo0, uint64_t lo1, uint64_t hi0, uint64_t hi1,
+                         const uint32_t* moduli, uint32_t nres, int* matcher) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    if (g.device < 0) return fail(PCUBE_INVALID, "pcube_init has not been called");
+    if (nres == 0 || nres > (uint32_t)kMaxResiduals) return fail(PCUBE_INVALID, "between 1 and 128 residual filters required");
+    auto mt = std::make_unique<Matcher>();
+    mt->A = A; mt->B = B;
+    mt->lo = U128{lo0, lo1};
+    mt->hi = U128{hi0, hi1};
+    mt->nres = nres;
+    mt->nbits = std::min<uint32_t>(nres, kMaxBitsetFilters);
+    mt->ncache = std::min<uint32_t>(nres - mt->nbits, kCachedResidues);
+    mt->ng = (nres + 3) / 4;
+    std::vector<uint32_t> allow;
+    uint32_t bit = 0;
+    for (uint32_t k = 0; k < nres; ++k) {
+        const uint32_t q = moduli[k];
+        if (q < 2 || q >= 65536) return fail(PCUBE_INVALID, "modulus %u out of range", q);
+        Modulus m;
+        m.q = q;
+        m.inv = (uint32_t)((1ull << 32) / q);
+        m.r32 = (uint32_t)((1ull << 32) % q);
+        m.r64 = (uint32_t)(((unsigned __int128)1 << 64) % q);
+        m.a = (uint32_t)(A % q);
+        m.b = (uint32_t)(B % q);
+        m.bit = bit;
+        m.pad = 0;
+        if (k < mt->nbits && m.a == 0) return fail(PCUBE_INVALID, "bitset modulus %u divides A", q);
+        for (uint32_t a = 0; a < q; ++a) {
+            const uint32_t idx = bit + a;
+            if (allow.size() <= idx / 32) allow.resize(idx / 32 + 1, 0);
+            if (allowed_residue(q, a)) allow[idx / 32] |= 1u << (idx % 32);
+        }
+        bit += q;
+        mt->mods.push_back(m);
+    }
+    if (allow.size() > (size_t)kMaxAllowWords) return fail(PCUBE_INVALID, "allowed-residue table too large");
+    mt->allow_words = (uint32_t)allow.size();
+    uint32_t rb = 0;
+    for (uint32_t k = 0; k < kMaxBitsetFilters; ++k) {
+        mt->row_base[k] = rb;
+        if (k < mt->nbits) rb += mt->mods[k].q;
+    }
+    mt->mask_rows = rb;
+    TRY(mt->d_mods.ensure(nres * sizeof(Modulus)));
+    TRY(mt->d_allow.ensure(allow.size() * 4));
+    TRY(mt->list_count.ensure(8));
+    TRY(mt->error.ensure(4));
+    HIP_TRY(hipMemcpyAsync(mt->d_mods.p, mt->mods.data(), nres * sizeof(Modulus), hipMemcpyHostToDevice, g.stream));
+    HIP_TRY(hipMemcpyAsync(mt->d_allow.p, allow.data(), allow.size() * 4, hipMemcpyHostToDevice, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    g_matchers.push_back(std::move(mt));
+    *matcher = (int)g_matchers.size() - 1;
+    return PCUBE_OK;
+}
+
+int pcube_matcher_destroy(int matcher) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    if (!matcher_at(matcher)) return fail(PCUBE_INVALID, "unknown matcher");
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    g_matchers[matcher].reset();
+    return PCUBE_OK;
+}
+
+int pcube_match(int matcher, uint64_t us, uint64_t nu, uint64_t vs, uint64_t nv, uint64_t v_first,
+                uint64_t tile_width, uint32_t ntiles, uint64_t cap, int engine, uint64_t* counters, uint64_t* values,
+                uint64_t values_cap, double* seconds) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Matcher* mt = matcher_at(matcher);
+    if (!mt) return fail(PCUBE_INVALID, "unknown matcher");
+    if (ntiles == 0 || ntiles > 65536 || tile_width == 0 || (engine != 0 && engine != 1))
+        return fail(PCUBE_INVALID, "invalid tile geometry or engine");
+    const uint64_t stride = mt->ng + 3, nc = (uint64_t)ntiles * stride;
+    const uint64_t list_cap = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)ntiles * cap, 1ull << 22));
+    TRY(mt->counters.ensure(nc * 8));
+    TRY(mt->list.ensure(list_cap * 24));
+    TRY(fill64(mt->counters.as<uint64_t>(), nc, 0));
+    TRY(fill64(mt->list_count.as<uint64_t>(), 1, 0));
+    TRY(fill32(mt->error.as<uint32_t>(), 1, 0));
+    hipEvent_t e0, e1, e2;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventCreate(&e2));
+    HIP_TRY(hipEventRecord(e0, g.stream));
+    int rc = match_enqueue(*mt, (const uint64_t*)us, nu, (const uint64_t*)vs, nv, v_first, tile_width, ntiles, cap,
+                           engine, mt->counters.as<uint64_t>(), 0, list_cap, e1, e2);
+    if (rc) return rc;
+    if (nv == 0) {
+        HIP_TRY(hipEventRecord(e1, g.stream));
+        HIP_TRY(hipEventRecord(e2, g.stream));
+    }
+    std::vector<uint64_t> host(nc);
+    uint32_t err = 0;
+    HIP_TRY(hipMemcpyAsync(host.data(), mt->counters.p, nc * 8, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipMemcpyAsync(&err, mt->error.p, 4, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    if (seconds) {
+        float a = 0, b = 0;
+        HIP_TRY(hipEventElapsedTime(&a, e0, e1));
+        HIP_TRY(hipEventElapsedTime(&b, e1, e2));
+        seconds[0] = a / 1000.0;
+        seconds[1] = b / 1000.0;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    memcpy(counters, host.data(), nc * 8);
+    rc = batch_status(*mt, host, ntiles, cap, err);
+    if (rc) return rc;
+    int status = PCUBE_OK;
+    TRY(collect_values(*mt, ntiles, host, cap, list_cap, values, values_cap, &status));
+    return status;
+}
+
+int pcube_lease(int matcher, int focus_u, int focus_v, uint32_t nbatches, const uint32_t* batch_tiles,
+                const uint64_t* tiles, uint64_t tile_width, uint64_t cap, uint64_t window_cap, uint64_t* bounds,
+                uint64_t* batch_sizes, uint64_t* counters, uint64_t* values, uint64_t values_cap, double* seconds) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    const double t_start = now_seconds();
+    Matcher* mt = matcher_at(matcher);
+    Focus* fu = focus_at(focus_u);
+    Focus* fv = focus_at(focus_v);
+    if (!mt || !fu || !fv || nbatches == 0) return fail(PCUBE_INVALID, "invalid lease arguments");
+    uint64_t ntot = 0;
+    for (uint32_t b = 0; b < nbatches; ++b) {
+        if (batch_tiles[b] == 0 || batch_tiles[b] > 64) return fail(PCUBE_INVALID, "1..64 subtiles per batch");
+        ntot += batch_tiles[b];
+    }
+    const uint64_t u_low = tiles[2], u_high = tiles[(ntot - 1) * 4 + 3];
+    const uint64_t v_low = tiles[0], v_high = tiles[(ntot - 1) * 4 + 1];
+    if (u_low > u_high || v_low >= v_high) return fail(PCUBE_INVALID, "invalid lease geometry");
+    // lease-wide windows
+    Buffer lus, lvs, queries, positions;
+    TRY(lus.ensure(std::max<uint64_t>(8, window_cap * 8)));
+    TRY(lvs.ensure(std::max<uint64_t>(8, window_cap * 8)));
+    hipEvent_t ev[4];
+    for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+    HIP_TRY(hipEventRecord(ev[0], g.stream));
+    uint64_t nbu = 0, nbv = 0;
+    uint32_t totals[2] = {0, 0};
+    if (u_high > u_low) {
+        TRY(window_enqueue(*fu, u_low, u_high, lus.as<uint64_t>(), window_cap, &nbu));
+        HIP_TRY(hipMemcpyAsync(&totals[0], fu->work.offsets.as<uint32_t>() + nbu, 4, hipMemcpyDeviceToHost, g.stream));
+    }
+    TRY(window_enqueue(*fv, v_low, v_high, lvs.as<uint64_t>(), window_cap, &nbv));
+    HIP_TRY(hipMemcpyAsync(&totals[1], fv->work.offsets.as<uint32_t>() + nbv, 4, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    if (totals[0] > window_cap || totals[1] > window_cap)
+        return fail(PCUBE_OVERFLOW, "lease focus window overflow (%u, %u > %llu)", totals[0], totals[1], (unsigned long long)window_cap);
+    // subtile bounds inside the lease arrays
+    // (u_min, u_end) queries into the u window, (v_first, v_end) into the v window
+    std::vector<uint64_t> uq(ntot * 2), vq(ntot * 2);
+    for (uint64_t t = 0; t < ntot; ++t) {
+        uq[2 * t] = tiles[t * 4 + 2];
+        uq[2 * t + 1] = tiles[t * 4 + 3];
+        vq[2 * t] = tiles[t * 4 + 0];
+        vq[2 * t + 1] = tiles[t * 4 + 1];
+    }
+    TRY(queries.ensure(ntot * 32));
+    TRY(positions.ensure(ntot * 32));
+    HIP_TRY(hipMemcpyAsync(queries.p, uq.data(), ntot * 16, hipMemcpyHostToDevice, g.stream));
+    HIP_TRY(hipMemcpyAsync(queries.as<uint64_t>() + 2 * ntot, vq.data(), ntot * 16, hipMemcpyHostToDevice, g.stream));
+    hipLaunchKernelGGL(k_searchsorted, dim3(blocks_for(2 * ntot, 256)), dim3(256), 0, g.stream, lus.as<uint64_t>(),
+                       (uint64_t)totals[0], queries.as<uint64_t>(), 2 * ntot, positions.as<uint64_t>());
+    hipLaunchKernelGGL(k_searchsorted, dim3(blocks_for(2 * ntot, 256)), dim3(256), 0, g.stream, lvs.as<uint64_t>(),
+                       (uint64_t)totals[1], queries.as<uint64_t>() + 2 * ntot, 2 * ntot,
+                       positions.as<uint64_t>() + 2 * ntot);
+    std::vector<uint64_t> up(ntot * 2), vp(ntot * 2);
+    HIP_TRY(hipMemcpyAsync(up.data(), positions.p, ntot * 16, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipMemcpyAsync(vp.data(), positions.as<uint64_t>() + 2 * ntot, ntot * 16, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    HIP_TRY(hipEventRecord(ev[1], g.stream));
+    // per batch geometry
+    const uint64_t stride = mt->ng + 3;
+    const uint64_t list_cap = std::max<uint64_t>(1, std::min<uint64_t>(ntot * cap, 1ull << 22));
+    TRY(mt->counters.ensure(ntot * stride * 8));
+    TRY(mt->list.ensure(list_cap * 24));
+    TRY(fill64(mt->counters.as<uint64_t>(), ntot * stride, 0));
+    TRY(fill64(mt->list_count.as<uint64_t>(), 1, 0));
+    TRY(fill32(mt->error.as<uint32_t>(), 1, 0));
+    uint64_t t0 = 0;
+    int geometry_rc = PCUBE_OK;
+    for (uint32_t b = 0; b < nbatches; ++b) {
+        const uint64_t nt = batch_tiles[b], t1 = t0 + nt;
+        const uint64_t ua = up[2 * t0], ub = up[2 * (t1 - 1) + 1];
+        const uint64_t va = vp[2 * t0], vb = vp[2 * (t1 - 1) + 1];
+        batch_sizes[2 * b] = ub - ua;
+        batch_sizes[2 * b + 1] = vb - va;
+        for (uint64_t t = t0; t < t1; ++t) {
+            bounds[4 * t + 0] = up[2 * t] - ua;
+            bounds[4 * t + 1] = up[2 * t + 1] - ua;
+            bounds[4 * t + 2] = vp[2 * t] - va;
+            bounds[4 * t + 3] = vp[2 * t + 1] - va;
+            const uint64_t du = bounds[4 * t + 1] - bounds[4 * t], dv = bounds[4 * t + 3] - bounds[4 * t + 2];
+            if (du * dv > 50000000ull && geometry_rc == PCUBE_OK)
+                geometry_rc = fail(PCUBE_OVERFLOW, "per-subtile candidate opportunity limit exceeded");
+        }
+        if (geometry_rc == PCUBE_OK) {
+            TRY(match_enqueue(*mt, lus.as<uint64_t>() + ua, ub - ua, lvs.as<uint64_t>() + va, vb - va,
+                              tiles[4 * t0], tile_width, (uint32_t)nt, cap, 0,
+                              mt->counters.as<uint64_t>() + t0 * stride, t0, list_cap, nullptr, nullptr));
+        }
+        t0 = t1;
+    }
+    HIP_TRY(hipEventRecord(ev[2], g.stream));
+    std::vector<uint64_t> host(ntot * stride);
+    uint32_t err = 0;
+    HIP_TRY(hipMemcpyAsync(host.data(), mt->counters.p, ntot * stride * 8, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipMemcpyAsync(&err, mt->error.p, 4, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipEventRecord(ev[3], g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    memcpy(counters, host.data(), ntot * stride * 8);
+    if (seconds) {
+        float a = 0, m = 0;
+        HIP_TRY(hipEventElapsedTime(&a, ev[0], ev[1]));
+        HIP_TRY(hipEventElapsedTime(&m, ev[1], ev[2]));
+        seconds[0] = a / 1000.0;
+        seconds[1] = 0.0;
+        seconds[2] = m / 1000.0;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    lus.release(); lvs.release(); queries.release(); positions.release();
+    if (geometry_rc) return geometry_rc;
+    int rc = batch_status(*mt, host, ntot, cap, err);
+    if (rc) return rc;
+    int status = PCUBE_OK;
+    TRY(collect_values(*mt, ntot, host, cap, list_cap, values, values_cap, &status));
+    if (seconds) seconds[3] = now_seconds() - t_start;
+    return status;
+}
+
+int pcube_arithmetic(const uint64_t* in, uint64_t* out, uint64_t count) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    if (!count) return PCUBE_OK;
+    void *din = nullptr, *dout = nullptr;
+    TRY(dev_alloc(count * 32, &din));
+    int rc = dev_alloc(count * 24, &dout);
+    if (rc) { dev_free(din); return rc; }
+    HIP_TRY(hipMemcpyAsync(din, in, count * 32, hipMemcpyHostToDevice, g.stream));
+    hipLaunchKernelGGL(k_arithmetic, dim3(blocks_for(count, 256)), dim3(256), 0, g.stream, (const uint64_t*)din,
+                       (uint64_t*)dout, count);
+    HIP_TRY(hipMemcpyAsync(out, dout, count * 24, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    dev_free(din);
+    dev_free(dout);
+    return PCUBE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,600 @@
+#pragma once
+// HIP kernels for the p=619 doubly-focused pseudocube batch engine (gfx950).
+//
+// Reference behaviour restated here (file:line of AlexanderEastwood/psuedocubes):
+//   k_partial        CRT partial sums of a ring group        pseudocube/gpu.py:82-92
+//   k_window_*       focus coordinates in a window           pseudocube/gpu.py:19-35,94-114
+//   k_searchsorted   subtile bounds                          pseudocube/batch_experiment.py:39-43
+//   k_rows           per-v exact u bounds, 128-bit           pseudocube/batch_experiment.py:209-215
+//   k_masks          first-four residue bitsets              pseudocube/batch_experiment.py:184-194
+//   k_match          bitset intersection, residual chain,
+//                    group counts, survivor output           pseudocube/batch_experiment.py:198-235
+//   k_match_direct   per-position chain without bitsets      pseudocube/gpu.py:147-181
+//   k_arithmetic     wide-limb self test                     kernels/pseudocube.cuh:6-11
+#include "pcube_kernels.h"
+
+namespace pcube {
+
+// --------------------------------------------------------------------------
+// Fills and scans
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t value) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    for (; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = value;
+}
+__global__ void k_fill_u64(uint64_t* __restrict__ p, uint64_t n, uint64_t value) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    for (; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = value;
+}
+
+constexpr int kScanThreads = 1024;
+constexpr int kScanPerThread = 4;
+constexpr int kScanSegment = kScanThreads * kScanPerThread;
+
+// Exclusive scan of one value per thread across a 1024-thread block.
+__device__ uint32_t block_exclusive_scan(uint32_t value, uint32_t* total) {
+    __shared__ uint32_t wave_sums[kScanThreads / kWave];
+    const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    uint32_t inclusive = value;
+    for (int d = 1; d < kWave; d <<= 1) {
+        uint32_t up = __shfl_up(inclusive, d, kWave);
+        if (lane >= (uint32_t)d) inclusive += up;
+    }
+    if (lane == kWave - 1) wave_sums[wave] = inclusive;
+    __syncthreads();
+    if (wave == 0) {
+        uint32_t s = lane < kScanThreads / kWave ? wave_sums[lane] : 0;
+        uint32_t inc = s;
+        for (int d = 1; d < kWave; d <<= 1) {
+            uint32_t up = __shfl_up(inc, d, kWave);
+            if (lane >= (uint32_t)d) inc += up;
+        }
+        if (lane < kScanThreads / kWave) wave_sums[lane] = inc - s;
+        if (lane == kScanThreads / kWave - 1) *total = inc;
+    }
+    __syncthreads();
+    return wave_sums[wave] + inclusive - value;
+}
+
+// out[i] = sum of in[0..i) within this block's segment; partial[block] = segment sum.
+__global__ __launch_bounds__(kScanThreads) void k_scan_local(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                             uint64_t n, uint32_t* __restrict__ partial) {
+    __shared__ uint32_t total;
+    uint64_t base = (uint64_t)blockIdx.x * kScanSegment + (uint64_t)threadIdx.x * kScanPerThread;
+    uint32_t v[kScanPerThread], sum = 0;
+    for (int e = 0; e < kScanPerThread; ++e) {
+        v[e] = base + e < n ? in[base + e] : 0;
+        sum += v[e];
+    }
+    uint32_t offset = block_exclusive_scan(sum, &total);
+    for (int e = 0; e < kScanPerThread; ++e) {
+        if (base + e < n) out[base + e] = offset;
+        offset += v[e];
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = total;
+}
+
+__global__ void k_scan_add(uint32_t* __restrict__ out, uint64_t n, const uint32_t* __restrict__ block_offsets) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i < n) out[i] += block_offsets[i / kScanSegment];
+}
+
+// --------------------------------------------------------------------------
+// Focus construction: partial CRT sums and bucket sort
+
+// out[i] = sum over rings k of terms_k[digit_k(i)] mod M, first ring least significant.
+__global__ void k_partial(const uint64_t* __restrict__ terms, const uint32_t* __restrict__ sizes, uint32_t nrings,
+                          uint64_t M, uint64_t n, uint64_t* __restrict__ out) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t rest = i, sum = 0;
+    uint32_t offset = 0;
+    for (uint32_t k = 0; k < nrings; ++k) {
+        uint32_t s = sizes[k];
+        uint64_t digit = rest % s;
+        rest /= s;
+        sum += terms[offset + digit];
+        if (sum >= M) sum -= M;
+        offset += s;
+    }
+    out[i] = sum;
+}
+
+__global__ void k_bucket_count(const uint64_t* __restrict__ values, uint64_t n, uint64_t low, uint32_t shift,
+                               uint32_t* __restrict__ hist) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i < n) atomicAdd(&hist[(values[i] - low) >> shift], 1u);
+}
+
+__global__ void k_bucket_scatter(const uint64_t* __restrict__ values, uint64_t n, uint64_t low, uint32_t shift,
+                                 const uint32_t* __restrict__ offsets, uint32_t* __restrict__ cursor,
+                                 uint64_t* __restrict__ out) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t y = values[i];
+    uint32_t b = uint32_t((y - low) >> shift);
+    out[offsets[b] + atomicAdd(&cursor[b], 1u)] = y;
+}
+
+// Insertion sort inside each bucket [offsets[b], offsets[b+1]) clamped to cap.
+__global__ void k_bucket_sort(uint64_t* __restrict__ out, const uint32_t* __restrict__ offsets, uint64_t nbuckets,
+                              uint64_t cap) {
+    uint64_t b = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (b >= nbuckets) return;
+    uint64_t s = offsets[b], e = offsets[b + 1];
+    if (e > cap) e = cap;
+    for (uint64_t i = s + 1; i < e; ++i) {
+        uint64_t key = out[i], j = i;
+        while (j > s && out[j - 1] > key) {
+            out[j] = out[j - 1];
+            --j;
+        }
+        out[j] = key;
+    }
+}
+
+// dir[k] = first index i with C[i] >= k << shift, for k in [0, nbk]; dir[nbk] = nc.
+__global__ void k_directory(const uint64_t* __restrict__ C, uint64_t nc, uint32_t shift, uint64_t nbk,
+                            uint32_t* __restrict__ dir) {
+    uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (k > nbk) return;
+    if (k == nbk) {
+        dir[k] = uint32_t(nc);
+        return;
+    }
+    uint64_t key = k << shift, l = 0, h = nc;
+    while (l < h) {
+        uint64_t m = l + ((h - l) >> 1);
+        if (C[m] < key) l = m + 1; else h = m;
+    }
+    dir[k] = uint32_t(l);
+}
+
+// --------------------------------------------------------------------------
+// Focus windows
+//
+// All coordinates y in [low, high) with (y mod M) = (c + d) mod M for c in the
+// sorted half C and d in the sorted half D.  Thread = (period, chunk of kChunk
+// consecutive D entries).  Per wrap w in {0, 1}, c must lie in
+// [a + wM - d, b + wM - d); walking the chunk's d values downwards moves that
+// interval upwards, so two pointers into C only advance.  The first pointer
+// position comes from the bucket directory of C.  Output is counted (pass 1)
+// then scattered into value buckets (pass 2) and sorted within buckets.
+
+constexpr int kChunk = 32;
+
+struct WindowArgs {
+    const uint64_t* C;
+    const uint64_t* D;
+    const uint32_t* dir;
+    uint64_t nc, nd, nbk, M, low, high, per0, cap;
+    uint32_t dshift, nper, nchunks, bshift;
+};
+
+__device__ __forceinline__ uint64_t lower_bound_dir(const WindowArgs& w, uint64_t x) {
+    uint64_t k = x >> w.dshift;
+    if (k >= w.nbk) return w.nc;
+    uint64_t i = w.dir[k];
+    while (i < w.nc && w.C[i] < x) ++i;
+    return i;
+}
+
+template <bool kEmit>
+__global__ void k_window(WindowArgs w, uint32_t* __restrict__ hist, const uint32_t* __restrict__ offsets,
+                         uint32_t* __restrict__ cursor, uint64_t* __restrict__ out) {
+    uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (t >= (uint64_t)w.nper * w.nchunks) return;
+    const uint32_t p = uint32_t(t / w.nchunks), chunk = uint32_t(t % w.nchunks);
+    const uint64_t base = (w.per0 + p) * w.M;
+    const uint64_t a = w.low > base ? w.low - base : 0;
+    const uint64_t b = (w.high - base) < w.M ? w.high - base : w.M;
+    const uint64_t j0 = (uint64_t)chunk * kChunk;
+    const uint64_t j1 = j0 + kChunk < w.nd ? j0 + kChunk : w.nd;
+    for (int wrap = 0; wrap < 2; ++wrap) {
+        const uint64_t lw = a + (wrap ? w.M : 0), hw = b + (wrap ? w.M : 0);
+        uint64_t pl = 0, pr = 0;
+        bool started = false;
+        for (uint64_t j = j1; j-- > j0;) {
+            const uint64_t d = w.D[j];
+            if (hw <= d) continue;
+            const uint64_t cl = lw > d ? lw - d : 0, ch = hw - d;
+            if (cl >= w.M) break;
+            if (!started) {
+                pl = lower_bound_dir(w, cl);
+                started = true;
+            } else {
+                while (pl < w.nc && w.C[pl] < cl) ++pl;
+            }
+            if (pr < pl) pr = pl;
+            while (pr < w.nc && w.C[pr] < ch) ++pr;
+            for (uint64_t i = pl; i < pr; ++i) {
+                const uint64_t y = w.C[i] + d - (wrap ? w.M : 0) + base;
+                const uint32_t bucket = uint32_t((y - w.low) >> w.bshift);
+                if (!kEmit) {
+                    atomicAdd(&hist[bucket], 1u);
+                } else {
+                    const uint64_t slot = (uint64_t)offsets[bucket] + atomicAdd(&cursor[bucket], 1u);
+                    if (slot < w.cap) out[slot] = y;
+                }
+            }
+        }
+    }
+}
+
+template __global__ void k_window<false>(WindowArgs, uint32_t*, const uint32_t*, uint32_t*, uint64_t*);
+template __global__ void k_window<true>(WindowArgs, uint32_t*, const uint32_t*, uint32_t*, uint64_t*);
+
+// np.searchsorted(sorted, queries, side='left')
+__global__ void k_searchsorted(const uint64_t* __restrict__ sorted, uint64_t n, const uint64_t* __restrict__ queries,
+                               uint64_t nq, uint64_t* __restrict__ out) {
+    uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (t >= nq) return;
+    uint64_t key = queries[t], l = 0, h = n;
+    while (l < h) {
+        uint64_t m = l + ((h - l) >> 1);
+        if (sorted[m] < key) l = m + 1; else h = m;
+    }
+    out[t] = l;
+}
+
+// --------------------------------------------------------------------------
+// Matching
+
+// Per-v row data computed once per batch.
+struct Row {
+    uint32_t left, right;   // u index range [left, right) with lo < u*B - v*A <= hi
+    uint32_t tile, flags;   // subtile index; bit0: a v outside [v_first, v_first + ntiles*width)
+    uint16_t r[kMaxBitsetFilters];  // v mod q for the bitset filters
+    uint64_t v;
+};
+
+struct MatchArgs {
+    const uint64_t* us;
+    const uint64_t* vs;
+    const Row* rows;
+    const uint64_t* masks;     // [sum of bitset q rows][words]
+    const uint16_t* rho;       // [nu][ncache] (u*B) mod q for the cached chain filters
+    const Modulus* mods;       // [nres] in filter order
+    const uint32_t* allow;     // allowed-residue bits, rows at Modulus::bit
+    uint64_t* counters;        // [ntiles][ng + 3]
+    uint64_t* list;            // emitted values as (global tile, low, high) triples
+    uint64_t* list_count;      // next free triple
+    uint32_t* error;           // bit0 invalid tile, bit1 value outside (lo, hi]
+    uint64_t A, B, v_first, tile_width, cap, list_cap, tile_base;
+    U128 lo, hi;
+    uint32_t nu, nv, words, nres, nbits, ncache, ng, ntiles, allow_words, rows_per_wave;
+    uint32_t row_base[kMaxBitsetFilters];
+};
+
+__global__ void k_rows(MatchArgs m) {
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= m.nv) return;
+    const uint64_t v = m.vs[j];
+    Row row;
+    row.v = v;
+    row.flags = 0;
+    uint64_t tile = 0;
+    if (v < m.v_first || (tile = (v - m.v_first) / m.tile_width) >= m.ntiles) {
+        row.flags = 1;
+        tile = 0;
+        atomicOr(m.error, 1u);
+    }
+    row.tile = uint32_t(tile);
+    const U128 shift = mul_wide(v, m.A);
+    const U128 lower = add_wide(shift, m.lo), upper = add_wide(shift, m.hi);
+    // first index with u*B > bound
+    uint32_t l = 0, h = m.nu;
+    while (l < h) {
+        uint32_t mid = l + ((h - l) >> 1);
+        if (less_wide(lower, mul_wide(m.us[mid], m.B))) h = mid; else l = mid + 1;
+    }
+    row.left = l;
+    h = m.nu;
+    while (l < h) {
+        uint32_t mid = l + ((h - l) >> 1);
+        if (less_wide(upper, mul_wide(m.us[mid], m.B))) h = mid; else l = mid + 1;
+    }
+    row.right = l;
+    if (row.flags) row.right = row.left;  // an invalid row contributes nothing
+    // u*B >= v*A holds for every u in range because u*B > v*A + lo; a violation
+    // (impossible for sorted input) is reported as the underflow counter.
+    if (row.left < row.right && less_wide(mul_wide(m.us[row.left], m.B), shift))
+        atomicAdd(&m.counters[(uint64_t)row.tile * (m.ng + 3) + m.ng + 2], 1ull);
+    for (uint32_t k = 0; k < kMaxBitsetFilters; ++k)
+        row.r[k] = k < m.nbits ? uint16_t(mod64(v, m.mods[k])) : 0;
+    ((Row*)m.rows)[j] = row;
+}
+
+// Residue bitsets for the first nbits filters.  One wave builds 64 rows
+// (residues r0..r0+63 of v mod q) of one 64-bit word: lane t tests
+// u index 64*w + t, and a wave ballot packs the 64 tests into the word.
+// Row r, bit t: allowed_q[(u*B - r*A) mod q].
+__global__ __launch_bounds__(256) void k_masks(MatchArgs m, uint32_t chunks_total, uint32_t chunk_base1,
+                                               uint32_t chunk_base2, uint32_t chunk_base3) {
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / kWave;
+    if (wave >= (uint64_t)chunks_total * m.words) return;
+    const uint32_t c = uint32_t(wave / m.words), w = uint32_t(wave % m.words);
+    const uint32_t k = c >= chunk_base3 ? 3 : c >= chunk_base2 ? 2 : c >= chunk_base1 ? 1 : 0;
+    const uint32_t kbase = k == 3 ? chunk_base3 : k == 2 ? chunk_base2 : k == 1 ? chunk_base1 : 0;
+    const Modulus mod = m.mods[k];
+    const uint32_t r0 = (c - kbase) * kWave;
+    const uint32_t i = w * kWave + lane;
+    const bool valid = i < m.nu;
+    // residue for row r0: (u*B - r0*A) mod q, then step by -A mod q per row
+    uint32_t res = 0;
+    if (valid) {
+        uint32_t rho = mod32(mod64(m.us[i], mod) * mod.b, mod);
+        uint32_t sub = mod32(r0 * mod.a, mod);   // r0 < 2^16, a < 2^16
+        res = rho >= sub ? rho - sub : rho + mod.q - sub;
+    }
+    uint64_t mine = 0;
+    const uint32_t rows = mod.q - r0 < (uint32_t)kWave ? mod.q - r0 : (uint32_t)kWave;
+    for (uint32_t rr = 0; rr < rows; ++rr) {
+        const bool ok = valid && allowed_bit(m.allow, mod.bit + res);
+        const uint64_t word = __ballot(ok);
+        if (lane == rr) mine = word;
+        res = res >= mod.a ? res - mod.a : res + mod.q - mod.a;
+    }
+    if (lane < rows) ((uint64_t*)m.masks)[(uint64_t)(m.row_base[k] + r0 + lane) * m.words + w] = mine;
+}
+
+// rho[i][c] = (u_i * B) mod q for the cached chain filters c = 0..ncache-1.
+__global__ void k_rho(MatchArgs m) {
+    uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (t >= (uint64_t)m.nu * m.ncache) return;
+    const uint32_t i = uint32_t(t / m.ncache), c = uint32_t(t % m.ncache);
+    const Modulus mod = m.mods[m.nbits + c];
+    ((uint16_t*)m.rho)[t] = uint16_t(mod32(mod64(m.us[i], mod) * mod.b, mod));
+}
+
+constexpr int kMatchThreads = 256;
+constexpr int kMatchWaves = kMatchThreads / kWave;
+constexpr int kQueue = 2 * kWave;
+
+struct MatchShared {
+    uint32_t allow[kMaxAllowWords];
+    Modulus mods[kMaxResiduals];
+    uint32_t queue[kMatchWaves][kQueue];
+    uint32_t groups[kMatchWaves][kMaxResiduals / 4 + 2];
+};
+
+__device__ __forceinline__ uint64_t wave_sum(uint64_t x) {
+    for (int d = kWave / 2; d > 0; d >>= 1) x += __shfl_xor(x, d, kWave);
+    return x;
+}
+
+// Flush one wave's counts for a subtile into the global counters.
+__device__ void flush_tile(const MatchArgs& m, MatchShared& sh, uint32_t wave, uint32_t lane, uint32_t tile,
+                           uint32_t& valid, uint64_t& survivors) {
+    const uint64_t total = wave_sum(valid);
+    uint64_t* cnt = m.counters + (uint64_t)tile * (m.ng + 3);
+    if (lane == 0) {
+        if (total) atomicAdd(&cnt[0], total);
+        if (survivors) atomicAdd(&cnt[1], survivors);
+    }
+    for (uint32_t g = 2 + lane; g <= m.ng; g += kWave) {
+        const uint32_t n = sh.groups[wave][g];
+        if (n) atomicAdd(&cnt[g], (uint64_t)n);
+        sh.groups[wave][g] = 0;
+    }
+    valid = 0;
+    survivors = 0;
+    wave_sync();
+}
+
+// Survivor output: the subtile's slot counter decides whether the value is
+// kept (slot < cap, pseudocube/batch_experiment.py:229-230); kept values go
+// to one shared list tagged with their subtile.
+__device__ __forceinline__ void emit(const MatchArgs& m, uint32_t tile, const U128& x) {
+    uint64_t* cnt = m.counters + (uint64_t)tile * (m.ng + 3);
+    const uint64_t slot = atomicAdd(&cnt[m.ng + 1], 1ull);
+    if (slot < m.cap) {
+        const uint64_t g = atomicAdd(m.list_count, 1ull);
+        if (g < m.list_cap) {
+            uint64_t* dst = m.list + g * 3;
+            dst[0] = m.tile_base + tile;
+            dst[1] = x.lo;
+            dst[2] = x.hi;
+        }
+    }
+}
+
+// Run up to 64 queued bitset survivors of one v row through the rest of the
+// filter chain.  Lane l takes queue entry l.  Filters nbits..nbits+ncache-1
+// use the per-u residue table plus the per-row offset sigma; later filters
+// build x = u*B - v*A exactly and reduce it.  Group ends count survivors
+// with a wave ballot.
+__device__ void run_chain(const MatchArgs& m, MatchShared& sh, uint32_t wave, uint32_t lane, uint32_t count,
+                          const Row& row, const U128& shift, uint32_t sigma) {
+    bool alive = lane < count;
+    const uint32_t i = alive ? sh.queue[wave][lane] : 0;
+    uint32_t k = m.nbits;
+    const uint32_t cached_end = m.nbits + m.ncache;
+    for (; k < cached_end; ++k) {
+        const Modulus mod = sh.mods[k];
+        const uint32_t s = __builtin_amdgcn_readlane(sigma, k - m.nbits);
+        if (alive) {
+            uint32_t r = m.rho[(uint64_t)i * m.ncache + (k - m.nbits)] + s;
+            r = r >= mod.q ? r - mod.q : r;
+            alive = allowed_bit(sh.allow, mod.bit + r);
+        }
+        const uint64_t live = __ballot(alive);
+        if ((k & 3) == 3 || k + 1 == m.nres) {
+            if (lane == 0 && live) sh.groups[wave][k / 4 + 1] += __popcll(live);
+        }
+        if (!live) return;
+    }
+    U128 x = {0, 0};
+    if (alive) x = sub_wide(mul_wide(m.us[i], m.B), shift);
+    for (; k < m.nres; ++k) {
+        const Modulus mod = sh.mods[k];
+        if (alive) alive = allowed_bit(sh.allow, mod.bit + mod128(x, mod));
+        const uint64_t live = __ballot(alive);
+        if ((k & 3) == 3 || k + 1 == m.nres) {
+            if (lane == 0 && live) sh.groups[wave][k / 4 + 1] += __popcll(live);
+        }
+        if (!live) return;
+    }
+    if (alive) {
+        if (!less_wide(m.lo, x) || less_wide(m.hi, x)) atomicOr(m.error, 2u);
+        emit(m, row.tile, x);
+    }
+}
+
+// Masked matcher.  One wave per v row at a time; each wave owns a contiguous
+// run of rows, and blocks are renumbered so that each XCD (blocks are dealt
+// round-robin over 8 XCDs) owns one contiguous eighth of the rows and so one
+// compact band of mask columns in its L2.
+__global__ __launch_bounds__(kMatchThreads) void k_match(MatchArgs m) {
+    __shared__ MatchShared sh;
+    for (uint32_t t = threadIdx.x; t < m.allow_words; t += kMatchThreads) sh.allow[t] = m.allow[t];
+    for (uint32_t t = threadIdx.x; t < m.nres; t += kMatchThreads) sh.mods[t] = m.mods[t];
+    for (uint32_t t = threadIdx.x; t < kMatchWaves * (kMaxResiduals / 4 + 2); t += kMatchThreads)
+        (&sh.groups[0][0])[t] = 0;
+    __syncthreads();
+
+    const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const uint32_t per_xcd = gridDim.x / 8;
+    const uint32_t logical = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    const uint64_t first_row = ((uint64_t)logical * kMatchWaves + wave) * m.rows_per_wave;
+    uint64_t last_row = first_row + m.rows_per_wave;
+    if (last_row > m.nv) last_row = m.nv;
+
+    uint32_t tile = 0xffffffffu, valid = 0;
+    uint64_t survivors = 0;
+    for (uint64_t j = first_row; j < last_row; ++j) {
+        const Row row = m.rows[j];
+        if (row.tile != tile) {
+            if (tile != 0xffffffffu) flush_tile(m, sh, wave, lane, tile, valid, survivors);
+            tile = row.tile;
+        }
+        if (row.left >= row.right) continue;
+        const U128 shift = mul_wide(row.v, m.A);
+        // sigma_c = (-v*A) mod q for the cached chain filters, one per lane
+        uint32_t sigma = 0;
+        if (lane < m.ncache) {
+            const Modulus mod = sh.mods[m.nbits + lane];
+            const uint32_t r = mod32(mod64(row.v, mod) * mod.a, mod);
+            sigma = r ? mod.q - r : 0;
+        }
+        const uint64_t* mrow[kMaxBitsetFilters];
+        for (uint32_t k = 0; k < kMaxBitsetFilters; ++k)
+            mrow[k] = m.masks + (uint64_t)(m.row_base[k] + (k < m.nbits ? row.r[k] : 0)) * m.words;
+        const uint32_t wl = row.left >> 6, wr = (row.right + 63) >> 6;
+        uint32_t queued = 0;
+        for (uint32_t wbase = wl; wbase < wr; wbase += kWave) {
+            const uint32_t w = wbase + lane;
+            uint64_t bits = 0;
+            if (w < wr) {
+                bits = ~0ull;
+                const uint32_t base = w << 6;
+                if (base < row.left) bits &= ~0ull << (row.left - base);
+                if (row.right < base + 64) bits &= (1ull << (row.right - base)) - 1ull;
+                valid += __popcll(bits);
+                for (uint32_t k = 0; k < m.nbits; ++k) bits &= mrow[k][w];
+            }
+            // wave-level compaction of the surviving u indices into the queue
+            for (;;) {
+                const bool has = bits != 0;
+                const uint64_t ballot = __ballot(has);
+                if (!ballot) break;
+                if (has) {
+                    sh.queue[wave][queued + lanes_below(ballot)] = (w << 6) + __builtin_ctzll(bits);
+                    bits &= bits - 1;
+                }
+                queued += __popcll(ballot);
+                if (queued >= kWave) {
+                    wave_sync();
+                    survivors += kWave;
+                    run_chain(m, sh, wave, lane, kWave, row, shift, sigma);
+                    wave_sync();
+                    const uint32_t rest = queued - kWave;
+                    uint32_t moved = 0;
+                    if (lane < rest) moved = sh.queue[wave][kWave + lane];
+                    wave_sync();
+                    if (lane < rest) sh.queue[wave][lane] = moved;
+                    queued = rest;
+                    wave_sync();
+                }
+            }
+        }
+        if (queued) {
+            wave_sync();
+            survivors += queued;
+            run_chain(m, sh, wave, lane, queued, row, shift, sigma);
+            wave_sync();
+        }
+    }
+    if (tile != 0xffffffffu) flush_tile(m, sh, wave, lane, tile, valid, survivors);
+}
+
+// Direct matcher: every position of every row goes through the whole chain
+// with x = u*B - v*A built in 128 bits and reduced limb by limb.  This is the
+// per-subtile baseline engine (no bitsets, no residue tables).
+__global__ __launch_bounds__(kMatchThreads) void k_match_direct(MatchArgs m) {
+    __shared__ MatchShared sh;
+    for (uint32_t t = threadIdx.x; t < m.allow_words; t += kMatchThreads) sh.allow[t] = m.allow[t];
+    for (uint32_t t = threadIdx.x; t < m.nres; t += kMatchThreads) sh.mods[t] = m.mods[t];
+    for (uint32_t t = threadIdx.x; t < kMatchWaves * (kMaxResiduals / 4 + 2); t += kMatchThreads)
+        (&sh.groups[0][0])[t] = 0;
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const uint64_t first_row = ((uint64_t)blockIdx.x * kMatchWaves + wave) * m.rows_per_wave;
+    uint64_t last_row = first_row + m.rows_per_wave;
+    if (last_row > m.nv) last_row = m.nv;
+    uint32_t tile = 0xffffffffu, valid = 0;
+    uint64_t survivors = 0;
+    for (uint64_t j = first_row; j < last_row; ++j) {
+        const Row row = m.rows[j];
+        if (row.tile != tile) {
+            if (tile != 0xffffffffu) flush_tile(m, sh, wave, lane, tile, valid, survivors);
+            tile = row.tile;
+        }
+        const U128 shift = mul_wide(row.v, m.A);
+        for (uint32_t base = row.left; base < row.right; base += kWave) {
+            const uint32_t i = base + lane;
+            bool alive = i < row.right;
+            U128 x = {0, 0};
+            if (alive) {
+                const U128 positive = mul_wide(m.us[i], m.B);
+                if (less_wide(positive, shift)) {
+                    atomicAdd(&m.counters[(uint64_t)row.tile * (m.ng + 3) + m.ng + 2], 1ull);
+                    alive = false;
+                } else {
+                    x = sub_wide(positive, shift);
+                    ++valid;
+                }
+            }
+            for (uint32_t k = 0; k < m.nres; ++k) {
+                const Modulus mod = sh.mods[k];
+                if (alive) alive = allowed_bit(sh.allow, mod.bit + mod128(x, mod));
+                const uint64_t live = __ballot(alive);
+                if ((k & 3) == 3 || k + 1 == m.nres) {
+                    const uint32_t g = k / 4 + 1;
+                    if (g == 1) survivors += __popcll(live);
+                    else if (lane == 0 && live) sh.groups[wave][g] += __popcll(live);
+                }
+                if (!live) break;
+            }
+            if (alive) {
+                if (!less_wide(m.lo, x) || less_wide(m.hi, x)) atomicOr(m.error, 2u);
+                emit(m, row.tile, x);
+            }
+        }
+    }
+    if (tile != 0xffffffffu) flush_tile(m, sh, wave, lane, tile, valid, survivors);
+}
+
+// Wide-limb self test: out[3i..3i+2] = low, high, borrow of in[4i]*in[4i+1] - in[4i+2]*in[4i+3].
+__global__ void k_arithmetic(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, uint64_t count) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const U128 left = mul_wide(in[4 * i], in[4 * i + 1]), right = mul_wide(in[4 * i + 2], in[4 * i + 3]);
+    const U128 diff = sub_wide(left, right);
+    out[3 * i] = diff.lo;
+    out[3 * i + 1] = diff.hi;
+    out[3 * i + 2] = less_wide(left, right) ? 1 : 0;
+}
+
+}  // namespace pcube

@@ -1,0 +1,226 @@
+"""Lease execution and compact, reproducible lease receipts.
+
+Drop-in for the reference's `pseudocube/optimized.py:15-90` (`PROFILE`,
+`BATCH_WIDTH`, `LEASE_WIDTH`, `Accumulator`, `compute_lease`) and the
+receipt check `validate_result` of `pseudocube/multi.py:36-67`.
+
+`compute_lease` hands the whole lease to libpcube in one call: one
+lease-wide focus window per wheel, every compute batch's bitset build and
+match queued on the device stream without host synchronisation, and one
+copy of all counters at the end.  The per-batch rows it then assembles are
+field for field those of `prepare` + `MaskedMatcher.run`, so the transcript
+is unchanged.
+"""
+from __future__ import annotations
+
+import ctypes
+import hashlib
+import time
+from typing import Any, Callable
+
+import numpy as np
+
+from . import native
+from .accounting import digest, encoded
+from .batch_experiment import MaskedMatcher, batch_row, prepare, tile_ranges
+from .gpu import GPUFocus
+
+PROFILE: dict[str, Any] = dict(engine='residue-bitset-batch-v1', tile_width=100_000_000,
+                               tiles_per_compute_batch=64, compute_batches_per_lease=256,
+                               receipt_storage='sqlite-gzip', receipt_format='compact-batch-v1',
+                               authorization='Deploy to production')
+BATCH_WIDTH = PROFILE['tile_width'] * PROFILE['tiles_per_compute_batch']
+LEASE_WIDTH = BATCH_WIDTH * PROFILE['compute_batches_per_lease']
+COUNTS = ('focus_u', 'focus_v', 'cartesian_opportunities', 'valid_focused_candidates',
+          'genuine_cubes', 'local_survivors', 'cube_checks')
+TIMINGS = ('kernel_seconds', 'focus_enumeration_seconds', 'transfer_seconds',
+           'verification_seconds', 'prefilter_build_seconds')
+
+
+def validate_result(first: int, end: int, result: dict[str, Any]) -> None:
+    """Reject rows that cannot become coverage (`pseudocube/multi.py:36-67`)."""
+    if (result.get('status') != 'PASS' or result.get('host_verified') is not True or result.get('overflow')
+            or result.get('kernel_error')):
+        raise ValueError('unverified execution cannot become coverage')
+    cursor = first
+    for tile in result['subtile_scopes']:
+        if int(tile['first']) != cursor or not cursor < int(tile['end']) <= end:
+            raise ValueError('gap or overlap in execution subtiles')
+        cursor = int(tile['end'])
+    if cursor != end:
+        raise ValueError('incomplete execution receipt')
+    for key in ('valid_focused_candidates', 'genuine_cubes'):
+        if int(result[key]) < 0:
+            raise ValueError('negative execution count')
+    if result.get('receipt_format') == 'compact-batch-v1':
+        width = int(result['tile_width'])
+        batch = int(result['compute_batch_width'])
+        if width <= 0 or batch != 64 * width or len(result['subtile_scopes']) != 1:
+            raise ValueError('invalid compact execution geometry')
+        if int(result['subtiles_executed']) != (end - first + width - 1) // width:
+            raise ValueError('compact subtile count mismatch')
+        if int(result['compute_batches']) != (end - first + batch - 1) // batch:
+            raise ValueError('compact batch count mismatch')
+        transcript = result['subtile_scopes'][0]['transcript_sha256']
+        if len(transcript) != 64 or any(c not in '0123456789abcdef' for c in transcript):
+            raise ValueError('invalid execution transcript hash')
+        values = result['all_values']
+        if len(values) != len(set(values)) or len(values) != int(result['genuine_cubes']) + len(result['cpu_confirmed_hits']):
+            raise ValueError('compact output count mismatch')
+        if result['subtile_scopes'][0]['output_sha256'] != digest(values):
+            raise ValueError('compact output hash mismatch')
+    elif 'subtiles_executed' in result or 'receipt_format' in result:
+        raise ValueError('unknown receipt format')
+
+
+class Accumulator:
+    """Folds batch rows into one compact lease row (`pseudocube/optimized.py:26-78`).
+
+    transcript = SHA-256 over the canonical encoding of every batch's
+    subtile_scopes, in batch order.
+    """
+
+    def __init__(self, first: int, end: int) -> None:
+        if not 0 <= first < end or end - first > LEASE_WIDTH:
+            raise ValueError('invalid optimized lease')
+        self.first, self.end, self.cursor = first, end, first
+        self.counts = dict.fromkeys(COUNTS, 0)
+        self.timings = dict.fromkeys(TIMINGS, 0.0)
+        self.values: list[str] = []
+        self.hits: list[dict[str, Any]] = []
+        self.groups: list[int] = []
+        self.filters: list[list[int]] | None = None
+        self.transcript = hashlib.sha256()
+        self.batches = self.subtiles = 0
+
+    def add(self, first: int, end: int, row: dict[str, Any]) -> None:
+        if first != self.cursor or end != min(first + BATCH_WIDTH, self.end):
+            raise ValueError('gap, overlap or changed compute batch geometry')
+        validate_result(first, end, row)
+        width = PROFILE['tile_width']
+        expected = [(v, min(v + width, end)) for v in range(first, end, width)]
+        actual = [(int(s['first']), int(s['end'])) for s in row['subtile_scopes']]
+        if actual != expected:
+            raise ValueError('changed arithmetic subtile geometry')
+        if self.filters is None:
+            self.filters = [list(group) for group in row['filter_groups']]
+            self.groups = [0] * len(self.filters)
+        if row['filter_groups'] != self.filters or len(row['group_survivors']) != len(self.groups):
+            raise ValueError('residue filters changed during execution')
+        for key in COUNTS:
+            self.counts[key] += int(row[key])
+        for key in TIMINGS:
+            self.timings[key] += float(row.get(key, 0.0))
+        for i, count in enumerate(row['group_survivors']):
+            self.groups[i] += int(count)
+        self.values.extend(row['all_values'])
+        self.hits.extend(row['cpu_confirmed_hits'])
+        self.transcript.update(encoded(row['subtile_scopes']))
+        self.batches += 1
+        self.subtiles += len(actual)
+        self.cursor = end
+
+    def finish(self, seconds: float) -> dict[str, Any]:
+        if self.cursor != self.end:
+            raise ValueError('unfinished lease cannot become coverage')
+        row = dict(status='PASS', host_verified=True, receipt_format=PROFILE['receipt_format'],
+                   tile_width=PROFILE['tile_width'], compute_batch_width=BATCH_WIDTH,
+                   compute_batches=self.batches, subtiles_executed=self.subtiles,
+                   subtile_scopes=[dict(first=str(self.first), end=str(self.end),
+                                        transcript_sha256=self.transcript.hexdigest(),
+                                        output_sha256=digest(self.values))],
+                   **self.counts, **self.timings, group_survivors=self.groups, filter_groups=self.filters,
+                   all_values=self.values, cpu_confirmed_hits=self.hits, overflow_splits=[], elapsed_seconds=seconds,
+                   claim='Executed primary coverage; independent arithmetic replay required before any minimum claim.')
+        validate_result(self.first, self.end, row)
+        return row
+
+
+def lease_rows(matcher: MaskedMatcher, af: GPUFocus, bf: GPUFocus, first: int, end: int,
+               batch_width: int = BATCH_WIDTH, window_cap: int | None = None) -> list[tuple[int, int, dict[str, Any]]]:
+    """Every batch row of [first, end) from one pipelined device pass.
+
+    Each row equals `MaskedMatcher.run(prepare(..., start, stop))` for its
+    batch in every field except the timings, which are the lease's phases
+    divided evenly over its batches.
+    """
+    config = matcher.config
+    width = PROFILE['tile_width']
+    starts = list(range(first, end, batch_width))
+    batches = [(s, min(s + batch_width, end)) for s in starts]
+    ranges = [tile_ranges(config, s, e, width) for s, e in batches]
+    tiles = np.array([x for r in ranges for t in r for x in t], dtype=np.uint64)
+    per_batch = np.array([len(r) for r in ranges], dtype=np.uint32)
+    ntot = int(per_batch.sum())
+    ng = len(matcher.filter_groups)
+    if window_cap is None:
+        # expected coordinates in each lease-wide window, with margin
+        u_span = ranges[-1][-1][3] - ranges[0][0][2]
+        v_span = end - first
+        expect = max(u_span * af.focus.count / af.focus.modulus, v_span * bf.focus.count / bf.focus.modulus)
+        window_cap = int(expect * 1.1) + 1_000_000
+    cap = 8192
+    bounds = np.zeros(ntot * 4, dtype=np.uint64)
+    sizes = np.zeros(len(batches) * 2, dtype=np.uint64)
+    counters = np.zeros(ntot * (ng + 3), dtype=np.uint64)
+    values = np.zeros(2 * (1 << 20), dtype=np.uint64)
+    seconds = (ctypes.c_double * 4)()
+    started = time.perf_counter()
+    matcher.runtime.check()
+    code = native.lib().pcube_lease(matcher.engine.handle, af.handle, bf.handle, len(batches),
+                                    native.ptr32(per_batch), native.ptr64(tiles), width, cap, window_cap,
+                                    native.ptr64(bounds), native.ptr64(sizes), native.ptr64(counters),
+                                    native.ptr64(values), values.size // 2, seconds)
+    native.check(code, 'pcube_lease')
+    matcher.runtime.check()
+    device_seconds = time.perf_counter() - started
+    counters = counters.reshape(ntot, ng + 3)
+    bounds = bounds.reshape(ntot, 4)
+    n = len(batches)
+    timings = dict(kernel_seconds=seconds[2] / n, focus_enumeration_seconds=seconds[0] / n,
+                   transfer_seconds=max(0.0, device_seconds - seconds[0] - seconds[2]) / n)
+    rows = []
+    t0 = 0
+    offset = 0
+    for b, (s, e) in enumerate(batches):
+        nt = len(ranges[b])
+        block = counters[t0:t0 + nt]
+        per_tile = []
+        for t in range(nt):
+            k = min(int(block[t, ng + 1]), cap)
+            limbs = values[2 * offset:2 * (offset + k)]
+            per_tile.append([int(limbs[2 * i]) + (int(limbs[2 * i + 1]) << 64) for i in range(k)])
+            offset += k
+        bnd = [tuple(int(x) for x in bounds[t0 + t]) for t in range(nt)]
+        row_started = time.perf_counter()
+        row = batch_row(config, matcher.filter_groups, ranges[b], bnd, int(sizes[2 * b + 1]), block, per_tile,
+                        cap, timings, row_started)
+        row['kernel_registers'] = int(matcher.kernel['registers'])
+        row['kernel_local_bytes'] = int(matcher.kernel['local_bytes'])
+        row['matching_layout'] = matcher.layout
+        row['prefilter_build_seconds'] = seconds[1] / n
+        rows.append((s, e, row))
+        t0 += nt
+    return rows
+
+
+def compute_lease(matcher: MaskedMatcher, af: GPUFocus, bf: GPUFocus, first: int, end: int,
+                  heartbeat: Callable[[], None] = lambda: None, pipelined: bool = True) -> dict[str, Any]:
+    """One lease of up to 256 compute batches (`pseudocube/optimized.py:81-90`).
+
+    pipelined=False runs the reference's loop literally, `prepare` then
+    `MaskedMatcher.run` per batch; both give the same receipt.
+    """
+    started = time.monotonic()
+    acc = Accumulator(first, end)
+    if pipelined:
+        for start, stop, row in lease_rows(matcher, af, bf, first, end):
+            acc.add(start, stop, row)
+            heartbeat()
+    else:
+        for start in range(first, end, BATCH_WIDTH):
+            stop = min(start + BATCH_WIDTH, end)
+            batch = prepare(matcher.runtime, matcher.config, af, bf, start, stop, PROFILE['tile_width'])
+            acc.add(start, stop, matcher.run(batch))
+            heartbeat()
+    return acc.finish(time.monotonic() - started)
