@@ -1,0 +1,145 @@
+#!/usr/bin/env python3
+"""Benchmark: focused candidate positions per second on one MI355X.
+
+Workload (the configuration the metric is quoted on): p = 619, 10^27 < x <= 3*10^27,
+A/B = Sorenson's split; one step is one full compute batch of 64 subtiles x 1e8
+canonical v (about 1.07e9 focused candidate positions), its batch index drawn
+uniformly from random.Random(619003) over [0, B / 6.4e9).  Each step runs the
+drop-in path end to end: both focus windows, subtile bounds, residue-bitset
+build, match kernel, device-to-host copy, exact host verification and the
+per-subtile output hashes.  The focus wheels and filter tables are resident
+in HBM before timing starts, as in the reference's long-running worker.
+
+Prints one JSON line.  Extra fields: kernel-only rate, the pipelined lease
+rate (256 batches in one device pass), the roofline of the match kernel, and
+the CPU baseline (this repository's C restatement on the host cores).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import random
+import subprocess
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+# Algorithmic work per focused candidate position, in int32 VALU operations
+# (fixed cost table: bitset stage 0.125, per-survivor wide build and filters
+# 0.40, per-row bounds 0.02, windows 0.01).
+OPS_PER_POSITION = 0.56
+# MI355X int32 vector peak: 256 CUs x 64 lanes x 2.4 GHz = 39.3e12 lane-ops/s
+# (one quarter of the published 157.3 TFLOP/s fp32 vector figure, which
+# counts FMA as two operations and packed dual issue twice).
+INT32_PEAK = 256 * 64 * 2.4e9
+
+
+def parse() -> argparse.Namespace:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=40)
+    ap.add_argument('--warmup', type=int, default=5)
+    ap.add_argument('--cpu-seconds', type=float, default=12.0, help='bounded CPU-baseline sample')
+    ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-lease', action='store_true')
+    return ap.parse_args()
+
+
+def cpu_baseline(config, seconds: float) -> dict:
+    """The oracle (C, all host threads) on whole seeded batches for about `seconds`."""
+    from oracle import oracle as O
+    from psuedocubes_amd.optimized import BATCH_WIDTH
+    fa, fb = (O.Focus(f) for f in config.focuses())
+    rng = random.Random(619003 + 1)
+    threads = O.threads()
+    positions, batches = 0, 0
+    started = time.perf_counter()
+    while time.perf_counter() - started < seconds or batches == 0:
+        k = rng.randrange(config.B // BATCH_WIDTH)
+        row = O.batch(config, fa, fb, k * BATCH_WIDTH, (k + 1) * BATCH_WIDTH, nthreads=threads)
+        positions += row['valid_focused_candidates']
+        batches += 1
+    elapsed = time.perf_counter() - started
+    try:
+        model = next(line.split(':', 1)[1].strip() for line in
+                     subprocess.run(['lscpu'], capture_output=True, text=True).stdout.splitlines()
+                     if line.startswith('Model name'))
+    except Exception:
+        model = 'unknown'
+    return dict(value=positions / elapsed, unit='focused candidate positions/s', cores=threads, kind='port',
+                sample=f'{batches} seeded full p=619 batches (windows + bitset matcher + counters), '
+                       f'{elapsed:.1f} s on {threads} threads of {os.cpu_count()} ({model})')
+
+
+def main() -> None:
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) > 1:
+        print('bench.py measures one GPU: this engine shards by disjoint canonical-v leases with no '
+              'inter-GPU traffic, and only the single-GPU figure is reported here', file=sys.stderr)
+        raise SystemExit(2)
+    from psuedocubes_amd.batch_experiment import MaskedMatcher, prepare
+    from psuedocubes_amd.focus import A619, B619, Config
+    from psuedocubes_amd.gpu import GPUFocus, Runtime
+    from psuedocubes_amd.optimized import BATCH_WIDTH, compute_lease
+
+    config = Config(619, 10**27, 3 * 10**27, A619, B619)
+    runtime = Runtime(time.monotonic() + 3600, 16 * 1024**3)
+    af, bf = (GPUFocus(runtime, f) for f in config.focuses())
+    matcher = MaskedMatcher(runtime, config)
+    rng = random.Random(619003)
+    nbatch = B619 // BATCH_WIDTH
+
+    def step() -> dict:
+        k = rng.randrange(nbatch)
+        batch = prepare(runtime, config, af, bf, k * BATCH_WIDTH, (k + 1) * BATCH_WIDTH)
+        return matcher.run(batch)
+
+    for _ in range(args.warmup):
+        step()
+    print(f'# warmup {args.warmup} done', file=sys.stderr, flush=True)
+    positions, kernel, prefilter = 0, 0.0, 0.0
+    started = time.perf_counter()
+    for _ in range(args.steps):
+        row = step()
+        positions += row['valid_focused_candidates']
+        kernel += row['kernel_seconds']
+        prefilter += row['prefilter_build_seconds']
+    elapsed = time.perf_counter() - started
+    print(f'# timed {args.steps} steps in {elapsed:.3f} s', file=sys.stderr, flush=True)
+
+    per_launch = positions / args.steps
+    match_seconds = kernel / args.steps
+    achieved = OPS_PER_POSITION * per_launch / match_seconds
+    result = dict(metric='focused candidate positions/s on 1 MI355X, plain vs batched; p50/p99 latency',
+                  value=positions / elapsed, unit='focused candidate positions/s', n_gpus=1, steps=args.steps,
+                  warmup=args.warmup, ms_per_step=1000 * elapsed / args.steps, higher_is_better=True,
+                  scaling='weak', vs_baseline=None, dtype='u64/u128 exact integer', data='synthetic',
+                  config=dict(workload='plain drop-in: p=619 full batch of 64 x 1e8 v, seeded batch index '
+                                       '(random.Random(619003)), prepare + MaskedMatcher.run end to end',
+                              p=619, tile_width=100_000_000, subtiles=64, seed=619003),
+                  kernel_only=dict(value=positions / kernel, unit='focused candidate positions/s',
+                                   match_ms=1000 * match_seconds, prefilter_ms=1000 * prefilter / args.steps),
+                  roofline=dict(bound='valu', achieved=achieved / 1e12, peak=INT32_PEAK / 1e12, unit='TOP/s',
+                                frac=achieved / INT32_PEAK, traffic=None,
+                                kernel='k_match', ops_per_position=OPS_PER_POSITION,
+                                positions_per_launch=per_launch))
+    if not args.no_lease:
+        first = 575222931200000000
+        compute_lease(matcher, af, bf, first, first + 4 * BATCH_WIDTH)  # warm the lease buffers
+        t0 = time.perf_counter()
+        lease = compute_lease(matcher, af, bf, first, first + 256 * BATCH_WIDTH)
+        t1 = time.perf_counter() - t0
+        result['lease'] = dict(value=lease['valid_focused_candidates'] / t1, unit='focused candidate positions/s',
+                               seconds=t1, batches=256, first=str(first),
+                               transcript_sha256=lease['subtile_scopes'][0]['transcript_sha256'])
+    if not args.no_cpu:
+        result['cpu_baseline'] = cpu_baseline(config, args.cpu_seconds)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == '__main__':
+    main()

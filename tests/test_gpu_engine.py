@@ -1,0 +1,244 @@
+"""GPU parity: every HIP kernel path against the CPU oracle and the golden fixtures.
+
+Recipes follow the reference's GPU validation scripts:
+tools/batch_window_experiment.py:100-205 (`single`: four batch scopes,
+complete small sets, forced overflow, high-magnitude canaries, CPU producer
+arrays equal GPU arrays) and the release's tools/qualify_optimized.py:36-61
+(full-lease equivalence).  All comparisons are bit-exact.
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+from pathlib import Path
+import random
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from psuedocubes_amd import native
+from psuedocubes_amd.accounting import encoded
+from psuedocubes_amd.batch_experiment import FusedMatcher, MaskedMatcher, compare, prepare, tile_ranges
+from psuedocubes_amd.focus import A619, B619, Config, Tile
+from psuedocubes_amd.gpu import DeviceArray, GPUFocus, Matcher
+from psuedocubes_amd.optimized import BATCH_WIDTH, compute_lease
+from psuedocubes_amd.predicate import local3
+from psuedocubes_amd.run_production import compute_batch
+
+pytestmark = pytest.mark.gpu
+GOLDEN = Path(__file__).parent / 'golden'
+LEASE = json.loads((GOLDEN / 'lease_575222931200000000.json').read_text())
+LEASE_BATCHES = json.loads((GOLDEN / 'lease_575222931200000000_batches.json').read_text())['batches']
+LEASE_FIRST = int(LEASE['first'])
+
+
+def small_config(p: int, lo: int = 0, hi: int = 100000) -> Config:
+    return Config(p, lo, hi, 14 * (19 if p >= 19 else 1), 9 * (13 if p >= 13 else 1) * (31 if p >= 31 else 1))
+
+
+def scopes_hash(row: dict) -> str:
+    return hashlib.sha256(encoded(row['subtile_scopes'])).hexdigest()
+
+
+def oracle_row_fields(row: dict) -> dict:
+    keys = ('focus_u', 'focus_v', 'cartesian_opportunities', 'valid_focused_candidates', 'local_survivors',
+            'subtile_scopes', 'group_survivors', 'filter_groups', 'all_values')
+    return {k: row[k] for k in keys}
+
+
+@pytest.fixture(scope='module')
+def masked(runtime, config619):
+    return MaskedMatcher(runtime, config619)
+
+
+def test_wide_limb_kernel(runtime) -> None:
+    rng = random.Random(619003)
+    cases = [(0, 0, 0, 0), (2**64 - 1, 2**64 - 1, 0, 0), (0, 0, 2**64 - 1, 2**64 - 1),
+             (2**32, 2**32, 1, 1), (2**64 - 1, 1, 0, 0)]
+    cases += [tuple(rng.getrandbits(64) for _ in range(4)) for _ in range(2000)]
+    flat = np.array([x for c in cases for x in c], dtype=np.uint64)
+    out = np.zeros(3 * len(cases), dtype=np.uint64)
+    native.call('pcube_arithmetic', native.ptr64(flat), native.ptr64(out), len(cases))
+    for i, (a, b, c, d) in enumerate(cases):
+        assert int(out[3 * i]) + (int(out[3 * i + 1]) << 64) == (a * b - c * d) % (1 << 128)
+        assert int(out[3 * i + 2]) == int(a * b < c * d)
+
+
+def test_focus_halves_equal_oracle(focuses619, oracle_focuses619) -> None:
+    for gpu, cpu in zip(focuses619, oracle_focuses619):
+        assert np.array_equal(gpu.C.get(), cpu.C)
+        assert np.array_equal(gpu.D.get(), cpu.D)
+    assert focuses619[0].half_sizes == [2096640, 2009280]
+    assert focuses619[1].half_sizes == [2592000, 2585088]
+
+
+def test_windows_equal_oracle(focuses619, oracle_focuses619) -> None:
+    rng = random.Random(619003)
+    for gpu, cpu in zip(focuses619, oracle_focuses619):
+        M = gpu.focus.modulus
+        cases = [(0, 1), (0, 9_370_000_000), (M - 5_000_000_000, M + 5_000_000_000), (M - 1, M + 1),
+                 (2 * M - 10**9, 2 * M), (12345, 12345)]
+        cases += [(s, s + rng.choice([10**6, 10**8, 6_400_000_000, 9_400_000_000]))
+                  for s in (rng.randrange(2 * M) for _ in range(12))]
+        for low, high in cases:
+            got = gpu.window(low, high).get()
+            assert np.array_equal(got, cpu.window(low, high)), (low, high)
+        with pytest.raises(OverflowError):
+            gpu.window(0, 10**12, cap=10)
+        with pytest.raises(ValueError):
+            gpu.window(5, 4)
+
+
+def test_small_config_windows_all_periods(runtime) -> None:
+    for p in (7, 13, 19, 31):
+        config = small_config(p)
+        for focus in config.focuses():
+            gpu, cpu = GPUFocus(runtime, focus), O.Focus(focus)
+            M = focus.modulus
+            for low, high in [(0, M), (1, 3 * M - 1), (M - 1, M + 2), (0, 4000 * M), (17, 17)]:
+                assert np.array_equal(gpu.window(low, high).get(), cpu.window(low, high, cap=10**7)), (p, low, high)
+            with pytest.raises(OverflowError):
+                gpu.window(0, 4097 * M)
+
+
+def test_device_searchsorted(runtime) -> None:
+    from psuedocubes_amd.batch_experiment import searchsorted
+    rng = np.random.default_rng(5)
+    data = np.sort(rng.integers(0, 2**62, 100_000, dtype=np.uint64))
+    arr = DeviceArray.from_host(data)
+    queries = [0, int(data[0]), int(data[-1]), int(data[-1]) + 1, 2**63] + [int(x) for x in rng.integers(0, 2**62, 1000, dtype=np.uint64)]
+    assert searchsorted(arr, queries) == np.searchsorted(data, np.array(queries, dtype=np.uint64)).tolist()
+
+
+@pytest.mark.parametrize('index', [0, 1, 127, 255])
+def test_masked_batches_of_reference_lease(runtime, config619, focuses619, masked, index) -> None:
+    golden = LEASE_BATCHES[index]
+    first = int(golden['first'])
+    batch = prepare(runtime, config619, *focuses619, first, int(golden['end']))
+    row = masked.run(batch)
+    assert scopes_hash(row) == golden['scopes_sha256']
+    assert row['valid_focused_candidates'] == golden['valid_focused_candidates']
+    assert row['group_survivors'] == golden['group_survivors']
+    assert row['all_values'] == golden['all_values']
+    assert (row['focus_u'], row['focus_v']) == (golden['focus_u'], golden['focus_v'])
+
+
+def seeded_scopes() -> list[tuple[int, int]]:
+    """Batch 0, the seeded full batch, B/2 and the tail ending at B (tools/batch_window_experiment.py:117-118)."""
+    rng = random.Random(619003)
+    k = rng.randrange(B619 // BATCH_WIDTH)
+    return [(0, BATCH_WIDTH), (k * BATCH_WIDTH, (k + 1) * BATCH_WIDTH),
+            (246112000000000 - BATCH_WIDTH, 246112000000000), (B619 // 2, B619 // 2 + BATCH_WIDTH),
+            (B619 - BATCH_WIDTH, B619), (B619 - 100000001, B619)]
+
+
+@pytest.mark.parametrize('scope', seeded_scopes())
+def test_masked_equals_oracle_on_seeded_batches(runtime, config619, focuses619, oracle_focuses619, masked, scope) -> None:
+    first, end = scope
+    row = masked.run(prepare(runtime, config619, *focuses619, first, end))
+    expected = O.batch(config619, *oracle_focuses619, first, end)
+    assert oracle_row_fields(row) == oracle_row_fields(expected)
+
+
+def test_fused_direct_engine_equals_masked(runtime, config619, focuses619, masked) -> None:
+    fused = FusedMatcher(runtime, config619)
+    batch = prepare(runtime, config619, *focuses619, LEASE_FIRST, LEASE_FIRST + BATCH_WIDTH)
+    a, b = fused.run(batch), masked.run(batch)
+    compare(a, b)
+    assert a['matching_layout'] == 'fused-batch' and b['matching_layout'] == 'residue-bitset-batch'
+
+
+def test_complete_small_sets_through_gpu(runtime) -> None:
+    for p in (7, 13, 19, 31):
+        config = small_config(p)
+        af, bf = (GPUFocus(runtime, f) for f in config.focuses())
+        width = max(1, (config.B + 6) // 7)
+        batch = prepare(runtime, config, af, bf, 0, config.B, width)
+        expected = [str(n) for n in range(1, 100001) if local3(n, p)]
+        for matcher in (FusedMatcher(runtime, config), MaskedMatcher(runtime, config)):
+            row = matcher.run(batch)
+            assert sorted(row['all_values'], key=int) == expected
+        fa, fb = (O.Focus(f) for f in config.focuses())
+        assert oracle_row_fields(MaskedMatcher(runtime, config).run(batch)) == \
+            oracle_row_fields(O.batch(config, fa, fb, 0, config.B, width))
+        if p == 7:
+            for matcher in (FusedMatcher(runtime, config), MaskedMatcher(runtime, config)):
+                with pytest.raises(OverflowError):
+                    matcher.run(batch, output_cap=1)
+
+
+@pytest.mark.parametrize('p,n', [(613, 674441580981249129037406633), (619, 1000000007**3), (619, 1000000009**3)])
+def test_high_magnitude_canaries(runtime, focuses619, p, n) -> None:
+    narrow = Config(p, n - 1, n, A619, B619)
+    v = narrow.canonical(n)[1]
+    first, end = max(0, v - 5000000), min(B619, v + 5000001)
+    batch = prepare(runtime, narrow, *focuses619, first, end, 5000000)
+    rows = [FusedMatcher(runtime, narrow).run(batch), MaskedMatcher(runtime, narrow).run(batch)]
+    compare(rows[0], rows[1])
+    assert rows[1]['all_values'] == [str(n)]
+    legacy = compute_batch(Matcher(runtime, narrow), *focuses619, first, end, 5000000)
+    compare(legacy, rows[1])
+
+
+def test_full_reference_lease_pipelined(runtime, config619, focuses619, masked) -> None:
+    result = compute_lease(masked, *focuses619, LEASE_FIRST, int(LEASE['end']))
+    ref = LEASE['result']
+    for key in ('focus_u', 'focus_v', 'cartesian_opportunities', 'valid_focused_candidates', 'genuine_cubes',
+                'local_survivors', 'cube_checks', 'compute_batches', 'subtiles_executed', 'group_survivors',
+                'filter_groups', 'all_values', 'cpu_confirmed_hits'):
+        assert result[key] == ref[key], key
+    assert result['subtile_scopes'] == ref['subtile_scopes']
+
+
+def test_pipelined_lease_equals_per_batch_loop(runtime, config619, focuses619, masked) -> None:
+    first = LEASE_FIRST + 7 * BATCH_WIDTH
+    end = first + 2 * BATCH_WIDTH + 300_000_001
+    a = compute_lease(masked, *focuses619, first, end, pipelined=True)
+    b = compute_lease(masked, *focuses619, first, end, pipelined=False)
+    strip = lambda r: {k: v for k, v in r.items() if not k.endswith('_seconds')}  # noqa: E731
+    assert strip(a) == strip(b)
+    assert a['subtiles_executed'] == 64 + 64 + 4 and a['compute_batches'] == 3
+
+
+def test_legacy_matcher_tile_and_split_replay(runtime) -> None:
+    config = small_config(7)
+    af, bf = (GPUFocus(runtime, f) for f in config.focuses())
+    legacy = Matcher(runtime, config)
+
+    class Tiny(Matcher):
+        def tile(self, tile, af, bf, cap=1000000, output_cap=8192):
+            return super().tile(tile, af, bf, cap, output_cap=1500)
+
+    tiny = Tiny(runtime, config)
+    full = compute_batch(legacy, af, bf, 0, config.B, config.B)
+    split = compute_batch(tiny, af, bf, 0, config.B, config.B)
+    assert full['overflow_splits'] == [] and split['overflow_splits']
+    assert sorted(full['all_values'], key=int) == sorted(split['all_values'], key=int)
+    assert full['valid_focused_candidates'] == split['valid_focused_candidates']
+    assert sorted(full['all_values'], key=int) == [str(n) for n in range(1, 100001) if local3(n, 7)]
+    with pytest.raises(ValueError):
+        Matcher(runtime, config, variant='float')
+    row = legacy.tile(Tile(0, config.B), af, bf)
+    assert row['status'] == 'PASS' and row['valid_focused_candidates'] == full['valid_focused_candidates']
+
+
+def test_endpoint_and_carry_windows(runtime, focuses619, oracle_focuses619) -> None:
+    """Narrow x intervals at the interval ends and at 2^64 / 2^90 carry boundaries."""
+    rng = random.Random(619005)
+    centres = [10**27 + 1, 3 * 10**27, 2**90, 2**90 + 1, 2**90 - 1, (10**27 // 2**64 + 1) * 2**64,
+               (3 * 10**27 // 2**64) * 2**64 - 1]
+    centres += [rng.randrange(10**27, 3 * 10**27) for _ in range(3)]
+    for n in centres:
+        config = Config(619, n - 10**9, min(3 * 10**27, n + 10**9), A619, B619)
+        v = config.canonical(n)[1]
+        first, end = max(0, v - 5000000), min(B619, v + 5000000)
+        row = MaskedMatcher(runtime, config).run(prepare(runtime, config, *focuses619, first, end, 1000000))
+        expected = O.batch(config, *oracle_focuses619, first, end, 1000000)
+        assert oracle_row_fields(row) == oracle_row_fields(expected), n
+
+
+def test_tile_ranges_match_oracle_geometry(config619) -> None:
+    for first, end in [(0, BATCH_WIDTH), (B619 - BATCH_WIDTH, B619), (0, BATCH_WIDTH + 1)]:
+        width = 100000001 if end - first > BATCH_WIDTH else 100000000
+        assert tile_ranges(config619, first, end, width) == O.tile_ranges(config619, first, end, width)
