@@ -21,6 +21,9 @@ from .focus import Config, Tile
 from .gpu import DeviceArray, GPUFocus, MatchEngine, Runtime
 from .predicate import is_perfect_cube, local3, verify
 
+# output_sha256 of a subtile that emitted nothing (almost every subtile at p=619)
+EMPTY_DIGEST = digest([])
+
 
 @dataclass
 class Prepared:
@@ -110,7 +113,7 @@ def batch_row(config: Config, filter_groups: list[list[int]], ranges: list[tuple
             raise ArithmeticError('output assigned to wrong subtile')
         scopes.append(dict(first=str(first), end=str(end), valid_focused_candidates=int(counters[row, 0]),
                            group_survivors=[int(n) for n in counters[row, 1:ng + 1]], local_survivors=number,
-                           output_sha256=digest([str(n) for n in values])))
+                           output_sha256=digest([str(n) for n in values]) if values else EMPTY_DIGEST))
         position += number
     verification_seconds = time.perf_counter() - verification_started
     groups = [int(n) for n in counters[:, 1:ng + 1].sum(axis=0)]

@@ -225,10 +225,10 @@ struct Focus {
     uint32_t dshift = 0;
     Buffer C, D, dir;
     BucketWork work;        // window buckets and scan scratch
-    Buffer total;           // device copy of the last window size (uint32 at offsets[nb])
+    Buffer slots, flag;     // single-pass window slots and their overflow flag
     ~Focus() {
         C.release(); D.release(); dir.release();
-        work.hist.release(); work.offsets.release(); total.release();
+        work.hist.release(); work.offsets.release(); slots.release(); flag.release();
         work.scan.partial.release(); work.scan.partial_off.release(); work.scan.partial_total.release();
     }
 };
@@ -264,23 +264,21 @@ int build_half(uint64_t M, const uint64_t* terms, const uint32_t* sizes, uint32_
     return PCUBE_OK;
 }
 
+// Expected number of coordinates of focus f in a window of the given span.
+double expected_count(const Focus& f, uint64_t span) {
+    return (double)span / (double)f.M * (double)f.nc * (double)f.nd;
+}
+
 // Enqueue a window [low, high) of focus f into out (capacity cap).  The
-// device total is left at f.work.offsets[nb]; *nb_out receives nb.
-int window_enqueue(Focus& f, uint64_t low, uint64_t high, uint64_t* out, uint64_t cap, uint64_t* nb_out) {
+// device total is left at f.work.offsets[nb] (*nb_out receives nb).  Small
+// windows take the single-pass slot path (*single = true) whose overflow flag
+// the caller must check (f.flag); exact = true forces the two-pass path.
+int window_enqueue(Focus& f, uint64_t low, uint64_t high, uint64_t* out, uint64_t cap, bool exact,
+                   uint64_t* nb_out, bool* single) {
     const uint64_t per0 = low / f.M, per1 = (high - 1) / f.M, nper = per1 - per0 + 1;
     if (nper > (1u << 20)) return fail(PCUBE_INVALID, "window spans too many periods");
     const uint64_t span = high - low;
-    const double expected = (double)span / (double)f.M * (double)f.nc * (double)f.nd;
-    uint64_t target = 1;
-    while ((double)(target * 2) <= expected / 2 && target < (1ull << 22)) target *= 2;
-    const uint32_t bshift = bucket_shift(span, target);
-    const uint64_t nb = ((span - 1) >> bshift) + 1;
-    TRY(f.work.hist.ensure(nb * 8));
-    TRY(f.work.offsets.ensure(nb * 4 + 4));
-    uint32_t* hist = f.work.hist.as<uint32_t>();
-    uint32_t* cursor = hist + nb;
-    uint32_t* offsets = f.work.offsets.as<uint32_t>();
-    TRY(fill32(hist, 2 * nb, 0));
+    const double expected = expected_count(f, span);
     WindowArgs w;
     w.C = f.C.as<uint64_t>();
     w.D = f.D.as<uint64_t>();
@@ -289,15 +287,60 @@ int window_enqueue(Focus& f, uint64_t low, uint64_t high, uint64_t* out, uint64_
     w.low = low; w.high = high; w.per0 = per0; w.cap = cap;
     w.dshift = f.dshift; w.nper = (uint32_t)nper;
     w.nchunks = (uint32_t)((f.nd + kChunk - 1) / kChunk);
-    w.bshift = bshift;
     const uint64_t threads = (uint64_t)w.nper * w.nchunks;
-    hipLaunchKernelGGL(k_window<false>, dim3(blocks_for(threads, 256)), dim3(256), 0, g.stream, w, hist,
-                       (const uint32_t*)offsets, cursor, out);
-    TRY(scan(hist, offsets, nb, f.work.scan));
-    hipLaunchKernelGGL(k_window<true>, dim3(blocks_for(threads, 256)), dim3(256), 0, g.stream, w, hist,
-                       (const uint32_t*)offsets, cursor, out);
-    hipLaunchKernelGGL(k_bucket_sort, dim3(blocks_for(nb, 256)), dim3(256), 0, g.stream, out, offsets, nb, cap);
+    *single = !exact && expected <= (double)(1u << 20);
+    uint64_t target = 1;
+    if (*single) {
+        while ((double)target < 2 * expected && target < (1ull << 22)) target *= 2;
+    } else {
+        while ((double)(target * 2) <= expected / 2 && target < (1ull << 22)) target *= 2;
+    }
+    w.bshift = bucket_shift(span, target);
+    const uint64_t nb = ((span - 1) >> w.bshift) + 1;
+    TRY(f.work.hist.ensure(nb * 8));
+    TRY(f.work.offsets.ensure(nb * 4 + 4));
+    uint32_t* hist = f.work.hist.as<uint32_t>();
+    uint32_t* cursor = hist + nb;
+    uint32_t* offsets = f.work.offsets.as<uint32_t>();
     *nb_out = nb;
+    if (*single) {
+        TRY(f.slots.ensure(nb * kSlotsPerBucket * 8));
+        TRY(f.flag.ensure(4));
+        TRY(fill32(hist, nb, 0));
+        TRY(fill32(f.flag.as<uint32_t>(), 1, 0));
+        hipLaunchKernelGGL(k_window<kSlots>, dim3(blocks_for(threads, 256)), dim3(256), 0, g.stream, w, hist,
+                           (const uint32_t*)offsets, cursor, f.slots.as<uint64_t>(), f.flag.as<uint32_t>());
+        TRY(scan(hist, offsets, nb, f.work.scan));
+        hipLaunchKernelGGL(k_slots_compact, dim3(blocks_for(nb, 256)), dim3(256), 0, g.stream,
+                           (const uint64_t*)f.slots.as<uint64_t>(), (const uint32_t*)offsets, nb, out, cap);
+        return PCUBE_OK;
+    }
+    TRY(fill32(hist, 2 * nb, 0));
+    hipLaunchKernelGGL(k_window<kCount>, dim3(blocks_for(threads, 256)), dim3(256), 0, g.stream, w, hist,
+                       (const uint32_t*)offsets, cursor, out, (uint32_t*)nullptr);
+    TRY(scan(hist, offsets, nb, f.work.scan));
+    hipLaunchKernelGGL(k_window<kEmit>, dim3(blocks_for(threads, 256)), dim3(256), 0, g.stream, w, hist,
+                       (const uint32_t*)offsets, cursor, out, (uint32_t*)nullptr);
+    hipLaunchKernelGGL(k_bucket_sort, dim3(blocks_for(nb, 256)), dim3(256), 0, g.stream, out, offsets, nb, cap);
+    return PCUBE_OK;
+}
+
+// Run a window to completion and return its size; reruns the exact path when
+// a single-pass bucket overflowed its slots.
+int window_run(Focus& f, uint64_t low, uint64_t high, uint64_t* out, uint64_t cap, uint64_t* count) {
+    uint64_t nb = 0;
+    bool single = false;
+    TRY(window_enqueue(f, low, high, out, cap, false, &nb, &single));
+    uint32_t total = 0, flag = 0;
+    HIP_TRY(hipMemcpyAsync(&total, f.work.offsets.as<uint32_t>() + nb, 4, hipMemcpyDeviceToHost, g.stream));
+    if (single) HIP_TRY(hipMemcpyAsync(&flag, f.flag.p, 4, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    if (flag) {
+        TRY(window_enqueue(f, low, high, out, cap, true, &nb, &single));
+        HIP_TRY(hipMemcpyAsync(&total, f.work.offsets.as<uint32_t>() + nb, 4, hipMemcpyDeviceToHost, g.stream));
+        HIP_TRY(hipStreamSynchronize(g.stream));
+    }
+    *count = total;
     return PCUBE_OK;
 }
 
@@ -346,7 +389,7 @@ int match_enqueue(Matcher& mt, const uint64_t* us, uint64_t nu, const uint64_t* 
     TRY(mt.rows.ensure(std::max<uint64_t>(1, nv) * sizeof(Row)));
     if (engine == 0) {
         TRY(mt.masks.ensure(std::max<uint64_t>(8, (uint64_t)mt.mask_rows * std::max(1u, words) * 8)));
-        TRY(mt.rho.ensure(std::max<uint64_t>(8, nu * mt.ncache * 2)));
+        TRY(mt.rho.ensure(std::max<uint64_t>(16, nu * kCachedResidues * 2)));
     }
     MatchArgs m;
     memset(&m, 0, sizeof m);
@@ -382,7 +425,7 @@ int match_enqueue(Matcher& mt, const uint64_t* us, uint64_t nu, const uint64_t* 
                            chunk_base[1], mt.nbits > 2 ? chunk_base[2] : 0xffffffffu,
                            mt.nbits > 3 ? chunk_base[3] : 0xffffffffu);
         if (mt.ncache)
-            hipLaunchKernelGGL(k_rho, dim3(blocks_for(nu * mt.ncache, 256)), dim3(256), 0, g.stream, m);
+            hipLaunchKernelGGL(k_rho, dim3(blocks_for(nu * kCachedResidues, 256)), dim3(256), 0, g.stream, m);
     }
     if (ev_pre) HIP_TRY(hipEventRecord(ev_pre, g.stream));
     const uint64_t rows_per_wave = std::max<uint64_t>(1, (nv + 8191) / 8192);
@@ -451,8 +494,10 @@ struct Kernel {
 };
 const Kernel kKernels[] = {
     {"partial", (const void*)k_partial},
-    {"window_count", (const void*)k_window<false>},
-    {"window_emit", (const void*)k_window<true>},
+    {"window_count", (const void*)k_window<kCount>},
+    {"window_emit", (const void*)k_window<kEmit>},
+    {"window_slots", (const void*)k_window<kSlots>},
+    {"slots_compact", (const void*)k_slots_compact},
     {"bucket_sort", (const void*)k_bucket_sort},
     {"scan_local", (const void*)k_scan_local},
     {"rows", (const void*)k_rows},
@@ -624,13 +669,10 @@ int pcube_focus_window(int focus, uint64_t low, uint64_t high, uint64_t out, uin
     Focus* f = focus_at(focus);
     if (!f || low > high) return fail(PCUBE_INVALID, "invalid focus window");
     if (low == high) return PCUBE_OK;
-    uint64_t nb = 0;
-    TRY(window_enqueue(*f, low, high, (uint64_t*)out, cap, &nb));
-    uint32_t total = 0;
-    HIP_TRY(hipMemcpyAsync(&total, f->work.offsets.as<uint32_t>() + nb, 4, hipMemcpyDeviceToHost, g.stream));
-    HIP_TRY(hipStreamSynchronize(g.stream));
+    uint64_t total = 0;
+    TRY(window_run(*f, low, high, (uint64_t*)out, cap, &total));
     *count = total;
-    if (total > cap) return fail(PCUBE_OVERFLOW, "focus buffer overflow: %u coordinates, capacity %llu", total, (unsigned long long)cap);
+    if (total > cap) return fail(PCUBE_OVERFLOW, "focus buffer overflow: %llu coordinates, capacity %llu", (unsigned long long)total, (unsigned long long)cap);
     return PCUBE_OK;
 }
 
@@ -791,11 +833,12 @@ int pcube_lease(int matcher, int focus_u, int focus_v, uint32_t nbatches, const 
     HIP_TRY(hipEventRecord(ev[0], g.stream));
     uint64_t nbu = 0, nbv = 0;
     uint32_t totals[2] = {0, 0};
+    bool single = false;
     if (u_high > u_low) {
-        TRY(window_enqueue(*fu, u_low, u_high, lus.as<uint64_t>(), window_cap, &nbu));
+        TRY(window_enqueue(*fu, u_low, u_high, lus.as<uint64_t>(), window_cap, true, &nbu, &single));
         HIP_TRY(hipMemcpyAsync(&totals[0], fu->work.offsets.as<uint32_t>() + nbu, 4, hipMemcpyDeviceToHost, g.stream));
     }
-    TRY(window_enqueue(*fv, v_low, v_high, lvs.as<uint64_t>(), window_cap, &nbv));
+    TRY(window_enqueue(*fv, v_low, v_high, lvs.as<uint64_t>(), window_cap, true, &nbv, &single));
     HIP_TRY(hipMemcpyAsync(&totals[1], fv->work.offsets.as<uint32_t>() + nbv, 4, hipMemcpyDeviceToHost, g.stream));
     HIP_TRY(hipStreamSynchronize(g.stream));
     if (totals[0] > window_cap || totals[1] > window_cap)

@@ -155,13 +155,21 @@ __global__ void k_directory(const uint64_t* __restrict__ C, uint64_t nc, uint32_
 //
 // All coordinates y in [low, high) with (y mod M) = (c + d) mod M for c in the
 // sorted half C and d in the sorted half D.  Thread = (period, chunk of kChunk
-// consecutive D entries).  Per wrap w in {0, 1}, c must lie in
-// [a + wM - d, b + wM - d); walking the chunk's d values downwards moves that
+// consecutive D entries, loaded up front).  Per wrap w in {0, 1}, c must lie
+// in [a + wM - d, b + wM - d); walking the chunk's d values downwards moves that
 // interval upwards, so two pointers into C only advance.  The first pointer
-// position comes from the bucket directory of C.  Output is counted (pass 1)
-// then scattered into value buckets (pass 2) and sorted within buckets.
+// position comes from the bucket directory of C.
+//
+// Output modes:
+//   kCount  count outputs per value bucket (first pass of the exact path)
+//   kEmit   scatter into bucket ranges fixed by the scanned counts (second pass)
+//   kSlots  single pass: append into kSlotsPerBucket fixed slots per value
+//           bucket; a full bucket raises the overflow flag and the caller
+//           reruns the exact two-pass path
 
-constexpr int kChunk = 32;
+constexpr int kChunk = 8;
+constexpr int kSlotsPerBucket = 16;
+enum WindowMode { kCount = 0, kEmit = 1, kSlots = 2 };
 
 struct WindowArgs {
     const uint64_t* C;
@@ -179,9 +187,10 @@ __device__ __forceinline__ uint64_t lower_bound_dir(const WindowArgs& w, uint64_
     return i;
 }
 
-template <bool kEmit>
-__global__ void k_window(WindowArgs w, uint32_t* __restrict__ hist, const uint32_t* __restrict__ offsets,
-                         uint32_t* __restrict__ cursor, uint64_t* __restrict__ out) {
+template <int kMode>
+__global__ __launch_bounds__(256) void k_window(WindowArgs w, uint32_t* __restrict__ hist,
+                                                const uint32_t* __restrict__ offsets, uint32_t* __restrict__ cursor,
+                                                uint64_t* __restrict__ out, uint32_t* __restrict__ flag) {
     uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (t >= (uint64_t)w.nper * w.nchunks) return;
     const uint32_t p = uint32_t(t / w.nchunks), chunk = uint32_t(t % w.nchunks);
@@ -189,13 +198,18 @@ __global__ void k_window(WindowArgs w, uint32_t* __restrict__ hist, const uint32
     const uint64_t a = w.low > base ? w.low - base : 0;
     const uint64_t b = (w.high - base) < w.M ? w.high - base : w.M;
     const uint64_t j0 = (uint64_t)chunk * kChunk;
-    const uint64_t j1 = j0 + kChunk < w.nd ? j0 + kChunk : w.nd;
+    const int n = int(w.nd - j0 < (uint64_t)kChunk ? w.nd - j0 : (uint64_t)kChunk);
+    uint64_t dv[kChunk];
+#pragma unroll
+    for (int e = 0; e < kChunk; ++e) dv[e] = e < n ? w.D[j0 + e] : 0;
     for (int wrap = 0; wrap < 2; ++wrap) {
         const uint64_t lw = a + (wrap ? w.M : 0), hw = b + (wrap ? w.M : 0);
         uint64_t pl = 0, pr = 0;
         bool started = false;
-        for (uint64_t j = j1; j-- > j0;) {
-            const uint64_t d = w.D[j];
+#pragma unroll
+        for (int e = kChunk - 1; e >= 0; --e) {
+            if (e >= n) continue;
+            const uint64_t d = dv[e];
             if (hw <= d) continue;
             const uint64_t cl = lw > d ? lw - d : 0, ch = hw - d;
             if (cl >= w.M) break;
@@ -210,19 +224,43 @@ __global__ void k_window(WindowArgs w, uint32_t* __restrict__ hist, const uint32
             for (uint64_t i = pl; i < pr; ++i) {
                 const uint64_t y = w.C[i] + d - (wrap ? w.M : 0) + base;
                 const uint32_t bucket = uint32_t((y - w.low) >> w.bshift);
-                if (!kEmit) {
+                if (kMode == kCount) {
                     atomicAdd(&hist[bucket], 1u);
-                } else {
+                } else if (kMode == kEmit) {
                     const uint64_t slot = (uint64_t)offsets[bucket] + atomicAdd(&cursor[bucket], 1u);
                     if (slot < w.cap) out[slot] = y;
+                } else {
+                    const uint32_t slot = atomicAdd(&hist[bucket], 1u);
+                    if (slot < (uint32_t)kSlotsPerBucket) out[(uint64_t)bucket * kSlotsPerBucket + slot] = y;
+                    else atomicOr(flag, 1u);
                 }
             }
         }
     }
 }
 
-template __global__ void k_window<false>(WindowArgs, uint32_t*, const uint32_t*, uint32_t*, uint64_t*);
-template __global__ void k_window<true>(WindowArgs, uint32_t*, const uint32_t*, uint32_t*, uint64_t*);
+// Single-pass windows, second step: bucket b holds min(count, kSlotsPerBucket)
+// values in its slots; sort them and write them at the scanned offset.
+__global__ void k_slots_compact(const uint64_t* __restrict__ slots, const uint32_t* __restrict__ offsets,
+                                uint64_t nbuckets, uint64_t* __restrict__ out, uint64_t cap) {
+    uint64_t bk = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (bk >= nbuckets) return;
+    const uint32_t s = offsets[bk];
+    uint32_t n = offsets[bk + 1] - s;
+    if (n == 0) return;
+    if (n > (uint32_t)kSlotsPerBucket) n = kSlotsPerBucket;
+    uint64_t v[kSlotsPerBucket];
+#pragma unroll
+    for (int i = 0; i < kSlotsPerBucket; ++i) v[i] = i < (int)n ? slots[bk * kSlotsPerBucket + i] : ~0ull;
+#pragma unroll
+    for (int i = 1; i < kSlotsPerBucket; ++i)
+#pragma unroll
+        for (int j = i; j > 0; --j)
+            if (v[j - 1] > v[j]) { uint64_t x = v[j]; v[j] = v[j - 1]; v[j - 1] = x; }
+#pragma unroll
+    for (int i = 0; i < kSlotsPerBucket; ++i)
+        if (i < (int)n && (uint64_t)s + i < cap) out[(uint64_t)s + i] = v[i];
+}
 
 // np.searchsorted(sorted, queries, side='left')
 __global__ void k_searchsorted(const uint64_t* __restrict__ sorted, uint64_t n, const uint64_t* __restrict__ queries,
@@ -340,12 +378,18 @@ __global__ __launch_bounds__(256) void k_masks(MatchArgs m, uint32_t chunks_tota
 }
 
 // rho[i][c] = (u_i * B) mod q for the cached chain filters c = 0..ncache-1.
+// Rows are kCachedResidues entries wide (16 bytes) whatever ncache is, so the
+// chain reads one u coordinate's residues with a single vector load.
 __global__ void k_rho(MatchArgs m) {
     uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (t >= (uint64_t)m.nu * m.ncache) return;
-    const uint32_t i = uint32_t(t / m.ncache), c = uint32_t(t % m.ncache);
-    const Modulus mod = m.mods[m.nbits + c];
-    ((uint16_t*)m.rho)[t] = uint16_t(mod32(mod64(m.us[i], mod) * mod.b, mod));
+    if (t >= (uint64_t)m.nu * kCachedResidues) return;
+    const uint32_t i = uint32_t(t / kCachedResidues), c = uint32_t(t % kCachedResidues);
+    uint16_t r = 0;
+    if (c < m.ncache) {
+        const Modulus mod = m.mods[m.nbits + c];
+        r = uint16_t(mod32(mod64(m.us[i], mod) * mod.b, mod));
+    }
+    ((uint16_t*)m.rho)[t] = r;
 }
 
 constexpr int kMatchThreads = 256;
@@ -410,12 +454,17 @@ __device__ void run_chain(const MatchArgs& m, MatchShared& sh, uint32_t wave, ui
     bool alive = lane < count;
     const uint32_t i = alive ? sh.queue[wave][lane] : 0;
     uint32_t k = m.nbits;
-    const uint32_t cached_end = m.nbits + m.ncache;
-    for (; k < cached_end; ++k) {
+    // all cached residues of this u in one 16-byte load
+    uint4 packed = make_uint4(0, 0, 0, 0);
+    if (alive && m.ncache) packed = ((const uint4*)m.rho)[i];
+    const uint32_t words[4] = {packed.x, packed.y, packed.z, packed.w};
+#pragma unroll
+    for (int c = 0; c < kCachedResidues; ++c) {
+        if ((uint32_t)c >= m.ncache) break;
         const Modulus mod = sh.mods[k];
-        const uint32_t s = __builtin_amdgcn_readlane(sigma, k - m.nbits);
+        const uint32_t s = __builtin_amdgcn_readlane(sigma, c);
         if (alive) {
-            uint32_t r = m.rho[(uint64_t)i * m.ncache + (k - m.nbits)] + s;
+            uint32_t r = ((words[c >> 1] >> (16 * (c & 1))) & 0xffffu) + s;
             r = r >= mod.q ? r - mod.q : r;
             alive = allowed_bit(sh.allow, mod.bit + r);
         }
@@ -424,6 +473,7 @@ __device__ void run_chain(const MatchArgs& m, MatchShared& sh, uint32_t wave, ui
             if (lane == 0 && live) sh.groups[wave][k / 4 + 1] += __popcll(live);
         }
         if (!live) return;
+        ++k;
     }
     U128 x = {0, 0};
     if (alive) x = sub_wide(mul_wide(m.us[i], m.B), shift);

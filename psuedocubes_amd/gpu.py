@@ -333,6 +333,7 @@ class MatchEngine:
         native.call('pcube_matcher_create', config.A, config.B, config.lo & MASK, config.lo >> 64,
                     config.hi & MASK, config.hi >> 64, native.ptr32(moduli), len(order), ctypes.byref(handle))
         self.handle = handle.value
+        self._values: np.ndarray | None = None
 
     def run(self, us: DeviceArray, vs: DeviceArray, v_first: int, tile_width: int, ntiles: int, cap: int,
             engine: int, overflow_ok: bool = False) -> tuple[np.ndarray, list[list[int]], list[float]]:
@@ -343,8 +344,10 @@ class MatchEngine:
         """
         stride = self.ng + 3
         counters = np.zeros(ntiles * stride, dtype=np.uint64)
-        values_cap = max(1, ntiles * cap)
-        values = np.zeros(2 * min(values_cap, 1 << 22), dtype=np.uint64)
+        values_cap = min(max(1, ntiles * cap), 1 << 22)
+        if self._values is None or self._values.size < 2 * values_cap:
+            self._values = np.empty(2 * values_cap, dtype=np.uint64)
+        values = self._values
         seconds = (ctypes.c_double * 2)()
         self.runtime.check()
         code = native.lib().pcube_match(self.handle, us.address, us.size, vs.address, vs.size, v_first, tile_width,
