@@ -46,7 +46,20 @@ def parse() -> argparse.Namespace:
     ap.add_argument('--cpu-seconds', type=float, default=12.0, help='bounded CPU-baseline sample')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-lease', action='store_true')
+    ap.add_argument('--copy-probe', type=int, default=0, metavar='BYTES',
+                    help='only run the plain copy kernel used to calibrate the HBM byte counters')
     return ap.parse_args()
+
+
+def measured_traffic() -> int | None:
+    """HBM bytes per match launch from the committed counter summary, if present."""
+    path = ROOT / 'profiles' / 'pmc_match_p619_batch.md'
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith('traffic_bytes_per_launch:'):
+            return int(line.split(':', 1)[1])
+    return None
 
 
 def cpu_baseline(config, seconds: float) -> dict:
@@ -88,6 +101,15 @@ def main() -> None:
 
     config = Config(619, 10**27, 3 * 10**27, A619, B619)
     runtime = Runtime(time.monotonic() + 3600, 16 * 1024**3)
+    if args.copy_probe:
+        import ctypes
+        from psuedocubes_amd import native
+        seconds = ctypes.c_double()
+        for _ in range(3):
+            native.call('pcube_copy_probe', args.copy_probe, ctypes.byref(seconds))
+        print(json.dumps(dict(copy_probe_bytes=args.copy_probe, seconds=seconds.value,
+                              gbps=2 * args.copy_probe / seconds.value / 1e9)), flush=True)
+        return
     af, bf = (GPUFocus(runtime, f) for f in config.focuses())
     matcher = MaskedMatcher(runtime, config)
     rng = random.Random(619003)
@@ -124,7 +146,7 @@ def main() -> None:
                   kernel_only=dict(value=positions / kernel, unit='focused candidate positions/s',
                                    match_ms=1000 * match_seconds, prefilter_ms=1000 * prefilter / args.steps),
                   roofline=dict(bound='valu', achieved=achieved / 1e12, peak=INT32_PEAK / 1e12, unit='TOP/s',
-                                frac=achieved / INT32_PEAK, traffic=None,
+                                frac=achieved / INT32_PEAK, traffic=measured_traffic(),
                                 kernel='k_match', ops_per_position=OPS_PER_POSITION,
                                 positions_per_launch=per_launch))
     if not args.no_lease:

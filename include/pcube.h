@@ -134,6 +134,11 @@ int pcube_lease(int matcher, int focus_u, int focus_v, uint32_t nbatches, const 
 /* out[3i..3i+2] = (low, high, borrow) of in[4i]*in[4i+1] - in[4i+2]*in[4i+3]; host arrays. */
 int pcube_arithmetic(const uint64_t *in, uint64_t *out, uint64_t count);
 
+/* --- measurement: calibration probe for the HBM byte counters ------------ */
+/* Copies `bytes` between two fresh device buffers with a plain streaming
+ * kernel; *seconds is its event-timed duration.  No reference counterpart. */
+int pcube_copy_probe(uint64_t bytes, double *seconds);
+
 #ifdef __cplusplus
 }
 #endif

@@ -506,6 +506,7 @@ const Kernel kKernels[] = {
     {"match", (const void*)k_match},
     {"match_direct", (const void*)k_match_direct},
     {"arithmetic", (const void*)k_arithmetic},
+    {"copy_probe", (const void*)k_copy_probe},
 };
 
 double now_seconds() {
@@ -923,6 +924,32 @@ int pcube_lease(int matcher, int focus_u, int focus_v, uint32_t nbatches, const 
     TRY(collect_values(*mt, ntot, host, cap, list_cap, values, values_cap, &status));
     if (seconds) seconds[3] = now_seconds() - t_start;
     return status;
+}
+
+int pcube_copy_probe(uint64_t bytes, double* seconds) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    const uint64_t n = bytes / 8;
+    if (!n) return fail(PCUBE_INVALID, "probe size must be at least 8 bytes");
+    void *src = nullptr, *dst = nullptr;
+    TRY(dev_alloc(n * 8, &src));
+    int rc = dev_alloc(n * 8, &dst);
+    if (rc) { dev_free(src); return rc; }
+    TRY(fill64((uint64_t*)src, n, 0x9e3779b97f4a7c15ull));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, g.stream));
+    hipLaunchKernelGGL(k_copy_probe, dim3(8192), dim3(256), 0, g.stream, (const uint64_t*)src, (uint64_t*)dst, n);
+    HIP_TRY(hipEventRecord(e1, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    if (seconds) *seconds = ms / 1000.0;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    dev_free(src);
+    dev_free(dst);
+    return PCUBE_OK;
 }
 
 int pcube_arithmetic(const uint64_t* in, uint64_t* out, uint64_t count) {

@@ -636,6 +636,13 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_direct(MatchArgs m) {
     if (tile != 0xffffffffu) flush_tile(m, sh, wave, lane, tile, valid, survivors);
 }
 
+// Plain streaming copy of n uint64 words: the calibration probe for the HBM
+// byte counters (FETCH_SIZE / WRITE_SIZE read for a known traffic).
+__global__ void k_copy_probe(const uint64_t* __restrict__ src, uint64_t* __restrict__ dst, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 // Wide-limb self test: out[3i..3i+2] = low, high, borrow of in[4i]*in[4i+1] - in[4i+2]*in[4i+3].
 __global__ void k_arithmetic(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, uint64_t count) {
     uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;

@@ -44,6 +44,7 @@ SIGNATURES: dict[str, list[Any]] = {
     'pcube_match': [c_int, u64, u64, u64, u64, u64, u64, u32, u64, c_int, P_u64, P_u64, u64, P_dbl],
     'pcube_lease': [c_int, c_int, c_int, u32, P_u32, P_u64, u64, u64, u64, P_u64, P_u64, P_u64, P_u64, u64, P_dbl],
     'pcube_arithmetic': [P_u64, P_u64, u64],
+    'pcube_copy_probe': [u64, P_dbl],
 }
 
 _lib: ctypes.CDLL | None = None
