@@ -242,3 +242,18 @@ def test_tile_ranges_match_oracle_geometry(config619) -> None:
     for first, end in [(0, BATCH_WIDTH), (B619 - BATCH_WIDTH, B619), (0, BATCH_WIDTH + 1)]:
         width = 100000001 if end - first > BATCH_WIDTH else 100000000
         assert tile_ranges(config619, first, end, width) == O.tile_ranges(config619, first, end, width)
+
+
+def test_reproduce_compare_baseline(tmp_path) -> None:
+    """tools/reproduce.py --compare-baseline: bitset lease receipt == per-subtile direct engine receipt."""
+    from psuedocubes_amd import reproduce
+    out = tmp_path / 'record.json'
+    summary = reproduce.main(['--first', str(LEASE_FIRST), '--batches', '2', '--compare-baseline',
+                              '--output', str(out)])
+    assert summary['status'] == 'PASS' and out.exists()
+    record = json.loads(out.read_text())
+    assert record['baseline_compared'] is True
+    assert record['result']['compute_batches'] == 2 and record['result']['subtiles_executed'] == 128
+    assert record['result']['valid_focused_candidates'] == sum(b['valid_focused_candidates'] for b in LEASE_BATCHES[:2])
+    with pytest.raises(ValueError):
+        reproduce.main(['--first', '0', '--batches', '1', '--output', str(out)])
