@@ -82,6 +82,23 @@ int pcube_focus_destroy(int focus);
  * full size, > cap), 2 invalid.  Replaces GPUFocus.window (pseudocube/gpu.py:94-114). */
 int pcube_focus_window(int focus, uint64_t low, uint64_t high, uint64_t out, uint64_t cap, uint64_t *count);
 
+/* Attach the CRT wheel generator to a focus: ring moduli in ascending order,
+ * their allowed residue counts and the concatenated (coefficient-adjusted)
+ * residues, exactly Focus.rings (pseudocube/focus.py:35-47, 108-114).  The
+ * inner ring prefix has modulus M1 < 2^32 and at most 2^21 residues; later
+ * windows of this focus are generated as t*M1 + r and need no sort.
+ * inner_modulus/inner_residues may be NULL. */
+int pcube_focus_wheel(int focus, const uint32_t *moduli, const uint32_t *sizes, const uint32_t *residues,
+                      uint32_t nrings, uint64_t *inner_modulus, uint32_t *inner_residues);
+
+/* n windows [lows[w], highs[w]) of one focus in one pass, concatenated in
+ * window order into out (capacity cap), each ascending; offsets[0..n] on the
+ * host delimit them.  OVERFLOW (nothing written, offsets[n] = total) when the
+ * total exceeds cap.  Needs the wheel.  The batched generalisation of
+ * GPUFocus.window (pseudocube/gpu.py:94-114). */
+int pcube_focus_windows(int focus, uint32_t n, const uint64_t *lows, const uint64_t *highs, uint64_t out,
+                        uint64_t cap, uint64_t *offsets);
+
 /* np.searchsorted(sorted, queries, 'left') on the device; host queries/out
  * (pseudocube/batch_experiment.py:41-42). */
 int pcube_searchsorted(uint64_t sorted, uint64_t n, const uint64_t *queries, uint64_t nq, uint64_t *out);

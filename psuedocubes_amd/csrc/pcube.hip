@@ -20,6 +20,7 @@
 
 #include "../../include/pcube.h"
 #include "pcube_device.h"
+#include "pcube_wheel.h"
 
 using namespace pcube;
 
@@ -220,16 +221,35 @@ int bucket_sort(const uint64_t* in, uint64_t n, uint64_t low, uint64_t span, uin
 // ---------------------------------------------------------------------------
 // Focus objects
 
+// CRT wheel of a focus: inner ring prefix with modulus M1 < 2^32 and its
+// sorted residues R1, outer rings tested per candidate (pcube_wheel.h).
+struct Wheel {
+    bool ready = false;
+    uint64_t M1 = 1;
+    uint32_t n1 = 0, nouter = 0, allow_words = 0, packed = 0;
+    std::vector<uint32_t> R1host;
+    Modulus outer[kMaxOuterRings];
+    Buffer R1, allow, windows, seg_first, seg_count, seg_offset, win_offsets;
+    ScanWork scan;
+    void release() {
+        R1.release(); allow.release(); windows.release(); seg_first.release(); seg_count.release();
+        seg_offset.release(); win_offsets.release();
+        scan.partial.release(); scan.partial_off.release(); scan.partial_total.release();
+    }
+};
+
 struct Focus {
     uint64_t M = 0, nc = 0, nd = 0, nbk = 0;
     uint32_t dshift = 0;
     Buffer C, D, dir;
     BucketWork work;        // window buckets and scan scratch
     Buffer slots, flag;     // single-pass window slots and their overflow flag
+    Wheel wheel;
     ~Focus() {
         C.release(); D.release(); dir.release();
         work.hist.release(); work.offsets.release(); slots.release(); flag.release();
         work.scan.partial.release(); work.scan.partial_off.release(); work.scan.partial_total.release();
+        wheel.release();
     }
 };
 std::vector<std::unique_ptr<Focus>> g_focus;
@@ -341,6 +361,185 @@ int window_run(Focus& f, uint64_t low, uint64_t high, uint64_t* out, uint64_t ca
         HIP_TRY(hipStreamSynchronize(g.stream));
     }
     *count = total;
+    return PCUBE_OK;
+}
+
+uint64_t modinv(uint64_t a, uint64_t m) {
+    // inverse of a modulo m (gcd 1), extended Euclid in signed 128-bit
+    __int128 t = 0, nt = 1, r = m, nr = a % m;
+    while (nr) {
+        __int128 q = r / nr, x;
+        x = t - q * nt; t = nt; nt = x;
+        x = r - q * nr; r = nr; nr = x;
+    }
+    if (t < 0) t += m;
+    return (uint64_t)t;
+}
+
+Modulus make_modulus(uint32_t q) {
+    Modulus m;
+    memset(&m, 0, sizeof m);
+    m.q = q;
+    m.inv = (uint32_t)((1ull << 32) / q);
+    m.r32 = (uint32_t)((1ull << 32) % q);
+    m.r64 = (uint32_t)(((unsigned __int128)1 << 64) % q);
+    return m;
+}
+
+constexpr uint32_t kMaxInnerResidues = 1u << 21;
+
+// Plan the segments of a set of windows; returns the total segment count.
+uint64_t wheel_plan(const Wheel& wh, uint32_t n, const uint64_t* lows, const uint64_t* highs,
+                    std::vector<WheelWindow>& plan, std::vector<uint64_t>& seg_first) {
+    plan.resize(n);
+    seg_first.resize(n + 1);
+    uint64_t total = 0;
+    const uint64_t S = kWheelSegment;
+    for (uint32_t w = 0; w < n; ++w) {
+        WheelWindow ww;
+        memset(&ww, 0, sizeof ww);
+        ww.low = lows[w];
+        ww.high = highs[w];
+        ww.seg_first = total;
+        seg_first[w] = total;
+        if (highs[w] > lows[w]) {
+            const uint64_t t0 = lows[w] / wh.M1, t1 = (highs[w] - 1) / wh.M1;
+            const uint64_t lo_r = lows[w] - t0 * wh.M1, hi_r = highs[w] - t1 * wh.M1;
+            ww.t0 = t0;
+            ww.nt = (uint32_t)(t1 - t0 + 1);
+            ww.ib = (uint32_t)(std::lower_bound(wh.R1host.begin(), wh.R1host.end(), lo_r) - wh.R1host.begin());
+            ww.ie = hi_r >= wh.M1 ? wh.n1
+                                  : (uint32_t)(std::lower_bound(wh.R1host.begin(), wh.R1host.end(), hi_r) - wh.R1host.begin());
+            uint64_t segs;
+            if (ww.nt == 1) {
+                ww.s0 = ww.ie > ww.ib ? (uint32_t)((ww.ie - ww.ib + S - 1) / S) : 0;
+                segs = ww.s0;
+            } else {
+                ww.s0 = (uint32_t)((wh.n1 - ww.ib + S - 1) / S);
+                ww.sm = (uint32_t)((wh.n1 + S - 1) / S);
+                ww.sl = (uint32_t)((ww.ie + S - 1) / S);
+                segs = ww.s0 + (uint64_t)(ww.nt - 2) * ww.sm + ww.sl;
+            }
+            if (segs >= (1ull << 32)) segs = ~0ull;  // rejected by wheel_count
+            total += segs;
+        }
+        plan[w] = ww;
+    }
+    seg_first[n] = total;
+    return total;
+}
+
+__global__ void k_window_offsets(const uint32_t* __restrict__ wave_offset, const uint64_t* __restrict__ seg_first,
+                                 uint32_t n, uint64_t* __restrict__ out) {
+    uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w <= n) out[w] = wave_offset[seg_first[w] * kWheelWaves];
+}
+
+// Count pass + scan for n windows; leaves per-window output offsets (n + 1,
+// the last being the total) in wh.win_offsets.  Enqueue only.
+int wheel_count(Wheel& wh, uint32_t n, const uint64_t* lows, const uint64_t* highs, uint64_t* nseg_out) {
+    std::vector<WheelWindow> plan;
+    std::vector<uint64_t> seg_first;
+    const uint64_t nseg = wheel_plan(wh, n, lows, highs, plan, seg_first);
+    if (nseg * kWheelWaves > (uint64_t)kScanSegment * kScanSegment) return fail(PCUBE_INVALID, "window set too large for one pass");
+    *nseg_out = nseg;
+    TRY(wh.windows.ensure(std::max<uint64_t>(1, n) * sizeof(WheelWindow)));
+    TRY(wh.seg_first.ensure((n + 1) * 8));
+    const uint64_t nwaves = nseg * kWheelWaves;
+    TRY(wh.seg_count.ensure(std::max<uint64_t>(1, nwaves) * 4));
+    TRY(wh.seg_offset.ensure((nwaves + 1) * 4));
+    TRY(wh.win_offsets.ensure((n + 1) * 8));
+    if (n) HIP_TRY(hipMemcpyAsync(wh.windows.p, plan.data(), n * sizeof(WheelWindow), hipMemcpyHostToDevice, g.stream));
+    HIP_TRY(hipMemcpyAsync(wh.seg_first.p, seg_first.data(), (n + 1) * 8, hipMemcpyHostToDevice, g.stream));
+    if (nseg) {
+        WheelArgs a;
+        memset(&a, 0, sizeof a);
+        a.R1 = wh.R1.as<uint2>();
+        a.windows = wh.windows.as<WheelWindow>();
+        a.seg_first = wh.seg_first.as<uint64_t>();
+        a.allow = wh.allow.as<uint32_t>();
+        a.wave_count = wh.seg_count.as<uint32_t>();
+        a.M1 = wh.M1; a.n1 = wh.n1; a.nwin = n; a.nouter = wh.nouter; a.allow_words = wh.allow_words;
+        a.packed = wh.packed;
+        for (uint32_t j = 0; j < wh.nouter; ++j) a.outer[j] = wh.outer[j];
+        {
+            void* args[] = {&a};
+            HIP_TRY(hipLaunchKernel(wheel_kernel<false>(wh.packed, wh.nouter), dim3((uint32_t)nseg),
+                                    dim3(kWheelThreads), args, 0, g.stream));
+        }
+        TRY(scan(wh.seg_count.as<uint32_t>(), wh.seg_offset.as<uint32_t>(), nwaves, wh.scan));
+    } else {
+        TRY(fill32(wh.seg_offset.as<uint32_t>(), 1, 0));
+    }
+    hipLaunchKernelGGL(k_window_offsets, dim3(blocks_for(n + 1, 256)), dim3(256), 0, g.stream,
+                       (const uint32_t*)wh.seg_offset.as<uint32_t>(), (const uint64_t*)wh.seg_first.as<uint64_t>(), n,
+                       wh.win_offsets.as<uint64_t>());
+    return PCUBE_OK;
+}
+
+// Write pass after wheel_count (same windows); values beyond cap are dropped.
+int wheel_write(Wheel& wh, uint32_t n, uint64_t nseg, uint64_t* out, uint64_t cap) {
+    if (!nseg) return PCUBE_OK;
+    WheelArgs a;
+    memset(&a, 0, sizeof a);
+    a.R1 = wh.R1.as<uint2>();
+    a.windows = wh.windows.as<WheelWindow>();
+    a.seg_first = wh.seg_first.as<uint64_t>();
+    a.allow = wh.allow.as<uint32_t>();
+    a.wave_offset = wh.seg_offset.as<uint32_t>();
+    a.out = out;
+    a.cap = cap;
+    a.M1 = wh.M1; a.n1 = wh.n1; a.nwin = n; a.nouter = wh.nouter; a.allow_words = wh.allow_words;
+    a.packed = wh.packed;
+    for (uint32_t j = 0; j < wh.nouter; ++j) a.outer[j] = wh.outer[j];
+    void* args[] = {&a};
+    HIP_TRY(hipLaunchKernel(wheel_kernel<true>(wh.packed, wh.nouter), dim3((uint32_t)nseg), dim3(kWheelThreads),
+                            args, 0, g.stream));
+    return PCUBE_OK;
+}
+
+// Windows of one focus into out; offsets (n + 1) on the host.  Returns
+// OVERFLOW, writing nothing, when the total exceeds cap.
+int focus_windows(Focus& f, uint32_t n, const uint64_t* lows, const uint64_t* highs, uint64_t* out, uint64_t cap,
+                  uint64_t* offsets) {
+    if (!f.wheel.ready) {
+        if (n != 1) return fail(PCUBE_INVALID, "multi-window generation needs the focus wheel");
+        uint64_t count = 0;
+        TRY(window_run(f, lows[0], highs[0], out, cap, &count));
+        offsets[0] = 0;
+        offsets[1] = count;
+        if (count > cap) return fail(PCUBE_OVERFLOW, "focus buffer overflow: %llu coordinates, capacity %llu",
+                                     (unsigned long long)count, (unsigned long long)cap);
+        return PCUBE_OK;
+    }
+    uint64_t nseg = 0;
+    TRY(wheel_count(f.wheel, n, lows, highs, &nseg));
+    HIP_TRY(hipMemcpyAsync(offsets, f.wheel.win_offsets.p, (n + 1) * 8, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    if (offsets[n] > cap) return fail(PCUBE_OVERFLOW, "focus buffer overflow: %llu coordinates, capacity %llu",
+                                      (unsigned long long)offsets[n], (unsigned long long)cap);
+    TRY(wheel_write(f.wheel, n, nseg, out, cap));
+    return PCUBE_OK;
+}
+
+// One lease-wide window: the wheel when the focus has one, else the exact
+// two-pass directory walk.  offsets[1] receives the size.
+int lease_window(Focus& f, uint64_t low, uint64_t high, uint64_t* out, uint64_t cap, uint64_t* offsets) {
+    if (f.wheel.ready) {
+        const int rc = focus_windows(f, 1, &low, &high, out, cap, offsets);
+        if (rc == PCUBE_OVERFLOW) return fail(PCUBE_OVERFLOW, "lease focus window overflow (%llu > %llu)",
+                                              (unsigned long long)offsets[1], (unsigned long long)cap);
+        return rc;
+    }
+    uint64_t nb = 0;
+    bool single = false;
+    TRY(window_enqueue(f, low, high, out, cap, true, &nb, &single));
+    uint32_t total = 0;
+    HIP_TRY(hipMemcpyAsync(&total, f.work.offsets.as<uint32_t>() + nb, 4, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    offsets[0] = 0;
+    offsets[1] = total;
+    if (total > cap) return fail(PCUBE_OVERFLOW, "lease focus window overflow (%u > %llu)", total, (unsigned long long)cap);
     return PCUBE_OK;
 }
 
@@ -498,6 +697,8 @@ const Kernel kKernels[] = {
     {"window_emit", (const void*)k_window<kEmit>},
     {"window_slots", (const void*)k_window<kSlots>},
     {"slots_compact", (const void*)k_slots_compact},
+    {"wheel_count", (const void*)k_wheel<false, 4>},
+    {"wheel_write", (const void*)k_wheel<true, 4>},
     {"bucket_sort", (const void*)k_bucket_sort},
     {"scan_local", (const void*)k_scan_local},
     {"rows", (const void*)k_rows},
@@ -670,10 +871,102 @@ int pcube_focus_window(int focus, uint64_t low, uint64_t high, uint64_t out, uin
     Focus* f = focus_at(focus);
     if (!f || low > high) return fail(PCUBE_INVALID, "invalid focus window");
     if (low == high) return PCUBE_OK;
-    uint64_t total = 0;
-    TRY(window_run(*f, low, high, (uint64_t*)out, cap, &total));
-    *count = total;
-    if (total > cap) return fail(PCUBE_OVERFLOW, "focus buffer overflow: %llu coordinates, capacity %llu", (unsigned long long)total, (unsigned long long)cap);
+    uint64_t offsets[2] = {0, 0};
+    const int rc = focus_windows(*f, 1, &low, &high, (uint64_t*)out, cap, offsets);
+    *count = offsets[1];
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    return PCUBE_OK;
+}
+
+int pcube_focus_wheel(int focus, const uint32_t* moduli, const uint32_t* sizes, const uint32_t* residues,
+                      uint32_t nrings, uint64_t* inner_modulus, uint32_t* inner_residues) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Focus* f = focus_at(focus);
+    if (!f) return fail(PCUBE_INVALID, "unknown focus");
+    uint64_t product = 1;
+    for (uint32_t k = 0; k < nrings; ++k) product *= moduli[k];
+    if (product != f->M) return fail(PCUBE_INVALID, "ring moduli do not multiply to the focus modulus");
+    Wheel& wh = f->wheel;
+    wh.release();
+    wh.ready = false;
+    uint64_t M1 = 1, n1 = 1;
+    uint32_t k = 0;
+    while (k < nrings && M1 * moduli[k] < (1ull << 32) && n1 * sizes[k] <= kMaxInnerResidues) {
+        M1 *= moduli[k];
+        n1 *= sizes[k];
+        ++k;
+    }
+    if (nrings - k > (uint32_t)kMaxOuterRings) return fail(PCUBE_INVALID, "too many outer rings for the wheel");
+    // inner residues by CRT: y = sum a_i * (M1/q_i) * ((M1/q_i)^-1 mod q_i) mod M1
+    std::vector<uint64_t> cur(1, 0), next;
+    uint64_t offset = 0;
+    for (uint32_t i = 0; i < k; ++i) {
+        const uint64_t q = moduli[i], co = M1 / q, lift = co * modinv(co % q, q) % M1;
+        next.clear();
+        next.reserve(cur.size() * sizes[i]);
+        for (uint64_t v : cur)
+            for (uint32_t e = 0; e < sizes[i]; ++e) next.push_back((v + (uint64_t)residues[offset + e] * lift) % M1);
+        cur.swap(next);
+        offset += sizes[i];
+    }
+    wh.R1host.assign(cur.begin(), cur.end());
+    std::sort(wh.R1host.begin(), wh.R1host.end());
+    wh.M1 = M1;
+    wh.n1 = (uint32_t)wh.R1host.size();
+    wh.nouter = nrings - k;
+    std::vector<uint32_t> bits;
+    uint32_t bit = 0;
+    for (uint32_t j = 0; j < wh.nouter; ++j) {
+        const uint32_t q = moduli[k + j];
+        Modulus m = make_modulus(q);
+        m.b = (uint32_t)(M1 % q);
+        m.bit = bit;
+        bits.resize((bit + q + 31) / 32 + 1, 0);
+        for (uint32_t e = 0; e < sizes[k + j]; ++e) {
+            const uint32_t idx = bit + residues[offset + e];
+            bits[idx / 32] |= 1u << (idx % 32);
+        }
+        offset += sizes[k + j];
+        bit += q;
+        wh.outer[j] = m;
+    }
+    if (bits.empty()) bits.push_back(0);
+    if (bits.size() > (size_t)kMaxOuterBits / 32) return fail(PCUBE_INVALID, "outer rings too large for the wheel");
+    wh.allow_words = (uint32_t)bits.size();
+    // entries (r, outer residues r mod q_j as bytes when every outer q < 256 and there are at most 4)
+    wh.packed = wh.nouter <= 4 ? 1u : 0u;
+    for (uint32_t j = 0; j < wh.nouter; ++j)
+        if (wh.outer[j].q >= 256) wh.packed = 0;
+    std::vector<uint32_t> entries(2 * (uint64_t)wh.n1);
+    for (uint32_t i = 0; i < wh.n1; ++i) {
+        const uint32_t r = wh.R1host[i];
+        uint32_t packed = 0;
+        if (wh.packed)
+            for (uint32_t j = 0; j < wh.nouter; ++j) packed |= (r % wh.outer[j].q) << (8 * j);
+        entries[2 * i] = r;
+        entries[2 * i + 1] = packed;
+    }
+    TRY(wh.R1.ensure(std::max<uint64_t>(8, (uint64_t)wh.n1 * 8)));
+    TRY(wh.allow.ensure(bits.size() * 4));
+    HIP_TRY(hipMemcpyAsync(wh.R1.p, entries.data(), (uint64_t)wh.n1 * 8, hipMemcpyHostToDevice, g.stream));
+    HIP_TRY(hipMemcpyAsync(wh.allow.p, bits.data(), bits.size() * 4, hipMemcpyHostToDevice, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    wh.ready = true;
+    if (inner_modulus) *inner_modulus = M1;
+    if (inner_residues) *inner_residues = wh.n1;
+    return PCUBE_OK;
+}
+
+int pcube_focus_windows(int focus, uint32_t n, const uint64_t* lows, const uint64_t* highs, uint64_t out,
+                        uint64_t cap, uint64_t* offsets) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Focus* f = focus_at(focus);
+    if (!f) return fail(PCUBE_INVALID, "unknown focus");
+    for (uint32_t w = 0; w < n; ++w)
+        if (lows[w] > highs[w]) return fail(PCUBE_INVALID, "window %u has low > high", w);
+    TRY(focus_windows(*f, n, lows, highs, (uint64_t*)out, cap, offsets));
+    HIP_TRY(hipStreamSynchronize(g.stream));
     return PCUBE_OK;
 }
 
@@ -832,18 +1125,15 @@ int pcube_lease(int matcher, int focus_u, int focus_v, uint32_t nbatches, const 
     hipEvent_t ev[4];
     for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
     HIP_TRY(hipEventRecord(ev[0], g.stream));
-    uint64_t nbu = 0, nbv = 0;
-    uint32_t totals[2] = {0, 0};
-    bool single = false;
+    uint64_t totals[2] = {0, 0};
+    uint64_t offs[2] = {0, 0};
     if (u_high > u_low) {
-        TRY(window_enqueue(*fu, u_low, u_high, lus.as<uint64_t>(), window_cap, true, &nbu, &single));
-        HIP_TRY(hipMemcpyAsync(&totals[0], fu->work.offsets.as<uint32_t>() + nbu, 4, hipMemcpyDeviceToHost, g.stream));
+        TRY(lease_window(*fu, u_low, u_high, lus.as<uint64_t>(), window_cap, offs));
+        totals[0] = offs[1];
     }
-    TRY(window_enqueue(*fv, v_low, v_high, lvs.as<uint64_t>(), window_cap, true, &nbv, &single));
-    HIP_TRY(hipMemcpyAsync(&totals[1], fv->work.offsets.as<uint32_t>() + nbv, 4, hipMemcpyDeviceToHost, g.stream));
+    TRY(lease_window(*fv, v_low, v_high, lvs.as<uint64_t>(), window_cap, offs));
+    totals[1] = offs[1];
     HIP_TRY(hipStreamSynchronize(g.stream));
-    if (totals[0] > window_cap || totals[1] > window_cap)
-        return fail(PCUBE_OVERFLOW, "lease focus window overflow (%u, %u > %llu)", totals[0], totals[1], (unsigned long long)window_cap);
     // subtile bounds inside the lease arrays
     // (u_min, u_end) queries into the u window, (v_first, v_end) into the v window
     std::vector<uint64_t> uq(ntot * 2), vq(ntot * 2);

@@ -395,6 +395,7 @@ __global__ void k_rho(MatchArgs m) {
 constexpr int kMatchThreads = 256;
 constexpr int kMatchWaves = kMatchThreads / kWave;
 constexpr int kQueue = 2 * kWave;
+constexpr int kWordPrefetch = 5;   // 64-word steps of one row whose loads are issued together
 
 struct MatchShared {
     uint32_t allow[kMaxAllowWords];
@@ -533,8 +534,12 @@ __global__ __launch_bounds__(kMatchThreads) void k_match(MatchArgs m) {
             mrow[k] = m.masks + (uint64_t)(m.row_base[k] + (k < m.nbits ? row.r[k] : 0)) * m.words;
         const uint32_t wl = row.left >> 6, wr = (row.right + 63) >> 6;
         uint32_t queued = 0;
-        for (uint32_t wbase = wl; wbase < wr; wbase += kWave) {
-            const uint32_t w = wbase + lane;
+        for (uint32_t cbase = wl; cbase < wr; cbase += kWave * kWordPrefetch) {
+          // all bitset words of up to kWordPrefetch steps are requested before any is used
+          uint64_t chunk[kWordPrefetch];
+#pragma unroll
+          for (int step = 0; step < kWordPrefetch; ++step) {
+            const uint32_t w = cbase + step * kWave + lane;
             uint64_t bits = 0;
             if (w < wr) {
                 bits = ~0ull;
@@ -542,8 +547,17 @@ __global__ __launch_bounds__(kMatchThreads) void k_match(MatchArgs m) {
                 if (base < row.left) bits &= ~0ull << (row.left - base);
                 if (row.right < base + 64) bits &= (1ull << (row.right - base)) - 1ull;
                 valid += __popcll(bits);
-                for (uint32_t k = 0; k < m.nbits; ++k) bits &= mrow[k][w];
+#pragma unroll
+                for (int k = 0; k < kMaxBitsetFilters; ++k)
+                    if ((uint32_t)k < m.nbits) bits &= mrow[k][w];
             }
+            chunk[step] = bits;
+          }
+#pragma unroll
+          for (int step = 0; step < kWordPrefetch; ++step) {
+            if (cbase + step * kWave >= wr) break;
+            const uint32_t w = cbase + step * kWave + lane;
+            uint64_t bits = chunk[step];
             // wave-level compaction of the surviving u indices into the queue
             for (;;) {
                 const bool has = bits != 0;
@@ -568,6 +582,7 @@ __global__ __launch_bounds__(kMatchThreads) void k_match(MatchArgs m) {
                     wave_sync();
                 }
             }
+          }
         }
         if (queued) {
             wave_sync();

@@ -162,7 +162,7 @@ def focus_terms(halves: tuple[tuple[tuple[int, ...], ...], tuple[tuple[int, ...]
 class GPUFocus:
     """Sorted CRT half arrays resident on the device (`pseudocube/gpu.py:70-114`)."""
 
-    def __init__(self, runtime: Runtime, focus: Focus) -> None:
+    def __init__(self, runtime: Runtime, focus: Focus, wheel: bool = True) -> None:
         self.runtime, self.focus = runtime, focus
         first, second = focus.halves()
         self.half_sizes = [prod(map(len, first)), prod(map(len, second))]
@@ -180,7 +180,19 @@ class GPUFocus:
         # D is sorted here too (the reference leaves it unsorted); C is sorted in both
         self.C = DeviceArray(c_addr.value, nc.value, owner=False)
         self.D = DeviceArray(d_addr.value, nd.value, owner=False)
-        runtime.compile('window_count')
+        self.wheel = None
+        if wheel:
+            # windows from the CRT wheel: inner ring prefix residues plus outer-ring tests
+            moduli = np.array([q for q, _ in focus.rings] or [1], dtype=np.uint32)
+            counts = np.array([len(r) for _, r in focus.rings] or [1], dtype=np.uint32)
+            residues = np.array([a for _, r in focus.rings for a in r] or [0], dtype=np.uint32)
+            inner, n1 = ctypes.c_uint64(), ctypes.c_uint32()
+            native.call('pcube_focus_wheel', self.handle, native.ptr32(moduli), native.ptr32(counts),
+                        native.ptr32(residues), len(focus.rings), ctypes.byref(inner), ctypes.byref(n1))
+            self.wheel = dict(inner_modulus=inner.value, inner_residues=n1.value)
+            runtime.compile('wheel_count')
+        else:
+            runtime.compile('window_count')
         runtime.check()
 
     def window(self, low: int, high: int, cap: int = 1000000) -> DeviceArray:
@@ -202,6 +214,27 @@ class GPUFocus:
         out.size = count.value
         self.runtime.check()
         return out
+
+    def windows(self, lows: list[int], highs: list[int], cap: int | None = None) -> tuple[DeviceArray, list[int]]:
+        """Many windows in one pass: (concatenated sorted coordinates, offsets[0..n])."""
+        if self.wheel is None:
+            raise ValueError('multi-window generation needs the focus wheel')
+        n = len(lows)
+        lo = np.array(lows, dtype=np.uint64)
+        hi = np.array(highs, dtype=np.uint64)
+        offsets = np.zeros(n + 1, dtype=np.uint64)
+        if cap is None:
+            expect = sum(h - l for l, h in zip(lows, highs)) * self.focus.count / self.focus.modulus
+            cap = int(expect * 1.05) + 64 * n + 4096
+        out = DeviceArray.empty(max(1, cap))
+        self.runtime.check()
+        code = native.lib().pcube_focus_windows(self.handle, n, native.ptr64(lo), native.ptr64(hi), out.address,
+                                                cap, native.ptr64(offsets))
+        if code == native.OVERFLOW:
+            return self.windows(lows, highs, int(offsets[n]) + 1)
+        native.check(code, 'pcube_focus_windows')
+        out.size = int(offsets[n])
+        return out, [int(x) for x in offsets]
 
     def __del__(self) -> None:
         if native._lib is not None and hasattr(self, 'handle'):

@@ -94,12 +94,40 @@ def test_small_config_windows_all_periods(runtime) -> None:
     for p in (7, 13, 19, 31):
         config = small_config(p)
         for focus in config.focuses():
-            gpu, cpu = GPUFocus(runtime, focus), O.Focus(focus)
-            M = focus.modulus
-            for low, high in [(0, M), (1, 3 * M - 1), (M - 1, M + 2), (0, 4000 * M), (17, 17)]:
-                assert np.array_equal(gpu.window(low, high).get(), cpu.window(low, high, cap=10**7)), (p, low, high)
-            with pytest.raises(OverflowError):
-                gpu.window(0, 4097 * M)
+            cpu = O.Focus(focus)
+            for gpu in (GPUFocus(runtime, focus), GPUFocus(runtime, focus, wheel=False)):
+                M = focus.modulus
+                for low, high in [(0, M), (1, 3 * M - 1), (M - 1, M + 2), (0, 4000 * M), (17, 17)]:
+                    assert np.array_equal(gpu.window(low, high).get(), cpu.window(low, high, cap=10**7)), (p, low, high)
+                with pytest.raises(OverflowError):
+                    gpu.window(0, 4097 * M)
+
+
+def test_directory_walk_windows_equal_oracle(runtime, config619, oracle_focuses619) -> None:
+    # the exact directory-walk generator (used when a focus has no wheel) on p=619 windows
+    rng = random.Random(77)
+    for focus, cpu in zip(config619.focuses(), oracle_focuses619):
+        gpu = GPUFocus(runtime, focus, wheel=False)
+        M = focus.modulus
+        for low, high in [(0, 9_370_000_000), (M - 10**9, M + 10**9)] + \
+                [(s, s + 6_400_000_000) for s in (rng.randrange(2 * M) for _ in range(4))]:
+            assert np.array_equal(gpu.window(low, high).get(), cpu.window(low, high)), (low, high)
+
+
+def test_multi_window_generation(focuses619, oracle_focuses619) -> None:
+    rng = random.Random(9)
+    for gpu, cpu in zip(focuses619, oracle_focuses619):
+        M = gpu.focus.modulus
+        lows = [rng.randrange(2 * M) for _ in range(300)]
+        highs = [lo + int(10 ** rng.uniform(0, 10)) for lo in lows]
+        highs[3] = lows[3]
+        lows[7], highs[7] = M - 5, M + 5
+        values, offsets = gpu.windows(lows, highs)
+        host = values.get()
+        for w, (lo, hi) in enumerate(zip(lows, highs)):
+            assert np.array_equal(host[offsets[w]:offsets[w + 1]], cpu.window(lo, hi)), w
+        small, small_offsets = gpu.windows(lows[:5], highs[:5], cap=1)   # grows past an undersized cap
+        assert small_offsets == [x - offsets[0] for x in offsets[:6]]
 
 
 def test_device_searchsorted(runtime) -> None:
