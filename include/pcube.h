@@ -147,6 +147,40 @@ int pcube_lease(int matcher, int focus_u, int focus_v, uint32_t nbatches, const 
                 uint64_t *batch_sizes, uint64_t *counters, uint64_t *values, uint64_t values_cap,
                 double *seconds);
 
+/* --- batched form: many uneven windows, mixed prime bounds, one device pass --- */
+
+/* A batched engine over up to 3 matchers (configurations sharing A, B and
+ * (lo, hi], e.g. p = 313, 433, 619) and the two focus wheels.  cap is the
+ * per-subtile output capacity; the persistent match kernel runs
+ * blocks_per_cu blocks of 256 threads per compute unit; use_graph captures
+ * the fixed-grid launch sequence in a hipGraph on first use and replays it.
+ * No reference counterpart as one call: it runs what
+ * tools/run_production.py:51-83 / pseudocube/optimized.py:81-90 run window
+ * by window, for many windows at once. */
+int pcube_batch_create(const int *matchers, uint32_t ncfg, int focus_u, int focus_v, uint64_t cap,
+                       uint32_t blocks_per_cu, int use_graph, int *engine);
+int pcube_batch_destroy(int engine);
+
+/* Run nwin windows.  Window w uses configuration win_cfg[w], owns
+ * win_tiles[w] consecutive subtiles of `tiles` ((v_first, v_end, u_min,
+ * u_end) each, as tile_ranges() gives), has subtile width tile_width[w] and
+ * engine mode[w] (0 residue bitsets, 1 direct).  Rows are handed to the
+ * persistent kernel rows_per_chunk at a time.  Outputs as pcube_lease:
+ * bounds (4 per subtile), sizes (nu, nv per window), counters (per window
+ * win_tiles[w] x (ng + 3) of its configuration, concatenated), values
+ * (emitted pairs in subtile order), seconds (windows, 0, match, total) and
+ * the engine's total graph launches. */
+int pcube_batch_run(int engine, uint32_t nwin, const uint32_t *win_cfg, const uint32_t *win_tiles,
+                    const uint64_t *tiles, const uint64_t *tile_width, const uint32_t *mode, uint32_t rows_per_chunk,
+                    uint64_t *bounds, uint64_t *sizes, uint64_t *counters, uint64_t *values, uint64_t values_cap,
+                    double *seconds, uint64_t *graph_launches);
+
+/* --- host verification (CPU): the exact local predicate of every emitted value
+ * For each (low, high) value: local_ok = local3(n, p) (pseudocube/oracle.py:36-41),
+ * is_cube = n is a perfect cube, cube_root = floor(n^(1/3)) (review/verify_cube.py:13-17). */
+int pcube_host_verify(const uint64_t *limbs, uint64_t n, uint32_t p, uint8_t *local_ok, uint8_t *is_cube,
+                      uint64_t *cube_root);
+
 /* --- self test: replaces the `arithmetic` kernel (kernels/pseudocube.cuh:6-11) */
 /* out[3i..3i+2] = (low, high, borrow) of in[4i]*in[4i+1] - in[4i+2]*in[4i+3]; host arrays. */
 int pcube_arithmetic(const uint64_t *in, uint64_t *out, uint64_t count);

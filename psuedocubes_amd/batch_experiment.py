@@ -77,9 +77,17 @@ def prepare(runtime: Runtime, config: Config, af: GPUFocus, bf: GPUFocus,
     return prepare_arrays(runtime, config, first, end, width, us, vs, started)
 
 
+def python_witnesses(values: list[int], p: int) -> list[dict[str, object]]:
+    """Host verification with the pure-Python predicate (the reference's own path)."""
+    if any(not local3(n, p) for n in values):
+        raise ArithmeticError('GPU/CPU disagreement')
+    return [verify(n, p) for n in values if not is_perfect_cube(n)]
+
+
 def batch_row(config: Config, filter_groups: list[list[int]], ranges: list[tuple[int, int, int, int]],
               bounds: list[tuple[int, int, int, int]], nv: int, counters: np.ndarray, per_tile: list[list[int]],
-              output_cap: int, timings: dict[str, float], started: float) -> dict[str, Any]:
+              output_cap: int, timings: dict[str, float], started: float,
+              witnesses: Any = python_witnesses) -> dict[str, Any]:
     """Host collection and verification of one batch (`pseudocube/batch_experiment.py:121-167`).
 
     Raises ArithmeticError on underflow, duplicates, a value failing the
@@ -99,9 +107,9 @@ def batch_row(config: Config, filter_groups: list[list[int]], ranges: list[tuple
     verification_started = time.perf_counter()
     if len(all_values) != len(set(all_values)):
         raise ArithmeticError('duplicate canonical output')
-    if any(not config.lo < n <= config.hi or not local3(n, config.p) for n in all_values):
+    if any(not config.lo < n <= config.hi for n in all_values):
         raise ArithmeticError('GPU/CPU disagreement')
-    hits = [verify(n, config.p) for n in all_values if not is_perfect_cube(n)]
+    hits = witnesses(all_values, config.p)
     if any(not row['valid_pseudocube'] for row in hits):
         raise ArithmeticError('independent witness verification failed')
     scopes = []

@@ -687,6 +687,85 @@ int batch_status(const Matcher& mt, const std::vector<uint64_t>& c, uint64_t nti
     return PCUBE_OK;
 }
 
+// out[q] = searchsorted(data[begin[q] .. begin[q] + len[q]), key[q], 'left')
+__global__ void k_searchsorted_seg(const uint64_t* __restrict__ data, const uint64_t* __restrict__ begin,
+                                   const uint64_t* __restrict__ len, const uint64_t* __restrict__ key, uint64_t nq,
+                                   uint64_t* __restrict__ out) {
+    uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (t >= nq) return;
+    const uint64_t* a = data + begin[t];
+    uint64_t l = 0, h = len[t];
+    const uint64_t k = key[t];
+    while (l < h) {
+        const uint64_t m = l + ((h - l) >> 1);
+        if (a[m] < k) l = m + 1; else h = m;
+    }
+    out[t] = l;
+}
+
+// Batched engine: configurations, focus wheels and the grow-only buffers one
+// batched launch needs, plus the captured launch sequence.
+struct BatchEngine {
+    std::vector<int> matchers;
+    int focus_u = -1, focus_v = -1;
+    uint64_t cap = 8192, list_cap = 1u << 22;
+    uint32_t blocks_match = 1024;
+    bool use_graph = true;
+    BatchArgs args;
+    Buffer win, totals, rows, counters, list, list_count, error, cursor, masks, rho, lus, lvs;
+    Buffer q_begin, q_len, q_key, q_out;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<const void*> key;
+    uint64_t launches = 0, graph_launches = 0;
+    ~BatchEngine() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (Buffer* b : {&win, &totals, &rows, &counters, &list, &list_count, &error, &cursor, &masks, &rho, &lus,
+                          &lvs, &q_begin, &q_len, &q_key, &q_out})
+            b->release();
+    }
+};
+std::vector<std::unique_ptr<BatchEngine>> g_batch;
+
+constexpr uint32_t kFillBlocks = 1024, kRowBlocks = 2048, kMaskBlocks = 4096, kRhoBlocks = 2048;
+
+// The fixed-grid launch sequence of one batched pass.
+int batch_sequence(BatchEngine& e) {
+    hipLaunchKernelGGL(k_batch_fill, dim3(kFillBlocks), dim3(256), 0, g.stream, e.args);
+    hipLaunchKernelGGL(k_batch_rows, dim3(kRowBlocks), dim3(256), 0, g.stream, e.args);
+    hipLaunchKernelGGL(k_batch_masks, dim3(kMaskBlocks), dim3(256), 0, g.stream, e.args);
+    hipLaunchKernelGGL(k_batch_rho, dim3(kRhoBlocks), dim3(256), 0, g.stream, e.args);
+    hipLaunchKernelGGL(k_batch_match, dim3(e.blocks_match), dim3(kMatchThreads), 0, g.stream, e.args);
+    return PCUBE_OK;
+}
+
+// Launch the sequence, through the captured graph when enabled; the graph is
+// (re)captured whenever a buffer it refers to has moved.
+int batch_launch(BatchEngine& e) {
+    ++e.launches;
+    if (!e.use_graph) return batch_sequence(e);
+    std::vector<const void*> key = {e.win.p, e.totals.p, e.rows.p, e.counters.p, e.list.p, e.list_count.p,
+                                    e.error.p, e.cursor.p};
+    if (!e.exec || key != e.key) {
+        if (e.exec) (void)hipGraphExecDestroy(e.exec);
+        if (e.graph) (void)hipGraphDestroy(e.graph);
+        e.exec = nullptr;
+        e.graph = nullptr;
+        HIP_TRY(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal));
+        int rc = batch_sequence(e);
+        hipGraph_t graph = nullptr;
+        HIP_TRY(hipStreamEndCapture(g.stream, &graph));
+        if (rc) return rc;
+        e.graph = graph;
+        HIP_TRY(hipGraphInstantiate(&e.exec, e.graph, nullptr, nullptr, 0));
+        e.key = key;
+    }
+    ++e.graph_launches;
+    HIP_TRY(hipGraphLaunch(e.exec, g.stream));
+    return PCUBE_OK;
+}
+
 struct Kernel {
     const char* name;
     const void* fn;
@@ -708,6 +787,10 @@ const Kernel kKernels[] = {
     {"match_direct", (const void*)k_match_direct},
     {"arithmetic", (const void*)k_arithmetic},
     {"copy_probe", (const void*)k_copy_probe},
+    {"batch_rows", (const void*)k_batch_rows},
+    {"batch_masks", (const void*)k_batch_masks},
+    {"batch_rho", (const void*)k_batch_rho},
+    {"batch_match", (const void*)k_batch_match},
 };
 
 double now_seconds() {
@@ -1214,6 +1297,329 @@ int pcube_lease(int matcher, int focus_u, int focus_v, uint32_t nbatches, const 
     TRY(collect_values(*mt, ntot, host, cap, list_cap, values, values_cap, &status));
     if (seconds) seconds[3] = now_seconds() - t_start;
     return status;
+}
+
+int pcube_batch_create(const int* matchers, uint32_t ncfg, int focus_u, int focus_v, uint64_t cap,
+                       uint32_t blocks_per_cu, int use_graph, int* engine) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    if (ncfg == 0 || ncfg > (uint32_t)kMaxConfigs) return fail(PCUBE_INVALID, "1 to %d configurations per batch", kMaxConfigs);
+    Focus* fu = focus_at(focus_u);
+    Focus* fv = focus_at(focus_v);
+    if (!fu || !fv || fu == fv || !fu->wheel.ready || !fv->wheel.ready)
+        return fail(PCUBE_INVALID, "batched form needs the two distinct focus wheels");
+    auto e = std::make_unique<BatchEngine>();
+    memset(&e->args, 0, sizeof e->args);
+    for (uint32_t c = 0; c < ncfg; ++c) {
+        Matcher* mt = matcher_at(matchers[c]);
+        if (!mt) return fail(PCUBE_INVALID, "unknown matcher");
+        Matcher* m0 = matcher_at(matchers[0]);
+        if (mt->A != m0->A || mt->B != m0->B || mt->lo.lo != m0->lo.lo || mt->lo.hi != m0->lo.hi ||
+            mt->hi.lo != m0->hi.lo || mt->hi.hi != m0->hi.hi)
+            return fail(PCUBE_INVALID, "batched configurations must share A, B and the interval");
+        BatchConfig& bc = e->args.cfg[c];
+        bc.mods = mt->d_mods.as<Modulus>();
+        bc.allow = mt->d_allow.as<uint32_t>();
+        bc.nres = mt->nres; bc.nbits = mt->nbits; bc.ncache = mt->ncache; bc.ng = mt->ng;
+        bc.allow_words = mt->allow_words; bc.mask_rows = mt->mask_rows;
+        bc.chunk_base[0] = 0;
+        for (uint32_t k = 0; k < kMaxBitsetFilters; ++k) {
+            bc.row_base[k] = mt->row_base[k];
+            bc.chunk_base[k + 1] = bc.chunk_base[k] + (k < mt->nbits ? (mt->mods[k].q + kWave - 1) / kWave : 0);
+        }
+        e->matchers.push_back(matchers[c]);
+    }
+    Matcher* m0 = matcher_at(matchers[0]);
+    e->args.A = m0->A; e->args.B = m0->B; e->args.lo = m0->lo; e->args.hi = m0->hi;
+    e->args.ncfg = ncfg;
+    e->focus_u = focus_u; e->focus_v = focus_v;
+    e->cap = cap;
+    e->args.cap = cap;
+    e->args.list_cap = e->list_cap;
+    e->blocks_match = std::max<uint32_t>(1, blocks_per_cu) * (uint32_t)std::max(1, g.compute_units);
+    e->use_graph = use_graph != 0;
+    TRY(e->totals.ensure(sizeof(BatchTotals)));
+    TRY(e->list.ensure(e->list_cap * 24));
+    TRY(e->list_count.ensure(8));
+    TRY(e->error.ensure(4));
+    TRY(e->cursor.ensure(4));
+    g_batch.push_back(std::move(e));
+    *engine = (int)g_batch.size() - 1;
+    return PCUBE_OK;
+}
+
+int pcube_batch_destroy(int engine) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    if (engine < 0 || engine >= (int)g_batch.size() || !g_batch[engine]) return fail(PCUBE_INVALID, "unknown batch engine");
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    g_batch[engine].reset();
+    return PCUBE_OK;
+}
+
+int pcube_batch_run(int engine, uint32_t nwin, const uint32_t* win_cfg, const uint32_t* win_tiles,
+                    const uint64_t* tiles, const uint64_t* tile_width, const uint32_t* mode, uint32_t rows_per_chunk,
+                    uint64_t* bounds, uint64_t* sizes, uint64_t* counters, uint64_t* values, uint64_t values_cap,
+                    double* seconds, uint64_t* graph_launches) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    const double t_start = now_seconds();
+    if (engine < 0 || engine >= (int)g_batch.size() || !g_batch[engine]) return fail(PCUBE_INVALID, "unknown batch engine");
+    BatchEngine& e = *g_batch[engine];
+    if (nwin == 0 || rows_per_chunk == 0) return fail(PCUBE_INVALID, "empty batch");
+    Focus* fu = focus_at(e.focus_u);
+    Focus* fv = focus_at(e.focus_v);
+    // window geometry from the subtiles
+    std::vector<uint64_t> tfirst(nwin + 1, 0);
+    for (uint32_t w = 0; w < nwin; ++w) {
+        if (win_cfg[w] >= e.args.ncfg || win_tiles[w] == 0 || tile_width[w] == 0 || mode[w] > 1)
+            return fail(PCUBE_INVALID, "window %u: bad configuration, subtiles, width or engine", w);
+        tfirst[w + 1] = tfirst[w] + win_tiles[w];
+    }
+    const uint64_t ntot = tfirst[nwin];
+    std::vector<uint64_t> ulo(nwin), uhi(nwin), vlo(nwin), vhi(nwin);
+    for (uint32_t w = 0; w < nwin; ++w) {
+        const uint64_t a = tfirst[w], z = tfirst[w + 1] - 1;
+        ulo[w] = tiles[4 * a + 2];
+        uhi[w] = tiles[4 * z + 3];
+        vlo[w] = tiles[4 * a + 0];
+        vhi[w] = tiles[4 * z + 1];
+        if (ulo[w] > uhi[w] || vlo[w] >= vhi[w]) return fail(PCUBE_INVALID, "window %u: empty or inverted range", w);
+    }
+    hipEvent_t ev[4];
+    for (auto& x : ev) HIP_TRY(hipEventCreate(&x));
+    HIP_TRY(hipEventRecord(ev[0], g.stream));
+    // all windows of both wheels: count, size, allocate, write
+    std::vector<uint64_t> offu(nwin + 1), offv(nwin + 1);
+    uint64_t nseg_u = 0, nseg_v = 0;
+    TRY(wheel_count(fu->wheel, nwin, ulo.data(), uhi.data(), &nseg_u));
+    HIP_TRY(hipMemcpyAsync(offu.data(), fu->wheel.win_offsets.p, (nwin + 1) * 8, hipMemcpyDeviceToHost, g.stream));
+    TRY(wheel_count(fv->wheel, nwin, vlo.data(), vhi.data(), &nseg_v));
+    HIP_TRY(hipMemcpyAsync(offv.data(), fv->wheel.win_offsets.p, (nwin + 1) * 8, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    TRY(e.lus.ensure(std::max<uint64_t>(8, offu[nwin] * 8)));
+    TRY(e.lvs.ensure(std::max<uint64_t>(8, offv[nwin] * 8)));
+    // each wheel keeps its own plan from the count pass
+    TRY(wheel_write(fu->wheel, nwin, nseg_u, e.lus.as<uint64_t>(), offu[nwin]));
+    TRY(wheel_write(fv->wheel, nwin, nseg_v, e.lvs.as<uint64_t>(), offv[nwin]));
+    // subtile bounds inside each window
+    const uint64_t nq = 4 * ntot;
+    std::vector<uint64_t> qb(nq), ql(nq), qk(nq), qo(nq);
+    for (uint32_t w = 0; w < nwin; ++w)
+        for (uint64_t t = tfirst[w]; t < tfirst[w + 1]; ++t) {
+            for (int k = 0; k < 2; ++k) {
+                qb[4 * t + k] = offu[w]; ql[4 * t + k] = offu[w + 1] - offu[w]; qk[4 * t + k] = tiles[4 * t + 2 + k];
+                qb[4 * t + 2 + k] = offv[w]; ql[4 * t + 2 + k] = offv[w + 1] - offv[w]; qk[4 * t + 2 + k] = tiles[4 * t + k];
+            }
+        }
+    TRY(e.q_begin.ensure(nq * 8)); TRY(e.q_len.ensure(nq * 8)); TRY(e.q_key.ensure(nq * 8)); TRY(e.q_out.ensure(nq * 8));
+    // u queries search the u array, v queries the v array: two launches over interleaved halves
+    std::vector<uint64_t> ub(2 * ntot), ul(2 * ntot), uk(2 * ntot), vb(2 * ntot), vl(2 * ntot), vk(2 * ntot);
+    for (uint64_t t = 0; t < ntot; ++t)
+        for (int k = 0; k < 2; ++k) {
+            ub[2 * t + k] = qb[4 * t + k]; ul[2 * t + k] = ql[4 * t + k]; uk[2 * t + k] = qk[4 * t + k];
+            vb[2 * t + k] = qb[4 * t + 2 + k]; vl[2 * t + k] = ql[4 * t + 2 + k]; vk[2 * t + k] = qk[4 * t + 2 + k];
+        }
+    HIP_TRY(hipMemcpyAsync(e.q_begin.p, ub.data(), 2 * ntot * 8, hipMemcpyHostToDevice, g.stream));
+    HIP_TRY(hipMemcpyAsync(e.q_len.p, ul.data(), 2 * ntot * 8, hipMemcpyHostToDevice, g.stream));
+    HIP_TRY(hipMemcpyAsync(e.q_key.p, uk.data(), 2 * ntot * 8, hipMemcpyHostToDevice, g.stream));
+    HIP_TRY(hipMemcpyAsync(e.q_begin.as<uint64_t>() + 2 * ntot, vb.data(), 2 * ntot * 8, hipMemcpyHostToDevice, g.stream));
+    HIP_TRY(hipMemcpyAsync(e.q_len.as<uint64_t>() + 2 * ntot, vl.data(), 2 * ntot * 8, hipMemcpyHostToDevice, g.stream));
+    HIP_TRY(hipMemcpyAsync(e.q_key.as<uint64_t>() + 2 * ntot, vk.data(), 2 * ntot * 8, hipMemcpyHostToDevice, g.stream));
+    hipLaunchKernelGGL(k_searchsorted_seg, dim3(blocks_for(2 * ntot, 256)), dim3(256), 0, g.stream,
+                       (const uint64_t*)e.lus.as<uint64_t>(), (const uint64_t*)e.q_begin.as<uint64_t>(),
+                       (const uint64_t*)e.q_len.as<uint64_t>(), (const uint64_t*)e.q_key.as<uint64_t>(), 2 * ntot,
+                       e.q_out.as<uint64_t>());
+    hipLaunchKernelGGL(k_searchsorted_seg, dim3(blocks_for(2 * ntot, 256)), dim3(256), 0, g.stream,
+                       (const uint64_t*)e.lvs.as<uint64_t>(), (const uint64_t*)(e.q_begin.as<uint64_t>() + 2 * ntot),
+                       (const uint64_t*)(e.q_len.as<uint64_t>() + 2 * ntot),
+                       (const uint64_t*)(e.q_key.as<uint64_t>() + 2 * ntot), 2 * ntot, e.q_out.as<uint64_t>() + 2 * ntot);
+    std::vector<uint64_t> pu(2 * ntot), pv(2 * ntot);
+    HIP_TRY(hipMemcpyAsync(pu.data(), e.q_out.p, 2 * ntot * 8, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipMemcpyAsync(pv.data(), e.q_out.as<uint64_t>() + 2 * ntot, 2 * ntot * 8, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    HIP_TRY(hipEventRecord(ev[1], g.stream));
+    // descriptors
+    std::vector<BatchWindow> desc(nwin);
+    uint64_t rows = 0, mask_words = 0, mask_waves = 0, rho_items = 0, chunks = 0, ncounters = 0;
+    int geometry_rc = PCUBE_OK;
+    for (uint32_t w = 0; w < nwin; ++w) {
+        const BatchConfig& c = e.args.cfg[win_cfg[w]];
+        BatchWindow d;
+        memset(&d, 0, sizeof d);
+        const uint64_t nu = offu[w + 1] - offu[w], nv = offv[w + 1] - offv[w];
+        if (nu >= (1ull << 32) - 64 || nv >= (1ull << 32)) return fail(PCUBE_INVALID, "window %u too large", w);
+        sizes[2 * w] = nu;
+        sizes[2 * w + 1] = nv;
+        for (uint64_t t = tfirst[w]; t < tfirst[w + 1]; ++t) {
+            for (int k = 0; k < 2; ++k) {
+                bounds[4 * t + k] = pu[2 * t + k];
+                bounds[4 * t + 2 + k] = pv[2 * t + k];
+            }
+            const uint64_t du = bounds[4 * t + 1] - bounds[4 * t], dv = bounds[4 * t + 3] - bounds[4 * t + 2];
+            if (du * dv > 50000000ull && geometry_rc == PCUBE_OK)
+                geometry_rc = fail(PCUBE_OVERFLOW, "per-subtile candidate opportunity limit exceeded");
+        }
+        d.nu = (uint32_t)nu; d.nv = (uint32_t)nv;
+        d.words = (uint32_t)((nu + 63) / 64);
+        d.ntiles = win_tiles[w];
+        d.cfg = win_cfg[w];
+        d.mode = mode[w];
+        d.v_first = vlo[w];
+        d.tile_width = tile_width[w];
+        d.tile_base = tfirst[w];
+        d.row_begin = rows;
+        d.mask_begin = mask_waves;
+        d.rho_begin = rho_items;
+        d.chunk_begin = chunks;
+        // pointers are filled after the buffers are sized; keep offsets for now
+        d.us = (const uint64_t*)(uintptr_t)(offu[w] * 8);
+        d.vs = (const uint64_t*)(uintptr_t)(offv[w] * 8);
+        d.counters = (uint64_t*)(uintptr_t)(ncounters * 8);
+        if (d.mode == 0 && nu) {
+            d.masks = (uint64_t*)(uintptr_t)(mask_words * 8);
+            d.rho = (uint16_t*)(uintptr_t)(rho_items * 2);
+            mask_words += (uint64_t)c.mask_rows * d.words;
+            mask_waves += (uint64_t)c.chunk_base[c.nbits] * d.words;
+            rho_items += nu * kCachedResidues;
+        }
+        rows += nv;
+        chunks += (nv + rows_per_chunk - 1) / rows_per_chunk;
+        ncounters += (uint64_t)win_tiles[w] * (c.ng + 3);
+        desc[w] = d;
+    }
+    if (geometry_rc) return geometry_rc;
+    if (chunks >= (1ull << 32)) return fail(PCUBE_INVALID, "too many row chunks");
+    TRY(e.win.ensure(nwin * sizeof(BatchWindow)));
+    TRY(e.rows.ensure(std::max<uint64_t>(1, rows) * sizeof(Row)));
+    TRY(e.counters.ensure(std::max<uint64_t>(1, ncounters) * 8));
+    TRY(e.masks.ensure(std::max<uint64_t>(8, mask_words * 8)));
+    TRY(e.rho.ensure(std::max<uint64_t>(16, rho_items * 2)));
+    for (auto& d : desc) {
+        d.us = (const uint64_t*)((char*)e.lus.p + (uintptr_t)d.us);
+        d.vs = (const uint64_t*)((char*)e.lvs.p + (uintptr_t)d.vs);
+        d.counters = (uint64_t*)((char*)e.counters.p + (uintptr_t)d.counters);
+        d.masks = (uint64_t*)((char*)e.masks.p + (uintptr_t)d.masks);
+        d.rho = (uint16_t*)((char*)e.rho.p + (uintptr_t)d.rho);
+    }
+    BatchTotals tot;
+    memset(&tot, 0, sizeof tot);
+    tot.rows = rows; tot.mask_waves = mask_waves; tot.rho_items = rho_items; tot.chunks = chunks;
+    tot.counters = ncounters; tot.nwin = nwin; tot.rows_per_chunk = rows_per_chunk;
+    HIP_TRY(hipMemcpyAsync(e.win.p, desc.data(), nwin * sizeof(BatchWindow), hipMemcpyHostToDevice, g.stream));
+    HIP_TRY(hipMemcpyAsync(e.totals.p, &tot, sizeof tot, hipMemcpyHostToDevice, g.stream));
+    e.args.win = e.win.as<BatchWindow>();
+    e.args.totals = e.totals.as<BatchTotals>();
+    e.args.rows = e.rows.as<Row>();
+    e.args.list = e.list.as<uint64_t>();
+    e.args.list_count = e.list_count.as<uint64_t>();
+    e.args.counters = e.counters.as<uint64_t>();
+    e.args.error = e.error.as<uint32_t>();
+    e.args.cursor = e.cursor.as<uint32_t>();
+    HIP_TRY(hipEventRecord(ev[2], g.stream));
+    TRY(batch_launch(e));
+    HIP_TRY(hipEventRecord(ev[3], g.stream));
+    std::vector<uint64_t> host(std::max<uint64_t>(1, ncounters));
+    uint32_t err = 0;
+    uint64_t listed = 0;
+    HIP_TRY(hipMemcpyAsync(host.data(), e.counters.p, ncounters * 8, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipMemcpyAsync(&err, e.error.p, 4, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipMemcpyAsync(&listed, e.list_count.p, 8, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    memcpy(counters, host.data(), ncounters * 8);
+    if (seconds) {
+        float a = 0, b = 0;
+        HIP_TRY(hipEventElapsedTime(&a, ev[0], ev[1]));
+        HIP_TRY(hipEventElapsedTime(&b, ev[2], ev[3]));
+        seconds[0] = a / 1000.0;
+        seconds[1] = 0.0;
+        seconds[2] = b / 1000.0;
+    }
+    for (auto& x : ev) (void)hipEventDestroy(x);
+    if (graph_launches) *graph_launches = e.graph_launches;
+    // status and values, subtile by subtile (each window's counter stride is its configuration's)
+    if (err & 1u) return fail(PCUBE_INVALID, "a v coordinate lies outside its window's subtiles");
+    std::vector<uint64_t> tile_ng(ntot), tile_counter(ntot);
+    {
+        uint64_t off = 0;
+        for (uint32_t w = 0; w < nwin; ++w) {
+            const uint32_t ng = e.args.cfg[win_cfg[w]].ng;
+            for (uint64_t t = tfirst[w]; t < tfirst[w + 1]; ++t) {
+                tile_ng[t] = ng;
+                tile_counter[t] = off;
+                off += ng + 3;
+            }
+        }
+    }
+    for (uint64_t t = 0; t < ntot; ++t)
+        if (host[tile_counter[t] + tile_ng[t] + 2]) return fail(PCUBE_UNDERFLOW, "unsigned underflow flagged");
+    if (err & 2u) return fail(PCUBE_RANGE, "an emitted value lies outside (lo, hi]");
+    std::vector<uint64_t> voff(ntot + 1, 0);
+    int status = PCUBE_OK;
+    for (uint64_t t = 0; t < ntot; ++t) {
+        const uint64_t emitted = host[tile_counter[t] + tile_ng[t] + 1];
+        if (emitted > e.cap && status == PCUBE_OK) status = fail(PCUBE_OVERFLOW, "output overflow in subtile %llu", (unsigned long long)t);
+        voff[t + 1] = voff[t] + std::min(emitted, e.cap);
+    }
+    if (status) return status;
+    if (listed != voff[ntot] || listed > e.list_cap) return fail(PCUBE_OVERFLOW, "survivor list overflow");
+    if (listed > values_cap) return fail(PCUBE_OVERFLOW, "host value buffer too small");
+    if (listed) {
+        std::vector<uint64_t> triples(listed * 3);
+        HIP_TRY(hipMemcpyAsync(triples.data(), e.list.p, listed * 24, hipMemcpyDeviceToHost, g.stream));
+        HIP_TRY(hipStreamSynchronize(g.stream));
+        std::vector<uint64_t> cur(voff.begin(), voff.end() - 1);
+        for (uint64_t i = 0; i < listed; ++i) {
+            const uint64_t t = triples[3 * i];
+            if (t >= ntot || cur[t] >= voff[t + 1]) return fail(PCUBE_INVALID, "corrupt survivor list");
+            values[2 * cur[t]] = triples[3 * i + 1];
+            values[2 * cur[t] + 1] = triples[3 * i + 2];
+            ++cur[t];
+        }
+    }
+    if (seconds) seconds[3] = now_seconds() - t_start;
+    return PCUBE_OK;
+}
+
+int pcube_host_verify(const uint64_t* limbs, uint64_t n, uint32_t p, uint8_t* local_ok, uint8_t* is_cube,
+                      uint64_t* cube_root) {
+    // host code only: the exact predicate of pseudocube/oracle.py:36-41 and the
+    // cube root of review/verify_cube.py:13-17, in unsigned __int128
+    typedef unsigned __int128 u128;
+    if (p < 7 || p > 100000) return fail(PCUBE_INVALID, "prime bound out of range");
+    std::vector<uint32_t> primes;
+    for (uint32_t q = 2; q <= p; ++q) {
+        bool prime = true;
+        for (uint32_t d = 2; d * d <= q; ++d)
+            if (q % d == 0) { prime = false; break; }
+        if (prime) primes.push_back(q);
+    }
+    if (primes.empty() || primes.back() != p || p % 3 != 1) return fail(PCUBE_INVALID, "p must be a prime = 1 mod 3");
+    for (uint64_t i = 0; i < n; ++i) {
+        const u128 x = ((u128)limbs[2 * i + 1] << 64) | limbs[2 * i];
+        bool ok = x > 0 && ((uint32_t)(x % 9) == 1 || (uint32_t)(x % 9) == 8);
+        for (size_t k = 0; ok && k < primes.size(); ++k) {
+            const uint64_t q = primes[k], r = (uint64_t)(x % q);
+            if (r == 0) { ok = false; break; }
+            if (q % 3 == 1) {
+                uint64_t e = (q - 1) / 3, base = r, acc = 1;
+                while (e) {
+                    if (e & 1) acc = acc * base % q;
+                    base = base * base % q;
+                    e >>= 1;
+                }
+                if (acc != 1) ok = false;
+            }
+        }
+        // floor cube root by bisection on [0, 2^43) (x < 2^128 needs < 2^43)
+        uint64_t lo = 0, hi = 1ull << 43;
+        while (hi - lo > 1) {
+            const uint64_t mid = (lo + hi) >> 1;
+            const u128 cube = (u128)mid * mid * mid;
+            if (cube <= x) lo = mid; else hi = mid;
+        }
+        local_ok[i] = ok ? 1 : 0;
+        is_cube[i] = ((u128)lo * lo * lo == x) ? 1 : 0;
+        cube_root[i] = lo;
+    }
+    return PCUBE_OK;
 }
 
 int pcube_copy_probe(uint64_t bytes, double* seconds) {

@@ -291,7 +291,7 @@ struct MatchArgs {
     const uint64_t* vs;
     const Row* rows;
     const uint64_t* masks;     // [sum of bitset q rows][words]
-    const uint16_t* rho;       // [nu][ncache] (u*B) mod q for the cached chain filters
+    const uint16_t* rho;       // [nu][kCachedResidues] (u*B) mod q for the cached chain filters
     const Modulus* mods;       // [nres] in filter order
     const uint32_t* allow;     // allowed-residue bits, rows at Modulus::bit
     uint64_t* counters;        // [ntiles][ng + 3]
@@ -304,9 +304,8 @@ struct MatchArgs {
     uint32_t row_base[kMaxBitsetFilters];
 };
 
-__global__ void k_rows(MatchArgs m) {
-    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= m.nv) return;
+// Exact u bounds of row j by 128-bit binary search, subtile, v mod q.
+__device__ __forceinline__ void build_row(const MatchArgs& m, uint32_t j) {
     const uint64_t v = m.vs[j];
     Row row;
     row.v = v;
@@ -343,20 +342,21 @@ __global__ void k_rows(MatchArgs m) {
     ((Row*)m.rows)[j] = row;
 }
 
-// Residue bitsets for the first nbits filters.  One wave builds 64 rows
-// (residues r0..r0+63 of v mod q) of one 64-bit word: lane t tests
-// u index 64*w + t, and a wave ballot packs the 64 tests into the word.
-// Row r, bit t: allowed_q[(u*B - r*A) mod q].
-__global__ __launch_bounds__(256) void k_masks(MatchArgs m, uint32_t chunks_total, uint32_t chunk_base1,
-                                               uint32_t chunk_base2, uint32_t chunk_base3) {
-    const uint32_t lane = threadIdx.x & (kWave - 1);
-    const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / kWave;
-    if (wave >= (uint64_t)chunks_total * m.words) return;
-    const uint32_t c = uint32_t(wave / m.words), w = uint32_t(wave % m.words);
-    const uint32_t k = c >= chunk_base3 ? 3 : c >= chunk_base2 ? 2 : c >= chunk_base1 ? 1 : 0;
-    const uint32_t kbase = k == 3 ? chunk_base3 : k == 2 ? chunk_base2 : k == 1 ? chunk_base1 : 0;
+__global__ void k_rows(MatchArgs m) {
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < m.nv) build_row(m, j);
+}
+
+// One 64-bit word of the residue bitsets: wave builds 64 rows (residues
+// r0..r0+63 of v mod q) of word w; lane t tests u index 64*w + t and a wave
+// ballot packs the 64 tests.  Row r, bit t: allowed_q[(u*B - r*A) mod q].
+// chunk c of the k-th bitset prime starts at chunk_base[k].
+__device__ __forceinline__ void build_mask_word(const MatchArgs& m, const uint32_t* chunk_base, uint32_t c,
+                                                uint32_t w, uint32_t lane) {
+    uint32_t k = 0;
+    while (k + 1 < m.nbits && c >= chunk_base[k + 1]) ++k;
     const Modulus mod = m.mods[k];
-    const uint32_t r0 = (c - kbase) * kWave;
+    const uint32_t r0 = (c - chunk_base[k]) * kWave;
     const uint32_t i = w * kWave + lane;
     const bool valid = i < m.nu;
     // residue for row r0: (u*B - r0*A) mod q, then step by -A mod q per row
@@ -377,12 +377,19 @@ __global__ __launch_bounds__(256) void k_masks(MatchArgs m, uint32_t chunks_tota
     if (lane < rows) ((uint64_t*)m.masks)[(uint64_t)(m.row_base[k] + r0 + lane) * m.words + w] = mine;
 }
 
+__global__ __launch_bounds__(256) void k_masks(MatchArgs m, uint32_t chunks_total, uint32_t chunk_base1,
+                                               uint32_t chunk_base2, uint32_t chunk_base3) {
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / kWave;
+    if (wave >= (uint64_t)chunks_total * m.words) return;
+    const uint32_t base[kMaxBitsetFilters] = {0, chunk_base1, chunk_base2, chunk_base3};
+    build_mask_word(m, base, uint32_t(wave / m.words), uint32_t(wave % m.words), lane);
+}
+
 // rho[i][c] = (u_i * B) mod q for the cached chain filters c = 0..ncache-1.
 // Rows are kCachedResidues entries wide (16 bytes) whatever ncache is, so the
 // chain reads one u coordinate's residues with a single vector load.
-__global__ void k_rho(MatchArgs m) {
-    uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (t >= (uint64_t)m.nu * kCachedResidues) return;
+__device__ __forceinline__ void build_rho(const MatchArgs& m, uint64_t t) {
     const uint32_t i = uint32_t(t / kCachedResidues), c = uint32_t(t % kCachedResidues);
     uint16_t r = 0;
     if (c < m.ncache) {
@@ -392,17 +399,42 @@ __global__ void k_rho(MatchArgs m) {
     ((uint16_t*)m.rho)[t] = r;
 }
 
+__global__ void k_rho(MatchArgs m) {
+    uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (t < (uint64_t)m.nu * kCachedResidues) build_rho(m, t);
+}
+
 constexpr int kMatchThreads = 256;
 constexpr int kMatchWaves = kMatchThreads / kWave;
 constexpr int kQueue = 2 * kWave;
 constexpr int kWordPrefetch = 5;   // 64-word steps of one row whose loads are issued together
+constexpr int kGroupSlots = kMaxResiduals / 4 + 2;
+constexpr int kMaxConfigs = 3;     // filter configurations resident in one batched launch
 
-struct MatchShared {
-    uint32_t allow[kMaxAllowWords];
-    Modulus mods[kMaxResiduals];
+// LDS of a matching block: filter tables of kConfigs configurations, and per
+// wave a survivor queue and group counters.
+template <int kConfigs>
+struct MatchSharedT {
+    uint32_t allow[kConfigs][kMaxAllowWords];
+    Modulus mods[kConfigs][kMaxResiduals];
     uint32_t queue[kMatchWaves][kQueue];
-    uint32_t groups[kMatchWaves][kMaxResiduals / 4 + 2];
+    uint32_t groups[kMatchWaves][kGroupSlots];
 };
+
+// One wave's view of the LDS for one configuration.
+struct Lds {
+    const uint32_t* allow;
+    const Modulus* mods;
+    uint32_t* queue;
+    uint32_t* groups;
+};
+
+template <int kConfigs>
+__device__ void load_tables(MatchSharedT<kConfigs>& sh, uint32_t cfg, const uint32_t* allow, uint32_t allow_words,
+                            const Modulus* mods, uint32_t nres) {
+    for (uint32_t t = threadIdx.x; t < allow_words; t += blockDim.x) sh.allow[cfg][t] = allow[t];
+    for (uint32_t t = threadIdx.x; t < nres; t += blockDim.x) sh.mods[cfg][t] = mods[t];
+}
 
 __device__ __forceinline__ uint64_t wave_sum(uint64_t x) {
     for (int d = kWave / 2; d > 0; d >>= 1) x += __shfl_xor(x, d, kWave);
@@ -410,8 +442,8 @@ __device__ __forceinline__ uint64_t wave_sum(uint64_t x) {
 }
 
 // Flush one wave's counts for a subtile into the global counters.
-__device__ void flush_tile(const MatchArgs& m, MatchShared& sh, uint32_t wave, uint32_t lane, uint32_t tile,
-                           uint32_t& valid, uint64_t& survivors) {
+__device__ void flush_tile(const MatchArgs& m, const Lds& L, uint32_t lane, uint32_t tile, uint32_t& valid,
+                           uint64_t& survivors) {
     const uint64_t total = wave_sum(valid);
     uint64_t* cnt = m.counters + (uint64_t)tile * (m.ng + 3);
     if (lane == 0) {
@@ -419,9 +451,9 @@ __device__ void flush_tile(const MatchArgs& m, MatchShared& sh, uint32_t wave, u
         if (survivors) atomicAdd(&cnt[1], survivors);
     }
     for (uint32_t g = 2 + lane; g <= m.ng; g += kWave) {
-        const uint32_t n = sh.groups[wave][g];
+        const uint32_t n = L.groups[g];
         if (n) atomicAdd(&cnt[g], (uint64_t)n);
-        sh.groups[wave][g] = 0;
+        L.groups[g] = 0;
     }
     valid = 0;
     survivors = 0;
@@ -450,10 +482,10 @@ __device__ __forceinline__ void emit(const MatchArgs& m, uint32_t tile, const U1
 // use the per-u residue table plus the per-row offset sigma; later filters
 // build x = u*B - v*A exactly and reduce it.  Group ends count survivors
 // with a wave ballot.
-__device__ void run_chain(const MatchArgs& m, MatchShared& sh, uint32_t wave, uint32_t lane, uint32_t count,
-                          const Row& row, const U128& shift, uint32_t sigma) {
+__device__ void run_chain(const MatchArgs& m, const Lds& L, uint32_t lane, uint32_t count, const Row& row,
+                          const U128& shift, uint32_t sigma) {
     bool alive = lane < count;
-    const uint32_t i = alive ? sh.queue[wave][lane] : 0;
+    const uint32_t i = alive ? L.queue[lane] : 0;
     uint32_t k = m.nbits;
     // all cached residues of this u in one 16-byte load
     uint4 packed = make_uint4(0, 0, 0, 0);
@@ -462,16 +494,16 @@ __device__ void run_chain(const MatchArgs& m, MatchShared& sh, uint32_t wave, ui
 #pragma unroll
     for (int c = 0; c < kCachedResidues; ++c) {
         if ((uint32_t)c >= m.ncache) break;
-        const Modulus mod = sh.mods[k];
+        const Modulus mod = L.mods[k];
         const uint32_t s = __builtin_amdgcn_readlane(sigma, c);
         if (alive) {
             uint32_t r = ((words[c >> 1] >> (16 * (c & 1))) & 0xffffu) + s;
             r = r >= mod.q ? r - mod.q : r;
-            alive = allowed_bit(sh.allow, mod.bit + r);
+            alive = allowed_bit(L.allow, mod.bit + r);
         }
         const uint64_t live = __ballot(alive);
         if ((k & 3) == 3 || k + 1 == m.nres) {
-            if (lane == 0 && live) sh.groups[wave][k / 4 + 1] += __popcll(live);
+            if (lane == 0 && live) L.groups[k / 4 + 1] += __popcll(live);
         }
         if (!live) return;
         ++k;
@@ -479,11 +511,11 @@ __device__ void run_chain(const MatchArgs& m, MatchShared& sh, uint32_t wave, ui
     U128 x = {0, 0};
     if (alive) x = sub_wide(mul_wide(m.us[i], m.B), shift);
     for (; k < m.nres; ++k) {
-        const Modulus mod = sh.mods[k];
-        if (alive) alive = allowed_bit(sh.allow, mod.bit + mod128(x, mod));
+        const Modulus mod = L.mods[k];
+        if (alive) alive = allowed_bit(L.allow, mod.bit + mod128(x, mod));
         const uint64_t live = __ballot(alive);
         if ((k & 3) == 3 || k + 1 == m.nres) {
-            if (lane == 0 && live) sh.groups[wave][k / 4 + 1] += __popcll(live);
+            if (lane == 0 && live) L.groups[k / 4 + 1] += __popcll(live);
         }
         if (!live) return;
     }
@@ -493,31 +525,14 @@ __device__ void run_chain(const MatchArgs& m, MatchShared& sh, uint32_t wave, ui
     }
 }
 
-// Masked matcher.  One wave per v row at a time; each wave owns a contiguous
-// run of rows, and blocks are renumbered so that each XCD (blocks are dealt
-// round-robin over 8 XCDs) owns one contiguous eighth of the rows and so one
-// compact band of mask columns in its L2.
-__global__ __launch_bounds__(kMatchThreads) void k_match(MatchArgs m) {
-    __shared__ MatchShared sh;
-    for (uint32_t t = threadIdx.x; t < m.allow_words; t += kMatchThreads) sh.allow[t] = m.allow[t];
-    for (uint32_t t = threadIdx.x; t < m.nres; t += kMatchThreads) sh.mods[t] = m.mods[t];
-    for (uint32_t t = threadIdx.x; t < kMatchWaves * (kMaxResiduals / 4 + 2); t += kMatchThreads)
-        (&sh.groups[0][0])[t] = 0;
-    __syncthreads();
-
-    const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    const uint32_t per_xcd = gridDim.x / 8;
-    const uint32_t logical = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
-    const uint64_t first_row = ((uint64_t)logical * kMatchWaves + wave) * m.rows_per_wave;
-    uint64_t last_row = first_row + m.rows_per_wave;
-    if (last_row > m.nv) last_row = m.nv;
-
+// Rows [j0, j1) of one window through the bitset engine, by one wave.
+__device__ void match_rows_masked(const MatchArgs& m, const Lds& L, uint32_t lane, uint64_t j0, uint64_t j1) {
     uint32_t tile = 0xffffffffu, valid = 0;
     uint64_t survivors = 0;
-    for (uint64_t j = first_row; j < last_row; ++j) {
+    for (uint64_t j = j0; j < j1; ++j) {
         const Row row = m.rows[j];
         if (row.tile != tile) {
-            if (tile != 0xffffffffu) flush_tile(m, sh, wave, lane, tile, valid, survivors);
+            if (tile != 0xffffffffu) flush_tile(m, L, lane, tile, valid, survivors);
             tile = row.tile;
         }
         if (row.left >= row.right) continue;
@@ -525,7 +540,7 @@ __global__ __launch_bounds__(kMatchThreads) void k_match(MatchArgs m) {
         // sigma_c = (-v*A) mod q for the cached chain filters, one per lane
         uint32_t sigma = 0;
         if (lane < m.ncache) {
-            const Modulus mod = sh.mods[m.nbits + lane];
+            const Modulus mod = L.mods[m.nbits + lane];
             const uint32_t r = mod32(mod64(row.v, mod) * mod.a, mod);
             sigma = r ? mod.q - r : 0;
         }
@@ -535,85 +550,75 @@ __global__ __launch_bounds__(kMatchThreads) void k_match(MatchArgs m) {
         const uint32_t wl = row.left >> 6, wr = (row.right + 63) >> 6;
         uint32_t queued = 0;
         for (uint32_t cbase = wl; cbase < wr; cbase += kWave * kWordPrefetch) {
-          // all bitset words of up to kWordPrefetch steps are requested before any is used
-          uint64_t chunk[kWordPrefetch];
+            // all bitset words of up to kWordPrefetch steps are requested before any is used
+            uint64_t chunk[kWordPrefetch];
 #pragma unroll
-          for (int step = 0; step < kWordPrefetch; ++step) {
-            const uint32_t w = cbase + step * kWave + lane;
-            uint64_t bits = 0;
-            if (w < wr) {
-                bits = ~0ull;
-                const uint32_t base = w << 6;
-                if (base < row.left) bits &= ~0ull << (row.left - base);
-                if (row.right < base + 64) bits &= (1ull << (row.right - base)) - 1ull;
-                valid += __popcll(bits);
+            for (int step = 0; step < kWordPrefetch; ++step) {
+                const uint32_t w = cbase + step * kWave + lane;
+                uint64_t bits = 0;
+                if (w < wr) {
+                    bits = ~0ull;
+                    const uint32_t base = w << 6;
+                    if (base < row.left) bits &= ~0ull << (row.left - base);
+                    if (row.right < base + 64) bits &= (1ull << (row.right - base)) - 1ull;
+                    valid += __popcll(bits);
 #pragma unroll
-                for (int k = 0; k < kMaxBitsetFilters; ++k)
-                    if ((uint32_t)k < m.nbits) bits &= mrow[k][w];
-            }
-            chunk[step] = bits;
-          }
-#pragma unroll
-          for (int step = 0; step < kWordPrefetch; ++step) {
-            if (cbase + step * kWave >= wr) break;
-            const uint32_t w = cbase + step * kWave + lane;
-            uint64_t bits = chunk[step];
-            // wave-level compaction of the surviving u indices into the queue
-            for (;;) {
-                const bool has = bits != 0;
-                const uint64_t ballot = __ballot(has);
-                if (!ballot) break;
-                if (has) {
-                    sh.queue[wave][queued + lanes_below(ballot)] = (w << 6) + __builtin_ctzll(bits);
-                    bits &= bits - 1;
+                    for (int k = 0; k < kMaxBitsetFilters; ++k)
+                        if ((uint32_t)k < m.nbits) bits &= mrow[k][w];
                 }
-                queued += __popcll(ballot);
-                if (queued >= kWave) {
-                    wave_sync();
-                    survivors += kWave;
-                    run_chain(m, sh, wave, lane, kWave, row, shift, sigma);
-                    wave_sync();
-                    const uint32_t rest = queued - kWave;
-                    uint32_t moved = 0;
-                    if (lane < rest) moved = sh.queue[wave][kWave + lane];
-                    wave_sync();
-                    if (lane < rest) sh.queue[wave][lane] = moved;
-                    queued = rest;
-                    wave_sync();
+                chunk[step] = bits;
+            }
+#pragma unroll
+            for (int step = 0; step < kWordPrefetch; ++step) {
+                if (cbase + step * kWave >= wr) break;
+                const uint32_t w = cbase + step * kWave + lane;
+                uint64_t bits = chunk[step];
+                // wave-level compaction of the surviving u indices into the queue
+                for (;;) {
+                    const bool has = bits != 0;
+                    const uint64_t ballot = __ballot(has);
+                    if (!ballot) break;
+                    if (has) {
+                        L.queue[queued + lanes_below(ballot)] = (w << 6) + __builtin_ctzll(bits);
+                        bits &= bits - 1;
+                    }
+                    queued += __popcll(ballot);
+                    if (queued >= kWave) {
+                        wave_sync();
+                        survivors += kWave;
+                        run_chain(m, L, lane, kWave, row, shift, sigma);
+                        wave_sync();
+                        const uint32_t rest = queued - kWave;
+                        uint32_t moved = 0;
+                        if (lane < rest) moved = L.queue[kWave + lane];
+                        wave_sync();
+                        if (lane < rest) L.queue[lane] = moved;
+                        queued = rest;
+                        wave_sync();
+                    }
                 }
             }
-          }
         }
         if (queued) {
             wave_sync();
             survivors += queued;
-            run_chain(m, sh, wave, lane, queued, row, shift, sigma);
+            run_chain(m, L, lane, queued, row, shift, sigma);
             wave_sync();
         }
     }
-    if (tile != 0xffffffffu) flush_tile(m, sh, wave, lane, tile, valid, survivors);
+    if (tile != 0xffffffffu) flush_tile(m, L, lane, tile, valid, survivors);
 }
 
-// Direct matcher: every position of every row goes through the whole chain
-// with x = u*B - v*A built in 128 bits and reduced limb by limb.  This is the
-// per-subtile baseline engine (no bitsets, no residue tables).
-__global__ __launch_bounds__(kMatchThreads) void k_match_direct(MatchArgs m) {
-    __shared__ MatchShared sh;
-    for (uint32_t t = threadIdx.x; t < m.allow_words; t += kMatchThreads) sh.allow[t] = m.allow[t];
-    for (uint32_t t = threadIdx.x; t < m.nres; t += kMatchThreads) sh.mods[t] = m.mods[t];
-    for (uint32_t t = threadIdx.x; t < kMatchWaves * (kMaxResiduals / 4 + 2); t += kMatchThreads)
-        (&sh.groups[0][0])[t] = 0;
-    __syncthreads();
-    const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    const uint64_t first_row = ((uint64_t)blockIdx.x * kMatchWaves + wave) * m.rows_per_wave;
-    uint64_t last_row = first_row + m.rows_per_wave;
-    if (last_row > m.nv) last_row = m.nv;
+// Rows [j0, j1) through the direct engine: every position of every row goes
+// through the whole chain with x = u*B - v*A built in 128 bits and reduced
+// limb by limb (no bitsets, no residue tables).
+__device__ void match_rows_direct(const MatchArgs& m, const Lds& L, uint32_t lane, uint64_t j0, uint64_t j1) {
     uint32_t tile = 0xffffffffu, valid = 0;
     uint64_t survivors = 0;
-    for (uint64_t j = first_row; j < last_row; ++j) {
+    for (uint64_t j = j0; j < j1; ++j) {
         const Row row = m.rows[j];
         if (row.tile != tile) {
-            if (tile != 0xffffffffu) flush_tile(m, sh, wave, lane, tile, valid, survivors);
+            if (tile != 0xffffffffu) flush_tile(m, L, lane, tile, valid, survivors);
             tile = row.tile;
         }
         const U128 shift = mul_wide(row.v, m.A);
@@ -632,13 +637,13 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_direct(MatchArgs m) {
                 }
             }
             for (uint32_t k = 0; k < m.nres; ++k) {
-                const Modulus mod = sh.mods[k];
-                if (alive) alive = allowed_bit(sh.allow, mod.bit + mod128(x, mod));
+                const Modulus mod = L.mods[k];
+                if (alive) alive = allowed_bit(L.allow, mod.bit + mod128(x, mod));
                 const uint64_t live = __ballot(alive);
                 if ((k & 3) == 3 || k + 1 == m.nres) {
                     const uint32_t g = k / 4 + 1;
                     if (g == 1) survivors += __popcll(live);
-                    else if (lane == 0 && live) sh.groups[wave][g] += __popcll(live);
+                    else if (lane == 0 && live) L.groups[g] += __popcll(live);
                 }
                 if (!live) break;
             }
@@ -648,7 +653,187 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_direct(MatchArgs m) {
             }
         }
     }
-    if (tile != 0xffffffffu) flush_tile(m, sh, wave, lane, tile, valid, survivors);
+    if (tile != 0xffffffffu) flush_tile(m, L, lane, tile, valid, survivors);
+}
+
+// Masked matcher for one batch.  One wave per v row at a time; each wave owns
+// a contiguous run of rows, and blocks are renumbered so that each XCD
+// (blocks are dealt round-robin over 8 XCDs) owns one contiguous eighth of
+// the rows and so one compact band of mask columns in its L2.
+__global__ __launch_bounds__(kMatchThreads) void k_match(MatchArgs m) {
+    __shared__ MatchSharedT<1> sh;
+    load_tables(sh, 0, m.allow, m.allow_words, m.mods, m.nres);
+    for (uint32_t t = threadIdx.x; t < kMatchWaves * kGroupSlots; t += kMatchThreads) (&sh.groups[0][0])[t] = 0;
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const Lds L{sh.allow[0], sh.mods[0], sh.queue[wave], sh.groups[wave]};
+    const uint32_t per_xcd = gridDim.x / 8;
+    const uint32_t logical = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    const uint64_t first_row = ((uint64_t)logical * kMatchWaves + wave) * m.rows_per_wave;
+    uint64_t last_row = first_row + m.rows_per_wave;
+    if (last_row > m.nv) last_row = m.nv;
+    match_rows_masked(m, L, lane, first_row, last_row);
+}
+
+__global__ __launch_bounds__(kMatchThreads) void k_match_direct(MatchArgs m) {
+    __shared__ MatchSharedT<1> sh;
+    load_tables(sh, 0, m.allow, m.allow_words, m.mods, m.nres);
+    for (uint32_t t = threadIdx.x; t < kMatchWaves * kGroupSlots; t += kMatchThreads) (&sh.groups[0][0])[t] = 0;
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const Lds L{sh.allow[0], sh.mods[0], sh.queue[wave], sh.groups[wave]};
+    const uint64_t first_row = ((uint64_t)blockIdx.x * kMatchWaves + wave) * m.rows_per_wave;
+    uint64_t last_row = first_row + m.rows_per_wave;
+    if (last_row > m.nv) last_row = m.nv;
+    match_rows_direct(m, L, lane, first_row, last_row);
+}
+
+// --------------------------------------------------------------------------
+// Batched form: many windows of uneven length and mixed configuration in one
+// pass.  Every kernel has a fixed grid and reads its work totals from device
+// memory, so the whole launch sequence can be captured once in a hipGraph and
+// replayed with new window descriptors.
+
+struct BatchConfig {       // one filter configuration (a libpcube matcher)
+    const Modulus* mods;
+    const uint32_t* allow;
+    uint32_t nres, nbits, ncache, ng, allow_words, mask_rows;
+    uint32_t row_base[kMaxBitsetFilters];
+    uint32_t chunk_base[kMaxBitsetFilters + 1];   // 64-row mask chunks per bitset prime, prefix
+};
+
+struct BatchWindow {       // one window: its coordinates, tables and counters
+    const uint64_t* us;
+    const uint64_t* vs;
+    uint64_t* masks;
+    uint16_t* rho;
+    uint64_t* counters;
+    uint64_t v_first, tile_width, tile_base;
+    uint64_t row_begin;    // first global row
+    uint64_t mask_begin;   // first global mask wave item
+    uint64_t rho_begin;    // first global residue-cache item
+    uint64_t chunk_begin;  // first global row chunk
+    uint32_t nu, nv, words, ntiles, cfg, mode;   // mode 0 bitset engine, 1 direct engine
+};
+
+struct BatchTotals {       // device resident, rewritten before each launch
+    uint64_t rows, mask_waves, rho_items, chunks, counters;
+    uint32_t nwin, rows_per_chunk;
+};
+
+struct BatchArgs {
+    const BatchWindow* win;
+    const BatchTotals* totals;
+    Row* rows;
+    uint64_t* list;
+    uint64_t* list_count;
+    uint64_t* counters;        // all windows' counters, totals->counters words
+    uint32_t* error;
+    uint32_t* cursor;          // persistent work-queue cursor
+    uint64_t A, B, cap, list_cap;
+    U128 lo, hi;
+    uint32_t ncfg;
+    BatchConfig cfg[kMaxConfigs];
+};
+
+// last window w with field(w) <= key (fields ascending over windows)
+template <class F>
+__device__ __forceinline__ uint32_t find_window(const BatchWindow* win, uint32_t nwin, uint64_t key, F field) {
+    uint32_t l = 0, h = nwin;
+    while (h - l > 1) {
+        const uint32_t mid = (l + h) >> 1;
+        if (field(win[mid]) <= key) l = mid; else h = mid;
+    }
+    return l;
+}
+
+__device__ __forceinline__ MatchArgs window_args(const BatchArgs& b, const BatchWindow& w) {
+    const BatchConfig& c = b.cfg[w.cfg];
+    MatchArgs m;
+    m.us = w.us; m.vs = w.vs;
+    m.rows = b.rows + w.row_begin;
+    m.masks = w.masks; m.rho = w.rho;
+    m.mods = c.mods; m.allow = c.allow;
+    m.counters = w.counters;
+    m.list = b.list; m.list_count = b.list_count; m.error = b.error;
+    m.A = b.A; m.B = b.B; m.v_first = w.v_first; m.tile_width = w.tile_width; m.cap = b.cap;
+    m.list_cap = b.list_cap; m.tile_base = w.tile_base;
+    m.lo = b.lo; m.hi = b.hi;
+    m.nu = w.nu; m.nv = w.nv; m.words = w.words;
+    m.nres = c.nres; m.nbits = c.nbits; m.ncache = c.ncache; m.ng = c.ng;
+    m.ntiles = w.ntiles; m.allow_words = c.allow_words; m.rows_per_wave = 0;
+    for (int k = 0; k < kMaxBitsetFilters; ++k) m.row_base[k] = c.row_base[k];
+    return m;
+}
+
+__global__ void k_batch_fill(BatchArgs b) {
+    const uint64_t n = b.totals->counters;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        b.counters[i] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *b.list_count = 0;
+        *b.error = 0;
+        *b.cursor = 0;
+    }
+}
+
+__global__ void k_batch_rows(BatchArgs b) {
+    const BatchTotals t = *b.totals;
+    for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < t.rows;
+         g += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t w = find_window(b.win, t.nwin, g, [](const BatchWindow& x) { return x.row_begin; });
+        const BatchWindow win = b.win[w];
+        build_row(window_args(b, win), uint32_t(g - win.row_begin));
+    }
+}
+
+__global__ __launch_bounds__(256) void k_batch_masks(BatchArgs b) {
+    const BatchTotals t = *b.totals;
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / kWave);
+    for (uint64_t g = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / kWave; g < t.mask_waves; g += waves) {
+        const uint32_t w = find_window(b.win, t.nwin, g, [](const BatchWindow& x) { return x.mask_begin; });
+        const BatchWindow win = b.win[w];
+        const uint64_t local = g - win.mask_begin;
+        build_mask_word(window_args(b, win), b.cfg[win.cfg].chunk_base, uint32_t(local / win.words),
+                        uint32_t(local % win.words), lane);
+    }
+}
+
+__global__ void k_batch_rho(BatchArgs b) {
+    const BatchTotals t = *b.totals;
+    for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < t.rho_items;
+         g += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t w = find_window(b.win, t.nwin, g, [](const BatchWindow& x) { return x.rho_begin; });
+        const BatchWindow win = b.win[w];
+        build_rho(window_args(b, win), g - win.rho_begin);
+    }
+}
+
+// Persistent matcher: each wave takes chunks of rows_per_chunk rows of one
+// window from the device queue until the queue is exhausted.
+__global__ __launch_bounds__(kMatchThreads) void k_batch_match(BatchArgs b) {
+    __shared__ MatchSharedT<kMaxConfigs> sh;
+    for (uint32_t c = 0; c < b.ncfg; ++c)
+        load_tables(sh, c, b.cfg[c].allow, b.cfg[c].allow_words, b.cfg[c].mods, b.cfg[c].nres);
+    for (uint32_t t = threadIdx.x; t < kMatchWaves * kGroupSlots; t += kMatchThreads) (&sh.groups[0][0])[t] = 0;
+    __syncthreads();
+    const BatchTotals t = *b.totals;
+    const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    for (;;) {
+        uint32_t chunk = 0;
+        if (lane == 0) chunk = atomicAdd(b.cursor, 1u);
+        chunk = __builtin_amdgcn_readfirstlane(chunk);
+        if (chunk >= t.chunks) break;
+        const uint32_t w = find_window(b.win, t.nwin, chunk, [](const BatchWindow& x) { return x.chunk_begin; });
+        const BatchWindow win = b.win[w];
+        const MatchArgs m = window_args(b, win);
+        const Lds L{sh.allow[win.cfg], sh.mods[win.cfg], sh.queue[wave], sh.groups[wave]};
+        const uint64_t j0 = (uint64_t)(chunk - win.chunk_begin) * t.rows_per_chunk;
+        const uint64_t j1 = min<uint64_t>(j0 + t.rows_per_chunk, win.nv);
+        if (win.mode == 0) match_rows_masked(m, L, lane, j0, j1);
+        else match_rows_direct(m, L, lane, j0, j1);
+    }
 }
 
 // Plain streaming copy of n uint64 words: the calibration probe for the HBM

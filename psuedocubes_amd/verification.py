@@ -1,0 +1,46 @@
+"""Native host verification of emitted values (C++ in libpcube, CPU only).
+
+Same semantics as the Python predicate (`pseudocube/oracle.py:36-41`,
+`review/verify_cube.py:8-20`): every value must satisfy local3(n, p), and
+each noncube becomes the witness record `verify(n, p)` would produce.  Used
+where emitted values are numerous (small prime bounds in the batched and
+served forms), where the pure-Python exponentiation would dominate.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import native
+
+CLAIM = 'Candidate validity only; no minimality claim.'
+
+
+def native_check(values: list[int], p: int) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """(local3 flags, perfect-cube flags, floor cube roots) for each value."""
+    n = len(values)
+    limbs = np.zeros(2 * max(1, n), dtype=np.uint64)
+    mask = (1 << 64) - 1
+    for i, v in enumerate(values):
+        limbs[2 * i] = v & mask
+        limbs[2 * i + 1] = v >> 64
+    ok = np.zeros(max(1, n), dtype=np.uint8)
+    cube = np.zeros(max(1, n), dtype=np.uint8)
+    root = np.zeros(max(1, n), dtype=np.uint64)
+    u8p = ctypes.POINTER(ctypes.c_uint8)
+    native.check(native.lib().pcube_host_verify(native.ptr64(limbs), n, p, ok.ctypes.data_as(u8p),
+                                                cube.ctypes.data_as(u8p), native.ptr64(root)), 'pcube_host_verify')
+    return ok[:n], cube[:n], root[:n]
+
+
+def native_witnesses(values: list[int], p: int) -> list[dict[str, object]]:
+    """Drop-in for batch_experiment.python_witnesses using the native predicate."""
+    if not values:
+        return []
+    ok, cube, root = native_check(values, p)
+    if not ok.all():
+        raise ArithmeticError('GPU/CPU disagreement')
+    return [dict(n=str(n), p=p, cube_root_floor=str(int(r)), perfect_cube=False, failing_primes=[],
+                 valid_pseudocube=True, claim=CLAIM)
+            for n, c, r in zip(values, cube, root) if not c]
