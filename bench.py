@@ -20,6 +20,7 @@ import argparse
 import json
 import os
 import random
+import statistics
 import subprocess
 import sys
 import time
@@ -46,6 +47,12 @@ def parse() -> argparse.Namespace:
     ap.add_argument('--cpu-seconds', type=float, default=12.0, help='bounded CPU-baseline sample')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-lease', action='store_true')
+    ap.add_argument('--no-batched', action='store_true')
+    ap.add_argument('--batched-windows', type=int, default=4096)
+    ap.add_argument('--plain-windows', type=int, default=256, help='prefix of the stream timed on the plain path')
+    ap.add_argument('--no-served', action='store_true')
+    ap.add_argument('--served-requests', type=int, default=20000)
+    ap.add_argument('--served-load', type=float, default=0.8, help='offered load as a fraction of capacity')
     ap.add_argument('--copy-probe', type=int, default=0, metavar='BYTES',
                     help='only run the plain copy kernel used to calibrate the HBM byte counters')
     return ap.parse_args()
@@ -86,6 +93,79 @@ def cpu_baseline(config, seconds: float) -> dict:
     return dict(value=positions / elapsed, unit='focused candidate positions/s', cores=threads, kind='port',
                 sample=f'{batches} seeded full p=619 batches (windows + bitset matcher + counters), '
                        f'{elapsed:.1f} s on {threads} threads of {os.cpu_count()} ({model})')
+
+
+def strip_timings(row: dict) -> dict:
+    return {k: v for k, v in row.items() if not k.endswith('_seconds') and k not in
+            ('matching_layout', 'kernel_registers', 'kernel_local_bytes', 'claim')}
+
+
+def batched_leg(runtime, af, bf, windows: int, plain_windows: int) -> dict:
+    """The uneven-window stream: 4096 windows, v-lengths log-uniform 1e6..1e9 from seed
+    619003, p uniform over {313, 433, 619}, starts at the canonical v of x uniform over
+    (10^27, 3*10^27].  Batched form over the whole stream; plain drop-in and batched form
+    on the same prefix for the ratio."""
+    from psuedocubes_amd.batch_experiment import MaskedMatcher, prepare
+    from psuedocubes_amd.batched import BatchedEngine, window_stream
+    from psuedocubes_amd.focus import A619, B619, Config
+    configs = {p: Config(p, 10**27, 3 * 10**27, A619, B619) for p in (313, 433, 619)}
+    engine = BatchedEngine(runtime, configs.values(), af, bf)
+    stream = window_stream(configs[619], windows, 619003, (1e6, 1e9), tuple(configs))
+    engine.run(stream[:64])  # capture the launch graph, size the buffers
+    t0 = time.perf_counter()
+    rows = engine.run(stream)
+    whole = time.perf_counter() - t0
+    positions = sum(r['valid_focused_candidates'] for r in rows)
+    prefix = stream[:plain_windows]
+    matchers = {p: MaskedMatcher(runtime, c) for p, c in configs.items()}
+    for w in prefix[:8]:
+        matchers[w.p].run(prepare(runtime, configs[w.p], af, bf, w.first, w.end, w.width))
+    t0 = time.perf_counter()
+    plain_rows = [matchers[w.p].run(prepare(runtime, configs[w.p], af, bf, w.first, w.end, w.width)) for w in prefix]
+    plain = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    batched_rows = engine.run(prefix)
+    batched_prefix = time.perf_counter() - t0
+    prefix_positions = sum(r['valid_focused_candidates'] for r in plain_rows)
+    identical = [strip_timings(a) for a in plain_rows] == [strip_timings(b) for b in batched_rows]
+    return dict(workload=f'{windows} uneven windows, L log-uniform 1e6..1e9 v, p in {{313,433,619}}, seed 619003',
+                value=positions / whole, unit='focused candidate positions/s', seconds=whole, positions=positions,
+                survivors=sum(r['local_survivors'] for r in rows), graph_launches=engine.graph_launches,
+                tuning=engine.tuning,
+                same_stream=dict(windows=len(prefix), plain_value=prefix_positions / plain,
+                                 batched_value=prefix_positions / batched_prefix,
+                                 ratio=plain / batched_prefix, rows_identical=identical))
+
+
+def served_leg(runtime, af, bf, requests: int, load: float) -> dict:
+    """Poisson stream of range requests (v-lengths log-uniform 1e7..6.4e9, p=619) at
+    `load` times the capacity measured on a saturated sample of the same stream."""
+    from psuedocubes_amd.batched import BatchedEngine
+    from psuedocubes_amd.focus import A619, B619, Config
+    from psuedocubes_amd.served import Server, latency_summary, request_stream
+    config = Config(619, 10**27, 3 * 10**27, A619, B619)
+    engine = BatchedEngine(runtime, [config], af, bf)
+    sample = request_stream(config, max(200, requests // 10), 619031, rate=0.0)
+    Server(engine).serve(sample[:50], realtime=False)   # warm buffers and the launch graph
+    t0 = time.perf_counter()
+    saturated = Server(engine).serve(sample, realtime=False)
+    seconds = time.perf_counter() - t0
+    positions = sum(c.row['valid_focused_candidates'] for c in saturated)
+    capacity = positions / seconds
+    per_request = positions / len(sample)
+    rate = load * capacity / per_request
+    stream = request_stream(config, requests, 619003, rate=rate)
+    server = Server(engine)
+    t0 = time.perf_counter()
+    done = server.serve(stream)
+    makespan = time.perf_counter() - t0
+    served = sum(c.row['valid_focused_candidates'] for c in done)
+    lat = latency_summary(done)
+    return dict(workload=f'{requests} Poisson range requests, L log-uniform 1e7..6.4e9 v, p=619, seed 619003',
+                value=served / makespan, unit='focused candidate positions/s', capacity=capacity,
+                offered_load=load, rate_per_second=rate, makespan_seconds=makespan, requests=len(done),
+                latency_seconds=lat, steps=server.steps, mean_requests_per_step=statistics.fmean(server.step_sizes),
+                max_requests_per_step=max(server.step_sizes), host_overlap_seconds=server.overlap_seconds)
 
 
 def main() -> None:
@@ -158,6 +238,12 @@ def main() -> None:
         result['lease'] = dict(value=lease['valid_focused_candidates'] / t1, unit='focused candidate positions/s',
                                seconds=t1, batches=256, first=str(first),
                                transcript_sha256=lease['subtile_scopes'][0]['transcript_sha256'])
+    if not args.no_batched:
+        result['batched'] = batched_leg(runtime, af, bf, args.batched_windows, args.plain_windows)
+        print('# batched leg done', file=sys.stderr, flush=True)
+    if not args.no_served:
+        result['served'] = served_leg(runtime, af, bf, args.served_requests, args.served_load)
+        print('# served leg done', file=sys.stderr, flush=True)
     if not args.no_cpu:
         result['cpu_baseline'] = cpu_baseline(config, args.cpu_seconds)
     print(json.dumps(result), flush=True)

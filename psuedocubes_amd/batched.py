@@ -118,6 +118,16 @@ class BatchedEngine:
         return rows  # type: ignore[return-value]
 
     def _run_group(self, windows: list[Window], group: list[int], rows: list[Any]) -> None:
+        raw = self.launch([windows[i] for i in group])
+        for index, row in zip(group, self.assemble(raw)):
+            rows[index] = row
+
+    def launch(self, windows: list[Window]) -> dict[str, Any]:
+        """Device pass over one group of windows; returns the raw results for assemble().
+
+        The library call releases the GIL, so another thread can assemble the
+        previous group's rows meanwhile."""
+        group = list(range(len(windows)))
         started = time.perf_counter()
         plans = [self.plan(windows[i]) for i in group]
         cfg = np.array([self.order.index(windows[i].p) for i in group], dtype=np.uint32)
@@ -145,9 +155,16 @@ class BatchedEngine:
         n = len(group)
         timings = dict(kernel_seconds=seconds[2] / n, focus_enumeration_seconds=seconds[0] / n,
                        transfer_seconds=max(0.0, device - seconds[0] - seconds[2]) / n)
-        bounds = bounds.reshape(ntot, 4)
+        return dict(windows=windows, plans=plans, bounds=bounds.reshape(ntot, 4), sizes=sizes, counters=counters,
+                    values=values, timings=timings)
+
+    def assemble(self, raw: dict[str, Any]) -> list[dict[str, Any]]:
+        """Verified result rows of a launched group, in its window order."""
+        windows, plans, bounds, sizes = raw['windows'], raw['plans'], raw['bounds'], raw['sizes']
+        counters, values, timings = raw['counters'], raw['values'], raw['timings']
+        rows: list[dict[str, Any]] = []
         t0 = c0 = v0 = 0
-        for k, index in enumerate(group):
+        for k, index in enumerate(range(len(windows))):
             window, (ranges, mode) = windows[index], plans[k]
             engine = self.engines[window.p]
             nt, stride = len(ranges), engine.ng + 3
@@ -163,9 +180,10 @@ class BatchedEngine:
                             per_tile, self.cap, timings, time.perf_counter(), witnesses=native_witnesses)
             row['matching_layout'] = 'batched-direct' if mode else 'batched-residue-bitset'
             row['prefilter_build_seconds'] = 0.0
-            rows[index] = row
+            rows.append(row)
             t0 += nt
             c0 += nt * stride
+        return rows
 
     def __del__(self) -> None:
         if native._lib is not None and hasattr(self, 'handle'):

@@ -1,0 +1,219 @@
+"""Served form: continuous batching of coverage-range requests.
+
+Requests are canonical v ranges [first, end) of up to 64 subtiles at one prime
+bound; each arrives at its own time.  The server keeps the focus wheels and
+filter tables resident and loops:
+
+  * arrivals join a waiting queue;
+  * a waiting request is admitted when a slot is free and its memory
+    reservation fits the budget.  It reserves the footprint of one piece,
+    since at most one of its pieces is on the device at a time;
+  * an admitted request is cut into pieces of at most `piece_tiles` subtiles;
+  * each device step takes the next piece of every admitted request (round
+    robin, so short requests are not stuck behind long ones) and runs them
+    as one batched pass.  A device thread runs step k while the host thread
+    verifies and assembles step k-1's rows and admits new arrivals, so
+    requests that arrive while others are on the GPU overlap them;
+  * a request completes when all its pieces are back.  Its row merges its
+    pieces' rows and equals the plain drop-in's row for the whole range
+    (`prepare` + `MaskedMatcher.run`, pseudocube/batch_experiment.py:104-167):
+    per-subtile scopes and values do not depend on what shares a launch.
+
+Latency is completion time minus arrival time.
+"""
+from __future__ import annotations
+
+from collections import deque
+from concurrent.futures import Future, ThreadPoolExecutor
+from dataclasses import dataclass, field
+import math
+import random
+import statistics
+import time
+from typing import Any, Callable
+
+from .batched import BatchedEngine, Window
+from .focus import Config
+
+MERGED_COUNTS = ('focus_u', 'focus_v', 'cartesian_opportunities', 'valid_focused_candidates', 'genuine_cubes',
+                 'local_survivors', 'cube_checks')
+MERGED_TIMES = ('kernel_seconds', 'transfer_seconds', 'verification_seconds', 'focus_enumeration_seconds',
+                'prefilter_build_seconds', 'elapsed_seconds')
+
+
+@dataclass
+class Request:
+    id: int
+    first: int
+    end: int
+    arrival: float = 0.0
+    p: int = 619
+
+
+@dataclass
+class Completed:
+    request: Request
+    row: dict[str, Any]
+    admitted: float
+    finished: float
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.request.arrival
+
+
+@dataclass
+class _Active:
+    request: Request
+    pieces: deque
+    rows: list = field(default_factory=list)
+    admitted: float = 0.0
+    reserved: int = 0
+    outstanding: int = 0
+
+
+def merge_rows(rows: list[dict[str, Any]]) -> dict[str, Any]:
+    """One row for consecutive pieces of a range: sums and concatenations in subtile order."""
+    out = dict(rows[0])
+    for key in MERGED_COUNTS + MERGED_TIMES:
+        out[key] = sum(r[key] for r in rows)
+    out['subtile_scopes'] = [s for r in rows for s in r['subtile_scopes']]
+    out['all_values'] = [v for r in rows for v in r['all_values']]
+    out['cpu_confirmed_hits'] = [h for r in rows for h in r['cpu_confirmed_hits']]
+    out['overflow_splits'] = [s for r in rows for s in r['overflow_splits']]
+    out['group_survivors'] = [sum(g) for g in zip(*(r['group_survivors'] for r in rows))]
+    return out
+
+
+class Server:
+    def __init__(self, engine: BatchedEngine, budget: int | None = None, max_slots: int = 256,
+                 piece_tiles: int = 8) -> None:
+        self.engine = engine
+        self.budget = budget if budget is not None else max(256 << 20, engine.runtime.cap // 4)
+        self.max_slots = max_slots
+        self.piece_tiles = piece_tiles
+        self.steps = 0
+        self.step_sizes: list[int] = []
+        self.overlap_seconds = 0.0
+
+    def pieces(self, request: Request, width: int = 100_000_000) -> deque:
+        span = self.piece_tiles * width
+        return deque(Window(s, min(s + span, request.end), request.p, width)
+                     for s in range(request.first, request.end, span))
+
+    def serve(self, requests: list[Request], realtime: bool = True,
+              on_complete: Callable[[Completed], None] | None = None) -> list[Completed]:
+        """Serve requests; with realtime, each becomes visible at its arrival time on the wall
+        clock, otherwise all are visible at once (saturated load)."""
+        clock = time.perf_counter
+        start = clock()
+        pending = deque(sorted(requests, key=lambda r: r.arrival))
+        waiting: deque = deque()
+        active: dict[int, _Active] = {}
+        order: deque = deque()           # round-robin order of active request ids
+        used = 0
+        done: list[Completed] = []
+        inflight: tuple[Future, list[tuple[int, Window]]] | None = None
+        pool = ThreadPoolExecutor(max_workers=1)
+
+        def now() -> float:
+            return clock() - start
+
+        def admit() -> None:
+            nonlocal used
+            t = now()
+            while pending and (pending[0].arrival <= t or not realtime):
+                waiting.append(pending.popleft())
+            while waiting and len(active) < self.max_slots:
+                request = waiting[0]
+                pieces = self.pieces(request)
+                reserve = max(self.engine.footprint(w) for w in pieces)
+                if active and used + reserve > self.budget:
+                    break
+                waiting.popleft()
+                active[request.id] = _Active(request, pieces, admitted=now(), reserved=reserve,
+                                             outstanding=len(pieces))
+                order.append(request.id)
+                used += reserve
+
+        def next_step() -> list[tuple[int, Window]] | None:
+            step = []
+            for _ in range(len(order)):
+                rid = order[0]
+                order.rotate(-1)
+                state = active[rid]
+                if state.pieces:
+                    step.append((rid, state.pieces.popleft()))
+            return step or None
+
+        def finish(raw: dict[str, Any], step: list[tuple[int, Window]]) -> None:
+            nonlocal used
+            rows = self.engine.assemble(raw)
+            for (rid, _), row in zip(step, rows):
+                state = active[rid]
+                state.rows.append(row)
+                state.outstanding -= 1
+                if state.outstanding == 0:
+                    completed = Completed(state.request, merge_rows(state.rows), state.admitted, now())
+                    done.append(completed)
+                    if on_complete:
+                        on_complete(completed)
+                    used -= state.reserved
+                    del active[rid]
+                    order.remove(rid)
+
+        try:
+            while pending or waiting or active or inflight:
+                admit()
+                if inflight is None:
+                    step = next_step()
+                    if step is not None:
+                        inflight = (pool.submit(self.engine.launch, [w for _, w in step]), step)
+                        self.steps += 1
+                        self.step_sizes.append(len(step))
+                    elif pending and realtime:
+                        time.sleep(max(0.0, min(0.002, pending[0].arrival - now())))
+                    continue
+                future, step = inflight
+                raw = future.result()
+                inflight = None
+                # keep the device busy: launch the next step before assembling this one
+                admit()
+                following = next_step()
+                if following is not None:
+                    inflight = (pool.submit(self.engine.launch, [w for _, w in following]), following)
+                    self.steps += 1
+                    self.step_sizes.append(len(following))
+                t0 = now()
+                finish(raw, step)
+                if inflight is not None:
+                    self.overlap_seconds += now() - t0
+        finally:
+            pool.shutdown(wait=True)
+        return done
+
+
+def request_stream(config: Config, count: int, seed: int, rate: float, length_range: tuple[float, float] = (1e7, 6.4e9),
+                   p: int = 619) -> list[Request]:
+    """Poisson arrivals at `rate` per second; v-lengths log-uniform over length_range;
+    start = canonical v of x uniform over (10^27, 3*10^27], clipped to [0, B)."""
+    rng = random.Random(seed)
+    lo, hi = math.log(length_range[0]), math.log(length_range[1])
+    t = 0.0
+    out = []
+    for i in range(count):
+        t += rng.expovariate(rate) if rate > 0 else 0.0
+        length = int(math.exp(rng.uniform(lo, hi)))
+        x = rng.randrange(config.lo + 1, config.hi + 1)
+        first = min(config.canonical(x)[1], config.B - 1)
+        out.append(Request(i, first, min(config.B, first + max(1, length)), t, p))
+    return out
+
+
+def latency_summary(done: list[Completed]) -> dict[str, float]:
+    lat = sorted(c.latency for c in done)
+    if not lat:
+        return dict(p50=0.0, p99=0.0, mean=0.0, max=0.0)
+    def q(f: float) -> float:
+        return lat[min(len(lat) - 1, int(math.ceil(f * len(lat))) - 1)]
+    return dict(p50=q(0.50), p99=q(0.99), mean=statistics.fmean(lat), max=lat[-1])

@@ -42,7 +42,8 @@ def mixed_windows() -> list[Window]:
 
 
 def test_batched_rows_equal_plain_rows(runtime, focuses619, plain) -> None:
-    engine = BatchedEngine(runtime, configs().values(), *focuses619)
+    # a threshold that sends the short windows to the direct engine, so both engines run
+    engine = BatchedEngine(runtime, configs().values(), *focuses619, tuning=dict(direct_below_rows=50))
     windows = mixed_windows()
     rows = engine.run(windows)
     modes = set()
