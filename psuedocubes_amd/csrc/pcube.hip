@@ -1431,10 +1431,10 @@ int pcube_batch_run(int engine, uint32_t nwin, const uint32_t* win_cfg, const ui
                        (const uint64_t*)e.lvs.as<uint64_t>(), (const uint64_t*)(e.q_begin.as<uint64_t>() + 2 * ntot),
                        (const uint64_t*)(e.q_len.as<uint64_t>() + 2 * ntot),
                        (const uint64_t*)(e.q_key.as<uint64_t>() + 2 * ntot), 2 * ntot, e.q_out.as<uint64_t>() + 2 * ntot);
+    // the subtile bounds travel back with the counters; descriptors need only the window sizes
     std::vector<uint64_t> pu(2 * ntot), pv(2 * ntot);
     HIP_TRY(hipMemcpyAsync(pu.data(), e.q_out.p, 2 * ntot * 8, hipMemcpyDeviceToHost, g.stream));
     HIP_TRY(hipMemcpyAsync(pv.data(), e.q_out.as<uint64_t>() + 2 * ntot, 2 * ntot * 8, hipMemcpyDeviceToHost, g.stream));
-    HIP_TRY(hipStreamSynchronize(g.stream));
     HIP_TRY(hipEventRecord(ev[1], g.stream));
     // descriptors
     std::vector<BatchWindow> desc(nwin);
@@ -1448,15 +1448,6 @@ int pcube_batch_run(int engine, uint32_t nwin, const uint32_t* win_cfg, const ui
         if (nu >= (1ull << 32) - 64 || nv >= (1ull << 32)) return fail(PCUBE_INVALID, "window %u too large", w);
         sizes[2 * w] = nu;
         sizes[2 * w + 1] = nv;
-        for (uint64_t t = tfirst[w]; t < tfirst[w + 1]; ++t) {
-            for (int k = 0; k < 2; ++k) {
-                bounds[4 * t + k] = pu[2 * t + k];
-                bounds[4 * t + 2 + k] = pv[2 * t + k];
-            }
-            const uint64_t du = bounds[4 * t + 1] - bounds[4 * t], dv = bounds[4 * t + 3] - bounds[4 * t + 2];
-            if (du * dv > 50000000ull && geometry_rc == PCUBE_OK)
-                geometry_rc = fail(PCUBE_OVERFLOW, "per-subtile candidate opportunity limit exceeded");
-        }
         d.nu = (uint32_t)nu; d.nv = (uint32_t)nv;
         d.words = (uint32_t)((nu + 63) / 64);
         d.ntiles = win_tiles[w];
@@ -1485,7 +1476,6 @@ int pcube_batch_run(int engine, uint32_t nwin, const uint32_t* win_cfg, const ui
         ncounters += (uint64_t)win_tiles[w] * (c.ng + 3);
         desc[w] = d;
     }
-    if (geometry_rc) return geometry_rc;
     if (chunks >= (1ull << 32)) return fail(PCUBE_INVALID, "too many row chunks");
     TRY(e.win.ensure(nwin * sizeof(BatchWindow)));
     TRY(e.rows.ensure(std::max<uint64_t>(1, rows) * sizeof(Row)));
@@ -1524,6 +1514,15 @@ int pcube_batch_run(int engine, uint32_t nwin, const uint32_t* win_cfg, const ui
     HIP_TRY(hipMemcpyAsync(&listed, e.list_count.p, 8, hipMemcpyDeviceToHost, g.stream));
     HIP_TRY(hipStreamSynchronize(g.stream));
     memcpy(counters, host.data(), ncounters * 8);
+    for (uint64_t t = 0; t < ntot; ++t) {
+        for (int k = 0; k < 2; ++k) {
+            bounds[4 * t + k] = pu[2 * t + k];
+            bounds[4 * t + 2 + k] = pv[2 * t + k];
+        }
+        const uint64_t du = bounds[4 * t + 1] - bounds[4 * t], dv = bounds[4 * t + 3] - bounds[4 * t + 2];
+        if (du * dv > 50000000ull && geometry_rc == PCUBE_OK)
+            geometry_rc = fail(PCUBE_OVERFLOW, "per-subtile candidate opportunity limit exceeded");
+    }
     if (seconds) {
         float a = 0, b = 0;
         HIP_TRY(hipEventElapsedTime(&a, ev[0], ev[1]));
@@ -1534,6 +1533,7 @@ int pcube_batch_run(int engine, uint32_t nwin, const uint32_t* win_cfg, const ui
     }
     for (auto& x : ev) (void)hipEventDestroy(x);
     if (graph_launches) *graph_launches = e.graph_launches;
+    if (geometry_rc) return geometry_rc;
     // status and values, subtile by subtile (each window's counter stride is its configuration's)
     if (err & 1u) return fail(PCUBE_INVALID, "a v coordinate lies outside its window's subtiles");
     std::vector<uint64_t> tile_ng(ntot), tile_counter(ntot);
