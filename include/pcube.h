@@ -99,6 +99,15 @@ int pcube_focus_wheel(int focus, const uint32_t *moduli, const uint32_t *sizes, 
 int pcube_focus_windows(int focus, uint32_t n, const uint64_t *lows, const uint64_t *highs, uint64_t out,
                         uint64_t cap, uint64_t *offsets);
 
+/* prepare() of one compute batch in one call (pseudocube/batch_experiment.py:51-57):
+ * the u window [u_min of the first subtile, u_end of the last) into us_out,
+ * the v window [v_first of the first, v_end of the last) into vs_out, and
+ * the subtile bounds (a, b, c, d) = searchsorted of (u_min, u_end) in us and
+ * (v_first, v_end) in vs.  tiles: ntiles x (v_first, v_end, u_min, u_end).
+ * OVERFLOW (nothing written, *nu/*nv set) when a window exceeds its cap. */
+int pcube_prepare(int focus_u, int focus_v, uint32_t ntiles, const uint64_t *tiles, uint64_t us_out, uint64_t us_cap,
+                  uint64_t vs_out, uint64_t vs_cap, uint64_t *nu, uint64_t *nv, uint64_t *bounds);
+
 /* np.searchsorted(sorted, queries, 'left') on the device; host queries/out
  * (pseudocube/batch_experiment.py:41-42). */
 int pcube_searchsorted(uint64_t sorted, uint64_t n, const uint64_t *queries, uint64_t nq, uint64_t *out);

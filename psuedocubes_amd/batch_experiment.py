@@ -10,6 +10,7 @@ row, key for key.
 from __future__ import annotations
 
 from dataclasses import dataclass
+import ctypes
 import time
 from typing import Any
 
@@ -70,11 +71,38 @@ def prepare_arrays(runtime: Runtime, config: Config, first: int, end: int, width
 
 def prepare(runtime: Runtime, config: Config, af: GPUFocus, bf: GPUFocus,
             first: int, end: int, width: int = 100000000) -> Prepared:
+    """Both focus windows and the subtile bounds of one batch (`pseudocube/batch_experiment.py:51-57`).
+
+    With both wheels present this is one library call (`pcube_prepare`);
+    otherwise it is the reference's sequence of two windows and two searches.
+    """
     started = time.perf_counter()
     ranges = tile_ranges(config, first, end, width)
-    us = af.window(ranges[0][2], ranges[-1][3])
-    vs = bf.window(first, end)
-    return prepare_arrays(runtime, config, first, end, width, us, vs, started)
+    if af.wheel is None or bf.wheel is None:
+        us = af.window(ranges[0][2], ranges[-1][3])
+        vs = bf.window(first, end)
+        return prepare_arrays(runtime, config, first, end, width, us, vs, started)
+    cap = 1000000
+    if (ranges[-1][3] - 1) // af.focus.modulus - ranges[0][2] // af.focus.modulus + 1 > 4096 or \
+            (end - 1) // bf.focus.modulus - first // bf.focus.modulus + 1 > 4096:
+        raise OverflowError('focus window exceeds bounded period count')
+    us, vs = DeviceArray.empty(cap), DeviceArray.empty(cap)
+    tiles = np.array([x for t in ranges for x in t], dtype=np.uint64)
+    nu, nv = ctypes.c_uint64(), ctypes.c_uint64()
+    bounds = np.zeros(4 * len(ranges), dtype=np.uint64)
+    runtime.check()
+    code = native.lib().pcube_prepare(af.handle, bf.handle, len(ranges), native.ptr64(tiles), us.address, cap,
+                                      vs.address, cap, ctypes.byref(nu), ctypes.byref(nv), native.ptr64(bounds))
+    if code == native.OVERFLOW:
+        raise OverflowError('focus buffer overflow; tile invalid')
+    native.check(code, 'pcube_prepare')
+    us.size, vs.size = nu.value, nv.value
+    rows = [tuple(int(x) for x in bounds[4 * i:4 * i + 4]) for i in range(len(ranges))]
+    for a, b, c, d in rows:
+        if (b - a) * (d - c) > 50_000_000:
+            raise OverflowError('per-subtile candidate opportunity limit exceeded')
+    runtime.check()
+    return Prepared(first, end, width, us, vs, rows, time.perf_counter() - started)  # type: ignore[arg-type]
 
 
 def python_witnesses(values: list[int], p: int) -> list[dict[str, object]]:

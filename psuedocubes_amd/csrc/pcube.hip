@@ -244,10 +244,11 @@ struct Focus {
     Buffer C, D, dir;
     BucketWork work;        // window buckets and scan scratch
     Buffer slots, flag;     // single-pass window slots and their overflow flag
+    Buffer queries;         // subtile-bound queries and results of pcube_prepare
     Wheel wheel;
     ~Focus() {
         C.release(); D.release(); dir.release();
-        work.hist.release(); work.offsets.release(); slots.release(); flag.release();
+        work.hist.release(); work.offsets.release(); slots.release(); flag.release(); queries.release();
         work.scan.partial.release(); work.scan.partial_off.release(); work.scan.partial_total.release();
         wheel.release();
     }
@@ -643,10 +644,7 @@ int match_enqueue(Matcher& mt, const uint64_t* us, uint64_t nu, const uint64_t* 
 
 // Copy the tagged value list to the host grouped by tile, in tile order.
 int collect_values(Matcher& mt, uint64_t ntiles_total, const std::vector<uint64_t>& host_counters, uint64_t cap,
-                   uint64_t list_cap, uint64_t* values, uint64_t values_cap, int* status) {
-    uint64_t listed = 0;
-    HIP_TRY(hipMemcpyAsync(&listed, mt.list_count.p, 8, hipMemcpyDeviceToHost, g.stream));
-    HIP_TRY(hipStreamSynchronize(g.stream));
+                   uint64_t list_cap, uint64_t* values, uint64_t values_cap, int* status, uint64_t listed) {
     uint64_t expected = 0;
     std::vector<uint64_t> offset(ntiles_total + 1, 0);
     for (uint64_t t = 0; t < ntiles_total; ++t) {
@@ -1053,6 +1051,59 @@ int pcube_focus_windows(int focus, uint32_t n, const uint64_t* lows, const uint6
     return PCUBE_OK;
 }
 
+int pcube_prepare(int focus_u, int focus_v, uint32_t ntiles, const uint64_t* tiles, uint64_t us_out, uint64_t us_cap,
+                  uint64_t vs_out, uint64_t vs_cap, uint64_t* nu, uint64_t* nv, uint64_t* bounds) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Focus* fu = focus_at(focus_u);
+    Focus* fv = focus_at(focus_v);
+    if (!fu || !fv || fu == fv || !fu->wheel.ready || !fv->wheel.ready || ntiles == 0)
+        return fail(PCUBE_INVALID, "prepare needs the two focus wheels and at least one subtile");
+    const uint64_t ulo = tiles[2], uhi = tiles[4 * (ntiles - 1) + 3];
+    const uint64_t vlo = tiles[0], vhi = tiles[4 * (ntiles - 1) + 1];
+    if (ulo > uhi || vlo > vhi) return fail(PCUBE_INVALID, "invalid batch geometry");
+    // subtile queries go up first: (u_min, u_end) into us, (v_first, v_end) into vs
+    std::vector<uint64_t> q(4 * (uint64_t)ntiles);
+    for (uint32_t t = 0; t < ntiles; ++t) {
+        q[2 * t] = tiles[4 * t + 2];
+        q[2 * t + 1] = tiles[4 * t + 3];
+        q[2 * ntiles + 2 * t] = tiles[4 * t];
+        q[2 * ntiles + 2 * t + 1] = tiles[4 * t + 1];
+    }
+    Buffer& dq = fu->queries;
+    TRY(dq.ensure(8 * 8 * (uint64_t)ntiles));
+    uint64_t* dqueries = dq.as<uint64_t>();
+    uint64_t* dpos = dqueries + 4 * (uint64_t)ntiles;
+    HIP_TRY(hipMemcpyAsync(dqueries, q.data(), 32 * (uint64_t)ntiles, hipMemcpyHostToDevice, g.stream));
+    uint64_t offu[2] = {0, 0}, offv[2] = {0, 0}, nseg_u = 0, nseg_v = 0;
+    TRY(wheel_count(fu->wheel, 1, &ulo, &uhi, &nseg_u));
+    TRY(wheel_count(fv->wheel, 1, &vlo, &vhi, &nseg_v));
+    HIP_TRY(hipMemcpyAsync(offu, fu->wheel.win_offsets.p, 16, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipMemcpyAsync(offv, fv->wheel.win_offsets.p, 16, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    *nu = offu[1];
+    *nv = offv[1];
+    if (offu[1] > us_cap || offv[1] > vs_cap)
+        return fail(PCUBE_OVERFLOW, "focus buffer overflow; tile invalid (%llu, %llu coordinates)",
+                    (unsigned long long)offu[1], (unsigned long long)offv[1]);
+    TRY(wheel_write(fu->wheel, 1, nseg_u, (uint64_t*)us_out, us_cap));
+    TRY(wheel_write(fv->wheel, 1, nseg_v, (uint64_t*)vs_out, vs_cap));
+    hipLaunchKernelGGL(k_searchsorted, dim3(blocks_for(2 * ntiles, 256)), dim3(256), 0, g.stream, (const uint64_t*)us_out,
+                       offu[1], (const uint64_t*)dqueries, 2 * (uint64_t)ntiles, dpos);
+    hipLaunchKernelGGL(k_searchsorted, dim3(blocks_for(2 * ntiles, 256)), dim3(256), 0, g.stream, (const uint64_t*)vs_out,
+                       offv[1], (const uint64_t*)(dqueries + 2 * (uint64_t)ntiles), 2 * (uint64_t)ntiles,
+                       dpos + 2 * (uint64_t)ntiles);
+    std::vector<uint64_t> pos(4 * (uint64_t)ntiles);
+    HIP_TRY(hipMemcpyAsync(pos.data(), dpos, 32 * (uint64_t)ntiles, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    for (uint32_t t = 0; t < ntiles; ++t) {
+        bounds[4 * t] = pos[2 * t];
+        bounds[4 * t + 1] = pos[2 * t + 1];
+        bounds[4 * t + 2] = pos[2 * ntiles + 2 * t];
+        bounds[4 * t + 3] = pos[2 * ntiles + 2 * t + 1];
+    }
+    return PCUBE_OK;
+}
+
 int pcube_searchsorted(uint64_t sorted, uint64_t n, const uint64_t* queries, uint64_t nq, uint64_t* out) {
     std::lock_guard<std::recursive_mutex> lock(g_mu);
     if (!nq) return PCUBE_OK;
@@ -1163,8 +1214,10 @@ int pcube_match(int matcher, uint64_t us, uint64_t nu, uint64_t vs, uint64_t nv,
     }
     std::vector<uint64_t> host(nc);
     uint32_t err = 0;
+    uint64_t listed = 0;
     HIP_TRY(hipMemcpyAsync(host.data(), mt->counters.p, nc * 8, hipMemcpyDeviceToHost, g.stream));
     HIP_TRY(hipMemcpyAsync(&err, mt->error.p, 4, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipMemcpyAsync(&listed, mt->list_count.p, 8, hipMemcpyDeviceToHost, g.stream));
     HIP_TRY(hipStreamSynchronize(g.stream));
     if (seconds) {
         float a = 0, b = 0;
@@ -1180,7 +1233,7 @@ int pcube_match(int matcher, uint64_t us, uint64_t nu, uint64_t vs, uint64_t nv,
     rc = batch_status(*mt, host, ntiles, cap, err);
     if (rc) return rc;
     int status = PCUBE_OK;
-    TRY(collect_values(*mt, ntiles, host, cap, list_cap, values, values_cap, &status));
+    TRY(collect_values(*mt, ntiles, host, cap, list_cap, values, values_cap, &status, listed));
     return status;
 }
 
@@ -1275,8 +1328,10 @@ int pcube_lease(int matcher, int focus_u, int focus_v, uint32_t nbatches, const 
     HIP_TRY(hipEventRecord(ev[2], g.stream));
     std::vector<uint64_t> host(ntot * stride);
     uint32_t err = 0;
+    uint64_t listed = 0;
     HIP_TRY(hipMemcpyAsync(host.data(), mt->counters.p, ntot * stride * 8, hipMemcpyDeviceToHost, g.stream));
     HIP_TRY(hipMemcpyAsync(&err, mt->error.p, 4, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipMemcpyAsync(&listed, mt->list_count.p, 8, hipMemcpyDeviceToHost, g.stream));
     HIP_TRY(hipEventRecord(ev[3], g.stream));
     HIP_TRY(hipStreamSynchronize(g.stream));
     memcpy(counters, host.data(), ntot * stride * 8);
@@ -1294,7 +1349,7 @@ int pcube_lease(int matcher, int focus_u, int focus_v, uint32_t nbatches, const 
     int rc = batch_status(*mt, host, ntot, cap, err);
     if (rc) return rc;
     int status = PCUBE_OK;
-    TRY(collect_values(*mt, ntot, host, cap, list_cap, values, values_cap, &status));
+    TRY(collect_values(*mt, ntot, host, cap, list_cap, values, values_cap, &status, listed));
     if (seconds) seconds[3] = now_seconds() - t_start;
     return status;
 }
