@@ -35,6 +35,7 @@ class Prepared:
     vs: Any
     bounds: list[tuple[int, int, int, int]]
     prep_seconds: float
+    ranges: list[tuple[int, int, int, int]] | None = None   # tile_ranges(first, end, width), when known
 
 
 def tile_ranges(config: Config, first: int, end: int, width: int) -> list[tuple[int, int, int, int]]:
@@ -42,10 +43,29 @@ def tile_ranges(config: Config, first: int, end: int, width: int) -> list[tuple[
     Tile(first, end).validate(config)
     if width <= 0 or (end - first + width - 1) // width > 64:
         raise ValueError('experiment allows at most 64 bounded subtiles')
+    # u_interval(v)[0] = (lo + v*A) // B + 1 and u_interval(v)[1] = (hi + v*A) // B + 1,
+    # stepped by exact quotient/remainder updates instead of one wide division per bound
+    A, B = config.A, config.B
+    step_q, step_r = divmod(width * A, B)
+    q_lo, r_lo = divmod(config.lo + first * A, B)
+    q_hi, r_hi = divmod(config.hi + (first + width - 1) * A, B)
     out = []
     for v in range(first, end, width):
-        stop = min(v + width, end)
-        out.append((v, stop, config.u_interval(v)[0], config.u_interval(stop - 1)[1]))
+        stop = v + width
+        if stop >= end:
+            stop = end
+            q_hi = (config.hi + (end - 1) * A) // B
+        out.append((v, stop, q_lo + 1, q_hi + 1))
+        q_lo += step_q
+        r_lo += step_r
+        if r_lo >= B:
+            q_lo += 1
+            r_lo -= B
+        q_hi += step_q
+        r_hi += step_r
+        if r_hi >= B:
+            q_hi += 1
+            r_hi -= B
     return out
 
 
@@ -66,7 +86,7 @@ def prepare_arrays(runtime: Runtime, config: Config, first: int, end: int, width
         if (b - a) * (d - c) > 50_000_000:
             raise OverflowError('per-subtile candidate opportunity limit exceeded')
     runtime.check()
-    return Prepared(first, end, width, us, vs, bounds, time.perf_counter() - started)
+    return Prepared(first, end, width, us, vs, bounds, time.perf_counter() - started, ranges)
 
 
 def prepare(runtime: Runtime, config: Config, af: GPUFocus, bf: GPUFocus,
@@ -97,12 +117,13 @@ def prepare(runtime: Runtime, config: Config, af: GPUFocus, bf: GPUFocus,
         raise OverflowError('focus buffer overflow; tile invalid')
     native.check(code, 'pcube_prepare')
     us.size, vs.size = nu.value, nv.value
-    rows = [tuple(int(x) for x in bounds[4 * i:4 * i + 4]) for i in range(len(ranges))]
+    flat = bounds.tolist()
+    rows = [tuple(flat[i:i + 4]) for i in range(0, len(flat), 4)]
     for a, b, c, d in rows:
         if (b - a) * (d - c) > 50_000_000:
             raise OverflowError('per-subtile candidate opportunity limit exceeded')
     runtime.check()
-    return Prepared(first, end, width, us, vs, rows, time.perf_counter() - started)  # type: ignore[arg-type]
+    return Prepared(first, end, width, us, vs, rows, time.perf_counter() - started, ranges)  # type: ignore[arg-type]
 
 
 def python_witnesses(values: list[int], p: int) -> list[dict[str, object]]:
@@ -124,9 +145,10 @@ def batch_row(config: Config, filter_groups: list[list[int]], ranges: list[tuple
     """
     ng = len(filter_groups)
     transfer_started = time.perf_counter()
-    if np.any(counters[:, ng + 2]):
+    table = counters.tolist()
+    if any(row[ng + 2] for row in table):
         raise ArithmeticError('unsigned underflow')
-    if np.any(counters[:, ng + 1] > output_cap):
+    if any(row[ng + 1] > output_cap for row in table):
         raise OverflowError('fused tile output overflow; no truncated coverage')
     all_values: list[int] = []
     for values in per_tile:
@@ -143,19 +165,20 @@ def batch_row(config: Config, filter_groups: list[list[int]], ranges: list[tuple
     scopes = []
     position = 0
     for row, (first, end, _, _) in enumerate(ranges):
-        number = int(counters[row, ng + 1])
+        counts = table[row]
+        number = counts[ng + 1]
         values = sorted(all_values[position:position + number])
         if any(not first <= config.canonical(n)[1] < end for n in values):
             raise ArithmeticError('output assigned to wrong subtile')
-        scopes.append(dict(first=str(first), end=str(end), valid_focused_candidates=int(counters[row, 0]),
-                           group_survivors=[int(n) for n in counters[row, 1:ng + 1]], local_survivors=number,
+        scopes.append(dict(first=str(first), end=str(end), valid_focused_candidates=counts[0],
+                           group_survivors=counts[1:ng + 1], local_survivors=number,
                            output_sha256=digest([str(n) for n in values]) if values else EMPTY_DIGEST))
         position += number
     verification_seconds = time.perf_counter() - verification_started
-    groups = [int(n) for n in counters[:, 1:ng + 1].sum(axis=0)]
+    groups = [sum(col) for col in zip(*(counts[1:ng + 1] for counts in table))] if table else [0] * ng
     return dict(status='PASS', host_verified=True, focus_u=sum(b - a for a, b, _, _ in bounds), focus_v=int(nv),
                 cartesian_opportunities=sum((b - a) * (d - c) for a, b, c, d in bounds),
-                valid_focused_candidates=sum(int(n) for n in counters[:, 0]),
+                valid_focused_candidates=sum(counts[0] for counts in table),
                 genuine_cubes=len(all_values) - len(hits), local_survivors=len(all_values), cube_checks=len(all_values),
                 kernel_seconds=timings.get('kernel_seconds', 0.0), transfer_seconds=transfer_seconds,
                 verification_seconds=verification_seconds,
@@ -182,7 +205,8 @@ class FusedMatcher:
 
     def run(self, batch: Prepared, output_cap: int = 8192) -> dict[str, Any]:
         started = time.perf_counter()
-        ranges = tile_ranges(self.config, batch.first, batch.end, batch.width)
+        ranges = batch.ranges if batch.ranges is not None else tile_ranges(self.config, batch.first, batch.end,
+                                                                             batch.width)
         if len(ranges) != len(batch.bounds) or not 0 < output_cap <= 8192:
             raise ValueError('invalid prepared geometry or output capacity')
         counters, per_tile, seconds = self.engine.run(batch.us, batch.vs, batch.first, batch.width, len(ranges),

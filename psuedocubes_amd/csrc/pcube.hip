@@ -223,18 +223,39 @@ int bucket_sort(const uint64_t* in, uint64_t n, uint64_t low, uint64_t span, uin
 
 // CRT wheel of a focus: inner ring prefix with modulus M1 < 2^32 and its
 // sorted residues R1, outer rings tested per candidate (pcube_wheel.h).
+// Pinned host staging for the small uploads and downloads of one call (plans,
+// queries, counters).  Every entry point that uses it synchronises before it
+// returns, and calls are serialised, so one grow-only pair serves them all.
+struct Pinned {
+    uint8_t* p = nullptr;
+    uint64_t bytes = 0;
+    int ensure(uint64_t need) {
+        if (need <= bytes && p) return PCUBE_OK;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+        const uint64_t n = std::max<uint64_t>(need, 1ull << 16);
+        HIP_TRY(hipHostMalloc((void**)&p, n, hipHostMallocDefault));
+        bytes = n;
+        return PCUBE_OK;
+    }
+    template <class T>
+    T* as() const { return (T*)p; }
+};
+Pinned g_up_host, g_down_host;
+Buffer g_up_dev;   // device image of the current call's uploads
+
 struct Wheel {
     bool ready = false;
     uint64_t M1 = 1;
     uint32_t n1 = 0, nouter = 0, allow_words = 0, packed = 0;
     std::vector<uint32_t> R1host;
     Modulus outer[kMaxOuterRings];
-    Buffer R1, allow, windows, seg_first, seg_count, seg_offset, win_offsets;
-    ScanWork scan;
+    double rate = 1.0;      // expected fraction of candidates that pass the outer rings
+    Buffer R1, allow, slots, wave_count, seg_excl, chunk_total, win_info;
     void release() {
-        R1.release(); allow.release(); windows.release(); seg_first.release(); seg_count.release();
-        seg_offset.release(); win_offsets.release();
-        scan.partial.release(); scan.partial_off.release(); scan.partial_total.release();
+        R1.release(); allow.release(); slots.release(); wave_count.release(); seg_excl.release();
+        chunk_total.release(); win_info.release();
     }
 };
 
@@ -421,7 +442,7 @@ uint64_t wheel_plan(const Wheel& wh, uint32_t n, const uint64_t* lows, const uin
                 ww.sl = (uint32_t)((ww.ie + S - 1) / S);
                 segs = ww.s0 + (uint64_t)(ww.nt - 2) * ww.sm + ww.sl;
             }
-            if (segs >= (1ull << 32)) segs = ~0ull;  // rejected by wheel_count
+            if (segs >= (1ull << 32)) segs = 1ull << 40;  // rejected by wheel_check
             total += segs;
         }
         plan[w] = ww;
@@ -430,77 +451,121 @@ uint64_t wheel_plan(const Wheel& wh, uint32_t n, const uint64_t* lows, const uin
     return total;
 }
 
-__global__ void k_window_offsets(const uint32_t* __restrict__ wave_offset, const uint64_t* __restrict__ seg_first,
-                                 uint32_t n, uint64_t* __restrict__ out) {
-    uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w <= n) out[w] = wave_offset[seg_first[w] * kWheelWaves];
+// Slot entries per wave for a wheel: twice the expected survivors of 1024
+// candidates plus a margin, in steps of 32; 1024 cannot overflow.
+uint32_t wheel_slot_cap(const Wheel& wh) {
+    const double expect = wh.rate * kWheelWaveSpan;
+    const uint64_t want = ((uint64_t)(2.0 * expect) + 32 + 31) / 32 * 32;
+    return (uint32_t)std::min<uint64_t>(kWheelWaveSpan, want);
 }
 
-// Count pass + scan for n windows; leaves per-window output offsets (n + 1,
-// the last being the total) in wh.win_offsets.  Enqueue only.
-int wheel_count(Wheel& wh, uint32_t n, const uint64_t* lows, const uint64_t* highs, uint64_t* nseg_out) {
-    std::vector<WheelWindow> plan;
-    std::vector<uint64_t> seg_first;
-    const uint64_t nseg = wheel_plan(wh, n, lows, highs, plan, seg_first);
-    if (nseg * kWheelWaves > (uint64_t)kScanSegment * kScanSegment) return fail(PCUBE_INVALID, "window set too large for one pass");
-    *nseg_out = nseg;
-    TRY(wh.windows.ensure(std::max<uint64_t>(1, n) * sizeof(WheelWindow)));
-    TRY(wh.seg_first.ensure((n + 1) * 8));
-    const uint64_t nwaves = nseg * kWheelWaves;
-    TRY(wh.seg_count.ensure(std::max<uint64_t>(1, nwaves) * 4));
-    TRY(wh.seg_offset.ensure((nwaves + 1) * 4));
-    TRY(wh.win_offsets.ensure((n + 1) * 8));
-    if (n) HIP_TRY(hipMemcpyAsync(wh.windows.p, plan.data(), n * sizeof(WheelWindow), hipMemcpyHostToDevice, g.stream));
-    HIP_TRY(hipMemcpyAsync(wh.seg_first.p, seg_first.data(), (n + 1) * 8, hipMemcpyHostToDevice, g.stream));
-    if (nseg) {
-        WheelArgs a;
-        memset(&a, 0, sizeof a);
-        a.R1 = wh.R1.as<uint2>();
-        a.windows = wh.windows.as<WheelWindow>();
-        a.seg_first = wh.seg_first.as<uint64_t>();
-        a.allow = wh.allow.as<uint32_t>();
-        a.wave_count = wh.seg_count.as<uint32_t>();
-        a.M1 = wh.M1; a.n1 = wh.n1; a.nwin = n; a.nouter = wh.nouter; a.allow_words = wh.allow_words;
-        a.packed = wh.packed;
-        for (uint32_t j = 0; j < wh.nouter; ++j) a.outer[j] = wh.outer[j];
-        {
-            void* args[] = {&a};
-            HIP_TRY(hipLaunchKernel(wheel_kernel<false>(wh.packed, wh.nouter), dim3((uint32_t)nseg),
-                                    dim3(kWheelThreads), args, 0, g.stream));
-        }
-        TRY(scan(wh.seg_count.as<uint32_t>(), wh.seg_offset.as<uint32_t>(), nwaves, wh.scan));
-    } else {
-        TRY(fill32(wh.seg_offset.as<uint32_t>(), 1, 0));
-    }
-    hipLaunchKernelGGL(k_window_offsets, dim3(blocks_for(n + 1, 256)), dim3(256), 0, g.stream,
-                       (const uint32_t*)wh.seg_offset.as<uint32_t>(), (const uint64_t*)wh.seg_first.as<uint64_t>(), n,
-                       wh.win_offsets.as<uint64_t>());
+// Words of the device image of one wheel launch's inputs: the window plans,
+// seg_first (n + 1) and a zero overflow flag.
+uint64_t wheel_image_words(uint32_t n) {
+    static_assert(sizeof(WheelWindow) % 8 == 0, "WheelWindow must be 8-byte words");
+    return (uint64_t)n * (sizeof(WheelWindow) / 8) + (n + 1) + 1;
+}
+void wheel_image(const std::vector<WheelWindow>& plan, const std::vector<uint64_t>& seg_first, uint64_t* img) {
+    const uint32_t n = (uint32_t)plan.size();
+    if (n) memcpy(img, plan.data(), n * sizeof(WheelWindow));
+    memcpy(img + (uint64_t)n * (sizeof(WheelWindow) / 8), seg_first.data(), (n + 1) * 8);
+    img[wheel_image_words(n) - 1] = 0;
+}
+
+struct WheelInputs {
+    const WheelWindow* windows;
+    const uint64_t* seg_first;
+    uint32_t* flag;
+    uint32_t n;
+    uint64_t nseg;
+};
+WheelInputs wheel_inputs(uint64_t* img_dev, uint32_t n, uint64_t nseg) {
+    WheelInputs in;
+    in.windows = (const WheelWindow*)img_dev;
+    in.seg_first = img_dev + (uint64_t)n * (sizeof(WheelWindow) / 8);
+    in.flag = (uint32_t*)(img_dev + wheel_image_words(n) - 1);
+    in.n = n;
+    in.nseg = nseg;
+    return in;
+}
+
+int wheel_check(uint64_t nseg) {
+    if (nseg >= (1ull << 31)) return fail(PCUBE_INVALID, "window set too large for one pass");
     return PCUBE_OK;
 }
 
-// Write pass after wheel_count (same windows); values beyond cap are dropped.
-int wheel_write(Wheel& wh, uint32_t n, uint64_t nseg, uint64_t* out, uint64_t cap) {
-    if (!nseg) return PCUBE_OK;
+// Enqueue the test, segment-scan and compaction passes of a wheel launch
+// whose inputs are on the device.  Coordinates at positions >= cap are
+// dropped.  wh.win_info receives the n + 1 window offsets (the last is the
+// total) and, at n + 1, the overflow flag.
+int wheel_enqueue(Wheel& wh, const WheelInputs& in, uint64_t* out, uint64_t cap, uint32_t slot_cap) {
+    TRY(wh.win_info.ensure(((uint64_t)in.n + 2) * 8));
+    if (in.nseg == 0) return fill64(wh.win_info.as<uint64_t>(), (uint64_t)in.n + 2, 0);
+    const uint64_t nwaves = in.nseg * kWheelWaves;
+    const uint64_t nchunks = (in.nseg + kWheelScanChunk - 1) / kWheelScanChunk;
+    TRY(wh.slots.ensure(nwaves * slot_cap * 4));
+    TRY(wh.wave_count.ensure(nwaves * 4));
+    TRY(wh.seg_excl.ensure(in.nseg * 4));
+    TRY(wh.chunk_total.ensure(nchunks * 8));
     WheelArgs a;
     memset(&a, 0, sizeof a);
     a.R1 = wh.R1.as<uint2>();
-    a.windows = wh.windows.as<WheelWindow>();
-    a.seg_first = wh.seg_first.as<uint64_t>();
+    a.windows = in.windows;
+    a.seg_first = in.seg_first;
     a.allow = wh.allow.as<uint32_t>();
-    a.wave_offset = wh.seg_offset.as<uint32_t>();
-    a.out = out;
-    a.cap = cap;
-    a.M1 = wh.M1; a.n1 = wh.n1; a.nwin = n; a.nouter = wh.nouter; a.allow_words = wh.allow_words;
-    a.packed = wh.packed;
+    a.slots = wh.slots.as<uint32_t>();
+    a.wave_count = wh.wave_count.as<uint32_t>();
+    a.flag = in.flag;
+    a.M1 = wh.M1; a.nseg = in.nseg;
+    a.n1 = wh.n1; a.nwin = in.n; a.nouter = wh.nouter; a.allow_words = wh.allow_words;
+    a.packed = wh.packed; a.slot_cap = slot_cap;
     for (uint32_t j = 0; j < wh.nouter; ++j) a.outer[j] = wh.outer[j];
     void* args[] = {&a};
-    HIP_TRY(hipLaunchKernel(wheel_kernel<true>(wh.packed, wh.nouter), dim3((uint32_t)nseg), dim3(kWheelThreads),
-                            args, 0, g.stream));
+    HIP_TRY(hipLaunchKernel(wheel_kernel(wh.packed, wh.nouter), dim3((uint32_t)in.nseg), dim3(kWheelThreads), args, 0,
+                            g.stream));
+    hipLaunchKernelGGL(k_wheel_seg_scan, dim3((uint32_t)nchunks), dim3(256), 0, g.stream,
+                       (const uint32_t*)wh.wave_count.as<uint32_t>(), in.nseg, wh.seg_excl.as<uint32_t>(),
+                       wh.chunk_total.as<uint64_t>());
+    hipLaunchKernelGGL(k_wheel_compact, dim3((uint32_t)in.nseg), dim3(kWheelThreads), 0, g.stream, a,
+                       (const uint32_t*)wh.seg_excl.as<uint32_t>(), (const uint64_t*)wh.chunk_total.as<uint64_t>(),
+                       out, cap, wh.win_info.as<uint64_t>());
+    return PCUBE_OK;
+}
+
+// n windows of a wheel into out, synchronously; offsets (n + 1) on the host.
+// Returns OVERFLOW when the total exceeds cap (out then holds the first cap
+// coordinates).
+int wheel_run(Wheel& wh, uint32_t n, const uint64_t* lows, const uint64_t* highs, uint64_t* out, uint64_t cap,
+              uint64_t* offsets) {
+    std::vector<WheelWindow> plan;
+    std::vector<uint64_t> seg_first;
+    const uint64_t nseg = wheel_plan(wh, n, lows, highs, plan, seg_first);
+    TRY(wheel_check(nseg));
+    const uint64_t words = wheel_image_words(n);
+    TRY(g_up_host.ensure(words * 8));
+    TRY(g_down_host.ensure(((uint64_t)n + 2) * 8));
+    TRY(g_up_dev.ensure(words * 8));
+    wheel_image(plan, seg_first, g_up_host.as<uint64_t>());
+    HIP_TRY(hipMemcpyAsync(g_up_dev.p, g_up_host.p, words * 8, hipMemcpyHostToDevice, g.stream));
+    const WheelInputs in = wheel_inputs(g_up_dev.as<uint64_t>(), n, nseg);
+    uint32_t slot_cap = wheel_slot_cap(wh);
+    for (;;) {
+        TRY(wheel_enqueue(wh, in, out, cap, slot_cap));
+        HIP_TRY(hipMemcpyAsync(g_down_host.p, wh.win_info.p, ((uint64_t)n + 2) * 8, hipMemcpyDeviceToHost, g.stream));
+        HIP_TRY(hipStreamSynchronize(g.stream));
+        const uint64_t* info = g_down_host.as<uint64_t>();
+        if (!info[n + 1] || slot_cap >= (uint32_t)kWheelWaveSpan) break;
+        slot_cap = kWheelWaveSpan;   // a slot overflowed: once more with room for every candidate
+        TRY(fill32(in.flag, 1, 0));
+    }
+    memcpy(offsets, g_down_host.p, ((uint64_t)n + 1) * 8);
+    if (offsets[n] > cap) return fail(PCUBE_OVERFLOW, "focus buffer overflow: %llu coordinates, capacity %llu",
+                                      (unsigned long long)offsets[n], (unsigned long long)cap);
     return PCUBE_OK;
 }
 
 // Windows of one focus into out; offsets (n + 1) on the host.  Returns
-// OVERFLOW, writing nothing, when the total exceeds cap.
+// OVERFLOW when the total exceeds cap; out's contents are then unspecified.
 int focus_windows(Focus& f, uint32_t n, const uint64_t* lows, const uint64_t* highs, uint64_t* out, uint64_t cap,
                   uint64_t* offsets) {
     if (!f.wheel.ready) {
@@ -513,14 +578,7 @@ int focus_windows(Focus& f, uint32_t n, const uint64_t* lows, const uint64_t* hi
                                      (unsigned long long)count, (unsigned long long)cap);
         return PCUBE_OK;
     }
-    uint64_t nseg = 0;
-    TRY(wheel_count(f.wheel, n, lows, highs, &nseg));
-    HIP_TRY(hipMemcpyAsync(offsets, f.wheel.win_offsets.p, (n + 1) * 8, hipMemcpyDeviceToHost, g.stream));
-    HIP_TRY(hipStreamSynchronize(g.stream));
-    if (offsets[n] > cap) return fail(PCUBE_OVERFLOW, "focus buffer overflow: %llu coordinates, capacity %llu",
-                                      (unsigned long long)offsets[n], (unsigned long long)cap);
-    TRY(wheel_write(f.wheel, n, nseg, out, cap));
-    return PCUBE_OK;
+    return wheel_run(f.wheel, n, lows, highs, out, cap, offsets);
 }
 
 // One lease-wide window: the wheel when the focus has one, else the exact
@@ -556,9 +614,12 @@ struct Matcher {
     std::vector<Modulus> mods;
     Buffer d_mods, d_allow;
     Buffer rows, masks, rho, counters, list, list_count, error;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};   // start, bitsets built, match done
     ~Matcher() {
         d_mods.release(); d_allow.release(); rows.release(); masks.release(); rho.release();
         counters.release(); list.release(); list_count.release(); error.release();
+        for (hipEvent_t& e : ev)
+            if (e) (void)hipEventDestroy(e);
     }
 };
 std::vector<std::unique_ptr<Matcher>> g_matchers;
@@ -660,8 +721,9 @@ int collect_values(Matcher& mt, uint64_t ntiles_total, const std::vector<uint64_
     }
     if (expected > values_cap) return fail(PCUBE_OVERFLOW, "host value buffer too small");
     if (!listed) return PCUBE_OK;
-    std::vector<uint64_t> triples(listed * 3);
-    HIP_TRY(hipMemcpyAsync(triples.data(), mt.list.p, listed * 24, hipMemcpyDeviceToHost, g.stream));
+    TRY(g_down_host.ensure(listed * 24));
+    const uint64_t* triples = g_down_host.as<uint64_t>();
+    HIP_TRY(hipMemcpyAsync(g_down_host.p, mt.list.p, listed * 24, hipMemcpyDeviceToHost, g.stream));
     HIP_TRY(hipStreamSynchronize(g.stream));
     std::vector<uint64_t> cursor(offset.begin(), offset.end() - 1);
     for (uint64_t e = 0; e < listed; ++e) {
@@ -699,6 +761,32 @@ __global__ void k_searchsorted_seg(const uint64_t* __restrict__ data, const uint
         if (a[m] < k) l = m + 1; else h = m;
     }
     out[t] = l;
+}
+
+// Subtile bounds of one prepared batch: lower bounds of the 2*ntiles u queries
+// in us and of the 2*ntiles v queries in vs (sizes from each wheel's window
+// info, clamped to the capacities), then both sizes and overflow flags.
+__global__ void k_prepare_bounds(const uint64_t* __restrict__ us, const uint64_t* __restrict__ uinfo, uint64_t us_cap,
+                                 const uint64_t* __restrict__ vs, const uint64_t* __restrict__ vinfo, uint64_t vs_cap,
+                                 const uint64_t* __restrict__ queries, uint32_t ntiles, uint64_t* __restrict__ out) {
+    const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    const uint64_t nq = 4 * (uint64_t)ntiles;
+    if (t < nq) {
+        const bool on_u = t < 2 * (uint64_t)ntiles;
+        const uint64_t* a = on_u ? us : vs;
+        uint64_t l = 0, h = on_u ? min(uinfo[1], us_cap) : min(vinfo[1], vs_cap);
+        const uint64_t k = queries[t];
+        while (l < h) {
+            const uint64_t m = l + ((h - l) >> 1);
+            if (a[m] < k) l = m + 1; else h = m;
+        }
+        out[t] = l;
+    } else if (t == nq) {
+        out[nq] = uinfo[1];
+        out[nq + 1] = vinfo[1];
+        out[nq + 2] = uinfo[2];
+        out[nq + 3] = vinfo[2];
+    }
 }
 
 // Batched engine: configurations, focus wheels and the grow-only buffers one
@@ -774,8 +862,8 @@ const Kernel kKernels[] = {
     {"window_emit", (const void*)k_window<kEmit>},
     {"window_slots", (const void*)k_window<kSlots>},
     {"slots_compact", (const void*)k_slots_compact},
-    {"wheel_count", (const void*)k_wheel<false, 4>},
-    {"wheel_write", (const void*)k_wheel<true, 4>},
+    {"wheel_slots", (const void*)k_wheel_slots<4>},
+    {"wheel_compact", (const void*)k_wheel_compact},
     {"bucket_sort", (const void*)k_bucket_sort},
     {"scan_local", (const void*)k_scan_local},
     {"rows", (const void*)k_rows},
@@ -971,6 +1059,7 @@ int pcube_focus_wheel(int focus, const uint32_t* moduli, const uint32_t* sizes, 
     Wheel& wh = f->wheel;
     wh.release();
     wh.ready = false;
+    wh.rate = 1.0;
     uint64_t M1 = 1, n1 = 1;
     uint32_t k = 0;
     while (k < nrings && M1 * moduli[k] < (1ull << 32) && n1 * sizes[k] <= kMaxInnerResidues) {
@@ -1011,6 +1100,7 @@ int pcube_focus_wheel(int focus, const uint32_t* moduli, const uint32_t* sizes, 
         offset += sizes[k + j];
         bit += q;
         wh.outer[j] = m;
+        wh.rate *= (double)sizes[k + j] / q;
     }
     if (bits.empty()) bits.push_back(0);
     if (bits.size() > (size_t)kMaxOuterBits / 32) return fail(PCUBE_INVALID, "outer rings too large for the wheel");
@@ -1061,45 +1151,59 @@ int pcube_prepare(int focus_u, int focus_v, uint32_t ntiles, const uint64_t* til
     const uint64_t ulo = tiles[2], uhi = tiles[4 * (ntiles - 1) + 3];
     const uint64_t vlo = tiles[0], vhi = tiles[4 * (ntiles - 1) + 1];
     if (ulo > uhi || vlo > vhi) return fail(PCUBE_INVALID, "invalid batch geometry");
-    // subtile queries go up first: (u_min, u_end) into us, (v_first, v_end) into vs
-    std::vector<uint64_t> q(4 * (uint64_t)ntiles);
+    std::vector<WheelWindow> pu, pv;
+    std::vector<uint64_t> su, sv;
+    const uint64_t nseg_u = wheel_plan(fu->wheel, 1, &ulo, &uhi, pu, su);
+    const uint64_t nseg_v = wheel_plan(fv->wheel, 1, &vlo, &vhi, pv, sv);
+    TRY(wheel_check(nseg_u));
+    TRY(wheel_check(nseg_v));
+    // one upload: subtile queries ((u_min, u_end) per subtile, then (v_first, v_end)), both wheel images
+    const uint64_t nq = 4 * (uint64_t)ntiles, iw = wheel_image_words(1);
+    const uint64_t up_words = nq + 2 * iw, down_words = nq + 4;
+    TRY(g_up_host.ensure(up_words * 8));
+    TRY(g_down_host.ensure(down_words * 8));
+    TRY(g_up_dev.ensure(up_words * 8));
+    TRY(fu->queries.ensure(down_words * 8));
+    uint64_t* h = g_up_host.as<uint64_t>();
     for (uint32_t t = 0; t < ntiles; ++t) {
-        q[2 * t] = tiles[4 * t + 2];
-        q[2 * t + 1] = tiles[4 * t + 3];
-        q[2 * ntiles + 2 * t] = tiles[4 * t];
-        q[2 * ntiles + 2 * t + 1] = tiles[4 * t + 1];
+        h[2 * t] = tiles[4 * t + 2];
+        h[2 * t + 1] = tiles[4 * t + 3];
+        h[2 * ntiles + 2 * t] = tiles[4 * t];
+        h[2 * ntiles + 2 * t + 1] = tiles[4 * t + 1];
     }
-    Buffer& dq = fu->queries;
-    TRY(dq.ensure(8 * 8 * (uint64_t)ntiles));
-    uint64_t* dqueries = dq.as<uint64_t>();
-    uint64_t* dpos = dqueries + 4 * (uint64_t)ntiles;
-    HIP_TRY(hipMemcpyAsync(dqueries, q.data(), 32 * (uint64_t)ntiles, hipMemcpyHostToDevice, g.stream));
-    uint64_t offu[2] = {0, 0}, offv[2] = {0, 0}, nseg_u = 0, nseg_v = 0;
-    TRY(wheel_count(fu->wheel, 1, &ulo, &uhi, &nseg_u));
-    TRY(wheel_count(fv->wheel, 1, &vlo, &vhi, &nseg_v));
-    HIP_TRY(hipMemcpyAsync(offu, fu->wheel.win_offsets.p, 16, hipMemcpyDeviceToHost, g.stream));
-    HIP_TRY(hipMemcpyAsync(offv, fv->wheel.win_offsets.p, 16, hipMemcpyDeviceToHost, g.stream));
-    HIP_TRY(hipStreamSynchronize(g.stream));
-    *nu = offu[1];
-    *nv = offv[1];
-    if (offu[1] > us_cap || offv[1] > vs_cap)
+    wheel_image(pu, su, h + nq);
+    wheel_image(pv, sv, h + nq + iw);
+    HIP_TRY(hipMemcpyAsync(g_up_dev.p, g_up_host.p, up_words * 8, hipMemcpyHostToDevice, g.stream));
+    uint64_t* d = g_up_dev.as<uint64_t>();
+    const WheelInputs in_u = wheel_inputs(d + nq, 1, nseg_u), in_v = wheel_inputs(d + nq + iw, 1, nseg_v);
+    uint32_t cap_u = wheel_slot_cap(fu->wheel), cap_v = wheel_slot_cap(fv->wheel);
+    const uint64_t* r = g_down_host.as<uint64_t>();
+    for (;;) {
+        TRY(wheel_enqueue(fu->wheel, in_u, (uint64_t*)us_out, us_cap, cap_u));
+        TRY(wheel_enqueue(fv->wheel, in_v, (uint64_t*)vs_out, vs_cap, cap_v));
+        hipLaunchKernelGGL(k_prepare_bounds, dim3(blocks_for(nq + 1, 256)), dim3(256), 0, g.stream,
+                           (const uint64_t*)us_out, (const uint64_t*)fu->wheel.win_info.as<uint64_t>(), us_cap,
+                           (const uint64_t*)vs_out, (const uint64_t*)fv->wheel.win_info.as<uint64_t>(), vs_cap,
+                           (const uint64_t*)d, ntiles, fu->queries.as<uint64_t>());
+        HIP_TRY(hipMemcpyAsync(g_down_host.p, fu->queries.p, down_words * 8, hipMemcpyDeviceToHost, g.stream));
+        HIP_TRY(hipStreamSynchronize(g.stream));
+        const bool again_u = r[nq + 2] && cap_u < (uint32_t)kWheelWaveSpan;
+        const bool again_v = r[nq + 3] && cap_v < (uint32_t)kWheelWaveSpan;
+        if (!again_u && !again_v) break;
+        // a slot overflowed: once more with room for every candidate
+        if (again_u) { cap_u = kWheelWaveSpan; TRY(fill32(in_u.flag, 1, 0)); }
+        if (again_v) { cap_v = kWheelWaveSpan; TRY(fill32(in_v.flag, 1, 0)); }
+    }
+    *nu = r[nq];
+    *nv = r[nq + 1];
+    if (*nu > us_cap || *nv > vs_cap)
         return fail(PCUBE_OVERFLOW, "focus buffer overflow; tile invalid (%llu, %llu coordinates)",
-                    (unsigned long long)offu[1], (unsigned long long)offv[1]);
-    TRY(wheel_write(fu->wheel, 1, nseg_u, (uint64_t*)us_out, us_cap));
-    TRY(wheel_write(fv->wheel, 1, nseg_v, (uint64_t*)vs_out, vs_cap));
-    hipLaunchKernelGGL(k_searchsorted, dim3(blocks_for(2 * ntiles, 256)), dim3(256), 0, g.stream, (const uint64_t*)us_out,
-                       offu[1], (const uint64_t*)dqueries, 2 * (uint64_t)ntiles, dpos);
-    hipLaunchKernelGGL(k_searchsorted, dim3(blocks_for(2 * ntiles, 256)), dim3(256), 0, g.stream, (const uint64_t*)vs_out,
-                       offv[1], (const uint64_t*)(dqueries + 2 * (uint64_t)ntiles), 2 * (uint64_t)ntiles,
-                       dpos + 2 * (uint64_t)ntiles);
-    std::vector<uint64_t> pos(4 * (uint64_t)ntiles);
-    HIP_TRY(hipMemcpyAsync(pos.data(), dpos, 32 * (uint64_t)ntiles, hipMemcpyDeviceToHost, g.stream));
-    HIP_TRY(hipStreamSynchronize(g.stream));
+                    (unsigned long long)*nu, (unsigned long long)*nv);
     for (uint32_t t = 0; t < ntiles; ++t) {
-        bounds[4 * t] = pos[2 * t];
-        bounds[4 * t + 1] = pos[2 * t + 1];
-        bounds[4 * t + 2] = pos[2 * ntiles + 2 * t];
-        bounds[4 * t + 3] = pos[2 * ntiles + 2 * t + 1];
+        bounds[4 * t] = r[2 * t];
+        bounds[4 * t + 1] = r[2 * t + 1];
+        bounds[4 * t + 2] = r[2 * ntiles + 2 * t];
+        bounds[4 * t + 3] = r[2 * ntiles + 2 * t + 1];
     }
     return PCUBE_OK;
 }
@@ -1200,10 +1304,9 @@ int pcube_match(int matcher, uint64_t us, uint64_t nu, uint64_t vs, uint64_t nv,
     TRY(fill64(mt->counters.as<uint64_t>(), nc, 0));
     TRY(fill64(mt->list_count.as<uint64_t>(), 1, 0));
     TRY(fill32(mt->error.as<uint32_t>(), 1, 0));
-    hipEvent_t e0, e1, e2;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
-    HIP_TRY(hipEventCreate(&e2));
+    for (hipEvent_t& e : mt->ev)
+        if (!e) HIP_TRY(hipEventCreate(&e));
+    hipEvent_t e0 = mt->ev[0], e1 = mt->ev[1], e2 = mt->ev[2];
     HIP_TRY(hipEventRecord(e0, g.stream));
     int rc = match_enqueue(*mt, (const uint64_t*)us, nu, (const uint64_t*)vs, nv, v_first, tile_width, ntiles, cap,
                            engine, mt->counters.as<uint64_t>(), 0, list_cap, e1, e2);
@@ -1212,13 +1315,17 @@ int pcube_match(int matcher, uint64_t us, uint64_t nu, uint64_t vs, uint64_t nv,
         HIP_TRY(hipEventRecord(e1, g.stream));
         HIP_TRY(hipEventRecord(e2, g.stream));
     }
-    std::vector<uint64_t> host(nc);
-    uint32_t err = 0;
-    uint64_t listed = 0;
-    HIP_TRY(hipMemcpyAsync(host.data(), mt->counters.p, nc * 8, hipMemcpyDeviceToHost, g.stream));
-    HIP_TRY(hipMemcpyAsync(&err, mt->error.p, 4, hipMemcpyDeviceToHost, g.stream));
-    HIP_TRY(hipMemcpyAsync(&listed, mt->list_count.p, 8, hipMemcpyDeviceToHost, g.stream));
+    // one pinned download: [survivor count, error word, counters]
+    TRY(g_down_host.ensure((nc + 2) * 8));
+    uint64_t* hd = g_down_host.as<uint64_t>();
+    hd[1] = 0;
+    HIP_TRY(hipMemcpyAsync(hd, mt->list_count.p, 8, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipMemcpyAsync(hd + 1, mt->error.p, 4, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipMemcpyAsync(hd + 2, mt->counters.p, nc * 8, hipMemcpyDeviceToHost, g.stream));
     HIP_TRY(hipStreamSynchronize(g.stream));
+    const uint64_t listed = hd[0];
+    const uint32_t err = (uint32_t)hd[1];
+    std::vector<uint64_t> host(hd + 2, hd + 2 + nc);
     if (seconds) {
         float a = 0, b = 0;
         HIP_TRY(hipEventElapsedTime(&a, e0, e1));
@@ -1226,9 +1333,6 @@ int pcube_match(int matcher, uint64_t us, uint64_t nu, uint64_t vs, uint64_t nv,
         seconds[0] = a / 1000.0;
         seconds[1] = b / 1000.0;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipEventDestroy(e2);
     memcpy(counters, host.data(), nc * 8);
     rc = batch_status(*mt, host, ntiles, cap, err);
     if (rc) return rc;
@@ -1441,19 +1545,26 @@ int pcube_batch_run(int engine, uint32_t nwin, const uint32_t* win_cfg, const ui
     hipEvent_t ev[4];
     for (auto& x : ev) HIP_TRY(hipEventCreate(&x));
     HIP_TRY(hipEventRecord(ev[0], g.stream));
-    // all windows of both wheels: count, size, allocate, write
+    // all windows of both wheels, into buffers sized by the expected counts;
+    // an overflow sizes them exactly and runs again
     std::vector<uint64_t> offu(nwin + 1), offv(nwin + 1);
-    uint64_t nseg_u = 0, nseg_v = 0;
-    TRY(wheel_count(fu->wheel, nwin, ulo.data(), uhi.data(), &nseg_u));
-    HIP_TRY(hipMemcpyAsync(offu.data(), fu->wheel.win_offsets.p, (nwin + 1) * 8, hipMemcpyDeviceToHost, g.stream));
-    TRY(wheel_count(fv->wheel, nwin, vlo.data(), vhi.data(), &nseg_v));
-    HIP_TRY(hipMemcpyAsync(offv.data(), fv->wheel.win_offsets.p, (nwin + 1) * 8, hipMemcpyDeviceToHost, g.stream));
-    HIP_TRY(hipStreamSynchronize(g.stream));
-    TRY(e.lus.ensure(std::max<uint64_t>(8, offu[nwin] * 8)));
-    TRY(e.lvs.ensure(std::max<uint64_t>(8, offv[nwin] * 8)));
-    // each wheel keeps its own plan from the count pass
-    TRY(wheel_write(fu->wheel, nwin, nseg_u, e.lus.as<uint64_t>(), offu[nwin]));
-    TRY(wheel_write(fv->wheel, nwin, nseg_v, e.lvs.as<uint64_t>(), offv[nwin]));
+    auto windows_of = [&](Focus& f, const std::vector<uint64_t>& lo, const std::vector<uint64_t>& hi, Buffer& buf,
+                          std::vector<uint64_t>& off) -> int {
+        double span = 0;
+        for (uint32_t w = 0; w < nwin; ++w) span += (double)(hi[w] - lo[w]);
+        const double density = f.wheel.rate * f.wheel.n1 / (double)f.wheel.M1;
+        uint64_t cap = std::max<uint64_t>((uint64_t)(span * density * 1.02) + 64ull * nwin + 4096, buf.bytes / 8);
+        TRY(buf.ensure(cap * 8));
+        int rc = wheel_run(f.wheel, nwin, lo.data(), hi.data(), buf.as<uint64_t>(), cap, off.data());
+        if (rc == PCUBE_OVERFLOW) {
+            cap = off[nwin];
+            TRY(buf.ensure(cap * 8));
+            rc = wheel_run(f.wheel, nwin, lo.data(), hi.data(), buf.as<uint64_t>(), cap, off.data());
+        }
+        return rc;
+    };
+    TRY(windows_of(*fu, ulo, uhi, e.lus, offu));
+    TRY(windows_of(*fv, vlo, vhi, e.lvs, offv));
     // subtile bounds inside each window
     const uint64_t nq = 4 * ntot;
     std::vector<uint64_t> qb(nq), ql(nq), qk(nq), qo(nq);

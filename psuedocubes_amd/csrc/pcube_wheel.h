@@ -11,8 +11,13 @@
 // to the window width times |R1| / M1, independent of the wheel's size.
 //
 // Work is cut into segments of kWheelSegment consecutive candidates of one t;
-// a block handles one segment.  Pass 1 counts each segment's survivors, a scan
-// turns the counts into offsets, pass 2 writes survivors in order.
+// a block handles one segment, each of its four waves 1024 candidates.  One
+// test pass writes each wave's survivors (their inner residues r) into a slot
+// of slot_cap entries and its count; a scan over segments gives each segment's
+// output offset; a compaction pass copies the slots to the dense, ascending
+// output and writes every window's offset.  A wave with more survivors than
+// slot_cap raises the overflow flag and the host runs again with
+// slot_cap = 1024, which cannot overflow.
 //
 // Replaces the role of GPUFocus.window (pseudocube/gpu.py:19-35, 94-114):
 // same coordinate set, same ascending order.
@@ -24,6 +29,8 @@ constexpr int kWheelThreads = 256;
 constexpr int kWheelWaves = kWheelThreads / kWave;
 constexpr int kWheelSteps = 16;                               // 64-candidate steps per wave
 constexpr int kWheelSegment = kWheelThreads * kWheelSteps;    // candidates per block
+constexpr int kWheelWaveSpan = kWheelSteps * kWave;           // candidates per wave
+constexpr int kWheelScanChunk = 1024;                         // segments per segment-scan block
 constexpr int kMaxOuterRings = 8;
 constexpr int kMaxOuterBits = 8 * 1024;   // allowed-residue bits of the outer rings
 
@@ -41,39 +48,27 @@ struct WheelArgs {
     const WheelWindow* windows;  // nwin windows
     const uint64_t* seg_first;   // nwin + 1 prefix of segment counts
     const uint32_t* allow;       // outer-ring allowed bits, rings at Modulus::bit
-    uint32_t* wave_count;        // pass 1 output, one per (segment, wave)
-    const uint32_t* wave_offset; // pass 2 input, exclusive scan of wave_count
-    uint64_t* out;               // pass 2 output
-    uint64_t M1, cap;
-    uint32_t n1, nwin, nouter, allow_words, packed;
+    uint32_t* slots;             // slot_cap inner residues per (segment, wave)
+    uint32_t* wave_count;        // survivors per (segment, wave)
+    uint32_t* flag;              // set to 1 when a wave overflows its slot
+    uint64_t M1, nseg;
+    uint32_t n1, nwin, nouter, allow_words, packed, slot_cap;
     Modulus outer[kMaxOuterRings];   // q, inv, r32 for reduction; b = M1 mod q; bit offset
 };
 
-// One block per segment, four waves per block, each wave 16 steps of 64
-// consecutive candidates: loads are coalesced and a wave ballot keeps the
-// survivors in candidate order.
-// kOuter >= 0: exactly kOuter outer rings with residues packed as bytes, checks
-// unrolled; kOuter < 0: any outer rings, residues reduced in the loop.
-template <bool kWrite, int kOuter>
-__global__ __launch_bounds__(kWheelThreads) void k_wheel(WheelArgs a) {
-    __shared__ uint32_t allow[kMaxOuterBits / 32];
-    __shared__ uint32_t s_window;
-    for (uint32_t t = threadIdx.x; t < a.allow_words; t += kWheelThreads) allow[t] = a.allow[t];
-    const uint64_t seg = blockIdx.x;
-    if (threadIdx.x == 0) {
-        uint32_t l = 0, h = a.nwin;  // last window with seg_first <= seg
-        while (h - l > 1) {
-            const uint32_t m = (l + h) >> 1;
-            if (a.seg_first[m] <= seg) l = m; else h = m;
-        }
-        s_window = l;
+// Window of a segment: the last window whose first segment is <= seg.
+__device__ __forceinline__ uint32_t wheel_window_of(const WheelArgs& a, uint64_t seg) {
+    uint32_t l = 0, h = a.nwin;
+    while (h - l > 1) {
+        const uint32_t m = (l + h) >> 1;
+        if (a.seg_first[m] <= seg) l = m; else h = m;
     }
-    __syncthreads();
-    const WheelWindow w = a.windows[s_window];
-    const uint32_t s = uint32_t(seg - w.seg_first);   // segments of one window fit 32 bits (host checks)
-    // turn and candidate range of this segment
-    uint64_t t;
-    uint32_t begin, end;
+    return l;
+}
+
+// Turn t and candidate range [begin, end) of segment s of window w.
+__device__ __forceinline__ void wheel_segment(const WheelWindow& w, uint32_t n1, uint32_t s, uint64_t& t,
+                                              uint32_t& begin, uint32_t& end) {
     if (w.nt == 1) {
         t = w.t0;
         begin = w.ib + s * kWheelSegment;
@@ -81,27 +76,46 @@ __global__ __launch_bounds__(kWheelThreads) void k_wheel(WheelArgs a) {
     } else if (s < w.s0) {
         t = w.t0;
         begin = w.ib + s * kWheelSegment;
-        end = min(begin + kWheelSegment, a.n1);
+        end = min(begin + kWheelSegment, n1);
     } else if (s - w.s0 < (w.nt - 2) * w.sm) {
         const uint32_t r = s - w.s0;
         t = w.t0 + 1 + r / w.sm;
         begin = (r % w.sm) * kWheelSegment;
-        end = min(begin + kWheelSegment, a.n1);
+        end = min(begin + kWheelSegment, n1);
     } else {
         const uint32_t r = s - w.s0 - (w.nt - 2) * w.sm;
         t = w.t0 + w.nt - 1;
         begin = r * kWheelSegment;
         end = min(begin + kWheelSegment, w.ie);
     }
+}
+
+// Test pass: one block per segment, four waves per block, each wave 16 steps
+// of 64 consecutive candidates.  Loads are coalesced and a wave ballot keeps
+// the survivors in candidate order within the wave's slot.
+// kOuter >= 0: exactly kOuter outer rings with residues packed as bytes, checks
+// unrolled; kOuter < 0: any outer rings, residues reduced in the loop.
+template <int kOuter>
+__global__ __launch_bounds__(kWheelThreads) void k_wheel_slots(WheelArgs a) {
+    __shared__ uint32_t allow[kMaxOuterBits / 32];
+    __shared__ uint32_t s_window;
+    for (uint32_t t = threadIdx.x; t < a.allow_words; t += kWheelThreads) allow[t] = a.allow[t];
+    const uint64_t seg = blockIdx.x;
+    if (threadIdx.x == 0) s_window = wheel_window_of(a, seg);
+    __syncthreads();
+    const WheelWindow w = a.windows[s_window];
+    uint64_t t;
+    uint32_t begin, end;
+    wheel_segment(w, a.n1, uint32_t(seg - w.seg_first), t, begin, end);  // segments of one window fit 32 bits
     const uint64_t ybase = t * a.M1;
     uint32_t off[kMaxOuterRings];
 #pragma unroll
     for (int j = 0; j < kMaxOuterRings; ++j)
         off[j] = j < (int)a.nouter ? mod32(mod64(t, a.outer[j]) * a.outer[j].b, a.outer[j]) : 0;
     const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    const uint32_t wbegin = begin + wave * (kWheelSteps * kWave);
-    const uint32_t wend = min(wbegin + kWheelSteps * kWave, end);
-    uint64_t pos = kWrite ? (uint64_t)a.wave_offset[seg * kWheelWaves + wave] : 0;
+    const uint32_t wbegin = begin + wave * kWheelWaveSpan;
+    const uint32_t wend = min(wbegin + kWheelWaveSpan, end);
+    uint32_t* slot = a.slots + (seg * kWheelWaves + wave) * a.slot_cap;
     uint32_t count = 0;
     // issue every load of this wave first: one memory latency instead of sixteen
     uint2 e[kWheelSteps];
@@ -130,19 +144,111 @@ __global__ __launch_bounds__(kWheelThreads) void k_wheel(WheelArgs a) {
             }
         }
         const uint64_t live = __ballot(ok);
-        if (kWrite && ok) {
-            const uint64_t slot = pos + lanes_below(live);
-            if (slot < a.cap) a.out[slot] = y;
+        if (ok) {
+            const uint32_t at = count + lanes_below(live);
+            if (at < a.slot_cap) slot[at] = e[k].x;
         }
-        pos += __popcll(live);
         count += __popcll(live);
     }
-    if (!kWrite && lane == 0) a.wave_count[seg * kWheelWaves + wave] = count;
+    if (lane == 0) {
+        a.wave_count[seg * kWheelWaves + wave] = count;
+        if (count > a.slot_cap) *a.flag = 1u;
+    }
 }
 
-#define PCUBE_WHEEL(K)                                      \
-    template __global__ void k_wheel<false, K>(WheelArgs);  \
-    template __global__ void k_wheel<true, K>(WheelArgs);
+// Inclusive scan over the 64 lanes of a wave.
+__device__ __forceinline__ uint32_t wheel_wave_scan(uint32_t x, uint32_t lane) {
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d, kWave);
+        if (lane >= (uint32_t)d) x += y;
+    }
+    return x;
+}
+
+// Segment scan: block c covers segments [1024c, 1024c + 1024), four per
+// thread.  seg_excl[s] = survivors of the chunk's segments before s;
+// chunk_total[c] = survivors of the chunk.
+__global__ __launch_bounds__(256) void k_wheel_seg_scan(const uint32_t* __restrict__ wave_count, uint64_t nseg,
+                                                        uint32_t* __restrict__ seg_excl,
+                                                        uint64_t* __restrict__ chunk_total) {
+    __shared__ uint32_t wave_sum[256 / kWave];
+    const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const uint64_t s0 = (uint64_t)blockIdx.x * kWheelScanChunk + 4 * threadIdx.x;
+    uint32_t c[4];
+    uint32_t mine = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        c[k] = 0;
+        if (s0 + k < nseg) {
+            const uint4 v = *(const uint4*)(wave_count + (s0 + k) * kWheelWaves);
+            c[k] = v.x + v.y + v.z + v.w;
+        }
+        mine += c[k];
+    }
+    const uint32_t inc = wheel_wave_scan(mine, lane);
+    if (lane == kWave - 1) wave_sum[wave] = inc;
+    __syncthreads();
+    uint32_t base = 0;
+    for (uint32_t k = 0; k < wave; ++k) base += wave_sum[k];
+    uint32_t run = base + inc - mine;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (s0 + k < nseg) seg_excl[s0 + k] = run;
+        run += c[k];
+    }
+    if (threadIdx.x == 255) chunk_total[blockIdx.x] = run;
+}
+
+// Compaction: one block per segment.  Output offset of the segment = the
+// chunk totals before its chunk + its exclusive offset within the chunk; each
+// wave copies its slot to out, dropping positions at or beyond cap.  The
+// first segment of each window writes that window's offset; the last
+// segment writes the total into every later entry of win_info, and
+// win_info[nwin + 1] receives the overflow flag.
+__global__ __launch_bounds__(kWheelThreads) void k_wheel_compact(WheelArgs a, const uint32_t* __restrict__ seg_excl,
+                                                                 const uint64_t* __restrict__ chunk_total,
+                                                                 uint64_t* __restrict__ out, uint64_t cap,
+                                                                 uint64_t* __restrict__ win_info) {
+    __shared__ uint64_t partial[kWheelThreads / kWave];
+    __shared__ uint32_t s_window;
+    const uint64_t seg = blockIdx.x;
+    const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const uint64_t chunk = seg / kWheelScanChunk;
+    uint64_t acc = 0;
+    for (uint64_t k = threadIdx.x; k < chunk; k += kWheelThreads) acc += chunk_total[k];
+#pragma unroll
+    for (int d = kWave / 2; d >= 1; d >>= 1) acc += __shfl_xor(acc, d, kWave);
+    if (lane == 0) partial[wave] = acc;
+    if (threadIdx.x == 0) s_window = wheel_window_of(a, seg);
+    __syncthreads();
+    const uint64_t prefix = partial[0] + partial[1] + partial[2] + partial[3] + seg_excl[seg];
+    const uint4 counts = *(const uint4*)(a.wave_count + seg * kWheelWaves);
+    const uint32_t cw[4] = {counts.x, counts.y, counts.z, counts.w};
+    uint64_t base = prefix;
+    for (uint32_t k = 0; k < wave; ++k) base += cw[k];
+    const WheelWindow w = a.windows[s_window];
+    uint64_t t;
+    uint32_t begin, end;
+    wheel_segment(w, a.n1, uint32_t(seg - w.seg_first), t, begin, end);
+    const uint64_t ybase = t * a.M1;
+    const uint32_t n = min(cw[wave], a.slot_cap);
+    const uint32_t* slot = a.slots + (seg * kWheelWaves + wave) * a.slot_cap;
+    for (uint32_t i = lane; i < n; i += kWave) {
+        const uint64_t o = base + i;
+        if (o < cap) out[o] = ybase + slot[i];
+    }
+    if (threadIdx.x == 0) {
+        for (int64_t v = s_window; v >= 0 && a.seg_first[v] == seg; --v) win_info[v] = prefix;
+        if (seg + 1 == a.nseg) {
+            const uint64_t total = prefix + cw[0] + cw[1] + cw[2] + cw[3];
+            for (uint32_t v = s_window + 1; v <= a.nwin; ++v) win_info[v] = total;
+            win_info[a.nwin + 1] = *a.flag;
+        }
+    }
+}
+
+#define PCUBE_WHEEL(K) template __global__ void k_wheel_slots<K>(WheelArgs);
 PCUBE_WHEEL(-1)
 PCUBE_WHEEL(0)
 PCUBE_WHEEL(1)
@@ -152,15 +258,14 @@ PCUBE_WHEEL(4)
 #undef PCUBE_WHEEL
 
 // Host-side selection of the instantiation for a wheel.
-template <bool kWrite>
 inline const void* wheel_kernel(uint32_t packed, uint32_t nouter) {
-    if (!packed) return (const void*)k_wheel<kWrite, -1>;
+    if (!packed) return (const void*)k_wheel_slots<-1>;
     switch (nouter) {
-        case 0: return (const void*)k_wheel<kWrite, 0>;
-        case 1: return (const void*)k_wheel<kWrite, 1>;
-        case 2: return (const void*)k_wheel<kWrite, 2>;
-        case 3: return (const void*)k_wheel<kWrite, 3>;
-        default: return (const void*)k_wheel<kWrite, 4>;
+        case 0: return (const void*)k_wheel_slots<0>;
+        case 1: return (const void*)k_wheel_slots<1>;
+        case 2: return (const void*)k_wheel_slots<2>;
+        case 3: return (const void*)k_wheel_slots<3>;
+        default: return (const void*)k_wheel_slots<4>;
     }
 }
 

@@ -190,7 +190,7 @@ class GPUFocus:
             native.call('pcube_focus_wheel', self.handle, native.ptr32(moduli), native.ptr32(counts),
                         native.ptr32(residues), len(focus.rings), ctypes.byref(inner), ctypes.byref(n1))
             self.wheel = dict(inner_modulus=inner.value, inner_residues=n1.value)
-            runtime.compile('wheel_count')
+            runtime.compile('wheel_slots')
         else:
             runtime.compile('window_count')
         runtime.check()
@@ -392,11 +392,13 @@ class MatchEngine:
         native.check(code, 'pcube_match')
         self.runtime.check()
         per_tile: list[list[int]] = []
+        emitted = counters[:, self.ng + 1].tolist()
+        total = sum(min(n, cap) for n in emitted)
+        limbs = values[:2 * total].tolist()
         offset = 0
-        for t in range(ntiles):
-            n = min(int(counters[t, self.ng + 1]), cap)
-            limbs = values[2 * offset:2 * (offset + n)]
-            per_tile.append([int(limbs[2 * i]) + (int(limbs[2 * i + 1]) << 64) for i in range(n)])
+        for n in emitted:
+            n = min(n, cap)
+            per_tile.append([limbs[2 * i] + (limbs[2 * i + 1] << 64) for i in range(offset, offset + n)])
             offset += n
         return counters, per_tile, [seconds[0], seconds[1]]
 

@@ -185,3 +185,22 @@ def test_oracle_wide_limbs() -> None:
             l2, h2 = ctypes.c_uint64(), ctypes.c_uint64()
             assert ref(a, b, c, d, ctypes.byref(l2), ctypes.byref(h2)) == borrow
             assert (l2.value, h2.value) == (low.value, high.value)
+
+
+def test_tile_ranges_against_one_division_per_bound() -> None:
+    """Subtile geometry stepped by quotient/remainder updates equals the direct
+    u_interval of every subtile's first and last v (the reference's tile_ranges,
+    pseudocube/batch_experiment.py:27-32), at interior, ragged and final tiles."""
+    from psuedocubes_amd.batch_experiment import tile_ranges
+    rng = random.Random(27)
+    for p in (313, 433, 619):
+        config = Config(p, 10**27, 3 * 10**27, A619, B619)
+        for _ in range(400):
+            width = rng.choice([1, 7, 10**8, rng.randrange(1, 10**10)])
+            first = rng.randrange(0, config.B - 1)
+            if rng.random() < 0.15:
+                first = max(0, config.B - rng.randrange(1, 64 * width + 1))
+            end = min(config.B, first + rng.randrange(1, 64 * width + 1))
+            expect = [(v, min(v + width, end), config.u_interval(v)[0], config.u_interval(min(v + width, end) - 1)[1])
+                      for v in range(first, end, width)]
+            assert tile_ranges(config, first, end, width) == expect
