@@ -146,13 +146,15 @@ def batch_row(config: Config, filter_groups: list[list[int]], ranges: list[tuple
     ng = len(filter_groups)
     transfer_started = time.perf_counter()
     table = counters.tolist()
-    if any(row[ng + 2] for row in table):
+    emitted = [row[ng + 1] for row in table]
+    if counters[:, ng + 2].any():
         raise ArithmeticError('unsigned underflow')
-    if any(row[ng + 1] > output_cap for row in table):
+    if emitted and max(emitted) > output_cap:
         raise OverflowError('fused tile output overflow; no truncated coverage')
     all_values: list[int] = []
     for values in per_tile:
-        all_values.extend(sorted(values))
+        if values:
+            all_values.extend(sorted(values))
     transfer_seconds = time.perf_counter() - transfer_started + timings.get('transfer_seconds', 0.0)
     verification_started = time.perf_counter()
     if len(all_values) != len(set(all_values)):
@@ -164,21 +166,25 @@ def batch_row(config: Config, filter_groups: list[list[int]], ranges: list[tuple
         raise ArithmeticError('independent witness verification failed')
     scopes = []
     position = 0
-    for row, (first, end, _, _) in enumerate(ranges):
-        counts = table[row]
-        number = counts[ng + 1]
-        values = sorted(all_values[position:position + number])
-        if any(not first <= config.canonical(n)[1] < end for n in values):
-            raise ArithmeticError('output assigned to wrong subtile')
-        scopes.append(dict(first=str(first), end=str(end), valid_focused_candidates=counts[0],
-                           group_survivors=counts[1:ng + 1], local_survivors=number,
-                           output_sha256=digest([str(n) for n in values]) if values else EMPTY_DIGEST))
-        position += number
+    for (first, end, _, _), counts, number in zip(ranges, table, emitted):
+        sha = EMPTY_DIGEST
+        if number:
+            values = sorted(all_values[position:position + number])
+            if any(not first <= config.canonical(n)[1] < end for n in values):
+                raise ArithmeticError('output assigned to wrong subtile')
+            sha = digest([str(n) for n in values])
+            position += number
+        scopes.append({'first': str(first), 'end': str(end), 'valid_focused_candidates': counts[0],
+                       'group_survivors': counts[1:ng + 1], 'local_survivors': number, 'output_sha256': sha})
     verification_seconds = time.perf_counter() - verification_started
-    groups = [sum(col) for col in zip(*(counts[1:ng + 1] for counts in table))] if table else [0] * ng
-    return dict(status='PASS', host_verified=True, focus_u=sum(b - a for a, b, _, _ in bounds), focus_v=int(nv),
-                cartesian_opportunities=sum((b - a) * (d - c) for a, b, c, d in bounds),
-                valid_focused_candidates=sum(counts[0] for counts in table),
+    groups = counters[:, 1:ng + 1].sum(axis=0, dtype=np.uint64).tolist() if table else [0] * ng
+    focus_u = opportunities = 0
+    for a, b, c, d in bounds:
+        focus_u += b - a
+        opportunities += (b - a) * (d - c)
+    return dict(status='PASS', host_verified=True, focus_u=focus_u, focus_v=int(nv),
+                cartesian_opportunities=opportunities,
+                valid_focused_candidates=int(counters[:, 0].sum(dtype=np.uint64)) if table else 0,
                 genuine_cubes=len(all_values) - len(hits), local_survivors=len(all_values), cube_checks=len(all_values),
                 kernel_seconds=timings.get('kernel_seconds', 0.0), transfer_seconds=transfer_seconds,
                 verification_seconds=verification_seconds,
@@ -224,18 +230,20 @@ class FusedMatcher:
 class MaskedMatcher(FusedMatcher):
     """Residue-bitset engine (`pseudocube/batch_experiment.py:170-254`).
 
-    The first four residual filters are intersected as 64-bit residue
-    bitsets before any 128-bit candidate is built.
+    The first residual filters (up to eight whose classes fit one LDS chunk;
+    193..277 at p = 619) are intersected as 64-bit residue bitsets staged in
+    LDS before any 128-bit candidate is built; group counts are those of the
+    reference's four-filter bitset form.
     """
     engine_id = 0
     layout = 'residue-bitset-batch'
-    kernel_name = 'match'
+    kernel_name = 'match_staged'
 
     def __init__(self, runtime: Runtime, config: Config) -> None:
         super().__init__(runtime, config)
         if not self.engine.order:
             raise ValueError('bitset experiment requires at least one residual filter')
-        runtime.compile('mask_build')
+        runtime.compile('mask_staged')
 
     def run(self, batch: Prepared, output_cap: int = 8192) -> dict[str, Any]:
         result = super().run(batch, output_cap)

@@ -11,7 +11,8 @@ import subprocess
 
 HERE = Path(__file__).resolve().parent
 SOURCES = [HERE / 'csrc' / 'pcube.hip']
-HEADERS = [HERE / 'csrc' / 'pcube_device.h', HERE / 'csrc' / 'pcube_kernels.h', HERE.parent / 'include' / 'pcube.h']
+HEADERS = [HERE / 'csrc' / name for name in ('pcube_device.h', 'pcube_kernels.h', 'pcube_wheel.h', 'pcube_staged.h')] + [
+    HERE.parent / 'include' / 'pcube.h']
 LIBRARY = HERE / 'libpcube.so'
 ARCH = os.environ.get('PCUBE_OFFLOAD_ARCH', 'gfx950')
 

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <numeric>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -21,6 +22,7 @@
 #include "../../include/pcube.h"
 #include "pcube_device.h"
 #include "pcube_wheel.h"
+#include "pcube_staged.h"
 
 using namespace pcube;
 
@@ -60,6 +62,7 @@ struct Context {
     std::unordered_map<void*, uint64_t> live;      // address -> class size
     std::multimap<uint64_t, void*> cached;          // class size -> free block
     int compute_units = 0;
+    uint32_t lds_per_block = 65536;                  // largest LDS one workgroup may use
     char name[256] = {0};
 } g;
 
@@ -611,13 +614,22 @@ struct Matcher {
     uint32_t nres = 0, nbits = 0, ncache = 0, ng = 0, allow_words = 0;
     uint32_t row_base[kMaxBitsetFilters] = {0, 0, 0, 0};
     uint32_t mask_rows = 0;
+    // staged engine: bitset filters, their rows (sum of q), cached chain filters, LDS bytes
+    uint32_t sbits = 0, srows = 0, sncache = 0, slds = 0;
+    uint32_t s_row_base[kStagedFilters] = {}, s_chunk_base[kStagedFilters + 1] = {};
+    uint32_t s_pstr_off[kStagedFilters] = {}, s_nbeta[kStagedFilters] = {};
     std::vector<Modulus> mods;
-    Buffer d_mods, d_allow;
-    Buffer rows, masks, rho, counters, list, list_count, error;
+    Buffer d_mods, d_allow, d_pstr;
+    // status words: [0] survivor list length, [1] error bits, [2] unused, [3] most bitset
+    // survivors of any staged work item that overflowed its region since the last clear
+    Buffer rows, masks, rho, counters, list, status, cand, items;
+    uint64_t cand_cap = 512;                            // bitset-survivor entries per staged work item
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};   // start, bitsets built, match done
+    uint64_t* list_count() const { return status.as<uint64_t>(); }
+    uint32_t* error() const { return (uint32_t*)(status.as<uint64_t>() + 1); }
     ~Matcher() {
-        d_mods.release(); d_allow.release(); rows.release(); masks.release(); rho.release();
-        counters.release(); list.release(); list_count.release(); error.release();
+        d_mods.release(); d_allow.release(); d_pstr.release(); rows.release(); masks.release(); rho.release();
+        counters.release(); list.release(); status.release(); cand.release(); items.release();
         for (hipEvent_t& e : ev)
             if (e) (void)hipEventDestroy(e);
     }
@@ -641,6 +653,74 @@ bool allowed_residue(uint32_t q, uint32_t a) {
     return cubic_residue(q, a);
 }
 
+// Staged engine for one batch (engine 0 when the matcher has staged filters):
+// rows with per-subtile valid counts, staged bitsets, cached residues, then
+// one block per (stage chunk, row piece).
+int staged_enqueue(Matcher& mt, MatchArgs m, hipEvent_t ev_pre, hipEvent_t ev_match) {
+    m.staged = 1;
+    m.nbits = mt.sbits;
+    m.ncache = mt.sncache;
+    for (int k = 0; k < kStagedFilters; ++k) {
+        m.row_base[k] = mt.s_row_base[k];
+        m.nbeta[k] = mt.s_nbeta[k];
+        m.pstr_off[k] = mt.s_pstr_off[k];
+    }
+    m.pstr = mt.d_pstr.as<uint32_t>();
+    m.stage_rows = mt.srows;
+    m.nstage = (m.words + kStageWords - 1) / kStageWords;
+    TRY(mt.masks.ensure(std::max<uint64_t>(8, (uint64_t)m.nstage * mt.srows * kStageWords * 8)));
+    TRY(mt.rho.ensure(std::max<uint64_t>(16, (uint64_t)m.nu * kCachedResidues * 2)));
+    m.masks = mt.masks.as<uint64_t>();
+    m.rho = mt.rho.as<uint16_t>();
+    m.cand_max = mt.status.as<uint64_t>() + 3;
+    m.item_cap = (uint32_t)mt.cand_cap;
+    hipLaunchKernelGGL(k_rows_staged, dim3(blocks_for(m.nv, 256)), dim3(256), 0, g.stream, m);
+    if (m.nu) {
+        const uint32_t chunks = mt.s_chunk_base[mt.sbits];
+        const uint64_t waves = (uint64_t)chunks * m.nstage;
+        const uint32_t* cb = mt.s_chunk_base;
+        hipLaunchKernelGGL(k_masks_staged, dim3(blocks_for(waves * kWave, 256)), dim3(256), 0, g.stream, m, chunks,
+                           make_uint4(cb[0], cb[1], cb[2], cb[3]), make_uint4(cb[4], cb[5], cb[6], cb[7]));
+        if (m.ncache)
+            hipLaunchKernelGGL(k_rho, dim3(blocks_for((uint64_t)m.nu * kCachedResidues, 256)), dim3(256), 0, g.stream, m);
+    }
+    if (ev_pre) HIP_TRY(hipEventRecord(ev_pre, g.stream));
+    if (m.nu) {
+        // work list of (piece, chunk) items, then one persistent block per CU over it
+        const uint32_t npieces = (m.nv + kPieceRows - 1) / kPieceRows;
+        const uint32_t items_cap = npieces * m.nstage;   // staged_fits bounds it
+        TRY(mt.items.ensure(((uint64_t)items_cap + 1) * sizeof(uint2) + (uint64_t)items_cap * 4));
+        uint2* items = mt.items.as<uint2>();
+        uint32_t* nitems = (uint32_t*)(items + items_cap);
+        m.item_count = nitems + 2;
+        TRY(mt.cand.ensure((uint64_t)items_cap * mt.cand_cap * sizeof(uint2)));
+        m.cand = mt.cand.as<uint2>();
+        hipLaunchKernelGGL(k_plan_staged, dim3(1), dim3(kStagedThreads), 0, g.stream, m, npieces, items, items_cap,
+                           nitems);
+        hipLaunchKernelGGL(k_match_staged, dim3((uint32_t)g.compute_units), dim3(kStagedThreads), mt.slds, g.stream, m,
+                           (const uint2*)items, items_cap, (const uint32_t*)nitems);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(k_chain_staged, dim3(2 * (uint32_t)g.compute_units), dim3(kMatchThreads), 0, g.stream, m,
+                           (const uint32_t*)nitems, items_cap);
+    }
+    if (ev_match) HIP_TRY(hipEventRecord(ev_match, g.stream));
+    return PCUBE_OK;
+}
+
+// Whether one batch of this geometry runs on the staged engine.
+bool staged_fits(const Matcher& mt, uint64_t nu, uint64_t nv) {
+    if (!mt.sbits || mt.slds > g.lds_per_block) return false;
+    const uint64_t words = (nu + 63) / 64, nstage = (words + kStageWords - 1) / kStageWords;
+    const uint64_t npieces = (nv + kPieceRows - 1) / kPieceRows;
+    return npieces * nstage <= (1ull << 22) && (nu + 64) * kCachedResidues * 2 < (1ull << 40);
+}
+
+// Internal engine code: the residue-bitset engine without LDS staging.
+constexpr int kUnstagedBitsets = 2;
+// Bitset survivors one staged work item may hold before the unstaged engine is
+// used instead (an item covers 2048 v rows x 256 u coordinates).
+constexpr uint64_t kMaxItemCandidates = 1ull << 16;
+
 // Enqueue the kernels of one batch; counters must be zeroed by the caller.
 int match_enqueue(Matcher& mt, const uint64_t* us, uint64_t nu, const uint64_t* vs, uint64_t nv, uint64_t v_first,
                   uint64_t tile_width, uint32_t ntiles, uint64_t cap, int engine, uint64_t* counters,
@@ -662,8 +742,8 @@ int match_enqueue(Matcher& mt, const uint64_t* us, uint64_t nu, const uint64_t* 
     m.allow = mt.d_allow.as<uint32_t>();
     m.counters = counters;
     m.list = mt.list.as<uint64_t>();
-    m.list_count = mt.list_count.as<uint64_t>();
-    m.error = mt.error.as<uint32_t>();
+    m.list_count = mt.list_count();
+    m.error = mt.error();
     m.A = mt.A; m.B = mt.B; m.v_first = v_first; m.tile_width = tile_width; m.cap = cap;
     m.list_cap = list_cap; m.tile_base = tile_base;
     m.lo = mt.lo; m.hi = mt.hi;
@@ -672,6 +752,8 @@ int match_enqueue(Matcher& mt, const uint64_t* us, uint64_t nu, const uint64_t* 
     m.ng = mt.ng; m.ntiles = ntiles; m.allow_words = mt.allow_words;
     for (int k = 0; k < kMaxBitsetFilters; ++k) m.row_base[k] = mt.row_base[k];
     if (nv == 0) return PCUBE_OK;
+    if (engine == 0 && staged_fits(mt, nu, nv)) return staged_enqueue(mt, m, ev_pre, ev_match);
+    if (engine == kUnstagedBitsets) engine = 0;
     // k_rows always computes v mod q for the bitset filters of the matcher
     MatchArgs mr = m;
     mr.nbits = mt.nbits;
@@ -868,6 +950,8 @@ const Kernel kKernels[] = {
     {"scan_local", (const void*)k_scan_local},
     {"rows", (const void*)k_rows},
     {"mask_build", (const void*)k_masks},
+    {"mask_staged", (const void*)k_masks_staged},
+    {"match_staged", (const void*)k_match_staged},
     {"residues", (const void*)k_rho},
     {"match", (const void*)k_match},
     {"match_direct", (const void*)k_match_direct},
@@ -912,6 +996,15 @@ int pcube_init(int device, uint64_t memory_cap, double fraction, uint64_t* cap, 
         HIP_TRY(hipGetDeviceProperties(&prop, device));
         g.compute_units = prop.multiProcessorCount;
         snprintf(g.name, sizeof g.name, "%s (%s)", prop.name, prop.gcnArchName);
+        int lds = 0;
+        HIP_TRY(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+        g.lds_per_block = (uint32_t)std::max(lds, 0);
+        // the staged engine uses more than 64 KB of dynamic LDS when the device allows it
+        if (hipFuncSetAttribute((const void*)k_match_staged, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)g.lds_per_block) != hipSuccess) {
+            (void)hipGetLastError();
+            g.lds_per_block = std::min<uint32_t>(g.lds_per_block, 65536);
+        }
         g.device = device;
     }
     size_t fr = 0, tot = 0;
@@ -1261,6 +1354,7 @@ int pcube_matcher_create(uint64_t A, uint64_t B, uint64_t lo0, uint64_t lo1, uin
         bit += q;
         mt->mods.push_back(m);
     }
+    allow.resize((allow.size() + 3) / 4 * 4, 0);   // whole 16-byte units for staged copies
     if (allow.size() > (size_t)kMaxAllowWords) return fail(PCUBE_INVALID, "allowed-residue table too large");
     mt->allow_words = (uint32_t)allow.size();
     uint32_t rb = 0;
@@ -1269,10 +1363,38 @@ int pcube_matcher_create(uint64_t A, uint64_t B, uint64_t lo0, uint64_t lo1, uin
         if (k < mt->nbits) rb += mt->mods[k].q;
     }
     mt->mask_rows = rb;
+    // staged engine: the longest prefix of at most kStagedFilters filters whose
+    // rows fit one LDS chunk and whose A residue is invertible; filter k keeps
+    // P[j] = allowed_q[(-j*A) mod q] for j < q + 64, plus two words of margin
+    std::vector<uint32_t> pstr;
+    for (uint32_t k = 0; k < std::min<uint32_t>(nres, kStagedFilters); ++k) {
+        const Modulus& md = mt->mods[k];
+        if (mt->srows + md.q > (uint32_t)kMaxStageRows || md.a == 0 || std::gcd(md.a, md.q) != 1) break;
+        const uint64_t ainv = modinv(md.a, md.q);
+        mt->s_nbeta[k] = (uint32_t)((md.q - (uint64_t)md.b * ainv % md.q) % md.q);
+        mt->s_row_base[k] = mt->srows;
+        mt->s_pstr_off[k] = (uint32_t)pstr.size();
+        const uint32_t nbits = md.q + kWave, nwords = (nbits + 31) / 32 + 2;
+        const size_t o = pstr.size();
+        pstr.resize(o + nwords, 0);
+        for (uint32_t j = 0; j < nbits; ++j) {
+            const uint32_t res = (uint32_t)((md.q - (uint64_t)(j % md.q) * md.a % md.q) % md.q);
+            if (allowed_residue(md.q, res)) pstr[o + j / 32] |= 1u << (j % 32);
+        }
+        mt->s_chunk_base[k + 1] = mt->s_chunk_base[k] + (md.q + kWave - 1) / kWave;
+        mt->srows += md.q;
+        ++mt->sbits;
+    }
+    for (uint32_t k = mt->sbits; k < (uint32_t)kStagedFilters; ++k) mt->s_chunk_base[k + 1] = mt->s_chunk_base[k];
+    mt->sncache = std::min<uint32_t>(nres - mt->sbits, kCachedResidues);
+    mt->slds = staged_lds_bytes(mt->srows, mt->allow_words, nres, mt->ng);
+    if (mt->sbits) {
+        TRY(mt->d_pstr.ensure(pstr.size() * 4));
+        HIP_TRY(hipMemcpyAsync(mt->d_pstr.p, pstr.data(), pstr.size() * 4, hipMemcpyHostToDevice, g.stream));
+    }
     TRY(mt->d_mods.ensure(nres * sizeof(Modulus)));
     TRY(mt->d_allow.ensure(allow.size() * 4));
-    TRY(mt->list_count.ensure(8));
-    TRY(mt->error.ensure(4));
+    TRY(mt->status.ensure(4 * 8));
     HIP_TRY(hipMemcpyAsync(mt->d_mods.p, mt->mods.data(), nres * sizeof(Modulus), hipMemcpyHostToDevice, g.stream));
     HIP_TRY(hipMemcpyAsync(mt->d_allow.p, allow.data(), allow.size() * 4, hipMemcpyHostToDevice, g.stream));
     HIP_TRY(hipStreamSynchronize(g.stream));
@@ -1301,31 +1423,35 @@ int pcube_match(int matcher, uint64_t us, uint64_t nu, uint64_t vs, uint64_t nv,
     const uint64_t list_cap = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)ntiles * cap, 1ull << 22));
     TRY(mt->counters.ensure(nc * 8));
     TRY(mt->list.ensure(list_cap * 24));
-    TRY(fill64(mt->counters.as<uint64_t>(), nc, 0));
-    TRY(fill64(mt->list_count.as<uint64_t>(), 1, 0));
-    TRY(fill32(mt->error.as<uint32_t>(), 1, 0));
     for (hipEvent_t& e : mt->ev)
         if (!e) HIP_TRY(hipEventCreate(&e));
     hipEvent_t e0 = mt->ev[0], e1 = mt->ev[1], e2 = mt->ev[2];
-    HIP_TRY(hipEventRecord(e0, g.stream));
-    int rc = match_enqueue(*mt, (const uint64_t*)us, nu, (const uint64_t*)vs, nv, v_first, tile_width, ntiles, cap,
-                           engine, mt->counters.as<uint64_t>(), 0, list_cap, e1, e2);
-    if (rc) return rc;
-    if (nv == 0) {
-        HIP_TRY(hipEventRecord(e1, g.stream));
-        HIP_TRY(hipEventRecord(e2, g.stream));
-    }
-    // one pinned download: [survivor count, error word, counters]
-    TRY(g_down_host.ensure((nc + 2) * 8));
+    // one pinned download: [status words, counters]; a bitset-survivor list that
+    // overflowed is sized from its count and the batch runs again
+    TRY(g_down_host.ensure((nc + 4) * 8));
     uint64_t* hd = g_down_host.as<uint64_t>();
-    hd[1] = 0;
-    HIP_TRY(hipMemcpyAsync(hd, mt->list_count.p, 8, hipMemcpyDeviceToHost, g.stream));
-    HIP_TRY(hipMemcpyAsync(hd + 1, mt->error.p, 4, hipMemcpyDeviceToHost, g.stream));
-    HIP_TRY(hipMemcpyAsync(hd + 2, mt->counters.p, nc * 8, hipMemcpyDeviceToHost, g.stream));
-    HIP_TRY(hipStreamSynchronize(g.stream));
+    int rc = PCUBE_OK;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        TRY(fill64(mt->counters.as<uint64_t>(), nc, 0));
+        TRY(fill64(mt->status.as<uint64_t>(), 4, 0));
+        HIP_TRY(hipEventRecord(e0, g.stream));
+        rc = match_enqueue(*mt, (const uint64_t*)us, nu, (const uint64_t*)vs, nv, v_first, tile_width, ntiles, cap,
+                           engine, mt->counters.as<uint64_t>(), 0, list_cap, e1, e2);
+        if (rc) return rc;
+        if (nv == 0) {
+            HIP_TRY(hipEventRecord(e1, g.stream));
+            HIP_TRY(hipEventRecord(e2, g.stream));
+        }
+        HIP_TRY(hipMemcpyAsync(hd, mt->status.p, 4 * 8, hipMemcpyDeviceToHost, g.stream));
+        HIP_TRY(hipMemcpyAsync(hd + 4, mt->counters.p, nc * 8, hipMemcpyDeviceToHost, g.stream));
+        HIP_TRY(hipStreamSynchronize(g.stream));
+        if (hd[3] <= mt->cand_cap) break;
+        if (hd[3] > kMaxItemCandidates) engine = kUnstagedBitsets;
+        else mt->cand_cap = (hd[3] + 63) / 64 * 64;
+    }
     const uint64_t listed = hd[0];
     const uint32_t err = (uint32_t)hd[1];
-    std::vector<uint64_t> host(hd + 2, hd + 2 + nc);
+    std::vector<uint64_t> host(hd + 4, hd + 4 + nc);
     if (seconds) {
         float a = 0, b = 0;
         HIP_TRY(hipEventElapsedTime(&a, e0, e1));
@@ -1402,42 +1528,50 @@ int pcube_lease(int matcher, int focus_u, int focus_v, uint32_t nbatches, const 
     const uint64_t list_cap = std::max<uint64_t>(1, std::min<uint64_t>(ntot * cap, 1ull << 22));
     TRY(mt->counters.ensure(ntot * stride * 8));
     TRY(mt->list.ensure(list_cap * 24));
-    TRY(fill64(mt->counters.as<uint64_t>(), ntot * stride, 0));
-    TRY(fill64(mt->list_count.as<uint64_t>(), 1, 0));
-    TRY(fill32(mt->error.as<uint32_t>(), 1, 0));
-    uint64_t t0 = 0;
-    int geometry_rc = PCUBE_OK;
-    for (uint32_t b = 0; b < nbatches; ++b) {
-        const uint64_t nt = batch_tiles[b], t1 = t0 + nt;
-        const uint64_t ua = up[2 * t0], ub = up[2 * (t1 - 1) + 1];
-        const uint64_t va = vp[2 * t0], vb = vp[2 * (t1 - 1) + 1];
-        batch_sizes[2 * b] = ub - ua;
-        batch_sizes[2 * b + 1] = vb - va;
-        for (uint64_t t = t0; t < t1; ++t) {
-            bounds[4 * t + 0] = up[2 * t] - ua;
-            bounds[4 * t + 1] = up[2 * t + 1] - ua;
-            bounds[4 * t + 2] = vp[2 * t] - va;
-            bounds[4 * t + 3] = vp[2 * t + 1] - va;
-            const uint64_t du = bounds[4 * t + 1] - bounds[4 * t], dv = bounds[4 * t + 3] - bounds[4 * t + 2];
-            if (du * dv > 50000000ull && geometry_rc == PCUBE_OK)
-                geometry_rc = fail(PCUBE_OVERFLOW, "per-subtile candidate opportunity limit exceeded");
-        }
-        if (geometry_rc == PCUBE_OK) {
-            TRY(match_enqueue(*mt, lus.as<uint64_t>() + ua, ub - ua, lvs.as<uint64_t>() + va, vb - va,
-                              tiles[4 * t0], tile_width, (uint32_t)nt, cap, 0,
-                              mt->counters.as<uint64_t>() + t0 * stride, t0, list_cap, nullptr, nullptr));
-        }
-        t0 = t1;
-    }
-    HIP_TRY(hipEventRecord(ev[2], g.stream));
+    // every batch of the lease, then one download; a bitset-survivor list that
+    // overflowed in any batch is sized from the largest count and the lease
+    // runs again
     std::vector<uint64_t> host(ntot * stride);
-    uint32_t err = 0;
-    uint64_t listed = 0;
-    HIP_TRY(hipMemcpyAsync(host.data(), mt->counters.p, ntot * stride * 8, hipMemcpyDeviceToHost, g.stream));
-    HIP_TRY(hipMemcpyAsync(&err, mt->error.p, 4, hipMemcpyDeviceToHost, g.stream));
-    HIP_TRY(hipMemcpyAsync(&listed, mt->list_count.p, 8, hipMemcpyDeviceToHost, g.stream));
-    HIP_TRY(hipEventRecord(ev[3], g.stream));
-    HIP_TRY(hipStreamSynchronize(g.stream));
+    uint64_t st[4] = {0, 0, 0, 0};
+    int geometry_rc = PCUBE_OK;
+    int engine = 0;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        TRY(fill64(mt->counters.as<uint64_t>(), ntot * stride, 0));
+        TRY(fill64(mt->status.as<uint64_t>(), 4, 0));
+        uint64_t t0 = 0;
+        for (uint32_t b = 0; b < nbatches; ++b) {
+            const uint64_t nt = batch_tiles[b], t1 = t0 + nt;
+            const uint64_t ua = up[2 * t0], ub = up[2 * (t1 - 1) + 1];
+            const uint64_t va = vp[2 * t0], vb = vp[2 * (t1 - 1) + 1];
+            batch_sizes[2 * b] = ub - ua;
+            batch_sizes[2 * b + 1] = vb - va;
+            for (uint64_t t = t0; t < t1; ++t) {
+                bounds[4 * t + 0] = up[2 * t] - ua;
+                bounds[4 * t + 1] = up[2 * t + 1] - ua;
+                bounds[4 * t + 2] = vp[2 * t] - va;
+                bounds[4 * t + 3] = vp[2 * t + 1] - va;
+                const uint64_t du = bounds[4 * t + 1] - bounds[4 * t], dv = bounds[4 * t + 3] - bounds[4 * t + 2];
+                if (du * dv > 50000000ull && geometry_rc == PCUBE_OK)
+                    geometry_rc = fail(PCUBE_OVERFLOW, "per-subtile candidate opportunity limit exceeded");
+            }
+            if (geometry_rc == PCUBE_OK) {
+                TRY(match_enqueue(*mt, lus.as<uint64_t>() + ua, ub - ua, lvs.as<uint64_t>() + va, vb - va,
+                                  tiles[4 * t0], tile_width, (uint32_t)nt, cap, engine,
+                                  mt->counters.as<uint64_t>() + t0 * stride, t0, list_cap, nullptr, nullptr));
+            }
+            t0 = t1;
+        }
+        HIP_TRY(hipEventRecord(ev[2], g.stream));
+        HIP_TRY(hipMemcpyAsync(host.data(), mt->counters.p, ntot * stride * 8, hipMemcpyDeviceToHost, g.stream));
+        HIP_TRY(hipMemcpyAsync(st, mt->status.p, 4 * 8, hipMemcpyDeviceToHost, g.stream));
+        HIP_TRY(hipEventRecord(ev[3], g.stream));
+        HIP_TRY(hipStreamSynchronize(g.stream));
+        if (geometry_rc || st[3] <= mt->cand_cap) break;
+        if (st[3] > kMaxItemCandidates) engine = kUnstagedBitsets;
+        else mt->cand_cap = (st[3] + 63) / 64 * 64;
+    }
+    const uint32_t err = (uint32_t)st[1];
+    const uint64_t listed = st[0];
     memcpy(counters, host.data(), ntot * stride * 8);
     if (seconds) {
         float a = 0, m = 0;

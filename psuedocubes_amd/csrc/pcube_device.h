@@ -278,13 +278,17 @@ __global__ void k_searchsorted(const uint64_t* __restrict__ sorted, uint64_t n, 
 // --------------------------------------------------------------------------
 // Matching
 
-// Per-v row data computed once per batch.
+// Per-v row data computed once per batch.  The first 32 bytes are what the
+// bitset stage reads; sigma and v serve the residual chain.
 struct Row {
     uint32_t left, right;   // u index range [left, right) with lo < u*B - v*A <= hi
     uint32_t tile, flags;   // subtile index; bit0: a v outside [v_first, v_first + ntiles*width)
-    uint16_t r[kMaxBitsetFilters];  // v mod q for the bitset filters
+    uint16_t r[kStagedFilters];       // v mod q for the bitset filters
+    uint16_t sigma[kCachedResidues];  // (-v*A) mod q for the cached chain filters (staged engine)
     uint64_t v;
+    uint64_t pad;
 };
+static_assert(sizeof(Row) == 64, "Row is one 64-byte record");
 
 struct MatchArgs {
     const uint64_t* us;
@@ -301,7 +305,16 @@ struct MatchArgs {
     uint64_t A, B, v_first, tile_width, cap, list_cap, tile_base;
     U128 lo, hi;
     uint32_t nu, nv, words, nres, nbits, ncache, ng, ntiles, allow_words, rows_per_wave;
-    uint32_t row_base[kMaxBitsetFilters];
+    uint32_t row_base[kStagedFilters];
+    // staged engine: masks in chunks of kStageWords words, [chunk][stage_rows][kStageWords]
+    const uint32_t* pstr;      // per bitset filter, its permuted allowed string, from pstr_off[k]
+    uint32_t staged, stage_rows, nstage;
+    uint32_t nbeta[kStagedFilters];    // (-B * A^-1) mod q per bitset filter
+    uint32_t pstr_off[kStagedFilters];
+    uint2* cand;               // bitset survivors (row, u index), item_cap entries per work item
+    uint32_t* item_count;      // survivors of each work item (entries beyond item_cap are dropped)
+    uint64_t* cand_max;        // largest item_count above item_cap seen since the last clear
+    uint32_t item_cap;
 };
 
 // Exact u bounds of row j by 128-bit binary search, subtile, v mod q.
@@ -337,14 +350,52 @@ __device__ __forceinline__ void build_row(const MatchArgs& m, uint32_t j) {
     // (impossible for sorted input) is reported as the underflow counter.
     if (row.left < row.right && less_wide(mul_wide(m.us[row.left], m.B), shift))
         atomicAdd(&m.counters[(uint64_t)row.tile * (m.ng + 3) + m.ng + 2], 1ull);
-    for (uint32_t k = 0; k < kMaxBitsetFilters; ++k)
+    for (uint32_t k = 0; k < kStagedFilters; ++k)
         row.r[k] = k < m.nbits ? uint16_t(mod64(v, m.mods[k])) : 0;
+    for (uint32_t c = 0; c < kCachedResidues; ++c) {
+        uint16_t s = 0;
+        if (m.staged && c < m.ncache) {
+            const Modulus mod = m.mods[m.nbits + c];
+            const uint32_t r = mod32(mod64(v, mod) * mod.a, mod);
+            s = uint16_t(r ? mod.q - r : 0);
+        }
+        row.sigma[c] = s;
+    }
+    row.pad = 0;
     ((Row*)m.rows)[j] = row;
 }
 
 __global__ void k_rows(MatchArgs m) {
     uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j < m.nv) build_row(m, j);
+}
+
+// Staged engine rows: as k_rows, plus each subtile's valid-position count,
+// sum of right - left over its rows.  Rows are sorted by v, so a wave's rows
+// span few subtiles: one atomic per (wave, subtile).
+__global__ void k_rows_staged(MatchArgs m) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t tile = 0xffffffffu;
+    uint64_t count = 0;
+    if (j < m.nv) {
+        build_row(m, j);
+        const Row& row = m.rows[j];
+        if (!row.flags) {
+            tile = row.tile;
+            count = row.right - row.left;
+        }
+    }
+    uint64_t pending = __ballot(tile != 0xffffffffu);
+    while (pending) {
+        const uint32_t first = __builtin_ctzll(pending);
+        const uint32_t t = __shfl(tile, first, kWave);
+        const bool mine = tile == t;
+        uint64_t part = mine ? count : 0;
+        for (int d = kWave / 2; d > 0; d >>= 1) part += __shfl_xor(part, d, kWave);
+        if ((threadIdx.x & (kWave - 1)) == first && part)
+            atomicAdd(&m.counters[(uint64_t)t * (m.ng + 3)], part);
+        pending &= ~__ballot(mine);
+    }
 }
 
 // One 64-bit word of the residue bitsets: wave builds 64 rows (residues

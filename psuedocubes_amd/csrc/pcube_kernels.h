@@ -11,6 +11,7 @@ namespace pcube {
 constexpr int kWave = 64;
 constexpr int kMaxResiduals = 128;   // residual filters per configuration
 constexpr int kMaxBitsetFilters = 4; // filters folded into the residue bitsets
+constexpr int kStagedFilters = 8;    // filters folded into the LDS-staged residue bitsets
 constexpr int kCachedResidues = 8;   // chain residues precomputed per u coordinate
 constexpr int kMaxAllowWords = 2048; // 32-bit words of allowed-residue bits
 

@@ -1,0 +1,441 @@
+#pragma once
+// LDS-staged residue-bitset engine for one batch (gfx950).
+//
+// The first kStagedFilters residual filters (193, 199, 211, 223, 229, 241,
+// 271, 277 at p = 619) are folded into residue bitsets: for filter q and
+// residue class r = v mod q, bit t of mask word w is allowed_q[(u*B - r*A)
+// mod q] for the u at index 64w + t.  Masks are laid out in chunks of
+// kStageWords words, [chunk][sum of q rows][kStageWords], so one chunk of every
+// filter is one contiguous block of stage_rows * 32 bytes.
+//
+// Work item = (stage chunk s, piece of kPieceRows consecutive v rows).  A
+// block copies chunk s into LDS once and intersects it with every row of its
+// piece: a lane takes whole rows and ANDs, for each of the chunk's words, the
+// kStagedFilters words of its row's residue classes.  Positions surviving all bitset filters (about 1 in 6561
+// at p = 619) go through the rest of the filter chain exactly as in the
+// residue-bitset engine: cached residues (u*B mod q) + sigma_v for the next
+// kCachedResidues filters, then exact 128-bit x = u*B - v*A reduced limb by limb.
+//
+// Counts follow the reference's group structure (filters in groups of four,
+// pseudocube/batch_experiment.py:198-235): group 1 and group 2 are popcounts
+// of the bitset intersections after filters 3 and 7, later groups are counted
+// in the chain; valid positions per subtile are summed per row by
+// k_rows_staged.  Counts and the emitted set equal the residue-bitset
+// engine's.
+#include "pcube_device.h"
+
+namespace pcube {
+
+constexpr int kStageWords = 4;        // mask words (256 u coordinates) per stage chunk
+constexpr int kPieceRows = 2048;      // v rows per work item
+constexpr int kStagedThreads = 1024;  // one persistent block per CU
+constexpr int kStagedWaves = kStagedThreads / kWave;
+constexpr int kRowsPerLane = kPieceRows / kStagedThreads;   // rows of a piece each thread takes
+constexpr int kTileSlots = 64;        // subtiles whose group counts a block keeps in LDS
+constexpr int kMaxStageRows = 2048;   // bitset rows (sum of q) one chunk may hold
+
+// A staged row is two 16-byte halves (words 0-1, words 2-3).  Rows whose bit 3
+// is set store them swapped, so that rows 8 apart start in different LDS bank
+// quads and a wave reading 64 rows spreads over all 16 quads.
+__host__ __device__ __forceinline__ uint32_t stage_swizzle(uint32_t row) { return (row >> 3) & 1u; }
+
+// One butterfly stage of a 64 x 64 bit transpose across a wave.
+template <int kB, uint64_t kMask>
+__device__ __forceinline__ uint64_t transpose_step(uint64_t x, uint32_t lane) {
+    const uint64_t y = __shfl_xor(x, kB, kWave);
+    return (lane & kB) ? ((x & kMask) | ((y & kMask) >> kB)) : ((x & ~kMask) | ((y & ~kMask) << kB));
+}
+
+// Lane t holds column t of a 64 x 64 bit matrix (bit k = element (k, t));
+// returns row `lane` (bit t = element (lane, t)).
+__device__ __forceinline__ uint64_t wave_transpose(uint64_t x, uint32_t lane) {
+    x = transpose_step<32, 0xffffffff00000000ull>(x, lane);
+    x = transpose_step<16, 0xffff0000ffff0000ull>(x, lane);
+    x = transpose_step<8, 0xff00ff00ff00ff00ull>(x, lane);
+    x = transpose_step<4, 0xf0f0f0f0f0f0f0f0ull>(x, lane);
+    x = transpose_step<2, 0xccccccccccccccccull>(x, lane);
+    x = transpose_step<1, 0xaaaaaaaaaaaaaaaaull>(x, lane);
+    return x;
+}
+
+// Staged residue bitsets.  Wave (s, c) builds rows r0..r0+63 of one bitset
+// filter for the kStageWords words of stage chunk s.  Lane t takes the u at
+// index 64w + t.  With P[k] = allowed_q[(-k*A) mod q] (stored repeated, q + 64
+// bits) and k0 = (r0 + (u mod q) * nbeta) mod q, bit j of the 64-bit window
+// P[k0 .. k0 + 63] is allowed_q[(u*B - (r0 + j)*A) mod q]: the column of this
+// u over rows r0..r0+63.  A wave transpose turns the 64 columns into the 64
+// row words.  chunk_base[k] = first 64-row chunk of filter k.
+__global__ __launch_bounds__(256) void k_masks_staged(MatchArgs m, uint32_t chunks_total, uint4 base_lo, uint4 base_hi) {
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint64_t g = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / kWave;
+    if (g >= (uint64_t)chunks_total * m.nstage) return;
+    const uint32_t s = uint32_t(g / chunks_total), c = uint32_t(g % chunks_total);
+    const uint32_t base[kStagedFilters] = {base_lo.x, base_lo.y, base_lo.z, base_lo.w,
+                                           base_hi.x, base_hi.y, base_hi.z, base_hi.w};
+    uint32_t k = 0;
+    while (k + 1 < m.nbits && c >= base[k + 1]) ++k;
+    const Modulus mod = m.mods[k];
+    const uint32_t r0 = (c - base[k]) * kWave;
+    const uint32_t rows = mod.q - r0 < (uint32_t)kWave ? mod.q - r0 : (uint32_t)kWave;
+    const uint32_t* P = m.pstr + m.pstr_off[k];
+    uint64_t out[kStageWords];
+#pragma unroll
+    for (int ww = 0; ww < kStageWords; ++ww) {
+        const uint32_t i = (s * kStageWords + ww) * kWave + lane;
+        uint64_t col = 0;
+        if (i < m.nu) {
+            uint32_t k0 = mod32(mod64(m.us[i], mod) * m.nbeta[k], mod) + r0;
+            k0 = k0 >= mod.q ? k0 - mod.q : k0;
+            const uint32_t w0 = k0 >> 5, sh = k0 & 31;
+            const uint64_t low = ((uint64_t)P[w0 + 1] << 32) | P[w0];
+            col = (low >> sh) | (sh ? (uint64_t)P[w0 + 2] << (64 - sh) : 0ull);
+        }
+        out[ww] = wave_transpose(col, lane);
+    }
+    if (lane < rows) {
+        const uint32_t row = m.row_base[k] + r0 + lane;
+        const uint32_t swap = stage_swizzle(row);
+        uint4* dst = (uint4*)((uint64_t*)m.masks + ((uint64_t)s * m.stage_rows + row) * kStageWords);
+        dst[swap] = make_uint4(uint32_t(out[0]), uint32_t(out[0] >> 32), uint32_t(out[1]), uint32_t(out[1] >> 32));
+        dst[swap ^ 1] = make_uint4(uint32_t(out[2]), uint32_t(out[2] >> 32), uint32_t(out[3]), uint32_t(out[3] >> 32));
+    }
+}
+
+// Copy n 16-byte units from global memory to LDS with the whole block, up to
+// kBatch loads per thread in flight before any store.
+// Loads are unconditional (indices past n read element n - 1) so that the
+// batch stays in registers and all its loads are in flight together.
+template <int kThreads, int kBatch>
+__device__ __forceinline__ void stage_copy(uint4* __restrict__ dst, const uint4* __restrict__ src, uint32_t n) {
+    for (uint32_t first = 0; first < n; first += kThreads * kBatch) {
+        uint4 v[kBatch];
+#pragma unroll
+        for (int k = 0; k < kBatch; ++k) {
+            const uint32_t t = first + k * kThreads + threadIdx.x;
+            v[k] = src[t < n ? t : n - 1];
+        }
+#pragma unroll
+        for (int k = 0; k < kBatch; ++k) {
+            const uint32_t t = first + k * kThreads + threadIdx.x;
+            if (t < n) dst[t] = v[k];
+        }
+    }
+}
+
+// LDS of a staged block, carved from dynamic shared memory.
+struct Staged {
+    const uint64_t* masks;   // [stage_rows][kStageWords] of the block's chunk
+    const uint32_t* allow;
+    const Modulus* mods;
+    uint32_t* tilecnt;       // [kTileSlots][ng]: groups 1..ng of subtiles t0..t0+63
+    uint2* queue;            // this wave's (row, u index) survivors
+};
+
+__host__ __device__ inline uint32_t staged_align(uint32_t bytes) { return (bytes + 15u) & ~15u; }
+__host__ __device__ inline uint32_t staged_lds_bytes(uint32_t stage_rows, uint32_t allow_words, uint32_t nres,
+                                                     uint32_t ng) {
+    return staged_align(stage_rows * kStageWords * 8) + staged_align(allow_words * 4) +
+           staged_align(nres * (uint32_t)sizeof(Modulus)) + staged_align(kTileSlots * ng * 4) +
+           kStagedWaves * kQueue * 8 + 16;
+}
+
+// Add n to group g (1-based) of a subtile: LDS when the subtile is within the
+// block's window, else the global counters.
+__device__ __forceinline__ void staged_add(const MatchArgs& m, const Staged& S, uint32_t t0, uint32_t tile, uint32_t g,
+                                           uint32_t n) {
+    const uint32_t slot = tile - t0;
+    if (slot < (uint32_t)kTileSlots) atomicAdd(&S.tilecnt[slot * m.ng + g - 1], n);
+    else atomicAdd(&m.counters[(uint64_t)tile * (m.ng + 3) + g], (uint64_t)n);
+}
+
+// Run up to 64 bitset survivors (lane l takes entry e, live when l < count)
+// through the filters after the bitsets.
+__device__ __forceinline__ void staged_chain(const MatchArgs& m, const Staged& S, uint32_t lane, uint32_t count,
+                                             uint2 e, uint32_t t0) {
+    bool alive = lane < count;
+    if (!alive) e = make_uint2(0, 0);
+    const Row row = m.rows[e.x];
+    const uint32_t i = e.y;
+    uint32_t k = m.nbits;
+    uint4 packed = make_uint4(0, 0, 0, 0);
+    if (alive && m.ncache) packed = ((const uint4*)m.rho)[i];
+    const uint32_t words[4] = {packed.x, packed.y, packed.z, packed.w};
+#pragma unroll
+    for (int c = 0; c < kCachedResidues; ++c) {
+        if ((uint32_t)c >= m.ncache) break;
+        const Modulus mod = S.mods[k];
+        if (alive) {
+            uint32_t r = ((words[c >> 1] >> (16 * (c & 1))) & 0xffffu) + row.sigma[c];
+            r = r >= mod.q ? r - mod.q : r;
+            alive = allowed_bit(S.allow, mod.bit + r);
+        }
+        if (((k & 3) == 3 || k + 1 == m.nres) && alive) staged_add(m, S, t0, row.tile, k / 4 + 1, 1);
+        if (!__ballot(alive)) return;
+        ++k;
+    }
+    U128 x = {0, 0};
+    if (alive) x = sub_wide(mul_wide(m.us[i], m.B), mul_wide(row.v, m.A));
+    for (; k < m.nres; ++k) {
+        const Modulus mod = S.mods[k];
+        if (alive) alive = allowed_bit(S.allow, mod.bit + mod128(x, mod));
+        if (((k & 3) == 3 || k + 1 == m.nres) && alive) staged_add(m, S, t0, row.tile, k / 4 + 1, 1);
+        if (!__ballot(alive)) return;
+    }
+    if (alive) {
+        if (!less_wide(m.lo, x) || less_wide(m.hi, x)) atomicOr(m.error, 2u);
+        emit(m, row.tile, x);
+    }
+}
+
+// Append count (<= 64) queued survivors to the work item's region of the
+// candidate list; the item's fill counter lives in LDS.
+__device__ __forceinline__ void cand_flush(const Staged& S, uint32_t lane, uint32_t count, uint2* region,
+                                           uint32_t cap, uint32_t* fill) {
+    uint32_t at = 0;
+    if (lane == 0) at = atomicAdd(fill, count);
+    at = __shfl(at, 0, kWave);
+    if (lane < count && at + lane < cap) region[at + lane] = S.queue[lane];
+}
+
+// Queue the set bits of one mask word (u indices base + bit) with row j.
+__device__ __forceinline__ void staged_enqueue(const Staged& S, uint32_t lane, uint64_t b, uint32_t j, uint32_t base,
+                                               uint32_t& queued, uint2* region, uint32_t cap, uint32_t* fill) {
+    for (;;) {
+        const bool has = b != 0;
+        const uint64_t ballot = __ballot(has);
+        if (!ballot) return;
+        if (has) {
+            S.queue[queued + lanes_below(ballot)] = make_uint2(j, base + __builtin_ctzll(b));
+            b &= b - 1;
+        }
+        queued += __popcll(ballot);
+        if (queued >= kWave) {
+            wave_sync();
+            cand_flush(S, lane, kWave, region, cap, fill);
+            wave_sync();
+            const uint32_t rest = queued - kWave;
+            uint2 moved = make_uint2(0, 0);
+            if (lane < rest) moved = S.queue[kWave + lane];
+            wave_sync();
+            if (lane < rest) S.queue[lane] = moved;
+            queued = rest;
+            wave_sync();
+        }
+    }
+}
+
+// Chunks [c_lo, c_hi] that the rows of piece p reach (none when c_lo > c_hi).
+// Rows are sorted by v, so left and right are nondecreasing over a piece.
+__device__ __forceinline__ void piece_chunks(const MatchArgs& m, uint32_t p, uint32_t& c_lo, uint32_t& c_hi) {
+    const uint32_t j0 = p * kPieceRows, j1 = min(m.nv, j0 + kPieceRows);
+    const uint32_t left0 = m.rows[j0].left, right1 = m.rows[j1 - 1].right;
+    c_lo = 1;
+    c_hi = 0;
+    if (right1 > left0) {
+        c_lo = (left0 >> 6) / kStageWords;
+        c_hi = min(((right1 - 1) >> 6) / kStageWords, m.nstage - 1);
+    }
+}
+
+// Work list of the staged match: every (piece, chunk) pair whose rows reach
+// the chunk, piece-major, at most items_cap of them; nitems receives the
+// full count.  One block: thread t takes a run of consecutive pieces, a block
+// scan turns the runs' counts into offsets.
+__global__ __launch_bounds__(kStagedThreads) void k_plan_staged(MatchArgs m, uint32_t npieces, uint2* items,
+                                                                uint32_t items_cap, uint32_t* nitems) {
+    __shared__ uint32_t wave_total[kStagedWaves];
+    const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const uint32_t per = (npieces + kStagedThreads - 1) / kStagedThreads;
+    const uint32_t p0 = min(npieces, threadIdx.x * per), p1 = min(npieces, p0 + per);
+    uint32_t count = 0;
+    for (uint32_t p = p0; p < p1; ++p) {
+        uint32_t lo, hi;
+        piece_chunks(m, p, lo, hi);
+        if (lo <= hi) count += hi - lo + 1;
+    }
+    uint32_t inc = count;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const uint32_t y = __shfl_up(inc, d, kWave);
+        if (lane >= (uint32_t)d) inc += y;
+    }
+    if (lane == kWave - 1) wave_total[wave] = inc;
+    __syncthreads();
+    uint32_t at = inc - count;
+    for (uint32_t w = 0; w < wave; ++w) at += wave_total[w];
+    for (uint32_t p = p0; p < p1; ++p) {
+        uint32_t lo, hi;
+        piece_chunks(m, p, lo, hi);
+        for (uint32_t c = lo; c <= hi && lo <= hi; ++c, ++at)
+            if (at < items_cap) items[at] = make_uint2(p, c);
+    }
+    if (threadIdx.x == kStagedThreads - 1) *nitems = at;
+}
+
+// Persistent staged match: block b takes work items b, b + grid, ...  For an
+// item (piece p, chunk s) the block stages chunk s of the masks in LDS, and
+// thread l intersects it with rows j0 + l + 1024 t of the piece (the row
+// headers are requested before the staging, so both latencies overlap).
+// Group counts of the piece's subtiles collect in LDS and go to the global
+// counters at the end of each item.
+__global__ __launch_bounds__(kStagedThreads) void k_match_staged(MatchArgs m, const uint2* items, uint32_t items_cap,
+                                                                 const uint32_t* nitems) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    uint8_t* at = smem;
+    uint64_t* masks = (uint64_t*)at;
+    at += staged_align(m.stage_rows * kStageWords * 8);
+    uint32_t* allow = (uint32_t*)at;
+    at += staged_align(m.allow_words * 4);
+    Modulus* mods = (Modulus*)at;
+    at += staged_align(m.nres * (uint32_t)sizeof(Modulus));
+    uint32_t* tilecnt = (uint32_t*)at;
+    at += staged_align(kTileSlots * m.ng * 4);
+    uint2* queues = (uint2*)at;
+    at += kStagedWaves * kQueue * 8;
+    uint32_t* fill = (uint32_t*)at;   // survivors of the current item
+    // allow and the Modulus table are whole 16-byte units (both buffers are allocated so)
+    stage_copy<kStagedThreads, 1>((uint4*)allow, (const uint4*)m.allow, (m.allow_words + 3) / 4);
+    stage_copy<kStagedThreads, 1>((uint4*)mods, (const uint4*)m.mods, m.nres * (uint32_t)sizeof(Modulus) / 16);
+    for (uint32_t t = threadIdx.x; t < (uint32_t)kTileSlots * m.ng; t += kStagedThreads) tilecnt[t] = 0;
+    const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const Staged S{masks, allow, mods, tilecnt, queues + wave * kQueue};
+    const uint32_t total = min(*nitems, items_cap);
+    for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
+        const uint2 it = items[item];
+        const uint32_t p = it.x, s = it.y;
+        const uint32_t j0 = p * kPieceRows, j1 = min(m.nv, j0 + kPieceRows);
+        const uint32_t t0 = m.rows[j0].tile;
+        uint4 h0[kRowsPerLane], h1[kRowsPerLane];   // (left, right, tile, flags), r[0..7]
+#pragma unroll
+        for (int t = 0; t < kRowsPerLane; ++t) {
+            const uint32_t j = j0 + threadIdx.x + kStagedThreads * t;
+            const uint4* rp = (const uint4*)&m.rows[j < j1 ? j : j1 - 1];
+            h0[t] = rp[0];
+            h1[t] = rp[1];
+        }
+        if (threadIdx.x == 0) *fill = 0;
+        uint2* region = m.cand + (uint64_t)item * m.item_cap;
+        __syncthreads();   // the previous item's readers of the masks are done
+        stage_copy<kStagedThreads, 4>((uint4*)masks, (const uint4*)(m.masks + (uint64_t)s * m.stage_rows * kStageWords),
+                                      m.stage_rows * kStageWords / 2);
+        __syncthreads();
+        const uint32_t base0 = (s * kStageWords) << 6;   // first u index of the chunk
+        uint32_t cur = 0xffffffffu, acc1 = 0, acc2 = 0;
+        uint32_t queued = 0;
+#pragma unroll
+        for (int t = 0; t < kRowsPerLane; ++t) {
+            const uint32_t j = j0 + threadIdx.x + kStagedThreads * t;
+            if (j0 + (threadIdx.x - lane) + kStagedThreads * t < j1) {   // the wave has rows in the piece
+                const bool live = j < j1;
+                const uint32_t left = h0[t].x, right = h0[t].y, tile = h0[t].z;
+                if (live && tile != cur) {
+                    if (cur != 0xffffffffu) {
+                        if (acc1) staged_add(m, S, t0, cur, 1, acc1);
+                        if (acc2) staged_add(m, S, t0, cur, 2, acc2);
+                    }
+                    cur = tile;
+                    acc1 = acc2 = 0;
+                }
+                uint64_t b[kStageWords];
+#pragma unroll
+                for (int ww = 0; ww < kStageWords; ++ww) {
+                    const uint32_t base = base0 + 64 * ww;
+                    b[ww] = 0;
+                    if (live && left < right && base < right && base + 64 > left) {
+                        uint64_t x = ~0ull;
+                        if (base < left) x &= ~0ull << (left - base);
+                        if (right < base + 64) x &= (1ull << (right - base)) - 1ull;
+                        b[ww] = x;
+                    }
+                }
+                // every bitset row of this v is read before any is used: independent LDS reads
+                uint4 mw[kStagedFilters][2];
+#pragma unroll
+                for (int k = 0; k < kStagedFilters; ++k) {
+                    const uint32_t half = k < 2 ? h1[t].x : k < 4 ? h1[t].y : k < 6 ? h1[t].z : h1[t].w;
+                    const uint32_t r = (half >> (16 * (k & 1))) & 0xffffu;
+                    mw[k][0] = mw[k][1] = make_uint4(~0u, ~0u, ~0u, ~0u);
+                    if ((uint32_t)k < m.nbits) {
+                        const uint32_t row = m.row_base[k] + r, swap = stage_swizzle(row);
+                        const uint4* src = (const uint4*)(S.masks + row * kStageWords);
+                        mw[k][0] = src[swap];
+                        mw[k][1] = src[swap ^ 1];
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < kStagedFilters; ++k) {
+                    if ((uint32_t)k < m.nbits) {
+                        b[0] &= ((uint64_t)mw[k][0].y << 32) | mw[k][0].x;
+                        b[1] &= ((uint64_t)mw[k][0].w << 32) | mw[k][0].z;
+                        b[2] &= ((uint64_t)mw[k][1].y << 32) | mw[k][1].x;
+                        b[3] &= ((uint64_t)mw[k][1].w << 32) | mw[k][1].z;
+                        if ((k & 3) == 3 || (uint32_t)k + 1 == m.nres) {
+                            const uint32_t n = __popcll(b[0]) + __popcll(b[1]) + __popcll(b[2]) + __popcll(b[3]);
+                            if (k < 4) acc1 += n;
+                            else acc2 += n;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int ww = 0; ww < kStageWords; ++ww)
+                    staged_enqueue(S, lane, b[ww], j, base0 + 64 * ww, queued, region, m.item_cap, fill);
+            }
+        }
+        if (queued) {
+            wave_sync();
+            cand_flush(S, lane, queued, region, m.item_cap, fill);
+        }
+        if (cur != 0xffffffffu) {
+            if (acc1) staged_add(m, S, t0, cur, 1, acc1);
+            if (acc2) staged_add(m, S, t0, cur, 2, acc2);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) m.item_count[item] = *fill;
+        for (uint32_t t = threadIdx.x; t < (uint32_t)kTileSlots * m.ng; t += kStagedThreads) {
+            const uint32_t n = tilecnt[t];
+            if (n) {
+                atomicAdd(&m.counters[(uint64_t)(t0 + t / m.ng) * (m.ng + 3) + t % m.ng + 1], (uint64_t)n);
+                tilecnt[t] = 0;
+            }
+        }
+    }
+}
+
+// The residual chain over the batch's bitset survivors: wave w of the grid
+// takes work items w, w + waves, ... and runs each item's survivors 64 at a
+// time.  Group counts of subtiles below kTileSlots are kept per block in
+// LDS.  An item whose survivors exceeded item_cap records its count in
+// cand_max so the host can size the regions and run again.
+__global__ __launch_bounds__(kMatchThreads) void k_chain_staged(MatchArgs m, const uint32_t* nitems, uint32_t items_cap) {
+    __shared__ uint32_t allow[kMaxAllowWords];
+    __shared__ Modulus mods[kMaxResiduals];
+    __shared__ uint32_t tilecnt[kTileSlots * (kMaxResiduals / 4)];
+    for (uint32_t t = threadIdx.x; t < m.allow_words; t += kMatchThreads) allow[t] = m.allow[t];
+    for (uint32_t t = threadIdx.x; t < m.nres; t += kMatchThreads) mods[t] = m.mods[t];
+    for (uint32_t t = threadIdx.x; t < (uint32_t)kTileSlots * m.ng; t += kMatchThreads) tilecnt[t] = 0;
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const Staged S{nullptr, allow, mods, tilecnt, nullptr};
+    const uint32_t total = min(*nitems, items_cap);
+    const uint32_t waves = gridDim.x * kMatchWaves;
+    for (uint32_t item = blockIdx.x * kMatchWaves + wave; item < total; item += waves) {
+        uint32_t n = m.item_count[item];
+        if (n > m.item_cap) {
+            if (lane == 0) atomicMax((unsigned long long*)m.cand_max, (unsigned long long)n);
+            n = m.item_cap;
+        }
+        const uint2* region = m.cand + (uint64_t)item * m.item_cap;
+        for (uint32_t first = 0; first < n; first += kWave) {
+            const uint32_t count = min(n - first, (uint32_t)kWave);
+            const uint2 e = lane < count ? region[first + lane] : make_uint2(0, 0);
+            staged_chain(m, S, lane, count, e, 0);
+        }
+    }
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < (uint32_t)kTileSlots * m.ng; t += kMatchThreads) {
+        const uint32_t c = tilecnt[t];
+        if (c) atomicAdd(&m.counters[(uint64_t)(t / m.ng) * (m.ng + 3) + t % m.ng + 1], (uint64_t)c);
+    }
+}
+
+}  // namespace pcube
