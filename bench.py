@@ -29,14 +29,14 @@ from pathlib import Path
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-# Algorithmic work per focused candidate position, in int32 VALU operations
-# (fixed cost table: bitset stage 0.125, per-survivor wide build and filters
-# 0.40, per-row bounds 0.02, windows 0.01).
-OPS_PER_POSITION = 0.56
-# MI355X int32 vector peak: 256 CUs x 64 lanes x 2.4 GHz = 39.3e12 lane-ops/s
-# (one quarter of the published 157.3 TFLOP/s fp32 vector figure, which
-# counts FMA as two operations and packed dual issue twice).
-INT32_PEAK = 256 * 64 * 2.4e9
+# The dominant kernel, the staged bitset intersection, is bound by LDS
+# bandwidth: per 64 focused positions (one mask word) it reads one 8-byte word
+# of each of the eight bitset filters' masks from LDS, so its algorithmic
+# traffic is 8 * 8 / 64 = 1 LDS byte per focused position.
+STAGED_FILTERS = 8
+LDS_BYTES_PER_POSITION = STAGED_FILTERS * 8 / 64
+# MI355X LDS bandwidth: 256 CUs x 128 bytes per clock x 2.4 GHz = 78.6e12 B/s.
+LDS_PEAK = 256 * 128 * 2.4e9
 
 
 def parse() -> argparse.Namespace:
@@ -59,8 +59,8 @@ def parse() -> argparse.Namespace:
 
 
 def measured_traffic() -> int | None:
-    """HBM bytes per match launch from the committed counter summary, if present."""
-    path = ROOT / 'profiles' / 'pmc_match_p619_batch.md'
+    """HBM bytes per intersection launch from the committed counter summary, if present."""
+    path = ROOT / 'profiles' / 'pmc_match_staged_p619_batch.md'
     if not path.exists():
         return None
     for line in path.read_text().splitlines():
@@ -223,19 +223,22 @@ def main() -> None:
     for _ in range(args.warmup):
         step()
     print(f'# warmup {args.warmup} done', file=sys.stderr, flush=True)
-    positions, kernel, prefilter = 0, 0.0, 0.0
+    positions, kernel, prefilter, intersection = 0, 0.0, 0.0, 0.0
     started = time.perf_counter()
     for _ in range(args.steps):
         row = step()
         positions += row['valid_focused_candidates']
         kernel += row['kernel_seconds']
         prefilter += row['prefilter_build_seconds']
+        intersection += matcher.intersection_seconds
     elapsed = time.perf_counter() - started
     print(f'# timed {args.steps} steps in {elapsed:.3f} s', file=sys.stderr, flush=True)
 
     per_launch = positions / args.steps
     match_seconds = kernel / args.steps
-    achieved = OPS_PER_POSITION * per_launch / match_seconds
+    # HIP events around each intersection launch on the library's stream
+    intersection_seconds = intersection / args.steps
+    achieved = LDS_BYTES_PER_POSITION * per_launch / intersection_seconds
     result = dict(metric='focused candidate positions/s on 1 MI355X, plain vs batched; p50/p99 latency',
                   value=positions / elapsed, unit='focused candidate positions/s', n_gpus=1, steps=args.steps,
                   warmup=args.warmup, ms_per_step=1000 * elapsed / args.steps, higher_is_better=True,
@@ -245,10 +248,10 @@ def main() -> None:
                               p=619, tile_width=100_000_000, subtiles=64, seed=619003),
                   kernel_only=dict(value=positions / kernel, unit='focused candidate positions/s',
                                    match_ms=1000 * match_seconds, prefilter_ms=1000 * prefilter / args.steps),
-                  roofline=dict(bound='valu', achieved=achieved / 1e12, peak=INT32_PEAK / 1e12, unit='TOP/s',
-                                frac=achieved / INT32_PEAK, traffic=measured_traffic(),
-                                kernel='k_match', ops_per_position=OPS_PER_POSITION,
-                                positions_per_launch=per_launch))
+                  roofline=dict(bound='lds', achieved=achieved / 1e12, peak=LDS_PEAK / 1e12, unit='TB/s',
+                                frac=achieved / LDS_PEAK, traffic=measured_traffic(),
+                                kernel='k_match_staged<8>', kernel_ms=1000 * intersection_seconds,
+                                lds_bytes_per_position=LDS_BYTES_PER_POSITION, positions_per_launch=per_launch))
     if not args.no_lease:
         first = 575222931200000000
         compute_lease(matcher, af, bf, first, first + 4 * BATCH_WIDTH)  # warm the lease buffers

@@ -129,7 +129,9 @@ int pcube_matcher_destroy(int matcher);
  *   values       host, min(emitted, cap) (low, high) limb pairs per subtile,
  *                concatenated in subtile order; values_cap pairs available
  *   engine       0 = residue-bitset engine, 1 = direct per-position engine
- *   seconds      host, 2 doubles: prefilter build, match kernel (may be NULL)
+ *   seconds      host, 3 doubles (may be NULL): prefilter build (rows, bitsets,
+ *                residue cache), matching (every kernel after the prefilter),
+ *                and of that the bitset-intersection kernel alone
  * Returns OK, OVERFLOW (a subtile emitted more than cap; counters valid),
  * UNDERFLOW, RANGE, INVALID. */
 int pcube_match(int matcher, uint64_t us, uint64_t nu, uint64_t vs, uint64_t nv, uint64_t v_first,

@@ -208,6 +208,7 @@ class FusedMatcher:
         self.kernel = runtime.compile(self.kernel_name)
         self.sm_count = runtime.compute_units
         self.prefilter_seconds = 0.0
+        self.intersection_seconds = 0.0   # the bitset-intersection kernel of the last run
 
     def run(self, batch: Prepared, output_cap: int = 8192) -> dict[str, Any]:
         started = time.perf_counter()
@@ -218,6 +219,7 @@ class FusedMatcher:
         counters, per_tile, seconds = self.engine.run(batch.us, batch.vs, batch.first, batch.width, len(ranges),
                                                       output_cap, self.engine_id, overflow_ok=True)
         self.prefilter_seconds = seconds[0]
+        self.intersection_seconds = seconds[2]
         timings = dict(kernel_seconds=seconds[1], focus_enumeration_seconds=batch.prep_seconds)
         row = batch_row(self.config, self.filter_groups, ranges, batch.bounds, int(batch.vs.size), counters,
                         per_tile, output_cap, timings, started)

@@ -624,7 +624,8 @@ struct Matcher {
     // survivors of any staged work item that overflowed its region since the last clear
     Buffer rows, masks, rho, counters, list, status, cand, items;
     uint64_t cand_cap = 512;                            // bitset-survivor entries per staged work item
-    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};   // start, bitsets built, match done
+    // start, bitsets built, intersection kernel start and end, match done
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     uint64_t* list_count() const { return status.as<uint64_t>(); }
     uint32_t* error() const { return (uint32_t*)(status.as<uint64_t>() + 1); }
     ~Matcher() {
@@ -656,7 +657,8 @@ bool allowed_residue(uint32_t q, uint32_t a) {
 // Staged engine for one batch (engine 0 when the matcher has staged filters):
 // rows with per-subtile valid counts, staged bitsets, cached residues, then
 // one block per (stage chunk, row piece).
-int staged_enqueue(Matcher& mt, MatchArgs m, hipEvent_t ev_pre, hipEvent_t ev_match) {
+int staged_enqueue(Matcher& mt, MatchArgs m, hipEvent_t ev_pre, hipEvent_t ev_match, hipEvent_t ev_k0,
+                   hipEvent_t ev_k1) {
     m.staged = 1;
     m.nbits = mt.sbits;
     m.ncache = mt.sncache;
@@ -693,15 +695,21 @@ int staged_enqueue(Matcher& mt, MatchArgs m, hipEvent_t ev_pre, hipEvent_t ev_ma
         uint2* items = mt.items.as<uint2>();
         uint32_t* nitems = (uint32_t*)(items + items_cap);
         m.item_count = nitems + 2;
-        TRY(mt.cand.ensure((uint64_t)items_cap * mt.cand_cap * sizeof(uint2)));
-        m.cand = mt.cand.as<uint2>();
+        TRY(mt.cand.ensure((uint64_t)items_cap * mt.cand_cap * sizeof(uint32_t)));
+        m.cand = mt.cand.as<uint32_t>();
         hipLaunchKernelGGL(k_plan_staged, dim3(1), dim3(kStagedThreads), 0, g.stream, m, npieces, items, items_cap,
                            nitems);
-        hipLaunchKernelGGL(k_match_staged, dim3((uint32_t)g.compute_units), dim3(kStagedThreads), mt.slds, g.stream, m,
-                           (const uint2*)items, items_cap, (const uint32_t*)nitems);
-        HIP_TRY(hipGetLastError());
+        {
+            const uint2* items_arg = items;
+            const uint32_t* nitems_arg = nitems;
+            void* args[] = {&m, &items_arg, (void*)&items_cap, &nitems_arg};
+            if (ev_k0) HIP_TRY(hipEventRecord(ev_k0, g.stream));
+            HIP_TRY(hipLaunchKernel(staged_kernel(mt.sbits, mt.nres), dim3((uint32_t)g.compute_units),
+                                    dim3(kStagedThreads), args, mt.slds, g.stream));
+            if (ev_k1) HIP_TRY(hipEventRecord(ev_k1, g.stream));
+        }
         hipLaunchKernelGGL(k_chain_staged, dim3(2 * (uint32_t)g.compute_units), dim3(kMatchThreads), 0, g.stream, m,
-                           (const uint32_t*)nitems, items_cap);
+                           (const uint2*)items, (const uint32_t*)nitems, items_cap);
     }
     if (ev_match) HIP_TRY(hipEventRecord(ev_match, g.stream));
     return PCUBE_OK;
@@ -712,7 +720,7 @@ bool staged_fits(const Matcher& mt, uint64_t nu, uint64_t nv) {
     if (!mt.sbits || mt.slds > g.lds_per_block) return false;
     const uint64_t words = (nu + 63) / 64, nstage = (words + kStageWords - 1) / kStageWords;
     const uint64_t npieces = (nv + kPieceRows - 1) / kPieceRows;
-    return npieces * nstage <= (1ull << 22) && (nu + 64) * kCachedResidues * 2 < (1ull << 40);
+    return npieces * nstage <= (1ull << 22) && nu < (1ull << 31);
 }
 
 // Internal engine code: the residue-bitset engine without LDS staging.
@@ -724,7 +732,8 @@ constexpr uint64_t kMaxItemCandidates = 1ull << 16;
 // Enqueue the kernels of one batch; counters must be zeroed by the caller.
 int match_enqueue(Matcher& mt, const uint64_t* us, uint64_t nu, const uint64_t* vs, uint64_t nv, uint64_t v_first,
                   uint64_t tile_width, uint32_t ntiles, uint64_t cap, int engine, uint64_t* counters,
-                  uint64_t tile_base, uint64_t list_cap, hipEvent_t ev_pre, hipEvent_t ev_match) {
+                  uint64_t tile_base, uint64_t list_cap, hipEvent_t ev_pre, hipEvent_t ev_match,
+                  hipEvent_t ev_k0 = nullptr, hipEvent_t ev_k1 = nullptr) {
     if (nu >= (1ull << 32) - 64 || nv >= (1ull << 32)) return fail(PCUBE_INVALID, "batch too large");
     const uint32_t words = (uint32_t)((nu + 63) / 64);
     TRY(mt.rows.ensure(std::max<uint64_t>(1, nv) * sizeof(Row)));
@@ -752,7 +761,7 @@ int match_enqueue(Matcher& mt, const uint64_t* us, uint64_t nu, const uint64_t* 
     m.ng = mt.ng; m.ntiles = ntiles; m.allow_words = mt.allow_words;
     for (int k = 0; k < kMaxBitsetFilters; ++k) m.row_base[k] = mt.row_base[k];
     if (nv == 0) return PCUBE_OK;
-    if (engine == 0 && staged_fits(mt, nu, nv)) return staged_enqueue(mt, m, ev_pre, ev_match);
+    if (engine == 0 && staged_fits(mt, nu, nv)) return staged_enqueue(mt, m, ev_pre, ev_match, ev_k0, ev_k1);
     if (engine == kUnstagedBitsets) engine = 0;
     // k_rows always computes v mod q for the bitset filters of the matcher
     MatchArgs mr = m;
@@ -771,6 +780,7 @@ int match_enqueue(Matcher& mt, const uint64_t* us, uint64_t nu, const uint64_t* 
             hipLaunchKernelGGL(k_rho, dim3(blocks_for(nu * kCachedResidues, 256)), dim3(256), 0, g.stream, m);
     }
     if (ev_pre) HIP_TRY(hipEventRecord(ev_pre, g.stream));
+    if (ev_k0) HIP_TRY(hipEventRecord(ev_k0, g.stream));
     const uint64_t rows_per_wave = std::max<uint64_t>(1, (nv + 8191) / 8192);
     m.rows_per_wave = (uint32_t)rows_per_wave;
     const uint64_t waves = (nv + rows_per_wave - 1) / rows_per_wave;
@@ -781,6 +791,7 @@ int match_enqueue(Matcher& mt, const uint64_t* us, uint64_t nu, const uint64_t* 
     } else {
         hipLaunchKernelGGL(k_match_direct, dim3((uint32_t)blocks), dim3(kMatchThreads), 0, g.stream, m);
     }
+    if (ev_k1) HIP_TRY(hipEventRecord(ev_k1, g.stream));
     if (ev_match) HIP_TRY(hipEventRecord(ev_match, g.stream));
     return PCUBE_OK;
 }
@@ -951,7 +962,7 @@ const Kernel kKernels[] = {
     {"rows", (const void*)k_rows},
     {"mask_build", (const void*)k_masks},
     {"mask_staged", (const void*)k_masks_staged},
-    {"match_staged", (const void*)k_match_staged},
+    {"match_staged", (const void*)k_match_staged<kStagedFilters, false>},
     {"residues", (const void*)k_rho},
     {"match", (const void*)k_match},
     {"match_direct", (const void*)k_match_direct},
@@ -1000,11 +1011,12 @@ int pcube_init(int device, uint64_t memory_cap, double fraction, uint64_t* cap, 
         HIP_TRY(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
         g.lds_per_block = (uint32_t)std::max(lds, 0);
         // the staged engine uses more than 64 KB of dynamic LDS when the device allows it
-        if (hipFuncSetAttribute((const void*)k_match_staged, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)g.lds_per_block) != hipSuccess) {
-            (void)hipGetLastError();
-            g.lds_per_block = std::min<uint32_t>(g.lds_per_block, 65536);
-        }
+        for (uint32_t f = 1; f <= 2 * (uint32_t)kStagedFilters; ++f)
+            if (hipFuncSetAttribute(staged_kernel((f + 1) / 2, f % 2 ? (f + 1) / 2 : 128), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)g.lds_per_block) != hipSuccess) {
+                (void)hipGetLastError();
+                g.lds_per_block = std::min<uint32_t>(g.lds_per_block, 65536);
+            }
         g.device = device;
     }
     size_t fr = 0, tot = 0;
@@ -1425,7 +1437,7 @@ int pcube_match(int matcher, uint64_t us, uint64_t nu, uint64_t vs, uint64_t nv,
     TRY(mt->list.ensure(list_cap * 24));
     for (hipEvent_t& e : mt->ev)
         if (!e) HIP_TRY(hipEventCreate(&e));
-    hipEvent_t e0 = mt->ev[0], e1 = mt->ev[1], e2 = mt->ev[2];
+    hipEvent_t e0 = mt->ev[0], e1 = mt->ev[1], e2 = mt->ev[2], k0 = mt->ev[3], k1 = mt->ev[4];
     // one pinned download: [status words, counters]; a bitset-survivor list that
     // overflowed is sized from its count and the batch runs again
     TRY(g_down_host.ensure((nc + 4) * 8));
@@ -1436,11 +1448,10 @@ int pcube_match(int matcher, uint64_t us, uint64_t nu, uint64_t vs, uint64_t nv,
         TRY(fill64(mt->status.as<uint64_t>(), 4, 0));
         HIP_TRY(hipEventRecord(e0, g.stream));
         rc = match_enqueue(*mt, (const uint64_t*)us, nu, (const uint64_t*)vs, nv, v_first, tile_width, ntiles, cap,
-                           engine, mt->counters.as<uint64_t>(), 0, list_cap, e1, e2);
+                           engine, mt->counters.as<uint64_t>(), 0, list_cap, e1, e2, k0, k1);
         if (rc) return rc;
         if (nv == 0) {
-            HIP_TRY(hipEventRecord(e1, g.stream));
-            HIP_TRY(hipEventRecord(e2, g.stream));
+            for (hipEvent_t e : {e1, k0, k1, e2}) HIP_TRY(hipEventRecord(e, g.stream));
         }
         HIP_TRY(hipMemcpyAsync(hd, mt->status.p, 4 * 8, hipMemcpyDeviceToHost, g.stream));
         HIP_TRY(hipMemcpyAsync(hd + 4, mt->counters.p, nc * 8, hipMemcpyDeviceToHost, g.stream));
@@ -1453,11 +1464,13 @@ int pcube_match(int matcher, uint64_t us, uint64_t nu, uint64_t vs, uint64_t nv,
     const uint32_t err = (uint32_t)hd[1];
     std::vector<uint64_t> host(hd + 4, hd + 4 + nc);
     if (seconds) {
-        float a = 0, b = 0;
+        float a = 0, b = 0, c = 0;
         HIP_TRY(hipEventElapsedTime(&a, e0, e1));
         HIP_TRY(hipEventElapsedTime(&b, e1, e2));
+        HIP_TRY(hipEventElapsedTime(&c, k0, k1));
         seconds[0] = a / 1000.0;
         seconds[1] = b / 1000.0;
+        seconds[2] = c / 1000.0;
     }
     memcpy(counters, host.data(), nc * 8);
     rc = batch_status(*mt, host, ntiles, cap, err);

@@ -283,7 +283,7 @@ __global__ void k_searchsorted(const uint64_t* __restrict__ sorted, uint64_t n, 
 struct Row {
     uint32_t left, right;   // u index range [left, right) with lo < u*B - v*A <= hi
     uint32_t tile, flags;   // subtile index; bit0: a v outside [v_first, v_first + ntiles*width)
-    uint16_t r[kStagedFilters];       // v mod q for the bitset filters
+    uint16_t r[kStagedFilters];       // v mod q for the bitset filters (staged engine: LDS offsets)
     uint16_t sigma[kCachedResidues];  // (-v*A) mod q for the cached chain filters (staged engine)
     uint64_t v;
     uint64_t pad;
@@ -311,7 +311,7 @@ struct MatchArgs {
     uint32_t staged, stage_rows, nstage;
     uint32_t nbeta[kStagedFilters];    // (-B * A^-1) mod q per bitset filter
     uint32_t pstr_off[kStagedFilters];
-    uint2* cand;               // bitset survivors (row, u index), item_cap entries per work item
+    uint32_t* cand;            // bitset survivors, item_cap entries per work item of the staged engine
     uint32_t* item_count;      // survivors of each work item (entries beyond item_cap are dropped)
     uint64_t* cand_max;        // largest item_count above item_cap seen since the last clear
     uint32_t item_cap;
@@ -350,8 +350,16 @@ __device__ __forceinline__ void build_row(const MatchArgs& m, uint32_t j) {
     // (impossible for sorted input) is reported as the underflow counter.
     if (row.left < row.right && less_wide(mul_wide(m.us[row.left], m.B), shift))
         atomicAdd(&m.counters[(uint64_t)row.tile * (m.ng + 3) + m.ng + 2], 1ull);
-    for (uint32_t k = 0; k < kStagedFilters; ++k)
-        row.r[k] = k < m.nbits ? uint16_t(mod64(v, m.mods[k])) : 0;
+    // the staged engine keeps instead the LDS byte offset of the filter's mask row
+    // R = row_base + v mod q: 32 R, plus 16 when bit 3 of R is set (its halves are swapped)
+    for (uint32_t k = 0; k < kStagedFilters; ++k) {
+        uint32_t r = k < m.nbits ? mod64(v, m.mods[k]) : 0u;
+        if (m.staged && k < m.nbits) {
+            const uint32_t R = r + m.row_base[k];
+            r = (R << 5) | ((R & 8u) << 1);
+        }
+        row.r[k] = uint16_t(r);
+    }
     for (uint32_t c = 0; c < kCachedResidues; ++c) {
         uint16_t s = 0;
         if (m.staged && c < m.ncache) {

@@ -8,13 +8,17 @@
 // kStageWords words, [chunk][sum of q rows][kStageWords], so one chunk of every
 // filter is one contiguous block of stage_rows * 32 bytes.
 //
-// Work item = (stage chunk s, piece of kPieceRows consecutive v rows).  A
-// block copies chunk s into LDS once and intersects it with every row of its
-// piece: a lane takes whole rows and ANDs, for each of the chunk's words, the
-// kStagedFilters words of its row's residue classes.  Positions surviving all bitset filters (about 1 in 6561
-// at p = 619) go through the rest of the filter chain exactly as in the
-// residue-bitset engine: cached residues (u*B mod q) + sigma_v for the next
-// kCachedResidues filters, then exact 128-bit x = u*B - v*A reduced limb by limb.
+// Work item = (stage chunk s, piece of kPieceRows consecutive v rows).  One
+// persistent block per CU walks a list of items; for each it intersects the
+// chunk, staged in LDS, with every row of the piece: a thread takes whole rows
+// and ANDs, for each of the chunk's words, the kStagedFilters words of its
+// row's residue classes.  The next item's chunk streams into a second LDS
+// buffer (global_load_lds) while the current one is matched.  Positions
+// surviving all bitset filters (about 1 in 6561 at p = 619) are written to the
+// item's region of a candidate list; k_chain_staged runs them through the rest
+// of the filter chain exactly as in the residue-bitset engine: cached
+// residues (u*B mod q) + sigma_v for the next kCachedResidues filters, then
+// exact 128-bit x = u*B - v*A reduced limb by limb.
 //
 // Counts follow the reference's group structure (filters in groups of four,
 // pseudocube/batch_experiment.py:198-235): group 1 and group 2 are popcounts
@@ -124,19 +128,19 @@ __device__ __forceinline__ void stage_copy(uint4* __restrict__ dst, const uint4*
 
 // LDS of a staged block, carved from dynamic shared memory.
 struct Staged {
-    const uint64_t* masks;   // [stage_rows][kStageWords] of the block's chunk
     const uint32_t* allow;
     const Modulus* mods;
     uint32_t* tilecnt;       // [kTileSlots][ng]: groups 1..ng of subtiles t0..t0+63
-    uint2* queue;            // this wave's (row, u index) survivors
+    uint32_t* queue;         // this wave's survivors, (row offset << 8) | u offset within the item
 };
 
-__host__ __device__ inline uint32_t staged_align(uint32_t bytes) { return (bytes + 15u) & ~15u; }
+__host__ __device__ inline uint32_t staged_align(uint32_t bytes) { return (bytes + 31u) & ~31u; }
+// Bytes of dynamic LDS: two mask buffers, tables, counters, queues, fill counter.
 __host__ __device__ inline uint32_t staged_lds_bytes(uint32_t stage_rows, uint32_t allow_words, uint32_t nres,
                                                      uint32_t ng) {
-    return staged_align(stage_rows * kStageWords * 8) + staged_align(allow_words * 4) +
+    return 2 * staged_align(stage_rows * kStageWords * 8) + staged_align(allow_words * 4) +
            staged_align(nres * (uint32_t)sizeof(Modulus)) + staged_align(kTileSlots * ng * 4) +
-           kStagedWaves * kQueue * 8 + 16;
+           kStagedWaves * kQueue * 4 + 32;
 }
 
 // Add n to group g (1-based) of a subtile: LDS when the subtile is within the
@@ -148,14 +152,13 @@ __device__ __forceinline__ void staged_add(const MatchArgs& m, const Staged& S, 
     else atomicAdd(&m.counters[(uint64_t)tile * (m.ng + 3) + g], (uint64_t)n);
 }
 
-// Run up to 64 bitset survivors (lane l takes entry e, live when l < count)
-// through the filters after the bitsets.
+// Run up to 64 bitset survivors (lane l takes row j, u index i, live when
+// l < count) through the filters after the bitsets.
 __device__ __forceinline__ void staged_chain(const MatchArgs& m, const Staged& S, uint32_t lane, uint32_t count,
-                                             uint2 e, uint32_t t0) {
+                                             uint32_t j, uint32_t i) {
     bool alive = lane < count;
-    if (!alive) e = make_uint2(0, 0);
-    const Row row = m.rows[e.x];
-    const uint32_t i = e.y;
+    if (!alive) j = i = 0;
+    const Row row = m.rows[j];
     uint32_t k = m.nbits;
     uint4 packed = make_uint4(0, 0, 0, 0);
     if (alive && m.ncache) packed = ((const uint4*)m.rho)[i];
@@ -169,7 +172,7 @@ __device__ __forceinline__ void staged_chain(const MatchArgs& m, const Staged& S
             r = r >= mod.q ? r - mod.q : r;
             alive = allowed_bit(S.allow, mod.bit + r);
         }
-        if (((k & 3) == 3 || k + 1 == m.nres) && alive) staged_add(m, S, t0, row.tile, k / 4 + 1, 1);
+        if (((k & 3) == 3 || k + 1 == m.nres) && alive) staged_add(m, S, 0, row.tile, k / 4 + 1, 1);
         if (!__ballot(alive)) return;
         ++k;
     }
@@ -178,7 +181,7 @@ __device__ __forceinline__ void staged_chain(const MatchArgs& m, const Staged& S
     for (; k < m.nres; ++k) {
         const Modulus mod = S.mods[k];
         if (alive) alive = allowed_bit(S.allow, mod.bit + mod128(x, mod));
-        if (((k & 3) == 3 || k + 1 == m.nres) && alive) staged_add(m, S, t0, row.tile, k / 4 + 1, 1);
+        if (((k & 3) == 3 || k + 1 == m.nres) && alive) staged_add(m, S, 0, row.tile, k / 4 + 1, 1);
         if (!__ballot(alive)) return;
     }
     if (alive) {
@@ -189,7 +192,7 @@ __device__ __forceinline__ void staged_chain(const MatchArgs& m, const Staged& S
 
 // Append count (<= 64) queued survivors to the work item's region of the
 // candidate list; the item's fill counter lives in LDS.
-__device__ __forceinline__ void cand_flush(const Staged& S, uint32_t lane, uint32_t count, uint2* region,
+__device__ __forceinline__ void cand_flush(const Staged& S, uint32_t lane, uint32_t count, uint32_t* region,
                                            uint32_t cap, uint32_t* fill) {
     uint32_t at = 0;
     if (lane == 0) at = atomicAdd(fill, count);
@@ -197,15 +200,15 @@ __device__ __forceinline__ void cand_flush(const Staged& S, uint32_t lane, uint3
     if (lane < count && at + lane < cap) region[at + lane] = S.queue[lane];
 }
 
-// Queue the set bits of one mask word (u indices base + bit) with row j.
-__device__ __forceinline__ void staged_enqueue(const Staged& S, uint32_t lane, uint64_t b, uint32_t j, uint32_t base,
-                                               uint32_t& queued, uint2* region, uint32_t cap, uint32_t* fill) {
+// Queue the set bits of one mask word as (row offset << 8) | (u offset of the word + bit).
+__device__ __forceinline__ void staged_enqueue(const Staged& S, uint32_t lane, uint64_t b, uint32_t tag,
+                                               uint32_t& queued, uint32_t* region, uint32_t cap, uint32_t* fill) {
     for (;;) {
         const bool has = b != 0;
         const uint64_t ballot = __ballot(has);
         if (!ballot) return;
         if (has) {
-            S.queue[queued + lanes_below(ballot)] = make_uint2(j, base + __builtin_ctzll(b));
+            S.queue[queued + lanes_below(ballot)] = tag + (uint32_t)__builtin_ctzll(b);
             b &= b - 1;
         }
         queued += __popcll(ballot);
@@ -214,7 +217,7 @@ __device__ __forceinline__ void staged_enqueue(const Staged& S, uint32_t lane, u
             cand_flush(S, lane, kWave, region, cap, fill);
             wave_sync();
             const uint32_t rest = queued - kWave;
-            uint2 moved = make_uint2(0, 0);
+            uint32_t moved = 0;
             if (lane < rest) moved = S.queue[kWave + lane];
             wave_sync();
             if (lane < rest) S.queue[lane] = moved;
@@ -272,53 +275,91 @@ __global__ __launch_bounds__(kStagedThreads) void k_plan_staged(MatchArgs m, uin
     if (threadIdx.x == kStagedThreads - 1) *nitems = at;
 }
 
+// Row headers of one work item's rows for this thread: (left, right, tile,
+// flags) and the LDS byte offsets of the filters' mask rows.
+struct ItemRows {
+    uint4 h0[kRowsPerLane], h1[kRowsPerLane];
+};
+
+__device__ __forceinline__ void load_item_rows(const MatchArgs& m, uint32_t p, ItemRows& R) {
+    const uint32_t j0 = p * kPieceRows, j1 = min(m.nv, j0 + kPieceRows);
+#pragma unroll
+    for (int t = 0; t < kRowsPerLane; ++t) {
+        const uint32_t j = j0 + threadIdx.x + kStagedThreads * t;
+        const uint4* rp = (const uint4*)&m.rows[j < j1 ? j : j1 - 1];
+        R.h0[t] = rp[0];
+        R.h1[t] = rp[1];
+    }
+}
+
+// Start the copy of stage chunk s into an LDS buffer: each wave moves 64
+// sixteen-byte units per instruction straight from global memory to LDS.
+// Complete once the wave's vector memory counter drains.
+__device__ __forceinline__ void stage_async(const MatchArgs& m, uint32_t s, uint8_t* buffer, uint32_t wave,
+                                            uint32_t lane) {
+    const uint4* src = (const uint4*)(m.masks + (uint64_t)s * m.stage_rows * kStageWords);
+    const uint32_t n = m.stage_rows * kStageWords / 2;
+    for (uint32_t q = wave; q * kWave < n; q += kStagedWaves) {
+        const uint32_t t = q * kWave + lane;
+        if (t < n) __builtin_amdgcn_global_load_lds((const void*)(src + t), (void*)(buffer + q * kWave * 16), 16, 0, 0);
+    }
+}
+
 // Persistent staged match: block b takes work items b, b + grid, ...  For an
-// item (piece p, chunk s) the block stages chunk s of the masks in LDS, and
-// thread l intersects it with rows j0 + l + 1024 t of the piece (the row
-// headers are requested before the staging, so both latencies overlap).
-// Group counts of the piece's subtiles collect in LDS and go to the global
-// counters at the end of each item.
+// item (piece p, chunk s) thread l intersects the staged chunk with rows
+// j0 + l + 1024 t of the piece; the next item's chunk and row headers are in
+// flight meanwhile.  Group counts of the piece's subtiles collect in LDS and
+// go to the global counters at the end of each item.  kF = bitset filters;
+// kLast: the last residual filter is the last bitset filter (it closes a
+// group).
+template <int kF, bool kLast>
 __global__ __launch_bounds__(kStagedThreads) void k_match_staged(MatchArgs m, const uint2* items, uint32_t items_cap,
                                                                  const uint32_t* nitems) {
-    extern __shared__ __align__(16) uint8_t smem[];
-    uint8_t* at = smem;
-    uint64_t* masks = (uint64_t*)at;
-    at += staged_align(m.stage_rows * kStageWords * 8);
+    extern __shared__ __align__(32) uint8_t smem[];
+    const uint32_t mask_bytes = staged_align(m.stage_rows * kStageWords * 8);
+    uint8_t* buffers = smem;   // two mask buffers of mask_bytes
+    uint8_t* at = smem + 2 * mask_bytes;
     uint32_t* allow = (uint32_t*)at;
     at += staged_align(m.allow_words * 4);
     Modulus* mods = (Modulus*)at;
     at += staged_align(m.nres * (uint32_t)sizeof(Modulus));
     uint32_t* tilecnt = (uint32_t*)at;
     at += staged_align(kTileSlots * m.ng * 4);
-    uint2* queues = (uint2*)at;
-    at += kStagedWaves * kQueue * 8;
+    uint32_t* queues = (uint32_t*)at;
+    at += kStagedWaves * kQueue * 4;
     uint32_t* fill = (uint32_t*)at;   // survivors of the current item
+    const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const uint32_t total = min(*nitems, items_cap);
+    uint32_t item = blockIdx.x;
+    if (item >= total) return;
     // allow and the Modulus table are whole 16-byte units (both buffers are allocated so)
     stage_copy<kStagedThreads, 1>((uint4*)allow, (const uint4*)m.allow, (m.allow_words + 3) / 4);
     stage_copy<kStagedThreads, 1>((uint4*)mods, (const uint4*)m.mods, m.nres * (uint32_t)sizeof(Modulus) / 16);
     for (uint32_t t = threadIdx.x; t < (uint32_t)kTileSlots * m.ng; t += kStagedThreads) tilecnt[t] = 0;
-    const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    const Staged S{masks, allow, mods, tilecnt, queues + wave * kQueue};
-    const uint32_t total = min(*nitems, items_cap);
-    for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
-        const uint2 it = items[item];
+    const Staged S{allow, mods, tilecnt, queues + wave * kQueue};
+    uint2 it = items[item];
+    ItemRows rows;
+    load_item_rows(m, it.x, rows);
+    stage_async(m, it.y, buffers, wave, lane);
+    uint32_t parity = 0;
+    for (;;) {
         const uint32_t p = it.x, s = it.y;
         const uint32_t j0 = p * kPieceRows, j1 = min(m.nv, j0 + kPieceRows);
         const uint32_t t0 = m.rows[j0].tile;
-        uint4 h0[kRowsPerLane], h1[kRowsPerLane];   // (left, right, tile, flags), r[0..7]
-#pragma unroll
-        for (int t = 0; t < kRowsPerLane; ++t) {
-            const uint32_t j = j0 + threadIdx.x + kStagedThreads * t;
-            const uint4* rp = (const uint4*)&m.rows[j < j1 ? j : j1 - 1];
-            h0[t] = rp[0];
-            h1[t] = rp[1];
-        }
+        __builtin_amdgcn_s_waitcnt(0);   // this wave's part of the chunk and its row headers have arrived
         if (threadIdx.x == 0) *fill = 0;
-        uint2* region = m.cand + (uint64_t)item * m.item_cap;
-        __syncthreads();   // the previous item's readers of the masks are done
-        stage_copy<kStagedThreads, 4>((uint4*)masks, (const uint4*)(m.masks + (uint64_t)s * m.stage_rows * kStageWords),
-                                      m.stage_rows * kStageWords / 2);
+        uint32_t* region = m.cand + (uint64_t)item * m.item_cap;
         __syncthreads();
+        const uint8_t* masks = buffers + parity * mask_bytes;
+        // the next item's chunk and rows are in flight while this one is matched
+        const uint32_t next = item + gridDim.x;
+        const ItemRows cur_rows = rows;
+        uint2 nit = it;
+        if (next < total) {
+            nit = items[next];
+            load_item_rows(m, nit.x, rows);
+            stage_async(m, nit.y, buffers + (parity ^ 1) * mask_bytes, wave, lane);
+        }
         const uint32_t base0 = (s * kStageWords) << 6;   // first u index of the chunk
         uint32_t cur = 0xffffffffu, acc1 = 0, acc2 = 0;
         uint32_t queued = 0;
@@ -327,7 +368,8 @@ __global__ __launch_bounds__(kStagedThreads) void k_match_staged(MatchArgs m, co
             const uint32_t j = j0 + threadIdx.x + kStagedThreads * t;
             if (j0 + (threadIdx.x - lane) + kStagedThreads * t < j1) {   // the wave has rows in the piece
                 const bool live = j < j1;
-                const uint32_t left = h0[t].x, right = h0[t].y, tile = h0[t].z;
+                const uint4 h0 = cur_rows.h0[t], h1 = cur_rows.h1[t];
+                const uint32_t tile = h0.z;
                 if (live && tile != cur) {
                     if (cur != 0xffffffffu) {
                         if (acc1) staged_add(m, S, t0, cur, 1, acc1);
@@ -336,49 +378,39 @@ __global__ __launch_bounds__(kStagedThreads) void k_match_staged(MatchArgs m, co
                     cur = tile;
                     acc1 = acc2 = 0;
                 }
+                // range of the row's u indices within the chunk, clamped to the chunk's words
+                const int dl = live ? (int)(h0.x - base0) : 0, dr = live ? (int)(h0.y - base0) : 0;
                 uint64_t b[kStageWords];
 #pragma unroll
                 for (int ww = 0; ww < kStageWords; ++ww) {
-                    const uint32_t base = base0 + 64 * ww;
-                    b[ww] = 0;
-                    if (live && left < right && base < right && base + 64 > left) {
-                        uint64_t x = ~0ull;
-                        if (base < left) x &= ~0ull << (left - base);
-                        if (right < base + 64) x &= (1ull << (right - base)) - 1ull;
-                        b[ww] = x;
-                    }
+                    const int lo = min(max(dl - 64 * ww, 0), 64), hi = min(max(dr - 64 * ww, 0), 64);
+                    b[ww] = hi > lo ? (~0ull << lo) & (~0ull >> (64 - hi)) : 0ull;
                 }
-                // every bitset row of this v is read before any is used: independent LDS reads
-                uint4 mw[kStagedFilters][2];
+                // each half (two words) of every filter's row is read before any is used
 #pragma unroll
-                for (int k = 0; k < kStagedFilters; ++k) {
-                    const uint32_t half = k < 2 ? h1[t].x : k < 4 ? h1[t].y : k < 6 ? h1[t].z : h1[t].w;
-                    const uint32_t r = (half >> (16 * (k & 1))) & 0xffffu;
-                    mw[k][0] = mw[k][1] = make_uint4(~0u, ~0u, ~0u, ~0u);
-                    if ((uint32_t)k < m.nbits) {
-                        const uint32_t row = m.row_base[k] + r, swap = stage_swizzle(row);
-                        const uint4* src = (const uint4*)(S.masks + row * kStageWords);
-                        mw[k][0] = src[swap];
-                        mw[k][1] = src[swap ^ 1];
+                for (int h = 0; h < 2; ++h) {
+                    uint4 mw[kF];
+#pragma unroll
+                    for (int k = 0; k < kF; ++k) {
+                        const uint32_t pair = k < 2 ? h1.x : k < 4 ? h1.y : k < 6 ? h1.z : h1.w;
+                        const uint32_t off = ((pair >> (16 * (k & 1))) & 0xffffu) ^ (16u * h);
+                        mw[k] = *(const uint4*)(masks + off);
                     }
-                }
 #pragma unroll
-                for (int k = 0; k < kStagedFilters; ++k) {
-                    if ((uint32_t)k < m.nbits) {
-                        b[0] &= ((uint64_t)mw[k][0].y << 32) | mw[k][0].x;
-                        b[1] &= ((uint64_t)mw[k][0].w << 32) | mw[k][0].z;
-                        b[2] &= ((uint64_t)mw[k][1].y << 32) | mw[k][1].x;
-                        b[3] &= ((uint64_t)mw[k][1].w << 32) | mw[k][1].z;
-                        if ((k & 3) == 3 || (uint32_t)k + 1 == m.nres) {
-                            const uint32_t n = __popcll(b[0]) + __popcll(b[1]) + __popcll(b[2]) + __popcll(b[3]);
+                    for (int k = 0; k < kF; ++k) {
+                        b[2 * h] &= ((uint64_t)mw[k].y << 32) | mw[k].x;
+                        b[2 * h + 1] &= ((uint64_t)mw[k].w << 32) | mw[k].z;
+                        if ((k & 3) == 3 || (kLast && k == kF - 1)) {
+                            const uint32_t n = __popcll(b[2 * h]) + __popcll(b[2 * h + 1]);
                             if (k < 4) acc1 += n;
                             else acc2 += n;
                         }
                     }
                 }
+                const uint32_t tag = ((j - j0) << 8);
 #pragma unroll
                 for (int ww = 0; ww < kStageWords; ++ww)
-                    staged_enqueue(S, lane, b[ww], j, base0 + 64 * ww, queued, region, m.item_cap, fill);
+                    staged_enqueue(S, lane, b[ww], tag + 64 * ww, queued, region, m.item_cap, fill);
             }
         }
         if (queued) {
@@ -398,7 +430,42 @@ __global__ __launch_bounds__(kStagedThreads) void k_match_staged(MatchArgs m, co
                 tilecnt[t] = 0;
             }
         }
+        if (next >= total) break;
+        item = next;
+        it = nit;
+        parity ^= 1;
     }
+}
+
+#define PCUBE_STAGED(F)                                                                               \
+    template __global__ void k_match_staged<F, false>(MatchArgs, const uint2*, uint32_t, const uint32_t*); \
+    template __global__ void k_match_staged<F, true>(MatchArgs, const uint2*, uint32_t, const uint32_t*);
+PCUBE_STAGED(1)
+PCUBE_STAGED(2)
+PCUBE_STAGED(3)
+PCUBE_STAGED(4)
+PCUBE_STAGED(5)
+PCUBE_STAGED(6)
+PCUBE_STAGED(7)
+PCUBE_STAGED(8)
+#undef PCUBE_STAGED
+
+template <bool kLast>
+inline const void* staged_kernel_for(uint32_t nbits) {
+    switch (nbits) {
+        case 1: return (const void*)k_match_staged<1, kLast>;
+        case 2: return (const void*)k_match_staged<2, kLast>;
+        case 3: return (const void*)k_match_staged<3, kLast>;
+        case 4: return (const void*)k_match_staged<4, kLast>;
+        case 5: return (const void*)k_match_staged<5, kLast>;
+        case 6: return (const void*)k_match_staged<6, kLast>;
+        case 7: return (const void*)k_match_staged<7, kLast>;
+        default: return (const void*)k_match_staged<8, kLast>;
+    }
+}
+// The instantiation for a matcher: nbits bitset filters out of nres.
+inline const void* staged_kernel(uint32_t nbits, uint32_t nres) {
+    return nbits == nres ? staged_kernel_for<true>(nbits) : staged_kernel_for<false>(nbits);
 }
 
 // The residual chain over the batch's bitset survivors: wave w of the grid
@@ -406,7 +473,8 @@ __global__ __launch_bounds__(kStagedThreads) void k_match_staged(MatchArgs m, co
 // time.  Group counts of subtiles below kTileSlots are kept per block in
 // LDS.  An item whose survivors exceeded item_cap records its count in
 // cand_max so the host can size the regions and run again.
-__global__ __launch_bounds__(kMatchThreads) void k_chain_staged(MatchArgs m, const uint32_t* nitems, uint32_t items_cap) {
+__global__ __launch_bounds__(kMatchThreads) void k_chain_staged(MatchArgs m, const uint2* items, const uint32_t* nitems,
+                                                                uint32_t items_cap) {
     __shared__ uint32_t allow[kMaxAllowWords];
     __shared__ Modulus mods[kMaxResiduals];
     __shared__ uint32_t tilecnt[kTileSlots * (kMaxResiduals / 4)];
@@ -415,7 +483,7 @@ __global__ __launch_bounds__(kMatchThreads) void k_chain_staged(MatchArgs m, con
     for (uint32_t t = threadIdx.x; t < (uint32_t)kTileSlots * m.ng; t += kMatchThreads) tilecnt[t] = 0;
     __syncthreads();
     const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    const Staged S{nullptr, allow, mods, tilecnt, nullptr};
+    const Staged S{allow, mods, tilecnt, nullptr};
     const uint32_t total = min(*nitems, items_cap);
     const uint32_t waves = gridDim.x * kMatchWaves;
     for (uint32_t item = blockIdx.x * kMatchWaves + wave; item < total; item += waves) {
@@ -424,11 +492,13 @@ __global__ __launch_bounds__(kMatchThreads) void k_chain_staged(MatchArgs m, con
             if (lane == 0) atomicMax((unsigned long long*)m.cand_max, (unsigned long long)n);
             n = m.item_cap;
         }
-        const uint2* region = m.cand + (uint64_t)item * m.item_cap;
+        const uint2 it = items[item];
+        const uint32_t j0 = it.x * kPieceRows, u0 = (it.y * kStageWords) << 6;
+        const uint32_t* region = m.cand + (uint64_t)item * m.item_cap;
         for (uint32_t first = 0; first < n; first += kWave) {
             const uint32_t count = min(n - first, (uint32_t)kWave);
-            const uint2 e = lane < count ? region[first + lane] : make_uint2(0, 0);
-            staged_chain(m, S, lane, count, e, 0);
+            const uint32_t e = lane < count ? region[first + lane] : 0u;
+            staged_chain(m, S, lane, count, j0 + (e >> 8), u0 + (e & 0xffu));
         }
     }
     __syncthreads();

@@ -370,7 +370,7 @@ class MatchEngine:
 
     def run(self, us: DeviceArray, vs: DeviceArray, v_first: int, tile_width: int, ntiles: int, cap: int,
             engine: int, overflow_ok: bool = False) -> tuple[np.ndarray, list[list[int]], list[float]]:
-        """Counters [ntiles, ng+3], emitted values per subtile, (prefilter, match) seconds.
+        """Counters [ntiles, ng+3], emitted values per subtile, (prefilter, match, intersection kernel) seconds.
 
         With overflow_ok an output overflow returns normally so the caller can
         raise with the reference's wording; other failures raise.
@@ -381,14 +381,14 @@ class MatchEngine:
         if self._values is None or self._values.size < 2 * values_cap:
             self._values = np.empty(2 * values_cap, dtype=np.uint64)
         values = self._values
-        seconds = (ctypes.c_double * 2)()
+        seconds = (ctypes.c_double * 3)()
         self.runtime.check()
         code = native.lib().pcube_match(self.handle, us.address, us.size, vs.address, vs.size, v_first, tile_width,
                                         ntiles, cap, engine, native.ptr64(counters), native.ptr64(values),
                                         values.size // 2, seconds)
         counters = counters.reshape(ntiles, stride)
         if code == native.OVERFLOW and overflow_ok:
-            return counters, [[] for _ in range(ntiles)], [seconds[0], seconds[1]]
+            return counters, [[] for _ in range(ntiles)], [seconds[0], seconds[1], seconds[2]]
         native.check(code, 'pcube_match')
         self.runtime.check()
         per_tile: list[list[int]] = []
@@ -400,7 +400,7 @@ class MatchEngine:
             n = min(n, cap)
             per_tile.append([limbs[2 * i] + (limbs[2 * i + 1] << 64) for i in range(offset, offset + n)])
             offset += n
-        return counters, per_tile, [seconds[0], seconds[1]]
+        return counters, per_tile, [seconds[0], seconds[1], seconds[2]]
 
     def __del__(self) -> None:
         if native._lib is not None and hasattr(self, 'handle'):
