@@ -138,6 +138,17 @@ int pcube_match(int matcher, uint64_t us, uint64_t nu, uint64_t vs, uint64_t nv,
                 uint64_t tile_width, uint32_t ntiles, uint64_t cap, int engine, uint64_t *counters,
                 uint64_t *values, uint64_t values_cap, double *seconds);
 
+/* pcube_match in two halves, so the caller can prepare the host side of the
+ * result row while the device works: pcube_match_start validates and enqueues
+ * the batch (same arguments and meaning as pcube_match) and returns at once;
+ * pcube_match_finish waits for it and fills counters, values and seconds as
+ * pcube_match does, with the same return codes.  One batch per matcher may be
+ * pending; other calls may run in between.  The split of the reference's
+ * MaskedMatcher.run (pseudocube/batch_experiment.py:104-167). */
+int pcube_match_start(int matcher, uint64_t us, uint64_t nu, uint64_t vs, uint64_t nv, uint64_t v_first,
+                      uint64_t tile_width, uint32_t ntiles, uint64_t cap, int engine);
+int pcube_match_finish(int matcher, uint64_t *counters, uint64_t *values, uint64_t values_cap, double *seconds);
+
 /* --- lease pipeline: replaces pseudocube/optimized.py:81-90 ---------------- */
 
 /* Run every compute batch of a lease with one lease-wide window per focus

@@ -133,10 +133,21 @@ def python_witnesses(values: list[int], p: int) -> list[dict[str, object]]:
     return [verify(n, p) for n in values if not is_perfect_cube(n)]
 
 
+def scope_skeleton(ranges: list[tuple[int, int, int, int]]) -> list[dict[str, Any]]:
+    """The per-subtile receipt records of a batch before its counts are known
+    (keys of `pseudocube/batch_experiment.py:149-158`), so the host can build
+    them while the device matches."""
+    firsts = [str(first) for first, _, _, _ in ranges]
+    # consecutive subtiles share a bound: a subtile's end is the next one's first
+    return [{'first': firsts[k],
+             'end': firsts[k + 1] if k + 1 < len(ranges) and ranges[k + 1][0] == end else str(end)}
+            for k, (_, end, _, _) in enumerate(ranges)]
+
+
 def batch_row(config: Config, filter_groups: list[list[int]], ranges: list[tuple[int, int, int, int]],
               bounds: list[tuple[int, int, int, int]], nv: int, counters: np.ndarray, per_tile: list[list[int]],
               output_cap: int, timings: dict[str, float], started: float,
-              witnesses: Any = python_witnesses) -> dict[str, Any]:
+              witnesses: Any = python_witnesses, skeleton: list[dict[str, Any]] | None = None) -> dict[str, Any]:
     """Host collection and verification of one batch (`pseudocube/batch_experiment.py:121-167`).
 
     Raises ArithmeticError on underflow, duplicates, a value failing the
@@ -164,9 +175,9 @@ def batch_row(config: Config, filter_groups: list[list[int]], ranges: list[tuple
     hits = witnesses(all_values, config.p)
     if any(not row['valid_pseudocube'] for row in hits):
         raise ArithmeticError('independent witness verification failed')
-    scopes = []
+    scopes = skeleton if skeleton is not None else scope_skeleton(ranges)
     position = 0
-    for (first, end, _, _), counts, number in zip(ranges, table, emitted):
+    for (first, end, _, _), counts, number, scope in zip(ranges, table, emitted, scopes):
         sha = EMPTY_DIGEST
         if number:
             values = sorted(all_values[position:position + number])
@@ -174,8 +185,10 @@ def batch_row(config: Config, filter_groups: list[list[int]], ranges: list[tuple
                 raise ArithmeticError('output assigned to wrong subtile')
             sha = digest([str(n) for n in values])
             position += number
-        scopes.append({'first': str(first), 'end': str(end), 'valid_focused_candidates': counts[0],
-                       'group_survivors': counts[1:ng + 1], 'local_survivors': number, 'output_sha256': sha})
+        scope['valid_focused_candidates'] = counts[0]
+        scope['group_survivors'] = counts[1:ng + 1]
+        scope['local_survivors'] = number
+        scope['output_sha256'] = sha
     verification_seconds = time.perf_counter() - verification_started
     groups = counters[:, 1:ng + 1].sum(axis=0, dtype=np.uint64).tolist() if table else [0] * ng
     focus_u = opportunities = 0
@@ -216,13 +229,15 @@ class FusedMatcher:
                                                                              batch.width)
         if len(ranges) != len(batch.bounds) or not 0 < output_cap <= 8192:
             raise ValueError('invalid prepared geometry or output capacity')
-        counters, per_tile, seconds = self.engine.run(batch.us, batch.vs, batch.first, batch.width, len(ranges),
-                                                      output_cap, self.engine_id, overflow_ok=True)
+        # the device matches while the host builds the receipt records
+        self.engine.start(batch.us, batch.vs, batch.first, batch.width, len(ranges), output_cap, self.engine_id)
+        skeleton = scope_skeleton(ranges)
+        counters, per_tile, seconds = self.engine.finish(overflow_ok=True)
         self.prefilter_seconds = seconds[0]
         self.intersection_seconds = seconds[2]
         timings = dict(kernel_seconds=seconds[1], focus_enumeration_seconds=batch.prep_seconds)
         row = batch_row(self.config, self.filter_groups, ranges, batch.bounds, int(batch.vs.size), counters,
-                        per_tile, output_cap, timings, started)
+                        per_tile, output_cap, timings, started, skeleton=skeleton)
         row['kernel_registers'] = int(self.kernel['registers'])
         row['kernel_local_bytes'] = int(self.kernel['local_bytes'])
         row['matching_layout'] = self.layout

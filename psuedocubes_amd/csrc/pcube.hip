@@ -608,6 +608,14 @@ int lease_window(Focus& f, uint64_t low, uint64_t high, uint64_t* out, uint64_t 
 // ---------------------------------------------------------------------------
 // Matchers
 
+// A batch enqueued by pcube_match_start and collected by pcube_match_finish.
+struct Pending {
+    bool active = false;
+    uint64_t us = 0, nu = 0, vs = 0, nv = 0, v_first = 0, tile_width = 0, cap = 0, list_cap = 0;
+    uint32_t ntiles = 0;
+    int engine = 0;
+};
+
 struct Matcher {
     uint64_t A = 0, B = 0;
     U128 lo{0, 0}, hi{0, 0};
@@ -615,7 +623,7 @@ struct Matcher {
     uint32_t row_base[kMaxBitsetFilters] = {0, 0, 0, 0};
     uint32_t mask_rows = 0;
     // staged engine: bitset filters, their rows (sum of q), cached chain filters, LDS bytes
-    uint32_t sbits = 0, srows = 0, sncache = 0, slds = 0;
+    uint32_t sbits = 0, srows = 0, sncache = 0;
     uint32_t s_row_base[kStagedFilters] = {}, s_chunk_base[kStagedFilters + 1] = {};
     uint32_t s_pstr_off[kStagedFilters] = {}, s_nbeta[kStagedFilters] = {};
     std::vector<Modulus> mods;
@@ -624,8 +632,10 @@ struct Matcher {
     // survivors of any staged work item that overflowed its region since the last clear
     Buffer rows, masks, rho, counters, list, status, cand, items;
     uint64_t cand_cap = 512;                            // bitset-survivor entries per staged work item
-    // start, bitsets built, intersection kernel start and end, match done
-    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    // start, bitsets built, match done, intersection kernel start and end, download done
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    Pending pending;       // the batch started by pcube_match_start
+    Pinned down;           // its download: [status words, counters]
     uint64_t* list_count() const { return status.as<uint64_t>(); }
     uint32_t* error() const { return (uint32_t*)(status.as<uint64_t>() + 1); }
     ~Matcher() {
@@ -633,6 +643,7 @@ struct Matcher {
         counters.release(); list.release(); status.release(); cand.release(); items.release();
         for (hipEvent_t& e : ev)
             if (e) (void)hipEventDestroy(e);
+        if (down.p) (void)hipHostFree(down.p);
     }
 };
 std::vector<std::unique_ptr<Matcher>> g_matchers;
@@ -705,7 +716,7 @@ int staged_enqueue(Matcher& mt, MatchArgs m, hipEvent_t ev_pre, hipEvent_t ev_ma
             void* args[] = {&m, &items_arg, (void*)&items_cap, &nitems_arg};
             if (ev_k0) HIP_TRY(hipEventRecord(ev_k0, g.stream));
             HIP_TRY(hipLaunchKernel(staged_kernel(mt.sbits, mt.nres), dim3((uint32_t)g.compute_units),
-                                    dim3(kStagedThreads), args, mt.slds, g.stream));
+                                    dim3(kStagedThreads), args, 0, g.stream));
             if (ev_k1) HIP_TRY(hipEventRecord(ev_k1, g.stream));
         }
         hipLaunchKernelGGL(k_chain_staged, dim3(2 * (uint32_t)g.compute_units), dim3(kMatchThreads), 0, g.stream, m,
@@ -717,7 +728,7 @@ int staged_enqueue(Matcher& mt, MatchArgs m, hipEvent_t ev_pre, hipEvent_t ev_ma
 
 // Whether one batch of this geometry runs on the staged engine.
 bool staged_fits(const Matcher& mt, uint64_t nu, uint64_t nv) {
-    if (!mt.sbits || mt.slds > g.lds_per_block) return false;
+    if (!mt.sbits || kStagedLdsBytes > g.lds_per_block) return false;
     const uint64_t words = (nu + 63) / 64, nstage = (words + kStageWords - 1) / kStageWords;
     const uint64_t npieces = (nv + kPieceRows - 1) / kPieceRows;
     return npieces * nstage <= (1ull << 22) && nu < (1ull << 31);
@@ -793,6 +804,27 @@ int match_enqueue(Matcher& mt, const uint64_t* us, uint64_t nu, const uint64_t* 
     }
     if (ev_k1) HIP_TRY(hipEventRecord(ev_k1, g.stream));
     if (ev_match) HIP_TRY(hipEventRecord(ev_match, g.stream));
+    return PCUBE_OK;
+}
+
+// Enqueue the pending batch of a matcher: clear its counters and status words,
+// run the engine, download [status words, counters] to the matcher's pinned
+// buffer and record ev[5] after the download.
+int match_launch(Matcher& mt) {
+    const Pending& pd = mt.pending;
+    const uint64_t nc = (uint64_t)pd.ntiles * (mt.ng + 3);
+    hipEvent_t e0 = mt.ev[0], e1 = mt.ev[1], e2 = mt.ev[2], k0 = mt.ev[3], k1 = mt.ev[4];
+    TRY(fill64(mt.counters.as<uint64_t>(), nc, 0));
+    TRY(fill64(mt.status.as<uint64_t>(), 4, 0));
+    HIP_TRY(hipEventRecord(e0, g.stream));
+    TRY(match_enqueue(mt, (const uint64_t*)pd.us, pd.nu, (const uint64_t*)pd.vs, pd.nv, pd.v_first, pd.tile_width,
+                      pd.ntiles, pd.cap, pd.engine, mt.counters.as<uint64_t>(), 0, pd.list_cap, e1, e2, k0, k1));
+    if (pd.nv == 0)
+        for (hipEvent_t e : {e1, k0, k1, e2}) HIP_TRY(hipEventRecord(e, g.stream));
+    uint64_t* hd = mt.down.as<uint64_t>();
+    HIP_TRY(hipMemcpyAsync(hd, mt.status.p, 4 * 8, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipMemcpyAsync(hd + 4, mt.counters.p, nc * 8, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipEventRecord(mt.ev[5], g.stream));
     return PCUBE_OK;
 }
 
@@ -1010,13 +1042,7 @@ int pcube_init(int device, uint64_t memory_cap, double fraction, uint64_t* cap, 
         int lds = 0;
         HIP_TRY(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
         g.lds_per_block = (uint32_t)std::max(lds, 0);
-        // the staged engine uses more than 64 KB of dynamic LDS when the device allows it
-        for (uint32_t f = 1; f <= 2 * (uint32_t)kStagedFilters; ++f)
-            if (hipFuncSetAttribute(staged_kernel((f + 1) / 2, f % 2 ? (f + 1) / 2 : 128), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)g.lds_per_block) != hipSuccess) {
-                (void)hipGetLastError();
-                g.lds_per_block = std::min<uint32_t>(g.lds_per_block, 65536);
-            }
+        // the staged engine's block holds kStagedLdsBytes of static LDS
         g.device = device;
     }
     size_t fr = 0, tot = 0;
@@ -1399,7 +1425,6 @@ int pcube_matcher_create(uint64_t A, uint64_t B, uint64_t lo0, uint64_t lo1, uin
     }
     for (uint32_t k = mt->sbits; k < (uint32_t)kStagedFilters; ++k) mt->s_chunk_base[k + 1] = mt->s_chunk_base[k];
     mt->sncache = std::min<uint32_t>(nres - mt->sbits, kCachedResidues);
-    mt->slds = staged_lds_bytes(mt->srows, mt->allow_words, nres, mt->ng);
     if (mt->sbits) {
         TRY(mt->d_pstr.ensure(pstr.size() * 4));
         HIP_TRY(hipMemcpyAsync(mt->d_pstr.p, pstr.data(), pstr.size() * 4, hipMemcpyHostToDevice, g.stream));
@@ -1423,61 +1448,72 @@ int pcube_matcher_destroy(int matcher) {
     return PCUBE_OK;
 }
 
-int pcube_match(int matcher, uint64_t us, uint64_t nu, uint64_t vs, uint64_t nv, uint64_t v_first,
-                uint64_t tile_width, uint32_t ntiles, uint64_t cap, int engine, uint64_t* counters, uint64_t* values,
-                uint64_t values_cap, double* seconds) {
+int pcube_match_start(int matcher, uint64_t us, uint64_t nu, uint64_t vs, uint64_t nv, uint64_t v_first,
+                      uint64_t tile_width, uint32_t ntiles, uint64_t cap, int engine) {
     std::lock_guard<std::recursive_mutex> lock(g_mu);
     Matcher* mt = matcher_at(matcher);
     if (!mt) return fail(PCUBE_INVALID, "unknown matcher");
     if (ntiles == 0 || ntiles > 65536 || tile_width == 0 || (engine != 0 && engine != 1))
         return fail(PCUBE_INVALID, "invalid tile geometry or engine");
-    const uint64_t stride = mt->ng + 3, nc = (uint64_t)ntiles * stride;
-    const uint64_t list_cap = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)ntiles * cap, 1ull << 22));
+    Pending& pd = mt->pending;
+    pd = Pending{};
+    pd.us = us; pd.nu = nu; pd.vs = vs; pd.nv = nv; pd.v_first = v_first; pd.tile_width = tile_width;
+    pd.ntiles = ntiles; pd.cap = cap; pd.engine = engine;
+    const uint64_t nc = (uint64_t)ntiles * (mt->ng + 3);
+    pd.list_cap = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)ntiles * cap, 1ull << 22));
     TRY(mt->counters.ensure(nc * 8));
-    TRY(mt->list.ensure(list_cap * 24));
+    TRY(mt->list.ensure(pd.list_cap * 24));
+    TRY(mt->down.ensure((nc + 4) * 8));
     for (hipEvent_t& e : mt->ev)
         if (!e) HIP_TRY(hipEventCreate(&e));
-    hipEvent_t e0 = mt->ev[0], e1 = mt->ev[1], e2 = mt->ev[2], k0 = mt->ev[3], k1 = mt->ev[4];
-    // one pinned download: [status words, counters]; a bitset-survivor list that
-    // overflowed is sized from its count and the batch runs again
-    TRY(g_down_host.ensure((nc + 4) * 8));
-    uint64_t* hd = g_down_host.as<uint64_t>();
-    int rc = PCUBE_OK;
-    for (int attempt = 0; attempt < 2; ++attempt) {
-        TRY(fill64(mt->counters.as<uint64_t>(), nc, 0));
-        TRY(fill64(mt->status.as<uint64_t>(), 4, 0));
-        HIP_TRY(hipEventRecord(e0, g.stream));
-        rc = match_enqueue(*mt, (const uint64_t*)us, nu, (const uint64_t*)vs, nv, v_first, tile_width, ntiles, cap,
-                           engine, mt->counters.as<uint64_t>(), 0, list_cap, e1, e2, k0, k1);
-        if (rc) return rc;
-        if (nv == 0) {
-            for (hipEvent_t e : {e1, k0, k1, e2}) HIP_TRY(hipEventRecord(e, g.stream));
-        }
-        HIP_TRY(hipMemcpyAsync(hd, mt->status.p, 4 * 8, hipMemcpyDeviceToHost, g.stream));
-        HIP_TRY(hipMemcpyAsync(hd + 4, mt->counters.p, nc * 8, hipMemcpyDeviceToHost, g.stream));
-        HIP_TRY(hipStreamSynchronize(g.stream));
-        if (hd[3] <= mt->cand_cap) break;
-        if (hd[3] > kMaxItemCandidates) engine = kUnstagedBitsets;
+    TRY(match_launch(*mt));
+    pd.active = true;
+    return PCUBE_OK;
+}
+
+int pcube_match_finish(int matcher, uint64_t* counters, uint64_t* values, uint64_t values_cap, double* seconds) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Matcher* mt = matcher_at(matcher);
+    if (!mt) return fail(PCUBE_INVALID, "unknown matcher");
+    Pending& pd = mt->pending;
+    if (!pd.active) return fail(PCUBE_INVALID, "no batch started on this matcher");
+    pd.active = false;
+    const uint64_t nc = (uint64_t)pd.ntiles * (mt->ng + 3);
+    uint64_t* hd = mt->down.as<uint64_t>();
+    HIP_TRY(hipEventSynchronize(mt->ev[5]));
+    if (hd[3] > mt->cand_cap) {
+        // a bitset-survivor region overflowed: size the regions from its count and run again
+        if (hd[3] > kMaxItemCandidates) pd.engine = kUnstagedBitsets;
         else mt->cand_cap = (hd[3] + 63) / 64 * 64;
+        TRY(match_launch(*mt));
+        HIP_TRY(hipEventSynchronize(mt->ev[5]));
     }
     const uint64_t listed = hd[0];
     const uint32_t err = (uint32_t)hd[1];
     std::vector<uint64_t> host(hd + 4, hd + 4 + nc);
     if (seconds) {
         float a = 0, b = 0, c = 0;
-        HIP_TRY(hipEventElapsedTime(&a, e0, e1));
-        HIP_TRY(hipEventElapsedTime(&b, e1, e2));
-        HIP_TRY(hipEventElapsedTime(&c, k0, k1));
+        HIP_TRY(hipEventElapsedTime(&a, mt->ev[0], mt->ev[1]));
+        HIP_TRY(hipEventElapsedTime(&b, mt->ev[1], mt->ev[2]));
+        HIP_TRY(hipEventElapsedTime(&c, mt->ev[3], mt->ev[4]));
         seconds[0] = a / 1000.0;
         seconds[1] = b / 1000.0;
         seconds[2] = c / 1000.0;
     }
     memcpy(counters, host.data(), nc * 8);
-    rc = batch_status(*mt, host, ntiles, cap, err);
+    int rc = batch_status(*mt, host, pd.ntiles, pd.cap, err);
     if (rc) return rc;
     int status = PCUBE_OK;
-    TRY(collect_values(*mt, ntiles, host, cap, list_cap, values, values_cap, &status, listed));
+    TRY(collect_values(*mt, pd.ntiles, host, pd.cap, pd.list_cap, values, values_cap, &status, listed));
     return status;
+}
+
+int pcube_match(int matcher, uint64_t us, uint64_t nu, uint64_t vs, uint64_t nv, uint64_t v_first,
+                uint64_t tile_width, uint32_t ntiles, uint64_t cap, int engine, uint64_t* counters, uint64_t* values,
+                uint64_t values_cap, double* seconds) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    TRY(pcube_match_start(matcher, us, nu, vs, nv, v_first, tile_width, ntiles, cap, engine));
+    return pcube_match_finish(matcher, counters, values, values_cap, seconds);
 }
 
 int pcube_lease(int matcher, int focus_u, int focus_v, uint32_t nbatches, const uint32_t* batch_tiles,

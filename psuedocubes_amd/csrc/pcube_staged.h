@@ -134,14 +134,6 @@ struct Staged {
     uint32_t* queue;         // this wave's survivors, (row offset << 8) | u offset within the item
 };
 
-__host__ __device__ inline uint32_t staged_align(uint32_t bytes) { return (bytes + 31u) & ~31u; }
-// Bytes of dynamic LDS: two mask buffers, tables, counters, queues, fill counter.
-__host__ __device__ inline uint32_t staged_lds_bytes(uint32_t stage_rows, uint32_t allow_words, uint32_t nres,
-                                                     uint32_t ng) {
-    return 2 * staged_align(stage_rows * kStageWords * 8) + staged_align(allow_words * 4) +
-           staged_align(nres * (uint32_t)sizeof(Modulus)) + staged_align(kTileSlots * ng * 4) +
-           kStagedWaves * kQueue * 4 + 32;
-}
 
 // Add n to group g (1-based) of a subtile: LDS when the subtile is within the
 // block's window, else the global counters.
@@ -305,6 +297,120 @@ __device__ __forceinline__ void stage_async(const MatchArgs& m, uint32_t s, uint
     }
 }
 
+// Static LDS of the staged match (one block per CU): two mask buffers, so the
+// next item's chunk can stream in while the current one is read, plus the
+// filter tables, per-subtile counters, survivor queues and the item's fill
+// counter.  Distinct arrays let the compiler see that reads of one buffer do
+// not wait for the direct-to-LDS copies into the other.
+constexpr int kStageBufferUnits = kMaxStageRows * kStageWords * 8 / 16;
+constexpr uint32_t kStagedLdsBytes = 2 * kStageBufferUnits * 16 + kMaxAllowWords * 4 + kMaxResiduals * 32 +
+                                     kTileSlots * (kMaxResiduals / 4) * 4 + kStagedWaves * kQueue * 4 + 16;
+
+struct ItemState {
+    uint32_t item, total;
+    uint2 it;
+    ItemRows rows;
+};
+
+// Match one work item reading chunk buffer `cur` while the next item's chunk
+// streams into `nxt`.  Returns false when the block has no further item.
+template <int kF, bool kLast>
+__device__ __forceinline__ bool staged_item(const MatchArgs& m, const uint2* items, const Staged& S, ItemState& st,
+                                            const uint4 (&cur)[kStageBufferUnits], uint4 (&nxt)[kStageBufferUnits],
+                                            uint32_t* fill, uint32_t lane, uint32_t wave) {
+    const uint32_t p = st.it.x, s = st.it.y;
+    const uint32_t j0 = p * kPieceRows, j1 = min(m.nv, j0 + kPieceRows);
+    const uint32_t t0 = m.rows[j0].tile;
+    __builtin_amdgcn_s_waitcnt(0);   // this wave's part of the chunk and its row headers have arrived
+    if (threadIdx.x == 0) *fill = 0;
+    uint32_t* region = m.cand + (uint64_t)st.item * m.item_cap;
+    __syncthreads();
+    // the next item's chunk and rows are in flight while this one is matched
+    const uint32_t next = st.item + gridDim.x;
+    const ItemRows cur_rows = st.rows;
+    uint2 nit = st.it;
+    if (next < st.total) {
+        nit = items[next];
+        load_item_rows(m, nit.x, st.rows);
+        stage_async(m, nit.y, (uint8_t*)nxt, wave, lane);
+    }
+    const uint8_t* masks = (const uint8_t*)cur;
+    const uint32_t base0 = (s * kStageWords) << 6;   // first u index of the chunk
+    uint32_t tile_cur = 0xffffffffu, acc1 = 0, acc2 = 0;
+    uint32_t queued = 0;
+#pragma unroll
+    for (int t = 0; t < kRowsPerLane; ++t) {
+        const uint32_t j = j0 + threadIdx.x + kStagedThreads * t;
+        if (j0 + (threadIdx.x - lane) + kStagedThreads * t < j1) {   // the wave has rows in the piece
+            const bool live = j < j1;
+            const uint4 h0 = cur_rows.h0[t], h1 = cur_rows.h1[t];
+            const uint32_t tile = h0.z;
+            if (live && tile != tile_cur) {
+                if (tile_cur != 0xffffffffu) {
+                    if (acc1) staged_add(m, S, t0, tile_cur, 1, acc1);
+                    if (acc2) staged_add(m, S, t0, tile_cur, 2, acc2);
+                }
+                tile_cur = tile;
+                acc1 = acc2 = 0;
+            }
+            // range of the row's u indices within the chunk, clamped to the chunk's words
+            const int dl = live ? (int)(h0.x - base0) : 0, dr = live ? (int)(h0.y - base0) : 0;
+            uint64_t b[kStageWords];
+#pragma unroll
+            for (int ww = 0; ww < kStageWords; ++ww) {
+                const int lo = min(max(dl - 64 * ww, 0), 64), hi = min(max(dr - 64 * ww, 0), 64);
+                b[ww] = hi > lo ? (~0ull << lo) & (~0ull >> (64 - hi)) : 0ull;
+            }
+            // each half (two words) of every filter's row is read before any is used
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                uint4 mw[kF];
+#pragma unroll
+                for (int k = 0; k < kF; ++k) {
+                    const uint32_t pair = k < 2 ? h1.x : k < 4 ? h1.y : k < 6 ? h1.z : h1.w;
+                    const uint32_t off = ((pair >> (16 * (k & 1))) & 0xffffu) ^ (16u * h);
+                    mw[k] = *(const uint4*)(masks + off);
+                }
+#pragma unroll
+                for (int k = 0; k < kF; ++k) {
+                    b[2 * h] &= ((uint64_t)mw[k].y << 32) | mw[k].x;
+                    b[2 * h + 1] &= ((uint64_t)mw[k].w << 32) | mw[k].z;
+                    if ((k & 3) == 3 || (kLast && k == kF - 1)) {
+                        const uint32_t n = __popcll(b[2 * h]) + __popcll(b[2 * h + 1]);
+                        if (k < 4) acc1 += n;
+                        else acc2 += n;
+                    }
+                }
+            }
+            const uint32_t tag = ((j - j0) << 8);
+#pragma unroll
+            for (int ww = 0; ww < kStageWords; ++ww)
+                staged_enqueue(S, lane, b[ww], tag + 64 * ww, queued, region, m.item_cap, fill);
+        }
+    }
+    if (queued) {
+        wave_sync();
+        cand_flush(S, lane, queued, region, m.item_cap, fill);
+    }
+    if (tile_cur != 0xffffffffu) {
+        if (acc1) staged_add(m, S, t0, tile_cur, 1, acc1);
+        if (acc2) staged_add(m, S, t0, tile_cur, 2, acc2);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) m.item_count[st.item] = *fill;
+    for (uint32_t t = threadIdx.x; t < (uint32_t)kTileSlots * m.ng; t += kStagedThreads) {
+        const uint32_t n = S.tilecnt[t];
+        if (n) {
+            atomicAdd(&m.counters[(uint64_t)(t0 + t / m.ng) * (m.ng + 3) + t % m.ng + 1], (uint64_t)n);
+            S.tilecnt[t] = 0;
+        }
+    }
+    if (next >= st.total) return false;
+    st.item = next;
+    st.it = nit;
+    return true;
+}
+
 // Persistent staged match: block b takes work items b, b + grid, ...  For an
 // item (piece p, chunk s) thread l intersects the staged chunk with rows
 // j0 + l + 1024 t of the piece; the next item's chunk and row headers are in
@@ -315,125 +421,29 @@ __device__ __forceinline__ void stage_async(const MatchArgs& m, uint32_t s, uint
 template <int kF, bool kLast>
 __global__ __launch_bounds__(kStagedThreads) void k_match_staged(MatchArgs m, const uint2* items, uint32_t items_cap,
                                                                  const uint32_t* nitems) {
-    extern __shared__ __align__(32) uint8_t smem[];
-    const uint32_t mask_bytes = staged_align(m.stage_rows * kStageWords * 8);
-    uint8_t* buffers = smem;   // two mask buffers of mask_bytes
-    uint8_t* at = smem + 2 * mask_bytes;
-    uint32_t* allow = (uint32_t*)at;
-    at += staged_align(m.allow_words * 4);
-    Modulus* mods = (Modulus*)at;
-    at += staged_align(m.nres * (uint32_t)sizeof(Modulus));
-    uint32_t* tilecnt = (uint32_t*)at;
-    at += staged_align(kTileSlots * m.ng * 4);
-    uint32_t* queues = (uint32_t*)at;
-    at += kStagedWaves * kQueue * 4;
-    uint32_t* fill = (uint32_t*)at;   // survivors of the current item
+    __shared__ uint4 buffer0[kStageBufferUnits];
+    __shared__ uint4 buffer1[kStageBufferUnits];
+    __shared__ uint32_t allow[kMaxAllowWords];
+    __shared__ Modulus mods[kMaxResiduals];
+    __shared__ uint32_t tilecnt[kTileSlots * (kMaxResiduals / 4)];
+    __shared__ uint32_t queues[kStagedWaves * kQueue];
+    __shared__ uint32_t fill;   // survivors of the current item
     const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    const uint32_t total = min(*nitems, items_cap);
-    uint32_t item = blockIdx.x;
-    if (item >= total) return;
+    ItemState st;
+    st.total = min(*nitems, items_cap);
+    st.item = blockIdx.x;
+    if (st.item >= st.total) return;
     // allow and the Modulus table are whole 16-byte units (both buffers are allocated so)
     stage_copy<kStagedThreads, 1>((uint4*)allow, (const uint4*)m.allow, (m.allow_words + 3) / 4);
     stage_copy<kStagedThreads, 1>((uint4*)mods, (const uint4*)m.mods, m.nres * (uint32_t)sizeof(Modulus) / 16);
     for (uint32_t t = threadIdx.x; t < (uint32_t)kTileSlots * m.ng; t += kStagedThreads) tilecnt[t] = 0;
     const Staged S{allow, mods, tilecnt, queues + wave * kQueue};
-    uint2 it = items[item];
-    ItemRows rows;
-    load_item_rows(m, it.x, rows);
-    stage_async(m, it.y, buffers, wave, lane);
-    uint32_t parity = 0;
+    st.it = items[st.item];
+    load_item_rows(m, st.it.x, st.rows);
+    stage_async(m, st.it.y, (uint8_t*)buffer0, wave, lane);
     for (;;) {
-        const uint32_t p = it.x, s = it.y;
-        const uint32_t j0 = p * kPieceRows, j1 = min(m.nv, j0 + kPieceRows);
-        const uint32_t t0 = m.rows[j0].tile;
-        __builtin_amdgcn_s_waitcnt(0);   // this wave's part of the chunk and its row headers have arrived
-        if (threadIdx.x == 0) *fill = 0;
-        uint32_t* region = m.cand + (uint64_t)item * m.item_cap;
-        __syncthreads();
-        const uint8_t* masks = buffers + parity * mask_bytes;
-        // the next item's chunk and rows are in flight while this one is matched
-        const uint32_t next = item + gridDim.x;
-        const ItemRows cur_rows = rows;
-        uint2 nit = it;
-        if (next < total) {
-            nit = items[next];
-            load_item_rows(m, nit.x, rows);
-            stage_async(m, nit.y, buffers + (parity ^ 1) * mask_bytes, wave, lane);
-        }
-        const uint32_t base0 = (s * kStageWords) << 6;   // first u index of the chunk
-        uint32_t cur = 0xffffffffu, acc1 = 0, acc2 = 0;
-        uint32_t queued = 0;
-#pragma unroll
-        for (int t = 0; t < kRowsPerLane; ++t) {
-            const uint32_t j = j0 + threadIdx.x + kStagedThreads * t;
-            if (j0 + (threadIdx.x - lane) + kStagedThreads * t < j1) {   // the wave has rows in the piece
-                const bool live = j < j1;
-                const uint4 h0 = cur_rows.h0[t], h1 = cur_rows.h1[t];
-                const uint32_t tile = h0.z;
-                if (live && tile != cur) {
-                    if (cur != 0xffffffffu) {
-                        if (acc1) staged_add(m, S, t0, cur, 1, acc1);
-                        if (acc2) staged_add(m, S, t0, cur, 2, acc2);
-                    }
-                    cur = tile;
-                    acc1 = acc2 = 0;
-                }
-                // range of the row's u indices within the chunk, clamped to the chunk's words
-                const int dl = live ? (int)(h0.x - base0) : 0, dr = live ? (int)(h0.y - base0) : 0;
-                uint64_t b[kStageWords];
-#pragma unroll
-                for (int ww = 0; ww < kStageWords; ++ww) {
-                    const int lo = min(max(dl - 64 * ww, 0), 64), hi = min(max(dr - 64 * ww, 0), 64);
-                    b[ww] = hi > lo ? (~0ull << lo) & (~0ull >> (64 - hi)) : 0ull;
-                }
-                // each half (two words) of every filter's row is read before any is used
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    uint4 mw[kF];
-#pragma unroll
-                    for (int k = 0; k < kF; ++k) {
-                        const uint32_t pair = k < 2 ? h1.x : k < 4 ? h1.y : k < 6 ? h1.z : h1.w;
-                        const uint32_t off = ((pair >> (16 * (k & 1))) & 0xffffu) ^ (16u * h);
-                        mw[k] = *(const uint4*)(masks + off);
-                    }
-#pragma unroll
-                    for (int k = 0; k < kF; ++k) {
-                        b[2 * h] &= ((uint64_t)mw[k].y << 32) | mw[k].x;
-                        b[2 * h + 1] &= ((uint64_t)mw[k].w << 32) | mw[k].z;
-                        if ((k & 3) == 3 || (kLast && k == kF - 1)) {
-                            const uint32_t n = __popcll(b[2 * h]) + __popcll(b[2 * h + 1]);
-                            if (k < 4) acc1 += n;
-                            else acc2 += n;
-                        }
-                    }
-                }
-                const uint32_t tag = ((j - j0) << 8);
-#pragma unroll
-                for (int ww = 0; ww < kStageWords; ++ww)
-                    staged_enqueue(S, lane, b[ww], tag + 64 * ww, queued, region, m.item_cap, fill);
-            }
-        }
-        if (queued) {
-            wave_sync();
-            cand_flush(S, lane, queued, region, m.item_cap, fill);
-        }
-        if (cur != 0xffffffffu) {
-            if (acc1) staged_add(m, S, t0, cur, 1, acc1);
-            if (acc2) staged_add(m, S, t0, cur, 2, acc2);
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) m.item_count[item] = *fill;
-        for (uint32_t t = threadIdx.x; t < (uint32_t)kTileSlots * m.ng; t += kStagedThreads) {
-            const uint32_t n = tilecnt[t];
-            if (n) {
-                atomicAdd(&m.counters[(uint64_t)(t0 + t / m.ng) * (m.ng + 3) + t % m.ng + 1], (uint64_t)n);
-                tilecnt[t] = 0;
-            }
-        }
-        if (next >= total) break;
-        item = next;
-        it = nit;
-        parity ^= 1;
+        if (!staged_item<kF, kLast>(m, items, S, st, buffer0, buffer1, &fill, lane, wave)) break;
+        if (!staged_item<kF, kLast>(m, items, S, st, buffer1, buffer0, &fill, lane, wave)) break;
     }
 }
 

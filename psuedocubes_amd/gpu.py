@@ -375,6 +375,19 @@ class MatchEngine:
         With overflow_ok an output overflow returns normally so the caller can
         raise with the reference's wording; other failures raise.
         """
+        self.start(us, vs, v_first, tile_width, ntiles, cap, engine)
+        return self.finish(overflow_ok)
+
+    def start(self, us: DeviceArray, vs: DeviceArray, v_first: int, tile_width: int, ntiles: int, cap: int,
+              engine: int) -> None:
+        """Enqueue one batch on the device and return at once; finish() collects it."""
+        self.runtime.check()
+        native.check(native.lib().pcube_match_start(self.handle, us.address, us.size, vs.address, vs.size, v_first,
+                                                    tile_width, ntiles, cap, engine), 'pcube_match_start')
+        self._pending = (ntiles, cap)
+
+    def finish(self, overflow_ok: bool = False) -> tuple[np.ndarray, list[list[int]], list[float]]:
+        ntiles, cap = self._pending
         stride = self.ng + 3
         counters = np.zeros(ntiles * stride, dtype=np.uint64)
         values_cap = min(max(1, ntiles * cap), 1 << 22)
@@ -382,10 +395,8 @@ class MatchEngine:
             self._values = np.empty(2 * values_cap, dtype=np.uint64)
         values = self._values
         seconds = (ctypes.c_double * 3)()
-        self.runtime.check()
-        code = native.lib().pcube_match(self.handle, us.address, us.size, vs.address, vs.size, v_first, tile_width,
-                                        ntiles, cap, engine, native.ptr64(counters), native.ptr64(values),
-                                        values.size // 2, seconds)
+        code = native.lib().pcube_match_finish(self.handle, native.ptr64(counters), native.ptr64(values),
+                                               values.size // 2, seconds)
         counters = counters.reshape(ntiles, stride)
         if code == native.OVERFLOW and overflow_ok:
             return counters, [[] for _ in range(ntiles)], [seconds[0], seconds[1], seconds[2]]

@@ -45,6 +45,8 @@ SIGNATURES: dict[str, list[Any]] = {
     'pcube_matcher_create': [u64, u64, u64, u64, u64, u64, P_u32, u32, P_int],
     'pcube_matcher_destroy': [c_int],
     'pcube_match': [c_int, u64, u64, u64, u64, u64, u64, u32, u64, c_int, P_u64, P_u64, u64, P_dbl],
+    'pcube_match_start': [c_int, u64, u64, u64, u64, u64, u64, u32, u64, c_int],
+    'pcube_match_finish': [c_int, P_u64, P_u64, u64, P_dbl],
     'pcube_lease': [c_int, c_int, c_int, u32, P_u32, P_u64, u64, u64, u64, P_u64, P_u64, P_u64, P_u64, u64, P_dbl],
     'pcube_batch_create': [P_int, u32, c_int, c_int, u64, u32, c_int, P_int],
     'pcube_batch_destroy': [c_int],
