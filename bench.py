@@ -17,6 +17,7 @@ the CPU baseline (this repository's C restatement on the host cores).
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import os
 import random
@@ -147,6 +148,7 @@ def served_leg(runtime, af, bf, requests: int, load: float) -> dict:
     engine = BatchedEngine(runtime, [config], af, bf)
     sample = request_stream(config, 2000, 619031, rate=0.0)
     Server(engine).serve(sample[:50], realtime=False)   # warm buffers and the launch graph
+    gc.collect()
     t0 = time.perf_counter()
     saturated = Server(engine).serve(sample, realtime=False)
     seconds = time.perf_counter() - t0
@@ -159,7 +161,7 @@ def served_leg(runtime, af, bf, requests: int, load: float) -> dict:
     probes = []
     capacity = 0.0
     for fraction in (0.9, 0.75, 0.6, 0.45, 0.3, 0.2):
-        probe = request_stream(config, 1500, 619041 + len(probes), rate=fraction * saturated_capacity / per_request)
+        probe = request_stream(config, 3000, 619041 + len(probes), rate=fraction * saturated_capacity / per_request)
         t0 = time.perf_counter()
         got = Server(engine).serve(probe)
         makespan = time.perf_counter() - t0
@@ -174,6 +176,7 @@ def served_leg(runtime, af, bf, requests: int, load: float) -> dict:
         capacity = min(p['offered'] for p in probes)
     rate = load * capacity / per_request
     stream = request_stream(config, requests, 619003, rate=rate)
+    gc.collect()
     server = Server(engine)
     t0 = time.perf_counter()
     done = server.serve(stream)

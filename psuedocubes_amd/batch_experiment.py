@@ -260,7 +260,7 @@ class MaskedMatcher(FusedMatcher):
         super().__init__(runtime, config)
         if not self.engine.order:
             raise ValueError('bitset experiment requires at least one residual filter')
-        runtime.compile('mask_staged')
+        runtime.compile('prefilter_staged')
 
     def run(self, batch: Prepared, output_cap: int = 8192) -> dict[str, Any]:
         result = super().run(batch, output_cap)

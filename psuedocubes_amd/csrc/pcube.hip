@@ -687,39 +687,40 @@ int staged_enqueue(Matcher& mt, MatchArgs m, hipEvent_t ev_pre, hipEvent_t ev_ma
     m.rho = mt.rho.as<uint16_t>();
     m.cand_max = mt.status.as<uint64_t>() + 3;
     m.item_cap = (uint32_t)mt.cand_cap;
-    hipLaunchKernelGGL(k_rows_staged, dim3(blocks_for(m.nv, 256)), dim3(256), 0, g.stream, m);
-    if (m.nu) {
-        const uint32_t chunks = mt.s_chunk_base[mt.sbits];
-        const uint64_t waves = (uint64_t)chunks * m.nstage;
-        const uint32_t* cb = mt.s_chunk_base;
-        hipLaunchKernelGGL(k_masks_staged, dim3(blocks_for(waves * kWave, 256)), dim3(256), 0, g.stream, m, chunks,
-                           make_uint4(cb[0], cb[1], cb[2], cb[3]), make_uint4(cb[4], cb[5], cb[6], cb[7]));
-        if (m.ncache)
-            hipLaunchKernelGGL(k_rho, dim3(blocks_for((uint64_t)m.nu * kCachedResidues, 256)), dim3(256), 0, g.stream, m);
-    }
+    // the work list of (piece, chunk) items, its survivor regions and their counts
+    const uint32_t npieces = (m.nv + kPieceRows - 1) / kPieceRows;
+    const uint32_t items_cap = npieces * m.nstage;   // staged_fits bounds it
+    TRY(mt.items.ensure(((uint64_t)items_cap + 1) * sizeof(uint2) + (uint64_t)items_cap * 4));
+    uint2* items = mt.items.as<uint2>();
+    uint32_t* nitems = (uint32_t*)(items + items_cap);
+    m.item_count = nitems + 2;
+    TRY(mt.cand.ensure(std::max<uint64_t>(4, (uint64_t)items_cap * mt.cand_cap * sizeof(uint32_t))));
+    m.cand = mt.cand.as<uint32_t>();
+    // one launch: the item plan, the rows, the bitsets and the residue cache
+    PrefilterArgs pa;
+    memset(&pa, 0, sizeof pa);
+    pa.rows_blocks = (uint32_t)blocks_for(m.nv, 256);
+    pa.chunks_total = mt.s_chunk_base[mt.sbits];
+    pa.mask_blocks = m.nu ? (uint32_t)blocks_for((uint64_t)pa.chunks_total * m.nstage * kWave, 256) : 0;
+    pa.rho_blocks = m.nu && m.ncache ? (uint32_t)blocks_for((uint64_t)m.nu * kCachedResidues, 256) : 0;
+    pa.npieces = npieces;
+    pa.items_cap = items_cap;
+    for (int k = 0; k < kStagedFilters; ++k) pa.chunk_base[k] = mt.s_chunk_base[k];
+    pa.items = items;
+    pa.nitems = nitems;
+    hipLaunchKernelGGL(k_prefilter_staged, dim3(1 + pa.rows_blocks + pa.mask_blocks + pa.rho_blocks), dim3(256), 0,
+                       g.stream, m, pa);
     if (ev_pre) HIP_TRY(hipEventRecord(ev_pre, g.stream));
     if (m.nu) {
-        // work list of (piece, chunk) items, then one persistent block per CU over it
-        const uint32_t npieces = (m.nv + kPieceRows - 1) / kPieceRows;
-        const uint32_t items_cap = npieces * m.nstage;   // staged_fits bounds it
-        TRY(mt.items.ensure(((uint64_t)items_cap + 1) * sizeof(uint2) + (uint64_t)items_cap * 4));
-        uint2* items = mt.items.as<uint2>();
-        uint32_t* nitems = (uint32_t*)(items + items_cap);
-        m.item_count = nitems + 2;
-        TRY(mt.cand.ensure((uint64_t)items_cap * mt.cand_cap * sizeof(uint32_t)));
-        m.cand = mt.cand.as<uint32_t>();
-        hipLaunchKernelGGL(k_plan_staged, dim3(1), dim3(kStagedThreads), 0, g.stream, m, npieces, items, items_cap,
-                           nitems);
-        {
-            const uint2* items_arg = items;
-            const uint32_t* nitems_arg = nitems;
-            void* args[] = {&m, &items_arg, (void*)&items_cap, &nitems_arg};
-            if (ev_k0) HIP_TRY(hipEventRecord(ev_k0, g.stream));
-            HIP_TRY(hipLaunchKernel(staged_kernel(mt.sbits, mt.nres), dim3((uint32_t)g.compute_units),
-                                    dim3(kStagedThreads), args, 0, g.stream));
-            if (ev_k1) HIP_TRY(hipEventRecord(ev_k1, g.stream));
-        }
-        hipLaunchKernelGGL(k_chain_staged, dim3(2 * (uint32_t)g.compute_units), dim3(kMatchThreads), 0, g.stream, m,
+        // one persistent block per CU over the work list, then the residual chain
+        const uint2* items_arg = items;
+        const uint32_t* nitems_arg = nitems;
+        void* args[] = {&m, &items_arg, (void*)&items_cap, &nitems_arg};
+        if (ev_k0) HIP_TRY(hipEventRecord(ev_k0, g.stream));
+        HIP_TRY(hipLaunchKernel(staged_kernel(mt.sbits, mt.nres), dim3((uint32_t)g.compute_units),
+                                dim3(kStagedThreads), args, 0, g.stream));
+        if (ev_k1) HIP_TRY(hipEventRecord(ev_k1, g.stream));
+        hipLaunchKernelGGL(k_chain_staged, dim3(4 * (uint32_t)g.compute_units), dim3(kMatchThreads), 0, g.stream, m,
                            (const uint2*)items, (const uint32_t*)nitems, items_cap);
     }
     if (ev_match) HIP_TRY(hipEventRecord(ev_match, g.stream));
@@ -814,8 +815,8 @@ int match_launch(Matcher& mt) {
     const Pending& pd = mt.pending;
     const uint64_t nc = (uint64_t)pd.ntiles * (mt.ng + 3);
     hipEvent_t e0 = mt.ev[0], e1 = mt.ev[1], e2 = mt.ev[2], k0 = mt.ev[3], k1 = mt.ev[4];
-    TRY(fill64(mt.counters.as<uint64_t>(), nc, 0));
-    TRY(fill64(mt.status.as<uint64_t>(), 4, 0));
+    hipLaunchKernelGGL(k_fill2_u64, dim3(std::min<uint64_t>(blocks_for(nc + 4, 256), 4096)), dim3(256), 0, g.stream,
+                       mt.counters.as<uint64_t>(), nc, mt.status.as<uint64_t>(), (uint64_t)4);
     HIP_TRY(hipEventRecord(e0, g.stream));
     TRY(match_enqueue(mt, (const uint64_t*)pd.us, pd.nu, (const uint64_t*)pd.vs, pd.nv, pd.v_first, pd.tile_width,
                       pd.ntiles, pd.cap, pd.engine, mt.counters.as<uint64_t>(), 0, pd.list_cap, e1, e2, k0, k1));
@@ -993,7 +994,7 @@ const Kernel kKernels[] = {
     {"scan_local", (const void*)k_scan_local},
     {"rows", (const void*)k_rows},
     {"mask_build", (const void*)k_masks},
-    {"mask_staged", (const void*)k_masks_staged},
+    {"prefilter_staged", (const void*)k_prefilter_staged},
     {"match_staged", (const void*)k_match_staged<kStagedFilters, false>},
     {"residues", (const void*)k_rho},
     {"match", (const void*)k_match},

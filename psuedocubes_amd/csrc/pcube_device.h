@@ -18,6 +18,14 @@ namespace pcube {
 // --------------------------------------------------------------------------
 // Fills and scans
 
+// Zero two ranges in one launch.
+__global__ void k_fill2_u64(uint64_t* __restrict__ a, uint64_t na, uint64_t* __restrict__ b, uint64_t nb) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < na + nb; i += (uint64_t)gridDim.x * blockDim.x) {
+        if (i < na) a[i] = 0;
+        else b[i - na] = 0;
+    }
+}
+
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t value) {
     uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     for (; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = value;
@@ -317,8 +325,33 @@ struct MatchArgs {
     uint32_t item_cap;
 };
 
+// First index i in [0, nu) with bound < us[i]*B, or nu.  With a directory
+// (dir[k] = us[k * stride], ndir entries, in LDS) the search first finds the
+// first directory entry above the bound and then the answer among the at most
+// stride coordinates before it, so few probes reach global memory.
+__device__ __forceinline__ uint32_t first_above(const MatchArgs& m, const U128& bound, const uint64_t* dir,
+                                                uint32_t ndir, uint32_t stride) {
+    uint32_t l = 0, h = m.nu;
+    if (dir) {
+        uint32_t a = 0, b = ndir;
+        while (a < b) {
+            const uint32_t mid = a + ((b - a) >> 1);
+            if (less_wide(bound, mul_wide(dir[mid], m.B))) b = mid; else a = mid + 1;
+        }
+        if (a == 0) return 0;
+        l = (a - 1) * stride + 1;
+        h = a * stride < m.nu ? a * stride : m.nu;
+    }
+    while (l < h) {
+        const uint32_t mid = l + ((h - l) >> 1);
+        if (less_wide(bound, mul_wide(m.us[mid], m.B))) h = mid; else l = mid + 1;
+    }
+    return l;
+}
+
 // Exact u bounds of row j by 128-bit binary search, subtile, v mod q.
-__device__ __forceinline__ void build_row(const MatchArgs& m, uint32_t j) {
+__device__ __forceinline__ void build_row(const MatchArgs& m, uint32_t j, const uint64_t* dir = nullptr,
+                                          uint32_t ndir = 0, uint32_t stride = 0) {
     const uint64_t v = m.vs[j];
     Row row;
     row.v = v;
@@ -332,19 +365,9 @@ __device__ __forceinline__ void build_row(const MatchArgs& m, uint32_t j) {
     row.tile = uint32_t(tile);
     const U128 shift = mul_wide(v, m.A);
     const U128 lower = add_wide(shift, m.lo), upper = add_wide(shift, m.hi);
-    // first index with u*B > bound
-    uint32_t l = 0, h = m.nu;
-    while (l < h) {
-        uint32_t mid = l + ((h - l) >> 1);
-        if (less_wide(lower, mul_wide(m.us[mid], m.B))) h = mid; else l = mid + 1;
-    }
-    row.left = l;
-    h = m.nu;
-    while (l < h) {
-        uint32_t mid = l + ((h - l) >> 1);
-        if (less_wide(upper, mul_wide(m.us[mid], m.B))) h = mid; else l = mid + 1;
-    }
-    row.right = l;
+    // first index with u*B > bound, for the lower and the upper bound
+    row.left = first_above(m, lower, dir, ndir, stride);
+    row.right = first_above(m, upper, dir, ndir, stride);
     if (row.flags) row.right = row.left;  // an invalid row contributes nothing
     // u*B >= v*A holds for every u in range because u*B > v*A + lo; a violation
     // (impossible for sorted input) is reported as the underflow counter.
@@ -381,12 +404,42 @@ __global__ void k_rows(MatchArgs m) {
 // Staged engine rows: as k_rows, plus each subtile's valid-position count,
 // sum of right - left over its rows.  Rows are sorted by v, so a wave's rows
 // span few subtiles: one atomic per (wave, subtile).
-__global__ void k_rows_staged(MatchArgs m) {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+constexpr int kRowDirectory = 4096;   // u coordinates a row block samples into LDS
+
+// Fill dir with every stride-th u coordinate (stride a power of two >= 64 with
+// at most kRowDirectory samples) with the whole 256-thread block; ends with a
+// barrier.
+__device__ __forceinline__ void load_directory(const MatchArgs& m, uint64_t* dir, uint32_t& ndir, uint32_t& stride) {
+    stride = 64;
+    while ((m.nu + stride - 1) / stride > (uint32_t)kRowDirectory) stride <<= 1;
+    ndir = (m.nu + stride - 1) / stride;
+    for (uint32_t first = 0; first < ndir; first += 256 * 16) {
+        uint64_t v[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const uint32_t t = first + 256 * k + threadIdx.x;
+            v[k] = m.us[(t < ndir ? t : ndir - 1) * stride];
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const uint32_t t = first + 256 * k + threadIdx.x;
+            if (t < ndir) dir[t] = v[k];
+        }
+    }
+    __syncthreads();
+}
+
+// Staged engine rows for one 256-row block: as k_rows, plus each subtile's
+// valid-position count, sum of right - left over its rows.  Rows are sorted by
+// v, so a wave's rows span few subtiles: one atomic per (wave, subtile).
+__device__ __forceinline__ void staged_rows_block(const MatchArgs& m, uint32_t block, uint64_t* dir) {
+    uint32_t ndir, stride;
+    load_directory(m, dir, ndir, stride);
+    const uint32_t j = block * 256 + threadIdx.x;
     uint32_t tile = 0xffffffffu;
     uint64_t count = 0;
     if (j < m.nv) {
-        build_row(m, j);
+        build_row(m, j, ndir ? dir : nullptr, ndir, stride);
         const Row& row = m.rows[j];
         if (!row.flags) {
             tile = row.tile;

@@ -23,9 +23,9 @@
 // Counts follow the reference's group structure (filters in groups of four,
 // pseudocube/batch_experiment.py:198-235): group 1 and group 2 are popcounts
 // of the bitset intersections after filters 3 and 7, later groups are counted
-// in the chain; valid positions per subtile are summed per row by
-// k_rows_staged.  Counts and the emitted set equal the residue-bitset
-// engine's.
+// in the chain; valid positions per subtile are summed per row by the row
+// blocks of k_prefilter_staged.  Counts and the emitted set equal the
+// residue-bitset engine's.
 #include "pcube_device.h"
 
 namespace pcube {
@@ -69,13 +69,11 @@ __device__ __forceinline__ uint64_t wave_transpose(uint64_t x, uint32_t lane) {
 // P[k0 .. k0 + 63] is allowed_q[(u*B - (r0 + j)*A) mod q]: the column of this
 // u over rows r0..r0+63.  A wave transpose turns the 64 columns into the 64
 // row words.  chunk_base[k] = first 64-row chunk of filter k.
-__global__ __launch_bounds__(256) void k_masks_staged(MatchArgs m, uint32_t chunks_total, uint4 base_lo, uint4 base_hi) {
+__device__ __forceinline__ void staged_mask_wave(const MatchArgs& m, uint64_t g, uint32_t chunks_total,
+                                                 const uint32_t* base) {
     const uint32_t lane = threadIdx.x & (kWave - 1);
-    const uint64_t g = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / kWave;
     if (g >= (uint64_t)chunks_total * m.nstage) return;
     const uint32_t s = uint32_t(g / chunks_total), c = uint32_t(g % chunks_total);
-    const uint32_t base[kStagedFilters] = {base_lo.x, base_lo.y, base_lo.z, base_lo.w,
-                                           base_hi.x, base_hi.y, base_hi.z, base_hi.w};
     uint32_t k = 0;
     while (k + 1 < m.nbits && c >= base[k + 1]) ++k;
     const Modulus mod = m.mods[k];
@@ -219,11 +217,16 @@ __device__ __forceinline__ void staged_enqueue(const Staged& S, uint32_t lane, u
     }
 }
 
-// Chunks [c_lo, c_hi] that the rows of piece p reach (none when c_lo > c_hi).
-// Rows are sorted by v, so left and right are nondecreasing over a piece.
-__device__ __forceinline__ void piece_chunks(const MatchArgs& m, uint32_t p, uint32_t& c_lo, uint32_t& c_hi) {
+// Chunks [c_lo, c_hi] that the rows of piece p reach (none when c_lo > c_hi):
+// rows are sorted by v, so from the u bounds of the piece's first and last
+// rows, computed here exactly as the row builder does.
+__device__ __forceinline__ void piece_chunks(const MatchArgs& m, uint32_t p, const uint64_t* dir, uint32_t ndir,
+                                             uint32_t stride, uint32_t& c_lo, uint32_t& c_hi) {
     const uint32_t j0 = p * kPieceRows, j1 = min(m.nv, j0 + kPieceRows);
-    const uint32_t left0 = m.rows[j0].left, right1 = m.rows[j1 - 1].right;
+    const U128 lower = add_wide(mul_wide(m.vs[j0], m.A), m.lo);
+    const U128 upper = add_wide(mul_wide(m.vs[j1 - 1], m.A), m.hi);
+    const uint32_t left0 = first_above(m, lower, dir, ndir, stride);
+    const uint32_t right1 = first_above(m, upper, dir, ndir, stride);
     c_lo = 1;
     c_hi = 0;
     if (right1 > left0) {
@@ -233,38 +236,77 @@ __device__ __forceinline__ void piece_chunks(const MatchArgs& m, uint32_t p, uin
 }
 
 // Work list of the staged match: every (piece, chunk) pair whose rows reach
-// the chunk, piece-major, at most items_cap of them; nitems receives the
-// full count.  One block: thread t takes a run of consecutive pieces, a block
-// scan turns the runs' counts into offsets.
-__global__ __launch_bounds__(kStagedThreads) void k_plan_staged(MatchArgs m, uint32_t npieces, uint2* items,
-                                                                uint32_t items_cap, uint32_t* nitems) {
-    __shared__ uint32_t wave_total[kStagedWaves];
+// the chunk, piece-major, at most items_cap of them; nitems receives the full
+// count.  One 256-thread block: thread t takes a run of consecutive pieces, a
+// block scan turns the runs' counts into offsets.
+__device__ __forceinline__ void staged_plan_block(const MatchArgs& m, uint32_t npieces, uint2* items,
+                                                  uint32_t items_cap, uint32_t* nitems, uint64_t* dir) {
+    __shared__ uint32_t wave_total[256 / kWave];
+    uint32_t ndir, stride;
+    load_directory(m, dir, ndir, stride);
+    const uint64_t* d = ndir ? dir : nullptr;
     const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    const uint32_t per = (npieces + kStagedThreads - 1) / kStagedThreads;
+    const uint32_t per = (npieces + 255) / 256;
     const uint32_t p0 = min(npieces, threadIdx.x * per), p1 = min(npieces, p0 + per);
-    uint32_t count = 0;
+    uint32_t count = 0, lo1 = 1, hi1 = 0;   // the range of a one-piece run is kept
     for (uint32_t p = p0; p < p1; ++p) {
         uint32_t lo, hi;
-        piece_chunks(m, p, lo, hi);
+        piece_chunks(m, p, d, ndir, stride, lo, hi);
         if (lo <= hi) count += hi - lo + 1;
+        lo1 = lo;
+        hi1 = hi;
     }
     uint32_t inc = count;
 #pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-        const uint32_t y = __shfl_up(inc, d, kWave);
-        if (lane >= (uint32_t)d) inc += y;
+    for (int dd = 1; dd < kWave; dd <<= 1) {
+        const uint32_t y = __shfl_up(inc, dd, kWave);
+        if (lane >= (uint32_t)dd) inc += y;
     }
     if (lane == kWave - 1) wave_total[wave] = inc;
     __syncthreads();
     uint32_t at = inc - count;
     for (uint32_t w = 0; w < wave; ++w) at += wave_total[w];
     for (uint32_t p = p0; p < p1; ++p) {
-        uint32_t lo, hi;
-        piece_chunks(m, p, lo, hi);
+        uint32_t lo = lo1, hi = hi1;
+        if (p1 - p0 > 1) piece_chunks(m, p, d, ndir, stride, lo, hi);
         for (uint32_t c = lo; c <= hi && lo <= hi; ++c, ++at)
             if (at < items_cap) items[at] = make_uint2(p, c);
     }
-    if (threadIdx.x == kStagedThreads - 1) *nitems = at;
+    if (threadIdx.x == 255) *nitems = at;
+}
+
+// Geometry of the fused prefilter launch.
+struct PrefilterArgs {
+    uint32_t rows_blocks, mask_blocks, rho_blocks, chunks_total, npieces, items_cap;
+    uint32_t chunk_base[kStagedFilters];
+    uint2* items;
+    uint32_t* nitems;
+};
+
+// Everything the staged match needs, in one launch of independent 256-thread
+// blocks: block 0 plans the work items; then the row blocks (exact u bounds,
+// mask row offsets, sigmas, valid counts), the mask waves and the residue
+// cache.
+__global__ __launch_bounds__(256) void k_prefilter_staged(MatchArgs m, PrefilterArgs pa) {
+    __shared__ uint64_t dir[kRowDirectory];
+    uint32_t b = blockIdx.x;
+    if (b == 0) {
+        staged_plan_block(m, pa.npieces, pa.items, pa.items_cap, pa.nitems, dir);
+        return;
+    }
+    b -= 1;
+    if (b < pa.rows_blocks) {
+        staged_rows_block(m, b, dir);
+        return;
+    }
+    b -= pa.rows_blocks;
+    if (b < pa.mask_blocks) {
+        staged_mask_wave(m, (uint64_t)b * (256 / kWave) + threadIdx.x / kWave, pa.chunks_total, pa.chunk_base);
+        return;
+    }
+    b -= pa.mask_blocks;
+    const uint64_t t = (uint64_t)b * 256 + threadIdx.x;
+    if (t < (uint64_t)m.nu * kCachedResidues) build_rho(m, t);
 }
 
 // Row headers of one work item's rows for this thread: (left, right, tile,
@@ -478,9 +520,8 @@ inline const void* staged_kernel(uint32_t nbits, uint32_t nres) {
     return nbits == nres ? staged_kernel_for<true>(nbits) : staged_kernel_for<false>(nbits);
 }
 
-// The residual chain over the batch's bitset survivors: wave w of the grid
-// takes work items w, w + waves, ... and runs each item's survivors 64 at a
-// time.  Group counts of subtiles below kTileSlots are kept per block in
+// The residual chain over the batch's bitset survivors: two waves of the grid
+// per work item, each running every other 64 of the item's survivors.  Group counts of subtiles below kTileSlots are kept per block in
 // LDS.  An item whose survivors exceeded item_cap records its count in
 // cand_max so the host can size the regions and run again.
 __global__ __launch_bounds__(kMatchThreads) void k_chain_staged(MatchArgs m, const uint2* items, const uint32_t* nitems,
@@ -488,24 +529,27 @@ __global__ __launch_bounds__(kMatchThreads) void k_chain_staged(MatchArgs m, con
     __shared__ uint32_t allow[kMaxAllowWords];
     __shared__ Modulus mods[kMaxResiduals];
     __shared__ uint32_t tilecnt[kTileSlots * (kMaxResiduals / 4)];
-    for (uint32_t t = threadIdx.x; t < m.allow_words; t += kMatchThreads) allow[t] = m.allow[t];
-    for (uint32_t t = threadIdx.x; t < m.nres; t += kMatchThreads) mods[t] = m.mods[t];
+    // allow and the Modulus table are whole 16-byte units (both buffers are allocated so)
+    stage_copy<kMatchThreads, 2>((uint4*)allow, (const uint4*)m.allow, (m.allow_words + 3) / 4);
+    stage_copy<kMatchThreads, 1>((uint4*)mods, (const uint4*)m.mods, m.nres * (uint32_t)sizeof(Modulus) / 16);
     for (uint32_t t = threadIdx.x; t < (uint32_t)kTileSlots * m.ng; t += kMatchThreads) tilecnt[t] = 0;
     __syncthreads();
     const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     const Staged S{allow, mods, tilecnt, nullptr};
     const uint32_t total = min(*nitems, items_cap);
     const uint32_t waves = gridDim.x * kMatchWaves;
-    for (uint32_t item = blockIdx.x * kMatchWaves + wave; item < total; item += waves) {
+    // two waves per item: wave half h takes the item's survivors 64h .. 64h + 63, +128, ...
+    for (uint32_t task = blockIdx.x * kMatchWaves + wave; task < 2 * total; task += waves) {
+        const uint32_t item = task >> 1, half = task & 1;
         uint32_t n = m.item_count[item];
         if (n > m.item_cap) {
-            if (lane == 0) atomicMax((unsigned long long*)m.cand_max, (unsigned long long)n);
+            if (lane == 0 && half == 0) atomicMax((unsigned long long*)m.cand_max, (unsigned long long)n);
             n = m.item_cap;
         }
         const uint2 it = items[item];
         const uint32_t j0 = it.x * kPieceRows, u0 = (it.y * kStageWords) << 6;
         const uint32_t* region = m.cand + (uint64_t)item * m.item_cap;
-        for (uint32_t first = 0; first < n; first += kWave) {
+        for (uint32_t first = half * kWave; first < n; first += 2 * kWave) {
             const uint32_t count = min(n - first, (uint32_t)kWave);
             const uint32_t e = lane < count ? region[first + lane] : 0u;
             staged_chain(m, S, lane, count, j0 + (e >> 8), u0 + (e & 0xffu));

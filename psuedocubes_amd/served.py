@@ -20,12 +20,18 @@ filter tables resident and loops:
     per-subtile scopes and values do not depend on what shares a launch.
 
 Latency is completion time minus arrival time.
+
+The cyclic garbage collector is paused while a stream is served: result rows
+hold no reference cycles, so reference counting frees them, and a full
+collection over the retained rows of a long stream stalls the loop for tens of
+milliseconds at a time.
 """
 from __future__ import annotations
 
 from collections import deque
 from concurrent.futures import Future, ThreadPoolExecutor
 from dataclasses import dataclass, field
+import gc
 import math
 import random
 import statistics
@@ -115,6 +121,8 @@ class Server:
         done: list[Completed] = []
         inflight: tuple[Future, list[tuple[int, Window]]] | None = None
         pool = ThreadPoolExecutor(max_workers=1)
+        collecting = gc.isenabled()
+        gc.disable()
 
         def now() -> float:
             return clock() - start
@@ -190,6 +198,8 @@ class Server:
                     self.overlap_seconds += now() - t0
         finally:
             pool.shutdown(wait=True)
+            if collecting:
+                gc.enable()
         return done
 
 
