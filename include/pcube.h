@@ -108,6 +108,25 @@ int pcube_focus_windows(int focus, uint32_t n, const uint64_t *lows, const uint6
 int pcube_prepare(int focus_u, int focus_v, uint32_t ntiles, const uint64_t *tiles, uint64_t us_out, uint64_t us_cap,
                   uint64_t vs_out, uint64_t vs_cap, uint64_t *nu, uint64_t *nv, uint64_t *bounds);
 
+/* tile_ranges(first, end, width) of the reference (pseudocube/batch_experiment.py:27-32)
+ * in exact 128-bit host arithmetic: up to 64 x (v_first, v_end, u_min, u_end)
+ * into tiles, *ntiles of them, with u_min = (lo + v A) / B + 1 at a subtile's
+ * first v and u_end = (hi + v A) / B + 1 at its last.  lo = lo1:lo0, hi =
+ * hi1:hi0.  INVALID for an empty range, end > B, more than 64 subtiles, A or B
+ * >= 2^63, or lo or hi >= 2^96.  Host only. */
+int pcube_tile_ranges(uint64_t A, uint64_t B, uint64_t lo0, uint64_t lo1, uint64_t hi0, uint64_t hi1, uint64_t first,
+                      uint64_t end, uint64_t width, uint64_t *tiles, uint32_t *ntiles);
+
+/* prepare() of one compute batch from its canonical v range [first, end) and
+ * subtile width (pseudocube/batch_experiment.py:27-32, 51-57): the geometry of
+ * pcube_tile_ranges into tiles, then pcube_prepare on it.  Errors as
+ * pcube_tile_ranges, and RANGE when a window spans more than 4096 focus
+ * periods (nothing is generated). */
+int pcube_prepare_range(int focus_u, int focus_v, uint64_t A, uint64_t B, uint64_t lo0, uint64_t lo1, uint64_t hi0,
+                        uint64_t hi1, uint64_t first, uint64_t end, uint64_t width, uint64_t *tiles, uint32_t *ntiles,
+                        uint64_t us_out, uint64_t us_cap, uint64_t vs_out, uint64_t vs_cap, uint64_t *nu, uint64_t *nv,
+                        uint64_t *bounds);
+
 /* np.searchsorted(sorted, queries, 'left') on the device; host queries/out
  * (pseudocube/batch_experiment.py:41-42). */
 int pcube_searchsorted(uint64_t sorted, uint64_t n, const uint64_t *queries, uint64_t nq, uint64_t *out);

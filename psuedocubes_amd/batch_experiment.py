@@ -93,32 +93,40 @@ def prepare(runtime: Runtime, config: Config, af: GPUFocus, bf: GPUFocus,
             first: int, end: int, width: int = 100000000) -> Prepared:
     """Both focus windows and the subtile bounds of one batch (`pseudocube/batch_experiment.py:51-57`).
 
-    With both wheels present this is one library call (`pcube_prepare`);
-    otherwise it is the reference's sequence of two windows and two searches.
+    With both wheels present this is one library call (`pcube_prepare_range`,
+    which also computes the subtile geometry); otherwise it is the reference's
+    sequence of two windows and two searches.
     """
     started = time.perf_counter()
-    ranges = tile_ranges(config, first, end, width)
     if af.wheel is None or bf.wheel is None:
+        ranges = tile_ranges(config, first, end, width)
         us = af.window(ranges[0][2], ranges[-1][3])
         vs = bf.window(first, end)
         return prepare_arrays(runtime, config, first, end, width, us, vs, started)
+    Tile(first, end).validate(config)
+    if width <= 0 or (end - first + width - 1) // width > 64:
+        raise ValueError('experiment allows at most 64 bounded subtiles')
     cap = 1000000
-    if (ranges[-1][3] - 1) // af.focus.modulus - ranges[0][2] // af.focus.modulus + 1 > 4096 or \
-            (end - 1) // bf.focus.modulus - first // bf.focus.modulus + 1 > 4096:
-        raise OverflowError('focus window exceeds bounded period count')
     us, vs = DeviceArray.empty(cap), DeviceArray.empty(cap)
-    tiles = np.array([x for t in ranges for x in t], dtype=np.uint64)
-    nu, nv = ctypes.c_uint64(), ctypes.c_uint64()
-    bounds = np.zeros(4 * len(ranges), dtype=np.uint64)
+    tiles = np.empty(4 * 64, dtype=np.uint64)
+    bounds = np.empty(4 * 64, dtype=np.uint64)
+    ntiles, nu, nv = ctypes.c_uint32(), ctypes.c_uint64(), ctypes.c_uint64()
     runtime.check()
-    code = native.lib().pcube_prepare(af.handle, bf.handle, len(ranges), native.ptr64(tiles), us.address, cap,
-                                      vs.address, cap, ctypes.byref(nu), ctypes.byref(nv), native.ptr64(bounds))
+    mask = (1 << 64) - 1
+    code = native.lib().pcube_prepare_range(
+        af.handle, bf.handle, config.A, config.B, config.lo & mask, config.lo >> 64, config.hi & mask, config.hi >> 64,
+        first, end, width, native.ptr64(tiles), ctypes.byref(ntiles), us.address, cap, vs.address, cap,
+        ctypes.byref(nu), ctypes.byref(nv), native.ptr64(bounds))
+    if code == native.RANGE:
+        raise OverflowError('focus window exceeds bounded period count')
     if code == native.OVERFLOW:
         raise OverflowError('focus buffer overflow; tile invalid')
-    native.check(code, 'pcube_prepare')
+    native.check(code, 'pcube_prepare_range')
     us.size, vs.size = nu.value, nv.value
-    flat = bounds.tolist()
-    rows = [tuple(flat[i:i + 4]) for i in range(0, len(flat), 4)]
+    n = 4 * ntiles.value
+    flat_t, flat_b = tiles[:n].tolist(), bounds[:n].tolist()
+    ranges = [tuple(flat_t[i:i + 4]) for i in range(0, n, 4)]
+    rows = [tuple(flat_b[i:i + 4]) for i in range(0, n, 4)]
     for a, b, c, d in rows:
         if (b - a) * (d - c) > 50_000_000:
             raise OverflowError('per-subtile candidate opportunity limit exceeded')

@@ -712,7 +712,7 @@ int staged_enqueue(Matcher& mt, MatchArgs m, hipEvent_t ev_pre, hipEvent_t ev_ma
                        g.stream, m, pa);
     if (ev_pre) HIP_TRY(hipEventRecord(ev_pre, g.stream));
     if (m.nu) {
-        // one persistent block per CU over the work list, then the residual chain
+        // one persistent block per CU over the work list and the residual chain
         const uint2* items_arg = items;
         const uint32_t* nitems_arg = nitems;
         void* args[] = {&m, &items_arg, (void*)&items_cap, &nitems_arg};
@@ -720,8 +720,6 @@ int staged_enqueue(Matcher& mt, MatchArgs m, hipEvent_t ev_pre, hipEvent_t ev_ma
         HIP_TRY(hipLaunchKernel(staged_kernel(mt.sbits, mt.nres), dim3((uint32_t)g.compute_units),
                                 dim3(kStagedThreads), args, 0, g.stream));
         if (ev_k1) HIP_TRY(hipEventRecord(ev_k1, g.stream));
-        hipLaunchKernelGGL(k_chain_staged, dim3(4 * (uint32_t)g.compute_units), dim3(kMatchThreads), 0, g.stream, m,
-                           (const uint2*)items, (const uint32_t*)nitems, items_cap);
     }
     if (ev_match) HIP_TRY(hipEventRecord(ev_match, g.stream));
     return PCUBE_OK;
@@ -1338,6 +1336,49 @@ int pcube_prepare(int focus_u, int focus_v, uint32_t ntiles, const uint64_t* til
         bounds[4 * t + 3] = r[2 * ntiles + 2 * t + 1];
     }
     return PCUBE_OK;
+}
+
+int pcube_tile_ranges(uint64_t A, uint64_t B, uint64_t lo0, uint64_t lo1, uint64_t hi0, uint64_t hi1, uint64_t first,
+                      uint64_t end, uint64_t width, uint64_t* tiles, uint32_t* ntiles) {
+    *ntiles = 0;
+    if (B == 0 || A >= (1ull << 63) || B >= (1ull << 63) || lo1 >= (1ull << 32) || hi1 >= (1ull << 32) ||
+        first >= end || end > B || width == 0 || (end - first - 1) / width + 1 > 64)
+        return fail(PCUBE_INVALID, "invalid batch geometry");
+    // u_min = (lo + v A) / B + 1 for the first v of a subtile, u_end = (hi + v A) / B + 1
+    // for its last v; lo, hi < 2^96 and v A < 2^126, so the sums fit 128 bits
+    const unsigned __int128 lo = ((unsigned __int128)lo1 << 64) | lo0, hi = ((unsigned __int128)hi1 << 64) | hi0;
+    uint32_t n = 0;
+    for (uint64_t v = first;; v += width) {
+        const bool last = end - v <= width;
+        const uint64_t stop = last ? end : v + width;
+        tiles[4 * n] = v;
+        tiles[4 * n + 1] = stop;
+        tiles[4 * n + 2] = (uint64_t)((lo + (unsigned __int128)v * A) / B) + 1;
+        tiles[4 * n + 3] = (uint64_t)((hi + (unsigned __int128)(stop - 1) * A) / B) + 1;
+        ++n;
+        if (last) break;
+    }
+    *ntiles = n;
+    return PCUBE_OK;
+}
+
+int pcube_prepare_range(int focus_u, int focus_v, uint64_t A, uint64_t B, uint64_t lo0, uint64_t lo1, uint64_t hi0,
+                        uint64_t hi1, uint64_t first, uint64_t end, uint64_t width, uint64_t* tiles, uint32_t* ntiles,
+                        uint64_t us_out, uint64_t us_cap, uint64_t vs_out, uint64_t vs_cap, uint64_t* nu, uint64_t* nv,
+                        uint64_t* bounds) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Focus* fu = focus_at(focus_u);
+    Focus* fv = focus_at(focus_v);
+    if (!fu || !fv) return fail(PCUBE_INVALID, "unknown focus");
+    TRY(pcube_tile_ranges(A, B, lo0, lo1, hi0, hi1, first, end, width, tiles, ntiles));
+    const uint32_t n = *ntiles;
+    // the reference's bounded period count for both windows (pseudocube/gpu.py:100-102)
+    const uint64_t ulo = tiles[2], uhi = tiles[4 * (n - 1) + 3];
+    if (uhi > ulo && (uhi - 1) / fu->M - ulo / fu->M + 1 > 4096)
+        return fail(PCUBE_RANGE, "focus window exceeds bounded period count");
+    if ((end - 1) / fv->M - first / fv->M + 1 > 4096)
+        return fail(PCUBE_RANGE, "focus window exceeds bounded period count");
+    return pcube_prepare(focus_u, focus_v, n, tiles, us_out, us_cap, vs_out, vs_cap, nu, nv, bounds);
 }
 
 int pcube_searchsorted(uint64_t sorted, uint64_t n, const uint64_t* queries, uint64_t nq, uint64_t* out) {

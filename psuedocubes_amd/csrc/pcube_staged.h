@@ -15,8 +15,8 @@
 // row's residue classes.  The next item's chunk streams into a second LDS
 // buffer (global_load_lds) while the current one is matched.  Positions
 // surviving all bitset filters (about 1 in 6561 at p = 619) are written to the
-// item's region of a candidate list; k_chain_staged runs them through the rest
-// of the filter chain exactly as in the residue-bitset engine: cached
+// item's region of a candidate list, and the block runs them through the rest
+// of the filter chain after its items, exactly as in the residue-bitset engine: cached
 // residues (u*B mod q) + sigma_v for the next kCachedResidues filters, then
 // exact 128-bit x = u*B - v*A reduced limb by limb.
 //
@@ -457,7 +457,8 @@ __device__ __forceinline__ bool staged_item(const MatchArgs& m, const uint2* ite
 // item (piece p, chunk s) thread l intersects the staged chunk with rows
 // j0 + l + 1024 t of the piece; the next item's chunk and row headers are in
 // flight meanwhile.  Group counts of the piece's subtiles collect in LDS and
-// go to the global counters at the end of each item.  kF = bitset filters;
+// go to the global counters at the end of each item.  After its last item the
+// block runs its items' survivors through the residual chain.  kF = bitset filters;
 // kLast: the last residual filter is the last bitset filter (it closes a
 // group).
 template <int kF, bool kLast>
@@ -486,6 +487,31 @@ __global__ __launch_bounds__(kStagedThreads) void k_match_staged(MatchArgs m, co
     for (;;) {
         if (!staged_item<kF, kLast>(m, items, S, st, buffer0, buffer1, &fill, lane, wave)) break;
         if (!staged_item<kF, kLast>(m, items, S, st, buffer1, buffer0, &fill, lane, wave)) break;
+    }
+    // the residual chain over this block's items' survivors, 64 at a time; the
+    // block's waves take the 64-survivor slices of its items in turn
+    __syncthreads();   // every item's region and count is written
+    uint32_t slice = 0;
+    for (uint32_t item = blockIdx.x; item < st.total; item += gridDim.x) {
+        uint32_t n = m.item_count[item];
+        if (n > m.item_cap) {
+            if (threadIdx.x == 0) atomicMax((unsigned long long*)m.cand_max, (unsigned long long)n);
+            n = m.item_cap;
+        }
+        const uint2 it = items[item];
+        const uint32_t j0 = it.x * kPieceRows, u0 = (it.y * kStageWords) << 6;
+        const uint32_t* region = m.cand + (uint64_t)item * m.item_cap;
+        for (uint32_t first = 0; first < n; first += kWave, ++slice) {
+            if (slice % kStagedWaves != wave) continue;
+            const uint32_t count = min(n - first, (uint32_t)kWave);
+            const uint32_t e = lane < count ? region[first + lane] : 0u;
+            staged_chain(m, S, lane, count, j0 + (e >> 8), u0 + (e & 0xffu));
+        }
+    }
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < (uint32_t)kTileSlots * m.ng; t += kStagedThreads) {
+        const uint32_t c = tilecnt[t];
+        if (c) atomicAdd(&m.counters[(uint64_t)(t / m.ng) * (m.ng + 3) + t % m.ng + 1], (uint64_t)c);
     }
 }
 
@@ -518,48 +544,6 @@ inline const void* staged_kernel_for(uint32_t nbits) {
 // The instantiation for a matcher: nbits bitset filters out of nres.
 inline const void* staged_kernel(uint32_t nbits, uint32_t nres) {
     return nbits == nres ? staged_kernel_for<true>(nbits) : staged_kernel_for<false>(nbits);
-}
-
-// The residual chain over the batch's bitset survivors: two waves of the grid
-// per work item, each running every other 64 of the item's survivors.  Group counts of subtiles below kTileSlots are kept per block in
-// LDS.  An item whose survivors exceeded item_cap records its count in
-// cand_max so the host can size the regions and run again.
-__global__ __launch_bounds__(kMatchThreads) void k_chain_staged(MatchArgs m, const uint2* items, const uint32_t* nitems,
-                                                                uint32_t items_cap) {
-    __shared__ uint32_t allow[kMaxAllowWords];
-    __shared__ Modulus mods[kMaxResiduals];
-    __shared__ uint32_t tilecnt[kTileSlots * (kMaxResiduals / 4)];
-    // allow and the Modulus table are whole 16-byte units (both buffers are allocated so)
-    stage_copy<kMatchThreads, 2>((uint4*)allow, (const uint4*)m.allow, (m.allow_words + 3) / 4);
-    stage_copy<kMatchThreads, 1>((uint4*)mods, (const uint4*)m.mods, m.nres * (uint32_t)sizeof(Modulus) / 16);
-    for (uint32_t t = threadIdx.x; t < (uint32_t)kTileSlots * m.ng; t += kMatchThreads) tilecnt[t] = 0;
-    __syncthreads();
-    const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    const Staged S{allow, mods, tilecnt, nullptr};
-    const uint32_t total = min(*nitems, items_cap);
-    const uint32_t waves = gridDim.x * kMatchWaves;
-    // two waves per item: wave half h takes the item's survivors 64h .. 64h + 63, +128, ...
-    for (uint32_t task = blockIdx.x * kMatchWaves + wave; task < 2 * total; task += waves) {
-        const uint32_t item = task >> 1, half = task & 1;
-        uint32_t n = m.item_count[item];
-        if (n > m.item_cap) {
-            if (lane == 0 && half == 0) atomicMax((unsigned long long*)m.cand_max, (unsigned long long)n);
-            n = m.item_cap;
-        }
-        const uint2 it = items[item];
-        const uint32_t j0 = it.x * kPieceRows, u0 = (it.y * kStageWords) << 6;
-        const uint32_t* region = m.cand + (uint64_t)item * m.item_cap;
-        for (uint32_t first = half * kWave; first < n; first += 2 * kWave) {
-            const uint32_t count = min(n - first, (uint32_t)kWave);
-            const uint32_t e = lane < count ? region[first + lane] : 0u;
-            staged_chain(m, S, lane, count, j0 + (e >> 8), u0 + (e & 0xffu));
-        }
-    }
-    __syncthreads();
-    for (uint32_t t = threadIdx.x; t < (uint32_t)kTileSlots * m.ng; t += kMatchThreads) {
-        const uint32_t c = tilecnt[t];
-        if (c) atomicAdd(&m.counters[(uint64_t)(t / m.ng) * (m.ng + 3) + t % m.ng + 1], (uint64_t)c);
-    }
 }
 
 }  // namespace pcube

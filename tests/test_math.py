@@ -204,3 +204,33 @@ def test_tile_ranges_against_one_division_per_bound() -> None:
             expect = [(v, min(v + width, end), config.u_interval(v)[0], config.u_interval(min(v + width, end) - 1)[1])
                       for v in range(first, end, width)]
             assert tile_ranges(config, first, end, width) == expect
+
+
+def test_library_tile_ranges_equal_python() -> None:
+    """The library's 128-bit subtile geometry (used by the one-call prepare)
+    equals tile_ranges on seeded ranges, ragged last subtiles and the range's
+    end at B; invalid geometry is refused."""
+    import numpy as np
+    from psuedocubes_amd import native
+    from psuedocubes_amd.batch_experiment import tile_ranges
+    rng = random.Random(3301)
+    mask = (1 << 64) - 1
+    for p in (313, 619):
+        config = Config(p, 10**27, 3 * 10**27, A619, B619)
+        for _ in range(300):
+            width = rng.choice([1, 3, 10**8, rng.randrange(1, 10**9)])
+            first = rng.randrange(0, config.B - 1)
+            if rng.random() < 0.2:
+                first = max(0, config.B - rng.randrange(1, 64 * width + 1))
+            end = min(config.B, first + rng.randrange(1, 64 * width + 1))
+            tiles = np.zeros(256, dtype=np.uint64)
+            n = ctypes.c_uint32()
+            native.call('pcube_tile_ranges', config.A, config.B, config.lo & mask, config.lo >> 64, config.hi & mask,
+                        config.hi >> 64, first, end, width, native.ptr64(tiles), ctypes.byref(n))
+            flat = tiles[:4 * n.value].tolist()
+            assert [tuple(flat[i:i + 4]) for i in range(0, len(flat), 4)] == tile_ranges(config, first, end, width)
+    tiles = np.zeros(256, dtype=np.uint64)
+    n = ctypes.c_uint32()
+    for first, end, width in ((5, 5, 1), (0, B619 + 1, 10**8), (0, 65, 1)):
+        assert native.lib().pcube_tile_ranges(A619, B619, 0, 0, 1, 0, first, end, width, native.ptr64(tiles),
+                                              ctypes.byref(n)) == native.INVALID
