@@ -623,8 +623,8 @@ struct Matcher {
     uint32_t row_base[kMaxBitsetFilters] = {0, 0, 0, 0};
     uint32_t mask_rows = 0;
     // staged engine: bitset filters, their rows (sum of q), cached chain filters, LDS bytes
-    uint32_t sbits = 0, srows = 0, sncache = 0;
-    uint32_t s_row_base[kStagedFilters] = {}, s_chunk_base[kStagedFilters + 1] = {};
+    uint32_t sbits = 0, slines = 0, sncache = 0;   // staged filters, lines of a chunk image (largest q)
+    uint32_t s_chunk_base[kStagedFilters + 1] = {};
     uint32_t s_pstr_off[kStagedFilters] = {}, s_nbeta[kStagedFilters] = {};
     std::vector<Modulus> mods;
     Buffer d_mods, d_allow, d_pstr;
@@ -674,14 +674,13 @@ int staged_enqueue(Matcher& mt, MatchArgs m, hipEvent_t ev_pre, hipEvent_t ev_ma
     m.nbits = mt.sbits;
     m.ncache = mt.sncache;
     for (int k = 0; k < kStagedFilters; ++k) {
-        m.row_base[k] = mt.s_row_base[k];
         m.nbeta[k] = mt.s_nbeta[k];
         m.pstr_off[k] = mt.s_pstr_off[k];
     }
     m.pstr = mt.d_pstr.as<uint32_t>();
-    m.stage_rows = mt.srows;
+    m.stage_lines = mt.slines;
     m.nstage = (m.words + kStageWords - 1) / kStageWords;
-    TRY(mt.masks.ensure(std::max<uint64_t>(8, (uint64_t)m.nstage * mt.srows * kStageWords * 8)));
+    TRY(mt.masks.ensure(std::max<uint64_t>(8, (uint64_t)m.nstage * mt.slines * kStageLineBytes)));
     TRY(mt.rho.ensure(std::max<uint64_t>(16, (uint64_t)m.nu * kCachedResidues * 2)));
     m.masks = mt.masks.as<uint64_t>();
     m.rho = mt.rho.as<uint16_t>();
@@ -1444,15 +1443,15 @@ int pcube_matcher_create(uint64_t A, uint64_t B, uint64_t lo0, uint64_t lo1, uin
     }
     mt->mask_rows = rb;
     // staged engine: the longest prefix of at most kStagedFilters filters whose
-    // rows fit one LDS chunk and whose A residue is invertible; filter k keeps
-    // P[j] = allowed_q[(-j*A) mod q] for j < q + 64, plus two words of margin
+    // q rows fit a column of the chunk image and whose A residue is invertible;
+    // filter k keeps P[j] = allowed_q[(-j*A) mod q] for j < q + 64, plus two
+    // words of margin
     std::vector<uint32_t> pstr;
     for (uint32_t k = 0; k < std::min<uint32_t>(nres, kStagedFilters); ++k) {
         const Modulus& md = mt->mods[k];
-        if (mt->srows + md.q > (uint32_t)kMaxStageRows || md.a == 0 || std::gcd(md.a, md.q) != 1) break;
+        if (md.q > (uint32_t)kStageLines || md.a == 0 || std::gcd(md.a, md.q) != 1) break;
         const uint64_t ainv = modinv(md.a, md.q);
         mt->s_nbeta[k] = (uint32_t)((md.q - (uint64_t)md.b * ainv % md.q) % md.q);
-        mt->s_row_base[k] = mt->srows;
         mt->s_pstr_off[k] = (uint32_t)pstr.size();
         const uint32_t nbits = md.q + kWave, nwords = (nbits + 31) / 32 + 2;
         const size_t o = pstr.size();
@@ -1462,7 +1461,7 @@ int pcube_matcher_create(uint64_t A, uint64_t B, uint64_t lo0, uint64_t lo1, uin
             if (allowed_residue(md.q, res)) pstr[o + j / 32] |= 1u << (j % 32);
         }
         mt->s_chunk_base[k + 1] = mt->s_chunk_base[k] + (md.q + kWave - 1) / kWave;
-        mt->srows += md.q;
+        mt->slines = std::max(mt->slines, md.q);
         ++mt->sbits;
     }
     for (uint32_t k = mt->sbits; k < (uint32_t)kStagedFilters; ++k) mt->s_chunk_base[k + 1] = mt->s_chunk_base[k];

@@ -291,12 +291,21 @@ __global__ void k_searchsorted(const uint64_t* __restrict__ sorted, uint64_t n, 
 struct Row {
     uint32_t left, right;   // u index range [left, right) with lo < u*B - v*A <= hi
     uint32_t tile, flags;   // subtile index; bit0: a v outside [v_first, v_first + ntiles*width)
-    uint16_t r[kStagedFilters];       // v mod q for the bitset filters (staged engine: LDS offsets)
+    uint16_t r[kStagedFilters];       // v mod q for the bitset filters (staged engine: 8 * line + column per read slot)
     uint16_t sigma[kCachedResidues];  // (-v*A) mod q for the cached chain filters (staged engine)
     uint64_t v;
     uint64_t pad;
 };
 static_assert(sizeof(Row) == 64, "Row is one 64-byte record");
+
+// Staged engine: the bitset filter (column of the chunk image) that the lane
+// of row j reads through offset slot i.  Slots 0-3 hold one group of four
+// filters and slots 4-7 the other, both rotated by j mod 16, the lane's
+// position in its 16-lane LDS cycle (pcube_staged.h).
+__host__ __device__ __forceinline__ uint32_t stage_column(uint32_t i, uint32_t j) {
+    const uint32_t g = j & 15;
+    return ((i & 3) ^ (g & 3)) + 4 * ((i >> 2) ^ (g >> 3));
+}
 
 struct MatchArgs {
     const uint64_t* us;
@@ -314,9 +323,9 @@ struct MatchArgs {
     U128 lo, hi;
     uint32_t nu, nv, words, nres, nbits, ncache, ng, ntiles, allow_words, rows_per_wave;
     uint32_t row_base[kStagedFilters];
-    // staged engine: masks in chunks of kStageWords words, [chunk][stage_rows][kStageWords]
+    // staged engine: masks in chunk images of stage_lines lines x 8 columns x kStageWords words
     const uint32_t* pstr;      // per bitset filter, its permuted allowed string, from pstr_off[k]
-    uint32_t staged, stage_rows, nstage;
+    uint32_t staged, stage_lines, nstage;
     uint32_t nbeta[kStagedFilters];    // (-B * A^-1) mod q per bitset filter
     uint32_t pstr_off[kStagedFilters];
     uint32_t* cand;            // bitset survivors, item_cap entries per work item of the staged engine
@@ -373,13 +382,16 @@ __device__ __forceinline__ void build_row(const MatchArgs& m, uint32_t j, const 
     // (impossible for sorted input) is reported as the underflow counter.
     if (row.left < row.right && less_wide(mul_wide(m.us[row.left], m.B), shift))
         atomicAdd(&m.counters[(uint64_t)row.tile * (m.ng + 3) + m.ng + 2], 1ull);
-    // the staged engine keeps instead the LDS byte offset of the filter's mask row
-    // R = row_base + v mod q: 32 R, plus 16 when bit 3 of R is set (its halves are swapped)
+    // the staged engine keeps instead, per offset slot, 8 * line + column of
+    // the mask row its lane reads there: line v mod q of the slot's filter
+    // (line 0, all ones, for an absent filter)
     for (uint32_t k = 0; k < kStagedFilters; ++k) {
-        uint32_t r = k < m.nbits ? mod64(v, m.mods[k]) : 0u;
-        if (m.staged && k < m.nbits) {
-            const uint32_t R = r + m.row_base[k];
-            r = (R << 5) | ((R & 8u) << 1);
+        uint32_t r;
+        if (m.staged) {
+            const uint32_t f = stage_column(k, j);
+            r = (f < m.nbits ? mod64(v, m.mods[f]) : 0u) * kStagedFilters + f;
+        } else {
+            r = k < m.nbits ? mod64(v, m.mods[k]) : 0u;
         }
         row.r[k] = uint16_t(r);
     }
@@ -861,7 +873,7 @@ __device__ __forceinline__ uint32_t find_window(const BatchWindow* win, uint32_t
 
 __device__ __forceinline__ MatchArgs window_args(const BatchArgs& b, const BatchWindow& w) {
     const BatchConfig& c = b.cfg[w.cfg];
-    MatchArgs m;
+    MatchArgs m{};   // every field the batched engine does not set is zero (staged = 0: plain row records)
     m.us = w.us; m.vs = w.vs;
     m.rows = b.rows + w.row_begin;
     m.masks = w.masks; m.rho = w.rho;

@@ -5,8 +5,18 @@
 // 271, 277 at p = 619) are folded into residue bitsets: for filter q and
 // residue class r = v mod q, bit t of mask word w is allowed_q[(u*B - r*A)
 // mod q] for the u at index 64w + t.  Masks are laid out in chunks of
-// kStageWords words, [chunk][sum of q rows][kStageWords], so one chunk of every
-// filter is one contiguous block of stage_rows * 32 bytes.
+// kStageWords words.  One chunk of every filter is one image of stage_lines
+// 256-byte lines: line r holds, in its 32-byte column f, filter f's row r
+// (words 0-1 in the first 16 bytes, words 2-3 in the second).
+//
+// Bank-conflict-free reads.  A ds_read_b128 serves 16 lanes per cycle, and
+// a line's 16 (column, half) pairs cover all 64 banks once.  Lane g = lane mod
+// 16 of every wave reads, at step s = 0..15, the pair s ^ g: filter
+// (s ^ g) & 3 of group (s ^ g) >> 3 (filters 0-3 or 4-7), half
+// ((s ^ g) >> 2) & 1.  The 16 lanes of a cycle then hit 16 distinct bank
+// groups whatever lines their rows use.  The row builder stores a row's eight
+// column offsets in the order its lane reads them (the row's lane is fixed by
+// its index).
 //
 // Work item = (stage chunk s, piece of kPieceRows consecutive v rows).  One
 // persistent block per CU walks a list of items; for each it intersects the
@@ -36,12 +46,9 @@ constexpr int kStagedThreads = 1024;  // one persistent block per CU
 constexpr int kStagedWaves = kStagedThreads / kWave;
 constexpr int kRowsPerLane = kPieceRows / kStagedThreads;   // rows of a piece each thread takes
 constexpr int kTileSlots = 64;        // subtiles whose group counts a block keeps in LDS
-constexpr int kMaxStageRows = 2048;   // bitset rows (sum of q) one chunk may hold
-
-// A staged row is two 16-byte halves (words 0-1, words 2-3).  Rows whose bit 3
-// is set store them swapped, so that rows 8 apart start in different LDS bank
-// quads and a wave reading 64 rows spreads over all 16 quads.
-__host__ __device__ __forceinline__ uint32_t stage_swizzle(uint32_t row) { return (row >> 3) & 1u; }
+constexpr int kStageLines = 280;      // lines of a chunk image: the largest staged q
+constexpr int kStageLineBytes = kStagedFilters * kStageWords * 8;   // 8 columns of 32 bytes
+static_assert(kStagedFilters == 8 && kStageLineBytes == 256, "the read rotation assumes 8 columns of 32 bytes");
 
 // One butterfly stage of a 64 x 64 bit transpose across a wave.
 template <int kB, uint64_t kMask>
@@ -63,7 +70,8 @@ __device__ __forceinline__ uint64_t wave_transpose(uint64_t x, uint32_t lane) {
 }
 
 // Staged residue bitsets.  Wave (s, c) builds rows r0..r0+63 of one bitset
-// filter for the kStageWords words of stage chunk s.  Lane t takes the u at
+// filter (lines r0..r0+63 of its column) for the kStageWords words of stage
+// chunk s.  Lane t takes the u at
 // index 64w + t.  With P[k] = allowed_q[(-k*A) mod q] (stored repeated, q + 64
 // bits) and k0 = (r0 + (u mod q) * nbeta) mod q, bit j of the 64-bit window
 // P[k0 .. k0 + 63] is allowed_q[(u*B - (r0 + j)*A) mod q]: the column of this
@@ -94,12 +102,17 @@ __device__ __forceinline__ void staged_mask_wave(const MatchArgs& m, uint64_t g,
         }
         out[ww] = wave_transpose(col, lane);
     }
+    uint64_t* image = (uint64_t*)m.masks + (uint64_t)s * m.stage_lines * (kStageLineBytes / 8);
     if (lane < rows) {
-        const uint32_t row = m.row_base[k] + r0 + lane;
-        const uint32_t swap = stage_swizzle(row);
-        uint4* dst = (uint4*)((uint64_t*)m.masks + ((uint64_t)s * m.stage_rows + row) * kStageWords);
-        dst[swap] = make_uint4(uint32_t(out[0]), uint32_t(out[0] >> 32), uint32_t(out[1]), uint32_t(out[1] >> 32));
-        dst[swap ^ 1] = make_uint4(uint32_t(out[2]), uint32_t(out[2] >> 32), uint32_t(out[3]), uint32_t(out[3] >> 32));
+        uint4* dst = (uint4*)(image + (r0 + lane) * (kStageLineBytes / 8) + k * kStageWords);
+        dst[0] = make_uint4(uint32_t(out[0]), uint32_t(out[0] >> 32), uint32_t(out[1]), uint32_t(out[1] >> 32));
+        dst[1] = make_uint4(uint32_t(out[2]), uint32_t(out[2] >> 32), uint32_t(out[3]), uint32_t(out[3] >> 32));
+    }
+    // the columns of absent filters (fewer than kStagedFilters staged) keep one
+    // all-ones line 0, which their rows read
+    if (c == 0 && lane < kStagedFilters - m.nbits) {
+        uint4* dst = (uint4*)(image + (m.nbits + lane) * kStageWords);
+        dst[0] = dst[1] = make_uint4(~0u, ~0u, ~0u, ~0u);
     }
 }
 
@@ -190,28 +203,37 @@ __device__ __forceinline__ void cand_flush(const Staged& S, uint32_t lane, uint3
     if (lane < count && at + lane < cap) region[at + lane] = S.queue[lane];
 }
 
-// Queue the set bits of one mask word as (row offset << 8) | (u offset of the word + bit).
-__device__ __forceinline__ void staged_enqueue(const Staged& S, uint32_t lane, uint64_t b, uint32_t tag,
-                                               uint32_t& queued, uint32_t* region, uint32_t cap, uint32_t* fill) {
+// Append the set bits of a lane's four survivor words (word i at u offset
+// first[i] of the chunk) to the wave's queue as (row offset << 8) | u offset,
+// one bit per lane per round; 64 queued entries go to the item's region.
+__device__ __forceinline__ void staged_append(const Staged& S, uint32_t lane, uint64_t (&b)[4], const int (&first)[4],
+                                              uint32_t tag, uint32_t& queued, uint32_t* region, uint32_t cap,
+                                              uint32_t* fill) {
     for (;;) {
-        const bool has = b != 0;
+        const bool has = (b[0] | b[1] | b[2] | b[3]) != 0;
         const uint64_t ballot = __ballot(has);
         if (!ballot) return;
         if (has) {
-            S.queue[queued + lanes_below(ballot)] = tag + (uint32_t)__builtin_ctzll(b);
-            b &= b - 1;
+            const int w = b[0] ? 0 : b[1] ? 1 : b[2] ? 2 : 3;
+            const uint64_t x = w == 0 ? b[0] : w == 1 ? b[1] : w == 2 ? b[2] : b[3];
+            S.queue[queued + lanes_below(ballot)] = tag + (uint32_t)(first[w] + __builtin_ctzll(x));
+            const uint64_t rest = x & (x - 1);
+            b[0] = w == 0 ? rest : b[0];
+            b[1] = w == 1 ? rest : b[1];
+            b[2] = w == 2 ? rest : b[2];
+            b[3] = w == 3 ? rest : b[3];
         }
         queued += __popcll(ballot);
         if (queued >= kWave) {
             wave_sync();
             cand_flush(S, lane, kWave, region, cap, fill);
             wave_sync();
-            const uint32_t rest = queued - kWave;
+            const uint32_t left = queued - kWave;
             uint32_t moved = 0;
-            if (lane < rest) moved = S.queue[kWave + lane];
+            if (lane < left) moved = S.queue[kWave + lane];
             wave_sync();
-            if (lane < rest) S.queue[lane] = moved;
-            queued = rest;
+            if (lane < left) S.queue[lane] = moved;
+            queued = left;
             wave_sync();
         }
     }
@@ -310,7 +332,7 @@ __global__ __launch_bounds__(256) void k_prefilter_staged(MatchArgs m, Prefilter
 }
 
 // Row headers of one work item's rows for this thread: (left, right, tile,
-// flags) and the LDS byte offsets of the filters' mask rows.
+// flags) and the offsets (8 * line + column) of the mask rows it reads.
 struct ItemRows {
     uint4 h0[kRowsPerLane], h1[kRowsPerLane];
 };
@@ -331,8 +353,8 @@ __device__ __forceinline__ void load_item_rows(const MatchArgs& m, uint32_t p, I
 // Complete once the wave's vector memory counter drains.
 __device__ __forceinline__ void stage_async(const MatchArgs& m, uint32_t s, uint8_t* buffer, uint32_t wave,
                                             uint32_t lane) {
-    const uint4* src = (const uint4*)(m.masks + (uint64_t)s * m.stage_rows * kStageWords);
-    const uint32_t n = m.stage_rows * kStageWords / 2;
+    const uint4* src = (const uint4*)(m.masks + (uint64_t)s * m.stage_lines * (kStageLineBytes / 8));
+    const uint32_t n = m.stage_lines * (kStageLineBytes / 16);
     for (uint32_t q = wave; q * kWave < n; q += kStagedWaves) {
         const uint32_t t = q * kWave + lane;
         if (t < n) __builtin_amdgcn_global_load_lds((const void*)(src + t), (void*)(buffer + q * kWave * 16), 16, 0, 0);
@@ -341,44 +363,134 @@ __device__ __forceinline__ void stage_async(const MatchArgs& m, uint32_t s, uint
 
 // Static LDS of the staged match (one block per CU): two mask buffers, so the
 // next item's chunk can stream in while the current one is read, plus the
-// filter tables, per-subtile counters, survivor queues and the item's fill
-// counter.  Distinct arrays let the compiler see that reads of one buffer do
-// not wait for the direct-to-LDS copies into the other.
-constexpr int kStageBufferUnits = kMaxStageRows * kStageWords * 8 / 16;
-constexpr uint32_t kStagedLdsBytes = 2 * kStageBufferUnits * 16 + kMaxAllowWords * 4 + kMaxResiduals * 32 +
-                                     kTileSlots * (kMaxResiduals / 4) * 4 + kStagedWaves * kQueue * 4 + 16;
+// per-subtile counters, survivor queues and the item's fill counter.  The
+// chain after the items keeps the filter tables in the first buffer.
+// Distinct arrays let the compiler see that reads of one buffer do not wait
+// for the direct-to-LDS copies into the other.
+constexpr int kStageBufferUnits = kStageLines * kStageLineBytes / 16;
+constexpr uint32_t kStagedLdsBytes =
+    2 * kStageBufferUnits * 16 + kTileSlots * (kMaxResiduals / 4) * 4 + kStagedWaves * kQueue * 4 + 16;
+static_assert(kMaxAllowWords * 4 + kMaxResiduals * sizeof(Modulus) <= kStageBufferUnits * 16,
+              "the chain's filter tables fit one mask buffer");
 
+// A block's walk over its work items: logical position k is item
+// first + k * stride (strided: block b takes b, b + grid, ...; so the blocks
+// running at one time share pieces and chunk images in L2).  Each lane holds
+// one entry of a 64-item window of the block's sequence (desc = its item at
+// logical position window + lane), so a descriptor is a register read.
 struct ItemState {
-    uint32_t item, total;
-    uint2 it;
-    ItemRows rows;
+    uint32_t k, count;       // logical position of the current item, length of the sequence
+    uint32_t first, stride;
+    uint32_t window;
+    uint2 desc;
+    uint2 it;                // the current item (piece, chunk)
+    ItemRows rows;           // row headers of the current item's piece
 };
 
+__device__ __forceinline__ uint32_t item_index(const ItemState& st, uint32_t k) { return st.first + k * st.stride; }
+
+__device__ __forceinline__ void load_window(const uint2* items, ItemState& st, uint32_t k0, uint32_t lane) {
+    st.window = k0;
+    st.desc = items[item_index(st, min(k0 + lane, st.count - 1))];
+}
+
+// Descriptor of logical item k (k >= the window's first position).
+__device__ __forceinline__ uint2 item_at(const uint2* items, ItemState& st, uint32_t k, uint32_t lane) {
+    if (k - st.window >= (uint32_t)kWave) load_window(items, st, k, lane);
+    const int at = (int)(k - st.window);
+    return make_uint2(__builtin_amdgcn_readlane(st.desc.x, at), __builtin_amdgcn_readlane(st.desc.y, at));
+}
+
+// The two 64-bit words of a 16-byte mask half.
+__device__ __forceinline__ uint64_t half_word(const uint4& h, int w) {
+    return w == 0 ? ((uint64_t)h.y << 32) | h.x : ((uint64_t)h.w << 32) | h.z;
+}
+
+__device__ __forceinline__ uint4 and3(const uint4& a, const uint4& b, const uint4& c) {
+    return make_uint4(a.x & b.x & c.x, a.y & b.y & c.y, a.z & b.z & c.z, a.w & b.w & c.w);
+}
+
+// Bits lo..hi-1 of a mask word for a row whose chunk-relative u range is
+// [dl, dr), the word starting at chunk offset `first`.
+__device__ __forceinline__ uint64_t word_range(int dl, int dr, int first) {
+    const int lo = min(max(dl - first, 0), 64), hi = min(max(dr - first, 0), 64);
+    return hi > lo ? (~0ull << lo) & (~0ull >> (64 - hi)) : 0ull;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
+#pragma unroll
+    for (int d = kWave / 2; d >= 1; d >>= 1) x += __shfl_xor(x, d, kWave);
+    return x;
+}
+
+// Group 1 and 2 counts of the lanes that have them (has) for their subtile:
+// one add per group when the wave's lanes share the subtile, else one per lane.
+__device__ __forceinline__ void flush_groups(const MatchArgs& m, const Staged& S, uint32_t t0, bool has,
+                                             uint32_t tile, uint32_t c1, uint32_t c2, uint32_t lane) {
+    const uint64_t active = __ballot(has && (c1 | c2));
+    if (!active) return;
+    const uint32_t lead = __shfl(tile, __builtin_ctzll(active), kWave);
+    if (!__ballot(has && (c1 | c2) && tile != lead)) {
+        const uint32_t s1 = wave_sum(has ? c1 : 0u), s2 = wave_sum(has ? c2 : 0u);
+        if (lane == 0) {
+            if (s1) staged_add(m, S, t0, lead, 1, s1);
+            if (s2) staged_add(m, S, t0, lead, 2, s2);
+        }
+    } else if (has) {
+        if (c1) staged_add(m, S, t0, tile, 1, c1);
+        if (c2) staged_add(m, S, t0, tile, 2, c2);
+    }
+}
+
 // Match one work item reading chunk buffer `cur` while the next item's chunk
-// streams into `nxt`.  Returns false when the block has no further item.
+// (and, when the piece changes, its row headers) stream in.  Returns false
+// when the block has no further item.
+//
+// Per row a thread reads 16 mask halves in the rotated order of its lane
+// g = lane mod 16 (see the header): slots 0-3 (group X: filters 0-3 for
+// g < 8, 4-7 otherwise) then slots 4-7 (group Y, the other four), first the
+// half P = (g >> 2) & 1 of every slot, then the other half Q.  Group 1 counts
+// positions passing group A (X for g < 8, Y otherwise), group 2 those passing
+// both.
 template <int kF, bool kLast>
 __device__ __forceinline__ bool staged_item(const MatchArgs& m, const uint2* items, const Staged& S, ItemState& st,
                                             const uint4 (&cur)[kStageBufferUnits], uint4 (&nxt)[kStageBufferUnits],
-                                            uint32_t* fill, uint32_t lane, uint32_t wave) {
+                                            uint32_t* fills, uint32_t* wave_counts, uint32_t* piece_t0, uint32_t lane,
+                                            uint32_t wave) {
+    static_assert(kRowsPerLane == 2, "a thread's rows of a piece are counted as two records");
+    // group 1 closes after filter 3 (or after the last one), group 2 after filter 7
+    constexpr bool kCount1 = kF >= 4 || kLast;
+    constexpr bool kCount2 = kF == 8 || (kLast && kF > 4);
     const uint32_t p = st.it.x, s = st.it.y;
     const uint32_t j0 = p * kPieceRows, j1 = min(m.nv, j0 + kPieceRows);
-    const uint32_t t0 = m.rows[j0].tile;
+    const uint32_t item = item_index(st, st.k);
+    // survivors flushed during the item; the counter alternates with the item's
+    // parity, so resetting it here cannot race with the last item's readers
+    uint32_t* fill = fills + (st.k & 1);
     __builtin_amdgcn_s_waitcnt(0);   // this wave's part of the chunk and its row headers have arrived
-    if (threadIdx.x == 0) *fill = 0;
-    uint32_t* region = m.cand + (uint64_t)st.item * m.item_cap;
+    if (threadIdx.x == 0) {
+        *fill = 0;
+        *piece_t0 = st.rows.h0[0].z;   // thread 0's first row is the piece's first
+    }
+    uint32_t* region = m.cand + (uint64_t)item * m.item_cap;
     __syncthreads();
-    // the next item's chunk and rows are in flight while this one is matched
-    const uint32_t next = st.item + gridDim.x;
+    const uint32_t t0 = *piece_t0;
+    // the next item's chunk, and its piece's rows when the piece changes, are in flight meanwhile
+    const uint32_t next = st.k + 1;
     const ItemRows cur_rows = st.rows;
     uint2 nit = st.it;
-    if (next < st.total) {
-        nit = items[next];
-        load_item_rows(m, nit.x, st.rows);
+    if (next < st.count) {
+        nit = item_at(items, st, next, lane);
+        if (nit.x != p) load_item_rows(m, nit.x, st.rows);
         stage_async(m, nit.y, (uint8_t*)nxt, wave, lane);
     }
-    const uint8_t* masks = (const uint8_t*)cur;
-    const uint32_t base0 = (s * kStageWords) << 6;   // first u index of the chunk
-    uint32_t tile_cur = 0xffffffffu, acc1 = 0, acc2 = 0;
+    const uint32_t g = lane & 15, hp = (g >> 2) & 1;
+    const uint8_t* masks_p = (const uint8_t*)cur + 16 * hp;          // half P of a column
+    const uint8_t* masks_q = (const uint8_t*)cur + 16 * (hp ^ 1);    // half Q
+    const int first_p = 128 * (int)hp, first_q = 128 - first_p;      // chunk offset of each half's first word
+    const uint32_t base0 = (s * kStageWords) << 6;                   // first u index of the chunk
+    uint32_t c1[kRowsPerLane] = {}, c2[kRowsPerLane] = {}, tiles[kRowsPerLane] = {};
+    bool has[kRowsPerLane] = {};
     uint32_t queued = 0;
 #pragma unroll
     for (int t = 0; t < kRowsPerLane; ++t) {
@@ -386,60 +498,68 @@ __device__ __forceinline__ bool staged_item(const MatchArgs& m, const uint2* ite
         if (j0 + (threadIdx.x - lane) + kStagedThreads * t < j1) {   // the wave has rows in the piece
             const bool live = j < j1;
             const uint4 h0 = cur_rows.h0[t], h1 = cur_rows.h1[t];
-            const uint32_t tile = h0.z;
-            if (live && tile != tile_cur) {
-                if (tile_cur != 0xffffffffu) {
-                    if (acc1) staged_add(m, S, t0, tile_cur, 1, acc1);
-                    if (acc2) staged_add(m, S, t0, tile_cur, 2, acc2);
-                }
-                tile_cur = tile;
-                acc1 = acc2 = 0;
-            }
-            // range of the row's u indices within the chunk, clamped to the chunk's words
+            has[t] = live;
+            tiles[t] = h0.z;
+            // range of the row's u indices within the chunk
             const int dl = live ? (int)(h0.x - base0) : 0, dr = live ? (int)(h0.y - base0) : 0;
-            uint64_t b[kStageWords];
-#pragma unroll
-            for (int ww = 0; ww < kStageWords; ++ww) {
-                const int lo = min(max(dl - 64 * ww, 0), 64), hi = min(max(dr - 64 * ww, 0), 64);
-                b[ww] = hi > lo ? (~0ull << lo) & (~0ull >> (64 - hi)) : 0ull;
-            }
-            // each half (two words) of every filter's row is read before any is used
+            const uint32_t offs[kStagedFilters] = {h1.x & 0xffffu, h1.x >> 16, h1.y & 0xffffu, h1.y >> 16,
+                                                   h1.z & 0xffffu, h1.z >> 16, h1.w & 0xffffu, h1.w >> 16};
+            uint64_t both[2][2];
+            uint32_t acc_x = 0, acc_y = 0, acc2 = 0;
+            // half P of all eight slots, then half Q: the eight reads of a half are in
+            // flight together, and a half is reduced to its counts and survivors at once
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
-                uint4 mw[kF];
+                const uint8_t* base = h == 0 ? masks_p : masks_q;
+                const int first = h == 0 ? first_p : first_q;
+                uint4 v[kStagedFilters];
 #pragma unroll
-                for (int k = 0; k < kF; ++k) {
-                    const uint32_t pair = k < 2 ? h1.x : k < 4 ? h1.y : k < 6 ? h1.z : h1.w;
-                    const uint32_t off = ((pair >> (16 * (k & 1))) & 0xffffu) ^ (16u * h);
-                    mw[k] = *(const uint4*)(masks + off);
-                }
+                for (int i = 0; i < kStagedFilters; ++i) v[i] = *(const uint4*)(base + (offs[i] << 5));
+                const uint4 x01 = and3(v[0], v[1], v[2]), y01 = and3(v[4], v[5], v[6]);
+                const uint4 xh = make_uint4(x01.x & v[3].x, x01.y & v[3].y, x01.z & v[3].z, x01.w & v[3].w);
+                const uint4 yh = make_uint4(y01.x & v[7].x, y01.y & v[7].y, y01.z & v[7].z, y01.w & v[7].w);
 #pragma unroll
-                for (int k = 0; k < kF; ++k) {
-                    b[2 * h] &= ((uint64_t)mw[k].y << 32) | mw[k].x;
-                    b[2 * h + 1] &= ((uint64_t)mw[k].w << 32) | mw[k].z;
-                    if ((k & 3) == 3 || (kLast && k == kF - 1)) {
-                        const uint32_t n = __popcll(b[2 * h]) + __popcll(b[2 * h + 1]);
-                        if (k < 4) acc1 += n;
-                        else acc2 += n;
+                for (int w = 0; w < 2; ++w) {
+                    const uint64_t range = word_range(dl, dr, first + 64 * w);
+                    const uint64_t x = range & half_word(xh, w), y = range & half_word(yh, w);
+                    both[h][w] = x & half_word(yh, w);
+                    if (kCount1) {
+                        acc_x += __popcll(x);
+                        acc_y += __popcll(y);
                     }
+                    if (kCount2) acc2 += __popcll(both[h][w]);
                 }
+                // eight reads in flight at a time keep the registers within four waves per SIMD
+                __builtin_amdgcn_sched_barrier(0);
             }
-            const uint32_t tag = ((j - j0) << 8);
-#pragma unroll
-            for (int ww = 0; ww < kStageWords; ++ww)
-                staged_enqueue(S, lane, b[ww], tag + 64 * ww, queued, region, m.item_cap, fill);
+            c1[t] = g < 8 ? acc_x : acc_y;
+            c2[t] = acc2;
+            // a few survivors per wave and row: all four words in one append loop
+            uint64_t words[4] = {both[0][0], both[0][1], both[1][0], both[1][1]};
+            const int firsts[4] = {first_p, first_p + 64, first_q, first_q + 64};
+            staged_append(S, lane, words, firsts, (j - j0) << 8, queued, region, m.item_cap, fill);
         }
     }
-    if (queued) {
-        wave_sync();
-        cand_flush(S, lane, queued, region, m.item_cap, fill);
+    if (lane == 0) wave_counts[wave] = queued;
+    // a thread's two rows usually share a subtile
+    if (has[1] && (!has[0] || tiles[1] == tiles[0])) {
+        c1[0] += c1[1];
+        c2[0] += c2[1];
+        tiles[0] = tiles[1];
+        has[0] = true;
+        has[1] = false;
     }
-    if (tile_cur != 0xffffffffu) {
-        if (acc1) staged_add(m, S, t0, tile_cur, 1, acc1);
-        if (acc2) staged_add(m, S, t0, tile_cur, 2, acc2);
-    }
+    flush_groups(m, S, t0, has[0], tiles[0], c1[0], c2[0], lane);
+    flush_groups(m, S, t0, has[1], tiles[1], c1[1], c2[1], lane);
     __syncthreads();
-    if (threadIdx.x == 0) m.item_count[st.item] = *fill;
+    // the waves' queued survivors follow the flushed ones, in wave order
+    {
+        const uint32_t n = lane < (uint32_t)kStagedWaves ? wave_counts[lane] : 0u;
+        const uint32_t below = wave_sum(lane < wave ? n : 0u), queued_all = wave_sum(n);
+        const uint32_t at = *fill + below;
+        if (lane < queued && at + lane < m.item_cap) region[at + lane] = S.queue[lane];
+        if (threadIdx.x == 0) m.item_count[item] = *fill + queued_all;
+    }
     for (uint32_t t = threadIdx.x; t < (uint32_t)kTileSlots * m.ng; t += kStagedThreads) {
         const uint32_t n = S.tilecnt[t];
         if (n) {
@@ -447,8 +567,8 @@ __device__ __forceinline__ bool staged_item(const MatchArgs& m, const uint2* ite
             S.tilecnt[t] = 0;
         }
     }
-    if (next >= st.total) return false;
-    st.item = next;
+    if (next >= st.count) return false;
+    st.k = next;
     st.it = nit;
     return true;
 }
@@ -458,41 +578,52 @@ __device__ __forceinline__ bool staged_item(const MatchArgs& m, const uint2* ite
 // j0 + l + 1024 t of the piece; the next item's chunk and row headers are in
 // flight meanwhile.  Group counts of the piece's subtiles collect in LDS and
 // go to the global counters at the end of each item.  After its last item the
-// block runs its items' survivors through the residual chain.  kF = bitset filters;
-// kLast: the last residual filter is the last bitset filter (it closes a
-// group).
+// block runs its items' survivors through the residual chain.  kF = bitset
+// filters; kLast: the last residual filter is the last bitset filter (it
+// closes a group).
 template <int kF, bool kLast>
 __global__ __launch_bounds__(kStagedThreads) void k_match_staged(MatchArgs m, const uint2* items, uint32_t items_cap,
                                                                  const uint32_t* nitems) {
     __shared__ uint4 buffer0[kStageBufferUnits];
     __shared__ uint4 buffer1[kStageBufferUnits];
-    __shared__ uint32_t allow[kMaxAllowWords];
-    __shared__ Modulus mods[kMaxResiduals];
     __shared__ uint32_t tilecnt[kTileSlots * (kMaxResiduals / 4)];
     __shared__ uint32_t queues[kStagedWaves * kQueue];
-    __shared__ uint32_t fill;   // survivors of the current item
+    __shared__ uint32_t fills[2];                     // survivors flushed during an item, by item parity
+    __shared__ uint32_t wave_counts[kStagedWaves];    // survivors each wave holds at an item's end
+    __shared__ uint32_t piece_t0;                     // subtile of the current piece's first row
     const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const uint32_t total = min(*nitems, items_cap);
+    if (blockIdx.x >= total) return;
     ItemState st;
-    st.total = min(*nitems, items_cap);
-    st.item = blockIdx.x;
-    if (st.item >= st.total) return;
-    // allow and the Modulus table are whole 16-byte units (both buffers are allocated so)
-    stage_copy<kStagedThreads, 1>((uint4*)allow, (const uint4*)m.allow, (m.allow_words + 3) / 4);
-    stage_copy<kStagedThreads, 1>((uint4*)mods, (const uint4*)m.mods, m.nres * (uint32_t)sizeof(Modulus) / 16);
+    st.first = blockIdx.x;
+    st.stride = gridDim.x;
+    st.count = (total - blockIdx.x + gridDim.x - 1) / gridDim.x;
+    st.k = 0;
     for (uint32_t t = threadIdx.x; t < (uint32_t)kTileSlots * m.ng; t += kStagedThreads) tilecnt[t] = 0;
-    const Staged S{allow, mods, tilecnt, queues + wave * kQueue};
-    st.it = items[st.item];
+    const Staged S{nullptr, nullptr, tilecnt, queues + wave * kQueue};
+    load_window(items, st, 0, lane);
+    st.it = item_at(items, st, 0, lane);
     load_item_rows(m, st.it.x, st.rows);
     stage_async(m, st.it.y, (uint8_t*)buffer0, wave, lane);
     for (;;) {
-        if (!staged_item<kF, kLast>(m, items, S, st, buffer0, buffer1, &fill, lane, wave)) break;
-        if (!staged_item<kF, kLast>(m, items, S, st, buffer1, buffer0, &fill, lane, wave)) break;
+        if (!staged_item<kF, kLast>(m, items, S, st, buffer0, buffer1, fills, wave_counts, &piece_t0, lane, wave))
+            break;
+        if (!staged_item<kF, kLast>(m, items, S, st, buffer1, buffer0, fills, wave_counts, &piece_t0, lane, wave))
+            break;
     }
-    // the residual chain over this block's items' survivors, 64 at a time; the
-    // block's waves take the 64-survivor slices of its items in turn
-    __syncthreads();   // every item's region and count is written
+    // the residual chain over this block's items' survivors, 64 at a time, with
+    // the filter tables in the first mask buffer; the block's waves take the
+    // 64-survivor slices of its items in turn
+    __syncthreads();   // every item's region and count is written, no chunk is read any more
+    uint32_t* allow = (uint32_t*)buffer0;
+    Modulus* mods = (Modulus*)(allow + kMaxAllowWords);
+    // allow and the Modulus table are whole 16-byte units (both buffers are allocated so)
+    stage_copy<kStagedThreads, 1>((uint4*)allow, (const uint4*)m.allow, (m.allow_words + 3) / 4);
+    stage_copy<kStagedThreads, 1>((uint4*)mods, (const uint4*)m.mods, m.nres * (uint32_t)sizeof(Modulus) / 16);
+    __syncthreads();
+    const Staged C{allow, mods, tilecnt, queues + wave * kQueue};
     uint32_t slice = 0;
-    for (uint32_t item = blockIdx.x; item < st.total; item += gridDim.x) {
+    for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
         uint32_t n = m.item_count[item];
         if (n > m.item_cap) {
             if (threadIdx.x == 0) atomicMax((unsigned long long*)m.cand_max, (unsigned long long)n);
@@ -505,7 +636,7 @@ __global__ __launch_bounds__(kStagedThreads) void k_match_staged(MatchArgs m, co
             if (slice % kStagedWaves != wave) continue;
             const uint32_t count = min(n - first, (uint32_t)kWave);
             const uint32_t e = lane < count ? region[first + lane] : 0u;
-            staged_chain(m, S, lane, count, j0 + (e >> 8), u0 + (e & 0xffu));
+            staged_chain(m, C, lane, count, j0 + (e >> 8), u0 + (e & 0xffu));
         }
     }
     __syncthreads();
