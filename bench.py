@@ -30,14 +30,19 @@ from pathlib import Path
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-# The dominant kernel, the staged bitset intersection, is bound by LDS
-# bandwidth: per 64 focused positions (one mask word) it reads one 8-byte word
-# of each of the eight bitset filters' masks from LDS, so its algorithmic
-# traffic is 8 * 8 / 64 = 1 LDS byte per focused position.
+# The dominant kernel is the staged bitset match.  Per 64 focused positions
+# (one mask word) it reads one 8-byte word of each of the eight bitset
+# filters' masks from LDS, so its algorithmic traffic is 8 * 8 / 64 = 1 LDS
+# byte per focused position.
 STAGED_FILTERS = 8
 LDS_BYTES_PER_POSITION = STAGED_FILTERS * 8 / 64
-# MI355X LDS bandwidth: 256 CUs x 128 bytes per clock x 2.4 GHz = 78.6e12 B/s.
-LDS_PEAK = 256 * 128 * 2.4e9
+# MI355X LDS read bandwidth for ds_read_b128: 256 CUs x 256 bytes per clock x
+# 2.4 GHz = 157.3e12 B/s (`bench.py --lds-probe` reads about 240 B/clk per CU
+# with conflict-free addresses).
+LDS_PEAK = 256 * 256 * 2.4e9
+# Integer VALU issue: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz = 39.3e12 lane operations/s.
+VALU_PEAK = 256 * 4 * 16 * 2.4e9
+HBM_PEAK = 8.0e12
 
 
 def parse() -> argparse.Namespace:
@@ -56,18 +61,38 @@ def parse() -> argparse.Namespace:
     ap.add_argument('--served-load', type=float, default=0.8, help='offered load as a fraction of capacity')
     ap.add_argument('--copy-probe', type=int, default=0, metavar='BYTES',
                     help='only run the plain copy kernel used to calibrate the HBM byte counters')
+    ap.add_argument('--lds-probe', action='store_true',
+                    help='only measure the LDS read bandwidth of the staged access pattern')
     return ap.parse_args()
 
 
-def measured_traffic() -> int | None:
-    """HBM bytes per intersection launch from the committed counter summary, if present."""
+def measured(key: str) -> int | None:
+    """A per-launch figure of the match kernel from the committed counter summary, if present."""
     path = ROOT / 'profiles' / 'pmc_match_staged_p619_batch.md'
     if not path.exists():
         return None
     for line in path.read_text().splitlines():
-        if line.startswith('traffic_bytes_per_launch:'):
+        if line.startswith(key + ':'):
             return int(line.split(':', 1)[1])
     return None
+
+
+def measured_traffic() -> int | None:
+    """HBM bytes per match launch (FETCH_SIZE and WRITE_SIZE, scaled by the copy calibration)."""
+    return measured('traffic_bytes_per_launch')
+
+
+def lds_probe() -> None:
+    """LDS read bandwidth of three access patterns (pcube_lds_probe), one JSON line each."""
+    import ctypes
+    from psuedocubes_amd import native
+    names = ('consecutive', 'staged rotated columns', 'random 16-byte units')
+    seconds, nbytes = ctypes.c_double(), ctypes.c_double()
+    for pattern, name in enumerate(names):
+        native.call('pcube_lds_probe', pattern, 2000, ctypes.byref(seconds), ctypes.byref(nbytes))
+        rate = nbytes.value / seconds.value
+        print(json.dumps(dict(lds_probe=name, seconds=seconds.value, bytes=nbytes.value, tbps=rate / 1e12,
+                              bytes_per_clock_per_cu_at_2_4ghz=rate / 256 / 2.4e9)), flush=True)
 
 
 def cpu_baseline(config, seconds: float) -> dict:
@@ -204,6 +229,9 @@ def main() -> None:
 
     config = Config(619, 10**27, 3 * 10**27, A619, B619)
     runtime = Runtime(time.monotonic() + 3600, 16 * 1024**3)
+    if args.lds_probe:
+        lds_probe()
+        return
     if args.copy_probe:
         import ctypes
         from psuedocubes_amd import native
@@ -255,6 +283,16 @@ def main() -> None:
                                 frac=achieved / LDS_PEAK, traffic=measured_traffic(),
                                 kernel='k_match_staged<8>', kernel_ms=1000 * intersection_seconds,
                                 lds_bytes_per_position=LDS_BYTES_PER_POSITION, positions_per_launch=per_launch))
+    # executed integer-VALU lane operations and HBM bytes of the same kernel, per
+    # launch from the committed counter summary, over this run's kernel time
+    valu = measured('valu_instructions_per_launch')
+    if valu:
+        ops = valu * 64 / intersection_seconds
+        result['valu'] = dict(achieved=ops / 1e12, peak=VALU_PEAK / 1e12, unit='Tops/s', frac=ops / VALU_PEAK,
+                              instructions_per_launch=valu)
+    if result['roofline']['traffic']:
+        rate = result['roofline']['traffic'] / intersection_seconds
+        result['hbm'] = dict(achieved=rate / 1e9, peak=HBM_PEAK / 1e9, unit='GB/s', frac=rate / HBM_PEAK)
     if not args.no_lease:
         first = 575222931200000000
         compute_lease(matcher, af, bf, first, first + 4 * BATCH_WIDTH)  # warm the lease buffers

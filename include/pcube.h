@@ -231,6 +231,15 @@ int pcube_arithmetic(const uint64_t *in, uint64_t *out, uint64_t count);
  * kernel; *seconds is its event-timed duration.  No reference counterpart. */
 int pcube_copy_probe(uint64_t bytes, double *seconds);
 
+/* --- measurement: LDS read bandwidth of the staged engine's access pattern - */
+/* One 1024-thread block per CU, each thread keeping 16 ds_read_b128 in flight
+ * for `rounds` rounds over 64 KB of LDS.  pattern 0: consecutive 16-byte units
+ * (the peak); 1: the staged engine's lane-rotated column reads at random lines;
+ * 2: random 16-byte units per lane.  *seconds is the event-timed duration of
+ * the second of two launches, *bytes the LDS bytes one launch reads.  No
+ * reference counterpart. */
+int pcube_lds_probe(uint32_t pattern, uint32_t rounds, double *seconds, double *bytes);
+
 #ifdef __cplusplus
 }
 #endif

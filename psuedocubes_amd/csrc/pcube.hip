@@ -998,6 +998,7 @@ const Kernel kKernels[] = {
     {"match_direct", (const void*)k_match_direct},
     {"arithmetic", (const void*)k_arithmetic},
     {"copy_probe", (const void*)k_copy_probe},
+    {"lds_probe", (const void*)k_lds_probe},
     {"batch_rows", (const void*)k_batch_rows},
     {"batch_masks", (const void*)k_batch_masks},
     {"batch_rho", (const void*)k_batch_rho},
@@ -2009,6 +2010,30 @@ int pcube_host_verify(const uint64_t* limbs, uint64_t n, uint32_t p, uint8_t* lo
         is_cube[i] = ((u128)lo * lo * lo == x) ? 1 : 0;
         cube_root[i] = lo;
     }
+    return PCUBE_OK;
+}
+
+int pcube_lds_probe(uint32_t pattern, uint32_t rounds, double* seconds, double* bytes) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    if (pattern > 2 || !rounds) return fail(PCUBE_INVALID, "probe pattern 0..2 and at least one round");
+    const uint32_t grid = (uint32_t)g.compute_units;
+    void* out = nullptr;
+    TRY(dev_alloc((uint64_t)grid * kStagedThreads * 4, &out));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    hipLaunchKernelGGL(k_lds_probe, dim3(grid), dim3(kStagedThreads), 0, g.stream, pattern, rounds, (uint32_t*)out);
+    HIP_TRY(hipEventRecord(e0, g.stream));
+    hipLaunchKernelGGL(k_lds_probe, dim3(grid), dim3(kStagedThreads), 0, g.stream, pattern, rounds, (uint32_t*)out);
+    HIP_TRY(hipEventRecord(e1, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    if (seconds) *seconds = ms / 1000.0;
+    if (bytes) *bytes = (double)grid * kStagedThreads * rounds * 16 * 16;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    dev_free(out);
     return PCUBE_OK;
 }
 

@@ -646,6 +646,50 @@ __global__ __launch_bounds__(kStagedThreads) void k_match_staged(MatchArgs m, co
     }
 }
 
+// LDS read probe, the calibration of the staged engine's LDS roofline: each
+// thread keeps 16 ds_read_b128 in flight per round over a 64 KB buffer.
+// Pattern 0: consecutive 16-byte units (no bank conflicts, the peak);
+// 1: the staged engine's reads (random lines, each step's lane-rotated column
+// and half); 2: every lane reads a random 16-byte unit (an unrotated layout).
+// Every address stays below 64 KB (lines < 256).  A round costs two 3-input
+// ORs per read, so the VALU keeps pace with the LDS.
+constexpr int kProbeUnits = 4096;
+__global__ __launch_bounds__(kStagedThreads) void k_lds_probe(uint32_t pattern, uint32_t rounds, uint32_t* out) {
+    __shared__ uint4 buf[kProbeUnits];
+    for (uint32_t t = threadIdx.x; t < (uint32_t)kProbeUnits; t += kStagedThreads)
+        buf[t] = make_uint4(t, 3 * t, 5 * t, 7 * t);
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave, g = lane & 15;
+    uint32_t a[16];
+#pragma unroll
+    for (uint32_t i = 0; i < 16; ++i) {
+        uint32_t h = (lane * 977u + wave * 131u + i * 7919u + blockIdx.x * 104729u) * 0x9e3779b1u;
+        h ^= h >> 15;
+        if (pattern == 0) {
+            a[i] = (((wave * 16 + i) * kWave + lane) * 16) & 0xfff0u;
+        } else if (pattern == 1) {
+            const uint32_t c = i ^ g;
+            a[i] = (h >> 8) % 256u * kStageLineBytes + ((c & 3) + 4 * (c >> 3)) * 32 + ((c >> 2) & 1) * 16;
+        } else {
+            a[i] = (h >> 8) % (uint32_t)kProbeUnits * 16;
+        }
+    }
+    uint32_t acc0 = 0, acc1 = 0;
+    const uint8_t* base = (const uint8_t*)buf;
+    for (uint32_t r = 0; r < rounds; ++r) {
+        asm volatile("" ::: "memory");   // the reads of every round are issued again
+        uint4 v[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) v[i] = *(const uint4*)(base + a[i]);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            acc0 = acc0 | v[i].x | v[i].y;
+            acc1 = acc1 | v[i].z | v[i].w;
+        }
+    }
+    out[blockIdx.x * kStagedThreads + threadIdx.x] = acc0 ^ acc1;
+}
+
 #define PCUBE_STAGED(F)                                                                               \
     template __global__ void k_match_staged<F, false>(MatchArgs, const uint2*, uint32_t, const uint32_t*); \
     template __global__ void k_match_staged<F, true>(MatchArgs, const uint2*, uint32_t, const uint32_t*);
