@@ -150,7 +150,7 @@ int pcube_matcher_destroy(int matcher);
  *   engine       0 = residue-bitset engine, 1 = direct per-position engine
  *   seconds      host, 3 doubles (may be NULL): prefilter build (rows, bitsets,
  *                residue cache), matching (every kernel after the prefilter),
- *                and of that the bitset-intersection kernel alone
+ *                and of that the staged match kernel alone (engine 0)
  * Returns OK, OVERFLOW (a subtile emitted more than cap; counters valid),
  * UNDERFLOW, RANGE, INVALID. */
 int pcube_match(int matcher, uint64_t us, uint64_t nu, uint64_t vs, uint64_t nv, uint64_t v_first,
