@@ -119,13 +119,17 @@ int pcube_tile_ranges(uint64_t A, uint64_t B, uint64_t lo0, uint64_t lo1, uint64
 
 /* prepare() of one compute batch from its canonical v range [first, end) and
  * subtile width (pseudocube/batch_experiment.py:27-32, 51-57): the geometry of
- * pcube_tile_ranges into tiles, then pcube_prepare on it.  Errors as
+ * pcube_tile_ranges into tiles, then pcube_prepare on it.  totals (host, 3
+ * uint64, may be NULL) receives the batch's focus_u = sum of (u_end - u_first),
+ * its cartesian opportunities = sum of (u_end - u_first) * (v_end - v_first)
+ * over the subtile bounds, and the largest single subtile's opportunities
+ * (pseudocube/batch_experiment.py:53-57, 150-166).  Errors as
  * pcube_tile_ranges, and RANGE when a window spans more than 4096 focus
  * periods (nothing is generated). */
 int pcube_prepare_range(int focus_u, int focus_v, uint64_t A, uint64_t B, uint64_t lo0, uint64_t lo1, uint64_t hi0,
                         uint64_t hi1, uint64_t first, uint64_t end, uint64_t width, uint64_t *tiles, uint32_t *ntiles,
                         uint64_t us_out, uint64_t us_cap, uint64_t vs_out, uint64_t vs_cap, uint64_t *nu, uint64_t *nv,
-                        uint64_t *bounds);
+                        uint64_t *bounds, uint64_t *totals);
 
 /* np.searchsorted(sorted, queries, 'left') on the device; host queries/out
  * (pseudocube/batch_experiment.py:41-42). */

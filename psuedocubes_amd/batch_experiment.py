@@ -36,6 +36,7 @@ class Prepared:
     bounds: list[tuple[int, int, int, int]]
     prep_seconds: float
     ranges: list[tuple[int, int, int, int]] | None = None   # tile_ranges(first, end, width), when known
+    totals: tuple[int, int] | None = None   # (focus_u, cartesian_opportunities) of the bounds, when known
 
 
 def tile_ranges(config: Config, first: int, end: int, width: int) -> list[tuple[int, int, int, int]]:
@@ -108,15 +109,15 @@ def prepare(runtime: Runtime, config: Config, af: GPUFocus, bf: GPUFocus,
         raise ValueError('experiment allows at most 64 bounded subtiles')
     cap = 1000000
     us, vs = DeviceArray.empty(cap), DeviceArray.empty(cap)
-    tiles = np.empty(4 * 64, dtype=np.uint64)
-    bounds = np.empty(4 * 64, dtype=np.uint64)
+    out = np.empty(8 * 64 + 3, dtype=np.uint64)   # tiles, bounds, totals
     ntiles, nu, nv = ctypes.c_uint32(), ctypes.c_uint64(), ctypes.c_uint64()
     runtime.check()
     mask = (1 << 64) - 1
+    base = out.ctypes.data
     code = native.lib().pcube_prepare_range(
         af.handle, bf.handle, config.A, config.B, config.lo & mask, config.lo >> 64, config.hi & mask, config.hi >> 64,
-        first, end, width, native.ptr64(tiles), ctypes.byref(ntiles), us.address, cap, vs.address, cap,
-        ctypes.byref(nu), ctypes.byref(nv), native.ptr64(bounds))
+        first, end, width, native.ptr64_at(base), ctypes.byref(ntiles), us.address, cap, vs.address, cap,
+        ctypes.byref(nu), ctypes.byref(nv), native.ptr64_at(base + 8 * 256), native.ptr64_at(base + 8 * 512))
     if code == native.RANGE:
         raise OverflowError('focus window exceeds bounded period count')
     if code == native.OVERFLOW:
@@ -124,14 +125,17 @@ def prepare(runtime: Runtime, config: Config, af: GPUFocus, bf: GPUFocus,
     native.check(code, 'pcube_prepare_range')
     us.size, vs.size = nu.value, nv.value
     n = 4 * ntiles.value
-    flat_t, flat_b = tiles[:n].tolist(), bounds[:n].tolist()
-    ranges = [tuple(flat_t[i:i + 4]) for i in range(0, n, 4)]
-    rows = [tuple(flat_b[i:i + 4]) for i in range(0, n, 4)]
-    for a, b, c, d in rows:
-        if (b - a) * (d - c) > 50_000_000:
-            raise OverflowError('per-subtile candidate opportunity limit exceeded')
+    flat = out.tolist()
+    focus_u, opportunities, largest = flat[512:515]
+    if largest > 50_000_000:
+        raise OverflowError('per-subtile candidate opportunity limit exceeded')
+    quads = iter(flat[:n])
+    ranges = list(zip(quads, quads, quads, quads))
+    quads = iter(flat[256:256 + n])
+    rows = list(zip(quads, quads, quads, quads))
     runtime.check()
-    return Prepared(first, end, width, us, vs, rows, time.perf_counter() - started, ranges)  # type: ignore[arg-type]
+    return Prepared(first, end, width, us, vs, rows, time.perf_counter() - started, ranges,  # type: ignore[arg-type]
+                    (focus_u, opportunities))
 
 
 def python_witnesses(values: list[int], p: int) -> list[dict[str, object]]:
@@ -155,7 +159,8 @@ def scope_skeleton(ranges: list[tuple[int, int, int, int]]) -> list[dict[str, An
 def batch_row(config: Config, filter_groups: list[list[int]], ranges: list[tuple[int, int, int, int]],
               bounds: list[tuple[int, int, int, int]], nv: int, counters: np.ndarray, per_tile: list[list[int]],
               output_cap: int, timings: dict[str, float], started: float,
-              witnesses: Any = python_witnesses, skeleton: list[dict[str, Any]] | None = None) -> dict[str, Any]:
+              witnesses: Any = python_witnesses, skeleton: list[dict[str, Any]] | None = None,
+              totals: tuple[int, int] | None = None) -> dict[str, Any]:
     """Host collection and verification of one batch (`pseudocube/batch_experiment.py:121-167`).
 
     Raises ArithmeticError on underflow, duplicates, a value failing the
@@ -199,10 +204,13 @@ def batch_row(config: Config, filter_groups: list[list[int]], ranges: list[tuple
         scope['output_sha256'] = sha
     verification_seconds = time.perf_counter() - verification_started
     groups = counters[:, 1:ng + 1].sum(axis=0, dtype=np.uint64).tolist() if table else [0] * ng
-    focus_u = opportunities = 0
-    for a, b, c, d in bounds:
-        focus_u += b - a
-        opportunities += (b - a) * (d - c)
+    if totals is not None:
+        focus_u, opportunities = totals
+    else:
+        focus_u = opportunities = 0
+        for a, b, c, d in bounds:
+            focus_u += b - a
+            opportunities += (b - a) * (d - c)
     return dict(status='PASS', host_verified=True, focus_u=focus_u, focus_v=int(nv),
                 cartesian_opportunities=opportunities,
                 valid_focused_candidates=int(counters[:, 0].sum(dtype=np.uint64)) if table else 0,
@@ -245,7 +253,7 @@ class FusedMatcher:
         self.intersection_seconds = seconds[2]
         timings = dict(kernel_seconds=seconds[1], focus_enumeration_seconds=batch.prep_seconds)
         row = batch_row(self.config, self.filter_groups, ranges, batch.bounds, int(batch.vs.size), counters,
-                        per_tile, output_cap, timings, started, skeleton=skeleton)
+                        per_tile, output_cap, timings, started, skeleton=skeleton, totals=batch.totals)
         row['kernel_registers'] = int(self.kernel['registers'])
         row['kernel_local_bytes'] = int(self.kernel['local_bytes'])
         row['matching_layout'] = self.layout

@@ -1365,7 +1365,7 @@ int pcube_tile_ranges(uint64_t A, uint64_t B, uint64_t lo0, uint64_t lo1, uint64
 int pcube_prepare_range(int focus_u, int focus_v, uint64_t A, uint64_t B, uint64_t lo0, uint64_t lo1, uint64_t hi0,
                         uint64_t hi1, uint64_t first, uint64_t end, uint64_t width, uint64_t* tiles, uint32_t* ntiles,
                         uint64_t us_out, uint64_t us_cap, uint64_t vs_out, uint64_t vs_cap, uint64_t* nu, uint64_t* nv,
-                        uint64_t* bounds) {
+                        uint64_t* bounds, uint64_t* totals) {
     std::lock_guard<std::recursive_mutex> lock(g_mu);
     Focus* fu = focus_at(focus_u);
     Focus* fv = focus_at(focus_v);
@@ -1378,7 +1378,21 @@ int pcube_prepare_range(int focus_u, int focus_v, uint64_t A, uint64_t B, uint64
         return fail(PCUBE_RANGE, "focus window exceeds bounded period count");
     if ((end - 1) / fv->M - first / fv->M + 1 > 4096)
         return fail(PCUBE_RANGE, "focus window exceeds bounded period count");
-    return pcube_prepare(focus_u, focus_v, n, tiles, us_out, us_cap, vs_out, vs_cap, nu, nv, bounds);
+    TRY(pcube_prepare(focus_u, focus_v, n, tiles, us_out, us_cap, vs_out, vs_cap, nu, nv, bounds));
+    if (totals) {
+        // bounds hold (u_first, u_end, v_first, v_end) index pairs < 2^32, so every product fits
+        uint64_t focus = 0, opportunities = 0, largest = 0;
+        for (uint32_t k = 0; k < n; ++k) {
+            const uint64_t du = bounds[4 * k + 1] - bounds[4 * k], dv = bounds[4 * k + 3] - bounds[4 * k + 2];
+            focus += du;
+            opportunities += du * dv;
+            largest = std::max(largest, du * dv);
+        }
+        totals[0] = focus;
+        totals[1] = opportunities;
+        totals[2] = largest;
+    }
+    return PCUBE_OK;
 }
 
 int pcube_searchsorted(uint64_t sorted, uint64_t n, const uint64_t* queries, uint64_t nq, uint64_t* out) {

@@ -623,20 +623,30 @@ __global__ __launch_bounds__(kStagedThreads) void k_match_staged(MatchArgs m, co
     __syncthreads();
     const Staged C{allow, mods, tilecnt, queues + wave * kQueue};
     uint32_t slice = 0;
-    for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
-        uint32_t n = m.item_count[item];
-        if (n > m.item_cap) {
-            if (threadIdx.x == 0) atomicMax((unsigned long long*)m.cand_max, (unsigned long long)n);
-            n = m.item_cap;
-        }
-        const uint2 it = items[item];
-        const uint32_t j0 = it.x * kPieceRows, u0 = (it.y * kStageWords) << 6;
-        const uint32_t* region = m.cand + (uint64_t)item * m.item_cap;
-        for (uint32_t first = 0; first < n; first += kWave, ++slice) {
-            if (slice % kStagedWaves != wave) continue;
-            const uint32_t count = min(n - first, (uint32_t)kWave);
-            const uint32_t e = lane < count ? region[first + lane] : 0u;
-            staged_chain(m, C, lane, count, j0 + (e >> 8), u0 + (e & 0xffu));
+    // the survivor counts and descriptors of up to 64 items arrive in one round
+    // trip, a lane each; the items are then walked in registers
+    for (uint32_t k0 = 0; k0 < st.count; k0 += kWave) {
+        const uint32_t mine = item_index(st, min(k0 + lane, st.count - 1));
+        const uint32_t n_lane = m.item_count[mine];
+        const uint2 it_lane = items[mine];
+        const uint32_t kn = min(st.count - k0, (uint32_t)kWave);
+        for (uint32_t k = 0; k < kn; ++k) {
+            const uint32_t item = item_index(st, k0 + k);
+            uint32_t n = __builtin_amdgcn_readlane(n_lane, (int)k);
+            if (n > m.item_cap) {
+                if (threadIdx.x == 0) atomicMax((unsigned long long*)m.cand_max, (unsigned long long)n);
+                n = m.item_cap;
+            }
+            const uint32_t piece = __builtin_amdgcn_readlane(it_lane.x, (int)k);
+            const uint32_t chunk = __builtin_amdgcn_readlane(it_lane.y, (int)k);
+            const uint32_t j0 = piece * kPieceRows, u0 = (chunk * kStageWords) << 6;
+            const uint32_t* region = m.cand + (uint64_t)item * m.item_cap;
+            for (uint32_t first = 0; first < n; first += kWave, ++slice) {
+                if (slice % kStagedWaves != wave) continue;
+                const uint32_t count = min(n - first, (uint32_t)kWave);
+                const uint32_t e = lane < count ? region[first + lane] : 0u;
+                staged_chain(m, C, lane, count, j0 + (e >> 8), u0 + (e & 0xffu));
+            }
         }
     }
     __syncthreads();

@@ -42,7 +42,7 @@ SIGNATURES: dict[str, list[Any]] = {
     'pcube_focus_windows': [c_int, u32, P_u64, P_u64, u64, u64, P_u64],
     'pcube_tile_ranges': [u64, u64, u64, u64, u64, u64, u64, u64, u64, P_u64, P_u32],
     'pcube_prepare_range': [c_int, c_int, u64, u64, u64, u64, u64, u64, u64, u64, u64, P_u64, P_u32, u64, u64, u64,
-                            u64, P_u64, P_u64, P_u64],
+                            u64, P_u64, P_u64, P_u64, P_u64],
     'pcube_prepare': [c_int, c_int, u32, P_u64, u64, u64, u64, u64, P_u64, P_u64, P_u64],
     'pcube_searchsorted': [u64, u64, P_u64, u64, P_u64],
     'pcube_matcher_create': [u64, u64, u64, u64, u64, u64, P_u32, u32, P_int],
@@ -108,6 +108,11 @@ def call(name: str, *args: Any) -> int:
 
 def ptr64(array: np.ndarray) -> Any:
     return array.ctypes.data_as(P_u64)
+
+
+def ptr64_at(address: int) -> Any:
+    """A uint64 pointer at a host address (inside an array the caller keeps alive)."""
+    return ctypes.cast(address, P_u64)
 
 
 def ptr32(array: np.ndarray) -> Any:
