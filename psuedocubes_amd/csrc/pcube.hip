@@ -805,23 +805,23 @@ int match_enqueue(Matcher& mt, const uint64_t* us, uint64_t nu, const uint64_t* 
     return PCUBE_OK;
 }
 
-// Enqueue the pending batch of a matcher: clear its counters and status words,
-// run the engine, download [status words, counters] to the matcher's pinned
-// buffer and record ev[5] after the download.
+// Enqueue the pending batch of a matcher: clear its status words and counters,
+// run the engine, download both to the matcher's pinned buffer and record
+// ev[5] after the download.  The counters follow the four status words in the
+// status buffer, so one fill clears and one copy downloads [status, counters].
 int match_launch(Matcher& mt) {
     const Pending& pd = mt.pending;
     const uint64_t nc = (uint64_t)pd.ntiles * (mt.ng + 3);
     hipEvent_t e0 = mt.ev[0], e1 = mt.ev[1], e2 = mt.ev[2], k0 = mt.ev[3], k1 = mt.ev[4];
+    uint64_t* head = mt.status.as<uint64_t>();
     hipLaunchKernelGGL(k_fill2_u64, dim3(std::min<uint64_t>(blocks_for(nc + 4, 256), 4096)), dim3(256), 0, g.stream,
-                       mt.counters.as<uint64_t>(), nc, mt.status.as<uint64_t>(), (uint64_t)4);
+                       head, nc + 4, head, (uint64_t)0);
     HIP_TRY(hipEventRecord(e0, g.stream));
     TRY(match_enqueue(mt, (const uint64_t*)pd.us, pd.nu, (const uint64_t*)pd.vs, pd.nv, pd.v_first, pd.tile_width,
-                      pd.ntiles, pd.cap, pd.engine, mt.counters.as<uint64_t>(), 0, pd.list_cap, e1, e2, k0, k1));
+                      pd.ntiles, pd.cap, pd.engine, head + 4, 0, pd.list_cap, e1, e2, k0, k1));
     if (pd.nv == 0)
         for (hipEvent_t e : {e1, k0, k1, e2}) HIP_TRY(hipEventRecord(e, g.stream));
-    uint64_t* hd = mt.down.as<uint64_t>();
-    HIP_TRY(hipMemcpyAsync(hd, mt.status.p, 4 * 8, hipMemcpyDeviceToHost, g.stream));
-    HIP_TRY(hipMemcpyAsync(hd + 4, mt.counters.p, nc * 8, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipMemcpyAsync(mt.down.p, head, (nc + 4) * 8, hipMemcpyDeviceToHost, g.stream));
     HIP_TRY(hipEventRecord(mt.ev[5], g.stream));
     return PCUBE_OK;
 }
@@ -1517,7 +1517,7 @@ int pcube_match_start(int matcher, uint64_t us, uint64_t nu, uint64_t vs, uint64
     pd.ntiles = ntiles; pd.cap = cap; pd.engine = engine;
     const uint64_t nc = (uint64_t)ntiles * (mt->ng + 3);
     pd.list_cap = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)ntiles * cap, 1ull << 22));
-    TRY(mt->counters.ensure(nc * 8));
+    TRY(mt->status.ensure((nc + 4) * 8));   // status words, then the batch's counters
     TRY(mt->list.ensure(pd.list_cap * 24));
     TRY(mt->down.ensure((nc + 4) * 8));
     for (hipEvent_t& e : mt->ev)
