@@ -501,16 +501,14 @@ int wheel_check(uint64_t nseg) {
 // whose inputs are on the device.  Coordinates at positions >= cap are
 // dropped.  wh.win_info receives the n + 1 window offsets (the last is the
 // total) and, at n + 1, the overflow flag.
-int wheel_enqueue(Wheel& wh, const WheelInputs& in, uint64_t* out, uint64_t cap, uint32_t slot_cap) {
-    TRY(wh.win_info.ensure(((uint64_t)in.n + 2) * 8));
-    if (in.nseg == 0) return fill64(wh.win_info.as<uint64_t>(), (uint64_t)in.n + 2, 0);
+// Scratch buffers and kernel arguments of one wheel pass (in.nseg > 0).
+int wheel_setup(Wheel& wh, const WheelInputs& in, uint32_t slot_cap, WheelArgs& a, uint64_t& nchunks) {
     const uint64_t nwaves = in.nseg * kWheelWaves;
-    const uint64_t nchunks = (in.nseg + kWheelScanChunk - 1) / kWheelScanChunk;
+    nchunks = (in.nseg + kWheelScanChunk - 1) / kWheelScanChunk;
     TRY(wh.slots.ensure(nwaves * slot_cap * 4));
     TRY(wh.wave_count.ensure(nwaves * 4));
     TRY(wh.seg_excl.ensure(in.nseg * 4));
     TRY(wh.chunk_total.ensure(nchunks * 8));
-    WheelArgs a;
     memset(&a, 0, sizeof a);
     a.R1 = wh.R1.as<uint2>();
     a.windows = in.windows;
@@ -523,6 +521,15 @@ int wheel_enqueue(Wheel& wh, const WheelInputs& in, uint64_t* out, uint64_t cap,
     a.n1 = wh.n1; a.nwin = in.n; a.nouter = wh.nouter; a.allow_words = wh.allow_words;
     a.packed = wh.packed; a.slot_cap = slot_cap;
     for (uint32_t j = 0; j < wh.nouter; ++j) a.outer[j] = wh.outer[j];
+    return PCUBE_OK;
+}
+
+int wheel_enqueue(Wheel& wh, const WheelInputs& in, uint64_t* out, uint64_t cap, uint32_t slot_cap) {
+    TRY(wh.win_info.ensure(((uint64_t)in.n + 2) * 8));
+    if (in.nseg == 0) return fill64(wh.win_info.as<uint64_t>(), (uint64_t)in.n + 2, 0);
+    WheelArgs a;
+    uint64_t nchunks = 0;
+    TRY(wheel_setup(wh, in, slot_cap, a, nchunks));
     void* args[] = {&a};
     HIP_TRY(hipLaunchKernel(wheel_kernel(wh.packed, wh.nouter), dim3((uint32_t)in.nseg), dim3(kWheelThreads), args, 0,
                             g.stream));
@@ -532,6 +539,39 @@ int wheel_enqueue(Wheel& wh, const WheelInputs& in, uint64_t* out, uint64_t cap,
     hipLaunchKernelGGL(k_wheel_compact, dim3((uint32_t)in.nseg), dim3(kWheelThreads), 0, g.stream, a,
                        (const uint32_t*)wh.seg_excl.as<uint32_t>(), (const uint64_t*)wh.chunk_total.as<uint64_t>(),
                        out, cap, wh.win_info.as<uint64_t>());
+    return PCUBE_OK;
+}
+
+// The passes of two different wheels (a batch's u and v windows) as three
+// paired launches, so their latency chains overlap; two single passes when
+// the wheels use different instantiations or a window is empty.
+int wheel_enqueue_pair(Wheel& wa, const WheelInputs& ia, uint64_t* out_a, uint64_t cap_a, uint32_t slot_a, Wheel& wb,
+                       const WheelInputs& ib, uint64_t* out_b, uint64_t cap_b, uint32_t slot_b) {
+    if (&wa == &wb || ia.nseg == 0 || ib.nseg == 0 || wa.packed != wb.packed || wa.nouter != wb.nouter ||
+        ia.nseg + ib.nseg > (1ull << 31)) {
+        TRY(wheel_enqueue(wa, ia, out_a, cap_a, slot_a));
+        return wheel_enqueue(wb, ib, out_b, cap_b, slot_b);
+    }
+    TRY(wa.win_info.ensure(((uint64_t)ia.n + 2) * 8));
+    TRY(wb.win_info.ensure(((uint64_t)ib.n + 2) * 8));
+    WheelArgs a, b;
+    uint64_t chunks_a = 0, chunks_b = 0;
+    TRY(wheel_setup(wa, ia, slot_a, a, chunks_a));
+    TRY(wheel_setup(wb, ib, slot_b, b, chunks_b));
+    const uint32_t blocks = (uint32_t)(ia.nseg + ib.nseg);
+    void* args[] = {&a, &b};
+    HIP_TRY(hipLaunchKernel(wheel_pair_kernel(wa.packed, wa.nouter), dim3(blocks), dim3(kWheelThreads), args, 0,
+                            g.stream));
+    const WheelScan sa{wa.wave_count.as<uint32_t>(), ia.nseg, wa.seg_excl.as<uint32_t>(),
+                       wa.chunk_total.as<uint64_t>(), (uint32_t)chunks_a};
+    const WheelScan sb{wb.wave_count.as<uint32_t>(), ib.nseg, wb.seg_excl.as<uint32_t>(),
+                       wb.chunk_total.as<uint64_t>(), (uint32_t)chunks_b};
+    hipLaunchKernelGGL(k_wheel_seg_scan_pair, dim3((uint32_t)(chunks_a + chunks_b)), dim3(256), 0, g.stream, sa, sb);
+    const WheelOut oa{wa.seg_excl.as<uint32_t>(), wa.chunk_total.as<uint64_t>(), out_a, cap_a,
+                      wa.win_info.as<uint64_t>()};
+    const WheelOut ob{wb.seg_excl.as<uint32_t>(), wb.chunk_total.as<uint64_t>(), out_b, cap_b,
+                      wb.win_info.as<uint64_t>()};
+    hipLaunchKernelGGL(k_wheel_compact_pair, dim3(blocks), dim3(kWheelThreads), 0, g.stream, a, oa, b, ob);
     return PCUBE_OK;
 }
 
@@ -1309,8 +1349,8 @@ int pcube_prepare(int focus_u, int focus_v, uint32_t ntiles, const uint64_t* til
     uint32_t cap_u = wheel_slot_cap(fu->wheel), cap_v = wheel_slot_cap(fv->wheel);
     const uint64_t* r = g_down_host.as<uint64_t>();
     for (;;) {
-        TRY(wheel_enqueue(fu->wheel, in_u, (uint64_t*)us_out, us_cap, cap_u));
-        TRY(wheel_enqueue(fv->wheel, in_v, (uint64_t*)vs_out, vs_cap, cap_v));
+        TRY(wheel_enqueue_pair(fu->wheel, in_u, (uint64_t*)us_out, us_cap, cap_u, fv->wheel, in_v, (uint64_t*)vs_out,
+                               vs_cap, cap_v));
         hipLaunchKernelGGL(k_prepare_bounds, dim3(blocks_for(nq + 1, 256)), dim3(256), 0, g.stream,
                            (const uint64_t*)us_out, (const uint64_t*)fu->wheel.win_info.as<uint64_t>(), us_cap,
                            (const uint64_t*)vs_out, (const uint64_t*)fv->wheel.win_info.as<uint64_t>(), vs_cap,

@@ -96,11 +96,10 @@ __device__ __forceinline__ void wheel_segment(const WheelWindow& w, uint32_t n1,
 // kOuter >= 0: exactly kOuter outer rings with residues packed as bytes, checks
 // unrolled; kOuter < 0: any outer rings, residues reduced in the loop.
 template <int kOuter>
-__global__ __launch_bounds__(kWheelThreads) void k_wheel_slots(WheelArgs a) {
+__device__ __forceinline__ void wheel_slots_block(const WheelArgs& a, uint64_t seg) {
     __shared__ uint32_t allow[kMaxOuterBits / 32];
     __shared__ uint32_t s_window;
     for (uint32_t t = threadIdx.x; t < a.allow_words; t += kWheelThreads) allow[t] = a.allow[t];
-    const uint64_t seg = blockIdx.x;
     if (threadIdx.x == 0) s_window = wheel_window_of(a, seg);
     __syncthreads();
     const WheelWindow w = a.windows[s_window];
@@ -156,6 +155,20 @@ __global__ __launch_bounds__(kWheelThreads) void k_wheel_slots(WheelArgs a) {
     }
 }
 
+template <int kOuter>
+__global__ __launch_bounds__(kWheelThreads) void k_wheel_slots(WheelArgs a) {
+    wheel_slots_block<kOuter>(a, blockIdx.x);
+}
+
+// Two wheels' test passes in one launch (the u and v windows of a batch):
+// blocks [0, a.nseg) take a's segments, the rest b's, so the two latency
+// chains overlap.  Both wheels use the same instantiation.
+template <int kOuter>
+__global__ __launch_bounds__(kWheelThreads) void k_wheel_slots_pair(WheelArgs a, WheelArgs b) {
+    const bool second = blockIdx.x >= a.nseg;
+    wheel_slots_block<kOuter>(second ? b : a, second ? blockIdx.x - a.nseg : blockIdx.x);
+}
+
 // Inclusive scan over the 64 lanes of a wave.
 __device__ __forceinline__ uint32_t wheel_wave_scan(uint32_t x, uint32_t lane) {
 #pragma unroll
@@ -169,12 +182,12 @@ __device__ __forceinline__ uint32_t wheel_wave_scan(uint32_t x, uint32_t lane) {
 // Segment scan: block c covers segments [1024c, 1024c + 1024), four per
 // thread.  seg_excl[s] = survivors of the chunk's segments before s;
 // chunk_total[c] = survivors of the chunk.
-__global__ __launch_bounds__(256) void k_wheel_seg_scan(const uint32_t* __restrict__ wave_count, uint64_t nseg,
-                                                        uint32_t* __restrict__ seg_excl,
-                                                        uint64_t* __restrict__ chunk_total) {
+__device__ __forceinline__ void wheel_seg_scan_block(const uint32_t* __restrict__ wave_count, uint64_t nseg,
+                                                     uint32_t* __restrict__ seg_excl,
+                                                     uint64_t* __restrict__ chunk_total, uint32_t chunk) {
     __shared__ uint32_t wave_sum[256 / kWave];
     const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    const uint64_t s0 = (uint64_t)blockIdx.x * kWheelScanChunk + 4 * threadIdx.x;
+    const uint64_t s0 = (uint64_t)chunk * kWheelScanChunk + 4 * threadIdx.x;
     uint32_t c[4];
     uint32_t mine = 0;
 #pragma unroll
@@ -197,7 +210,31 @@ __global__ __launch_bounds__(256) void k_wheel_seg_scan(const uint32_t* __restri
         if (s0 + k < nseg) seg_excl[s0 + k] = run;
         run += c[k];
     }
-    if (threadIdx.x == 255) chunk_total[blockIdx.x] = run;
+    if (threadIdx.x == 255) chunk_total[chunk] = run;
+}
+
+__global__ __launch_bounds__(256) void k_wheel_seg_scan(const uint32_t* __restrict__ wave_count, uint64_t nseg,
+                                                        uint32_t* __restrict__ seg_excl,
+                                                        uint64_t* __restrict__ chunk_total) {
+    wheel_seg_scan_block(wave_count, nseg, seg_excl, chunk_total, blockIdx.x);
+}
+
+// One wheel's scan state: its per-wave counts and segment count, the segment
+// offsets and chunk totals it receives, and its number of scan chunks.
+struct WheelScan {
+    const uint32_t* wave_count;
+    uint64_t nseg;
+    uint32_t* seg_excl;
+    uint64_t* chunk_total;
+    uint32_t nchunks;
+};
+
+// Two wheels' segment scans in one launch: blocks [0, a.nchunks) scan a.
+__global__ __launch_bounds__(256) void k_wheel_seg_scan_pair(WheelScan a, WheelScan b) {
+    const bool second = blockIdx.x >= a.nchunks;
+    const WheelScan& w = second ? b : a;
+    wheel_seg_scan_block(w.wave_count, w.nseg, w.seg_excl, w.chunk_total,
+                         second ? blockIdx.x - a.nchunks : blockIdx.x);
 }
 
 // Compaction: one block per segment.  Output offset of the segment = the
@@ -206,13 +243,12 @@ __global__ __launch_bounds__(256) void k_wheel_seg_scan(const uint32_t* __restri
 // first segment of each window writes that window's offset; the last
 // segment writes the total into every later entry of win_info, and
 // win_info[nwin + 1] receives the overflow flag.
-__global__ __launch_bounds__(kWheelThreads) void k_wheel_compact(WheelArgs a, const uint32_t* __restrict__ seg_excl,
-                                                                 const uint64_t* __restrict__ chunk_total,
-                                                                 uint64_t* __restrict__ out, uint64_t cap,
-                                                                 uint64_t* __restrict__ win_info) {
+__device__ __forceinline__ void wheel_compact_block(const WheelArgs& a, const uint32_t* __restrict__ seg_excl,
+                                                    const uint64_t* __restrict__ chunk_total,
+                                                    uint64_t* __restrict__ out, uint64_t cap,
+                                                    uint64_t* __restrict__ win_info, uint64_t seg) {
     __shared__ uint64_t partial[kWheelThreads / kWave];
     __shared__ uint32_t s_window;
-    const uint64_t seg = blockIdx.x;
     const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     const uint64_t chunk = seg / kWheelScanChunk;
     uint64_t acc = 0;
@@ -248,7 +284,34 @@ __global__ __launch_bounds__(kWheelThreads) void k_wheel_compact(WheelArgs a, co
     }
 }
 
-#define PCUBE_WHEEL(K) template __global__ void k_wheel_slots<K>(WheelArgs);
+__global__ __launch_bounds__(kWheelThreads) void k_wheel_compact(WheelArgs a, const uint32_t* __restrict__ seg_excl,
+                                                                 const uint64_t* __restrict__ chunk_total,
+                                                                 uint64_t* __restrict__ out, uint64_t cap,
+                                                                 uint64_t* __restrict__ win_info) {
+    wheel_compact_block(a, seg_excl, chunk_total, out, cap, win_info, blockIdx.x);
+}
+
+// One wheel's compaction outputs.
+struct WheelOut {
+    const uint32_t* seg_excl;
+    const uint64_t* chunk_total;
+    uint64_t* out;
+    uint64_t cap;
+    uint64_t* win_info;
+};
+
+// Two wheels' compactions in one launch: blocks [0, a.nseg) compact a.
+__global__ __launch_bounds__(kWheelThreads) void k_wheel_compact_pair(WheelArgs a, WheelOut oa, WheelArgs b,
+                                                                      WheelOut ob) {
+    const bool second = blockIdx.x >= a.nseg;
+    const WheelOut& o = second ? ob : oa;
+    wheel_compact_block(second ? b : a, o.seg_excl, o.chunk_total, o.out, o.cap, o.win_info,
+                        second ? blockIdx.x - a.nseg : blockIdx.x);
+}
+
+#define PCUBE_WHEEL(K)                                              \
+    template __global__ void k_wheel_slots<K>(WheelArgs);           \
+    template __global__ void k_wheel_slots_pair<K>(WheelArgs, WheelArgs);
 PCUBE_WHEEL(-1)
 PCUBE_WHEEL(0)
 PCUBE_WHEEL(1)
@@ -266,6 +329,18 @@ inline const void* wheel_kernel(uint32_t packed, uint32_t nouter) {
         case 2: return (const void*)k_wheel_slots<2>;
         case 3: return (const void*)k_wheel_slots<3>;
         default: return (const void*)k_wheel_slots<4>;
+    }
+}
+
+// The paired test pass for two wheels of the same instantiation.
+inline const void* wheel_pair_kernel(uint32_t packed, uint32_t nouter) {
+    if (!packed) return (const void*)k_wheel_slots_pair<-1>;
+    switch (nouter) {
+        case 0: return (const void*)k_wheel_slots_pair<0>;
+        case 1: return (const void*)k_wheel_slots_pair<1>;
+        case 2: return (const void*)k_wheel_slots_pair<2>;
+        case 3: return (const void*)k_wheel_slots_pair<3>;
+        default: return (const void*)k_wheel_slots_pair<4>;
     }
 }
 
