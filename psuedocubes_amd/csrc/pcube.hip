@@ -928,28 +928,41 @@ __global__ void k_searchsorted_seg(const uint64_t* __restrict__ data, const uint
 
 // Subtile bounds of one prepared batch: lower bounds of the 2*ntiles u queries
 // in us and of the 2*ntiles v queries in vs (sizes from each wheel's window
-// info, clamped to the capacities), then both sizes and overflow flags.
-__global__ void k_prepare_bounds(const uint64_t* __restrict__ us, const uint64_t* __restrict__ uinfo, uint64_t us_cap,
-                                 const uint64_t* __restrict__ vs, const uint64_t* __restrict__ vinfo, uint64_t vs_cap,
-                                 const uint64_t* __restrict__ queries, uint32_t ntiles, uint64_t* __restrict__ out) {
-    const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+// info, clamped to the capacities), then both sizes and overflow flags.  One
+// wave per query: each round its 64 lanes probe evenly spaced entries of the
+// remaining range [l, h) and a ballot cuts the range 64-fold, so a million
+// entries take four dependent loads.  Invariant: the first index with
+// a[i] >= k lies in [l, h] (h = the size when every entry is smaller).
+__global__ __launch_bounds__(256) void k_prepare_bounds(const uint64_t* __restrict__ us,
+                                                        const uint64_t* __restrict__ uinfo, uint64_t us_cap,
+                                                        const uint64_t* __restrict__ vs,
+                                                        const uint64_t* __restrict__ vinfo, uint64_t vs_cap,
+                                                        const uint64_t* __restrict__ queries, uint32_t ntiles,
+                                                        uint64_t* __restrict__ out) {
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint64_t q = blockIdx.x * (uint64_t)(blockDim.x / kWave) + threadIdx.x / kWave;
     const uint64_t nq = 4 * (uint64_t)ntiles;
-    if (t < nq) {
-        const bool on_u = t < 2 * (uint64_t)ntiles;
-        const uint64_t* a = on_u ? us : vs;
-        uint64_t l = 0, h = on_u ? min(uinfo[1], us_cap) : min(vinfo[1], vs_cap);
-        const uint64_t k = queries[t];
-        while (l < h) {
-            const uint64_t m = l + ((h - l) >> 1);
-            if (a[m] < k) l = m + 1; else h = m;
-        }
-        out[t] = l;
-    } else if (t == nq) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         out[nq] = uinfo[1];
         out[nq + 1] = vinfo[1];
         out[nq + 2] = uinfo[2];
         out[nq + 3] = vinfo[2];
     }
+    if (q >= nq) return;
+    const bool on_u = q < 2 * (uint64_t)ntiles;
+    const uint64_t* a = on_u ? us : vs;
+    uint64_t l = 0, h = on_u ? min(uinfo[1], us_cap) : min(vinfo[1], vs_cap);
+    const uint64_t k = queries[q];
+    while (l < h) {
+        const uint64_t step = (h - l + kWave - 1) / kWave;
+        const uint64_t at = l + lane * step;
+        const uint32_t below = (uint32_t)__popcll(__ballot(at < h && a[at] < k));
+        if (below == 0) break;   // a[l] >= k
+        const uint64_t top = l + below * step;
+        l += (below - 1) * step + 1;
+        h = min(h, top);
+    }
+    if (lane == 0) out[q] = l;
 }
 
 // Batched engine: configurations, focus wheels and the grow-only buffers one
@@ -1351,7 +1364,7 @@ int pcube_prepare(int focus_u, int focus_v, uint32_t ntiles, const uint64_t* til
     for (;;) {
         TRY(wheel_enqueue_pair(fu->wheel, in_u, (uint64_t*)us_out, us_cap, cap_u, fv->wheel, in_v, (uint64_t*)vs_out,
                                vs_cap, cap_v));
-        hipLaunchKernelGGL(k_prepare_bounds, dim3(blocks_for(nq + 1, 256)), dim3(256), 0, g.stream,
+        hipLaunchKernelGGL(k_prepare_bounds, dim3(blocks_for(nq, 256 / kWave)), dim3(256), 0, g.stream,
                            (const uint64_t*)us_out, (const uint64_t*)fu->wheel.win_info.as<uint64_t>(), us_cap,
                            (const uint64_t*)vs_out, (const uint64_t*)fv->wheel.win_info.as<uint64_t>(), vs_cap,
                            (const uint64_t*)d, ntiles, fu->queries.as<uint64_t>());
