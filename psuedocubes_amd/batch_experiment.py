@@ -147,13 +147,19 @@ def python_witnesses(values: list[int], p: int) -> list[dict[str, object]]:
 
 def scope_skeleton(ranges: list[tuple[int, int, int, int]]) -> list[dict[str, Any]]:
     """The per-subtile receipt records of a batch before its counts are known
-    (keys of `pseudocube/batch_experiment.py:149-158`), so the host can build
-    them while the device matches."""
+    (keys and key order of `pseudocube/batch_experiment.py:149-158`), so the
+    host can build them while the device matches.  Each starts as the record
+    of a subtile that emitted nothing; batch_row fills in the counts."""
     firsts = [str(first) for first, _, _, _ in ranges]
-    # consecutive subtiles share a bound: a subtile's end is the next one's first
-    return [{'first': firsts[k],
-             'end': firsts[k + 1] if k + 1 < len(ranges) and ranges[k + 1][0] == end else str(end)}
-            for k, (_, end, _, _) in enumerate(ranges)]
+    n = len(ranges)
+    out = []
+    for k, (_, end, _, _) in enumerate(ranges):
+        # consecutive subtiles share a bound: a subtile's end is the next one's first
+        out.append({'first': firsts[k],
+                    'end': firsts[k + 1] if k + 1 < n and ranges[k + 1][0] == end else str(end),
+                    'valid_focused_candidates': 0, 'group_survivors': None, 'local_survivors': 0,
+                    'output_sha256': EMPTY_DIGEST})
+    return out
 
 
 def batch_row(config: Config, filter_groups: list[list[int]], ranges: list[tuple[int, int, int, int]],
@@ -189,19 +195,20 @@ def batch_row(config: Config, filter_groups: list[list[int]], ranges: list[tuple
     if any(not row['valid_pseudocube'] for row in hits):
         raise ArithmeticError('independent witness verification failed')
     scopes = skeleton if skeleton is not None else scope_skeleton(ranges)
-    position = 0
-    for (first, end, _, _), counts, number, scope in zip(ranges, table, emitted, scopes):
-        sha = EMPTY_DIGEST
-        if number:
-            values = sorted(all_values[position:position + number])
-            if any(not first <= config.canonical(n)[1] < end for n in values):
-                raise ArithmeticError('output assigned to wrong subtile')
-            sha = digest([str(n) for n in values])
-            position += number
+    end_group = ng + 1
+    for counts, scope in zip(table, scopes):
         scope['valid_focused_candidates'] = counts[0]
-        scope['group_survivors'] = counts[1:ng + 1]
-        scope['local_survivors'] = number
-        scope['output_sha256'] = sha
+        scope['group_survivors'] = counts[1:end_group]
+    if all_values:
+        position = 0
+        for (first, end, _, _), number, scope in zip(ranges, emitted, scopes):
+            if number:
+                values = sorted(all_values[position:position + number])
+                if any(not first <= config.canonical(n)[1] < end for n in values):
+                    raise ArithmeticError('output assigned to wrong subtile')
+                scope['local_survivors'] = number
+                scope['output_sha256'] = digest([str(n) for n in values])
+                position += number
     verification_seconds = time.perf_counter() - verification_started
     groups = counters[:, 1:ng + 1].sum(axis=0, dtype=np.uint64).tolist() if table else [0] * ng
     if totals is not None:
