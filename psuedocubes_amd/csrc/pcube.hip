@@ -232,13 +232,14 @@ int bucket_sort(const uint64_t* in, uint64_t n, uint64_t low, uint64_t span, uin
 struct Pinned {
     uint8_t* p = nullptr;
     uint64_t bytes = 0;
+    unsigned flags = hipHostMallocDefault;
     int ensure(uint64_t need) {
         if (need <= bytes && p) return PCUBE_OK;
         if (p) (void)hipHostFree(p);
         p = nullptr;
         bytes = 0;
         const uint64_t n = std::max<uint64_t>(need, 1ull << 16);
-        HIP_TRY(hipHostMalloc((void**)&p, n, hipHostMallocDefault));
+        HIP_TRY(hipHostMalloc((void**)&p, n, flags));
         bytes = n;
         return PCUBE_OK;
     }
@@ -246,6 +247,10 @@ struct Pinned {
     T* as() const { return (T*)p; }
 };
 Pinned g_up_host, g_down_host;
+// Coherent pinned memory that kernels write results into directly: the
+// stores reach host memory before the kernel's completion is signalled, so a
+// stream synchronisation makes them visible without a copy.
+Pinned g_result_host{nullptr, 0, hipHostMallocCoherent};
 Buffer g_up_dev;   // device image of the current call's uploads
 
 struct Wheel {
@@ -268,11 +273,10 @@ struct Focus {
     Buffer C, D, dir;
     BucketWork work;        // window buckets and scan scratch
     Buffer slots, flag;     // single-pass window slots and their overflow flag
-    Buffer queries;         // subtile-bound queries and results of pcube_prepare
     Wheel wheel;
     ~Focus() {
         C.release(); D.release(); dir.release();
-        work.hist.release(); work.offsets.release(); slots.release(); flag.release(); queries.release();
+        work.hist.release(); work.offsets.release(); slots.release(); flag.release();
         work.scan.partial.release(); work.scan.partial_off.release(); work.scan.partial_total.release();
         wheel.release();
     }
@@ -1344,9 +1348,8 @@ int pcube_prepare(int focus_u, int focus_v, uint32_t ntiles, const uint64_t* til
     const uint64_t nq = 4 * (uint64_t)ntiles, iw = wheel_image_words(1);
     const uint64_t up_words = nq + 2 * iw, down_words = nq + 4;
     TRY(g_up_host.ensure(up_words * 8));
-    TRY(g_down_host.ensure(down_words * 8));
+    TRY(g_result_host.ensure(down_words * 8));
     TRY(g_up_dev.ensure(up_words * 8));
-    TRY(fu->queries.ensure(down_words * 8));
     uint64_t* h = g_up_host.as<uint64_t>();
     for (uint32_t t = 0; t < ntiles; ++t) {
         h[2 * t] = tiles[4 * t + 2];
@@ -1360,15 +1363,15 @@ int pcube_prepare(int focus_u, int focus_v, uint32_t ntiles, const uint64_t* til
     uint64_t* d = g_up_dev.as<uint64_t>();
     const WheelInputs in_u = wheel_inputs(d + nq, 1, nseg_u), in_v = wheel_inputs(d + nq + iw, 1, nseg_v);
     uint32_t cap_u = wheel_slot_cap(fu->wheel), cap_v = wheel_slot_cap(fv->wheel);
-    const uint64_t* r = g_down_host.as<uint64_t>();
+    // the bounds kernel writes its results straight into coherent pinned memory
+    const uint64_t* r = g_result_host.as<uint64_t>();
     for (;;) {
         TRY(wheel_enqueue_pair(fu->wheel, in_u, (uint64_t*)us_out, us_cap, cap_u, fv->wheel, in_v, (uint64_t*)vs_out,
                                vs_cap, cap_v));
         hipLaunchKernelGGL(k_prepare_bounds, dim3(blocks_for(nq, 256 / kWave)), dim3(256), 0, g.stream,
                            (const uint64_t*)us_out, (const uint64_t*)fu->wheel.win_info.as<uint64_t>(), us_cap,
                            (const uint64_t*)vs_out, (const uint64_t*)fv->wheel.win_info.as<uint64_t>(), vs_cap,
-                           (const uint64_t*)d, ntiles, fu->queries.as<uint64_t>());
-        HIP_TRY(hipMemcpyAsync(g_down_host.p, fu->queries.p, down_words * 8, hipMemcpyDeviceToHost, g.stream));
+                           (const uint64_t*)d, ntiles, g_result_host.as<uint64_t>());
         HIP_TRY(hipStreamSynchronize(g.stream));
         const bool again_u = r[nq + 2] && cap_u < (uint32_t)kWheelWaveSpan;
         const bool again_v = r[nq + 3] && cap_v < (uint32_t)kWheelWaveSpan;
