@@ -680,6 +680,7 @@ struct Matcher {
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     Pending pending;       // the batch started by pcube_match_start
     Pinned down;           // its download: [status words, counters]
+    uint64_t head_zero = 0;   // leading bytes of status known to be zero (cleared after each download)
     uint64_t* list_count() const { return status.as<uint64_t>(); }
     uint32_t* error() const { return (uint32_t*)(status.as<uint64_t>() + 1); }
     ~Matcher() {
@@ -858,8 +859,11 @@ int match_launch(Matcher& mt) {
     const uint64_t nc = (uint64_t)pd.ntiles * (mt.ng + 3);
     hipEvent_t e0 = mt.ev[0], e1 = mt.ev[1], e2 = mt.ev[2], k0 = mt.ev[3], k1 = mt.ev[4];
     uint64_t* head = mt.status.as<uint64_t>();
-    hipLaunchKernelGGL(k_fill2_u64, dim3(std::min<uint64_t>(blocks_for(nc + 4, 256), 4096)), dim3(256), 0, g.stream,
-                       head, nc + 4, head, (uint64_t)0);
+    // the last batch's finish cleared the buffer already, unless it has grown or been used since
+    if (mt.head_zero < (nc + 4) * 8)
+        hipLaunchKernelGGL(k_fill2_u64, dim3(std::min<uint64_t>(blocks_for(nc + 4, 256), 4096)), dim3(256), 0,
+                           g.stream, head, nc + 4, head, (uint64_t)0);
+    mt.head_zero = 0;
     HIP_TRY(hipEventRecord(e0, g.stream));
     TRY(match_enqueue(mt, (const uint64_t*)pd.us, pd.nu, (const uint64_t*)pd.vs, pd.nv, pd.v_first, pd.tile_width,
                       pd.ntiles, pd.cap, pd.engine, head + 4, 0, pd.list_cap, e1, e2, k0, k1));
@@ -1573,6 +1577,7 @@ int pcube_match_start(int matcher, uint64_t us, uint64_t nu, uint64_t vs, uint64
     pd.ntiles = ntiles; pd.cap = cap; pd.engine = engine;
     const uint64_t nc = (uint64_t)ntiles * (mt->ng + 3);
     pd.list_cap = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)ntiles * cap, 1ull << 22));
+    if (mt->status.bytes < (nc + 4) * 8) mt->head_zero = 0;   // a new buffer holds no zeros yet
     TRY(mt->status.ensure((nc + 4) * 8));   // status words, then the batch's counters
     TRY(mt->list.ensure(pd.list_cap * 24));
     TRY(mt->down.ensure((nc + 4) * 8));
@@ -1614,10 +1619,14 @@ int pcube_match_finish(int matcher, uint64_t* counters, uint64_t* values, uint64
     }
     memcpy(counters, host.data(), nc * 8);
     int rc = batch_status(*mt, host, pd.ntiles, pd.cap, err);
-    if (rc) return rc;
     int status = PCUBE_OK;
-    TRY(collect_values(*mt, pd.ntiles, host, pd.cap, pd.list_cap, values, values_cap, &status, listed));
-    return status;
+    if (!rc) rc = collect_values(*mt, pd.ntiles, host, pd.cap, pd.list_cap, values, values_cap, &status, listed);
+    // clear status and counters for the next batch now, off its critical path
+    const uint64_t words = mt->status.bytes / 8;
+    hipLaunchKernelGGL(k_fill2_u64, dim3(std::min<uint64_t>(blocks_for(words, 256), 4096)), dim3(256), 0, g.stream,
+                       mt->status.as<uint64_t>(), words, mt->status.as<uint64_t>(), (uint64_t)0);
+    if (hipGetLastError() == hipSuccess) mt->head_zero = words * 8;
+    return rc ? rc : status;
 }
 
 int pcube_match(int matcher, uint64_t us, uint64_t nu, uint64_t vs, uint64_t nv, uint64_t v_first,
@@ -1698,6 +1707,7 @@ int pcube_lease(int matcher, int focus_u, int focus_v, uint32_t nbatches, const 
     int engine = 0;
     for (int attempt = 0; attempt < 2; ++attempt) {
         TRY(fill64(mt->counters.as<uint64_t>(), ntot * stride, 0));
+        mt->head_zero = 0;
         TRY(fill64(mt->status.as<uint64_t>(), 4, 0));
         uint64_t t0 = 0;
         for (uint32_t b = 0; b < nbatches; ++b) {

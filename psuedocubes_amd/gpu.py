@@ -117,7 +117,16 @@ class Runtime:
     def check(self) -> None:
         if time.monotonic() >= self.deadline - 3:
             raise TimeoutError('BUDGET_EXHAUSTED')
-        self.peak_device = max(self.peak_device, self.memory()[1])
+        # called several times per batch: the pool's peak through one prebuilt ctypes call
+        try:
+            memory_call = self._memory_call
+        except AttributeError:
+            peak = ctypes.c_uint64()
+            memory_call = self._memory_call = (native.lib().pcube_memory, (None, ctypes.byref(peak), None), peak)
+        fn, args, peak = memory_call
+        native.check(fn(*args))
+        if peak.value > self.peak_device:
+            self.peak_device = peak.value
         if self.peak_device > self.cap:
             raise MemoryError('device memory cap exceeded')
 
@@ -402,8 +411,10 @@ class MatchEngine:
             return counters, [[] for _ in range(ntiles)], [seconds[0], seconds[1], seconds[2]]
         native.check(code, 'pcube_match')
         self.runtime.check()
-        per_tile: list[list[int]] = []
         emitted = counters[:, self.ng + 1].tolist()
+        if not any(emitted):   # the usual full p = 619 batch
+            return counters, [[] for _ in emitted], [seconds[0], seconds[1], seconds[2]]
+        per_tile: list[list[int]] = []
         total = sum(min(n, cap) for n in emitted)
         limbs = values[:2 * total].tolist()
         offset = 0
