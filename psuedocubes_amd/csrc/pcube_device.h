@@ -351,11 +351,26 @@ __device__ __forceinline__ uint32_t first_above(const MatchArgs& m, const U128& 
         l = (a - 1) * stride + 1;
         h = a * stride < m.nu ? a * stride : m.nu;
     }
-    while (l < h) {
-        const uint32_t mid = l + ((h - l) >> 1);
-        if (less_wide(bound, mul_wide(m.us[mid], m.B))) h = mid; else l = mid + 1;
+    // the answer lies in [l, h]; P(i) = (us[i]*B <= bound) holds exactly below it.
+    // Eight independent probes per round instead of one: a 63-coordinate window
+    // takes two rounds of loads, not six dependent ones.
+    while (h - l > 8) {
+        const uint32_t step = (h - l + 7) / 8;
+        uint32_t below = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < 8; ++k) {
+            const uint32_t at = l + k * step;
+            below += at < h && !less_wide(bound, mul_wide(m.us[at], m.B)) ? 1u : 0u;
+        }
+        if (below == 0) return l;
+        const uint32_t top = l + below * step;
+        l += (below - 1) * step + 1;
+        h = top < h ? top : h;
     }
-    return l;
+    uint32_t below = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k) below += l + k < h && !less_wide(bound, mul_wide(m.us[l + k], m.B)) ? 1u : 0u;
+    return l + below;
 }
 
 // Exact u bounds of row j by 128-bit binary search, subtile, v mod q.
