@@ -1099,6 +1099,11 @@ int pcube_init(int device, uint64_t memory_cap, double fraction, uint64_t* cap, 
     if (g.device >= 0 && g.device != device) return fail(PCUBE_INVALID, "already initialised on device %d", g.device);
     if (g.device < 0) {
         HIP_TRY(hipSetDevice(device));
+        // host waits spin instead of sleeping: the library synchronises twice per
+        // batch and each wake-up from a sleep costs tens of microseconds.  The
+        // flag can only be set before the device is in use; when another
+        // component initialised it first, its setting stays.
+        if (hipSetDeviceFlags(hipDeviceScheduleSpin) != hipSuccess) (void)hipGetLastError();
         HIP_TRY(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
         hipDeviceProp_t prop;
         HIP_TRY(hipGetDeviceProperties(&prop, device));
@@ -1107,7 +1112,6 @@ int pcube_init(int device, uint64_t memory_cap, double fraction, uint64_t* cap, 
         int lds = 0;
         HIP_TRY(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
         g.lds_per_block = (uint32_t)std::max(lds, 0);
-        // the staged engine's block holds kStagedLdsBytes of static LDS
         g.device = device;
     }
     size_t fr = 0, tot = 0;
