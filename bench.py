@@ -30,18 +30,25 @@ from pathlib import Path
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-# The dominant kernel is the staged bitset match.  Per 64 focused positions
-# (one mask word) it reads one 8-byte word of each of the eight bitset
-# filters' masks from LDS, so its algorithmic traffic is 8 * 8 / 64 = 1 LDS
-# byte per focused position.
+# The dominant kernel is the staged bitset match, bound by integer VALU (no
+# MFMA, no floating point).  Its algorithmic work is the fixed cost table of
+# the batch engine: 0.56 int32 operations per focused position (bitset ANDs,
+# the 128-bit survivor construction and its expected filters, per-v bounds,
+# windows), so roofline.achieved is 0.56 x positions / kernel time.
+OPS_PER_POSITION = 0.56
+# Per 64 focused positions (one mask word) it reads one 8-byte word of each of
+# the eight bitset filters' masks from LDS: 8 * 8 / 64 = 1 LDS byte per position.
 STAGED_FILTERS = 8
 LDS_BYTES_PER_POSITION = STAGED_FILTERS * 8 / 64
 # MI355X LDS read bandwidth for ds_read_b128: 256 CUs x 256 bytes per clock x
 # 2.4 GHz = 157.3e12 B/s (`bench.py --lds-probe` reads about 240 B/clk per CU
 # with conflict-free addresses).
 LDS_PEAK = 256 * 256 * 2.4e9
-# Integer VALU issue: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz = 39.3e12 lane operations/s.
-VALU_PEAK = 256 * 4 * 16 * 2.4e9
+# Integer VALU issue on MI355X: the spec-derived figure, 256 CUs x 4 SIMDs x 16
+# lanes x 2.4 GHz = 39.3e12 lane operations/s, is exceeded by the build's own
+# microbenchmark (`bench.py --valu-probe`: independent 32-bit add/xor chains at
+# 32 waves per CU run at 52.3e12), so the measured rate is the peak.
+VALU_PEAK = 52.3e12
 HBM_PEAK = 8.0e12
 
 
@@ -63,6 +70,7 @@ def parse() -> argparse.Namespace:
                     help='only run the plain copy kernel used to calibrate the HBM byte counters')
     ap.add_argument('--lds-probe', action='store_true',
                     help='only measure the LDS read bandwidth of the staged access pattern')
+    ap.add_argument('--valu-probe', action='store_true', help='only measure the integer-VALU issue rate')
     return ap.parse_args()
 
 
@@ -80,6 +88,17 @@ def measured(key: str) -> int | None:
 def measured_traffic() -> int | None:
     """HBM bytes per match launch (FETCH_SIZE and WRITE_SIZE, scaled by the copy calibration)."""
     return measured('traffic_bytes_per_launch')
+
+
+def valu_probe() -> None:
+    """Integer-VALU issue rate (pcube_valu_probe), one JSON line."""
+    import ctypes
+    from psuedocubes_amd import native
+    seconds, ops = ctypes.c_double(), ctypes.c_double()
+    native.call('pcube_valu_probe', 20000, ctypes.byref(seconds), ctypes.byref(ops))
+    rate = ops.value / seconds.value
+    print(json.dumps(dict(valu_probe='8 independent add/xor chains per lane, 32 waves per CU', seconds=seconds.value,
+                          lane_ops=ops.value, tops=rate / 1e12, frac_of_peak=rate / VALU_PEAK)), flush=True)
 
 
 def lds_probe() -> None:
@@ -232,6 +251,9 @@ def main() -> None:
     if args.lds_probe:
         lds_probe()
         return
+    if args.valu_probe:
+        valu_probe()
+        return
     if args.copy_probe:
         import ctypes
         from psuedocubes_amd import native
@@ -269,7 +291,8 @@ def main() -> None:
     match_seconds = kernel / args.steps
     # HIP events around each intersection launch on the library's stream
     intersection_seconds = intersection / args.steps
-    achieved = LDS_BYTES_PER_POSITION * per_launch / intersection_seconds
+    achieved = OPS_PER_POSITION * per_launch / intersection_seconds
+    lds_rate = LDS_BYTES_PER_POSITION * per_launch / intersection_seconds
     result = dict(metric='focused candidate positions/s on 1 MI355X, plain vs batched; p50/p99 latency',
                   value=positions / elapsed, unit='focused candidate positions/s', n_gpus=1, steps=args.steps,
                   warmup=args.warmup, ms_per_step=1000 * elapsed / args.steps, higher_is_better=True,
@@ -279,17 +302,19 @@ def main() -> None:
                               p=619, tile_width=100_000_000, subtiles=64, seed=619003),
                   kernel_only=dict(value=positions / kernel, unit='focused candidate positions/s',
                                    match_ms=1000 * match_seconds, prefilter_ms=1000 * prefilter / args.steps),
-                  roofline=dict(bound='lds', achieved=achieved / 1e12, peak=LDS_PEAK / 1e12, unit='TB/s',
-                                frac=achieved / LDS_PEAK, traffic=measured_traffic(),
+                  roofline=dict(bound='valu', achieved=achieved / 1e12, peak=VALU_PEAK / 1e12, unit='Tops/s',
+                                frac=achieved / VALU_PEAK, traffic=measured_traffic(),
                                 kernel='k_match_staged<8>', kernel_ms=1000 * intersection_seconds,
-                                lds_bytes_per_position=LDS_BYTES_PER_POSITION, positions_per_launch=per_launch))
+                                ops_per_position=OPS_PER_POSITION, positions_per_launch=per_launch),
+                  lds=dict(achieved=lds_rate / 1e12, peak=LDS_PEAK / 1e12, unit='TB/s', frac=lds_rate / LDS_PEAK,
+                           bytes_per_position=LDS_BYTES_PER_POSITION))
     # executed integer-VALU lane operations and HBM bytes of the same kernel, per
     # launch from the committed counter summary, over this run's kernel time
     valu = measured('valu_instructions_per_launch')
     if valu:
         ops = valu * 64 / intersection_seconds
-        result['valu'] = dict(achieved=ops / 1e12, peak=VALU_PEAK / 1e12, unit='Tops/s', frac=ops / VALU_PEAK,
-                              instructions_per_launch=valu)
+        result['valu_executed'] = dict(achieved=ops / 1e12, peak=VALU_PEAK / 1e12, unit='Tops/s',
+                                       frac=ops / VALU_PEAK, instructions_per_launch=valu)
     if result['roofline']['traffic']:
         rate = result['roofline']['traffic'] / intersection_seconds
         result['hbm'] = dict(achieved=rate / 1e9, peak=HBM_PEAK / 1e9, unit='GB/s', frac=rate / HBM_PEAK)

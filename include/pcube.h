@@ -244,6 +244,13 @@ int pcube_copy_probe(uint64_t bytes, double *seconds);
  * reference counterpart. */
 int pcube_lds_probe(uint32_t pattern, uint32_t rounds, double *seconds, double *bytes);
 
+/* --- measurement: integer-VALU issue rate --------------------------------- */
+/* 32 waves per CU, each lane running eight independent add/xor chains: 16
+ * 32-bit VALU instructions per round for `rounds` rounds.  *seconds is the
+ * event-timed duration of the second of two launches, *lane_ops the VALU lane
+ * operations one launch executes.  No reference counterpart. */
+int pcube_valu_probe(uint32_t rounds, double *seconds, double *lane_ops);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1065,6 +1065,7 @@ const Kernel kKernels[] = {
     {"arithmetic", (const void*)k_arithmetic},
     {"copy_probe", (const void*)k_copy_probe},
     {"lds_probe", (const void*)k_lds_probe},
+    {"valu_probe", (const void*)k_valu_probe},
     {"batch_rows", (const void*)k_batch_rows},
     {"batch_masks", (const void*)k_batch_masks},
     {"batch_rho", (const void*)k_batch_rho},
@@ -2121,6 +2122,30 @@ int pcube_lds_probe(uint32_t pattern, uint32_t rounds, double* seconds, double* 
     HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
     if (seconds) *seconds = ms / 1000.0;
     if (bytes) *bytes = (double)grid * kStagedThreads * rounds * 16 * 16;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    dev_free(out);
+    return PCUBE_OK;
+}
+
+int pcube_valu_probe(uint32_t rounds, double* seconds, double* lane_ops) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    if (!rounds) return fail(PCUBE_INVALID, "at least one probe round");
+    const uint32_t grid = (uint32_t)g.compute_units * 8;   // 32 waves per CU
+    void* out = nullptr;
+    TRY(dev_alloc((uint64_t)grid * 256 * 4, &out));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    hipLaunchKernelGGL(k_valu_probe, dim3(grid), dim3(256), 0, g.stream, rounds, (uint32_t*)out);
+    HIP_TRY(hipEventRecord(e0, g.stream));
+    hipLaunchKernelGGL(k_valu_probe, dim3(grid), dim3(256), 0, g.stream, rounds, (uint32_t*)out);
+    HIP_TRY(hipEventRecord(e1, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    if (seconds) *seconds = ms / 1000.0;
+    if (lane_ops) *lane_ops = (double)grid * 256 * rounds * kValuProbeOps;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     dev_free(out);

@@ -700,6 +700,30 @@ __global__ __launch_bounds__(kStagedThreads) void k_lds_probe(uint32_t pattern, 
     out[blockIdx.x * kStagedThreads + threadIdx.x] = acc0 ^ acc1;
 }
 
+// Integer-VALU issue probe, the calibration of the VALU peak: eight independent
+// add/xor chains per thread, kValuProbeOps 32-bit VALU instructions per round
+// (the count of this build's loop body), 4 blocks of 256 threads per CU.
+constexpr uint32_t kValuProbeOps = 16;
+__global__ __launch_bounds__(256) void k_valu_probe(uint32_t rounds, uint32_t* out) {
+    uint32_t a[8], b[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        a[k] = threadIdx.x + 977u * k + blockIdx.x;
+        b[k] = 0x9e3779b9u * (k + 1);
+    }
+    for (uint32_t r = 0; r < rounds; ++r) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            a[k] += b[k];
+            b[k] ^= a[k];
+        }
+    }
+    uint32_t x = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) x ^= a[k] ^ b[k];
+    out[blockIdx.x * 256 + threadIdx.x] = x;
+}
+
 #define PCUBE_STAGED(F)                                                                               \
     template __global__ void k_match_staged<F, false>(MatchArgs, const uint2*, uint32_t, const uint32_t*); \
     template __global__ void k_match_staged<F, true>(MatchArgs, const uint2*, uint32_t, const uint32_t*);

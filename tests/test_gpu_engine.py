@@ -8,6 +8,7 @@ arrays equal GPU arrays) and the release's tools/qualify_optimized.py:36-61
 """
 from __future__ import annotations
 
+import ctypes
 import hashlib
 import json
 from pathlib import Path
@@ -50,6 +51,22 @@ def oracle_row_fields(row: dict) -> dict:
 @pytest.fixture(scope='module')
 def masked(runtime, config619):
     return MaskedMatcher(runtime, config619)
+
+
+def test_lds_probe_patterns(runtime) -> None:
+    """The LDS probe behind the roofline's LDS peak: every pattern reads the bytes
+    it reports, and the staged engine's lane-rotated reads beat random per-lane
+    units by more than 2x (no bank conflicts against 16-lane conflicts)."""
+    seconds, nbytes = ctypes.c_double(), ctypes.c_double()
+    rates = []
+    for pattern in range(3):
+        native.call('pcube_lds_probe', pattern, 200, ctypes.byref(seconds), ctypes.byref(nbytes))
+        assert nbytes.value == runtime.compute_units * 1024 * 200 * 16 * 16
+        assert seconds.value > 0
+        rates.append(nbytes.value / seconds.value)
+    assert rates[1] > 2 * rates[2]
+    with pytest.raises(ValueError):
+        native.call('pcube_lds_probe', 3, 1, ctypes.byref(seconds), ctypes.byref(nbytes))
 
 
 def test_wide_limb_kernel(runtime) -> None:
