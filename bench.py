@@ -129,6 +129,15 @@ def cpu_baseline(config, seconds: float) -> dict:
         positions += row['valid_focused_candidates']
         batches += 1
     elapsed = time.perf_counter() - started
+    # the same work on one thread: about a quarter of the sample's time
+    one_positions, one_batches = 0, 0
+    one_started = time.perf_counter()
+    while time.perf_counter() - one_started < seconds / 4 or one_batches == 0:
+        k = rng.randrange(config.B // BATCH_WIDTH)
+        row = O.batch(config, fa, fb, k * BATCH_WIDTH, (k + 1) * BATCH_WIDTH, nthreads=1)
+        one_positions += row['valid_focused_candidates']
+        one_batches += 1
+    one_elapsed = time.perf_counter() - one_started
     try:
         model = next(line.split(':', 1)[1].strip() for line in
                      subprocess.run(['lscpu'], capture_output=True, text=True).stdout.splitlines()
@@ -137,7 +146,9 @@ def cpu_baseline(config, seconds: float) -> dict:
         model = 'unknown'
     return dict(value=positions / elapsed, unit='focused candidate positions/s', cores=threads, kind='port',
                 sample=f'{batches} seeded full p=619 batches (windows + bitset matcher + counters), '
-                       f'{elapsed:.1f} s on {threads} threads of {os.cpu_count()} ({model})')
+                       f'{elapsed:.1f} s on {threads} threads of {os.cpu_count()} ({model})',
+                single_thread=dict(value=one_positions / one_elapsed, cores=1,
+                                   sample=f'{one_batches} more seeded batches, {one_elapsed:.1f} s on one thread'))
 
 
 def strip_timings(row: dict) -> dict:
