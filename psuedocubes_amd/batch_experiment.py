@@ -244,7 +244,7 @@ class FusedMatcher:
         self.kernel = runtime.compile(self.kernel_name)
         self.sm_count = runtime.compute_units
         self.prefilter_seconds = 0.0
-        self.intersection_seconds = 0.0   # the bitset-intersection kernel of the last run
+        self.intersection_seconds = 0.0   # the match kernel alone (intersection and chain) of the last run
 
     def run(self, batch: Prepared, output_cap: int = 8192) -> dict[str, Any]:
         started = time.perf_counter()

@@ -379,7 +379,7 @@ class MatchEngine:
 
     def run(self, us: DeviceArray, vs: DeviceArray, v_first: int, tile_width: int, ntiles: int, cap: int,
             engine: int, overflow_ok: bool = False) -> tuple[np.ndarray, list[list[int]], list[float]]:
-        """Counters [ntiles, ng+3], emitted values per subtile, (prefilter, match, intersection kernel) seconds.
+        """Counters [ntiles, ng+3], emitted values per subtile, (prefilter, match, staged match kernel) seconds.
 
         With overflow_ok an output overflow returns normally so the caller can
         raise with the reference's wording; other failures raise.
