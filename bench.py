@@ -329,6 +329,13 @@ def main() -> None:
     if result['roofline']['traffic']:
         rate = result['roofline']['traffic'] / intersection_seconds
         result['hbm'] = dict(achieved=rate / 1e9, peak=HBM_PEAK / 1e9, unit='GB/s', frac=rate / HBM_PEAK)
+    # the full-domain control: genuine cubes r^3 of the interval with r coprime to 619#
+    from psuedocubes_amd.cube_count import cube_count
+    cube_count(1000000001, 1000100001, 619)   # warm
+    control = cube_count(1000000001, 1442249570, 619)
+    result['cube_count'] = dict(count=int(control['count']), expected=38322775,
+                                kernel_ms=1000 * control['kernel_seconds'],
+                                roots_per_second=(1442249570 - 1000000001 + 1) / control['kernel_seconds'])
     if not args.no_lease:
         first = 575222931200000000
         compute_lease(matcher, af, bf, first, first + 4 * BATCH_WIDTH)  # warm the lease buffers

@@ -230,6 +230,16 @@ int pcube_host_verify(const uint64_t *limbs, uint64_t n, uint32_t p, uint8_t *lo
 /* out[3i..3i+2] = (low, high, borrow) of in[4i]*in[4i+1] - in[4i+2]*in[4i+3]; host arrays. */
 int pcube_arithmetic(const uint64_t *in, uint64_t *out, uint64_t count);
 
+/* --- full-domain control: replaces review/cube_count.c (main 12-38) -------- */
+/* Number of roots r in [first, last] coprime to every prime q <= p, counted
+ * on the device (the reference's control: 38,322,775 for [1000000001,
+ * 1442249570], p = 619, tools/check_cpu.py:24-29).  1 <= first <= last <=
+ * 2^32 - 1 and 2 <= p <= 2000, as the reference requires, else INVALID.
+ * good (host, may be NULL): one bit per odd root from first | 1, set when the
+ * root is coprime, ceil(odds / 32) words; *seconds the kernel's duration. */
+int pcube_cube_count(uint64_t first, uint64_t last, uint32_t p, uint64_t *count, uint32_t *good, uint64_t good_words,
+                     double *seconds);
+
 /* --- measurement: calibration probe for the HBM byte counters ------------ */
 /* Copies `bytes` between two fresh device buffers with a plain streaming
  * kernel; *seconds is its event-timed duration.  No reference counterpart. */

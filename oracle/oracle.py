@@ -13,6 +13,7 @@ What it restates (reference file:line):
   * lease accumulation and transcript: pseudocube/optimized.py:40-78
   * brute force: pseudocube/focus.py:170-179 (enumerate_cpu) and the
     complete-set recipe of tests/test_pseudocube_math.py:74-88
+  * full-domain cube count: review/cube_count.c:12-38
 
 Pinning (each function has at least one independent check in tests/):
   * oracle_window    - brute-force multisets (tests/test_pseudocube_batch_experiment.py:18-34
@@ -23,6 +24,10 @@ Pinning (each function has at least one independent check in tests/):
                        (results/continuation-2026-09-15/positive-lease-replay.json)
   * oracle_partial   - Focus.at on sampled indices; focus_u/focus_v fixture sums
   * oracle_products  - 5 extremes + 2000 seeded cases vs Python ints and oracle/_ref/wide.so
+  * oracle_cube_count - gcd brute force on seeded short ranges; the reference's
+                       cube_count.c and cube_count_independent.c compiled under
+                       oracle/_ref (counts and root lists); the reference's
+                       known answer 38,322,775 (results/current.json:25)
 """
 from __future__ import annotations
 
@@ -58,6 +63,10 @@ def lib() -> ctypes.CDLL:
                                         + [_u32p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64,
                                            ctypes.c_uint32, ctypes.c_uint64, _u64p, _u64p, ctypes.c_int])
         handle.oracle_products.argtypes = [ctypes.c_uint64] * 4 + [ctypes.POINTER(ctypes.c_uint64)] * 2
+        handle.oracle_cube_count.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32,
+                                             ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint64,
+                                             ctypes.POINTER(ctypes.c_uint64)]
+        handle.oracle_cube_count.restype = ctypes.c_uint64
         for name in ('oracle_partial', 'oracle_window', 'oracle_match', 'oracle_products'):
             getattr(handle, name).restype = ctypes.c_int
         _lib = handle
@@ -246,6 +255,18 @@ def lease(config: Any, fa: Focus, fb: Focus, first: int, end: int, batch_width: 
         batches += 1
     return dict(totals, group_survivors=groups, all_values=values, compute_batches=batches,
                 transcript_sha256=transcript.hexdigest(), output_sha256=_digest(values))
+
+
+def cube_count(first: int, last: int, p: int, list_roots: bool = False) -> tuple[int, list[int] | None]:
+    """Roots in [first, last] coprime to every prime <= p (review/cube_count.c:12-38):
+    the count, and with list_roots the roots themselves."""
+    count = ctypes.c_uint64()
+    if not list_roots:
+        return int(lib().oracle_cube_count(first, last, p, None, 0, ctypes.byref(count))), None
+    cap = (last - first) // 2 + 2
+    roots = (ctypes.c_uint64 * cap)()
+    n = int(lib().oracle_cube_count(first, last, p, roots, cap, ctypes.byref(count)))
+    return n, [int(roots[i]) for i in range(min(n, cap))]
 
 
 def enumerate_small(config: Any, first: int, end: int) -> list[int]:

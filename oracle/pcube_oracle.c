@@ -270,3 +270,42 @@ int oracle_products(uint64_t a, uint64_t b, uint64_t c, uint64_t d, uint64_t *lo
     *high = (uint64_t)(diff >> 64);
     return left < right;
 }
+
+
+/* ---- Full-domain control: roots coprime to every prime <= p ------------------
+ * Restates review/cube_count.c:12-38: the number of r in [first, last] with no
+ * prime factor q <= p.  Odd roots only (2 <= p), sieved in byte segments of
+ * 2^20 odd roots.  When roots is given, the first roots_cap coprime roots are
+ * stored and *nroots receives how many there are in all. */
+uint64_t oracle_cube_count(uint64_t first, uint64_t last, uint32_t p, uint64_t *roots, uint64_t roots_cap,
+                           uint64_t *nroots) {
+    uint32_t primes[400], np = 0;
+    for (uint32_t q = 3; q <= p; q += 2) {
+        int prime = 1;
+        for (uint32_t d = 3; d * d <= q; d += 2)
+            if (q % d == 0) { prime = 0; break; }
+        if (prime) primes[np++] = q;
+    }
+    const uint64_t seg = 1u << 20;
+    unsigned char *flag = malloc(seg);
+    uint64_t count = 0;
+    for (uint64_t start = first | 1u; start <= last; start += 2 * seg) {
+        uint64_t n = (last - start) / 2 + 1;
+        if (n > seg) n = seg;
+        memset(flag, 1, n);
+        for (uint32_t k = 0; k < np; ++k) {
+            const uint64_t q = primes[k];
+            uint64_t m = (start + q - 1) / q * q;
+            if (!(m & 1)) m += q;
+            for (uint64_t i = (m - start) / 2; i < n; i += q) flag[i] = 0;
+        }
+        for (uint64_t i = 0; i < n; ++i)
+            if (flag[i]) {
+                if (roots && count < roots_cap) roots[count] = start + 2 * i;
+                ++count;
+            }
+    }
+    free(flag);
+    if (nroots) *nroots = count;
+    return count;
+}

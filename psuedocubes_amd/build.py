@@ -11,7 +11,7 @@ import subprocess
 
 HERE = Path(__file__).resolve().parent
 SOURCES = [HERE / 'csrc' / 'pcube.hip']
-HEADERS = [HERE / 'csrc' / name for name in ('pcube_device.h', 'pcube_kernels.h', 'pcube_wheel.h', 'pcube_staged.h')] + [
+HEADERS = [HERE / 'csrc' / name for name in ('pcube_device.h', 'pcube_kernels.h', 'pcube_wheel.h', 'pcube_staged.h', 'pcube_cubecount.h')] + [
     HERE.parent / 'include' / 'pcube.h']
 LIBRARY = HERE / 'libpcube.so'
 ARCH = os.environ.get('PCUBE_OFFLOAD_ARCH', 'gfx950')

@@ -23,6 +23,7 @@
 #include "pcube_device.h"
 #include "pcube_wheel.h"
 #include "pcube_staged.h"
+#include "pcube_cubecount.h"
 
 using namespace pcube;
 
@@ -1066,6 +1067,7 @@ const Kernel kKernels[] = {
     {"copy_probe", (const void*)k_copy_probe},
     {"lds_probe", (const void*)k_lds_probe},
     {"valu_probe", (const void*)k_valu_probe},
+    {"cube_count", (const void*)k_cube_count},
     {"batch_rows", (const void*)k_batch_rows},
     {"batch_masks", (const void*)k_batch_masks},
     {"batch_rho", (const void*)k_batch_rho},
@@ -2150,6 +2152,61 @@ int pcube_valu_probe(uint32_t rounds, double* seconds, double* lane_ops) {
     (void)hipEventDestroy(e1);
     dev_free(out);
     return PCUBE_OK;
+}
+
+int pcube_cube_count(uint64_t first, uint64_t last, uint32_t p, uint64_t* count, uint32_t* good, uint64_t good_words,
+                     double* seconds) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    if (first < 1 || last < first || last > UINT32_MAX || p < 2 || p > 2000)
+        return fail(PCUBE_INVALID, "cube count needs 1 <= first <= last <= 2^32 - 1 and 2 <= p <= 2000");
+    if (g.device < 0) return fail(PCUBE_INVALID, "pcube_init has not been called");
+    // the odd primes up to p
+    std::vector<uint32_t> primes;
+    for (uint32_t q = 3; q <= p; q += 2) {
+        bool prime = true;
+        for (uint32_t d = 3; d * d <= q && prime; d += 2) prime = q % d != 0;
+        if (prime) primes.push_back(q);
+    }
+    const uint64_t r0 = first | 1u;
+    const uint64_t nodd = r0 > last ? 0 : (last - r0) / 2 + 1;
+    const uint64_t need_words = (nodd + 31) / 32;
+    if (good && good_words < need_words) return fail(PCUBE_INVALID, "root bitset needs %llu words",
+                                                     (unsigned long long)need_words);
+    void *dprimes = nullptr, *dcount = nullptr, *dgood = nullptr;
+    TRY(dev_alloc(std::max<uint64_t>(4, primes.size() * 4), &dprimes));
+    int rc = dev_alloc(8, &dcount);
+    if (!rc && good && need_words) rc = dev_alloc(need_words * 4, &dgood);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!rc) {
+        if (!primes.empty())
+            HIP_TRY(hipMemcpyAsync(dprimes, primes.data(), primes.size() * 4, hipMemcpyHostToDevice, g.stream));
+        rc = fill64((uint64_t*)dcount, 1, 0);
+    }
+    if (!rc && nodd) {
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        HIP_TRY(hipEventRecord(e0, g.stream));
+        hipLaunchKernelGGL(k_cube_count, dim3((uint32_t)((nodd + kCubeSegment - 1) / kCubeSegment)),
+                           dim3(kCubeThreads), 0, g.stream, r0, nodd, (const uint32_t*)dprimes,
+                           (uint32_t)primes.size(), (unsigned long long*)dcount, (uint32_t*)dgood);
+        HIP_TRY(hipEventRecord(e1, g.stream));
+    }
+    uint64_t total = 0;
+    if (!rc) {
+        HIP_TRY(hipMemcpyAsync(&total, dcount, 8, hipMemcpyDeviceToHost, g.stream));
+        if (dgood) HIP_TRY(hipMemcpyAsync(good, dgood, need_words * 4, hipMemcpyDeviceToHost, g.stream));
+        HIP_TRY(hipStreamSynchronize(g.stream));
+        float ms = 0;
+        if (e0) HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        if (seconds) *seconds = ms / 1000.0;
+        if (count) *count = total;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    dev_free(dprimes);
+    if (dcount) dev_free(dcount);
+    if (dgood) dev_free(dgood);
+    return rc;
 }
 
 int pcube_copy_probe(uint64_t bytes, double* seconds) {
