@@ -137,14 +137,14 @@ __device__ __forceinline__ void stage_copy(uint4* __restrict__ dst, const uint4*
     }
 }
 
-// LDS of a staged block, carved from dynamic shared memory.
+// The LDS a staged block's helpers use (static arrays of k_match_staged).
+// The filter tables are set only for the residual chain after the items.
 struct Staged {
     const uint32_t* allow;
     const Modulus* mods;
     uint32_t* tilecnt;       // [kTileSlots][ng]: groups 1..ng of subtiles t0..t0+63
     uint32_t* queue;         // this wave's survivors, (row offset << 8) | u offset within the item
 };
-
 
 // Add n to group g (1-based) of a subtile: LDS when the subtile is within the
 // block's window, else the global counters.
