@@ -16,13 +16,27 @@ from typing import Any
 import uuid
 
 
+_EXACT = 1 << 53
+_PLAIN_INTS = {int}
+
+
 def interoperable(value: Any) -> Any:
-    """Replace integers that JSON readers could round with strings (`pseudocube/accounting.py:17-24`)."""
-    if type(value) is int and abs(value) >= 1 << 53:
-        return str(value)
+    """Replace integers that JSON readers could round with strings (`pseudocube/accounting.py:17-24`).
+
+    A list whose elements are all plain ints inside (-2^53, 2^53) is copied
+    as it is, checked with C-level min/max instead of one call per element:
+    receipts carry many such count lists.
+    """
+    kind = type(value)
+    if kind is int:
+        return value if -_EXACT < value < _EXACT else str(value)
+    if kind is str:
+        return value
     if isinstance(value, dict):
         return {str(k): interoperable(v) for k, v in value.items()}
     if isinstance(value, (list, tuple)):
+        if value and set(map(type, value)) == _PLAIN_INTS and -_EXACT < min(value) and max(value) < _EXACT:
+            return list(value)
         return [interoperable(v) for v in value]
     return value
 

@@ -182,16 +182,17 @@ def lease_rows(matcher: MaskedMatcher, af: GPUFocus, bf: GPUFocus, first: int, e
     rows = []
     t0 = 0
     offset = 0
+    emitted = [min(k, cap) for k in counters[:, ng + 1].tolist()]
+    limbs = values[:2 * sum(emitted)].tolist()
+    bound_rows = [tuple(r) for r in bounds.tolist()]
     for b, (s, e) in enumerate(batches):
         nt = len(ranges[b])
         block = counters[t0:t0 + nt]
         per_tile = []
-        for t in range(nt):
-            k = min(int(block[t, ng + 1]), cap)
-            limbs = values[2 * offset:2 * (offset + k)]
-            per_tile.append([int(limbs[2 * i]) + (int(limbs[2 * i + 1]) << 64) for i in range(k)])
+        for k in emitted[t0:t0 + nt]:
+            per_tile.append([limbs[2 * i] + (limbs[2 * i + 1] << 64) for i in range(offset, offset + k)])
             offset += k
-        bnd = [tuple(int(x) for x in bounds[t0 + t]) for t in range(nt)]
+        bnd = bound_rows[t0:t0 + nt]
         row_started = time.perf_counter()
         row = batch_row(config, matcher.filter_groups, ranges[b], bnd, int(sizes[2 * b + 1]), block, per_tile,
                         cap, timings, row_started)
