@@ -60,6 +60,10 @@ int pcube_kernel_info(int index, char *name, size_t name_len, int *registers, in
                       int *shared_bytes);
 
 /* --- device buffers (the capped pool behind every array) ------------------ */
+/* Every library kernel and copy runs on the library's one stream, so
+ * pcube_free returns a block to the pool at once, without waiting: work that
+ * later reuses the block is enqueued behind every earlier use of it.  Blocks
+ * go back to the driver only after that stream has drained. */
 int pcube_alloc(uint64_t bytes, uint64_t *address);
 int pcube_free(uint64_t address);
 int pcube_upload(uint64_t dst, const void *src, uint64_t bytes);

@@ -176,8 +176,10 @@ def batch_row(config: Config, filter_groups: list[list[int]], ranges: list[tuple
     ng = len(filter_groups)
     transfer_started = time.perf_counter()
     table = counters.tolist()
-    emitted = [row[ng + 1] for row in table]
-    if counters[:, ng + 2].any():
+    # a full p = 619 batch usually emits nothing: skip the per-subtile scan then
+    flagged = bool(counters[:, ng + 1:ng + 3].any())
+    emitted = [row[ng + 1] for row in table] if flagged else [0] * len(table)
+    if flagged and counters[:, ng + 2].any():
         raise ArithmeticError('unsigned underflow')
     if emitted and max(emitted) > output_cap:
         raise OverflowError('fused tile output overflow; no truncated coverage')
@@ -210,7 +212,8 @@ def batch_row(config: Config, filter_groups: list[list[int]], ranges: list[tuple
                 scope['output_sha256'] = digest([str(n) for n in values])
                 position += number
     verification_seconds = time.perf_counter() - verification_started
-    groups = counters[:, 1:ng + 1].sum(axis=0, dtype=np.uint64).tolist() if table else [0] * ng
+    sums = counters[:, :ng + 1].sum(axis=0, dtype=np.uint64).tolist() if table else [0] * (ng + 1)
+    groups = sums[1:]
     if totals is not None:
         focus_u, opportunities = totals
     else:
@@ -220,7 +223,7 @@ def batch_row(config: Config, filter_groups: list[list[int]], ranges: list[tuple
             opportunities += (b - a) * (d - c)
     return dict(status='PASS', host_verified=True, focus_u=focus_u, focus_v=int(nv),
                 cartesian_opportunities=opportunities,
-                valid_focused_candidates=int(counters[:, 0].sum(dtype=np.uint64)) if table else 0,
+                valid_focused_candidates=sums[0],
                 genuine_cubes=len(all_values) - len(hits), local_survivors=len(all_values), cube_checks=len(all_values),
                 kernel_seconds=timings.get('kernel_seconds', 0.0), transfer_seconds=transfer_seconds,
                 verification_seconds=verification_seconds,

@@ -78,7 +78,11 @@ uint64_t size_class(uint64_t bytes) {
     return (bytes + mb - 1) / mb * mb;
 }
 
+// Cached blocks may still be read or written by work queued on the stream
+// (frees do not wait), so drain it before handing them back to the driver.
 void release_cache() {
+    if (g.cached.empty()) return;
+    if (g.stream) (void)hipStreamSynchronize(g.stream);
     for (auto& kv : g.cached) {
         (void)hipFree(kv.second);
         g.reserved -= kv.first;
@@ -1171,7 +1175,7 @@ int pcube_alloc(uint64_t bytes, uint64_t* address) {
 
 int pcube_free(uint64_t address) {
     std::lock_guard<std::recursive_mutex> lock(g_mu);
-    if (g.stream) HIP_TRY(hipStreamSynchronize(g.stream));
+    // stream-ordered reuse: the next user of the block queues behind this one
     return dev_free((void*)address);
 }
 
