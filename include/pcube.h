@@ -170,11 +170,36 @@ int pcube_match(int matcher, uint64_t us, uint64_t nu, uint64_t vs, uint64_t nv,
  * the batch (same arguments and meaning as pcube_match) and returns at once;
  * pcube_match_finish waits for it and fills counters, values and seconds as
  * pcube_match does, with the same return codes.  One batch per matcher may be
- * pending; other calls may run in between.  The split of the reference's
- * MaskedMatcher.run (pseudocube/batch_experiment.py:104-167). */
+ * pending; other calls may run in between, except a pcube_lease on the same
+ * matcher, which reuses its batch buffers: finish then returns INVALID.  The
+ * split of the reference's MaskedMatcher.run
+ * (pseudocube/batch_experiment.py:104-167). */
 int pcube_match_start(int matcher, uint64_t us, uint64_t nu, uint64_t vs, uint64_t nv, uint64_t v_first,
                       uint64_t tile_width, uint32_t ntiles, uint64_t cap, int engine);
 int pcube_match_finish(int matcher, uint64_t *counters, uint64_t *values, uint64_t values_cap, double *seconds);
+
+/* prepare() and the start of the batch's match in one call, so the match is
+ * enqueued the moment the subtile bounds are known, with no return to the
+ * caller in between (the reference's prepare then MaskedMatcher.run,
+ * pseudocube/batch_experiment.py:51-57, 104-167).  Arguments up to totals and
+ * the return codes are those of pcube_prepare_range (totals is required here).
+ * Then, when the matcher has no batch pending and the largest subtile's
+ * opportunities are at most max_opportunities, it runs
+ * pcube_match_start(matcher, us_out, *nu, vs_out, *nv, first, width, *ntiles,
+ * cap, engine); a start that fails is not reported (pcube_match_start reports
+ * it when the caller starts the batch itself).  launch receives a nonzero
+ * token when the match was started, else 0. */
+int pcube_prepare_match(int focus_u, int focus_v, uint64_t A, uint64_t B, uint64_t lo0, uint64_t lo1, uint64_t hi0,
+                        uint64_t hi1, uint64_t first, uint64_t end, uint64_t width, uint64_t *tiles, uint32_t *ntiles,
+                        uint64_t us_out, uint64_t us_cap, uint64_t vs_out, uint64_t vs_cap, uint64_t *nu, uint64_t *nv,
+                        uint64_t *bounds, uint64_t *totals, int matcher, uint64_t cap, int engine,
+                        uint64_t max_opportunities, uint64_t *launch);
+
+/* adoptable = 1 when the batch that pcube_prepare_match started with token
+ * launch is still the matcher's pending batch (nothing was started on the
+ * matcher and no lease used it since), so pcube_match_finish collects it;
+ * else 0 and the caller starts the batch itself.  Host only. */
+int pcube_match_adoptable(int matcher, uint64_t launch, int *adoptable);
 
 /* --- lease pipeline: replaces pseudocube/optimized.py:81-90 ---------------- */
 

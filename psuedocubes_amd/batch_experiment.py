@@ -24,6 +24,10 @@ from .predicate import is_perfect_cube, local3, verify
 
 # output_sha256 of a subtile that emitted nothing (almost every subtile at p=619)
 EMPTY_DIGEST = digest([])
+# values one subtile may emit per batch (run's default output capacity)
+OUTPUT_CAP = 8192
+# candidate opportunities one subtile may hold (`pseudocube/batch_experiment.py:45`)
+OPPORTUNITY_LIMIT = 50_000_000
 
 
 @dataclass
@@ -37,6 +41,7 @@ class Prepared:
     prep_seconds: float
     ranges: list[tuple[int, int, int, int]] | None = None   # tile_ranges(first, end, width), when known
     totals: tuple[int, int] | None = None   # (focus_u, cartesian_opportunities) of the bounds, when known
+    started: tuple[Any, int, int] | None = None   # (engine, launch token, cap) of a match prepare() started
 
 
 def tile_ranges(config: Config, first: int, end: int, width: int) -> list[tuple[int, int, int, int]]:
@@ -84,7 +89,7 @@ def prepare_arrays(runtime: Runtime, config: Config, first: int, end: int, width
     vb = searchsorted(vs, [bound for lo, hi, _, _ in ranges for bound in (lo, hi)])
     bounds = [(ub[2 * i], ub[2 * i + 1], vb[2 * i], vb[2 * i + 1]) for i in range(len(ranges))]
     for a, b, c, d in bounds:
-        if (b - a) * (d - c) > 50_000_000:
+        if (b - a) * (d - c) > OPPORTUNITY_LIMIT:
             raise OverflowError('per-subtile candidate opportunity limit exceeded')
     runtime.check()
     return Prepared(first, end, width, us, vs, bounds, time.perf_counter() - started, ranges)
@@ -96,7 +101,10 @@ def prepare(runtime: Runtime, config: Config, af: GPUFocus, bf: GPUFocus,
 
     With both wheels present this is one library call (`pcube_prepare_range`,
     which also computes the subtile geometry); otherwise it is the reference's
-    sequence of two windows and two searches.
+    sequence of two windows and two searches.  When a MaskedMatcher of this
+    configuration exists, the call is `pcube_prepare_match`: the batch's match
+    is enqueued as soon as its bounds are known, and that matcher's run()
+    collects it (prep_seconds then includes the enqueue).
     """
     started = time.perf_counter()
     if af.wheel is None or bf.wheel is None:
@@ -114,10 +122,17 @@ def prepare(runtime: Runtime, config: Config, af: GPUFocus, bf: GPUFocus,
     runtime.check()
     mask = (1 << 64) - 1
     base = out.ctypes.data
-    code = native.lib().pcube_prepare_range(
-        af.handle, bf.handle, config.A, config.B, config.lo & mask, config.lo >> 64, config.hi & mask, config.hi >> 64,
-        first, end, width, native.ptr64_at(base), ctypes.byref(ntiles), us.address, cap, vs.address, cap,
-        ctypes.byref(nu), ctypes.byref(nv), native.ptr64_at(base + 8 * 256), native.ptr64_at(base + 8 * 512))
+    args = (af.handle, bf.handle, config.A, config.B, config.lo & mask, config.lo >> 64, config.hi & mask,
+            config.hi >> 64, first, end, width, native.ptr64_at(base), ctypes.byref(ntiles), us.address, cap,
+            vs.address, cap, ctypes.byref(nu), ctypes.byref(nv), native.ptr64_at(base + 8 * 256),
+            native.ptr64_at(base + 8 * 512))
+    matcher = runtime.matcher_for(config)
+    launch = ctypes.c_uint64()
+    if matcher is None:
+        code = native.lib().pcube_prepare_range(*args)
+    else:
+        code = native.lib().pcube_prepare_match(*args, matcher.engine.handle, OUTPUT_CAP, matcher.engine_id,
+                                                OPPORTUNITY_LIMIT, ctypes.byref(launch))
     if code == native.RANGE:
         raise OverflowError('focus window exceeds bounded period count')
     if code == native.OVERFLOW:
@@ -127,7 +142,7 @@ def prepare(runtime: Runtime, config: Config, af: GPUFocus, bf: GPUFocus,
     n = 4 * ntiles.value
     flat = out.tolist()
     focus_u, opportunities, largest = flat[512:515]
-    if largest > 50_000_000:
+    if largest > OPPORTUNITY_LIMIT:
         raise OverflowError('per-subtile candidate opportunity limit exceeded')
     quads = iter(flat[:n])
     ranges = list(zip(quads, quads, quads, quads))
@@ -135,7 +150,8 @@ def prepare(runtime: Runtime, config: Config, af: GPUFocus, bf: GPUFocus,
     rows = list(zip(quads, quads, quads, quads))
     runtime.check()
     return Prepared(first, end, width, us, vs, rows, time.perf_counter() - started, ranges,  # type: ignore[arg-type]
-                    (focus_u, opportunities))
+                    (focus_u, opportunities),
+                    (matcher.engine, launch.value, OUTPUT_CAP) if launch.value else None)
 
 
 def python_witnesses(values: list[int], p: int) -> list[dict[str, object]]:
@@ -249,14 +265,18 @@ class FusedMatcher:
         self.prefilter_seconds = 0.0
         self.intersection_seconds = 0.0   # the match kernel alone (intersection and chain) of the last run
 
-    def run(self, batch: Prepared, output_cap: int = 8192) -> dict[str, Any]:
+    def run(self, batch: Prepared, output_cap: int = OUTPUT_CAP) -> dict[str, Any]:
         started = time.perf_counter()
         ranges = batch.ranges if batch.ranges is not None else tile_ranges(self.config, batch.first, batch.end,
                                                                              batch.width)
         if len(ranges) != len(batch.bounds) or not 0 < output_cap <= 8192:
             raise ValueError('invalid prepared geometry or output capacity')
-        # the device matches while the host builds the receipt records
-        self.engine.start(batch.us, batch.vs, batch.first, batch.width, len(ranges), output_cap, self.engine_id)
+        # the device matches while the host builds the receipt records; prepare()
+        # may have started this batch on this matcher already
+        eager = batch.started
+        if not (eager is not None and eager[0] is self.engine and eager[2] == output_cap
+                and self.engine.adopt(eager[1], len(ranges), output_cap)):
+            self.engine.start(batch.us, batch.vs, batch.first, batch.width, len(ranges), output_cap, self.engine_id)
         skeleton = scope_skeleton(ranges)
         counters, per_tile, seconds = self.engine.finish(overflow_ok=True)
         self.prefilter_seconds = seconds[0]
@@ -287,8 +307,9 @@ class MaskedMatcher(FusedMatcher):
         if not self.engine.order:
             raise ValueError('bitset experiment requires at least one residual filter')
         runtime.compile('prefilter_staged')
+        runtime.register_matcher(config, self)
 
-    def run(self, batch: Prepared, output_cap: int = 8192) -> dict[str, Any]:
+    def run(self, batch: Prepared, output_cap: int = OUTPUT_CAP) -> dict[str, Any]:
         result = super().run(batch, output_cap)
         result['prefilter_build_seconds'] = self.prefilter_seconds
         return result

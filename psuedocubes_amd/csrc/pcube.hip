@@ -663,6 +663,7 @@ struct Pending {
     uint64_t us = 0, nu = 0, vs = 0, nv = 0, v_first = 0, tile_width = 0, cap = 0, list_cap = 0;
     uint32_t ntiles = 0;
     int engine = 0;
+    uint64_t launch = 0;   // the matcher's launch count right after this batch was enqueued
 };
 
 struct Matcher {
@@ -687,6 +688,7 @@ struct Matcher {
     Pinned down;           // its download: [status words, counters]
     bool kernel_span = false;   // the last batch's match kernel was timed by ev[1] and ev[2]
     uint64_t head_zero = 0;   // leading bytes of status known to be zero (cleared after each download)
+    uint64_t launches = 0;    // enqueues that used the batch buffers (status, counters, list, rows, masks)
     uint64_t* list_count() const { return status.as<uint64_t>(); }
     uint32_t* error() const { return (uint32_t*)(status.as<uint64_t>() + 1); }
     ~Matcher() {
@@ -869,6 +871,7 @@ int match_launch(Matcher& mt) {
     const uint64_t nc = (uint64_t)pd.ntiles * (mt.ng + 3);
     hipEvent_t e0 = mt.ev[0], e1 = mt.ev[1], e2 = mt.ev[2], k0 = mt.ev[3], k1 = mt.ev[4];
     uint64_t* head = mt.status.as<uint64_t>();
+    ++mt.launches;
     // the last batch's finish cleared the buffer already, unless it has grown or been used since
     if (mt.head_zero < (nc + 4) * 8)
         hipLaunchKernelGGL(k_fill2_u64, dim3(std::min<uint64_t>(blocks_for(nc + 4, 256), 4096)), dim3(256), 0,
@@ -1471,6 +1474,33 @@ int pcube_prepare_range(int focus_u, int focus_v, uint64_t A, uint64_t B, uint64
     return PCUBE_OK;
 }
 
+int pcube_prepare_match(int focus_u, int focus_v, uint64_t A, uint64_t B, uint64_t lo0, uint64_t lo1, uint64_t hi0,
+                        uint64_t hi1, uint64_t first, uint64_t end, uint64_t width, uint64_t* tiles, uint32_t* ntiles,
+                        uint64_t us_out, uint64_t us_cap, uint64_t vs_out, uint64_t vs_cap, uint64_t* nu, uint64_t* nv,
+                        uint64_t* bounds, uint64_t* totals, int matcher, uint64_t cap, int engine,
+                        uint64_t max_opportunities, uint64_t* launch) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    if (!totals || !launch) return fail(PCUBE_INVALID, "null totals or launch");
+    *launch = 0;
+    TRY(pcube_prepare_range(focus_u, focus_v, A, B, lo0, lo1, hi0, hi1, first, end, width, tiles, ntiles, us_out,
+                            us_cap, vs_out, vs_cap, nu, nv, bounds, totals));
+    Matcher* mt = matcher_at(matcher);
+    // a batch the caller will reject, or a matcher with a batch still to collect, is left alone
+    if (!mt || mt->pending.active || totals[2] > max_opportunities) return PCUBE_OK;
+    // a start that fails here is left for pcube_match_start to report when the batch is run
+    if (pcube_match_start(matcher, us_out, *nu, vs_out, *nv, first, width, *ntiles, cap, engine) == PCUBE_OK)
+        *launch = mt->launches;
+    return PCUBE_OK;
+}
+
+int pcube_match_adoptable(int matcher, uint64_t launch, int* adoptable) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    if (!adoptable) return fail(PCUBE_INVALID, "null result");
+    Matcher* mt = matcher_at(matcher);
+    *adoptable = mt && launch && mt->pending.active && mt->launches == launch ? 1 : 0;
+    return PCUBE_OK;
+}
+
 int pcube_searchsorted(uint64_t sorted, uint64_t n, const uint64_t* queries, uint64_t nq, uint64_t* out) {
     std::lock_guard<std::recursive_mutex> lock(g_mu);
     if (!nq) return PCUBE_OK;
@@ -1600,6 +1630,7 @@ int pcube_match_start(int matcher, uint64_t us, uint64_t nu, uint64_t vs, uint64
     for (hipEvent_t& e : mt->ev)
         if (!e) HIP_TRY(hipEventCreate(&e));
     TRY(match_launch(*mt));
+    pd.launch = mt->launches;
     pd.active = true;
     return PCUBE_OK;
 }
@@ -1611,6 +1642,8 @@ int pcube_match_finish(int matcher, uint64_t* counters, uint64_t* values, uint64
     Pending& pd = mt->pending;
     if (!pd.active) return fail(PCUBE_INVALID, "no batch started on this matcher");
     pd.active = false;
+    if (mt->launches != pd.launch)
+        return fail(PCUBE_INVALID, "a lease on this matcher reused the batch buffers before the batch was collected");
     const uint64_t nc = (uint64_t)pd.ntiles * (mt->ng + 3);
     uint64_t* hd = mt->down.as<uint64_t>();
     HIP_TRY(hipEventSynchronize(mt->ev[5]));
@@ -1713,6 +1746,7 @@ int pcube_lease(int matcher, int focus_u, int focus_v, uint32_t nbatches, const 
     // per batch geometry
     const uint64_t stride = mt->ng + 3;
     const uint64_t list_cap = std::max<uint64_t>(1, std::min<uint64_t>(ntot * cap, 1ull << 22));
+    ++mt->launches;   // the batch buffers are reused: a started batch can no longer be collected
     TRY(mt->counters.ensure(ntot * stride * 8));
     TRY(mt->list.ensure(list_cap * 24));
     // every batch of the lease, then one download; a bitset-survivor list that

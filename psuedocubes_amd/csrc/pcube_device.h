@@ -334,43 +334,91 @@ struct MatchArgs {
     uint32_t item_cap;
 };
 
-// First index i in [0, nu) with bound < us[i]*B, or nu.  With a directory
-// (dir[k] = us[k * stride], ndir entries, in LDS) the search first finds the
-// first directory entry above the bound and then the answer among the at most
-// stride coordinates before it, so few probes reach global memory.
-__device__ __forceinline__ uint32_t first_above(const MatchArgs& m, const U128& bound, const uint64_t* dir,
-                                                uint32_t ndir, uint32_t stride) {
-    uint32_t l = 0, h = m.nu;
-    if (dir) {
-        uint32_t a = 0, b = ndir;
-        while (a < b) {
-            const uint32_t mid = a + ((b - a) >> 1);
-            if (less_wide(bound, mul_wide(dir[mid], m.B))) b = mid; else a = mid + 1;
-        }
-        if (a == 0) return 0;
-        l = (a - 1) * stride + 1;
-        h = a * stride < m.nu ? a * stride : m.nu;
+// State of one search for the first index i in [0, nu) with bound < us[i]*B
+// (nu when there is none): the answer lies in [l, h], and P(i) =
+// (us[i]*B <= bound) holds exactly below it; done once the answer is in l.
+struct AboveSearch {
+    U128 bound;
+    uint32_t l, h;
+    bool done;
+};
+
+// With a directory (dir[k] = us[k * stride], ndir entries, in LDS): the first
+// directory entry above the bound leaves the answer among the at most stride
+// coordinates before it, so few probes reach global memory.
+__device__ __forceinline__ void above_directory(const MatchArgs& m, AboveSearch& s, const uint64_t* dir, uint32_t ndir,
+                                                uint32_t stride, uint32_t a) {
+    if (a == 0) {
+        s.l = 0;
+        s.done = true;
+        return;
     }
-    // the answer lies in [l, h]; P(i) = (us[i]*B <= bound) holds exactly below it.
-    // Eight independent probes per round instead of one: a 63-coordinate window
-    // takes two rounds of loads, not six dependent ones.
-    while (h - l > 8) {
-        const uint32_t step = (h - l + 7) / 8;
-        uint32_t below = 0;
-#pragma unroll
-        for (uint32_t k = 0; k < 8; ++k) {
-            const uint32_t at = l + k * step;
-            below += at < h && !less_wide(bound, mul_wide(m.us[at], m.B)) ? 1u : 0u;
-        }
-        if (below == 0) return l;
-        const uint32_t top = l + below * step;
-        l += (below - 1) * step + 1;
-        h = top < h ? top : h;
-    }
+    s.l = (a - 1) * stride + 1;
+    s.h = a * stride < m.nu ? a * stride : m.nu;
+}
+
+// Narrow a search with eight probes of us already loaded (x[k] = us[l + k*step]
+// for the in-range k).
+__device__ __forceinline__ void above_round(const MatchArgs& m, AboveSearch& s, uint32_t step, const uint64_t (&x)[8]) {
     uint32_t below = 0;
 #pragma unroll
-    for (uint32_t k = 0; k < 8; ++k) below += l + k < h && !less_wide(bound, mul_wide(m.us[l + k], m.B)) ? 1u : 0u;
-    return l + below;
+    for (uint32_t k = 0; k < 8; ++k)
+        below += s.l + k * step < s.h && !less_wide(s.bound, mul_wide(x[k], m.B)) ? 1u : 0u;
+    if (step == 1) {   // the last round: the window held at most eight coordinates
+        s.l += below;
+        s.done = true;
+        return;
+    }
+    if (below == 0) {
+        s.done = true;
+        return;
+    }
+    const uint32_t top = s.l + below * step;
+    s.l += (below - 1) * step + 1;
+    s.h = top < s.h ? top : s.h;
+}
+
+// first index with bound < us[i]*B for two bounds at once (the two ends of a
+// row or of a piece).  The searches advance together, so each round's LDS or
+// global probes of both are in flight at the same time: eight probes per
+// search per round, a 63-coordinate window in two rounds of loads and a
+// final round over the last eight.
+__device__ __forceinline__ void first_above_pair(const MatchArgs& m, const U128& b0, const U128& b1,
+                                                 const uint64_t* dir, uint32_t ndir, uint32_t stride, uint32_t& r0,
+                                                 uint32_t& r1) {
+    AboveSearch s[2] = {{b0, 0, m.nu, false}, {b1, 0, m.nu, false}};
+    if (dir) {
+        uint32_t a[2] = {0, 0}, e[2] = {ndir, ndir};
+        while (a[0] < e[0] || a[1] < e[1]) {
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                if (a[i] < e[i]) {
+                    const uint32_t mid = a[i] + ((e[i] - a[i]) >> 1);
+                    if (less_wide(s[i].bound, mul_wide(dir[mid], m.B))) e[i] = mid; else a[i] = mid + 1;
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i) above_directory(m, s[i], dir, ndir, stride, a[i]);
+    }
+    while (!s[0].done || !s[1].done) {
+        uint32_t step[2];
+        uint64_t x[2][8];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            step[i] = s[i].h - s[i].l > 8 ? (s[i].h - s[i].l + 7) / 8 : 1u;
+#pragma unroll
+            for (uint32_t k = 0; k < 8; ++k) {
+                const uint32_t at = s[i].l + k * step[i];
+                x[i][k] = !s[i].done && at < s[i].h ? m.us[at] : 0ull;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+            if (!s[i].done) above_round(m, s[i], step[i], x[i]);
+    }
+    r0 = s[0].l;
+    r1 = s[1].l;
 }
 
 // Exact u bounds of row j by 128-bit binary search, subtile, v mod q.
@@ -390,8 +438,10 @@ __device__ __forceinline__ void build_row(const MatchArgs& m, uint32_t j, const 
     const U128 shift = mul_wide(v, m.A);
     const U128 lower = add_wide(shift, m.lo), upper = add_wide(shift, m.hi);
     // first index with u*B > bound, for the lower and the upper bound
-    row.left = first_above(m, lower, dir, ndir, stride);
-    row.right = first_above(m, upper, dir, ndir, stride);
+    uint32_t left, right;
+    first_above_pair(m, lower, upper, dir, ndir, stride, left, right);
+    row.left = left;
+    row.right = right;
     if (row.flags) row.right = row.left;  // an invalid row contributes nothing
     // u*B >= v*A holds for every u in range because u*B > v*A + lo; a violation
     // (impossible for sorted input) is reported as the underflow counter.

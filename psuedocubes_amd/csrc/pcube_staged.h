@@ -247,8 +247,8 @@ __device__ __forceinline__ void piece_chunks(const MatchArgs& m, uint32_t p, con
     const uint32_t j0 = p * kPieceRows, j1 = min(m.nv, j0 + kPieceRows);
     const U128 lower = add_wide(mul_wide(m.vs[j0], m.A), m.lo);
     const U128 upper = add_wide(mul_wide(m.vs[j1 - 1], m.A), m.hi);
-    const uint32_t left0 = first_above(m, lower, dir, ndir, stride);
-    const uint32_t right1 = first_above(m, upper, dir, ndir, stride);
+    uint32_t left0, right1;
+    first_above_pair(m, lower, upper, dir, ndir, stride, left0, right1);
     c_lo = 1;
     c_hi = 0;
     if (right1 > left0) {

@@ -11,6 +11,7 @@ import ctypes
 import hashlib
 import time
 from math import prod
+import weakref
 from typing import Any, Callable
 
 import numpy as np
@@ -102,6 +103,20 @@ class Runtime:
         self.device_name = name.value.decode()
         self.compute_units = cus.value
         self._library_sha = hashlib.sha256(native.LIBRARY.read_bytes()).hexdigest()
+        # per configuration, the matcher whose match prepare() starts with each batch
+        self._batch_matchers: dict[tuple[int, ...], weakref.ref] = {}
+
+    @staticmethod
+    def _config_key(config: Config) -> tuple[int, ...]:
+        return (config.p, config.lo, config.hi, config.A, config.B)
+
+    def register_matcher(self, config: Config, matcher: Any) -> None:
+        """prepare() of a batch of config starts its match on matcher (the latest registered)."""
+        self._batch_matchers[self._config_key(config)] = weakref.ref(matcher)
+
+    def matcher_for(self, config: Config) -> Any:
+        ref = self._batch_matchers.get(self._config_key(config))
+        return ref() if ref is not None else None
 
     def device_count(self) -> int:
         """Visible devices (the reference asks CuPy, `tools/reproduce.py:36`)."""
@@ -376,6 +391,18 @@ class MatchEngine:
                     config.hi & MASK, config.hi >> 64, native.ptr32(moduli), len(order), ctypes.byref(handle))
         self.handle = handle.value
         self._values: np.ndarray | None = None
+
+    def adopt(self, launch: int, ntiles: int, cap: int) -> bool:
+        """Take over the batch that pcube_prepare_match started with token launch, if it is still
+        this matcher's pending batch; finish() then collects it."""
+        if not launch:
+            return False
+        self.runtime.check()
+        ok = ctypes.c_int()
+        native.check(native.lib().pcube_match_adoptable(self.handle, launch, ctypes.byref(ok)), 'pcube_match_adoptable')
+        if ok.value:
+            self._pending = (ntiles, cap)
+        return bool(ok.value)
 
     def run(self, us: DeviceArray, vs: DeviceArray, v_first: int, tile_width: int, ntiles: int, cap: int,
             engine: int, overflow_ok: bool = False) -> tuple[np.ndarray, list[list[int]], list[float]]:

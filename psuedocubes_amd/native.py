@@ -43,6 +43,9 @@ SIGNATURES: dict[str, list[Any]] = {
     'pcube_tile_ranges': [u64, u64, u64, u64, u64, u64, u64, u64, u64, P_u64, P_u32],
     'pcube_prepare_range': [c_int, c_int, u64, u64, u64, u64, u64, u64, u64, u64, u64, P_u64, P_u32, u64, u64, u64,
                             u64, P_u64, P_u64, P_u64, P_u64],
+    'pcube_prepare_match': [c_int, c_int, u64, u64, u64, u64, u64, u64, u64, u64, u64, P_u64, P_u32, u64, u64, u64,
+                            u64, P_u64, P_u64, P_u64, P_u64, c_int, u64, c_int, u64, P_u64],
+    'pcube_match_adoptable': [c_int, u64, P_int],
     'pcube_prepare': [c_int, c_int, u32, P_u64, u64, u64, u64, u64, P_u64, P_u64, P_u64],
     'pcube_searchsorted': [u64, u64, P_u64, u64, P_u64],
     'pcube_matcher_create': [u64, u64, u64, u64, u64, u64, P_u32, u32, P_int],
