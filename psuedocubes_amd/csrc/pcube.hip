@@ -262,7 +262,7 @@ struct Wheel {
     bool ready = false;
     uint64_t M1 = 1;
     uint32_t n1 = 0, nouter = 0, allow_words = 0, packed = 0;
-    std::vector<uint32_t> R1host;
+    std::vector<uint64_t> R1host;
     Modulus outer[kMaxOuterRings];
     double rate = 1.0;      // expected fraction of candidates that pass the outer rings
     Buffer R1, allow, slots, wave_count, seg_excl, chunk_total, win_info;
@@ -420,7 +420,8 @@ Modulus make_modulus(uint32_t q) {
     return m;
 }
 
-constexpr uint32_t kMaxInnerResidues = 1u << 21;
+constexpr uint32_t kMaxInnerResidues = 1u << 27;           // 8-byte entries: at most 1 GiB
+constexpr uint64_t kInnerModulusLimit = 1ull << kResidueBits;   // inner residues fit an entry's low bits
 
 // Plan the segments of a set of windows; returns the total segment count.
 uint64_t wheel_plan(const Wheel& wh, uint32_t n, const uint64_t* lows, const uint64_t* highs,
@@ -514,16 +515,16 @@ int wheel_check(uint64_t nseg) {
 int wheel_setup(Wheel& wh, const WheelInputs& in, uint32_t slot_cap, WheelArgs& a, uint64_t& nchunks) {
     const uint64_t nwaves = in.nseg * kWheelWaves;
     nchunks = (in.nseg + kWheelScanChunk - 1) / kWheelScanChunk;
-    TRY(wh.slots.ensure(nwaves * slot_cap * 4));
+    TRY(wh.slots.ensure(nwaves * slot_cap * 8));
     TRY(wh.wave_count.ensure(nwaves * 4));
     TRY(wh.seg_excl.ensure(in.nseg * 4));
     TRY(wh.chunk_total.ensure(nchunks * 8));
     memset(&a, 0, sizeof a);
-    a.R1 = wh.R1.as<uint2>();
+    a.R1 = wh.R1.as<uint64_t>();
     a.windows = in.windows;
     a.seg_first = in.seg_first;
     a.allow = wh.allow.as<uint32_t>();
-    a.slots = wh.slots.as<uint32_t>();
+    a.slots = wh.slots.as<uint64_t>();
     a.wave_count = wh.wave_count.as<uint32_t>();
     a.flag = in.flag;
     a.M1 = wh.M1; a.nseg = in.nseg;
@@ -1272,26 +1273,89 @@ int pcube_focus_wheel(int focus, const uint32_t* moduli, const uint32_t* sizes, 
     wh.rate = 1.0;
     uint64_t M1 = 1, n1 = 1;
     uint32_t k = 0;
-    while (k < nrings && M1 * moduli[k] < (1ull << 32) && n1 * sizes[k] <= kMaxInnerResidues) {
+    while (k < nrings && M1 * moduli[k] < kInnerModulusLimit && n1 * sizes[k] <= kMaxInnerResidues) {
         M1 *= moduli[k];
         n1 *= sizes[k];
         ++k;
     }
     if (nrings - k > (uint32_t)kMaxOuterRings) return fail(PCUBE_INVALID, "too many outer rings for the wheel");
-    // inner residues by CRT: y = sum a_i * (M1/q_i) * ((M1/q_i)^-1 mod q_i) mod M1
+    // entries carry r mod q_j of the outer rings as bytes above r when every
+    // outer q < 256 and there are at most kPackedOuterRings of them
+    wh.packed = nrings - k <= (uint32_t)kPackedOuterRings ? 1u : 0u;
+    for (uint32_t j = k; j < nrings; ++j)
+        if (moduli[j] >= 256) wh.packed = 0;
+    const uint32_t npack = wh.packed ? nrings - k : 0;
+    // inner residues in ascending order, one ring at a time: the residues mod
+    // M*q are t*M + r for t in [0, q) and r in the sorted residues mod M, kept
+    // when (t*M + r) mod q is allowed; visiting t, then r, yields them sorted.
+    // The last ring's pass also forms the packed outer residues, each as
+    // (r mod q_j + t*M mod q_j) mod q_j.
     std::vector<uint64_t> cur(1, 0), next;
-    uint64_t offset = 0;
+    std::vector<uint32_t> bytes(1, 0), next_bytes, rq;
+    std::vector<uint8_t> ok, ro[kPackedOuterRings];
+    uint64_t M = 1, offset = 0;
     for (uint32_t i = 0; i < k; ++i) {
-        const uint64_t q = moduli[i], co = M1 / q, lift = co * modinv(co % q, q) % M1;
-        next.clear();
-        next.reserve(cur.size() * sizes[i]);
-        for (uint64_t v : cur)
-            for (uint32_t e = 0; e < sizes[i]; ++e) next.push_back((v + (uint64_t)residues[offset + e] * lift) % M1);
+        const uint32_t q = moduli[i];
+        ok.assign(q, 0);
+        for (uint32_t e = 0; e < sizes[i]; ++e) {
+            if (residues[offset + e] >= q) return fail(PCUBE_INVALID, "ring residue not below its modulus");
+            ok[residues[offset + e]] = 1;
+        }
+        rq.resize(cur.size());
+        for (size_t x = 0; x < cur.size(); ++x) rq[x] = (uint32_t)(cur[x] % q);
+        const bool last = i + 1 == k;
+        uint32_t oq[kPackedOuterRings] = {}, osh[kPackedOuterRings] = {}, ostep[kPackedOuterRings] = {};
+        if (last)
+            for (uint32_t j = 0; j < npack; ++j) {
+                oq[j] = moduli[k + j];
+                ostep[j] = (uint32_t)(M % oq[j]);
+                ro[j].resize(cur.size());
+                for (size_t x = 0; x < cur.size(); ++x) ro[j][x] = (uint8_t)(cur[x] % oq[j]);
+            }
+        const uint32_t mq = (uint32_t)(M % q);
+        // distinct residues of coprime rings give exactly |R| * |R_q| survivors;
+        // candidates are written unconditionally and kept by advancing n
+        const size_t cap = cur.size() * sizes[i];
+        next.resize(cap + 1);
+        if (last) next_bytes.resize(cap + 1);
+        size_t n = 0;
+        uint32_t shift = 0;   // t*M mod q
+        for (uint32_t t = 0; t < q; ++t) {
+            const uint64_t base = (uint64_t)t * M;
+            for (size_t x = 0; x < cur.size(); ++x) {
+                uint32_t res = rq[x] + shift;
+                res = res >= q ? res - q : res;
+                const size_t at = n < cap ? n : cap;
+                next[at] = base + cur[x];
+                if (last) {
+                    uint32_t b = 0;
+                    for (uint32_t j = 0; j < npack; ++j) {
+                        uint32_t m = ro[j][x] + osh[j];
+                        b |= (m >= oq[j] ? m - oq[j] : m) << (8 * j);
+                    }
+                    next_bytes[at] = b;
+                }
+                n += ok[res];
+            }
+            shift += mq;
+            shift = shift >= q ? shift - q : shift;
+            for (uint32_t j = 0; j < npack; ++j) {
+                osh[j] += ostep[j];
+                osh[j] = osh[j] >= oq[j] ? osh[j] - oq[j] : osh[j];
+            }
+        }
+        if (n > cap) return fail(PCUBE_INVALID, "focus rings are not pairwise coprime");
+        next.resize(n);
         cur.swap(next);
+        if (last) {
+            next_bytes.resize(n);
+            bytes.swap(next_bytes);
+        }
+        M *= q;
         offset += sizes[i];
     }
-    wh.R1host.assign(cur.begin(), cur.end());
-    std::sort(wh.R1host.begin(), wh.R1host.end());
+    if (k == 0) bytes.assign(1, 0);   // r = 0: every outer residue is 0
+    wh.R1host.swap(cur);
     wh.M1 = M1;
     wh.n1 = (uint32_t)wh.R1host.size();
     wh.nouter = nrings - k;
@@ -1315,19 +1379,9 @@ int pcube_focus_wheel(int focus, const uint32_t* moduli, const uint32_t* sizes, 
     if (bits.empty()) bits.push_back(0);
     if (bits.size() > (size_t)kMaxOuterBits / 32) return fail(PCUBE_INVALID, "outer rings too large for the wheel");
     wh.allow_words = (uint32_t)bits.size();
-    // entries (r, outer residues r mod q_j as bytes when every outer q < 256 and there are at most 4)
-    wh.packed = wh.nouter <= 4 ? 1u : 0u;
-    for (uint32_t j = 0; j < wh.nouter; ++j)
-        if (wh.outer[j].q >= 256) wh.packed = 0;
-    std::vector<uint32_t> entries(2 * (uint64_t)wh.n1);
-    for (uint32_t i = 0; i < wh.n1; ++i) {
-        const uint32_t r = wh.R1host[i];
-        uint32_t packed = 0;
-        if (wh.packed)
-            for (uint32_t j = 0; j < wh.nouter; ++j) packed |= (r % wh.outer[j].q) << (8 * j);
-        entries[2 * i] = r;
-        entries[2 * i + 1] = packed;
-    }
+    std::vector<uint64_t> entries(wh.n1);
+    for (uint32_t i = 0; i < wh.n1; ++i)
+        entries[i] = wh.R1host[i] | (wh.packed ? (uint64_t)bytes[i] << kResidueBits : 0ull);
     TRY(wh.R1.ensure(std::max<uint64_t>(8, (uint64_t)wh.n1 * 8)));
     TRY(wh.allow.ensure(bits.size() * 4));
     HIP_TRY(hipMemcpyAsync(wh.R1.p, entries.data(), (uint64_t)wh.n1 * 8, hipMemcpyHostToDevice, g.stream));

@@ -8,7 +8,10 @@
 // each candidate is then tested against the outer rings with
 // (r mod q + t*M1 mod q) mod q in R_q.  Candidates are visited in (t, r) order,
 // so each window's output is ascending with no sort.  The cost is proportional
-// to the window width times |R1| / M1, independent of the wheel's size.
+// to the window width times |R1| / M1, independent of the wheel's size, so the
+// inner prefix takes every ring it can (M1 < 2^40, |R1| <= 2^27 entries of 8
+// bytes): each ring moved inside divides the candidates read and tested by
+// q / |R_q|, about 3 for the cubic-residue rings.
 //
 // Work is cut into segments of kWheelSegment consecutive candidates of one t;
 // a block handles one segment, each of its four waves 1024 candidates.  One
@@ -33,6 +36,12 @@ constexpr int kWheelWaveSpan = kWheelSteps * kWave;           // candidates per 
 constexpr int kWheelScanChunk = 1024;                         // segments per segment-scan block
 constexpr int kMaxOuterRings = 8;
 constexpr int kMaxOuterBits = 8 * 1024;   // allowed-residue bits of the outer rings
+// An R1 entry is one 64-bit word: the inner residue r in the low
+// kResidueBits bits and, when the wheel is packed, r mod q_j of the first
+// kPackedOuterRings outer rings as bytes above it.
+constexpr int kResidueBits = 40;
+constexpr uint64_t kResidueMask = (1ull << kResidueBits) - 1;
+constexpr int kPackedOuterRings = (64 - kResidueBits) / 8;
 
 struct WheelWindow {
     uint64_t low, high;  // the window [low, high)
@@ -44,11 +53,11 @@ struct WheelWindow {
 };
 
 struct WheelArgs {
-    const uint2* R1;             // sorted inner residues r with their outer residues packed as bytes
+    const uint64_t* R1;          // sorted inner residues r, outer residues packed above them
     const WheelWindow* windows;  // nwin windows
     const uint64_t* seg_first;   // nwin + 1 prefix of segment counts
     const uint32_t* allow;       // outer-ring allowed bits, rings at Modulus::bit
-    uint32_t* slots;             // slot_cap inner residues per (segment, wave)
+    uint64_t* slots;             // slot_cap inner residues per (segment, wave)
     uint32_t* wave_count;        // survivors per (segment, wave)
     uint32_t* flag;              // set to 1 when a wave overflows its slot
     uint64_t M1, nseg;
@@ -114,30 +123,32 @@ __device__ __forceinline__ void wheel_slots_block(const WheelArgs& a, uint64_t s
     const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     const uint32_t wbegin = begin + wave * kWheelWaveSpan;
     const uint32_t wend = min(wbegin + kWheelWaveSpan, end);
-    uint32_t* slot = a.slots + (seg * kWheelWaves + wave) * a.slot_cap;
+    uint64_t* slot = a.slots + (seg * kWheelWaves + wave) * a.slot_cap;
     uint32_t count = 0;
     // issue every load of this wave first: one memory latency instead of sixteen
-    uint2 e[kWheelSteps];
+    uint64_t e[kWheelSteps];
 #pragma unroll
     for (int k = 0; k < kWheelSteps; ++k) {
         const uint32_t i = wbegin + k * kWave + lane;
-        e[k] = i < wend ? a.R1[i] : make_uint2(0, 0);
+        e[k] = i < wend ? a.R1[i] : 0ull;
     }
 #pragma unroll
     for (int k = 0; k < kWheelSteps; ++k) {
         const uint32_t i = wbegin + k * kWave + lane;
-        const uint64_t y = ybase + e[k].x;
+        const uint64_t r = e[k] & kResidueMask;
+        const uint64_t y = ybase + r;
         bool ok = i < wend && y >= w.low && y < w.high;
         if (kOuter >= 0) {
+            const uint32_t bytes = uint32_t(e[k] >> kResidueBits);
 #pragma unroll
             for (int j = 0; j < (kOuter > 0 ? kOuter : 0); ++j) {
-                uint32_t m = ((e[k].y >> (8 * j)) & 0xffu) + off[j];
+                uint32_t m = ((bytes >> (8 * j)) & 0xffu) + off[j];
                 m = m >= a.outer[j].q ? m - a.outer[j].q : m;
                 ok &= allowed_bit(allow, a.outer[j].bit + m);
             }
         } else {
             for (uint32_t j = 0; j < a.nouter; ++j) {
-                uint32_t m = mod32(e[k].x, a.outer[j]) + off[j];
+                uint32_t m = mod64(r, a.outer[j]) + off[j];
                 m = m >= a.outer[j].q ? m - a.outer[j].q : m;
                 ok = ok && allowed_bit(allow, a.outer[j].bit + m);
             }
@@ -145,7 +156,7 @@ __device__ __forceinline__ void wheel_slots_block(const WheelArgs& a, uint64_t s
         const uint64_t live = __ballot(ok);
         if (ok) {
             const uint32_t at = count + lanes_below(live);
-            if (at < a.slot_cap) slot[at] = e[k].x;
+            if (at < a.slot_cap) slot[at] = r;
         }
         count += __popcll(live);
     }
@@ -269,7 +280,7 @@ __device__ __forceinline__ void wheel_compact_block(const WheelArgs& a, const ui
     wheel_segment(w, a.n1, uint32_t(seg - w.seg_first), t, begin, end);
     const uint64_t ybase = t * a.M1;
     const uint32_t n = min(cw[wave], a.slot_cap);
-    const uint32_t* slot = a.slots + (seg * kWheelWaves + wave) * a.slot_cap;
+    const uint64_t* slot = a.slots + (seg * kWheelWaves + wave) * a.slot_cap;
     for (uint32_t i = lane; i < n; i += kWave) {
         const uint64_t o = base + i;
         if (o < cap) out[o] = ybase + slot[i];
@@ -317,7 +328,6 @@ PCUBE_WHEEL(0)
 PCUBE_WHEEL(1)
 PCUBE_WHEEL(2)
 PCUBE_WHEEL(3)
-PCUBE_WHEEL(4)
 #undef PCUBE_WHEEL
 
 // Host-side selection of the instantiation for a wheel.
@@ -327,8 +337,7 @@ inline const void* wheel_kernel(uint32_t packed, uint32_t nouter) {
         case 0: return (const void*)k_wheel_slots<0>;
         case 1: return (const void*)k_wheel_slots<1>;
         case 2: return (const void*)k_wheel_slots<2>;
-        case 3: return (const void*)k_wheel_slots<3>;
-        default: return (const void*)k_wheel_slots<4>;
+        default: return (const void*)k_wheel_slots<3>;
     }
 }
 
@@ -339,8 +348,7 @@ inline const void* wheel_pair_kernel(uint32_t packed, uint32_t nouter) {
         case 0: return (const void*)k_wheel_slots_pair<0>;
         case 1: return (const void*)k_wheel_slots_pair<1>;
         case 2: return (const void*)k_wheel_slots_pair<2>;
-        case 3: return (const void*)k_wheel_slots_pair<3>;
-        default: return (const void*)k_wheel_slots_pair<4>;
+        default: return (const void*)k_wheel_slots_pair<3>;
     }
 }
 
