@@ -761,8 +761,10 @@ int staged_enqueue(Matcher& mt, MatchArgs m, hipEvent_t ev_pre, hipEvent_t ev_ma
     for (int k = 0; k < kStagedFilters; ++k) pa.chunk_base[k] = mt.s_chunk_base[k];
     pa.items = items;
     pa.nitems = nitems;
-    hipLaunchKernelGGL(k_prefilter_staged, dim3(1 + pa.rows_blocks + pa.mask_blocks + pa.rho_blocks), dim3(256), 0,
-                       g.stream, m, pa);
+    uint32_t dir_stride, dir_entries;
+    directory_shape(m.nu, dir_stride, dir_entries);
+    hipLaunchKernelGGL(k_prefilter_staged, dim3(1 + pa.rows_blocks + pa.mask_blocks + pa.rho_blocks), dim3(256),
+                       std::max<uint32_t>(1, dir_entries) * 8, g.stream, m, pa);
     if (ev_pre) HIP_TRY(hipEventRecord(ev_pre, g.stream));
     // with both of these events recorded, the match kernel alone lies between
     // them and stands for its own timing events

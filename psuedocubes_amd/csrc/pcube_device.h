@@ -337,10 +337,13 @@ struct MatchArgs {
 // State of one search for the first index i in [0, nu) with bound < us[i]*B
 // (nu when there is none): the answer lies in [l, h], and P(i) =
 // (us[i]*B <= bound) holds exactly below it; done once the answer is in l.
+// hval = us[h] whenever h < nu, and once done, val = us[l] whenever l < nu:
+// every probe that moves h or ends the search has loaded that coordinate.
 struct AboveSearch {
     U128 bound;
     uint32_t l, h;
     bool done;
+    uint64_t hval, val;
 };
 
 // With a directory (dir[k] = us[k * stride], ndir entries, in LDS): the first
@@ -350,11 +353,13 @@ __device__ __forceinline__ void above_directory(const MatchArgs& m, AboveSearch&
                                                 uint32_t stride, uint32_t a) {
     if (a == 0) {
         s.l = 0;
+        s.val = dir[0];
         s.done = true;
         return;
     }
     s.l = (a - 1) * stride + 1;
     s.h = a * stride < m.nu ? a * stride : m.nu;
+    if (s.h < m.nu) s.hval = dir[a];   // a * stride < nu, so a < ndir
 }
 
 // Narrow a search with eight probes of us already loaded (x[k] = us[l + k*step]
@@ -364,18 +369,27 @@ __device__ __forceinline__ void above_round(const MatchArgs& m, AboveSearch& s, 
 #pragma unroll
     for (uint32_t k = 0; k < 8; ++k)
         below += s.l + k * step < s.h && !less_wide(s.bound, mul_wide(x[k], m.B)) ? 1u : 0u;
+    // x[below] = us[l + below*step] when that probe is in range (below < 8)
+    uint64_t xb = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k) xb = k == below ? x[k] : xb;
     if (step == 1) {   // the last round: the window held at most eight coordinates
+        s.val = s.l + below < s.h ? xb : s.hval;
         s.l += below;
         s.done = true;
         return;
     }
     if (below == 0) {
+        s.val = x[0];
         s.done = true;
         return;
     }
     const uint32_t top = s.l + below * step;
     s.l += (below - 1) * step + 1;
-    s.h = top < s.h ? top : s.h;
+    if (top < s.h) {
+        s.h = top;
+        s.hval = xb;
+    }
 }
 
 // first index with bound < us[i]*B for two bounds at once (the two ends of a
@@ -385,8 +399,8 @@ __device__ __forceinline__ void above_round(const MatchArgs& m, AboveSearch& s, 
 // final round over the last eight.
 __device__ __forceinline__ void first_above_pair(const MatchArgs& m, const U128& b0, const U128& b1,
                                                  const uint64_t* dir, uint32_t ndir, uint32_t stride, uint32_t& r0,
-                                                 uint32_t& r1) {
-    AboveSearch s[2] = {{b0, 0, m.nu, false}, {b1, 0, m.nu, false}};
+                                                 uint32_t& r1, uint64_t& u0) {
+    AboveSearch s[2] = {{b0, 0, m.nu, false, 0, 0}, {b1, 0, m.nu, false, 0, 0}};
     if (dir) {
         uint32_t a[2] = {0, 0}, e[2] = {ndir, ndir};
         while (a[0] < e[0] || a[1] < e[1]) {
@@ -419,12 +433,13 @@ __device__ __forceinline__ void first_above_pair(const MatchArgs& m, const U128&
     }
     r0 = s[0].l;
     r1 = s[1].l;
+    u0 = s[0].val;   // us[r0] when r0 < nu
 }
 
-// Exact u bounds of row j by 128-bit binary search, subtile, v mod q.
-__device__ __forceinline__ void build_row(const MatchArgs& m, uint32_t j, const uint64_t* dir = nullptr,
+// Exact u bounds of row j (coordinate v = vs[j]) by 128-bit binary search,
+// subtile, v mod q.
+__device__ __forceinline__ void build_row(const MatchArgs& m, uint32_t j, uint64_t v, const uint64_t* dir = nullptr,
                                           uint32_t ndir = 0, uint32_t stride = 0) {
-    const uint64_t v = m.vs[j];
     Row row;
     row.v = v;
     row.flags = 0;
@@ -439,13 +454,14 @@ __device__ __forceinline__ void build_row(const MatchArgs& m, uint32_t j, const 
     const U128 lower = add_wide(shift, m.lo), upper = add_wide(shift, m.hi);
     // first index with u*B > bound, for the lower and the upper bound
     uint32_t left, right;
-    first_above_pair(m, lower, upper, dir, ndir, stride, left, right);
+    uint64_t u_left;
+    first_above_pair(m, lower, upper, dir, ndir, stride, left, right, u_left);
     row.left = left;
     row.right = right;
     if (row.flags) row.right = row.left;  // an invalid row contributes nothing
     // u*B >= v*A holds for every u in range because u*B > v*A + lo; a violation
     // (impossible for sorted input) is reported as the underflow counter.
-    if (row.left < row.right && less_wide(mul_wide(m.us[row.left], m.B), shift))
+    if (row.left < row.right && less_wide(mul_wide(u_left, m.B), shift))
         atomicAdd(&m.counters[(uint64_t)row.tile * (m.ng + 3) + m.ng + 2], 1ull);
     // the staged engine keeps instead, per offset slot, 8 * line + column of
     // the mask row its lane reads there: line v mod q of the slot's filter
@@ -475,7 +491,7 @@ __device__ __forceinline__ void build_row(const MatchArgs& m, uint32_t j, const 
 
 __global__ void k_rows(MatchArgs m) {
     uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < m.nv) build_row(m, j);
+    if (j < m.nv) build_row(m, j, m.vs[j]);
 }
 
 // Staged engine rows: as k_rows, plus each subtile's valid-position count,
@@ -483,13 +499,19 @@ __global__ void k_rows(MatchArgs m) {
 // span few subtiles: one atomic per (wave, subtile).
 constexpr int kRowDirectory = 4096;   // u coordinates a row block samples into LDS
 
+// Stride (a power of two >= 64) and entry count of the directory of nu u
+// coordinates: every stride-th coordinate, at most kRowDirectory entries.
+__host__ __device__ inline void directory_shape(uint32_t nu, uint32_t& stride, uint32_t& ndir) {
+    stride = 64;
+    while ((nu + stride - 1) / stride > (uint32_t)kRowDirectory) stride <<= 1;
+    ndir = (nu + stride - 1) / stride;
+}
+
 // Fill dir with every stride-th u coordinate (stride a power of two >= 64 with
 // at most kRowDirectory samples) with the whole 256-thread block; ends with a
 // barrier.
 __device__ __forceinline__ void load_directory(const MatchArgs& m, uint64_t* dir, uint32_t& ndir, uint32_t& stride) {
-    stride = 64;
-    while ((m.nu + stride - 1) / stride > (uint32_t)kRowDirectory) stride <<= 1;
-    ndir = (m.nu + stride - 1) / stride;
+    directory_shape(m.nu, stride, ndir);
     for (uint32_t first = 0; first < ndir; first += 256 * 16) {
         uint64_t v[16];
 #pragma unroll
@@ -510,13 +532,15 @@ __device__ __forceinline__ void load_directory(const MatchArgs& m, uint64_t* dir
 // valid-position count, sum of right - left over its rows.  Rows are sorted by
 // v, so a wave's rows span few subtiles: one atomic per (wave, subtile).
 __device__ __forceinline__ void staged_rows_block(const MatchArgs& m, uint32_t block, uint64_t* dir) {
+    const uint32_t j = block * 256 + threadIdx.x;
+    // issued before the directory's loads, so it arrives with them
+    const uint64_t v = j < m.nv ? m.vs[j] : 0ull;
     uint32_t ndir, stride;
     load_directory(m, dir, ndir, stride);
-    const uint32_t j = block * 256 + threadIdx.x;
     uint32_t tile = 0xffffffffu;
     uint64_t count = 0;
     if (j < m.nv) {
-        build_row(m, j, ndir ? dir : nullptr, ndir, stride);
+        build_row(m, j, v, ndir ? dir : nullptr, ndir, stride);
         const Row& row = m.rows[j];
         if (!row.flags) {
             tile = row.tile;
@@ -972,7 +996,9 @@ __global__ void k_batch_rows(BatchArgs b) {
          g += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t w = find_window(b.win, t.nwin, g, [](const BatchWindow& x) { return x.row_begin; });
         const BatchWindow win = b.win[w];
-        build_row(window_args(b, win), uint32_t(g - win.row_begin));
+        const MatchArgs m = window_args(b, win);
+        const uint32_t j = uint32_t(g - win.row_begin);
+        build_row(m, j, m.vs[j]);
     }
 }
 

@@ -239,16 +239,24 @@ __device__ __forceinline__ void staged_append(const Staged& S, uint32_t lane, ui
     }
 }
 
-// Chunks [c_lo, c_hi] that the rows of piece p reach (none when c_lo > c_hi):
-// rows are sorted by v, so from the u bounds of the piece's first and last
-// rows, computed here exactly as the row builder does.
-__device__ __forceinline__ void piece_chunks(const MatchArgs& m, uint32_t p, const uint64_t* dir, uint32_t ndir,
-                                             uint32_t stride, uint32_t& c_lo, uint32_t& c_hi) {
+// v coordinates of the first and last rows of piece p.
+__device__ __forceinline__ void piece_ends(const MatchArgs& m, uint32_t p, uint64_t& first, uint64_t& last) {
     const uint32_t j0 = p * kPieceRows, j1 = min(m.nv, j0 + kPieceRows);
-    const U128 lower = add_wide(mul_wide(m.vs[j0], m.A), m.lo);
-    const U128 upper = add_wide(mul_wide(m.vs[j1 - 1], m.A), m.hi);
+    first = m.vs[j0];
+    last = m.vs[j1 - 1];
+}
+
+// Chunks [c_lo, c_hi] that the rows of a piece reach (none when c_lo > c_hi):
+// rows are sorted by v, so from the u bounds of the piece's first and last
+// rows (coordinates v_first, v_last), computed exactly as the row builder does.
+__device__ __forceinline__ void piece_chunks(const MatchArgs& m, uint64_t v_first, uint64_t v_last,
+                                             const uint64_t* dir, uint32_t ndir, uint32_t stride, uint32_t& c_lo,
+                                             uint32_t& c_hi) {
+    const U128 lower = add_wide(mul_wide(v_first, m.A), m.lo);
+    const U128 upper = add_wide(mul_wide(v_last, m.A), m.hi);
     uint32_t left0, right1;
-    first_above_pair(m, lower, upper, dir, ndir, stride, left0, right1);
+    uint64_t unused;
+    first_above_pair(m, lower, upper, dir, ndir, stride, left0, right1, unused);
     c_lo = 1;
     c_hi = 0;
     if (right1 > left0) {
@@ -260,20 +268,26 @@ __device__ __forceinline__ void piece_chunks(const MatchArgs& m, uint32_t p, con
 // Work list of the staged match: every (piece, chunk) pair whose rows reach
 // the chunk, piece-major, at most items_cap of them; nitems receives the full
 // count.  One 256-thread block: thread t takes a run of consecutive pieces, a
-// block scan turns the runs' counts into offsets.
+// block scan turns the runs' counts into offsets.  With at most 256 pieces
+// (one per thread) the block then writes the items together, coalesced.
 __device__ __forceinline__ void staged_plan_block(const MatchArgs& m, uint32_t npieces, uint2* items,
                                                   uint32_t items_cap, uint32_t* nitems, uint64_t* dir) {
     __shared__ uint32_t wave_total[256 / kWave];
-    uint32_t ndir, stride;
-    load_directory(m, dir, ndir, stride);
-    const uint64_t* d = ndir ? dir : nullptr;
+    __shared__ uint32_t piece_at[256], piece_lo[256];   // first item and first chunk of each piece
     const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     const uint32_t per = (npieces + 255) / 256;
     const uint32_t p0 = min(npieces, threadIdx.x * per), p1 = min(npieces, p0 + per);
+    // the first piece's end rows, issued before the directory's loads
+    uint64_t v_first = 0, v_last = 0;
+    if (p0 < p1) piece_ends(m, p0, v_first, v_last);
+    uint32_t ndir, stride;
+    load_directory(m, dir, ndir, stride);
+    const uint64_t* d = ndir ? dir : nullptr;
     uint32_t count = 0, lo1 = 1, hi1 = 0;   // the range of a one-piece run is kept
     for (uint32_t p = p0; p < p1; ++p) {
+        if (p != p0) piece_ends(m, p, v_first, v_last);
         uint32_t lo, hi;
-        piece_chunks(m, p, d, ndir, stride, lo, hi);
+        piece_chunks(m, v_first, v_last, d, ndir, stride, lo, hi);
         if (lo <= hi) count += hi - lo + 1;
         lo1 = lo;
         hi1 = hi;
@@ -288,9 +302,31 @@ __device__ __forceinline__ void staged_plan_block(const MatchArgs& m, uint32_t n
     __syncthreads();
     uint32_t at = inc - count;
     for (uint32_t w = 0; w < wave; ++w) at += wave_total[w];
+    if (per == 1) {
+        const uint32_t total = wave_total[0] + wave_total[1] + wave_total[2] + wave_total[3];
+        piece_at[threadIdx.x] = at;
+        piece_lo[threadIdx.x] = lo1;
+        __syncthreads();
+        const uint32_t n = min(total, items_cap);
+        for (uint32_t idx = threadIdx.x; idx < n; idx += 256) {
+            // the piece holding item idx: the last p with piece_at[p] <= idx
+            // (pieces without items share the next piece's start)
+            uint32_t l = 0, h = npieces;
+            while (h - l > 1) {
+                const uint32_t mid = (l + h) >> 1;
+                if (piece_at[mid] <= idx) l = mid; else h = mid;
+            }
+            items[idx] = make_uint2(l, piece_lo[l] + (idx - piece_at[l]));
+        }
+        if (threadIdx.x == 0) *nitems = total;
+        return;
+    }
     for (uint32_t p = p0; p < p1; ++p) {
         uint32_t lo = lo1, hi = hi1;
-        if (p1 - p0 > 1) piece_chunks(m, p, d, ndir, stride, lo, hi);
+        if (p1 - p0 > 1) {
+            piece_ends(m, p, v_first, v_last);
+            piece_chunks(m, v_first, v_last, d, ndir, stride, lo, hi);
+        }
         for (uint32_t c = lo; c <= hi && lo <= hi; ++c, ++at)
             if (at < items_cap) items[at] = make_uint2(p, c);
     }
@@ -309,8 +345,11 @@ struct PrefilterArgs {
 // blocks: block 0 plans the work items; then the row blocks (exact u bounds,
 // mask row offsets, sigmas, valid counts), the mask waves and the residue
 // cache.
+// The u directory lives in dynamic LDS sized to the batch (directory_shape of
+// nu, 8 bytes an entry), so the blocks that never read it stay small.
 __global__ __launch_bounds__(256) void k_prefilter_staged(MatchArgs m, PrefilterArgs pa) {
-    __shared__ uint64_t dir[kRowDirectory];
+    extern __shared__ uint64_t prefilter_dir[];
+    uint64_t* dir = prefilter_dir;
     uint32_t b = blockIdx.x;
     if (b == 0) {
         staged_plan_block(m, pa.npieces, pa.items, pa.items_cap, pa.nitems, dir);
