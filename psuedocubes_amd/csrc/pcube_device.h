@@ -363,7 +363,8 @@ __device__ __forceinline__ void above_directory(const MatchArgs& m, AboveSearch&
 }
 
 // Narrow a search with eight probes of us already loaded (x[k] = us[l + k*step]
-// for the in-range k).
+// for the in-range k).  kTrack keeps hval and val.
+template <bool kTrack>
 __device__ __forceinline__ void above_round(const MatchArgs& m, AboveSearch& s, uint32_t step, const uint64_t (&x)[8]) {
     uint32_t below = 0;
 #pragma unroll
@@ -371,16 +372,18 @@ __device__ __forceinline__ void above_round(const MatchArgs& m, AboveSearch& s, 
         below += s.l + k * step < s.h && !less_wide(s.bound, mul_wide(x[k], m.B)) ? 1u : 0u;
     // x[below] = us[l + below*step] when that probe is in range (below < 8)
     uint64_t xb = 0;
+    if (kTrack) {
 #pragma unroll
-    for (uint32_t k = 0; k < 8; ++k) xb = k == below ? x[k] : xb;
+        for (uint32_t k = 0; k < 8; ++k) xb = k == below ? x[k] : xb;
+    }
     if (step == 1) {   // the last round: the window held at most eight coordinates
-        s.val = s.l + below < s.h ? xb : s.hval;
+        if (kTrack) s.val = s.l + below < s.h ? xb : s.hval;
         s.l += below;
         s.done = true;
         return;
     }
     if (below == 0) {
-        s.val = x[0];
+        if (kTrack) s.val = x[0];
         s.done = true;
         return;
     }
@@ -388,7 +391,7 @@ __device__ __forceinline__ void above_round(const MatchArgs& m, AboveSearch& s, 
     s.l += (below - 1) * step + 1;
     if (top < s.h) {
         s.h = top;
-        s.hval = xb;
+        if (kTrack) s.hval = xb;
     }
 }
 
@@ -396,7 +399,9 @@ __device__ __forceinline__ void above_round(const MatchArgs& m, AboveSearch& s, 
 // row or of a piece).  The searches advance together, so each round's LDS or
 // global probes of both are in flight at the same time: eight probes per
 // search per round, a 63-coordinate window in two rounds of loads and a
-// final round over the last eight.
+// final round over the last eight.  kTrack also returns u0 = us[r0] (when
+// r0 < nu) from the probes, at the cost of the registers that follow it.
+template <bool kTrack>
 __device__ __forceinline__ void first_above_pair(const MatchArgs& m, const U128& b0, const U128& b1,
                                                  const uint64_t* dir, uint32_t ndir, uint32_t stride, uint32_t& r0,
                                                  uint32_t& r1, uint64_t& u0) {
@@ -429,15 +434,17 @@ __device__ __forceinline__ void first_above_pair(const MatchArgs& m, const U128&
         }
 #pragma unroll
         for (int i = 0; i < 2; ++i)
-            if (!s[i].done) above_round(m, s[i], step[i], x[i]);
+            if (!s[i].done) above_round<kTrack>(m, s[i], step[i], x[i]);
     }
     r0 = s[0].l;
     r1 = s[1].l;
-    u0 = s[0].val;   // us[r0] when r0 < nu
+    if (kTrack) u0 = s[0].val;   // us[r0] when r0 < nu
 }
 
 // Exact u bounds of row j (coordinate v = vs[j]) by 128-bit binary search,
-// subtile, v mod q.
+// subtile, v mod q.  With a directory (kDir) the search also yields us[left];
+// without one, us[left] is loaded.
+template <bool kDir>
 __device__ __forceinline__ void build_row(const MatchArgs& m, uint32_t j, uint64_t v, const uint64_t* dir = nullptr,
                                           uint32_t ndir = 0, uint32_t stride = 0) {
     Row row;
@@ -454,10 +461,11 @@ __device__ __forceinline__ void build_row(const MatchArgs& m, uint32_t j, uint64
     const U128 lower = add_wide(shift, m.lo), upper = add_wide(shift, m.hi);
     // first index with u*B > bound, for the lower and the upper bound
     uint32_t left, right;
-    uint64_t u_left;
-    first_above_pair(m, lower, upper, dir, ndir, stride, left, right, u_left);
+    uint64_t u_left = 0;
+    first_above_pair<kDir>(m, lower, upper, dir, ndir, stride, left, right, u_left);
     row.left = left;
     row.right = right;
+    if (!kDir && left < right) u_left = m.us[left];
     if (row.flags) row.right = row.left;  // an invalid row contributes nothing
     // u*B >= v*A holds for every u in range because u*B > v*A + lo; a violation
     // (impossible for sorted input) is reported as the underflow counter.
@@ -491,7 +499,7 @@ __device__ __forceinline__ void build_row(const MatchArgs& m, uint32_t j, uint64
 
 __global__ void k_rows(MatchArgs m) {
     uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < m.nv) build_row(m, j, m.vs[j]);
+    if (j < m.nv) build_row<false>(m, j, m.vs[j]);
 }
 
 // Staged engine rows: as k_rows, plus each subtile's valid-position count,
@@ -540,7 +548,7 @@ __device__ __forceinline__ void staged_rows_block(const MatchArgs& m, uint32_t b
     uint32_t tile = 0xffffffffu;
     uint64_t count = 0;
     if (j < m.nv) {
-        build_row(m, j, v, ndir ? dir : nullptr, ndir, stride);
+        build_row<true>(m, j, v, ndir ? dir : nullptr, ndir, stride);
         const Row& row = m.rows[j];
         if (!row.flags) {
             tile = row.tile;
@@ -998,7 +1006,7 @@ __global__ void k_batch_rows(BatchArgs b) {
         const BatchWindow win = b.win[w];
         const MatchArgs m = window_args(b, win);
         const uint32_t j = uint32_t(g - win.row_begin);
-        build_row(m, j, m.vs[j]);
+        build_row<false>(m, j, m.vs[j]);
     }
 }
 

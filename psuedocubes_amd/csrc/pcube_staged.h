@@ -256,7 +256,7 @@ __device__ __forceinline__ void piece_chunks(const MatchArgs& m, uint64_t v_firs
     const U128 upper = add_wide(mul_wide(v_last, m.A), m.hi);
     uint32_t left0, right1;
     uint64_t unused;
-    first_above_pair(m, lower, upper, dir, ndir, stride, left0, right1, unused);
+    first_above_pair<false>(m, lower, upper, dir, ndir, stride, left0, right1, unused);
     c_lo = 1;
     c_hi = 0;
     if (right1 > left0) {
