@@ -1,0 +1,77 @@
+"""Canonical encoding of receipts (`psuedocubes_amd.accounting`).
+
+The per-subtile output hashes and the lease transcript are SHA-256 over these
+bytes (`pseudocube/accounting.py:17-49`), so the encoder is part of the
+bit-exact contract.  Pinned three ways: against the reference lease's golden
+per-batch scope hashes (scope records computed by the oracle, encoded here),
+against hand-written bytes at the 2^53 boundary, and the fast path for lists
+of small ints against the element-by-element rule on random nested values.
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+from pathlib import Path
+import random
+
+from oracle import oracle as O
+from psuedocubes_amd.accounting import digest, encoded, interoperable
+
+GOLDEN = Path(__file__).parent / 'golden'
+LIMIT = 1 << 53
+
+
+def element_rule(value):
+    """Integers of magnitude >= 2^53 become strings, dict keys become strings,
+    tuples become lists, visiting every element (the reference's rule)."""
+    if type(value) is int and abs(value) >= LIMIT:
+        return str(value)
+    if isinstance(value, dict):
+        return {str(k): element_rule(v) for k, v in value.items()}
+    if isinstance(value, (list, tuple)):
+        return [element_rule(v) for v in value]
+    return value
+
+
+class Tagged(int):
+    pass
+
+
+def random_value(rng: random.Random, depth: int = 0):
+    kind = rng.randrange(10 if depth < 3 else 6)
+    if kind == 0:
+        return rng.choice([0, 1, -1, LIMIT - 1, LIMIT, -LIMIT, -LIMIT + 1, 10**27, -10**30, Tagged(7), Tagged(LIMIT)])
+    if kind == 1:
+        return rng.choice([True, False, None, 'x', 2.5])
+    if kind == 2:   # lists the fast path takes, and near misses it must not
+        return [rng.choice([3, LIMIT - 1, -LIMIT + 1, LIMIT, -LIMIT, True, Tagged(2)]) for _ in range(rng.randrange(5))]
+    if kind == 3:
+        return [rng.randrange(-10**6, 10**6) for _ in range(rng.randrange(30))]
+    if kind in (4, 5):
+        return tuple(random_value(rng, depth + 1) for _ in range(rng.randrange(4)))
+    if kind in (6, 7):
+        return [random_value(rng, depth + 1) for _ in range(rng.randrange(5))]
+    return {rng.choice(['a', 'b', 3, True, None]): random_value(rng, depth + 1) for _ in range(rng.randrange(4))}
+
+
+def test_fast_path_equals_element_rule() -> None:
+    rng = random.Random(619)
+    for _ in range(20000):
+        value = random_value(rng)
+        expected = element_rule(value)
+        got = interoperable(value)
+        assert got == expected
+        assert json.dumps(got, sort_keys=True) == json.dumps(expected, sort_keys=True)
+
+
+def test_boundary_bytes() -> None:
+    value = {'b': [1, LIMIT - 1, LIMIT, -LIMIT, -LIMIT + 1], 'a': True, 'c': (None, 10**20)}
+    assert encoded(value) == (b'{"a":true,"b":[1,9007199254740991,"9007199254740992","-9007199254740992",'
+                              b'-9007199254740991],"c":[null,"100000000000000000000"]}\n')
+    assert digest([]) == hashlib.sha256(b'[]\n').hexdigest()
+
+
+def test_scope_records_hash_to_the_golden_batch(config619, oracle_focuses619) -> None:
+    golden = json.loads((GOLDEN / 'lease_575222931200000000_batches.json').read_text())['batches'][0]
+    row = O.batch(config619, *oracle_focuses619, int(golden['first']), int(golden['end']))
+    assert hashlib.sha256(encoded(row['subtile_scopes'])).hexdigest() == golden['scopes_sha256']
