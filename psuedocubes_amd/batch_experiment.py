@@ -11,6 +11,7 @@ from __future__ import annotations
 
 from dataclasses import dataclass
 import ctypes
+import threading
 import time
 from typing import Any
 
@@ -95,6 +96,33 @@ def prepare_arrays(runtime: Runtime, config: Config, first: int, end: int, width
     return Prepared(first, end, width, us, vs, bounds, time.perf_counter() - started, ranges)
 
 
+class _PrepareScratch:
+    """Host buffers of prepare()'s library call: the geometry, bounds and totals
+    array, pointers into it and the scalar results.  One set per thread, reused
+    by its calls; every value is copied out before prepare() returns."""
+
+    def __init__(self) -> None:
+        self.out = np.empty(8 * 64 + 3, dtype=np.uint64)   # tiles, bounds, totals
+        base = self.out.ctypes.data
+        self.tiles = native.ptr64_at(base)
+        self.bounds = native.ptr64_at(base + 8 * 256)
+        self.totals = native.ptr64_at(base + 8 * 512)
+        self.ntiles = ctypes.c_uint32()
+        self.nu, self.nv, self.launch = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        self.r_ntiles, self.r_nu, self.r_nv, self.r_launch = (
+            ctypes.byref(x) for x in (self.ntiles, self.nu, self.nv, self.launch))
+
+
+_scratch = threading.local()
+
+
+def _prepare_scratch() -> _PrepareScratch:
+    scratch = getattr(_scratch, 'value', None)
+    if scratch is None:
+        scratch = _scratch.value = _PrepareScratch()
+    return scratch
+
+
 def prepare(runtime: Runtime, config: Config, af: GPUFocus, bf: GPUFocus,
             first: int, end: int, width: int = 100000000) -> Prepared:
     """Both focus windows and the subtile bounds of one batch (`pseudocube/batch_experiment.py:51-57`).
@@ -117,30 +145,28 @@ def prepare(runtime: Runtime, config: Config, af: GPUFocus, bf: GPUFocus,
         raise ValueError('experiment allows at most 64 bounded subtiles')
     cap = 1000000
     us, vs = DeviceArray.empty(cap), DeviceArray.empty(cap)
-    out = np.empty(8 * 64 + 3, dtype=np.uint64)   # tiles, bounds, totals
-    ntiles, nu, nv = ctypes.c_uint32(), ctypes.c_uint64(), ctypes.c_uint64()
+    sc = _prepare_scratch()
     runtime.check()
     mask = (1 << 64) - 1
-    base = out.ctypes.data
     args = (af.handle, bf.handle, config.A, config.B, config.lo & mask, config.lo >> 64, config.hi & mask,
-            config.hi >> 64, first, end, width, native.ptr64_at(base), ctypes.byref(ntiles), us.address, cap,
-            vs.address, cap, ctypes.byref(nu), ctypes.byref(nv), native.ptr64_at(base + 8 * 256),
-            native.ptr64_at(base + 8 * 512))
+            config.hi >> 64, first, end, width, sc.tiles, sc.r_ntiles, us.address, cap, vs.address, cap,
+            sc.r_nu, sc.r_nv, sc.bounds, sc.totals)
     matcher = runtime.matcher_for(config)
-    launch = ctypes.c_uint64()
     if matcher is None:
         code = native.lib().pcube_prepare_range(*args)
+        launch = 0
     else:
         code = native.lib().pcube_prepare_match(*args, matcher.engine.handle, OUTPUT_CAP, matcher.engine_id,
-                                                OPPORTUNITY_LIMIT, ctypes.byref(launch))
+                                                OPPORTUNITY_LIMIT, sc.r_launch)
+        launch = sc.launch.value
     if code == native.RANGE:
         raise OverflowError('focus window exceeds bounded period count')
     if code == native.OVERFLOW:
         raise OverflowError('focus buffer overflow; tile invalid')
     native.check(code, 'pcube_prepare_range')
-    us.size, vs.size = nu.value, nv.value
-    n = 4 * ntiles.value
-    flat = out.tolist()
+    us.size, vs.size = sc.nu.value, sc.nv.value
+    n = 4 * sc.ntiles.value
+    flat = sc.out.tolist()
     focus_u, opportunities, largest = flat[512:515]
     if largest > OPPORTUNITY_LIMIT:
         raise OverflowError('per-subtile candidate opportunity limit exceeded')
@@ -151,7 +177,7 @@ def prepare(runtime: Runtime, config: Config, af: GPUFocus, bf: GPUFocus,
     runtime.check()
     return Prepared(first, end, width, us, vs, rows, time.perf_counter() - started, ranges,  # type: ignore[arg-type]
                     (focus_u, opportunities),
-                    (matcher.engine, launch.value, OUTPUT_CAP) if launch.value else None)
+                    (matcher.engine, launch, OUTPUT_CAP) if launch else None)
 
 
 def python_witnesses(values: list[int], p: int) -> list[dict[str, object]]:

@@ -391,6 +391,10 @@ class MatchEngine:
                     config.hi & MASK, config.hi >> 64, native.ptr32(moduli), len(order), ctypes.byref(handle))
         self.handle = handle.value
         self._values: np.ndarray | None = None
+        self._values_ptr: Any = None
+        self._counters = np.empty(0, dtype=np.uint64)   # host copy of the counters, reused
+        self._counters_ptr: Any = None
+        self._seconds = (ctypes.c_double * 3)()
 
     def adopt(self, launch: int, ntiles: int, cap: int) -> bool:
         """Take over the batch that pcube_prepare_match started with token launch, if it is still
@@ -425,15 +429,19 @@ class MatchEngine:
     def finish(self, overflow_ok: bool = False) -> tuple[np.ndarray, list[list[int]], list[float]]:
         ntiles, cap = self._pending
         stride = self.ng + 3
-        counters = np.zeros(ntiles * stride, dtype=np.uint64)
+        if self._counters.size < ntiles * stride:
+            self._counters = np.zeros(ntiles * stride, dtype=np.uint64)
+            self._counters_ptr = native.ptr64(self._counters)
         values_cap = min(max(1, ntiles * cap), 1 << 22)
         if self._values is None or self._values.size < 2 * values_cap:
             self._values = np.empty(2 * values_cap, dtype=np.uint64)
+            self._values_ptr = native.ptr64(self._values)
         values = self._values
-        seconds = (ctypes.c_double * 3)()
-        code = native.lib().pcube_match_finish(self.handle, native.ptr64(counters), native.ptr64(values),
+        seconds = self._seconds
+        code = native.lib().pcube_match_finish(self.handle, self._counters_ptr, self._values_ptr,
                                                values.size // 2, seconds)
-        counters = counters.reshape(ntiles, stride)
+        # the caller owns its counters: a copy of the reused host buffer
+        counters = self._counters[:ntiles * stride].reshape(ntiles, stride).copy()
         if code == native.OVERFLOW and overflow_ok:
             return counters, [[] for _ in range(ntiles)], [seconds[0], seconds[1], seconds[2]]
         native.check(code, 'pcube_match')
