@@ -217,10 +217,10 @@ def batch_row(config: Config, filter_groups: list[list[int]], ranges: list[tuple
     """
     ng = len(filter_groups)
     transfer_started = time.perf_counter()
-    table = counters.tolist()
+    nrows = len(counters)
     # a full p = 619 batch usually emits nothing: skip the per-subtile scan then
     flagged = bool(counters[:, ng + 1:ng + 3].any())
-    emitted = [row[ng + 1] for row in table] if flagged else [0] * len(table)
+    emitted = counters[:, ng + 1].tolist() if flagged else [0] * nrows
     if flagged and counters[:, ng + 2].any():
         raise ArithmeticError('unsigned underflow')
     if emitted and max(emitted) > output_cap:
@@ -239,10 +239,10 @@ def batch_row(config: Config, filter_groups: list[list[int]], ranges: list[tuple
     if any(not row['valid_pseudocube'] for row in hits):
         raise ArithmeticError('independent witness verification failed')
     scopes = skeleton if skeleton is not None else scope_skeleton(ranges)
-    end_group = ng + 1
-    for counts, scope in zip(table, scopes):
-        scope['valid_focused_candidates'] = counts[0]
-        scope['group_survivors'] = counts[1:end_group]
+    # only the columns the records hold, each subtile's group counts already a list
+    for valid, groups_of, scope in zip(counters[:, 0].tolist(), counters[:, 1:ng + 1].tolist(), scopes):
+        scope['valid_focused_candidates'] = valid
+        scope['group_survivors'] = groups_of
     if all_values:
         position = 0
         for (first, end, _, _), number, scope in zip(ranges, emitted, scopes):
@@ -254,7 +254,7 @@ def batch_row(config: Config, filter_groups: list[list[int]], ranges: list[tuple
                 scope['output_sha256'] = digest([str(n) for n in values])
                 position += number
     verification_seconds = time.perf_counter() - verification_started
-    sums = counters[:, :ng + 1].sum(axis=0, dtype=np.uint64).tolist() if table else [0] * (ng + 1)
+    sums = counters[:, :ng + 1].sum(axis=0, dtype=np.uint64).tolist() if nrows else [0] * (ng + 1)
     groups = sums[1:]
     if totals is not None:
         focus_u, opportunities = totals
