@@ -669,6 +669,7 @@ struct Pending {
 
 struct Matcher {
     uint64_t A = 0, B = 0;
+    U128 B_mu{0, 0};    // floor((2^128 - 1) / B)
     U128 lo{0, 0}, hi{0, 0};
     uint32_t nres = 0, nbits = 0, ncache = 0, ng = 0, allow_words = 0;
     uint32_t row_base[kMaxBitsetFilters] = {0, 0, 0, 0};
@@ -821,7 +822,7 @@ int match_enqueue(Matcher& mt, const uint64_t* us, uint64_t nu, const uint64_t* 
     m.list = mt.list.as<uint64_t>();
     m.list_count = mt.list_count();
     m.error = mt.error();
-    m.A = mt.A; m.B = mt.B; m.v_first = v_first; m.tile_width = tile_width; m.cap = cap;
+    m.A = mt.A; m.B = mt.B; m.B_mu = mt.B_mu; m.v_first = v_first; m.tile_width = tile_width; m.cap = cap;
     m.list_cap = list_cap; m.tile_base = tile_base;
     m.lo = mt.lo; m.hi = mt.hi;
     m.nu = (uint32_t)nu; m.nv = (uint32_t)nv; m.words = words;
@@ -1579,8 +1580,11 @@ int pcube_matcher_create(uint64_t A, uint64_t B, uint64_t lo0, uint64_t lo1, uin
     std::lock_guard<std::recursive_mutex> lock(g_mu);
     if (g.device < 0) return fail(PCUBE_INVALID, "pcube_init has not been called");
     if (nres == 0 || nres > (uint32_t)kMaxResiduals) return fail(PCUBE_INVALID, "between 1 and 128 residual filters required");
+    if (B == 0) return fail(PCUBE_INVALID, "B must be positive");
     auto mt = std::make_unique<Matcher>();
     mt->A = A; mt->B = B;
+    const unsigned __int128 mu = ~(unsigned __int128)0 / B;
+    mt->B_mu = U128{(uint64_t)mu, (uint64_t)(mu >> 64)};
     mt->lo = U128{lo0, lo1};
     mt->hi = U128{hi0, hi1};
     mt->nres = nres;
@@ -1900,7 +1904,7 @@ int pcube_batch_create(const int* matchers, uint32_t ncfg, int focus_u, int focu
         e->matchers.push_back(matchers[c]);
     }
     Matcher* m0 = matcher_at(matchers[0]);
-    e->args.A = m0->A; e->args.B = m0->B; e->args.lo = m0->lo; e->args.hi = m0->hi;
+    e->args.A = m0->A; e->args.B = m0->B; e->args.B_mu = m0->B_mu; e->args.lo = m0->lo; e->args.hi = m0->hi;
     e->args.ncfg = ncfg;
     e->focus_u = focus_u; e->focus_v = focus_v;
     e->cap = cap;

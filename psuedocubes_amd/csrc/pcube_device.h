@@ -320,6 +320,7 @@ struct MatchArgs {
     uint64_t* list_count;      // next free triple
     uint32_t* error;           // bit0 invalid tile, bit1 value outside (lo, hi]
     uint64_t A, B, v_first, tile_width, cap, list_cap, tile_base;
+    U128 B_mu;                 // floor((2^128 - 1) / B), for exact divisions by B
     U128 lo, hi;
     uint32_t nu, nv, words, nres, nbits, ncache, ng, ntiles, allow_words, rows_per_wave;
     uint32_t row_base[kStagedFilters];
@@ -337,10 +338,12 @@ struct MatchArgs {
 // State of one search for the first index i in [0, nu) with bound < us[i]*B
 // (nu when there is none): the answer lies in [l, h], and P(i) =
 // (us[i]*B <= bound) holds exactly below it; done once the answer is in l.
+// P is tested as us[i] < key with key = floor(bound / B) + 1, one 64-bit
+// compare per probe instead of a 64 x 64 -> 128 product.
 // hval = us[h] whenever h < nu, and once done, val = us[l] whenever l < nu:
 // every probe that moves h or ends the search has loaded that coordinate.
 struct AboveSearch {
-    U128 bound;
+    AboveKey key;
     uint32_t l, h;
     bool done;
     uint64_t hval, val;
@@ -369,7 +372,7 @@ __device__ __forceinline__ void above_round(const MatchArgs& m, AboveSearch& s, 
     uint32_t below = 0;
 #pragma unroll
     for (uint32_t k = 0; k < 8; ++k)
-        below += s.l + k * step < s.h && !less_wide(s.bound, mul_wide(x[k], m.B)) ? 1u : 0u;
+        below += s.l + k * step < s.h && below_key(x[k], s.key) ? 1u : 0u;
     // x[below] = us[l + below*step] when that probe is in range (below < 8)
     uint64_t xb = 0;
     if (kTrack) {
@@ -405,7 +408,8 @@ template <bool kTrack>
 __device__ __forceinline__ void first_above_pair(const MatchArgs& m, const U128& b0, const U128& b1,
                                                  const uint64_t* dir, uint32_t ndir, uint32_t stride, uint32_t& r0,
                                                  uint32_t& r1, uint64_t& u0) {
-    AboveSearch s[2] = {{b0, 0, m.nu, false, 0, 0}, {b1, 0, m.nu, false, 0, 0}};
+    AboveSearch s[2] = {{above_key(b0, m.B, m.B_mu), 0, m.nu, false, 0, 0},
+                        {above_key(b1, m.B, m.B_mu), 0, m.nu, false, 0, 0}};
     if (dir) {
         uint32_t a[2] = {0, 0}, e[2] = {ndir, ndir};
         while (a[0] < e[0] || a[1] < e[1]) {
@@ -413,7 +417,7 @@ __device__ __forceinline__ void first_above_pair(const MatchArgs& m, const U128&
             for (int i = 0; i < 2; ++i) {
                 if (a[i] < e[i]) {
                     const uint32_t mid = a[i] + ((e[i] - a[i]) >> 1);
-                    if (less_wide(s[i].bound, mul_wide(dir[mid], m.B))) e[i] = mid; else a[i] = mid + 1;
+                    if (!below_key(dir[mid], s[i].key)) e[i] = mid; else a[i] = mid + 1;
                 }
             }
         }
@@ -952,6 +956,7 @@ struct BatchArgs {
     uint32_t* error;
     uint32_t* cursor;          // persistent work-queue cursor
     uint64_t A, B, cap, list_cap;
+    U128 B_mu;                 // floor((2^128 - 1) / B)
     U128 lo, hi;
     uint32_t ncfg;
     BatchConfig cfg[kMaxConfigs];
@@ -977,7 +982,7 @@ __device__ __forceinline__ MatchArgs window_args(const BatchArgs& b, const Batch
     m.mods = c.mods; m.allow = c.allow;
     m.counters = w.counters;
     m.list = b.list; m.list_count = b.list_count; m.error = b.error;
-    m.A = b.A; m.B = b.B; m.v_first = w.v_first; m.tile_width = w.tile_width; m.cap = b.cap;
+    m.A = b.A; m.B = b.B; m.B_mu = b.B_mu; m.v_first = w.v_first; m.tile_width = w.tile_width; m.cap = b.cap;
     m.list_cap = b.list_cap; m.tile_base = w.tile_base;
     m.lo = b.lo; m.hi = b.hi;
     m.nu = w.nu; m.nv = w.nv; m.words = w.words;

@@ -33,6 +33,43 @@ __device__ __forceinline__ bool less_wide(U128 a, U128 b) {
     return a.hi < b.hi || (a.hi == b.hi && a.lo < b.lo);
 }
 
+// floor(x / B) for any x < 2^128, with mu = floor((2^128 - 1) / B) (Barrett).
+// t = floor(x * mu / 2^128), formed exactly from the four 64 x 64 products, is
+// floor(x / B) or one less, so one comparison of the remainder settles it.
+__device__ __forceinline__ U128 floor_div_wide(U128 x, uint64_t B, U128 mu) {
+    const U128 p0 = mul_wide(x.lo, mu.lo), a = mul_wide(x.lo, mu.hi), b = mul_wide(x.hi, mu.lo);
+    const U128 c = mul_wide(x.hi, mu.hi);
+    // the 2^64 column: p0.hi + a.lo + b.lo and its carry into the 2^128 column
+    uint64_t s = p0.hi + a.lo;
+    uint64_t carry = s < a.lo ? 1ull : 0ull;
+    s += b.lo;
+    carry += s < b.lo ? 1ull : 0ull;
+    U128 t = add_wide(add_wide(c, U128{a.hi, 0}), add_wide(U128{b.hi, 0}, U128{carry, 0}));
+    // r = x - t*B (mod 2^128) lies in [0, 2B)
+    U128 tb = mul_wide(t.lo, B);
+    tb.hi += t.hi * B;
+    const U128 r = sub_wide(x, tb);
+    if (r.hi != 0 || r.lo >= B) t = add_wide(t, U128{1, 0});
+    return t;
+}
+
+// The least u with u*B > bound, as a 64-bit key (u >= key exactly when
+// u*B > bound); beyond every 64-bit u when inf.
+struct AboveKey {
+    uint64_t key;
+    bool inf;
+};
+
+__device__ __forceinline__ AboveKey above_key(U128 bound, uint64_t B, U128 mu) {
+    const U128 q = floor_div_wide(bound, B, mu);
+    return AboveKey{q.lo + 1, q.hi != 0 || q.lo == ~0ull};
+}
+
+// u*B <= bound, for the bound the key was made from
+__device__ __forceinline__ bool below_key(uint64_t u, const AboveKey& k) {
+    return k.inf || u < k.key;
+}
+
 // Per-modulus constants for exact reduction.
 //   q     modulus (2 <= q < 2^16)
 //   inv   floor(2^32 / q): Barrett factor, x - q*mulhi(x, inv) lies in [0, 2q)
