@@ -89,8 +89,9 @@ int pcube_focus_window(int focus, uint64_t low, uint64_t high, uint64_t out, uin
 /* Attach the CRT wheel generator to a focus: ring moduli in ascending order,
  * their allowed residue counts and the concatenated (coefficient-adjusted)
  * residues, exactly Focus.rings (pseudocube/focus.py:35-47, 108-114).  The
- * inner ring prefix has modulus M1 < 2^32 and at most 2^21 residues; later
- * windows of this focus are generated as t*M1 + r and need no sort.
+ * inner ring prefix takes every ring that keeps its modulus M1 < 2^40 and at
+ * most 2^27 residues (8 bytes each on the device); later windows of this
+ * focus are generated as t*M1 + r and need no sort.
  * inner_modulus/inner_residues may be NULL. */
 int pcube_focus_wheel(int focus, const uint32_t *moduli, const uint32_t *sizes, const uint32_t *residues,
                       uint32_t nrings, uint64_t *inner_modulus, uint32_t *inner_residues);
@@ -258,6 +259,13 @@ int pcube_host_verify(const uint64_t *limbs, uint64_t n, uint32_t p, uint8_t *lo
 /* --- self test: replaces the `arithmetic` kernel (kernels/pseudocube.cuh:6-11) */
 /* out[3i..3i+2] = (low, high, borrow) of in[4i]*in[4i+1] - in[4i+2]*in[4i+3]; host arrays. */
 int pcube_arithmetic(const uint64_t *in, uint64_t *out, uint64_t count);
+
+/* --- self test: the division behind u_interval (pseudocube/focus.py:120-121) --- */
+/* The row builders find a row's u bounds as first indices with u >= floor((v*A + lo)/B) + 1
+ * and u >= floor((v*A + hi)/B) + 1, dividing 128-bit values by B exactly (Barrett, with
+ * floor((2^128 - 1)/B) precomputed).  out[2i..2i+1] = (low, high) of floor(x / B) for
+ * x = in[3i] + 2^64 in[3i+1] and B = in[3i+2]; host arrays.  INVALID when a B is 0. */
+int pcube_floor_div(const uint64_t *in, uint64_t *out, uint64_t count);
 
 /* --- full-domain control: replaces review/cube_count.c (main 12-38) -------- */
 /* Number of roots r in [first, last] coprime to every prime q <= p, counted

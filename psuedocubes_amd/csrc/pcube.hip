@@ -2333,6 +2333,34 @@ int pcube_copy_probe(uint64_t bytes, double* seconds) {
     return PCUBE_OK;
 }
 
+int pcube_floor_div(const uint64_t* in, uint64_t* out, uint64_t count) {
+    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    if (!count) return PCUBE_OK;
+    std::vector<uint64_t> image(5 * count);
+    for (uint64_t i = 0; i < count; ++i) {
+        const uint64_t B = in[3 * i + 2];
+        if (B == 0) return fail(PCUBE_INVALID, "division by zero in case %llu", (unsigned long long)i);
+        const unsigned __int128 mu = ~(unsigned __int128)0 / B;
+        image[5 * i] = in[3 * i];
+        image[5 * i + 1] = in[3 * i + 1];
+        image[5 * i + 2] = B;
+        image[5 * i + 3] = (uint64_t)mu;
+        image[5 * i + 4] = (uint64_t)(mu >> 64);
+    }
+    void *din = nullptr, *dout = nullptr;
+    TRY(dev_alloc(count * 40, &din));
+    int rc = dev_alloc(count * 16, &dout);
+    if (rc) { dev_free(din); return rc; }
+    HIP_TRY(hipMemcpyAsync(din, image.data(), count * 40, hipMemcpyHostToDevice, g.stream));
+    hipLaunchKernelGGL(k_floor_div, dim3(blocks_for(count, 256)), dim3(256), 0, g.stream, (const uint64_t*)din,
+                       (uint64_t*)dout, count);
+    HIP_TRY(hipMemcpyAsync(out, dout, count * 16, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    dev_free(din);
+    dev_free(dout);
+    return PCUBE_OK;
+}
+
 int pcube_arithmetic(const uint64_t* in, uint64_t* out, uint64_t count) {
     std::lock_guard<std::recursive_mutex> lock(g_mu);
     if (!count) return PCUBE_OK;

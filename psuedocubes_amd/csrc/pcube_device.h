@@ -11,6 +11,7 @@
 //                    group counts, survivor output           pseudocube/batch_experiment.py:198-235
 //   k_match_direct   per-position chain without bitsets      pseudocube/gpu.py:147-181
 //   k_arithmetic     wide-limb self test                     kernels/pseudocube.cuh:6-11
+//   k_floor_div      the u-interval division, self test      pseudocube/focus.py:120-121
 #include "pcube_kernels.h"
 
 namespace pcube {
@@ -1072,6 +1073,16 @@ __global__ void k_copy_probe(const uint64_t* __restrict__ src, uint64_t* __restr
 }
 
 // Wide-limb self test: out[3i..3i+2] = low, high, borrow of in[4i]*in[4i+1] - in[4i+2]*in[4i+3].
+// in[5i..5i+4] = (x.lo, x.hi, B, mu.lo, mu.hi) with mu = floor((2^128 - 1) / B);
+// out[2i..2i+1] = floor(x / B), the division the u-bound searches use.
+__global__ void k_floor_div(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, uint64_t count) {
+    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const U128 q = floor_div_wide(U128{in[5 * i], in[5 * i + 1]}, in[5 * i + 2], U128{in[5 * i + 3], in[5 * i + 4]});
+    out[2 * i] = q.lo;
+    out[2 * i + 1] = q.hi;
+}
+
 __global__ void k_arithmetic(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, uint64_t count) {
     uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (i >= count) return;

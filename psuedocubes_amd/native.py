@@ -59,6 +59,7 @@ SIGNATURES: dict[str, list[Any]] = {
     'pcube_batch_run': [c_int, u32, P_u32, P_u32, P_u64, P_u64, P_u32, u32, P_u64, P_u64, P_u64, P_u64, u64, P_dbl,
                         P_u64],
     'pcube_host_verify': [P_u64, u64, u32, ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(ctypes.c_uint8), P_u64],
+    'pcube_floor_div': [P_u64, P_u64, u64],
     'pcube_arithmetic': [P_u64, P_u64, u64],
     'pcube_copy_probe': [u64, P_dbl],
     'pcube_lds_probe': [u32, u32, P_dbl, P_dbl],

@@ -82,6 +82,28 @@ def test_wide_limb_kernel(runtime) -> None:
         assert int(out[3 * i + 2]) == int(a * b < c * d)
 
 
+def test_floor_div_kernel(runtime) -> None:
+    """The exact 128/64 division behind every row's u bounds (u_interval,
+    pseudocube/focus.py:120-121): Python's integer division on both coefficients,
+    small and maximal divisors, exact multiples and their neighbours."""
+    rng = random.Random(619121)
+    divisors = [B619, A619, 1, 2, 3, 9 * 19 * 37, 2**32 - 1, 2**32 + 1, 2**63, 2**63 + 1, 2**64 - 1]
+    cases = []
+    for B in divisors:
+        cases += [(0, B), (1, B), (B - 1, B), (B, B), (B + 1, B), (2**128 - 1, B), (2**127, B)]
+        cases += [(rng.getrandbits(rng.choice([64, 96, 124, 128])), B) for _ in range(300)]
+        multiples = [B * rng.getrandbits(rng.choice([30, 64, 66])) for _ in range(100)]
+        cases += [(min(2**128 - 1, max(0, k + d)), B) for k in multiples for d in (-1, 0, 1)]
+    flat = np.array([w for x, B in cases for w in (x & (2**64 - 1), x >> 64, B)], dtype=np.uint64)
+    out = np.zeros(2 * len(cases), dtype=np.uint64)
+    native.call('pcube_floor_div', native.ptr64(flat), native.ptr64(out), len(cases))
+    for i, (x, B) in enumerate(cases):
+        assert int(out[2 * i]) + (int(out[2 * i + 1]) << 64) == x // B, (x, B)
+    bad = np.array([5, 0, 0], dtype=np.uint64)
+    with pytest.raises(ValueError):
+        native.call('pcube_floor_div', native.ptr64(bad), native.ptr64(out), 1)
+
+
 def test_focus_halves_equal_oracle(focuses619, oracle_focuses619) -> None:
     for gpu, cpu in zip(focuses619, oracle_focuses619):
         assert np.array_equal(gpu.C.get(), cpu.C)
