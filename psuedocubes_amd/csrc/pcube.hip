@@ -572,15 +572,21 @@ int wheel_enqueue_pair(Wheel& wa, const WheelInputs& ia, uint64_t* out_a, uint64
     void* args[] = {&a, &b};
     HIP_TRY(hipLaunchKernel(wheel_pair_kernel(wa.packed, wa.nouter), dim3(blocks), dim3(kWheelThreads), args, 0,
                             g.stream));
-    const WheelScan sa{wa.wave_count.as<uint32_t>(), ia.nseg, wa.seg_excl.as<uint32_t>(),
-                       wa.chunk_total.as<uint64_t>(), (uint32_t)chunks_a};
-    const WheelScan sb{wb.wave_count.as<uint32_t>(), ib.nseg, wb.seg_excl.as<uint32_t>(),
-                       wb.chunk_total.as<uint64_t>(), (uint32_t)chunks_b};
-    hipLaunchKernelGGL(k_wheel_seg_scan_pair, dim3((uint32_t)(chunks_a + chunks_b)), dim3(256), 0, g.stream, sa, sb);
+    // a batch's windows have a few hundred segments each: the compaction sums
+    // the earlier segments' counts itself and the scan pass is not launched
+    const uint32_t direct = ia.nseg <= kDirectPrefixSegments && ib.nseg <= kDirectPrefixSegments ? 1u : 0u;
+    if (!direct) {
+        const WheelScan sa{wa.wave_count.as<uint32_t>(), ia.nseg, wa.seg_excl.as<uint32_t>(),
+                           wa.chunk_total.as<uint64_t>(), (uint32_t)chunks_a};
+        const WheelScan sb{wb.wave_count.as<uint32_t>(), ib.nseg, wb.seg_excl.as<uint32_t>(),
+                           wb.chunk_total.as<uint64_t>(), (uint32_t)chunks_b};
+        hipLaunchKernelGGL(k_wheel_seg_scan_pair, dim3((uint32_t)(chunks_a + chunks_b)), dim3(256), 0, g.stream, sa,
+                           sb);
+    }
     const WheelOut oa{wa.seg_excl.as<uint32_t>(), wa.chunk_total.as<uint64_t>(), out_a, cap_a,
-                      wa.win_info.as<uint64_t>()};
+                      wa.win_info.as<uint64_t>(), direct};
     const WheelOut ob{wb.seg_excl.as<uint32_t>(), wb.chunk_total.as<uint64_t>(), out_b, cap_b,
-                      wb.win_info.as<uint64_t>()};
+                      wb.win_info.as<uint64_t>(), direct};
     hipLaunchKernelGGL(k_wheel_compact_pair, dim3(blocks), dim3(kWheelThreads), 0, g.stream, a, oa, b, ob);
     return PCUBE_OK;
 }

@@ -17,8 +17,9 @@
 // a block handles one segment, each of its four waves 1024 candidates.  One
 // test pass writes each wave's survivors (their inner residues r) into a slot
 // of slot_cap entries and its count; a scan over segments gives each segment's
-// output offset; a compaction pass copies the slots to the dense, ascending
-// output and writes every window's offset.  A wave with more survivors than
+// output offset (a paired launch with few segments skips it: each compaction
+// block sums its predecessors' counts); a compaction pass copies the slots to
+// the dense, ascending output and writes every window's offset.  A wave with more survivors than
 // slot_cap raises the overflow flag and the host runs again with
 // slot_cap = 1024, which cannot overflow.
 //
@@ -250,26 +251,34 @@ __global__ __launch_bounds__(256) void k_wheel_seg_scan_pair(WheelScan a, WheelS
 
 // Compaction: one block per segment.  Output offset of the segment = the
 // chunk totals before its chunk + its exclusive offset within the chunk; each
-// wave copies its slot to out, dropping positions at or beyond cap.  The
-// first segment of each window writes that window's offset; the last
-// segment writes the total into every later entry of win_info, and
-// win_info[nwin + 1] receives the overflow flag.
+// wave copies its slot to out, dropping positions at or beyond cap.  With
+// direct (few segments, no scan pass), the block instead sums the survivor
+// counts of every earlier segment itself.  The first segment of each window
+// writes that window's offset; the last segment writes the total into every
+// later entry of win_info, and win_info[nwin + 1] receives the overflow flag.
 __device__ __forceinline__ void wheel_compact_block(const WheelArgs& a, const uint32_t* __restrict__ seg_excl,
                                                     const uint64_t* __restrict__ chunk_total,
                                                     uint64_t* __restrict__ out, uint64_t cap,
-                                                    uint64_t* __restrict__ win_info, uint64_t seg) {
+                                                    uint64_t* __restrict__ win_info, uint64_t seg, bool direct) {
     __shared__ uint64_t partial[kWheelThreads / kWave];
     __shared__ uint32_t s_window;
     const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    const uint64_t chunk = seg / kWheelScanChunk;
     uint64_t acc = 0;
-    for (uint64_t k = threadIdx.x; k < chunk; k += kWheelThreads) acc += chunk_total[k];
+    if (direct) {
+        for (uint64_t k = threadIdx.x; k < seg; k += kWheelThreads) {
+            const uint4 c = *(const uint4*)(a.wave_count + k * kWheelWaves);
+            acc += (uint64_t)c.x + c.y + c.z + c.w;
+        }
+    } else {
+        const uint64_t chunk = seg / kWheelScanChunk;
+        for (uint64_t k = threadIdx.x; k < chunk; k += kWheelThreads) acc += chunk_total[k];
+    }
 #pragma unroll
     for (int d = kWave / 2; d >= 1; d >>= 1) acc += __shfl_xor(acc, d, kWave);
     if (lane == 0) partial[wave] = acc;
     if (threadIdx.x == 0) s_window = wheel_window_of(a, seg);
     __syncthreads();
-    const uint64_t prefix = partial[0] + partial[1] + partial[2] + partial[3] + seg_excl[seg];
+    const uint64_t prefix = partial[0] + partial[1] + partial[2] + partial[3] + (direct ? 0u : seg_excl[seg]);
     const uint4 counts = *(const uint4*)(a.wave_count + seg * kWheelWaves);
     const uint32_t cw[4] = {counts.x, counts.y, counts.z, counts.w};
     uint64_t base = prefix;
@@ -299,7 +308,7 @@ __global__ __launch_bounds__(kWheelThreads) void k_wheel_compact(WheelArgs a, co
                                                                  const uint64_t* __restrict__ chunk_total,
                                                                  uint64_t* __restrict__ out, uint64_t cap,
                                                                  uint64_t* __restrict__ win_info) {
-    wheel_compact_block(a, seg_excl, chunk_total, out, cap, win_info, blockIdx.x);
+    wheel_compact_block(a, seg_excl, chunk_total, out, cap, win_info, blockIdx.x, false);
 }
 
 // One wheel's compaction outputs.
@@ -309,7 +318,12 @@ struct WheelOut {
     uint64_t* out;
     uint64_t cap;
     uint64_t* win_info;
+    uint32_t direct;   // offsets summed by the compaction itself (no scan pass)
 };
+
+// Segments per wheel up to which a paired launch skips the scan pass and each
+// compaction block sums the earlier segments' counts (at most 8 loads a thread).
+constexpr uint64_t kDirectPrefixSegments = 2048;
 
 // Two wheels' compactions in one launch: blocks [0, a.nseg) compact a.
 __global__ __launch_bounds__(kWheelThreads) void k_wheel_compact_pair(WheelArgs a, WheelOut oa, WheelArgs b,
@@ -317,7 +331,7 @@ __global__ __launch_bounds__(kWheelThreads) void k_wheel_compact_pair(WheelArgs 
     const bool second = blockIdx.x >= a.nseg;
     const WheelOut& o = second ? ob : oa;
     wheel_compact_block(second ? b : a, o.seg_excl, o.chunk_total, o.out, o.cap, o.win_info,
-                        second ? blockIdx.x - a.nseg : blockIdx.x);
+                        second ? blockIdx.x - a.nseg : blockIdx.x, o.direct != 0);
 }
 
 #define PCUBE_WHEEL(K)                                              \
