@@ -266,9 +266,10 @@ struct Wheel {
     Modulus outer[kMaxOuterRings];
     double rate = 1.0;      // expected fraction of candidates that pass the outer rings
     Buffer R1, allow, slots, wave_count, seg_excl, chunk_total, win_info;
+    Buffer flag;            // overflow flag of one-window launches (zero except while a retry is due)
     void release() {
         R1.release(); allow.release(); slots.release(); wave_count.release(); seg_excl.release();
-        chunk_total.release(); win_info.release();
+        chunk_total.release(); win_info.release(); flag.release();
     }
 };
 
@@ -486,14 +487,15 @@ void wheel_image(const std::vector<WheelWindow>& plan, const std::vector<uint64_
 }
 
 struct WheelInputs {
-    const WheelWindow* windows;
-    const uint64_t* seg_first;
+    const WheelWindow* windows;   // null: one window, win0
+    const uint64_t* seg_first;    // null with win0
     uint32_t* flag;
     uint32_t n;
     uint64_t nseg;
+    WheelWindow win0;
 };
 WheelInputs wheel_inputs(uint64_t* img_dev, uint32_t n, uint64_t nseg) {
-    WheelInputs in;
+    WheelInputs in{};
     in.windows = (const WheelWindow*)img_dev;
     in.seg_first = img_dev + (uint64_t)n * (sizeof(WheelWindow) / 8);
     in.flag = (uint32_t*)(img_dev + wheel_image_words(n) - 1);
@@ -531,6 +533,7 @@ int wheel_setup(Wheel& wh, const WheelInputs& in, uint32_t slot_cap, WheelArgs& 
     a.n1 = wh.n1; a.nwin = in.n; a.nouter = wh.nouter; a.allow_words = wh.allow_words;
     a.packed = wh.packed; a.slot_cap = slot_cap;
     for (uint32_t j = 0; j < wh.nouter; ++j) a.outer[j] = wh.outer[j];
+    a.win0 = in.win0;
     return PCUBE_OK;
 }
 
@@ -964,11 +967,18 @@ __global__ void k_searchsorted_seg(const uint64_t* __restrict__ data, const uint
 // remaining range [l, h) and a ballot cuts the range 64-fold, so a million
 // entries take four dependent loads.  Invariant: the first index with
 // a[i] >= k lies in [l, h] (h = the size when every entry is smaller).
+// The queries of a prepared batch, passed with the kernel arguments: (u_min, u_end)
+// per subtile, then (v_first, v_end) per subtile, at most 64 subtiles.
+constexpr uint32_t kMaxPrepareTiles = 64;
+struct PrepareQueries {
+    uint64_t q[4 * kMaxPrepareTiles];
+};
+
 __global__ __launch_bounds__(256) void k_prepare_bounds(const uint64_t* __restrict__ us,
                                                         const uint64_t* __restrict__ uinfo, uint64_t us_cap,
                                                         const uint64_t* __restrict__ vs,
                                                         const uint64_t* __restrict__ vinfo, uint64_t vs_cap,
-                                                        const uint64_t* __restrict__ queries, uint32_t ntiles,
+                                                        const PrepareQueries queries, uint32_t ntiles,
                                                         uint64_t* __restrict__ out) {
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint64_t q = blockIdx.x * (uint64_t)(blockDim.x / kWave) + threadIdx.x / kWave;
@@ -983,7 +993,7 @@ __global__ __launch_bounds__(256) void k_prepare_bounds(const uint64_t* __restri
     const bool on_u = q < 2 * (uint64_t)ntiles;
     const uint64_t* a = on_u ? us : vs;
     uint64_t l = 0, h = on_u ? min(uinfo[1], us_cap) : min(vinfo[1], vs_cap);
-    const uint64_t k = queries[q];
+    const uint64_t k = queries.q[q];
     while (l < h) {
         const uint64_t step = (h - l + kWave - 1) / kWave;
         const uint64_t at = l + lane * step;
@@ -1393,6 +1403,8 @@ int pcube_focus_wheel(int focus, const uint32_t* moduli, const uint32_t* sizes, 
         entries[i] = wh.R1host[i] | (wh.packed ? (uint64_t)bytes[i] << kResidueBits : 0ull);
     TRY(wh.R1.ensure(std::max<uint64_t>(8, (uint64_t)wh.n1 * 8)));
     TRY(wh.allow.ensure(bits.size() * 4));
+    TRY(wh.flag.ensure(4));
+    TRY(fill32(wh.flag.as<uint32_t>(), 1, 0));
     HIP_TRY(hipMemcpyAsync(wh.R1.p, entries.data(), (uint64_t)wh.n1 * 8, hipMemcpyHostToDevice, g.stream));
     HIP_TRY(hipMemcpyAsync(wh.allow.p, bits.data(), bits.size() * 4, hipMemcpyHostToDevice, g.stream));
     HIP_TRY(hipStreamSynchronize(g.stream));
@@ -1419,8 +1431,8 @@ int pcube_prepare(int focus_u, int focus_v, uint32_t ntiles, const uint64_t* til
     std::lock_guard<std::recursive_mutex> lock(g_mu);
     Focus* fu = focus_at(focus_u);
     Focus* fv = focus_at(focus_v);
-    if (!fu || !fv || fu == fv || !fu->wheel.ready || !fv->wheel.ready || ntiles == 0)
-        return fail(PCUBE_INVALID, "prepare needs the two focus wheels and at least one subtile");
+    if (!fu || !fv || fu == fv || !fu->wheel.ready || !fv->wheel.ready || ntiles == 0 || ntiles > kMaxPrepareTiles)
+        return fail(PCUBE_INVALID, "prepare needs the two focus wheels and 1 to 64 subtiles");
     const uint64_t ulo = tiles[2], uhi = tiles[4 * (ntiles - 1) + 3];
     const uint64_t vlo = tiles[0], vhi = tiles[4 * (ntiles - 1) + 1];
     if (ulo > uhi || vlo > vhi) return fail(PCUBE_INVALID, "invalid batch geometry");
@@ -1430,24 +1442,26 @@ int pcube_prepare(int focus_u, int focus_v, uint32_t ntiles, const uint64_t* til
     const uint64_t nseg_v = wheel_plan(fv->wheel, 1, &vlo, &vhi, pv, sv);
     TRY(wheel_check(nseg_u));
     TRY(wheel_check(nseg_v));
-    // one upload: subtile queries ((u_min, u_end) per subtile, then (v_first, v_end)), both wheel images
-    const uint64_t nq = 4 * (uint64_t)ntiles, iw = wheel_image_words(1);
-    const uint64_t up_words = nq + 2 * iw, down_words = nq + 4;
-    TRY(g_up_host.ensure(up_words * 8));
+    // nothing is uploaded: the subtile queries ((u_min, u_end) per subtile, then
+    // (v_first, v_end)) and both one-window plans travel with the kernel arguments
+    const uint64_t nq = 4 * (uint64_t)ntiles, down_words = nq + 4;
     TRY(g_result_host.ensure(down_words * 8));
-    TRY(g_up_dev.ensure(up_words * 8));
-    uint64_t* h = g_up_host.as<uint64_t>();
+    PrepareQueries queries;
     for (uint32_t t = 0; t < ntiles; ++t) {
-        h[2 * t] = tiles[4 * t + 2];
-        h[2 * t + 1] = tiles[4 * t + 3];
-        h[2 * ntiles + 2 * t] = tiles[4 * t];
-        h[2 * ntiles + 2 * t + 1] = tiles[4 * t + 1];
+        queries.q[2 * t] = tiles[4 * t + 2];
+        queries.q[2 * t + 1] = tiles[4 * t + 3];
+        queries.q[2 * ntiles + 2 * t] = tiles[4 * t];
+        queries.q[2 * ntiles + 2 * t + 1] = tiles[4 * t + 1];
     }
-    wheel_image(pu, su, h + nq);
-    wheel_image(pv, sv, h + nq + iw);
-    HIP_TRY(hipMemcpyAsync(g_up_dev.p, g_up_host.p, up_words * 8, hipMemcpyHostToDevice, g.stream));
-    uint64_t* d = g_up_dev.as<uint64_t>();
-    const WheelInputs in_u = wheel_inputs(d + nq, 1, nseg_u), in_v = wheel_inputs(d + nq + iw, 1, nseg_v);
+    WheelInputs in_u{}, in_v{};
+    in_u.flag = fu->wheel.flag.as<uint32_t>();
+    in_u.n = 1;
+    in_u.nseg = nseg_u;
+    in_u.win0 = pu[0];
+    in_v.flag = fv->wheel.flag.as<uint32_t>();
+    in_v.n = 1;
+    in_v.nseg = nseg_v;
+    in_v.win0 = pv[0];
     uint32_t cap_u = wheel_slot_cap(fu->wheel), cap_v = wheel_slot_cap(fv->wheel);
     // the bounds kernel writes its results straight into coherent pinned memory
     const uint64_t* r = g_result_host.as<uint64_t>();
@@ -1457,7 +1471,7 @@ int pcube_prepare(int focus_u, int focus_v, uint32_t ntiles, const uint64_t* til
         hipLaunchKernelGGL(k_prepare_bounds, dim3(blocks_for(nq, 256 / kWave)), dim3(256), 0, g.stream,
                            (const uint64_t*)us_out, (const uint64_t*)fu->wheel.win_info.as<uint64_t>(), us_cap,
                            (const uint64_t*)vs_out, (const uint64_t*)fv->wheel.win_info.as<uint64_t>(), vs_cap,
-                           (const uint64_t*)d, ntiles, g_result_host.as<uint64_t>());
+                           queries, ntiles, g_result_host.as<uint64_t>());
         HIP_TRY(hipStreamSynchronize(g.stream));
         const bool again_u = r[nq + 2] && cap_u < (uint32_t)kWheelWaveSpan;
         const bool again_v = r[nq + 3] && cap_v < (uint32_t)kWheelWaveSpan;

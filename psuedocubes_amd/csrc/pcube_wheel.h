@@ -55,8 +55,8 @@ struct WheelWindow {
 
 struct WheelArgs {
     const uint64_t* R1;          // sorted inner residues r, outer residues packed above them
-    const WheelWindow* windows;  // nwin windows
-    const uint64_t* seg_first;   // nwin + 1 prefix of segment counts
+    const WheelWindow* windows;  // nwin windows; null for one window carried in win0
+    const uint64_t* seg_first;   // nwin + 1 prefix of segment counts; null with win0
     const uint32_t* allow;       // outer-ring allowed bits, rings at Modulus::bit
     uint64_t* slots;             // slot_cap inner residues per (segment, wave)
     uint32_t* wave_count;        // survivors per (segment, wave)
@@ -64,14 +64,25 @@ struct WheelArgs {
     uint64_t M1, nseg;
     uint32_t n1, nwin, nouter, allow_words, packed, slot_cap;
     Modulus outer[kMaxOuterRings];   // q, inv, r32 for reduction; b = M1 mod q; bit offset
+    WheelWindow win0;                // the plan of a one-window launch, passed with the arguments
 };
+
+// Plan of window i: from the arguments for a one-window launch, else from memory.
+__device__ __forceinline__ WheelWindow wheel_window(const WheelArgs& a, uint32_t i) {
+    return a.windows ? a.windows[i] : a.win0;
+}
+
+// First segment of window v (v <= nwin); a one-window launch has 0 and nseg.
+__device__ __forceinline__ uint64_t wheel_seg_first(const WheelArgs& a, uint32_t v) {
+    return a.seg_first ? a.seg_first[v] : (v == 0 ? 0ull : a.nseg);
+}
 
 // Window of a segment: the last window whose first segment is <= seg.
 __device__ __forceinline__ uint32_t wheel_window_of(const WheelArgs& a, uint64_t seg) {
     uint32_t l = 0, h = a.nwin;
     while (h - l > 1) {
         const uint32_t m = (l + h) >> 1;
-        if (a.seg_first[m] <= seg) l = m; else h = m;
+        if (wheel_seg_first(a, m) <= seg) l = m; else h = m;
     }
     return l;
 }
@@ -112,7 +123,7 @@ __device__ __forceinline__ void wheel_slots_block(const WheelArgs& a, uint64_t s
     for (uint32_t t = threadIdx.x; t < a.allow_words; t += kWheelThreads) allow[t] = a.allow[t];
     if (threadIdx.x == 0) s_window = wheel_window_of(a, seg);
     __syncthreads();
-    const WheelWindow w = a.windows[s_window];
+    const WheelWindow w = wheel_window(a, s_window);
     uint64_t t;
     uint32_t begin, end;
     wheel_segment(w, a.n1, uint32_t(seg - w.seg_first), t, begin, end);  // segments of one window fit 32 bits
@@ -175,10 +186,12 @@ __global__ __launch_bounds__(kWheelThreads) void k_wheel_slots(WheelArgs a) {
 // Two wheels' test passes in one launch (the u and v windows of a batch):
 // blocks [0, a.nseg) take a's segments, the rest b's, so the two latency
 // chains overlap.  Both wheels use the same instantiation.
+// (one call per branch: a reference chosen at run time between the two argument
+// structs would be copied to scratch memory)
 template <int kOuter>
 __global__ __launch_bounds__(kWheelThreads) void k_wheel_slots_pair(WheelArgs a, WheelArgs b) {
-    const bool second = blockIdx.x >= a.nseg;
-    wheel_slots_block<kOuter>(second ? b : a, second ? blockIdx.x - a.nseg : blockIdx.x);
+    if (blockIdx.x >= a.nseg) wheel_slots_block<kOuter>(b, blockIdx.x - a.nseg);
+    else wheel_slots_block<kOuter>(a, blockIdx.x);
 }
 
 // Inclusive scan over the 64 lanes of a wave.
@@ -283,7 +296,7 @@ __device__ __forceinline__ void wheel_compact_block(const WheelArgs& a, const ui
     const uint32_t cw[4] = {counts.x, counts.y, counts.z, counts.w};
     uint64_t base = prefix;
     for (uint32_t k = 0; k < wave; ++k) base += cw[k];
-    const WheelWindow w = a.windows[s_window];
+    const WheelWindow w = wheel_window(a, s_window);
     uint64_t t;
     uint32_t begin, end;
     wheel_segment(w, a.n1, uint32_t(seg - w.seg_first), t, begin, end);
@@ -295,7 +308,7 @@ __device__ __forceinline__ void wheel_compact_block(const WheelArgs& a, const ui
         if (o < cap) out[o] = ybase + slot[i];
     }
     if (threadIdx.x == 0) {
-        for (int64_t v = s_window; v >= 0 && a.seg_first[v] == seg; --v) win_info[v] = prefix;
+        for (int64_t v = s_window; v >= 0 && wheel_seg_first(a, (uint32_t)v) == seg; --v) win_info[v] = prefix;
         if (seg + 1 == a.nseg) {
             const uint64_t total = prefix + cw[0] + cw[1] + cw[2] + cw[3];
             for (uint32_t v = s_window + 1; v <= a.nwin; ++v) win_info[v] = total;
@@ -328,10 +341,11 @@ constexpr uint64_t kDirectPrefixSegments = 2048;
 // Two wheels' compactions in one launch: blocks [0, a.nseg) compact a.
 __global__ __launch_bounds__(kWheelThreads) void k_wheel_compact_pair(WheelArgs a, WheelOut oa, WheelArgs b,
                                                                       WheelOut ob) {
-    const bool second = blockIdx.x >= a.nseg;
-    const WheelOut& o = second ? ob : oa;
-    wheel_compact_block(second ? b : a, o.seg_excl, o.chunk_total, o.out, o.cap, o.win_info,
-                        second ? blockIdx.x - a.nseg : blockIdx.x, o.direct != 0);
+    if (blockIdx.x >= a.nseg)
+        wheel_compact_block(b, ob.seg_excl, ob.chunk_total, ob.out, ob.cap, ob.win_info, blockIdx.x - a.nseg,
+                            ob.direct != 0);
+    else
+        wheel_compact_block(a, oa.seg_excl, oa.chunk_total, oa.out, oa.cap, oa.win_info, blockIdx.x, oa.direct != 0);
 }
 
 #define PCUBE_WHEEL(K)                                              \
