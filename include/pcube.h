@@ -108,7 +108,8 @@ int pcube_focus_windows(int focus, uint32_t n, const uint64_t *lows, const uint6
  * the u window [u_min of the first subtile, u_end of the last) into us_out,
  * the v window [v_first of the first, v_end of the last) into vs_out, and
  * the subtile bounds (a, b, c, d) = searchsorted of (u_min, u_end) in us and
- * (v_first, v_end) in vs.  tiles: ntiles x (v_first, v_end, u_min, u_end).
+ * (v_first, v_end) in vs.  tiles: ntiles x (v_first, v_end, u_min, u_end),
+ * 1 <= ntiles <= 64 (INVALID otherwise).
  * OVERFLOW (nu and nv set, window contents unspecified) when a window exceeds its cap. */
 int pcube_prepare(int focus_u, int focus_v, uint32_t ntiles, const uint64_t *tiles, uint64_t us_out, uint64_t us_cap,
                   uint64_t vs_out, uint64_t vs_cap, uint64_t *nu, uint64_t *nv, uint64_t *bounds);
