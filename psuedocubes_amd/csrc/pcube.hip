@@ -901,7 +901,7 @@ int match_launch(Matcher& mt) {
 }
 
 // Copy the tagged value list to the host grouped by tile, in tile order.
-int collect_values(Matcher& mt, uint64_t ntiles_total, const std::vector<uint64_t>& host_counters, uint64_t cap,
+int collect_values(Matcher& mt, uint64_t ntiles_total, const uint64_t* host_counters, uint64_t cap,
                    uint64_t list_cap, uint64_t* values, uint64_t values_cap, int* status, uint64_t listed) {
     uint64_t expected = 0;
     std::vector<uint64_t> offset(ntiles_total + 1, 0);
@@ -934,7 +934,7 @@ int collect_values(Matcher& mt, uint64_t ntiles_total, const std::vector<uint64_
 }
 
 // Translate counters and error bits into the status code.
-int batch_status(const Matcher& mt, const std::vector<uint64_t>& c, uint64_t ntiles, uint64_t cap, uint32_t err) {
+int batch_status(const Matcher& mt, const uint64_t* c, uint64_t ntiles, uint64_t cap, uint32_t err) {
     if (err & 1u) return fail(PCUBE_INVALID, "a v coordinate lies outside the subtiles");
     for (uint64_t t = 0; t < ntiles; ++t)
         if (c[t * (mt.ng + 3) + mt.ng + 2]) return fail(PCUBE_UNDERFLOW, "unsigned underflow flagged");
@@ -1736,7 +1736,9 @@ int pcube_match_finish(int matcher, uint64_t* counters, uint64_t* values, uint64
     }
     const uint64_t listed = hd[0];
     const uint32_t err = (uint32_t)hd[1];
-    std::vector<uint64_t> host(hd + 4, hd + 4 + nc);
+    // the counters are read where the download left them: the pinned buffer is
+    // rewritten only by this matcher's next launch
+    const uint64_t* host = hd + 4;
     if (seconds) {
         float a = 0, b = 0, c = 0;
         HIP_TRY(hipEventElapsedTime(&a, mt->ev[0], mt->ev[1]));
@@ -1747,7 +1749,7 @@ int pcube_match_finish(int matcher, uint64_t* counters, uint64_t* values, uint64
         seconds[1] = b / 1000.0;
         seconds[2] = c / 1000.0;
     }
-    memcpy(counters, host.data(), nc * 8);
+    memcpy(counters, host, nc * 8);
     int rc = batch_status(*mt, host, pd.ntiles, pd.cap, err);
     int status = PCUBE_OK;
     if (!rc) rc = collect_values(*mt, pd.ntiles, host, pd.cap, pd.list_cap, values, values_cap, &status, listed);
@@ -1886,10 +1888,10 @@ int pcube_lease(int matcher, int focus_u, int focus_v, uint32_t nbatches, const 
     for (auto& e : ev) (void)hipEventDestroy(e);
     lus.release(); lvs.release(); queries.release(); positions.release();
     if (geometry_rc) return geometry_rc;
-    int rc = batch_status(*mt, host, ntot, cap, err);
+    int rc = batch_status(*mt, host.data(), ntot, cap, err);
     if (rc) return rc;
     int status = PCUBE_OK;
-    TRY(collect_values(*mt, ntot, host, cap, list_cap, values, values_cap, &status, listed));
+    TRY(collect_values(*mt, ntot, host.data(), cap, list_cap, values, values_cap, &status, listed));
     if (seconds) seconds[3] = now_seconds() - t_start;
     return status;
 }
