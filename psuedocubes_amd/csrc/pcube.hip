@@ -229,7 +229,7 @@ int bucket_sort(const uint64_t* in, uint64_t n, uint64_t low, uint64_t span, uin
 // ---------------------------------------------------------------------------
 // Focus objects
 
-// CRT wheel of a focus: inner ring prefix with modulus M1 < 2^32 and its
+// CRT wheel of a focus: inner ring prefix with modulus M1 < 2^40 and its
 // sorted residues R1, outer rings tested per candidate (pcube_wheel.h).
 // Pinned host staging for the small uploads and downloads of one call (plans,
 // queries, counters).  Every entry point that uses it synchronises before it

@@ -19,9 +19,9 @@
 // of slot_cap entries and its count; a scan over segments gives each segment's
 // output offset (a paired launch with few segments skips it: each compaction
 // block sums its predecessors' counts); a compaction pass copies the slots to
-// the dense, ascending output and writes every window's offset.  A wave with more survivors than
-// slot_cap raises the overflow flag and the host runs again with
-// slot_cap = 1024, which cannot overflow.
+// the dense, ascending output and writes every window's offset.  A wave with
+// more survivors than slot_cap raises the overflow flag and the host runs
+// again with slot_cap = 1024, which cannot overflow.
 //
 // Replaces the role of GPUFocus.window (pseudocube/gpu.py:19-35, 94-114):
 // same coordinate set, same ascending order.
