@@ -261,6 +261,21 @@ int pcube_host_verify(const uint64_t *limbs, uint64_t n, uint32_t p, uint8_t *lo
 /* out[3i..3i+2] = (low, high, borrow) of in[4i]*in[4i+1] - in[4i+2]*in[4i+3]; host arrays. */
 int pcube_arithmetic(const uint64_t *in, uint64_t *out, uint64_t count);
 
+/* --- receipts: the lease transcript's bytes (pseudocube/optimized.py:26-78) --- */
+
+/* The canonical encoding (pseudocube/accounting.py:17-31: sorted keys, no
+ * spaces, integers >= 2^53 as decimal strings, a final newline) of one batch's
+ * subtile_scopes list, whose records hold end and first as decimal strings,
+ * group_survivors, local_survivors, output_sha256 and valid_focused_candidates
+ * (pseudocube/batch_experiment.py:149-158).  Subtile k: bounds[2k..2k+1] =
+ * (first, end); counters + k*stride = [valid, group_1 .. group_ng, ...];
+ * local[k] its local survivors (local NULL: all 0); sha + 64k its 64 hex
+ * digits.  Writes at most cap bytes to out, the length to *length; OVERFLOW
+ * when they do not fit.  Host only. */
+int pcube_scope_records(uint32_t ntiles, const uint64_t *bounds, const uint64_t *counters, uint32_t stride,
+                        uint32_t ng, const uint64_t *local, const char *sha, char *out, uint64_t cap,
+                        uint64_t *length);
+
 /* --- self test: the division behind u_interval (pseudocube/focus.py:120-121) --- */
 /* The row builders find a row's u bounds as first indices with u >= floor((v*A + lo)/B) + 1
  * and u >= floor((v*A + hi)/B) + 1, dividing 128-bit values by B exactly (Barrett, with

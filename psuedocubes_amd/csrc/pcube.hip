@@ -2355,6 +2355,71 @@ int pcube_copy_probe(uint64_t bytes, double* seconds) {
     return PCUBE_OK;
 }
 
+// Text writer for pcube_scope_records: fails once cap would be exceeded.
+struct TextOut {
+    char* p;
+    uint64_t n = 0, cap;
+    bool ok = true;
+    void put(const char* s, size_t len) {
+        if (!ok || n + len > cap) {
+            ok = false;
+            return;
+        }
+        memcpy(p + n, s, len);
+        n += len;
+    }
+    void put(const char* s) { put(s, strlen(s)); }
+    void digits(uint64_t v) {
+        char buf[24];
+        int i = 24;
+        do {
+            buf[--i] = char('0' + v % 10);
+            v /= 10;
+        } while (v);
+        put(buf + i, (size_t)(24 - i));
+    }
+    // a JSON integer, or its decimal string from 2^53 on (the encoder's rule)
+    void integer(uint64_t v) {
+        const bool quoted = v >= (1ull << 53);
+        if (quoted) put("\"", 1);
+        digits(v);
+        if (quoted) put("\"", 1);
+    }
+};
+
+int pcube_scope_records(uint32_t ntiles, const uint64_t* bounds, const uint64_t* counters, uint32_t stride,
+                        uint32_t ng, const uint64_t* local, const char* sha, char* out, uint64_t cap,
+                        uint64_t* length) {
+    if (!bounds && ntiles) return fail(PCUBE_INVALID, "null subtile bounds");
+    if ((!counters || !sha) && ntiles) return fail(PCUBE_INVALID, "null counters or digests");
+    if (stride < ng + 1 || !out || !length) return fail(PCUBE_INVALID, "invalid record layout");
+    TextOut t{out, 0, cap};
+    t.put("[", 1);
+    for (uint32_t k = 0; k < ntiles; ++k) {
+        const uint64_t* c = counters + (uint64_t)k * stride;
+        t.put(k ? ",{\"end\":\"" : "{\"end\":\"");
+        t.digits(bounds[2 * k + 1]);
+        t.put("\",\"first\":\"");
+        t.digits(bounds[2 * k]);
+        t.put("\",\"group_survivors\":[");
+        for (uint32_t g = 0; g < ng; ++g) {
+            if (g) t.put(",", 1);
+            t.integer(c[1 + g]);
+        }
+        t.put("],\"local_survivors\":");
+        t.integer(local ? local[k] : 0);
+        t.put(",\"output_sha256\":\"");
+        t.put(sha + 64 * (uint64_t)k, 64);
+        t.put("\",\"valid_focused_candidates\":");
+        t.integer(c[0]);
+        t.put("}", 1);
+    }
+    t.put("]\n", 2);
+    if (!t.ok) return fail(PCUBE_OVERFLOW, "scope records exceed %llu bytes", (unsigned long long)cap);
+    *length = t.n;
+    return PCUBE_OK;
+}
+
 int pcube_floor_div(const uint64_t* in, uint64_t* out, uint64_t count) {
     std::lock_guard<std::recursive_mutex> lock(g_mu);
     if (!count) return PCUBE_OK;

@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import ctypes
 import hashlib
+import threading
 import time
 from typing import Any, Callable
 
@@ -22,7 +23,7 @@ import numpy as np
 
 from . import native
 from .accounting import digest, encoded
-from .batch_experiment import MaskedMatcher, batch_row, prepare, tile_ranges
+from .batch_experiment import EMPTY_DIGEST, MaskedMatcher, batch_row, prepare, tile_ranges
 from .gpu import GPUFocus
 
 PROFILE: dict[str, Any] = dict(engine='residue-bitset-batch-v1', tile_width=100_000_000,
@@ -93,7 +94,9 @@ class Accumulator:
         self.transcript = hashlib.sha256()
         self.batches = self.subtiles = 0
 
-    def add(self, first: int, end: int, row: dict[str, Any]) -> None:
+    def add(self, first: int, end: int, row: dict[str, Any], encoded_scopes: bytes | None = None) -> None:
+        """Fold one batch row.  encoded_scopes, when given, is the row's
+        encoded(row['subtile_scopes']) already formed (the pipelined lease's)."""
         if first != self.cursor or end != min(first + BATCH_WIDTH, self.end):
             raise ValueError('gap, overlap or changed compute batch geometry')
         validate_result(first, end, row)
@@ -115,7 +118,7 @@ class Accumulator:
             self.groups[i] += int(count)
         self.values.extend(row['all_values'])
         self.hits.extend(row['cpu_confirmed_hits'])
-        self.transcript.update(encoded(row['subtile_scopes']))
+        self.transcript.update(encoded_scopes if encoded_scopes is not None else encoded(row['subtile_scopes']))
         self.batches += 1
         self.subtiles += len(actual)
         self.cursor = end
@@ -136,13 +139,40 @@ class Accumulator:
         return row
 
 
+_records = threading.local()
+
+
+def scope_record_bytes(bounds: np.ndarray, counters: np.ndarray, ng: int, local: np.ndarray | None,
+                       shas: bytes) -> bytes:
+    """encoded(subtile_scopes) of one batch, written by the library (`pcube_scope_records`).
+
+    bounds: [ntiles, 2] (first, end); counters: [ntiles, stride] rows
+    [valid, group_1..group_ng, ...]; local: local survivors per subtile (None:
+    all 0); shas: 64 hex digits per subtile.  Every array C-contiguous uint64.
+    """
+    nt, stride = counters.shape
+    need = 256 + nt * (400 + 24 * ng)
+    buf = getattr(_records, 'buf', None)
+    if buf is None or len(buf) < need:
+        buf = _records.buf = ctypes.create_string_buffer(need)
+        _records.length = ctypes.c_uint64()
+    code = native.lib().pcube_scope_records(nt, native.ptr64(bounds), native.ptr64(counters), stride, ng,
+                                            native.ptr64(local) if local is not None else None, shas, buf,
+                                            len(buf), ctypes.byref(_records.length))
+    native.check(code, 'pcube_scope_records')
+    return buf.raw[:_records.length.value]
+
+
 def lease_rows(matcher: MaskedMatcher, af: GPUFocus, bf: GPUFocus, first: int, end: int,
-               batch_width: int = BATCH_WIDTH, window_cap: int | None = None) -> list[tuple[int, int, dict[str, Any]]]:
+               batch_width: int = BATCH_WIDTH,
+               window_cap: int | None = None) -> list[tuple[int, int, dict[str, Any], bytes]]:
     """Every batch row of [first, end) from one pipelined device pass.
 
     Each row equals `MaskedMatcher.run(prepare(..., start, stop))` for its
     batch in every field except the timings, which are the lease's phases
-    divided evenly over its batches.
+    divided evenly over its batches.  With each row come the bytes of
+    encoded(row['subtile_scopes']), formatted by the library from the same
+    counters and bounds.
     """
     config = matcher.config
     width = PROFILE['tile_width']
@@ -182,6 +212,7 @@ def lease_rows(matcher: MaskedMatcher, af: GPUFocus, bf: GPUFocus, first: int, e
     rows = []
     t0 = 0
     offset = 0
+    tile_ends = tiles.reshape(-1, 4)[:, :2].copy()   # (v_first, v_end) per subtile, contiguous
     emitted = [min(k, cap) for k in counters[:, ng + 1].tolist()]
     limbs = values[:2 * sum(emitted)].tolist()
     bound_rows = [tuple(r) for r in bounds.tolist()]
@@ -200,7 +231,14 @@ def lease_rows(matcher: MaskedMatcher, af: GPUFocus, bf: GPUFocus, first: int, e
         row['kernel_local_bytes'] = int(matcher.kernel['local_bytes'])
         row['matching_layout'] = matcher.layout
         row['prefilter_build_seconds'] = seconds[1] / n
-        rows.append((s, e, row))
+        scopes = row['subtile_scopes']
+        if row['local_survivors']:
+            local = np.array([sc['local_survivors'] for sc in scopes], dtype=np.uint64)
+            shas = ''.join(sc['output_sha256'] for sc in scopes).encode()
+        else:   # nothing emitted: every record holds 0 and the digest of []
+            local, shas = None, EMPTY_DIGEST.encode() * nt
+        records = scope_record_bytes(tile_ends[t0:t0 + nt], block, ng, local, shas)
+        rows.append((s, e, row, records))
         t0 += nt
     return rows
 
@@ -215,8 +253,8 @@ def compute_lease(matcher: MaskedMatcher, af: GPUFocus, bf: GPUFocus, first: int
     started = time.monotonic()
     acc = Accumulator(first, end)
     if pipelined:
-        for start, stop, row in lease_rows(matcher, af, bf, first, end):
-            acc.add(start, stop, row)
+        for start, stop, row, records in lease_rows(matcher, af, bf, first, end):
+            acc.add(start, stop, row, records)
             heartbeat()
     else:
         for start in range(first, end, BATCH_WIDTH):

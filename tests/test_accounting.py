@@ -6,6 +6,8 @@ bit-exact contract.  Pinned three ways: against the reference lease's golden
 per-batch scope hashes (scope records computed by the oracle, encoded here),
 against hand-written bytes at the 2^53 boundary, and the fast path for lists
 of small ints against the element-by-element rule on random nested values.
+The library's own scope-record formatter, which the pipelined lease hashes,
+is then checked byte for byte against this encoder on random records.
 """
 from __future__ import annotations
 
@@ -13,6 +15,8 @@ import hashlib
 import json
 from pathlib import Path
 import random
+
+import pytest
 
 from oracle import oracle as O
 from psuedocubes_amd.accounting import digest, encoded, interoperable
@@ -75,3 +79,32 @@ def test_scope_records_hash_to_the_golden_batch(config619, oracle_focuses619) ->
     golden = json.loads((GOLDEN / 'lease_575222931200000000_batches.json').read_text())['batches'][0]
     row = O.batch(config619, *oracle_focuses619, int(golden['first']), int(golden['end']))
     assert hashlib.sha256(encoded(row['subtile_scopes'])).hexdigest() == golden['scopes_sha256']
+
+
+def test_library_scope_records_equal_the_encoder() -> None:
+    """The pipelined lease hashes scope records the library formats
+    (`pcube_scope_records`); they must be byte for byte encoded(subtile_scopes),
+    large counts (quoted from 2^53) and emitted subtiles included."""
+    import numpy as np
+    from psuedocubes_amd.batch_experiment import EMPTY_DIGEST
+    from psuedocubes_amd.optimized import scope_record_bytes
+    rng = random.Random(61903)
+    values = [0, 1, 7, 12345, LIMIT - 1, LIMIT, 2**64 - 1]
+    for _ in range(300):
+        nt, ng = rng.randrange(0, 65), rng.choice([1, 4, 24])
+        counters = np.array([[rng.choice(values) if rng.random() < 0.3 else rng.getrandbits(40)
+                              for _ in range(ng + 3)] for _ in range(nt)], dtype=np.uint64).reshape(nt, ng + 3)
+        firsts = sorted(rng.getrandbits(62) for _ in range(nt))
+        bounds = np.array([(f, f + rng.randrange(1, 10**8)) for f in firsts], dtype=np.uint64).reshape(nt, 2)
+        emitting = rng.random() < 0.5
+        local = [rng.choice([0, 0, 3, LIMIT]) if emitting else 0 for _ in range(nt)]
+        shas = [EMPTY_DIGEST if n == 0 else hashlib.sha256(bytes([k])).hexdigest() for k, n in enumerate(local)]
+        scopes = [{'first': str(int(b[0])), 'end': str(int(b[1])), 'valid_focused_candidates': int(c[0]),
+                   'group_survivors': [int(x) for x in c[1:ng + 1]], 'local_survivors': n, 'output_sha256': h}
+                  for b, c, n, h in zip(bounds, counters, local, shas)]
+        got = scope_record_bytes(bounds, counters, ng, np.array(local, dtype=np.uint64) if emitting else None,
+                                 ''.join(shas).encode())
+        assert got == encoded(scopes)
+    with pytest.raises(ValueError):
+        scope_record_bytes(np.zeros((1, 2), dtype=np.uint64), np.zeros((1, 1), dtype=np.uint64), 4, None,
+                           EMPTY_DIGEST.encode())
