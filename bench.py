@@ -11,7 +11,7 @@ per-subtile output hashes.  The focus wheels and filter tables are resident
 in HBM before timing starts, as in the reference's long-running worker.
 
 Prints one JSON line.  Extra fields: kernel-only rate, the pipelined lease
-rate (256 batches in one device pass), the roofline of the match kernel, and
+rate (256 batches in 32-batch device passes overlapped with the host fold), the roofline of the match kernel, and
 the CPU baseline (this repository's C restatement on the host cores).
 """
 from __future__ import annotations

@@ -6,12 +6,14 @@
  *   - Device buffers are passed as uint64_t addresses obtained from
  *     pcube_alloc (or offsets into them); host buffers as ordinary pointers.
  *   - Every call returns a status code; pcube_last_error() describes the most
- *     recent failure.  The Python layer maps the codes to the reference's
+ *     recent failure of the calling thread.  The Python layer maps the codes to the reference's
  *     exceptions: OVERFLOW -> OverflowError, INVALID -> ValueError,
  *     UNDERFLOW/RANGE -> ArithmeticError, NOMEM -> MemoryError.
  *   - All work runs on one library-owned HIP stream of default priority; each
  *     entry point returns after its results are on the host unless it says
- *     otherwise.  One host thread drives the library (calls are serialised).
+ *     otherwise.  Calls may come from any host thread: one library lock
+ *     serialises them, and each makes the library's device current on its
+ *     thread before it allocates or launches.
  *   - 128-bit quantities are split into (low, high) uint64 limbs.
  */
 #ifndef PCUBE_H
@@ -205,8 +207,8 @@ int pcube_match_adoptable(int matcher, uint64_t launch, int *adoptable);
 
 /* --- lease pipeline: replaces pseudocube/optimized.py:81-90 ---------------- */
 
-/* Run every compute batch of a lease with one lease-wide window per focus
- * and no host synchronisation between batches.
+/* Run consecutive compute batches (a lease, or one pass of it) with one window
+ * per focus over all of them and no host synchronisation between batches.
  *   tiles        host, ntiles_total * 4 uint64: (v_first, v_end, u_min, u_end)
  *                per subtile, exactly tile_ranges() of each batch, batches
  *                consecutive; batch_tiles[b] subtiles belong to batch b

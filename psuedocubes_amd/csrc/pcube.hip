@@ -30,7 +30,8 @@ using namespace pcube;
 namespace {
 
 std::recursive_mutex g_mu;
-std::string g_err;
+// the last error of the calling thread: calls on other threads leave it alone
+thread_local std::string g_err;
 
 int fail(int code, const char* fmt, ...) {
     char buf[512];
@@ -66,6 +67,18 @@ struct Context {
     uint32_t lds_per_block = 65536;                  // largest LDS one workgroup may use
     char name[256] = {0};
 } g;
+
+// HIP's current device belongs to the calling thread: an entry point called
+// from a thread other than pcube_init's makes the library's device current
+// there first, so its allocations land on that device
+thread_local int t_device = -1;
+
+struct Entry {
+    std::lock_guard<std::recursive_mutex> lock{g_mu};
+    Entry() {
+        if (g.device >= 0 && t_device != g.device && hipSetDevice(g.device) == hipSuccess) t_device = g.device;
+    }
+};
 
 uint64_t size_class(uint64_t bytes) {
     if (bytes == 0) bytes = 1;
@@ -1115,7 +1128,7 @@ extern "C" {
 const char* pcube_last_error(void) { return g_err.c_str(); }
 
 int pcube_device_count(int* count) {
-    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Entry lock;
     int n = 0;
     hipError_t e = hipGetDeviceCount(&n);
     if (e != hipSuccess) n = 0;
@@ -1125,7 +1138,7 @@ int pcube_device_count(int* count) {
 
 int pcube_init(int device, uint64_t memory_cap, double fraction, uint64_t* cap, uint64_t* free_bytes,
                uint64_t* total_bytes) {
-    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Entry lock;
     if (g.device >= 0 && g.device != device) return fail(PCUBE_INVALID, "already initialised on device %d", g.device);
     if (g.device < 0) {
         HIP_TRY(hipSetDevice(device));
@@ -1159,7 +1172,7 @@ int pcube_init(int device, uint64_t memory_cap, double fraction, uint64_t* cap, 
 }
 
 int pcube_memory(uint64_t* reserved, uint64_t* peak, uint64_t* cap) {
-    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Entry lock;
     if (reserved) *reserved = g.reserved;
     if (peak) *peak = g.peak;
     if (cap) *cap = g.cap;
@@ -1167,7 +1180,7 @@ int pcube_memory(uint64_t* reserved, uint64_t* peak, uint64_t* cap) {
 }
 
 int pcube_device_info(char* name, size_t name_len, int* compute_units) {
-    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Entry lock;
     if (g.device < 0) return fail(PCUBE_INVALID, "pcube_init has not been called");
     if (name && name_len) snprintf(name, name_len, "%s", g.name);
     if (compute_units) *compute_units = g.compute_units;
@@ -1175,7 +1188,7 @@ int pcube_device_info(char* name, size_t name_len, int* compute_units) {
 }
 
 int pcube_kernel_info(int index, char* name, size_t name_len, int* registers, int* local_bytes, int* shared_bytes) {
-    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Entry lock;
     const int n = (int)(sizeof kKernels / sizeof kKernels[0]);
     if (index < 0 || index >= n) return fail(PCUBE_INVALID, "kernel index out of range");
     if (name && name_len) snprintf(name, name_len, "%s", kKernels[index].name);
@@ -1189,7 +1202,7 @@ int pcube_kernel_info(int index, char* name, size_t name_len, int* registers, in
 }
 
 int pcube_alloc(uint64_t bytes, uint64_t* address) {
-    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Entry lock;
     void* p = nullptr;
     TRY(dev_alloc(bytes, &p));
     *address = (uint64_t)p;
@@ -1197,13 +1210,13 @@ int pcube_alloc(uint64_t bytes, uint64_t* address) {
 }
 
 int pcube_free(uint64_t address) {
-    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Entry lock;
     // stream-ordered reuse: the next user of the block queues behind this one
     return dev_free((void*)address);
 }
 
 int pcube_upload(uint64_t dst, const void* src, uint64_t bytes) {
-    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Entry lock;
     if (!bytes) return PCUBE_OK;
     HIP_TRY(hipMemcpyAsync((void*)dst, src, bytes, hipMemcpyHostToDevice, g.stream));
     HIP_TRY(hipStreamSynchronize(g.stream));
@@ -1211,7 +1224,7 @@ int pcube_upload(uint64_t dst, const void* src, uint64_t bytes) {
 }
 
 int pcube_download(void* dst, uint64_t src, uint64_t bytes) {
-    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Entry lock;
     if (!bytes) return PCUBE_OK;
     HIP_TRY(hipMemcpyAsync(dst, (const void*)src, bytes, hipMemcpyDeviceToHost, g.stream));
     HIP_TRY(hipStreamSynchronize(g.stream));
@@ -1220,7 +1233,7 @@ int pcube_download(void* dst, uint64_t src, uint64_t bytes) {
 
 int pcube_focus_create(uint64_t modulus, const uint64_t* terms, const uint32_t* sizes, uint32_t nrings_c,
                        uint32_t nrings_d, int* focus, uint64_t* nc, uint64_t* nd) {
-    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Entry lock;
     if (g.device < 0) return fail(PCUBE_INVALID, "pcube_init has not been called");
     if (modulus == 0 || modulus >= (1ull << 63)) return fail(PCUBE_INVALID, "focus modulus must lie in [1, 2^63)");
     auto f = std::make_unique<Focus>();
@@ -1248,7 +1261,7 @@ int pcube_focus_create(uint64_t modulus, const uint64_t* terms, const uint32_t* 
 }
 
 int pcube_focus_arrays(int focus, uint64_t* c_address, uint64_t* d_address) {
-    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Entry lock;
     Focus* f = focus_at(focus);
     if (!f) return fail(PCUBE_INVALID, "unknown focus");
     *c_address = (uint64_t)f->C.p;
@@ -1257,7 +1270,7 @@ int pcube_focus_arrays(int focus, uint64_t* c_address, uint64_t* d_address) {
 }
 
 int pcube_focus_destroy(int focus) {
-    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Entry lock;
     if (!focus_at(focus)) return fail(PCUBE_INVALID, "unknown focus");
     HIP_TRY(hipStreamSynchronize(g.stream));
     g_focus[focus].reset();
@@ -1265,7 +1278,7 @@ int pcube_focus_destroy(int focus) {
 }
 
 int pcube_focus_window(int focus, uint64_t low, uint64_t high, uint64_t out, uint64_t cap, uint64_t* count) {
-    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Entry lock;
     *count = 0;
     Focus* f = focus_at(focus);
     if (!f || low > high) return fail(PCUBE_INVALID, "invalid focus window");
@@ -1280,7 +1293,7 @@ int pcube_focus_window(int focus, uint64_t low, uint64_t high, uint64_t out, uin
 
 int pcube_focus_wheel(int focus, const uint32_t* moduli, const uint32_t* sizes, const uint32_t* residues,
                       uint32_t nrings, uint64_t* inner_modulus, uint32_t* inner_residues) {
-    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Entry lock;
     Focus* f = focus_at(focus);
     if (!f) return fail(PCUBE_INVALID, "unknown focus");
     uint64_t product = 1;
@@ -1416,7 +1429,7 @@ int pcube_focus_wheel(int focus, const uint32_t* moduli, const uint32_t* sizes, 
 
 int pcube_focus_windows(int focus, uint32_t n, const uint64_t* lows, const uint64_t* highs, uint64_t out,
                         uint64_t cap, uint64_t* offsets) {
-    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Entry lock;
     Focus* f = focus_at(focus);
     if (!f) return fail(PCUBE_INVALID, "unknown focus");
     for (uint32_t w = 0; w < n; ++w)
@@ -1428,7 +1441,7 @@ int pcube_focus_windows(int focus, uint32_t n, const uint64_t* lows, const uint6
 
 int pcube_prepare(int focus_u, int focus_v, uint32_t ntiles, const uint64_t* tiles, uint64_t us_out, uint64_t us_cap,
                   uint64_t vs_out, uint64_t vs_cap, uint64_t* nu, uint64_t* nv, uint64_t* bounds) {
-    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Entry lock;
     Focus* fu = focus_at(focus_u);
     Focus* fv = focus_at(focus_v);
     if (!fu || !fv || fu == fv || !fu->wheel.ready || !fv->wheel.ready || ntiles == 0 || ntiles > kMaxPrepareTiles)
@@ -1522,7 +1535,7 @@ int pcube_prepare_range(int focus_u, int focus_v, uint64_t A, uint64_t B, uint64
                         uint64_t hi1, uint64_t first, uint64_t end, uint64_t width, uint64_t* tiles, uint32_t* ntiles,
                         uint64_t us_out, uint64_t us_cap, uint64_t vs_out, uint64_t vs_cap, uint64_t* nu, uint64_t* nv,
                         uint64_t* bounds, uint64_t* totals) {
-    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Entry lock;
     Focus* fu = focus_at(focus_u);
     Focus* fv = focus_at(focus_v);
     if (!fu || !fv) return fail(PCUBE_INVALID, "unknown focus");
@@ -1556,7 +1569,7 @@ int pcube_prepare_match(int focus_u, int focus_v, uint64_t A, uint64_t B, uint64
                         uint64_t us_out, uint64_t us_cap, uint64_t vs_out, uint64_t vs_cap, uint64_t* nu, uint64_t* nv,
                         uint64_t* bounds, uint64_t* totals, int matcher, uint64_t cap, int engine,
                         uint64_t max_opportunities, uint64_t* launch) {
-    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Entry lock;
     if (!totals || !launch) return fail(PCUBE_INVALID, "null totals or launch");
     *launch = 0;
     TRY(pcube_prepare_range(focus_u, focus_v, A, B, lo0, lo1, hi0, hi1, first, end, width, tiles, ntiles, us_out,
@@ -1571,7 +1584,7 @@ int pcube_prepare_match(int focus_u, int focus_v, uint64_t A, uint64_t B, uint64
 }
 
 int pcube_match_adoptable(int matcher, uint64_t launch, int* adoptable) {
-    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Entry lock;
     if (!adoptable) return fail(PCUBE_INVALID, "null result");
     Matcher* mt = matcher_at(matcher);
     *adoptable = mt && launch && mt->pending.active && mt->launches == launch ? 1 : 0;
@@ -1579,7 +1592,7 @@ int pcube_match_adoptable(int matcher, uint64_t launch, int* adoptable) {
 }
 
 int pcube_searchsorted(uint64_t sorted, uint64_t n, const uint64_t* queries, uint64_t nq, uint64_t* out) {
-    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Entry lock;
     if (!nq) return PCUBE_OK;
     void *dq = nullptr, *dout = nullptr;
     TRY(dev_alloc(nq * 8, &dq));
@@ -1597,7 +1610,7 @@ int pcube_searchsorted(uint64_t sorted, uint64_t n, const uint64_t* queries, uin
 
 int pcube_matcher_create(uint64_t A, uint64_t B, uint64_t lo0, uint64_t lo1, uint64_t hi0, uint64_t hi1,
                          const uint32_t* moduli, uint32_t nres, int* matcher) {
-    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Entry lock;
     if (g.device < 0) return fail(PCUBE_INVALID, "pcube_init has not been called");
     if (nres == 0 || nres > (uint32_t)kMaxResiduals) return fail(PCUBE_INVALID, "between 1 and 128 residual filters required");
     if (B == 0) return fail(PCUBE_INVALID, "B must be positive");
@@ -1683,7 +1696,7 @@ int pcube_matcher_create(uint64_t A, uint64_t B, uint64_t lo0, uint64_t lo1, uin
 }
 
 int pcube_matcher_destroy(int matcher) {
-    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Entry lock;
     if (!matcher_at(matcher)) return fail(PCUBE_INVALID, "unknown matcher");
     HIP_TRY(hipStreamSynchronize(g.stream));
     g_matchers[matcher].reset();
@@ -1692,7 +1705,7 @@ int pcube_matcher_destroy(int matcher) {
 
 int pcube_match_start(int matcher, uint64_t us, uint64_t nu, uint64_t vs, uint64_t nv, uint64_t v_first,
                       uint64_t tile_width, uint32_t ntiles, uint64_t cap, int engine) {
-    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Entry lock;
     Matcher* mt = matcher_at(matcher);
     if (!mt) return fail(PCUBE_INVALID, "unknown matcher");
     if (ntiles == 0 || ntiles > 65536 || tile_width == 0 || (engine != 0 && engine != 1))
@@ -1716,7 +1729,7 @@ int pcube_match_start(int matcher, uint64_t us, uint64_t nu, uint64_t vs, uint64
 }
 
 int pcube_match_finish(int matcher, uint64_t* counters, uint64_t* values, uint64_t values_cap, double* seconds) {
-    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Entry lock;
     Matcher* mt = matcher_at(matcher);
     if (!mt) return fail(PCUBE_INVALID, "unknown matcher");
     Pending& pd = mt->pending;
@@ -1764,7 +1777,7 @@ int pcube_match_finish(int matcher, uint64_t* counters, uint64_t* values, uint64
 int pcube_match(int matcher, uint64_t us, uint64_t nu, uint64_t vs, uint64_t nv, uint64_t v_first,
                 uint64_t tile_width, uint32_t ntiles, uint64_t cap, int engine, uint64_t* counters, uint64_t* values,
                 uint64_t values_cap, double* seconds) {
-    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Entry lock;
     TRY(pcube_match_start(matcher, us, nu, vs, nv, v_first, tile_width, ntiles, cap, engine));
     return pcube_match_finish(matcher, counters, values, values_cap, seconds);
 }
@@ -1772,7 +1785,7 @@ int pcube_match(int matcher, uint64_t us, uint64_t nu, uint64_t vs, uint64_t nv,
 int pcube_lease(int matcher, int focus_u, int focus_v, uint32_t nbatches, const uint32_t* batch_tiles,
                 const uint64_t* tiles, uint64_t tile_width, uint64_t cap, uint64_t window_cap, uint64_t* bounds,
                 uint64_t* batch_sizes, uint64_t* counters, uint64_t* values, uint64_t values_cap, double* seconds) {
-    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Entry lock;
     const double t_start = now_seconds();
     Matcher* mt = matcher_at(matcher);
     Focus* fu = focus_at(focus_u);
@@ -1898,7 +1911,7 @@ int pcube_lease(int matcher, int focus_u, int focus_v, uint32_t nbatches, const 
 
 int pcube_batch_create(const int* matchers, uint32_t ncfg, int focus_u, int focus_v, uint64_t cap,
                        uint32_t blocks_per_cu, int use_graph, int* engine) {
-    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Entry lock;
     if (ncfg == 0 || ncfg > (uint32_t)kMaxConfigs) return fail(PCUBE_INVALID, "1 to %d configurations per batch", kMaxConfigs);
     Focus* fu = focus_at(focus_u);
     Focus* fv = focus_at(focus_v);
@@ -1945,7 +1958,7 @@ int pcube_batch_create(const int* matchers, uint32_t ncfg, int focus_u, int focu
 }
 
 int pcube_batch_destroy(int engine) {
-    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Entry lock;
     if (engine < 0 || engine >= (int)g_batch.size() || !g_batch[engine]) return fail(PCUBE_INVALID, "unknown batch engine");
     HIP_TRY(hipStreamSynchronize(g.stream));
     g_batch[engine].reset();
@@ -1956,7 +1969,7 @@ int pcube_batch_run(int engine, uint32_t nwin, const uint32_t* win_cfg, const ui
                     const uint64_t* tiles, const uint64_t* tile_width, const uint32_t* mode, uint32_t rows_per_chunk,
                     uint64_t* bounds, uint64_t* sizes, uint64_t* counters, uint64_t* values, uint64_t values_cap,
                     double* seconds, uint64_t* graph_launches) {
-    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Entry lock;
     const double t_start = now_seconds();
     if (engine < 0 || engine >= (int)g_batch.size() || !g_batch[engine]) return fail(PCUBE_INVALID, "unknown batch engine");
     BatchEngine& e = *g_batch[engine];
@@ -2227,7 +2240,7 @@ int pcube_host_verify(const uint64_t* limbs, uint64_t n, uint32_t p, uint8_t* lo
 }
 
 int pcube_lds_probe(uint32_t pattern, uint32_t rounds, double* seconds, double* bytes) {
-    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Entry lock;
     if (pattern > 2 || !rounds) return fail(PCUBE_INVALID, "probe pattern 0..2 and at least one round");
     const uint32_t grid = (uint32_t)g.compute_units;
     void* out = nullptr;
@@ -2251,7 +2264,7 @@ int pcube_lds_probe(uint32_t pattern, uint32_t rounds, double* seconds, double* 
 }
 
 int pcube_valu_probe(uint32_t rounds, double* seconds, double* lane_ops) {
-    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Entry lock;
     if (!rounds) return fail(PCUBE_INVALID, "at least one probe round");
     const uint32_t grid = (uint32_t)g.compute_units * 8;   // 32 waves per CU
     void* out = nullptr;
@@ -2276,7 +2289,7 @@ int pcube_valu_probe(uint32_t rounds, double* seconds, double* lane_ops) {
 
 int pcube_cube_count(uint64_t first, uint64_t last, uint32_t p, uint64_t* count, uint32_t* good, uint64_t good_words,
                      double* seconds) {
-    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Entry lock;
     if (first < 1 || last < first || last > UINT32_MAX || p < 2 || p > 2000)
         return fail(PCUBE_INVALID, "cube count needs 1 <= first <= last <= 2^32 - 1 and 2 <= p <= 2000");
     if (g.device < 0) return fail(PCUBE_INVALID, "pcube_init has not been called");
@@ -2330,7 +2343,7 @@ int pcube_cube_count(uint64_t first, uint64_t last, uint32_t p, uint64_t* count,
 }
 
 int pcube_copy_probe(uint64_t bytes, double* seconds) {
-    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Entry lock;
     const uint64_t n = bytes / 8;
     if (!n) return fail(PCUBE_INVALID, "probe size must be at least 8 bytes");
     void *src = nullptr, *dst = nullptr;
@@ -2421,7 +2434,7 @@ int pcube_scope_records(uint32_t ntiles, const uint64_t* bounds, const uint64_t*
 }
 
 int pcube_floor_div(const uint64_t* in, uint64_t* out, uint64_t count) {
-    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Entry lock;
     if (!count) return PCUBE_OK;
     std::vector<uint64_t> image(5 * count);
     for (uint64_t i = 0; i < count; ++i) {
@@ -2449,7 +2462,7 @@ int pcube_floor_div(const uint64_t* in, uint64_t* out, uint64_t count) {
 }
 
 int pcube_arithmetic(const uint64_t* in, uint64_t* out, uint64_t count) {
-    std::lock_guard<std::recursive_mutex> lock(g_mu);
+    Entry lock;
     if (!count) return PCUBE_OK;
     void *din = nullptr, *dout = nullptr;
     TRY(dev_alloc(count * 32, &din));

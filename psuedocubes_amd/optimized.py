@@ -4,12 +4,13 @@ Drop-in for the reference's `pseudocube/optimized.py:15-90` (`PROFILE`,
 `BATCH_WIDTH`, `LEASE_WIDTH`, `Accumulator`, `compute_lease`) and the
 receipt check `validate_result` of `pseudocube/multi.py:36-67`.
 
-`compute_lease` hands the whole lease to libpcube in one call: one
-lease-wide focus window per wheel, every compute batch's bitset build and
-match queued on the device stream without host synchronisation, and one
-copy of all counters at the end.  The per-batch rows it then assembles are
-field for field those of `prepare` + `MaskedMatcher.run`, so the transcript
-is unchanged.
+`compute_lease` hands the lease to libpcube in passes of LEASE_CHUNK
+consecutive batches, each one call on a worker thread: one focus window per
+wheel over the pass, every batch's bitset build and match queued on the
+device stream without host synchronisation, and one copy of the pass's
+counters at the end.  The caller's thread assembles one pass's rows while the
+device computes the next; they are field for field those of `prepare` +
+`MaskedMatcher.run`, so the transcript is unchanged.
 """
 from __future__ import annotations
 
@@ -17,7 +18,8 @@ import ctypes
 import hashlib
 import threading
 import time
-from typing import Any, Callable
+from concurrent.futures import ThreadPoolExecutor
+from typing import Any, Callable, Iterator
 
 import numpy as np
 
@@ -163,33 +165,24 @@ def scope_record_bytes(bounds: np.ndarray, counters: np.ndarray, ng: int, local:
     return buf.raw[:_records.length.value]
 
 
-def lease_rows(matcher: MaskedMatcher, af: GPUFocus, bf: GPUFocus, first: int, end: int,
-               batch_width: int = BATCH_WIDTH,
-               window_cap: int | None = None) -> list[tuple[int, int, dict[str, Any], bytes]]:
-    """Every batch row of [first, end) from one pipelined device pass.
+LEASE_CHUNK = 32   # batches per device pass of a pipelined lease
 
-    Each row equals `MaskedMatcher.run(prepare(..., start, stop))` for its
-    batch in every field except the timings, which are the lease's phases
-    divided evenly over its batches.  With each row come the bytes of
-    encoded(row['subtile_scopes']), formatted by the library from the same
-    counters and bounds.
-    """
-    config = matcher.config
+
+def _lease_pass(matcher: MaskedMatcher, af: GPUFocus, bf: GPUFocus, batches: list[tuple[int, int]],
+                ranges: list[list[tuple[int, int, int, int]]], window_cap: int | None) -> tuple[Any, ...]:
+    """One device pass (`pcube_lease`) over consecutive batches: their windows,
+    bounds, matches and counters, with one host synchronisation."""
     width = PROFILE['tile_width']
-    starts = list(range(first, end, batch_width))
-    batches = [(s, min(s + batch_width, end)) for s in starts]
-    ranges = [tile_ranges(config, s, e, width) for s, e in batches]
+    first, end = batches[0][0], batches[-1][1]
     tiles = np.array([x for r in ranges for t in r for x in t], dtype=np.uint64)
     per_batch = np.array([len(r) for r in ranges], dtype=np.uint32)
     ntot = int(per_batch.sum())
     ng = len(matcher.filter_groups)
     if window_cap is None:
-        # expected coordinates in each lease-wide window, with margin
+        # expected coordinates in each window of the pass, with margin
         u_span = ranges[-1][-1][3] - ranges[0][0][2]
-        v_span = end - first
-        expect = max(u_span * af.focus.count / af.focus.modulus, v_span * bf.focus.count / bf.focus.modulus)
+        expect = max(u_span * af.focus.count / af.focus.modulus, (end - first) * bf.focus.count / bf.focus.modulus)
         window_cap = int(expect * 1.1) + 1_000_000
-    cap = 8192
     bounds = np.zeros(ntot * 4, dtype=np.uint64)
     sizes = np.zeros(len(batches) * 2, dtype=np.uint64)
     counters = np.zeros(ntot * (ng + 3), dtype=np.uint64)
@@ -198,49 +191,81 @@ def lease_rows(matcher: MaskedMatcher, af: GPUFocus, bf: GPUFocus, first: int, e
     started = time.perf_counter()
     matcher.runtime.check()
     code = native.lib().pcube_lease(matcher.engine.handle, af.handle, bf.handle, len(batches),
-                                    native.ptr32(per_batch), native.ptr64(tiles), width, cap, window_cap,
+                                    native.ptr32(per_batch), native.ptr64(tiles), width, 8192, window_cap,
                                     native.ptr64(bounds), native.ptr64(sizes), native.ptr64(counters),
                                     native.ptr64(values), values.size // 2, seconds)
     native.check(code, 'pcube_lease')
     matcher.runtime.check()
-    device_seconds = time.perf_counter() - started
-    counters = counters.reshape(ntot, ng + 3)
-    bounds = bounds.reshape(ntot, 4)
-    n = len(batches)
-    timings = dict(kernel_seconds=seconds[2] / n, focus_enumeration_seconds=seconds[0] / n,
-                   transfer_seconds=max(0.0, device_seconds - seconds[0] - seconds[2]) / n)
-    rows = []
-    t0 = 0
-    offset = 0
-    tile_ends = tiles.reshape(-1, 4)[:, :2].copy()   # (v_first, v_end) per subtile, contiguous
-    emitted = [min(k, cap) for k in counters[:, ng + 1].tolist()]
-    limbs = values[:2 * sum(emitted)].tolist()
-    bound_rows = [tuple(r) for r in bounds.tolist()]
-    for b, (s, e) in enumerate(batches):
-        nt = len(ranges[b])
-        block = counters[t0:t0 + nt]
-        per_tile = []
-        for k in emitted[t0:t0 + nt]:
-            per_tile.append([limbs[2 * i] + (limbs[2 * i + 1] << 64) for i in range(offset, offset + k)])
-            offset += k
-        bnd = bound_rows[t0:t0 + nt]
-        row_started = time.perf_counter()
-        row = batch_row(config, matcher.filter_groups, ranges[b], bnd, int(sizes[2 * b + 1]), block, per_tile,
-                        cap, timings, row_started)
-        row['kernel_registers'] = int(matcher.kernel['registers'])
-        row['kernel_local_bytes'] = int(matcher.kernel['local_bytes'])
-        row['matching_layout'] = matcher.layout
-        row['prefilter_build_seconds'] = seconds[1] / n
-        scopes = row['subtile_scopes']
-        if row['local_survivors']:
-            local = np.array([sc['local_survivors'] for sc in scopes], dtype=np.uint64)
-            shas = ''.join(sc['output_sha256'] for sc in scopes).encode()
-        else:   # nothing emitted: every record holds 0 and the digest of []
-            local, shas = None, EMPTY_DIGEST.encode() * nt
-        records = scope_record_bytes(tile_ends[t0:t0 + nt], block, ng, local, shas)
-        rows.append((s, e, row, records))
-        t0 += nt
-    return rows
+    return (tiles, bounds.reshape(ntot, 4), sizes, counters.reshape(ntot, ng + 3), values, list(seconds),
+            time.perf_counter() - started)
+
+
+def lease_rows(matcher: MaskedMatcher, af: GPUFocus, bf: GPUFocus, first: int, end: int,
+               batch_width: int = BATCH_WIDTH,
+               window_cap: int | None = None) -> Iterator[tuple[int, int, dict[str, Any], bytes]]:
+    """Every batch row of [first, end), from pipelined device passes.
+
+    The batches go to the device LEASE_CHUNK at a time, each group in one
+    pass (`pcube_lease`) on a worker thread; the caller folds one group's rows
+    while the device computes the next.  Each row equals
+    `MaskedMatcher.run(prepare(..., start, stop))` for its batch in every
+    field except the timings, which are its pass's phases divided evenly over
+    the pass's batches.  With each row come the bytes of
+    encoded(row['subtile_scopes']), formatted by the library from the same
+    counters and bounds.
+    """
+    config = matcher.config
+    width = PROFILE['tile_width']
+    ng = len(matcher.filter_groups)
+    cap = 8192
+    starts = list(range(first, end, batch_width))
+    groups = [[(s, min(s + batch_width, end)) for s in starts[i:i + LEASE_CHUNK]]
+              for i in range(0, len(starts), LEASE_CHUNK)]
+    pool = ThreadPoolExecutor(max_workers=1)
+    try:
+        pending = []
+        for batches in groups:
+            # the geometry of later groups is computed while the device runs the first
+            ranges = [tile_ranges(config, s, e, width) for s, e in batches]
+            pending.append((batches, ranges, pool.submit(_lease_pass, matcher, af, bf, batches, ranges, window_cap)))
+        for batches, ranges, future in pending:
+            tiles, bounds, sizes, counters, values, seconds, device_seconds = future.result()
+            n = len(batches)
+            timings = dict(kernel_seconds=seconds[2] / n, focus_enumeration_seconds=seconds[0] / n,
+                           transfer_seconds=max(0.0, device_seconds - seconds[0] - seconds[2]) / n)
+            t0 = 0
+            offset = 0
+            tile_ends = tiles.reshape(-1, 4)[:, :2].copy()   # (v_first, v_end) per subtile, contiguous
+            emitted = [min(k, cap) for k in counters[:, ng + 1].tolist()]
+            limbs = values[:2 * sum(emitted)].tolist()
+            bound_rows = [tuple(r) for r in bounds.tolist()]
+            for b, (s, e) in enumerate(batches):
+                nt = len(ranges[b])
+                block = counters[t0:t0 + nt]
+                per_tile = []
+                for k in emitted[t0:t0 + nt]:
+                    per_tile.append([limbs[2 * i] + (limbs[2 * i + 1] << 64) for i in range(offset, offset + k)])
+                    offset += k
+                bnd = bound_rows[t0:t0 + nt]
+                row_started = time.perf_counter()
+                row = batch_row(config, matcher.filter_groups, ranges[b], bnd, int(sizes[2 * b + 1]), block,
+                                per_tile, cap, timings, row_started)
+                row['kernel_registers'] = int(matcher.kernel['registers'])
+                row['kernel_local_bytes'] = int(matcher.kernel['local_bytes'])
+                row['matching_layout'] = matcher.layout
+                row['prefilter_build_seconds'] = seconds[1] / n
+                scopes = row['subtile_scopes']
+                if row['local_survivors']:
+                    local = np.array([sc['local_survivors'] for sc in scopes], dtype=np.uint64)
+                    shas = ''.join(sc['output_sha256'] for sc in scopes).encode()
+                else:   # nothing emitted: every record holds 0 and the digest of []
+                    local, shas = None, EMPTY_DIGEST.encode() * nt
+                records = scope_record_bytes(tile_ends[t0:t0 + nt], block, ng, local, shas)
+                yield s, e, row, records
+                t0 += nt
+    finally:
+        # a caller that stops early (an error in its fold) leaves no pass behind
+        pool.shutdown(wait=True, cancel_futures=True)
 
 
 def compute_lease(matcher: MaskedMatcher, af: GPUFocus, bf: GPUFocus, first: int, end: int,

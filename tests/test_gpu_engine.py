@@ -268,6 +268,25 @@ def test_pipelined_lease_equals_per_batch_loop(runtime, config619, focuses619, m
     assert a['subtiles_executed'] == 64 + 64 + 4 and a['compute_batches'] == 3
 
 
+def test_pipelined_lease_over_several_passes(runtime, config619, focuses619, masked, monkeypatch) -> None:
+    """Passes of three batches: a lease of seven (the last ragged) spans three
+    device passes, the last with one batch, and folds to the per-batch loop's
+    receipt; a fold stopped after its first row leaves no pass behind."""
+    from psuedocubes_amd import optimized
+    monkeypatch.setattr(optimized, 'LEASE_CHUNK', 3)
+    first = LEASE_FIRST + 40 * BATCH_WIDTH
+    end = first + 6 * BATCH_WIDTH + 250_000_001
+    rows = optimized.lease_rows(masked, *focuses619, first, end)
+    s, e, row, _ = next(rows)
+    assert (s, e) == (first, first + BATCH_WIDTH)
+    rows.close()
+    a = compute_lease(masked, *focuses619, first, end, pipelined=True)
+    b = compute_lease(masked, *focuses619, first, end, pipelined=False)
+    strip = lambda r: {k: v for k, v in r.items() if not k.endswith('_seconds')}  # noqa: E731
+    assert strip(a) == strip(b)
+    assert a['compute_batches'] == 7 and a['subtiles_executed'] == 6 * 64 + 3
+
+
 def test_legacy_matcher_tile_and_split_replay(runtime) -> None:
     config = small_config(7)
     af, bf = (GPUFocus(runtime, f) for f in config.focuses())
