@@ -16,10 +16,11 @@ from __future__ import annotations
 
 import ctypes
 import hashlib
+import sys
 import threading
 import time
 from concurrent.futures import ThreadPoolExecutor
-from typing import Any, Callable, Iterator
+from typing import Any, Callable, Iterator, NamedTuple
 
 import numpy as np
 
@@ -76,6 +77,22 @@ def validate_result(first: int, end: int, result: dict[str, Any]) -> None:
         raise ValueError('unknown receipt format')
 
 
+class QuietBatch(NamedTuple):
+    """A batch whose subtiles emitted nothing, by the totals its row would hold:
+    focus_u, focus_v and cartesian_opportunities from its bounds and window,
+    valid_focused_candidates and group_survivors summed over its counters;
+    every survivor count 0 and no values or hits."""
+    subtiles: int
+    span: tuple[int, int]        # (first subtile's first, last subtile's end)
+    focus_u: int
+    focus_v: int
+    opportunities: int
+    valid: int
+    groups: list[int]
+    filter_groups: list[list[int]]
+    timings: dict[str, float]
+
+
 class Accumulator:
     """Folds batch rows into one compact lease row (`pseudocube/optimized.py:26-78`).
 
@@ -125,6 +142,32 @@ class Accumulator:
         self.subtiles += len(actual)
         self.cursor = end
 
+    def add_quiet(self, first: int, end: int, batch: QuietBatch, encoded_scopes: bytes) -> None:
+        """Fold a batch that emitted nothing from its totals: the checks and sums
+        of add() on the row batch_row builds for it, without building the row."""
+        if first != self.cursor or end != min(first + BATCH_WIDTH, self.end):
+            raise ValueError('gap, overlap or changed compute batch geometry')
+        width = PROFILE['tile_width']
+        if batch.span != (first, end) or batch.subtiles != (end - first + width - 1) // width:
+            raise ValueError('changed arithmetic subtile geometry')
+        if self.filters is None:
+            self.filters = [list(group) for group in batch.filter_groups]
+            self.groups = [0] * len(self.filters)
+        if batch.filter_groups != self.filters or len(batch.groups) != len(self.groups):
+            raise ValueError('residue filters changed during execution')
+        counts = self.counts
+        counts['focus_u'] += batch.focus_u
+        counts['focus_v'] += batch.focus_v
+        counts['cartesian_opportunities'] += batch.opportunities
+        counts['valid_focused_candidates'] += batch.valid
+        for key in TIMINGS:
+            self.timings[key] += batch.timings.get(key, 0.0)
+        self.groups = [a + b for a, b in zip(self.groups, batch.groups)]
+        self.transcript.update(encoded_scopes)
+        self.batches += 1
+        self.subtiles += batch.subtiles
+        self.cursor = end
+
     def finish(self, seconds: float) -> dict[str, Any]:
         if self.cursor != self.end:
             raise ValueError('unfinished lease cannot become coverage')
@@ -168,10 +211,30 @@ def scope_record_bytes(bounds: np.ndarray, counters: np.ndarray, ng: int, local:
 LEASE_CHUNK = 32   # batches per device pass of a pipelined lease
 
 
-def _lease_pass(matcher: MaskedMatcher, af: GPUFocus, bf: GPUFocus, batches: list[tuple[int, int]],
-                ranges: list[list[tuple[int, int, int, int]]], window_cap: int | None) -> tuple[Any, ...]:
-    """One device pass (`pcube_lease`) over consecutive batches: their windows,
-    bounds, matches and counters, with one host synchronisation."""
+def pass_totals(per_batch: list[int], bounds: np.ndarray, counters: np.ndarray,
+                ng: int) -> tuple[list[bool], list[int], list[int], list[list[int]]]:
+    """Per-batch totals of a device pass whose batch b has per_batch[b]
+    consecutive subtiles: whether it emitted nothing and saw no underflow,
+    focus_u (sum of u_end - u_first), cartesian_opportunities (sum of
+    (u_end - u_first) (v_end - v_first)) and the column sums [valid,
+    group_1..group_ng], all as batch_row forms them.  bounds: [ntiles, 4]
+    index rows; counters: [ntiles, ng + 3].  A pass with a reversed bound pair
+    marks no batch quiet (its rows go through batch_row)."""
+    heads = np.cumsum([0] + per_batch[:-1])
+    du = bounds[:, 1] - bounds[:, 0]
+    ordered = bool((bounds[:, 1] >= bounds[:, 0]).all() and (bounds[:, 3] >= bounds[:, 2]).all())
+    flagged = np.add.reduceat((counters[:, ng + 1:ng + 3] != 0).any(axis=1).astype(np.uint32), heads)
+    quiet = (flagged == 0).tolist() if ordered else [False] * len(per_batch)
+    return (quiet, np.add.reduceat(du, heads).tolist(),
+            np.add.reduceat(du * (bounds[:, 3] - bounds[:, 2]), heads).tolist(),
+            np.add.reduceat(counters[:, :ng + 1], heads, axis=0).tolist())
+
+
+def _pass_inputs(matcher: MaskedMatcher, af: GPUFocus, bf: GPUFocus, batches: list[tuple[int, int]],
+                 ranges: list[list[tuple[int, int, int, int]]], window_cap: int | None) -> tuple[Any, ...]:
+    """The host arrays of one device pass over consecutive batches and the
+    arguments of its `pcube_lease` call (their windows, bounds, matches and
+    counters, with one host synchronisation)."""
     width = PROFILE['tile_width']
     first, end = batches[0][0], batches[-1][1]
     tiles = np.array([x for r in ranges for t in r for x in t], dtype=np.uint64)
@@ -188,21 +251,29 @@ def _lease_pass(matcher: MaskedMatcher, af: GPUFocus, bf: GPUFocus, batches: lis
     counters = np.zeros(ntot * (ng + 3), dtype=np.uint64)
     values = np.zeros(2 * (1 << 20), dtype=np.uint64)
     seconds = (ctypes.c_double * 4)()
+    args = (matcher.engine.handle, af.handle, bf.handle, len(batches), native.ptr32(per_batch),
+            native.ptr64(tiles), width, 8192, window_cap, native.ptr64(bounds), native.ptr64(sizes),
+            native.ptr64(counters), native.ptr64(values), values.size // 2, seconds)
+    arrays = (tiles, per_batch, bounds.reshape(ntot, 4), sizes, counters.reshape(ntot, ng + 3), values, seconds)
+    return arrays, args
+
+
+def _run_pass(check: Callable[[], None], call: Callable[..., int], args: tuple[Any, ...]) -> float:
+    """The worker's part of a pass: the library call between two budget checks.
+    Both check the library on this thread: its error text is per thread, and
+    the folding thread makes no call that waits for the library's lock."""
+    check()
     started = time.perf_counter()
-    matcher.runtime.check()
-    code = native.lib().pcube_lease(matcher.engine.handle, af.handle, bf.handle, len(batches),
-                                    native.ptr32(per_batch), native.ptr64(tiles), width, 8192, window_cap,
-                                    native.ptr64(bounds), native.ptr64(sizes), native.ptr64(counters),
-                                    native.ptr64(values), values.size // 2, seconds)
-    native.check(code, 'pcube_lease')
-    matcher.runtime.check()
-    return (tiles, bounds.reshape(ntot, 4), sizes, counters.reshape(ntot, ng + 3), values, list(seconds),
-            time.perf_counter() - started)
+    native.check(call(*args), 'pcube_lease')
+    seconds = time.perf_counter() - started
+    check()
+    return seconds
 
 
 def lease_rows(matcher: MaskedMatcher, af: GPUFocus, bf: GPUFocus, first: int, end: int,
                batch_width: int = BATCH_WIDTH,
-               window_cap: int | None = None) -> Iterator[tuple[int, int, dict[str, Any], bytes]]:
+               window_cap: int | None = None,
+               quiet: bool = False) -> Iterator[tuple[int, int, dict[str, Any] | QuietBatch, bytes]]:
     """Every batch row of [first, end), from pipelined device passes.
 
     The batches go to the device LEASE_CHUNK at a time, each group in one
@@ -212,7 +283,8 @@ def lease_rows(matcher: MaskedMatcher, af: GPUFocus, bf: GPUFocus, first: int, e
     field except the timings, which are its pass's phases divided evenly over
     the pass's batches.  With each row come the bytes of
     encoded(row['subtile_scopes']), formatted by the library from the same
-    counters and bounds.
+    counters and bounds.  With quiet, a batch whose subtiles emitted nothing
+    (and saw no underflow) comes as its QuietBatch totals instead of a row.
     """
     config = matcher.config
     width = PROFILE['tile_width']
@@ -222,14 +294,22 @@ def lease_rows(matcher: MaskedMatcher, af: GPUFocus, bf: GPUFocus, first: int, e
     groups = [[(s, min(s + batch_width, end)) for s in starts[i:i + LEASE_CHUNK]]
               for i in range(0, len(starts), LEASE_CHUNK)]
     pool = ThreadPoolExecutor(max_workers=1)
+    call = native.lib().pcube_lease
+    # the worker takes the interpreter lock only between passes, to return from one
+    # call and make the next; a short switch interval hands it over at once while
+    # this thread folds, where the default (5 ms) would leave the device idle
+    interval = sys.getswitchinterval()
+    sys.setswitchinterval(min(interval, 1e-4))
     try:
         pending = []
         for batches in groups:
             # the geometry of later groups is computed while the device runs the first
             ranges = [tile_ranges(config, s, e, width) for s, e in batches]
-            pending.append((batches, ranges, pool.submit(_lease_pass, matcher, af, bf, batches, ranges, window_cap)))
-        for batches, ranges, future in pending:
-            tiles, bounds, sizes, counters, values, seconds, device_seconds = future.result()
+            arrays, args = _pass_inputs(matcher, af, bf, batches, ranges, window_cap)
+            pending.append((batches, ranges, arrays, pool.submit(_run_pass, matcher.runtime.check, call, args)))
+        for batches, ranges, arrays, future in pending:
+            device_seconds = future.result()
+            tiles, _, bounds, sizes, counters, values, seconds = arrays
             n = len(batches)
             timings = dict(kernel_seconds=seconds[2] / n, focus_enumeration_seconds=seconds[0] / n,
                            transfer_seconds=max(0.0, device_seconds - seconds[0] - seconds[2]) / n)
@@ -239,9 +319,20 @@ def lease_rows(matcher: MaskedMatcher, af: GPUFocus, bf: GPUFocus, first: int, e
             emitted = [min(k, cap) for k in counters[:, ng + 1].tolist()]
             limbs = values[:2 * sum(emitted)].tolist()
             bound_rows = [tuple(r) for r in bounds.tolist()]
+            if quiet:
+                quiet_of, focus_of, pairs_of, sums_of = pass_totals([len(r) for r in ranges], bounds, counters, ng)
+                shares = dict(timings, verification_seconds=0.0, prefilter_build_seconds=seconds[1] / n)
+                filters = [list(group) for group in matcher.filter_groups]
             for b, (s, e) in enumerate(batches):
                 nt = len(ranges[b])
                 block = counters[t0:t0 + nt]
+                if quiet and quiet_of[b]:
+                    totals = QuietBatch(nt, (ranges[b][0][0], ranges[b][-1][1]), focus_of[b], int(sizes[2 * b + 1]),
+                                        pairs_of[b], sums_of[b][0], sums_of[b][1:], filters, shares)
+                    yield s, e, totals, scope_record_bytes(tile_ends[t0:t0 + nt], block, ng, None,
+                                                           EMPTY_DIGEST.encode() * nt)
+                    t0 += nt
+                    continue
                 per_tile = []
                 for k in emitted[t0:t0 + nt]:
                     per_tile.append([limbs[2 * i] + (limbs[2 * i + 1] << 64) for i in range(offset, offset + k)])
@@ -266,6 +357,7 @@ def lease_rows(matcher: MaskedMatcher, af: GPUFocus, bf: GPUFocus, first: int, e
     finally:
         # a caller that stops early (an error in its fold) leaves no pass behind
         pool.shutdown(wait=True, cancel_futures=True)
+        sys.setswitchinterval(interval)
 
 
 def compute_lease(matcher: MaskedMatcher, af: GPUFocus, bf: GPUFocus, first: int, end: int,
@@ -278,8 +370,11 @@ def compute_lease(matcher: MaskedMatcher, af: GPUFocus, bf: GPUFocus, first: int
     started = time.monotonic()
     acc = Accumulator(first, end)
     if pipelined:
-        for start, stop, row, records in lease_rows(matcher, af, bf, first, end):
-            acc.add(start, stop, row, records)
+        for start, stop, row, records in lease_rows(matcher, af, bf, first, end, quiet=True):
+            if isinstance(row, QuietBatch):
+                acc.add_quiet(start, stop, row, records)
+            else:
+                acc.add(start, stop, row, records)
             heartbeat()
     else:
         for start in range(first, end, BATCH_WIDTH):
