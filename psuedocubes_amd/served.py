@@ -35,6 +35,7 @@ import gc
 import math
 import random
 import statistics
+import sys
 import time
 from typing import Any, Callable
 
@@ -123,6 +124,11 @@ class Server:
         pool = ThreadPoolExecutor(max_workers=1)
         collecting = gc.isenabled()
         gc.disable()
+        # the device worker needs the interpreter lock to start and end each step; a
+        # short switch interval hands it over at once while this thread assembles rows,
+        # where the default (5 ms) would delay every step that long
+        interval = sys.getswitchinterval()
+        sys.setswitchinterval(min(interval, 1e-4))
 
         def now() -> float:
             return clock() - start
@@ -198,6 +204,7 @@ class Server:
                     self.overlap_seconds += now() - t0
         finally:
             pool.shutdown(wait=True)
+            sys.setswitchinterval(interval)
             if collecting:
                 gc.enable()
         return done
