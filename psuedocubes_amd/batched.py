@@ -180,13 +180,15 @@ class BatchedEngine:
             engine = self.engines[window.p]
             nt, stride = len(ranges), engine.ng + 3
             block = counters[c0:c0 + nt * stride].reshape(nt, stride)
+            kept_of = [min(k, self.cap) for k in block[:, engine.ng + 1].tolist()]
+            limbs = values[2 * v0:2 * (v0 + sum(kept_of))].tolist()
             per_tile = []
-            for t in range(nt):
-                kept = min(int(block[t, engine.ng + 1]), self.cap)
-                limbs = values[2 * v0:2 * (v0 + kept)]
-                per_tile.append([int(limbs[2 * i]) + (int(limbs[2 * i + 1]) << 64) for i in range(kept)])
-                v0 += kept
-            bnd = [tuple(int(x) for x in bounds[t0 + t]) for t in range(nt)]
+            at = 0
+            for kept in kept_of:
+                per_tile.append([limbs[2 * i] + (limbs[2 * i + 1] << 64) for i in range(at, at + kept)])
+                at += kept
+            v0 += at
+            bnd = [tuple(r) for r in bounds[t0:t0 + nt].tolist()]
             row = batch_row(self.configs[window.p], engine.filter_groups, ranges, bnd, int(sizes[2 * k + 1]), block,
                             per_tile, self.cap, timings, time.perf_counter(), witnesses=native_witnesses)
             row['matching_layout'] = 'batched-direct' if mode else 'batched-residue-bitset'

@@ -8,6 +8,7 @@ constrain v, and every other prime up to p is a residual filter applied to x.
 from __future__ import annotations
 
 from dataclasses import dataclass
+from functools import lru_cache
 from math import gcd, prod
 from typing import Any, Iterator
 
@@ -113,6 +114,12 @@ class Focus:
         return first, second
 
 
+@lru_cache(maxsize=64)
+def _inverse(a: int, m: int) -> int:
+    """a^-1 mod m, once per (a, m): host verification asks for it per emitted value."""
+    return pow(a, -1, m)
+
+
 @dataclass(frozen=True)
 class Config:
     """One search definition: prime bound p, interval (lo, hi], and the A/B split."""
@@ -145,7 +152,7 @@ class Config:
 
     def canonical(self, n: int) -> tuple[int, int]:
         """(u, v) with n = u*B - v*A and 0 <= v < B (`pseudocube/focus.py:116-118`)."""
-        v = (-n * pow(self.A, -1, self.B)) % self.B if self.B > 1 else 0
+        v = (-n * _inverse(self.A, self.B)) % self.B if self.B > 1 else 0
         return (n + v * self.A) // self.B, v
 
     def u_interval(self, v: int) -> tuple[int, int]:

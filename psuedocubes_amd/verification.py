@@ -20,11 +20,8 @@ CLAIM = 'Candidate validity only; no minimality claim.'
 def native_check(values: list[int], p: int) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
     """(local3 flags, perfect-cube flags, floor cube roots) for each value."""
     n = len(values)
-    limbs = np.zeros(2 * max(1, n), dtype=np.uint64)
     mask = (1 << 64) - 1
-    for i, v in enumerate(values):
-        limbs[2 * i] = v & mask
-        limbs[2 * i + 1] = v >> 64
+    limbs = np.array([x for v in values for x in (v & mask, v >> 64)] or [0, 0], dtype=np.uint64)
     ok = np.zeros(max(1, n), dtype=np.uint8)
     cube = np.zeros(max(1, n), dtype=np.uint8)
     root = np.zeros(max(1, n), dtype=np.uint64)
@@ -41,6 +38,6 @@ def native_witnesses(values: list[int], p: int) -> list[dict[str, object]]:
     ok, cube, root = native_check(values, p)
     if not ok.all():
         raise ArithmeticError('GPU/CPU disagreement')
-    return [dict(n=str(n), p=p, cube_root_floor=str(int(r)), perfect_cube=False, failing_primes=[],
-                 valid_pseudocube=True, claim=CLAIM)
-            for n, c, r in zip(values, cube, root) if not c]
+    return [{'n': str(n), 'p': p, 'cube_root_floor': str(r), 'perfect_cube': False, 'failing_primes': [],
+             'valid_pseudocube': True, 'claim': CLAIM}
+            for n, c, r in zip(values, cube.tolist(), root.tolist()) if not c]
