@@ -23,11 +23,13 @@ One `run` call:
 """
 from __future__ import annotations
 
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass
 import ctypes
 import gc
 import json
 from pathlib import Path
+import sys
 import time
 from typing import Any, Iterable
 
@@ -41,6 +43,7 @@ from .verification import native_witnesses
 
 TUNING = Path(__file__).resolve().parent / 'tuning.json'
 DEFAULT_TUNING = dict(rows_per_chunk=8, blocks_per_cu=4, direct_below_rows=96)
+GROUP_WINDOWS = 1024   # windows per device pass of run(): one pass runs while the previous is assembled
 
 
 def load_tuning() -> dict[str, int]:
@@ -103,35 +106,45 @@ class BatchedEngine:
     def run(self, windows: list[Window], budget: int | None = None) -> list[dict[str, Any]]:
         """One result row per window, in input order.
 
-        The cyclic garbage collector is paused meanwhile: the rows hold no
-        reference cycles, and full collections over thousands of fresh rows
-        only add pauses."""
+        Windows go to the device in groups of at most GROUP_WINDOWS within the
+        memory budget, each one device pass (launch) on a worker thread, in
+        order; this thread assembles a group's rows while the device runs the
+        next.  The cyclic garbage collector is paused meanwhile (the rows hold
+        no reference cycles, and full collections over thousands of fresh rows
+        only add pauses), and Python's switch interval is 0.1 ms, so the
+        worker gets the interpreter lock between passes at once."""
         if budget is None:
             budget = max(256 << 20, self.runtime.cap // 4)
         rows: list[dict[str, Any] | None] = [None] * len(windows)
+        groups: list[list[int]] = []
         group: list[int] = []
         used = 0
+        for index, window in enumerate(windows):
+            need = self.footprint(window)
+            if group and (used + need > budget or len(group) >= GROUP_WINDOWS):
+                groups.append(group)
+                group, used = [], 0
+            group.append(index)
+            used += need
+        if group:
+            groups.append(group)
         collecting = gc.isenabled()
         gc.disable()
+        interval = sys.getswitchinterval()
+        sys.setswitchinterval(min(interval, 1e-4))
+        pool = ThreadPoolExecutor(max_workers=1)
         try:
-            for index, window in enumerate(windows):
-                need = self.footprint(window)
-                if group and (used + need > budget or len(group) >= 65536):
-                    self._run_group(windows, group, rows)
-                    group, used = [], 0
-                group.append(index)
-                used += need
-            if group:
-                self._run_group(windows, group, rows)
+            passes = [pool.submit(self.launch, [windows[i] for i in g]) for g in groups]
+            for g, launched in zip(groups, passes):
+                for index, row in zip(g, self.assemble(launched.result())):
+                    rows[index] = row
         finally:
+            # an error leaves no pass behind
+            pool.shutdown(wait=True, cancel_futures=True)
+            sys.setswitchinterval(interval)
             if collecting:
                 gc.enable()
         return rows  # type: ignore[return-value]
-
-    def _run_group(self, windows: list[Window], group: list[int], rows: list[Any]) -> None:
-        raw = self.launch([windows[i] for i in group])
-        for index, row in zip(group, self.assemble(raw)):
-            rows[index] = row
 
     def launch(self, windows: list[Window]) -> dict[str, Any]:
         """Device pass over one group of windows; returns the raw results for assemble().
