@@ -295,6 +295,21 @@ int pcube_floor_div(const uint64_t *in, uint64_t *out, uint64_t count);
 int pcube_cube_count(uint64_t first, uint64_t last, uint32_t p, uint64_t *count, uint32_t *good, uint64_t good_words,
                      double *seconds);
 
+/* pcube_cube_count without device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops and harnesses that forbid hipMalloc).  Same
+ * arguments, limits and results as pcube_cube_count, without the timing.  Its
+ * working memory (prime table, counter, root bitset) is one static device
+ * array of 264,448 bytes, which each device holds until the process ends;
+ * nothing is allocated or freed.  Limit: with good given, the range may hold
+ * at most 2^21 odd roots (65,536 bitset words), else INVALID; a count without
+ * good has no limit beyond pcube_cube_count's.  Every call on a device shares
+ * that device's array and keeps the last call's contents: within a process
+ * the library lock and the call's final synchronisation order the calls;
+ * calls that overlapped on one device would race on it. */
+int pcube_cube_count_noalloc(uint64_t first, uint64_t last, uint32_t p, uint64_t *count, uint32_t *good,
+                             uint64_t good_words);
+
 /* --- measurement: calibration probe for the HBM byte counters ------------ */
 /* Copies `bytes` between two fresh device buffers with a plain streaming
  * kernel; *seconds is its event-timed duration.  No reference counterpart. */

@@ -27,6 +27,18 @@
 
 using namespace pcube;
 
+// Device scratch of pcube_cube_count_noalloc, for callers that cannot allocate
+// device memory.  Layout: the counter at byte 0, the odd primes <= p (at most
+// 302 for p <= 2000) at kNoAllocPrimes, the root bitset at kNoAllocGood.  Each
+// device holds its own copy for the life of the process; host code reaches it
+// only through the address hipGetSymbolAddress returns on that device.
+constexpr uint32_t kNoAllocPrimes = 256;                                  // byte offset
+constexpr uint32_t kNoAllocMaxPrimes = 512;
+constexpr uint32_t kNoAllocGood = kNoAllocPrimes + kNoAllocMaxPrimes * 4;  // byte offset, 2304
+constexpr uint32_t kNoAllocGoodWords = 1u << 16;                          // 2^21 odd roots
+constexpr uint32_t kNoAllocBytes = kNoAllocGood + kNoAllocGoodWords * 4;   // 264,448
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
 namespace {
 
 std::recursive_mutex g_mu;
@@ -66,6 +78,9 @@ struct Context {
     int compute_units = 0;
     uint32_t lds_per_block = 65536;                  // largest LDS one workgroup may use
     char name[256] = {0};
+    // device address of `scratch` per device, from hipGetSymbolAddress at init
+    static constexpr int kMaxDevices = 64;
+    void* scratch_at[kMaxDevices] = {};
 } g;
 
 // HIP's current device belongs to the calling thread: an entry point called
@@ -1155,6 +1170,14 @@ int pcube_init(int device, uint64_t memory_cap, double fraction, uint64_t* cap, 
         int lds = 0;
         HIP_TRY(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
         g.lds_per_block = (uint32_t)std::max(lds, 0);
+        // the allocation-free entry point's scratch: this device's copy of the
+        // array, looked up once with the device current
+        if (device < 0 || device >= Context::kMaxDevices)
+            return fail(PCUBE_INVALID, "device index %d beyond the library's %d", device, Context::kMaxDevices);
+        void* at = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&at, HIP_SYMBOL(scratch)));
+        if (!at) return fail(PCUBE_DEVICE, "no device address for the static scratch array");
+        g.scratch_at[device] = at;
         g.device = device;
     }
     size_t fr = 0, tot = 0;
@@ -2340,6 +2363,50 @@ int pcube_cube_count(uint64_t first, uint64_t last, uint32_t p, uint64_t* count,
     if (dcount) dev_free(dcount);
     if (dgood) dev_free(dgood);
     return rc;
+}
+
+int pcube_cube_count_noalloc(uint64_t first, uint64_t last, uint32_t p, uint64_t* count, uint32_t* good,
+                             uint64_t good_words) {
+    Entry lock;
+    if (first < 1 || last < first || last > UINT32_MAX || p < 2 || p > 2000)
+        return fail(PCUBE_INVALID, "cube count needs 1 <= first <= last <= 2^32 - 1 and 2 <= p <= 2000");
+    if (g.device < 0) return fail(PCUBE_INVALID, "pcube_init has not been called");
+    uint8_t* base = (uint8_t*)g.scratch_at[g.device];
+    if (!base) return fail(PCUBE_DEVICE, "the static scratch array has no address on device %d", g.device);
+    // the odd primes up to p, on the host stack (no allocation)
+    uint32_t primes[kNoAllocMaxPrimes];
+    uint32_t nprimes = 0;
+    for (uint32_t q = 3; q <= p; q += 2) {
+        bool prime = true;
+        for (uint32_t d = 3; d * d <= q && prime; d += 2) prime = q % d != 0;
+        if (prime) primes[nprimes++] = q;   // p <= 2000 keeps nprimes <= 302
+    }
+    const uint64_t r0 = first | 1u;
+    const uint64_t nodd = r0 > last ? 0 : (last - r0) / 2 + 1;
+    const uint64_t need_words = (nodd + 31) / 32;
+    if (good && good_words < need_words)
+        return fail(PCUBE_INVALID, "root bitset needs %llu words", (unsigned long long)need_words);
+    if (good && need_words > kNoAllocGoodWords)
+        return fail(PCUBE_INVALID, "a listed range of %llu odd roots exceeds the static scratch maximum of %u",
+                    (unsigned long long)nodd, kNoAllocGoodWords * 32);
+    unsigned long long* dcount = (unsigned long long*)base;
+    uint32_t* dprimes = (uint32_t*)(base + kNoAllocPrimes);
+    uint32_t* dgood = good && need_words ? (uint32_t*)(base + kNoAllocGood) : nullptr;
+    // the array keeps the previous call's contents: the counter starts at zero,
+    // the prime table and every root word the kernel reports are rewritten
+    if (nprimes) HIP_TRY(hipMemcpyAsync(dprimes, primes, nprimes * 4, hipMemcpyHostToDevice, g.stream));
+    TRY(fill64((uint64_t*)dcount, 1, 0));
+    if (nodd)
+        hipLaunchKernelGGL(k_cube_count, dim3((uint32_t)((nodd + kCubeSegment - 1) / kCubeSegment)),
+                           dim3(kCubeThreads), 0, g.stream, r0, nodd, (const uint32_t*)dprimes, nprimes, dcount,
+                           dgood);
+    HIP_TRY(hipGetLastError());
+    uint64_t total = 0;
+    HIP_TRY(hipMemcpyAsync(&total, dcount, 8, hipMemcpyDeviceToHost, g.stream));
+    if (dgood) HIP_TRY(hipMemcpyAsync(good, dgood, need_words * 4, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    if (count) *count = total;
+    return PCUBE_OK;
 }
 
 int pcube_copy_probe(uint64_t bytes, double* seconds) {

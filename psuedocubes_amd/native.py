@@ -66,6 +66,7 @@ SIGNATURES: dict[str, list[Any]] = {
     'pcube_lds_probe': [u32, u32, P_dbl, P_dbl],
     'pcube_valu_probe': [u32, P_dbl, P_dbl],
     'pcube_cube_count': [u64, u64, u32, P_u64, P_u32, u64, P_dbl],
+    'pcube_cube_count_noalloc': [u64, u64, u32, P_u64, P_u32, u64],
 }
 
 _lib: ctypes.CDLL | None = None
