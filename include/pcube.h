@@ -8,7 +8,7 @@
  *   - Every call returns a status code; pcube_last_error() describes the most
  *     recent failure of the calling thread.  The Python layer maps the codes to the reference's
  *     exceptions: OVERFLOW -> OverflowError, INVALID -> ValueError,
- *     UNDERFLOW/RANGE -> ArithmeticError, NOMEM -> MemoryError.
+ *     UNDERFLOW/RANGE/MISMATCH -> ArithmeticError, NOMEM -> MemoryError.
  *   - All work runs on one library-owned HIP stream of default priority; each
  *     entry point returns after its results are on the host unless it says
  *     otherwise.  Calls may come from any host thread: one library lock
@@ -33,7 +33,8 @@ enum {
     PCUBE_UNDERFLOW = 3,  /* unsigned underflow flagged while building x      */
     PCUBE_NOMEM = 4,      /* device memory cap exceeded                        */
     PCUBE_DEVICE = 5,     /* HIP runtime failure                               */
-    PCUBE_RANGE = 6       /* an emitted x fell outside (lo, hi]               */
+    PCUBE_RANGE = 6,      /* an emitted x fell outside (lo, hi]               */
+    PCUBE_MISMATCH = 7    /* the survivor list disagrees with the counters    */
 };
 
 const char *pcube_last_error(void);
@@ -151,6 +152,15 @@ int pcube_matcher_create(uint64_t A, uint64_t B, uint64_t lo0, uint64_t lo1, uin
                          const uint32_t *moduli, uint32_t nres, int *matcher);
 int pcube_matcher_destroy(int matcher);
 
+/* Overwrite the matcher's device allowed-residue tables (the allowed bits of
+ * every filter and the bitset builders' permuted copies) with all residues
+ * allowed (value 1) or none (value 0).  Every later batch of the matcher, in
+ * any engine, reads the mutated tables.  For tests that a corrupted device
+ * table cannot reach a receipt: the reference fills its device byte table
+ * with 1s and requires host verification to raise ArithmeticError
+ * (REL!tools/gpu_worker.py:96-103, `bad.table.fill(1)`). */
+int pcube_matcher_table_fill(int matcher, int value);
+
 /* Match one batch of up to 64 subtiles (or one subtile with ntiles = 1).
  *   us/vs        device addresses of sorted u and v coordinates
  *   v_first      tile_id = (v - v_first) / tile_width must be < ntiles
@@ -164,7 +174,8 @@ int pcube_matcher_destroy(int matcher);
  *                residue cache), matching (every kernel after the prefilter),
  *                and of that the staged match kernel alone (engine 0)
  * Returns OK, OVERFLOW (a subtile emitted more than cap; counters valid),
- * UNDERFLOW, RANGE, INVALID. */
+ * UNDERFLOW, RANGE, MISMATCH (the survivor list disagrees with the emitted
+ * counts; counters valid, values not), INVALID. */
 int pcube_match(int matcher, uint64_t us, uint64_t nu, uint64_t vs, uint64_t nv, uint64_t v_first,
                 uint64_t tile_width, uint32_t ntiles, uint64_t cap, int engine, uint64_t *counters,
                 uint64_t *values, uint64_t values_cap, double *seconds);

@@ -225,6 +225,9 @@ def batch_row(config: Config, filter_groups: list[list[int]], ranges: list[tuple
         raise ArithmeticError('unsigned underflow')
     if emitted and max(emitted) > output_cap:
         raise OverflowError('fused tile output overflow; no truncated coverage')
+    # every counted value must have come back, and nothing else
+    if len(per_tile) != nrows or any(len(values) != n for values, n in zip(per_tile, emitted)):
+        raise ArithmeticError('emitted values disagree with the subtile counters')
     all_values: list[int] = []
     for values in per_tile:
         if values:
@@ -290,6 +293,11 @@ class FusedMatcher:
         self.sm_count = runtime.compute_units
         self.prefilter_seconds = 0.0
         self.intersection_seconds = 0.0   # the match kernel alone (intersection and chain) of the last run
+
+    @property
+    def table(self) -> Any:
+        """The device allowed-residue tables every engine of this matcher reads."""
+        return self.engine.table
 
     def run(self, batch: Prepared, output_cap: int = OUTPUT_CAP) -> dict[str, Any]:
         started = time.perf_counter()

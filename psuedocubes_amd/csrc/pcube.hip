@@ -938,10 +938,18 @@ int collect_values(Matcher& mt, uint64_t ntiles_total, const uint64_t* host_coun
         offset[t + 1] = offset[t] + std::min(emitted, cap);
     }
     expected = offset[ntiles_total];
-    if (listed != expected || listed > list_cap) {
+    if (expected > list_cap) {
+        // more survivors than the shared list holds (no subtile above cap): capacity, not disagreement
         *status = PCUBE_OVERFLOW;
-        fail(PCUBE_OVERFLOW, "survivor list holds %llu values, counters say %llu (list capacity %llu)",
-             (unsigned long long)listed, (unsigned long long)expected, (unsigned long long)list_cap);
+        fail(PCUBE_OVERFLOW, "survivor list capacity %llu below the %llu values the counters report",
+             (unsigned long long)list_cap, (unsigned long long)expected);
+        return PCUBE_OK;
+    }
+    if (listed != expected) {
+        // every emitted value is counted and listed once: a difference means lost or extra values
+        *status = PCUBE_MISMATCH;
+        fail(PCUBE_MISMATCH, "survivor list holds %llu values, counters say %llu",
+             (unsigned long long)listed, (unsigned long long)expected);
         return PCUBE_OK;
     }
     if (expected > values_cap) return fail(PCUBE_OVERFLOW, "host value buffer too small");
@@ -1718,6 +1726,21 @@ int pcube_matcher_create(uint64_t A, uint64_t B, uint64_t lo0, uint64_t lo1, uin
     return PCUBE_OK;
 }
 
+int pcube_matcher_table_fill(int matcher, int value) {
+    Entry lock;
+    Matcher* mt = matcher_at(matcher);
+    if (!mt) return fail(PCUBE_INVALID, "unknown matcher");
+    if (value != 0 && value != 1) return fail(PCUBE_INVALID, "a table fill takes 0 or 1");
+    const uint32_t word = value ? 0xffffffffu : 0u;
+    for (Buffer* b : {&mt->d_allow, &mt->d_pstr})
+        if (b->p && b->bytes / 4)
+            hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(b->bytes / 4, 256)), dim3(256), 0, g.stream,
+                               b->as<uint32_t>(), b->bytes / 4, word);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    return PCUBE_OK;
+}
+
 int pcube_matcher_destroy(int matcher) {
     Entry lock;
     if (!matcher_at(matcher)) return fail(PCUBE_INVALID, "unknown matcher");
@@ -2199,7 +2222,10 @@ int pcube_batch_run(int engine, uint32_t nwin, const uint32_t* win_cfg, const ui
         voff[t + 1] = voff[t] + std::min(emitted, e.cap);
     }
     if (status) return status;
-    if (listed != voff[ntot] || listed > e.list_cap) return fail(PCUBE_OVERFLOW, "survivor list overflow");
+    if (voff[ntot] > e.list_cap) return fail(PCUBE_OVERFLOW, "survivor list overflow");
+    if (listed != voff[ntot])
+        return fail(PCUBE_MISMATCH, "survivor list holds %llu values, counters say %llu", (unsigned long long)listed,
+                    (unsigned long long)voff[ntot]);
     if (listed > values_cap) return fail(PCUBE_OVERFLOW, "host value buffer too small");
     if (listed) {
         std::vector<uint64_t> triples(listed * 3);
@@ -2248,8 +2274,9 @@ int pcube_host_verify(const uint64_t* limbs, uint64_t n, uint32_t p, uint8_t* lo
                 if (acc != 1) ok = false;
             }
         }
-        // floor cube root by bisection on [0, 2^43) (x < 2^128 needs < 2^43)
-        uint64_t lo = 0, hi = 1ull << 43;
+        // floor cube root by bisection on [0, 6981463658332): 6981463658331^3 < 2^128 <= 6981463658332^3,
+        // so every probe's cube fits 128 bits
+        uint64_t lo = 0, hi = 6981463658332ull;
         while (hi - lo > 1) {
             const uint64_t mid = (lo + hi) >> 1;
             const u128 cube = (u128)mid * mid * mid;

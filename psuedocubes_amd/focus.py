@@ -93,6 +93,25 @@ class Focus:
         for index in range(start, end):
             yield period * self.modulus + self.at(index)
 
+    def window(self, low: int, high: int) -> Iterator[int]:
+        """Focus coordinates in [low, high), a CPU stream for small focuses (`pseudocube/focus.py:72-79`).
+
+        Same values as the reference's stream, in ascending order: the period's
+        coordinates are formed once by CRT composition, ring by ring, and sorted;
+        each period of the window then yields its in-range shifts."""
+        if low < 0 or high < low:
+            raise ValueError('invalid half-open window')
+        if high == low:
+            return
+        period_values = _period_coordinates(self)
+        for period in range(low // self.modulus, (high - 1) // self.modulus + 1):
+            base = period * self.modulus
+            for value in period_values:
+                if value + base >= high:
+                    break
+                if value + base >= low:
+                    yield value + base
+
     def halves(self) -> tuple[tuple[tuple[int, ...], ...], tuple[tuple[int, ...], ...]]:
         """Split the rings into two groups with the smallest larger product.
 
@@ -112,6 +131,16 @@ class Focus:
         first = tuple(t for i, t in enumerate(terms) if best_mask >> i & 1)
         second = tuple(t for i, t in enumerate(terms) if not best_mask >> i & 1)
         return first, second
+
+
+@lru_cache(maxsize=16)
+def _period_coordinates(focus: Focus) -> tuple[int, ...]:
+    """Every coordinate of one period of a focus, ascending: the CRT sums of one
+    residue per ring, built ring by ring as {s + t mod M : s so far, t in the ring's terms}."""
+    sums = [0]
+    for ring in focus.terms:
+        sums = [(s + term) % focus.modulus for s in sums for term in ring]
+    return tuple(sorted(sums))
 
 
 @lru_cache(maxsize=64)
@@ -223,6 +252,23 @@ class Tile:
 
     def manifest(self, config: Config) -> dict[str, Any]:
         return dict(kind='v-window', first=str(self.first), end=str(self.end), envelope=self.envelope(config))
+
+
+def enumerate_cpu(config: Config, tile: Tile) -> Iterator[int]:
+    """Every x = uB - vA of a tile that passes all residual filters, on the CPU
+    (`pseudocube/focus.py:170-179`): v over the B-focus window of the tile, u over
+    the A-focus window of v's exact interval, each residual prime's allowed
+    residues as a set.  For small configurations (tests and regression checks)."""
+    tile.validate(config)
+    af, bf = config.focuses()
+    filters = [(q, frozenset(allowed(q))) for q in config.residuals()]
+    for v in bf.window(tile.first, tile.end):
+        low, high = config.u_interval(v)
+        shift = v * config.A
+        for u in af.window(low, high):
+            x = u * config.B - shift
+            if all(x % q in residues for q, residues in filters):
+                yield x
 
 
 def targeted_coordinates(config: Config, n: int, radius: int = 1) -> tuple[list[int], list[int]]:

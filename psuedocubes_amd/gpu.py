@@ -308,6 +308,11 @@ class Matcher:
         self.filter_groups = self.engine.filter_groups
         self.kernel = runtime.compile('match_direct')
 
+    @property
+    def table(self) -> DeviceTable:
+        """The device allowed-residue tables (`pseudocube/gpu.py:199`, the reference's `self.table`)."""
+        return self.engine.table
+
     def arrays(self, us: DeviceArray, vs: DeviceArray, cap: int = 8192) -> dict[str, Any]:
         started = time.perf_counter()
         ng = len(self.filter_groups)
@@ -374,6 +379,19 @@ class Matcher:
         return result
 
 
+class DeviceTable:
+    """A matcher's device allowed-residue tables, with the one operation the
+    reference applies to its device byte table, `fill` (`REL!tools/gpu_worker.py:96-103`):
+    a nonzero value marks every residue of every filter allowed, zero none.
+    Every later batch of the matcher reads the filled tables."""
+
+    def __init__(self, engine: 'MatchEngine') -> None:
+        self.engine = engine
+
+    def fill(self, value: int) -> None:
+        native.call('pcube_matcher_table_fill', self.engine.handle, 1 if value else 0)
+
+
 class MatchEngine:
     """One configuration's filter tables resident on the device (a libpcube matcher)."""
 
@@ -390,6 +408,7 @@ class MatchEngine:
         native.call('pcube_matcher_create', config.A, config.B, config.lo & MASK, config.lo >> 64,
                     config.hi & MASK, config.hi >> 64, native.ptr32(moduli), len(order), ctypes.byref(handle))
         self.handle = handle.value
+        self.table = DeviceTable(self)
         self._values: np.ndarray | None = None
         self._values_ptr: Any = None
         self._counters = np.empty(0, dtype=np.uint64)   # host copy of the counters, reused
@@ -442,7 +461,9 @@ class MatchEngine:
                                                values.size // 2, seconds)
         # the caller owns its counters: a copy of the reused host buffer
         counters = self._counters[:ntiles * stride].reshape(ntiles, stride).copy()
-        if code == native.OVERFLOW and overflow_ok:
+        # overflow_ok covers a subtile that emitted more than cap, which the caller
+        # reports itself; any other overflow (or a list/counter mismatch) raises here
+        if code == native.OVERFLOW and overflow_ok and bool((counters[:, self.ng + 1] > cap).any()):
             return counters, [[] for _ in range(ntiles)], [seconds[0], seconds[1], seconds[2]]
         native.check(code, 'pcube_match')
         self.runtime.check()

@@ -13,7 +13,7 @@ import numpy as np
 
 LIBRARY = Path(__file__).resolve().parent / 'libpcube.so'
 
-OK, OVERFLOW, INVALID, UNDERFLOW, NOMEM, DEVICE, RANGE = range(7)
+OK, OVERFLOW, INVALID, UNDERFLOW, NOMEM, DEVICE, RANGE, MISMATCH = range(8)
 
 u64 = ctypes.c_uint64
 u32 = ctypes.c_uint32
@@ -50,6 +50,7 @@ SIGNATURES: dict[str, list[Any]] = {
     'pcube_searchsorted': [u64, u64, P_u64, u64, P_u64],
     'pcube_matcher_create': [u64, u64, u64, u64, u64, u64, P_u32, u32, P_int],
     'pcube_matcher_destroy': [c_int],
+    'pcube_matcher_table_fill': [c_int, c_int],
     'pcube_match': [c_int, u64, u64, u64, u64, u64, u64, u32, u64, c_int, P_u64, P_u64, u64, P_dbl],
     'pcube_match_start': [c_int, u64, u64, u64, u64, u64, u64, u32, u64, c_int],
     'pcube_match_finish': [c_int, P_u64, P_u64, u64, P_dbl],
@@ -103,7 +104,7 @@ def check(code: int, what: str = '') -> int:
         raise OverflowError(message)
     if code == INVALID:
         raise ValueError(message)
-    if code in (UNDERFLOW, RANGE):
+    if code in (UNDERFLOW, RANGE, MISMATCH):
         raise ArithmeticError(message)
     if code == NOMEM:
         raise MemoryError(message)

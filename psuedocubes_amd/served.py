@@ -40,7 +40,7 @@ import time
 from typing import Any, Callable
 
 from .batched import BatchedEngine, Window
-from .focus import Config
+from .focus import Config, Tile
 
 MERGED_COUNTS = ('focus_u', 'focus_v', 'cartesian_opportunities', 'valid_focused_candidates', 'genuine_cubes',
                  'local_survivors', 'cube_checks')
@@ -63,6 +63,7 @@ class Completed:
     row: dict[str, Any]
     admitted: float
     finished: float
+    error: str | None = None   # set for a request rejected at admission; row is then its rejection record
 
     @property
     def latency(self) -> float:
@@ -103,6 +104,18 @@ class Server:
         self.step_sizes: list[int] = []
         self.overlap_seconds = 0.0
 
+    def validate(self, request: Request) -> str | None:
+        """Why a request cannot be served (a prime bound the engine lacks, or not a
+        nonempty canonical v range), or None."""
+        config = self.engine.configs.get(request.p)
+        if config is None:
+            return f'no resident tables for p={request.p}'
+        try:
+            Tile(request.first, request.end).validate(config)
+        except ValueError as error:
+            return str(error)
+        return None
+
     def pieces(self, request: Request, width: int = 100_000_000) -> deque:
         span = self.piece_tiles * width
         return deque(Window(s, min(s + span, request.end), request.p, width)
@@ -140,6 +153,17 @@ class Server:
                 waiting.append(pending.popleft())
             while waiting and len(active) < self.max_slots:
                 request = waiting[0]
+                error = self.validate(request)
+                if error is not None:
+                    # one malformed request is answered on its own; the loop and the
+                    # other requests go on
+                    waiting.popleft()
+                    t = now()
+                    rejected = Completed(request, dict(status='REJECTED', error=error), t, t, error)
+                    done.append(rejected)
+                    if on_complete:
+                        on_complete(rejected)
+                    continue
                 pieces = self.pieces(request)
                 reserve = max(self.engine.footprint(w) for w in pieces)
                 if active and used + reserve > self.budget:

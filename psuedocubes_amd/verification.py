@@ -38,6 +38,13 @@ def native_witnesses(values: list[int], p: int) -> list[dict[str, object]]:
     ok, cube, root = native_check(values, p)
     if not ok.all():
         raise ArithmeticError('GPU/CPU disagreement')
-    return [{'n': str(n), 'p': p, 'cube_root_floor': str(r), 'perfect_cube': False, 'failing_primes': [],
-             'valid_pseudocube': True, 'claim': CLAIM}
-            for n, c, r in zip(values, cube.tolist(), root.tolist()) if not c]
+    # every value passed local3(n, p), so n > 0, n mod 9 is 1 or 8, and no prime <= p
+    # divides n or (for q = 1 mod 3) fails the cubic-residue test: verify() would list
+    # no failing prime, and a noncube among them is a valid pseudocube
+    out = []
+    for n, local, c, r in zip(values, ok.tolist(), cube.tolist(), root.tolist()):
+        if c:
+            continue
+        out.append({'n': str(n), 'p': p, 'cube_root_floor': str(r), 'perfect_cube': False, 'failing_primes': [],
+                    'valid_pseudocube': bool(local), 'claim': CLAIM})
+    return out

@@ -287,6 +287,33 @@ def test_pipelined_lease_over_several_passes(runtime, config619, focuses619, mas
     assert a['compute_batches'] == 7 and a['subtiles_executed'] == 6 * 64 + 3
 
 
+def test_matcher_coordinates(runtime) -> None:
+    """Matcher.coordinates (`pseudocube/gpu.py:236-244`) on host coordinate lists:
+    every (u, v) pair with x = uB - vA in (lo, hi] is visited and the values
+    passing the filters are exactly the local3 ones among them; the reference's
+    complete case (every u and v of Config(13, 0, 100000, 14, 117)) gives the
+    whole local3 set; a forced output overflow raises."""
+    from psuedocubes_amd.focus import targeted_coordinates
+    config = Config(7, 0, 100000, 14, 9)
+    matcher = Matcher(runtime, config)
+    for n in (1, 125, 2197, 4913, 6859):
+        us, vs = targeted_coordinates(config, n, radius=3)
+        xs = [u * config.B - v * config.A for u in us for v in vs]
+        inside = [x for x in xs if config.lo < x <= config.hi]
+        row = matcher.coordinates(us, vs)
+        assert row['valid_focused_candidates'] == len(inside)
+        assert row['all_values'] == [str(x) for x in sorted(x for x in inside if local3(x, 7))]
+        assert row['focus_u'] == len(us) and row['focus_v'] == len(vs)
+    full = Config(13, 0, 100000, 14, 117)
+    fa, fb = full.focuses()
+    us = list(fa.window(full.u_interval(0)[0], full.u_interval(full.B - 1)[1]))
+    vs = list(fb.window(0, fb.modulus))
+    row = Matcher(runtime, full).coordinates(us, vs)
+    assert row['all_values'] == [str(n) for n in range(1, 100001) if local3(n, 13)]
+    with pytest.raises(OverflowError):
+        Matcher(runtime, full).coordinates(us, vs, cap=0)
+
+
 def test_legacy_matcher_tile_and_split_replay(runtime) -> None:
     config = small_config(7)
     af, bf = (GPUFocus(runtime, f) for f in config.focuses())
@@ -341,5 +368,20 @@ def test_reproduce_compare_baseline(tmp_path) -> None:
     assert record['baseline_compared'] is True
     assert record['result']['compute_batches'] == 2 and record['result']['subtiles_executed'] == 128
     assert record['result']['valid_focused_candidates'] == sum(b['valid_focused_candidates'] for b in LEASE_BATCHES[:2])
+    # the transcript chains the two batches' encoded scope lists: recompute it from the
+    # oracle's scopes, each of which must hash to the golden per-batch value first
+    config = Config(619, 10**27, 3 * 10**27, A619, B619)
+    fa, fb = (O.Focus(f) for f in config.focuses())
+    chained = hashlib.sha256()
+    values = []
+    for golden in LEASE_BATCHES[:2]:
+        row = O.batch(config, fa, fb, int(golden['first']), int(golden['end']))
+        assert scopes_hash(row) == golden['scopes_sha256']
+        chained.update(encoded(row['subtile_scopes']))
+        values += row['all_values']
+    scope = record['result']['subtile_scopes'][0]
+    assert scope['transcript_sha256'] == chained.hexdigest()
+    assert scope['output_sha256'] == hashlib.sha256(encoded(values)).hexdigest()
+    assert record['result']['all_values'] == values
     with pytest.raises(ValueError):
         reproduce.main(['--first', '0', '--batches', '1', '--output', str(out)])

@@ -234,3 +234,47 @@ def test_library_tile_ranges_equal_python() -> None:
     for first, end, width in ((5, 5, 1), (0, B619 + 1, 10**8), (0, 65, 1)):
         assert native.lib().pcube_tile_ranges(A619, B619, 0, 0, 1, 0, first, end, width, native.ptr64(tiles),
                                               ctypes.byref(n)) == native.INVALID
+
+
+def test_focus_window_equals_brute_force() -> None:
+    """Focus.window (`pseudocube/focus.py:72-79`) against the brute-force set of
+    coordinates: windows inside one period, across periods, empty and at 0."""
+    from psuedocubes_amd.focus import Focus  # noqa: F401  (the class the drop-in exports)
+    for p in (7, 13, 31):
+        config = small_config(p)
+        for focus in config.focuses():
+            members = {focus.at(i) for i in range(focus.count)}
+            M = focus.modulus
+            for low, high in [(0, M), (0, 3 * M + 5), (M - 3, 2 * M + 3), (5, 5), (7, 8), (2 * M, 2 * M + M // 2)]:
+                got = list(focus.window(low, high))
+                expected = [y for y in range(low, high) if y % M in members]
+                assert got == expected, (p, M, low, high)
+            with pytest.raises(ValueError):
+                list(focus.window(5, 4))
+
+
+def test_enumerate_cpu_complete_small_sets_and_partitions() -> None:
+    """enumerate_cpu (`pseudocube/focus.py:170-179`) as the reference's own test uses it
+    (tests/test_pseudocube_math.py:74-88): the complete set of x <= 1e5 passing
+    local3, no duplicates, the same set over a 7-way partition and over the two
+    halves of Tile.split, and the cubes among them the cubes of roots coprime to p#."""
+    from psuedocubes_amd.focus import enumerate_cpu
+    for p in (7, 13, 19, 31):
+        config = small_config(p)
+        expected = {n for n in range(1, 100001) if local3(n, p)}
+        actual = list(enumerate_cpu(config, Tile(0, config.B)))
+        assert len(actual) == len(set(actual)) and set(actual) == expected
+        pieces: list[int] = []
+        step = max(1, config.B // 7)
+        for first in range(0, config.B, step):
+            pieces.extend(enumerate_cpu(config, Tile(first, min(config.B, first + step))))
+        assert len(pieces) == len(set(pieces)) and set(pieces) == expected
+        a, b = Tile(0, config.B).split()
+        assert set(enumerate_cpu(config, a)) | set(enumerate_cpu(config, b)) == expected
+        assert {n for n in actual if is_perfect_cube(n)} == {
+            r**3 for r in range(1, icbrt(100000) + 1) if gcd(r, prod(primes_upto(p))) == 1}
+    # endpoints: lo excluded, hi included (tests/test_pseudocube_math.py:116-117)
+    for n in (1, 125, 343, 1001):
+        if local3(n, 7):
+            assert n in list(enumerate_cpu(small_config(7, 0, n), Tile(0, 9)))
+            assert n not in list(enumerate_cpu(small_config(7, n, n + 100), Tile(0, 9)))

@@ -41,6 +41,7 @@ def test_status_codes_map_to_reference_exceptions() -> None:
     import pytest
     for code, exc in ((native.OVERFLOW, OverflowError), (native.INVALID, ValueError),
                       (native.UNDERFLOW, ArithmeticError), (native.RANGE, ArithmeticError),
+                      (native.MISMATCH, ArithmeticError),
                       (native.NOMEM, MemoryError), (native.DEVICE, RuntimeError)):
         with pytest.raises(exc):
             native.check(code)

@@ -137,3 +137,45 @@ def test_oracle_reproduces_reference_lease(config619: Config, oracle_focuses619)
     assert result['output_sha256'] == ref['subtile_scopes'][0]['output_sha256']
     assert result['transcript_sha256'] == ref['subtile_scopes'][0]['transcript_sha256']
     assert seen == [b['scopes_sha256'] for b in batches]
+
+
+def test_prime_bound_nesting_invariants(config619: Config, oracle_focuses619) -> None:
+    """The oracle's p-generic path at p = 313 and 433, pinned against p = 619 (which
+    the reference lease pins) by what the mathematics fixes for one window:
+
+    * A, B and the focus wheels are the same for every p, so the focused
+      positions (focus_u, focus_v, opportunities, valid candidates) are equal;
+    * the first eight residual filters are 193 ... 277 for every p, so group
+      populations 1 and 2, per subtile, are equal;
+    * the survivors at 433 are exactly the survivors at 313 that pass the
+      residual filters q in (313, 433] (x mod q an allowed residue), and the
+      survivors at 619 those of 433 that pass q in (433, 619];
+    * every survivor satisfies local3 at its own p.
+    Windows: the reference lease's batches 0 and 34 (one and three survivors at
+    p = 619) and the first windows of the seeded mixed-prime stream."""
+    from psuedocubes_amd.batched import window_stream
+    from psuedocubes_amd.focus import A619, B619
+    from psuedocubes_amd.tables import allowed
+    configs = {p: Config(p, 10**27, 3 * 10**27, A619, B619) for p in (313, 433, 619)}
+    fa, fb = oracle_focuses619
+    batches = json.loads((GOLDEN / 'lease_575222931200000000_batches.json').read_text())['batches']
+    windows = [(int(batches[i]['first']), int(batches[i]['end'])) for i in (0, 34)]
+    windows += [(w.first, w.end) for w in window_stream(config619, 4096, 619003, (1e6, 1e9), (313, 433, 619))[:6]]
+    found = {313: 0, 433: 0, 619: 0}
+    for first, end in windows:
+        rows = {p: O.batch(configs[p], fa, fb, first, end) for p in configs}
+        for key in ('focus_u', 'focus_v', 'cartesian_opportunities', 'valid_focused_candidates'):
+            assert rows[313][key] == rows[433][key] == rows[619][key], (first, key)
+        for p in (313, 433):
+            for a, b in zip(rows[p]['subtile_scopes'], rows[619]['subtile_scopes']):
+                assert a['valid_focused_candidates'] == b['valid_focused_candidates']
+                assert a['group_survivors'][:2] == b['group_survivors'][:2]
+        survivors = {p: [int(x) for x in rows[p]['all_values']] for p in configs}
+        for small, large in ((313, 433), (433, 619)):
+            extra = [q for q in configs[large].residuals() if small < q <= large]
+            passing = [x for x in survivors[small] if all(x % q in allowed(q) for q in extra)]
+            assert passing == survivors[large], (first, small, large)
+        for p in configs:
+            assert all(local3(x, p) for x in survivors[p][:200])
+            found[p] += len(survivors[p])
+    assert found[619] >= 4 and found[313] > found[433] >= found[619]
