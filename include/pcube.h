@@ -270,6 +270,25 @@ int pcube_batch_run(int engine, uint32_t nwin, const uint32_t *win_cfg, const ui
 int pcube_host_verify(const uint64_t *limbs, uint64_t n, uint32_t p, uint8_t *local_ok, uint8_t *is_cube,
                       uint64_t *cube_root);
 
+/* --- host verification of a device pass's emitted values (CPU) ------------
+ * The checks of the reference's host collection (pseudocube/batch_experiment.py:121-167)
+ * on the values of nsub subtiles at once.  counts[s] values of subtile s follow those
+ * of subtile s - 1 in limbs ((low, high) per value).  Each subtile's values are sorted
+ * ascending.  A duplicate, a value outside (lo, hi], a canonical v = -x A^-1 mod B
+ * outside [firsts[s], ends[s]) (pseudocube/focus.py:116-118), or a value failing
+ * local3(x, primes_of[s]) (pseudocube/oracle.py:36-41) returns MISMATCH with *failure
+ * 1, 2, 4 or 3.  Otherwise, per value in sorted order, cube[i] = x is a perfect cube
+ * and roots[i] = floor(x^(1/3)) (review/verify_cube.py:13-17); text holds each
+ * subtile's sorted values as the canonical JSON list of decimal strings with the
+ * encoder's final newline (pseudocube/accounting.py:17-31), subtile s at
+ * [text_offsets[s], text_offsets[s + 1]), empty when it has no values.  OVERFLOW when
+ * the text exceeds text_cap; INVALID when the counts do not sum to nvalues, a prime
+ * bound is not a prime = 1 mod 3, or A is not invertible mod B.  Host only. */
+int pcube_batch_values(uint64_t nvalues, const uint64_t *limbs, uint32_t nsub, const uint64_t *counts,
+                       const uint64_t *firsts, const uint64_t *ends, const uint32_t *primes_of, uint64_t A,
+                       uint64_t B, uint64_t lo0, uint64_t lo1, uint64_t hi0, uint64_t hi1, uint8_t *cube,
+                       uint64_t *roots, char *text, uint64_t text_cap, uint64_t *text_offsets, uint32_t *failure);
+
 /* --- self test: replaces the `arithmetic` kernel (kernels/pseudocube.cuh:6-11) */
 /* out[3i..3i+2] = (low, high, borrow) of in[4i]*in[4i+1] - in[4i+2]*in[4i+3]; host arrays. */
 int pcube_arithmetic(const uint64_t *in, uint64_t *out, uint64_t count);

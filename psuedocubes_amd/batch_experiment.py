@@ -13,7 +13,7 @@ from dataclasses import dataclass
 import ctypes
 import threading
 import time
-from typing import Any
+from typing import Any, NamedTuple
 
 import numpy as np
 
@@ -204,16 +204,26 @@ def scope_skeleton(ranges: list[tuple[int, int, int, int]]) -> list[dict[str, An
     return out
 
 
+class Verified(NamedTuple):
+    """Values of a batch already verified natively (pcube_batch_values): per subtile its
+    sorted values as decimal strings and their digest (None without values), and the
+    witness records of the noncubes in value order."""
+    values: list[list[str]]
+    digests: list[str | None]
+    hits: list[dict[str, object]]
+
+
 def batch_row(config: Config, filter_groups: list[list[int]], ranges: list[tuple[int, int, int, int]],
-              bounds: list[tuple[int, int, int, int]], nv: int, counters: np.ndarray, per_tile: list[list[int]],
+              bounds: list[tuple[int, int, int, int]], nv: int, counters: np.ndarray, per_tile: list[list[int]] | None,
               output_cap: int, timings: dict[str, float], started: float,
               witnesses: Any = python_witnesses, skeleton: list[dict[str, Any]] | None = None,
-              totals: tuple[int, int] | None = None) -> dict[str, Any]:
+              totals: tuple[int, int] | None = None, verified: Verified | None = None) -> dict[str, Any]:
     """Host collection and verification of one batch (`pseudocube/batch_experiment.py:121-167`).
 
     Raises ArithmeticError on underflow, duplicates, a value failing the
     exact predicate or landing in the wrong subtile; OverflowError when a
-    subtile emitted more than output_cap values.
+    subtile emitted more than output_cap values.  With `verified`, the values'
+    checks have run natively (the same checks) and per_tile is not read.
     """
     ng = len(filter_groups)
     transfer_started = time.perf_counter()
@@ -226,36 +236,49 @@ def batch_row(config: Config, filter_groups: list[list[int]], ranges: list[tuple
     if emitted and max(emitted) > output_cap:
         raise OverflowError('fused tile output overflow; no truncated coverage')
     # every counted value must have come back, and nothing else
-    if len(per_tile) != nrows or any(len(values) != n for values, n in zip(per_tile, emitted)):
+    tiles = verified.values if verified is not None else per_tile
+    if tiles is None or len(tiles) != nrows or any(len(values) != n for values, n in zip(tiles, emitted)):
         raise ArithmeticError('emitted values disagree with the subtile counters')
-    all_values: list[int] = []
-    for values in per_tile:
-        if values:
-            all_values.extend(sorted(values))
-    transfer_seconds = time.perf_counter() - transfer_started + timings.get('transfer_seconds', 0.0)
-    verification_started = time.perf_counter()
-    if len(all_values) != len(set(all_values)):
-        raise ArithmeticError('duplicate canonical output')
-    if any(not config.lo < n <= config.hi for n in all_values):
-        raise ArithmeticError('GPU/CPU disagreement')
-    hits = witnesses(all_values, config.p)
-    if any(not row['valid_pseudocube'] for row in hits):
-        raise ArithmeticError('independent witness verification failed')
     scopes = skeleton if skeleton is not None else scope_skeleton(ranges)
+    if verified is not None:
+        transfer_seconds = time.perf_counter() - transfer_started + timings.get('transfer_seconds', 0.0)
+        verification_started = time.perf_counter()
+        all_text = [s for values in verified.values if values for s in values]
+        hits = verified.hits
+        if all_text:
+            for number, text_digest, scope in zip(emitted, verified.digests, scopes):
+                if number:
+                    scope['local_survivors'] = number
+                    scope['output_sha256'] = text_digest
+    else:
+        all_values: list[int] = []
+        for values in tiles:
+            if values:
+                all_values.extend(sorted(values))
+        transfer_seconds = time.perf_counter() - transfer_started + timings.get('transfer_seconds', 0.0)
+        verification_started = time.perf_counter()
+        if len(all_values) != len(set(all_values)):
+            raise ArithmeticError('duplicate canonical output')
+        if any(not config.lo < n <= config.hi for n in all_values):
+            raise ArithmeticError('GPU/CPU disagreement')
+        hits = witnesses(all_values, config.p)
+        if any(not row['valid_pseudocube'] for row in hits):
+            raise ArithmeticError('independent witness verification failed')
+        if all_values:
+            position = 0
+            for (first, end, _, _), number, scope in zip(ranges, emitted, scopes):
+                if number:
+                    values = sorted(all_values[position:position + number])
+                    if any(not first <= config.canonical(n)[1] < end for n in values):
+                        raise ArithmeticError('output assigned to wrong subtile')
+                    scope['local_survivors'] = number
+                    scope['output_sha256'] = digest([str(n) for n in values])
+                    position += number
+        all_text = [str(n) for n in all_values]
     # only the columns the records hold, each subtile's group counts already a list
     for valid, groups_of, scope in zip(counters[:, 0].tolist(), counters[:, 1:ng + 1].tolist(), scopes):
         scope['valid_focused_candidates'] = valid
         scope['group_survivors'] = groups_of
-    if all_values:
-        position = 0
-        for (first, end, _, _), number, scope in zip(ranges, emitted, scopes):
-            if number:
-                values = sorted(all_values[position:position + number])
-                if any(not first <= config.canonical(n)[1] < end for n in values):
-                    raise ArithmeticError('output assigned to wrong subtile')
-                scope['local_survivors'] = number
-                scope['output_sha256'] = digest([str(n) for n in values])
-                position += number
     verification_seconds = time.perf_counter() - verification_started
     sums = counters[:, :ng + 1].sum(axis=0, dtype=np.uint64).tolist() if nrows else [0] * (ng + 1)
     groups = sums[1:]
@@ -269,11 +292,11 @@ def batch_row(config: Config, filter_groups: list[list[int]], ranges: list[tuple
     return dict(status='PASS', host_verified=True, focus_u=focus_u, focus_v=int(nv),
                 cartesian_opportunities=opportunities,
                 valid_focused_candidates=sums[0],
-                genuine_cubes=len(all_values) - len(hits), local_survivors=len(all_values), cube_checks=len(all_values),
+                genuine_cubes=len(all_text) - len(hits), local_survivors=len(all_text), cube_checks=len(all_text),
                 kernel_seconds=timings.get('kernel_seconds', 0.0), transfer_seconds=transfer_seconds,
                 verification_seconds=verification_seconds,
                 focus_enumeration_seconds=timings.get('focus_enumeration_seconds', 0.0), subtile_scopes=scopes,
-                filter_groups=filter_groups, group_survivors=groups, all_values=[str(n) for n in all_values],
+                filter_groups=filter_groups, group_survivors=groups, all_values=all_text,
                 cpu_confirmed_hits=hits, overflow_splits=[],
                 elapsed_seconds=time.perf_counter() - started + timings.get('focus_enumeration_seconds', 0.0),
                 claim='Isolated validation only; no production coverage.')

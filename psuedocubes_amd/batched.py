@@ -27,6 +27,7 @@ from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass
 import ctypes
 import gc
+import hashlib
 import json
 from pathlib import Path
 import sys
@@ -36,10 +37,10 @@ from typing import Any, Iterable
 import numpy as np
 
 from . import native
-from .batch_experiment import batch_row, tile_ranges
+from .batch_experiment import Verified, batch_row, tile_ranges
 from .focus import Config
 from .gpu import GPUFocus, MatchEngine, Runtime
-from .verification import native_witnesses
+from .verification import CLAIM
 
 TUNING = Path(__file__).resolve().parent / 'tuning.json'
 DEFAULT_TUNING = dict(rows_per_chunk=8, blocks_per_cu=4, direct_below_rows=96)
@@ -186,30 +187,86 @@ class BatchedEngine:
         """Verified result rows of a launched group, in its window order."""
         windows, plans, bounds, sizes = raw['windows'], raw['plans'], raw['bounds'], raw['sizes']
         counters, values, timings = raw['counters'], raw['values'], raw['timings']
+        blocks, kept = [], []
+        c0 = 0
+        for window, (ranges, _) in zip(windows, plans):
+            ng = self.engines[window.p].ng
+            nt = len(ranges)
+            block = counters[c0:c0 + nt * (ng + 3)].reshape(nt, ng + 3)
+            blocks.append(block)
+            kept.append(np.minimum(block[:, ng + 1], self.cap))
+            c0 += nt * (ng + 3)
+        verified = self.verify_values(windows, plans, kept, values)
+        # each window's focus_u and cartesian opportunities, summed over its subtiles' bounds at once
+        starts = np.cumsum([0] + [len(ranges) for ranges, _ in plans[:-1]])
+        du = (bounds[:, 1] - bounds[:, 0]).astype(np.int64)
+        dv = (bounds[:, 3] - bounds[:, 2]).astype(np.int64)
+        focus_u = np.add.reduceat(du, starts).tolist() if len(du) else []
+        pairs = np.add.reduceat(du * dv, starts).tolist() if len(du) else []
+        nvs = sizes[1::2].tolist()
         rows: list[dict[str, Any]] = []
-        t0 = c0 = v0 = 0
-        for k, index in enumerate(range(len(windows))):
-            window, (ranges, mode) = windows[index], plans[k]
+        for k, (window, (ranges, mode), block, checked) in enumerate(zip(windows, plans, blocks, verified)):
             engine = self.engines[window.p]
-            nt, stride = len(ranges), engine.ng + 3
-            block = counters[c0:c0 + nt * stride].reshape(nt, stride)
-            kept_of = [min(k, self.cap) for k in block[:, engine.ng + 1].tolist()]
-            limbs = values[2 * v0:2 * (v0 + sum(kept_of))].tolist()
-            per_tile = []
-            at = 0
-            for kept in kept_of:
-                per_tile.append([limbs[2 * i] + (limbs[2 * i + 1] << 64) for i in range(at, at + kept)])
-                at += kept
-            v0 += at
-            bnd = [tuple(r) for r in bounds[t0:t0 + nt].tolist()]
-            row = batch_row(self.configs[window.p], engine.filter_groups, ranges, bnd, int(sizes[2 * k + 1]), block,
-                            per_tile, self.cap, timings, time.perf_counter(), witnesses=native_witnesses)
+            row = batch_row(self.configs[window.p], engine.filter_groups, ranges, [], nvs[k], block, None, self.cap,
+                            timings, time.perf_counter(), verified=checked, totals=(focus_u[k], pairs[k]))
             row['matching_layout'] = 'batched-direct' if mode else 'batched-residue-bitset'
             row['prefilter_build_seconds'] = 0.0
             rows.append(row)
-            t0 += nt
-            c0 += nt * stride
         return rows
+
+    def verify_values(self, windows: list[Window], plans: list, kept: list[np.ndarray],
+                      values: np.ndarray) -> list[Verified]:
+        """Every emitted value of a pass checked in one native call (pcube_batch_values:
+        sorted per subtile, no duplicates, inside (lo, hi], canonical v inside its
+        subtile, local3 at its window's p), then per window its subtiles' decimal
+        strings, digests and the noncubes' witness records."""
+        kept_all = np.concatenate(kept) if kept else np.zeros(0, dtype=np.uint64)
+        nvalues = int(kept_all.sum())
+        if nvalues == 0:
+            return [Verified([[] for _ in k], [None] * len(k), []) for k in kept]
+        firsts = np.array([r[0] for ranges, _ in plans for r in ranges], dtype=np.uint64)
+        ends = np.array([r[1] for ranges, _ in plans for r in ranges], dtype=np.uint64)
+        primes = np.repeat(np.array([w.p for w in windows], dtype=np.uint32), [len(k) for k in kept])
+        counts = np.ascontiguousarray(kept_all, dtype=np.uint64)
+        limbs = np.ascontiguousarray(values[:2 * nvalues])
+        cube = np.zeros(nvalues, dtype=np.uint8)
+        roots = np.zeros(nvalues, dtype=np.uint64)
+        text = ctypes.create_string_buffer(nvalues * 44 + len(counts) * 3 + 16)
+        offsets = np.zeros(len(counts) + 1, dtype=np.uint64)
+        failure = ctypes.c_uint32()
+        base = self.base
+        mask = (1 << 64) - 1
+        native.call('pcube_batch_values', nvalues, native.ptr64(limbs), len(counts), native.ptr64(counts),
+                    native.ptr64(firsts), native.ptr64(ends), native.ptr32(primes), base.A, base.B, base.lo & mask,
+                    base.lo >> 64, base.hi & mask, base.hi >> 64, cube.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                    native.ptr64(roots), text, len(text), native.ptr64(offsets), ctypes.byref(failure))
+        raw_text = text.raw
+        offs = offsets.tolist()
+        cubes = cube.tolist()
+        root_list = roots.tolist()
+        out: list[Verified] = []
+        s = v = 0
+        for window, k in zip(windows, kept):
+            tile_values: list[list[str]] = []
+            digests: list[str | None] = []
+            hits: list[dict[str, object]] = []
+            for n in k.tolist():
+                if n:
+                    chunk = raw_text[offs[s]:offs[s + 1]]
+                    strings = json.loads(chunk)
+                    tile_values.append(strings)
+                    digests.append(hashlib.sha256(chunk).hexdigest())
+                    p = window.p
+                    hits += [{'n': value, 'p': p, 'cube_root_floor': str(root), 'perfect_cube': False,
+                              'failing_primes': [], 'valid_pseudocube': True, 'claim': CLAIM}
+                             for value, is_cube, root in zip(strings, cubes[v:v + n], root_list[v:v + n]) if not is_cube]
+                    v += n
+                else:
+                    tile_values.append([])
+                    digests.append(None)
+                s += 1
+            out.append(Verified(tile_values, digests, hits))
+        return out
 
     def __del__(self) -> None:
         if native._lib is not None and hasattr(self, 'handle'):

@@ -16,6 +16,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -1977,6 +1978,14 @@ int pcube_batch_create(const int* matchers, uint32_t ncfg, int focus_u, int focu
         bc.allow = mt->d_allow.as<uint32_t>();
         bc.nres = mt->nres; bc.nbits = mt->nbits; bc.ncache = mt->ncache; bc.ng = mt->ng;
         bc.allow_words = mt->allow_words; bc.mask_rows = mt->mask_rows;
+        // the staged builder's permuted strings cover the four bitset filters when its
+        // staged prefix is at least as long
+        bc.pstr = mt->d_pstr.as<uint32_t>();
+        bc.transpose = mt->sbits >= mt->nbits && mt->d_pstr.p ? 1u : 0u;
+        for (uint32_t k = 0; k < kMaxBitsetFilters; ++k) {
+            bc.pstr_off[k] = k < mt->sbits ? mt->s_pstr_off[k] : 0u;
+            bc.nbeta[k] = k < mt->sbits ? mt->s_nbeta[k] : 0u;
+        }
         bc.chunk_base[0] = 0;
         for (uint32_t k = 0; k < kMaxBitsetFilters; ++k) {
             bc.row_base[k] = mt->row_base[k];
@@ -2128,11 +2137,15 @@ int pcube_batch_run(int engine, uint32_t nwin, const uint32_t* win_cfg, const ui
         d.vs = (const uint64_t*)(uintptr_t)(offv[w] * 8);
         d.counters = (uint64_t*)(uintptr_t)(ncounters * 8);
         if (d.mode == 0 && nu) {
+            // mask items: (run of kMaskRun words, filter) pairs for a transposed build, else
+            // (chunk, word) pairs;
+            // residue items: one per u (a 16-byte row of kCachedResidues residues)
             d.masks = (uint64_t*)(uintptr_t)(mask_words * 8);
-            d.rho = (uint16_t*)(uintptr_t)(rho_items * 2);
+            d.rho = (uint16_t*)(uintptr_t)(rho_items * 2 * kCachedResidues);
             mask_words += (uint64_t)c.mask_rows * d.words;
-            mask_waves += (uint64_t)c.chunk_base[c.nbits] * d.words;
-            rho_items += nu * kCachedResidues;
+            mask_waves += c.transpose ? (uint64_t)c.nbits * ((d.words + kMaskRun - 1) / kMaskRun)
+                                      : (uint64_t)c.chunk_base[c.nbits] * d.words;
+            rho_items += nu;
         }
         rows += nv;
         chunks += (nv + rows_per_chunk - 1) / rows_per_chunk;
@@ -2144,7 +2157,7 @@ int pcube_batch_run(int engine, uint32_t nwin, const uint32_t* win_cfg, const ui
     TRY(e.rows.ensure(std::max<uint64_t>(1, rows) * sizeof(Row)));
     TRY(e.counters.ensure(std::max<uint64_t>(1, ncounters) * 8));
     TRY(e.masks.ensure(std::max<uint64_t>(8, mask_words * 8)));
-    TRY(e.rho.ensure(std::max<uint64_t>(16, rho_items * 2)));
+    TRY(e.rho.ensure(std::max<uint64_t>(16, rho_items * 2 * kCachedResidues)));
     for (auto& d : desc) {
         d.us = (const uint64_t*)((char*)e.lus.p + (uintptr_t)d.us);
         d.vs = (const uint64_t*)((char*)e.lvs.p + (uintptr_t)d.vs);
@@ -2244,48 +2257,231 @@ int pcube_batch_run(int engine, uint32_t nwin, const uint32_t* win_cfg, const ui
     return PCUBE_OK;
 }
 
-int pcube_host_verify(const uint64_t* limbs, uint64_t n, uint32_t p, uint8_t* local_ok, uint8_t* is_cube,
-                      uint64_t* cube_root) {
-    // host code only: the exact predicate of pseudocube/oracle.py:36-41 and the
-    // cube root of review/verify_cube.py:13-17, in unsigned __int128
-    typedef unsigned __int128 u128;
-    if (p < 7 || p > 100000) return fail(PCUBE_INVALID, "prime bound out of range");
-    std::vector<uint32_t> primes;
+}  // extern "C"
+
+namespace {
+
+typedef unsigned __int128 u128_t;
+
+struct TextOut {
+    char* p;
+    uint64_t n = 0, cap;
+    bool ok = true;
+    void put(const char* s, size_t len) {
+        if (!ok || n + len > cap) {
+            ok = false;
+            return;
+        }
+        memcpy(p + n, s, len);
+        n += len;
+    }
+    void put(const char* s) { put(s, strlen(s)); }
+    void digits(uint64_t v) {
+        char buf[24];
+        int i = 24;
+        do {
+            buf[--i] = char('0' + v % 10);
+            v /= 10;
+        } while (v);
+        put(buf + i, (size_t)(24 - i));
+    }
+    // a JSON integer, or its decimal string from 2^53 on (the encoder's rule)
+    void integer(uint64_t v) {
+        const bool quoted = v >= (1ull << 53);
+        if (quoted) put("\"", 1);
+        digits(v);
+        if (quoted) put("\"", 1);
+    }
+};
+
+// The primes up to p by trial division, and whether p is a prime = 1 mod 3
+// (the predicate's precondition, pseudocube/oracle.py:14-16).
+bool predicate_primes(uint32_t p, std::vector<uint32_t>& primes) {
+    primes.clear();
     for (uint32_t q = 2; q <= p; ++q) {
         bool prime = true;
         for (uint32_t d = 2; d * d <= q; ++d)
             if (q % d == 0) { prime = false; break; }
         if (prime) primes.push_back(q);
     }
-    if (primes.empty() || primes.back() != p || p % 3 != 1) return fail(PCUBE_INVALID, "p must be a prime = 1 mod 3");
-    for (uint64_t i = 0; i < n; ++i) {
-        const u128 x = ((u128)limbs[2 * i + 1] << 64) | limbs[2 * i];
-        bool ok = x > 0 && ((uint32_t)(x % 9) == 1 || (uint32_t)(x % 9) == 8);
-        for (size_t k = 0; ok && k < primes.size(); ++k) {
-            const uint64_t q = primes[k], r = (uint64_t)(x % q);
-            if (r == 0) { ok = false; break; }
-            if (q % 3 == 1) {
-                uint64_t e = (q - 1) / 3, base = r, acc = 1;
-                while (e) {
-                    if (e & 1) acc = acc * base % q;
-                    base = base * base % q;
-                    e >>= 1;
-                }
-                if (acc != 1) ok = false;
+    return !primes.empty() && primes.back() == p && p % 3 == 1 && p >= 7;
+}
+
+// local3(x, p) of pseudocube/oracle.py:36-41: x > 0, x mod 9 in {1, 8}, no prime
+// q <= p divides x, and x^((q-1)/3) = 1 mod q for every q = 1 mod 3.  Residues of x
+// come from its 64-bit halves: x mod q = ((hi mod q)(2^64 mod q) + lo mod q) mod q,
+// with 2^64 mod q in wrap[k] (wrap empty: computed here).
+bool local_predicate(u128_t x, const std::vector<uint32_t>& primes, const std::vector<uint32_t>& wrap = {}) {
+    const uint64_t lo = (uint64_t)x, hi = (uint64_t)(x >> 64);
+    const uint64_t m9 = ((hi % 9) * 7 + lo % 9) % 9;   // 2^64 = 7 mod 9
+    bool ok = x > 0 && (m9 == 1 || m9 == 8);
+    for (size_t k = 0; ok && k < primes.size(); ++k) {
+        const uint64_t q = primes[k];
+        const uint64_t r64 = k < wrap.size() ? wrap[k] : (UINT64_MAX % q + 1) % q;
+        const uint64_t r = ((hi % q) * r64 + lo % q) % q;
+        if (r == 0) return false;
+        if (q % 3 == 1) {
+            uint64_t e = (q - 1) / 3, base = r, acc = 1;
+            while (e) {
+                if (e & 1) acc = acc * base % q;
+                base = base * base % q;
+                e >>= 1;
             }
+            if (acc != 1) return false;
         }
-        // floor cube root by bisection on [0, 6981463658332): 6981463658331^3 < 2^128 <= 6981463658332^3,
-        // so every probe's cube fits 128 bits
-        uint64_t lo = 0, hi = 6981463658332ull;
-        while (hi - lo > 1) {
-            const uint64_t mid = (lo + hi) >> 1;
-            const u128 cube = (u128)mid * mid * mid;
-            if (cube <= x) lo = mid; else hi = mid;
-        }
-        local_ok[i] = ok ? 1 : 0;
-        is_cube[i] = ((u128)lo * lo * lo == x) ? 1 : 0;
-        cube_root[i] = lo;
     }
+    return ok;
+}
+
+// floor(x^(1/3)) by bisection on [0, 6981463658332): 6981463658331^3 < 2^128 <=
+// 6981463658332^3, so every probe's cube fits 128 bits (review/verify_cube.py:13-17).
+uint64_t cube_root_floor(u128_t x) {
+    uint64_t lo = 0, hi = 6981463658332ull;
+    while (hi - lo > 1) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if ((u128_t)mid * mid * mid <= x) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// a^-1 mod m for gcd(a, m) = 1 (extended Euclid), or 0 when not invertible.
+uint64_t inverse_mod(uint64_t a, uint64_t m) {
+    int64_t t = 0, nt = 1;
+    uint64_t r = m, nr = a % m;
+    while (nr) {
+        const uint64_t q = r / nr;
+        const int64_t tt = t - (int64_t)q * nt;
+        t = nt; nt = tt;
+        const uint64_t rr = r - q * nr;
+        r = nr; nr = rr;
+    }
+    if (r != 1) return 0;
+    return t < 0 ? (uint64_t)(t + (int64_t)m) : (uint64_t)t;
+}
+
+void decimal_u128(u128_t x, char* buf, int& len) {
+    char tmp[48];
+    int i = 48;
+    do {
+        tmp[--i] = char('0' + (int)(x % 10));
+        x /= 10;
+    } while (x);
+    len = 48 - i;
+    memcpy(buf, tmp + i, (size_t)len);
+}
+
+}  // namespace
+
+extern "C" {
+
+int pcube_host_verify(const uint64_t* limbs, uint64_t n, uint32_t p, uint8_t* local_ok, uint8_t* is_cube,
+                      uint64_t* cube_root) {
+    // host code only: the exact predicate of pseudocube/oracle.py:36-41 and the
+    // cube root of review/verify_cube.py:13-17, in unsigned __int128
+    if (p < 7 || p > 100000) return fail(PCUBE_INVALID, "prime bound out of range");
+    std::vector<uint32_t> primes;
+    if (!predicate_primes(p, primes)) return fail(PCUBE_INVALID, "p must be a prime = 1 mod 3");
+    for (uint64_t i = 0; i < n; ++i) {
+        const u128_t x = ((u128_t)limbs[2 * i + 1] << 64) | limbs[2 * i];
+        const uint64_t r = cube_root_floor(x);
+        local_ok[i] = local_predicate(x, primes) ? 1 : 0;
+        is_cube[i] = ((u128_t)r * r * r == x) ? 1 : 0;
+        cube_root[i] = r;
+    }
+    return PCUBE_OK;
+}
+
+int pcube_batch_values(uint64_t nvalues, const uint64_t* limbs, uint32_t nsub, const uint64_t* counts,
+                       const uint64_t* firsts, const uint64_t* ends, const uint32_t* primes_of, uint64_t A,
+                       uint64_t B, uint64_t lo0, uint64_t lo1, uint64_t hi0, uint64_t hi1, uint8_t* cube,
+                       uint64_t* roots, char* text, uint64_t text_cap, uint64_t* text_offsets, uint32_t* failure) {
+    // host code only: the checks of pseudocube/batch_experiment.py:121-167 on every
+    // emitted value of many subtiles at once, subtiles spread over a few threads
+    *failure = 0;
+    if (B == 0) return fail(PCUBE_INVALID, "B must be positive");
+    std::vector<uint64_t> at(nsub + 1, 0);
+    for (uint32_t s = 0; s < nsub; ++s) at[s + 1] = at[s] + counts[s];
+    if (at[nsub] != nvalues) return fail(PCUBE_INVALID, "subtile counts sum to %llu, %llu values given",
+                                         (unsigned long long)at[nsub], (unsigned long long)nvalues);
+    const uint64_t ainv = inverse_mod(A % B, B);
+    if (!ainv) return fail(PCUBE_INVALID, "A is not invertible mod B");
+    const u128_t lo = ((u128_t)lo1 << 64) | lo0, hi = ((u128_t)hi1 << 64) | hi0;
+    // the prime lists (and 2^64 mod q) of the bounds in use
+    std::map<uint32_t, std::pair<std::vector<uint32_t>, std::vector<uint32_t>>> lists;
+    for (uint32_t s = 0; s < nsub; ++s) {
+        if (!counts[s] || lists.count(primes_of[s])) continue;
+        std::vector<uint32_t> primes, wrap;
+        if (!predicate_primes(primes_of[s], primes)) return fail(PCUBE_INVALID, "p must be a prime = 1 mod 3");
+        for (uint32_t q : primes) wrap.push_back((uint32_t)((UINT64_MAX % q + 1) % q));
+        lists.emplace(primes_of[s], std::make_pair(std::move(primes), std::move(wrap)));
+    }
+    std::vector<std::string> texts(nsub);
+    std::vector<uint32_t> failed(nsub, 0);
+    auto work = [&](uint32_t s) {
+        const uint64_t n = counts[s], a = at[s];
+        if (!n) return;
+        const auto& pl = lists.at(primes_of[s]);
+        std::vector<u128_t> xs(n);
+        for (uint64_t i = 0; i < n; ++i) xs[i] = ((u128_t)limbs[2 * (a + i) + 1] << 64) | limbs[2 * (a + i)];
+        std::sort(xs.begin(), xs.end());
+        std::string& out = texts[s];
+        out.reserve(n * 44 + 3);
+        out.push_back('[');
+        for (uint64_t i = 0; i < n; ++i) {
+            const u128_t x = xs[i];
+            if (i && xs[i - 1] == x) { failed[s] = 1; return; }
+            if (!(lo < x && x <= hi)) { failed[s] = 2; return; }
+            // canonical v = -x * A^-1 mod B must lie in the subtile (pseudocube/focus.py:116-118)
+            const uint64_t xm = (uint64_t)(x % B);
+            const uint64_t v = (uint64_t)((u128_t)((B - xm) % B) * ainv % B);
+            if (v < firsts[s] || v >= ends[s]) { failed[s] = 4; return; }
+            if (!local_predicate(x, pl.first, pl.second)) { failed[s] = 3; return; }
+            const uint64_t r = cube_root_floor(x);
+            cube[a + i] = ((u128_t)r * r * r == x) ? 1 : 0;
+            roots[a + i] = r;
+            char buf[48];
+            int len = 0;
+            decimal_u128(x, buf, len);
+            if (i) out.push_back(',');
+            out.push_back('"');
+            out.append(buf, (size_t)len);
+            out.push_back('"');
+        }
+        out.append("]\n");
+    };
+    const uint32_t workers = (uint32_t)std::min<uint64_t>(
+        {8, std::max(1u, std::thread::hardware_concurrency()), nvalues / 2048 + 1});
+    if (workers <= 1) {
+        for (uint32_t s = 0; s < nsub; ++s) work(s);
+    } else {
+        // subtiles dealt out in value-balanced runs
+        std::vector<std::thread> pool;
+        std::vector<uint32_t> cut(workers + 1, nsub);
+        cut[0] = 0;
+        for (uint32_t w = 1, s = 0; w < workers; ++w) {
+            while (s < nsub && at[s] < nvalues * w / workers) ++s;
+            cut[w] = s;
+        }
+        for (uint32_t w = 0; w < workers; ++w)
+            pool.emplace_back([&, w] {
+                for (uint32_t s = cut[w]; s < cut[w + 1]; ++s) work(s);
+            });
+        for (auto& th : pool) th.join();
+    }
+    static const char* const kFailure[] = {"", "duplicate canonical output", "value outside (lo, hi]",
+                                           "GPU/CPU disagreement", "output assigned to wrong subtile"};
+    for (uint32_t s = 0; s < nsub; ++s)
+        if (failed[s]) {
+            *failure = failed[s];
+            return fail(PCUBE_MISMATCH, "%s (subtile %u of the pass)", kFailure[failed[s]], s);
+        }
+    TextOut out{text, 0, text_cap};
+    text_offsets[0] = 0;
+    for (uint32_t s = 0; s < nsub; ++s) {
+        out.put(texts[s].data(), texts[s].size());
+        text_offsets[s + 1] = out.n;
+    }
+    if (!out.ok) return fail(PCUBE_OVERFLOW, "value text exceeds %llu bytes", (unsigned long long)text_cap);
     return PCUBE_OK;
 }
 
@@ -2463,36 +2659,6 @@ int pcube_copy_probe(uint64_t bytes, double* seconds) {
 }
 
 // Text writer for pcube_scope_records: fails once cap would be exceeded.
-struct TextOut {
-    char* p;
-    uint64_t n = 0, cap;
-    bool ok = true;
-    void put(const char* s, size_t len) {
-        if (!ok || n + len > cap) {
-            ok = false;
-            return;
-        }
-        memcpy(p + n, s, len);
-        n += len;
-    }
-    void put(const char* s) { put(s, strlen(s)); }
-    void digits(uint64_t v) {
-        char buf[24];
-        int i = 24;
-        do {
-            buf[--i] = char('0' + v % 10);
-            v /= 10;
-        } while (v);
-        put(buf + i, (size_t)(24 - i));
-    }
-    // a JSON integer, or its decimal string from 2^53 on (the encoder's rule)
-    void integer(uint64_t v) {
-        const bool quoted = v >= (1ull << 53);
-        if (quoted) put("\"", 1);
-        digits(v);
-        if (quoted) put("\"", 1);
-    }
-};
 
 int pcube_scope_records(uint32_t ntiles, const uint64_t* bounds, const uint64_t* counters, uint32_t stride,
                         uint32_t ng, const uint64_t* local, const char* sha, char* out, uint64_t cap,

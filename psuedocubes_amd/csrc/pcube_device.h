@@ -573,6 +573,25 @@ __device__ __forceinline__ void staged_rows_block(const MatchArgs& m, uint32_t b
     }
 }
 
+// One butterfly stage of a 64 x 64 bit transpose across a wave.
+template <int kB, uint64_t kMask>
+__device__ __forceinline__ uint64_t transpose_step(uint64_t x, uint32_t lane) {
+    const uint64_t y = __shfl_xor(x, kB, kWave);
+    return (lane & kB) ? ((x & kMask) | ((y & kMask) >> kB)) : ((x & ~kMask) | ((y & ~kMask) << kB));
+}
+
+// Lane t holds column t of a 64 x 64 bit matrix (bit k = element (k, t));
+// returns row `lane` (bit t = element (lane, t)).
+__device__ __forceinline__ uint64_t wave_transpose(uint64_t x, uint32_t lane) {
+    x = transpose_step<32, 0xffffffff00000000ull>(x, lane);
+    x = transpose_step<16, 0xffff0000ffff0000ull>(x, lane);
+    x = transpose_step<8, 0xff00ff00ff00ff00ull>(x, lane);
+    x = transpose_step<4, 0xf0f0f0f0f0f0f0f0ull>(x, lane);
+    x = transpose_step<2, 0xccccccccccccccccull>(x, lane);
+    x = transpose_step<1, 0xaaaaaaaaaaaaaaaaull>(x, lane);
+    return x;
+}
+
 // One 64-bit word of the residue bitsets: wave builds 64 rows (residues
 // r0..r0+63 of v mod q) of word w; lane t tests u index 64*w + t and a wave
 // ballot packs the 64 tests.  Row r, bit t: allowed_q[(u*B - r*A) mod q].
@@ -923,10 +942,62 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_direct(MatchArgs m) {
 struct BatchConfig {       // one filter configuration (a libpcube matcher)
     const Modulus* mods;
     const uint32_t* allow;
+    const uint32_t* pstr;      // permuted allowed strings P[j] = allowed_q[(-j*A) mod q] of the bitset filters
     uint32_t nres, nbits, ncache, ng, allow_words, mask_rows;
+    uint32_t transpose;        // every bitset filter has its permuted string: masks by wave transpose
     uint32_t row_base[kMaxBitsetFilters];
     uint32_t chunk_base[kMaxBitsetFilters + 1];   // 64-row mask chunks per bitset prime, prefix
+    uint32_t pstr_off[kMaxBitsetFilters], nbeta[kMaxBitsetFilters];   // string offset, (-B/A) mod q
 };
+
+// Words w0 .. w0+7 of every row of bitset filter k, row-major batch layout, by the
+// method of the staged builder: for word w lane t takes the u at index 64w + t,
+// and for the 64 rows r0..r0+63 of each chunk the 64-bit window P[k0 .. k0+63],
+// k0 = (r0 + (u mod q) * nbeta) mod q, whose bit j is allowed_q[(u*B - (r0 + j)*A)
+// mod q]; a wave transpose turns the 64 columns into the 64 row words (u indices
+// past nu give zero bits, as in build_mask_word).  Lane j then stores its row's
+// eight words as one 64-byte run, so every store fills whole segments.
+constexpr uint32_t kMaskRun = 8;   // words one batched mask item builds per row
+__device__ __forceinline__ void build_mask_filter_run(const MatchArgs& m, const BatchConfig& c, uint32_t k,
+                                                      uint32_t w0, uint32_t lane) {
+    const Modulus mod = m.mods[k];
+    const uint32_t* P = c.pstr + c.pstr_off[k];
+    uint32_t beta[kMaskRun];
+    bool live[kMaskRun];
+#pragma unroll
+    for (uint32_t ww = 0; ww < kMaskRun; ++ww) {
+        const uint32_t i = (w0 + ww) * kWave + lane;
+        live[ww] = i < m.nu;
+        beta[ww] = live[ww] ? mod32(mod64(m.us[i], mod) * c.nbeta[k], mod) : 0u;
+    }
+    const uint32_t nwords = min(kMaskRun, m.words - w0);
+    for (uint32_t r0 = 0; r0 < mod.q; r0 += kWave) {
+        uint64_t row[kMaskRun];
+#pragma unroll
+        for (uint32_t ww = 0; ww < kMaskRun; ++ww) {
+            uint64_t col = 0;
+            if (live[ww]) {
+                uint32_t k0 = beta[ww] + r0;
+                k0 = k0 >= mod.q ? k0 - mod.q : k0;
+                const uint32_t a = k0 >> 5, sh = k0 & 31;
+                const uint64_t low = ((uint64_t)P[a + 1] << 32) | P[a];
+                col = (low >> sh) | (sh ? (uint64_t)P[a + 2] << (64 - sh) : 0ull);
+            }
+            row[ww] = wave_transpose(col, lane);
+        }
+        if (r0 + lane < mod.q) {
+            uint64_t* out = (uint64_t*)m.masks + (uint64_t)(m.row_base[k] + r0 + lane) * m.words + w0;
+            if (nwords == kMaskRun && ((uintptr_t)out & 15) == 0) {
+#pragma unroll
+                for (uint32_t ww = 0; ww < kMaskRun; ww += 2)
+                    *(uint4*)(out + ww) = make_uint4(uint32_t(row[ww]), uint32_t(row[ww] >> 32), uint32_t(row[ww + 1]),
+                                                     uint32_t(row[ww + 1] >> 32));
+            } else {
+                for (uint32_t ww = 0; ww < nwords; ++ww) out[ww] = row[ww];
+            }
+        }
+    }
+}
 
 struct BatchWindow {       // one window: its coordinates, tables and counters
     const uint64_t* us;
@@ -1016,26 +1087,55 @@ __global__ void k_batch_rows(BatchArgs b) {
     }
 }
 
+// Mask items: per window, (run of kMaskRun words, filter) pairs when its
+// configuration builds by transpose (every row of the filter), else (64-row chunk, word)
+// pairs.  Each wave takes one contiguous run of the items, so it finds its first
+// window once and then walks the windows in order.
 __global__ __launch_bounds__(256) void k_batch_masks(BatchArgs b) {
     const BatchTotals t = *b.totals;
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / kWave);
-    for (uint64_t g = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / kWave; g < t.mask_waves; g += waves) {
-        const uint32_t w = find_window(b.win, t.nwin, g, [](const BatchWindow& x) { return x.mask_begin; });
+    const uint64_t per = (t.mask_waves + waves - 1) / waves;
+    uint64_t g = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / kWave * per;
+    const uint64_t stop = min(t.mask_waves, g + per);
+    if (g >= stop) return;
+    uint32_t w = find_window(b.win, t.nwin, g, [](const BatchWindow& x) { return x.mask_begin; });
+    while (g < stop && w < t.nwin) {
         const BatchWindow win = b.win[w];
-        const uint64_t local = g - win.mask_begin;
-        build_mask_word(window_args(b, win), b.cfg[win.cfg].chunk_base, uint32_t(local / win.words),
-                        uint32_t(local % win.words), lane);
+        const uint64_t next = w + 1 < t.nwin ? b.win[w + 1].mask_begin : t.mask_waves;
+        const MatchArgs m = window_args(b, win);
+        const BatchConfig& c = b.cfg[win.cfg];
+        for (; g < stop && g < next; ++g) {
+            const uint64_t local = g - win.mask_begin;
+            if (c.transpose)
+                build_mask_filter_run(m, c, uint32_t(local % m.nbits), uint32_t(local / m.nbits) * kMaskRun, lane);
+            else
+                build_mask_word(m, c.chunk_base, uint32_t(local / win.words), uint32_t(local % win.words), lane);
+        }
+        ++w;
     }
 }
 
+// Residue cache rows: one thread per u coordinate writes its kCachedResidues
+// residues (u*B) mod q as one 16-byte row.
 __global__ void k_batch_rho(BatchArgs b) {
     const BatchTotals t = *b.totals;
     for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < t.rho_items;
          g += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t w = find_window(b.win, t.nwin, g, [](const BatchWindow& x) { return x.rho_begin; });
         const BatchWindow win = b.win[w];
-        build_rho(window_args(b, win), g - win.rho_begin);
+        const BatchConfig& c = b.cfg[win.cfg];
+        const uint64_t i = g - win.rho_begin;
+        const uint64_t u = win.us[i];
+        uint32_t packed[kCachedResidues / 2] = {};
+#pragma unroll
+        for (int k = 0; k < kCachedResidues; ++k) {
+            if ((uint32_t)k < c.ncache) {
+                const Modulus mod = c.mods[c.nbits + k];
+                packed[k / 2] |= mod32(mod64(u, mod) * mod.b, mod) << (16 * (k & 1));
+            }
+        }
+        ((uint4*)win.rho)[i] = make_uint4(packed[0], packed[1], packed[2], packed[3]);
     }
 }
 

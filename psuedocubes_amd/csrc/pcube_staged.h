@@ -50,25 +50,6 @@ constexpr int kStageLines = 280;      // lines of a chunk image: the largest sta
 constexpr int kStageLineBytes = kStagedFilters * kStageWords * 8;   // 8 columns of 32 bytes
 static_assert(kStagedFilters == 8 && kStageLineBytes == 256, "the read rotation assumes 8 columns of 32 bytes");
 
-// One butterfly stage of a 64 x 64 bit transpose across a wave.
-template <int kB, uint64_t kMask>
-__device__ __forceinline__ uint64_t transpose_step(uint64_t x, uint32_t lane) {
-    const uint64_t y = __shfl_xor(x, kB, kWave);
-    return (lane & kB) ? ((x & kMask) | ((y & kMask) >> kB)) : ((x & ~kMask) | ((y & ~kMask) << kB));
-}
-
-// Lane t holds column t of a 64 x 64 bit matrix (bit k = element (k, t));
-// returns row `lane` (bit t = element (lane, t)).
-__device__ __forceinline__ uint64_t wave_transpose(uint64_t x, uint32_t lane) {
-    x = transpose_step<32, 0xffffffff00000000ull>(x, lane);
-    x = transpose_step<16, 0xffff0000ffff0000ull>(x, lane);
-    x = transpose_step<8, 0xff00ff00ff00ff00ull>(x, lane);
-    x = transpose_step<4, 0xf0f0f0f0f0f0f0f0ull>(x, lane);
-    x = transpose_step<2, 0xccccccccccccccccull>(x, lane);
-    x = transpose_step<1, 0xaaaaaaaaaaaaaaaaull>(x, lane);
-    return x;
-}
-
 // Staged residue bitsets.  Wave (s, c) builds rows r0..r0+63 of one bitset
 // filter (lines r0..r0+63 of its column) for the kStageWords words of stage
 // chunk s.  Lane t takes the u at

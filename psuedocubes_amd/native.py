@@ -60,6 +60,8 @@ SIGNATURES: dict[str, list[Any]] = {
     'pcube_batch_run': [c_int, u32, P_u32, P_u32, P_u64, P_u64, P_u32, u32, P_u64, P_u64, P_u64, P_u64, u64, P_dbl,
                         P_u64],
     'pcube_host_verify': [P_u64, u64, u32, ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(ctypes.c_uint8), P_u64],
+    'pcube_batch_values': [u64, P_u64, u32, P_u64, P_u64, P_u64, P_u32, u64, u64, u64, u64, u64, u64,
+                           ctypes.POINTER(ctypes.c_uint8), P_u64, ctypes.c_void_p, u64, P_u64, P_u32],
     'pcube_floor_div': [P_u64, P_u64, u64],
     'pcube_scope_records': [u32, P_u64, P_u64, u32, u32, P_u64, ctypes.c_char_p, ctypes.c_void_p, u64, P_u64],
     'pcube_arithmetic': [P_u64, P_u64, u64],
