@@ -126,6 +126,15 @@ int pcube_prepare(int focus_u, int focus_v, uint32_t ntiles, const uint64_t *til
 int pcube_tile_ranges(uint64_t A, uint64_t B, uint64_t lo0, uint64_t lo1, uint64_t hi0, uint64_t hi1, uint64_t first,
                       uint64_t end, uint64_t width, uint64_t *tiles, uint32_t *ntiles);
 
+/* pcube_tile_ranges of n windows [firsts[w], ends[w]) with subtile widths
+ * widths[w], concatenated in window order into tiles (room for tiles_cap
+ * subtiles, 4 uint64 each), counts[w] subtiles for window w.  INVALID (naming
+ * the window) as pcube_tile_ranges, or when the tiles do not fit.  Host only:
+ * the geometry of the batched form's windows in one call. */
+int pcube_tile_ranges_many(uint64_t A, uint64_t B, uint64_t lo0, uint64_t lo1, uint64_t hi0, uint64_t hi1, uint32_t n,
+                           const uint64_t *firsts, const uint64_t *ends, const uint64_t *widths, uint64_t *tiles,
+                           uint64_t tiles_cap, uint32_t *counts);
+
 /* prepare() of one compute batch from its canonical v range [first, end) and
  * subtile width (pseudocube/batch_experiment.py:27-32, 51-57): the geometry of
  * pcube_tile_ranges into tiles, then pcube_prepare on it.  totals (host, 3

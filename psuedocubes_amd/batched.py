@@ -96,6 +96,33 @@ class BatchedEngine:
         mode = 1 if rows < self.tuning['direct_below_rows'] else 0
         return ranges, mode
 
+    def plan_many(self, windows: list[Window]) -> tuple[list[tuple[list[list[int]], int]], np.ndarray]:
+        """plan() of every window of a pass, the subtile geometry in one library call
+        (pcube_tile_ranges_many, exact 128-bit arithmetic), and the flat tiles array
+        the device pass takes."""
+        n = len(windows)
+        base = self.base
+        firsts = np.array([w.first for w in windows], dtype=np.uint64)
+        ends = np.array([w.end for w in windows], dtype=np.uint64)
+        widths = np.array([w.width for w in windows], dtype=np.uint64)
+        tiles = np.zeros(4 * 64 * max(1, n), dtype=np.uint64)
+        counts = np.zeros(max(1, n), dtype=np.uint32)
+        mask = (1 << 64) - 1
+        native.call('pcube_tile_ranges_many', base.A, base.B, base.lo & mask, base.lo >> 64, base.hi & mask,
+                    base.hi >> 64, n, native.ptr64(firsts), native.ptr64(ends), native.ptr64(widths),
+                    native.ptr64(tiles), 64 * max(1, n), native.ptr32(counts))
+        total = int(counts[:n].sum())
+        tiles = tiles[:4 * total]
+        flat = tiles.reshape(total, 4).tolist()
+        density = self.bf.focus.count / self.bf.focus.modulus
+        threshold = self.tuning['direct_below_rows']
+        plans = []
+        at = 0
+        for w, count in zip(windows, counts[:n].tolist()):
+            plans.append((flat[at:at + count], 1 if (w.end - w.first) * density < threshold else 0))
+            at += count
+        return plans, tiles
+
     def footprint(self, window: Window) -> int:
         """Estimated device bytes of one window in a group."""
         u_span = (window.end - window.first) * self.base.A / self.base.B + (self.base.hi - self.base.lo) / self.base.B
@@ -154,10 +181,9 @@ class BatchedEngine:
         previous group's rows meanwhile."""
         group = list(range(len(windows)))
         started = time.perf_counter()
-        plans = [self.plan(windows[i]) for i in group]
+        plans, tiles = self.plan_many(windows)
         cfg = np.array([self.order.index(windows[i].p) for i in group], dtype=np.uint32)
         ntiles = np.array([len(r) for r, _ in plans], dtype=np.uint32)
-        tiles = np.array([x for r, _ in plans for t in r for x in t], dtype=np.uint64)
         widths = np.array([windows[i].width for i in group], dtype=np.uint64)
         modes = np.array([m for _, m in plans], dtype=np.uint32)
         ntot = int(ntiles.sum())

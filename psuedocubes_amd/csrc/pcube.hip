@@ -1563,6 +1563,24 @@ int pcube_tile_ranges(uint64_t A, uint64_t B, uint64_t lo0, uint64_t lo1, uint64
     return PCUBE_OK;
 }
 
+int pcube_tile_ranges_many(uint64_t A, uint64_t B, uint64_t lo0, uint64_t lo1, uint64_t hi0, uint64_t hi1, uint32_t n,
+                           const uint64_t* firsts, const uint64_t* ends, const uint64_t* widths, uint64_t* tiles,
+                           uint64_t tiles_cap, uint32_t* counts) {
+    uint64_t at = 0;
+    uint64_t one[4 * 64];
+    for (uint32_t w = 0; w < n; ++w) {
+        uint32_t got = 0;
+        const int rc = pcube_tile_ranges(A, B, lo0, lo1, hi0, hi1, firsts[w], ends[w], widths[w], one, &got);
+        if (rc) return fail(rc, "window %u: invalid batch geometry (a nonempty v range inside [0, B) of at most 64 subtiles)", w);
+        if (at + got > tiles_cap)
+            return fail(PCUBE_INVALID, "tile buffer of %llu subtiles too small at window %u", (unsigned long long)tiles_cap, w);
+        memcpy(tiles + 4 * at, one, (size_t)got * 32);
+        counts[w] = got;
+        at += got;
+    }
+    return PCUBE_OK;
+}
+
 int pcube_prepare_range(int focus_u, int focus_v, uint64_t A, uint64_t B, uint64_t lo0, uint64_t lo1, uint64_t hi0,
                         uint64_t hi1, uint64_t first, uint64_t end, uint64_t width, uint64_t* tiles, uint32_t* ntiles,
                         uint64_t us_out, uint64_t us_cap, uint64_t vs_out, uint64_t vs_cap, uint64_t* nu, uint64_t* nv,

@@ -41,6 +41,7 @@ SIGNATURES: dict[str, list[Any]] = {
     'pcube_focus_wheel': [c_int, P_u32, P_u32, P_u32, u32, P_u64, P_u32],
     'pcube_focus_windows': [c_int, u32, P_u64, P_u64, u64, u64, P_u64],
     'pcube_tile_ranges': [u64, u64, u64, u64, u64, u64, u64, u64, u64, P_u64, P_u32],
+    'pcube_tile_ranges_many': [u64, u64, u64, u64, u64, u64, u32, P_u64, P_u64, P_u64, P_u64, u64, P_u32],
     'pcube_prepare_range': [c_int, c_int, u64, u64, u64, u64, u64, u64, u64, u64, u64, P_u64, P_u32, u64, u64, u64,
                             u64, P_u64, P_u64, P_u64, P_u64],
     'pcube_prepare_match': [c_int, c_int, u64, u64, u64, u64, u64, u64, u64, u64, u64, P_u64, P_u32, u64, u64, u64,
