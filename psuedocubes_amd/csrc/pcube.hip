@@ -782,7 +782,8 @@ int staged_enqueue(Matcher& mt, MatchArgs m, hipEvent_t ev_pre, hipEvent_t ev_ma
     // the work list of (piece, chunk) items, its survivor regions and their counts
     const uint32_t npieces = (m.nv + kPieceRows - 1) / kPieceRows;
     const uint32_t items_cap = npieces * m.nstage;   // staged_fits bounds it
-    TRY(mt.items.ensure(((uint64_t)items_cap + 1) * sizeof(uint2) + (uint64_t)items_cap * 4));
+    // items, their count, each item's survivor count, each piece's reach
+    TRY(mt.items.ensure(((uint64_t)items_cap + 1) * sizeof(uint2) + (uint64_t)items_cap * 4 + (uint64_t)npieces * 8));
     uint2* items = mt.items.as<uint2>();
     uint32_t* nitems = (uint32_t*)(items + items_cap);
     m.item_count = nitems + 2;
@@ -800,6 +801,7 @@ int staged_enqueue(Matcher& mt, MatchArgs m, hipEvent_t ev_pre, hipEvent_t ev_ma
     for (int k = 0; k < kStagedFilters; ++k) pa.chunk_base[k] = mt.s_chunk_base[k];
     pa.items = items;
     pa.nitems = nitems;
+    pa.reach = m.item_count + items_cap;
     uint32_t dir_stride, dir_entries;
     directory_shape(m.nu, dir_stride, dir_entries);
     hipLaunchKernelGGL(k_prefilter_staged, dim3(1 + pa.rows_blocks + pa.mask_blocks + pa.rho_blocks), dim3(256),
@@ -809,13 +811,14 @@ int staged_enqueue(Matcher& mt, MatchArgs m, hipEvent_t ev_pre, hipEvent_t ev_ma
     // them and stands for its own timing events
     mt.kernel_span = ev_pre && ev_match;
     if (m.nu) {
-        // one persistent block per CU over the work list and the residual chain
+        // kStagedBlocksPerCU persistent blocks per CU over the work list and the residual chain
         const uint2* items_arg = items;
         const uint32_t* nitems_arg = nitems;
         void* args[] = {&m, &items_arg, (void*)&items_cap, &nitems_arg};
         if (ev_k0 && !mt.kernel_span) HIP_TRY(hipEventRecord(ev_k0, g.stream));
-        HIP_TRY(hipLaunchKernel(staged_kernel(mt.sbits, mt.nres), dim3((uint32_t)g.compute_units),
-                                dim3(kStagedThreads), args, 0, g.stream));
+        HIP_TRY(hipLaunchKernel(staged_kernel(mt.sbits, mt.nres),
+                                dim3((uint32_t)(kStagedBlocksPerCU * g.compute_units)), dim3(kStagedThreads), args, 0,
+                                g.stream));
         if (ev_k1 && !mt.kernel_span) HIP_TRY(hipEventRecord(ev_k1, g.stream));
     }
     if (ev_match) HIP_TRY(hipEventRecord(ev_match, g.stream));
@@ -833,7 +836,7 @@ bool staged_fits(const Matcher& mt, uint64_t nu, uint64_t nv) {
 // Internal engine code: the residue-bitset engine without LDS staging.
 constexpr int kUnstagedBitsets = 2;
 // Bitset survivors one staged work item may hold before the unstaged engine is
-// used instead (an item covers 2048 v rows x 256 u coordinates).
+// used instead (an item covers kPieceRows = 1024 v rows x 256 u coordinates).
 constexpr uint64_t kMaxItemCandidates = 1ull << 16;
 
 // Enqueue the kernels of one batch; counters must be zeroed by the caller.
@@ -2508,19 +2511,19 @@ int pcube_lds_probe(uint32_t pattern, uint32_t rounds, double* seconds, double* 
     if (pattern > 2 || !rounds) return fail(PCUBE_INVALID, "probe pattern 0..2 and at least one round");
     const uint32_t grid = (uint32_t)g.compute_units;
     void* out = nullptr;
-    TRY(dev_alloc((uint64_t)grid * kStagedThreads * 4, &out));
+    TRY(dev_alloc((uint64_t)grid * kProbeThreads * 4, &out));
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
-    hipLaunchKernelGGL(k_lds_probe, dim3(grid), dim3(kStagedThreads), 0, g.stream, pattern, rounds, (uint32_t*)out);
+    hipLaunchKernelGGL(k_lds_probe, dim3(grid), dim3(kProbeThreads), 0, g.stream, pattern, rounds, (uint32_t*)out);
     HIP_TRY(hipEventRecord(e0, g.stream));
-    hipLaunchKernelGGL(k_lds_probe, dim3(grid), dim3(kStagedThreads), 0, g.stream, pattern, rounds, (uint32_t*)out);
+    hipLaunchKernelGGL(k_lds_probe, dim3(grid), dim3(kProbeThreads), 0, g.stream, pattern, rounds, (uint32_t*)out);
     HIP_TRY(hipEventRecord(e1, g.stream));
     HIP_TRY(hipStreamSynchronize(g.stream));
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
     if (seconds) *seconds = ms / 1000.0;
-    if (bytes) *bytes = (double)grid * kStagedThreads * rounds * 16 * 16;
+    if (bytes) *bytes = (double)grid * kProbeThreads * rounds * 16 * 16;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     dev_free(out);

@@ -18,12 +18,14 @@
 // column offsets in the order its lane reads them (the row's lane is fixed by
 // its index).
 //
-// Work item = (stage chunk s, piece of kPieceRows consecutive v rows).  One
-// persistent block per CU walks a list of items; for each it intersects the
-// chunk, staged in LDS, with every row of the piece: a thread takes whole rows
-// and ANDs, for each of the chunk's words, the kStagedFilters words of its
-// row's residue classes.  The next item's chunk streams into a second LDS
-// buffer (global_load_lds) while the current one is matched.  Positions
+// Work item = (stage chunk s, piece of kPieceRows consecutive v rows).  Two
+// persistent blocks per CU walk lists of items; for each, a block intersects the
+// chunk, staged in its LDS, with every row of the piece: a thread takes whole
+// rows and ANDs, for each of the chunk's words, the kStagedFilters words of its
+// row's residue classes.  Each block holds one chunk image: the next item's
+// chunk streams in (global_load_lds) once every wave has finished with the
+// current one, while the block writes the item's survivors and counts, and the
+// other block of the CU keeps its SIMDs busy meanwhile.  Positions
 // surviving all bitset filters (about 1 in 6561 at p = 619) are written to the
 // item's region of a candidate list, and the block runs them through the rest
 // of the filter chain after its items, exactly as in the residue-bitset engine: cached
@@ -41,11 +43,13 @@
 namespace pcube {
 
 constexpr int kStageWords = 4;        // mask words (256 u coordinates) per stage chunk
-constexpr int kPieceRows = 2048;      // v rows per work item
-constexpr int kStagedThreads = 1024;  // one persistent block per CU
+constexpr int kPieceRows = 1024;      // v rows per work item
+constexpr int kStagedThreads = 512;   // threads of a persistent block
+constexpr int kStagedBlocksPerCU = 2; // persistent blocks per CU (4 waves per SIMD, 128 VGPRs)
 constexpr int kStagedWaves = kStagedThreads / kWave;
 constexpr int kRowsPerLane = kPieceRows / kStagedThreads;   // rows of a piece each thread takes
 constexpr int kTileSlots = 64;        // subtiles whose group counts a block keeps in LDS
+constexpr int kItemGroups = 2;        // groups counted by the intersection itself (popcounts)
 constexpr int kStageLines = 280;      // lines of a chunk image: the largest staged q
 constexpr int kStageLineBytes = kStagedFilters * kStageWords * 8;   // 8 columns of 32 bytes
 static_assert(kStagedFilters == 8 && kStageLineBytes == 256, "the read rotation assumes 8 columns of 32 bytes");
@@ -123,7 +127,8 @@ __device__ __forceinline__ void stage_copy(uint4* __restrict__ dst, const uint4*
 struct Staged {
     const uint32_t* allow;
     const Modulus* mods;
-    uint32_t* tilecnt;       // [kTileSlots][ng]: groups 1..ng of subtiles t0..t0+63
+    uint32_t* tilecnt;       // [kTileSlots][stride]: groups 1..stride of subtiles t0..t0+63
+    uint32_t stride;         // groups per subtile: kItemGroups for the items, ng for the chain
     uint32_t* queue;         // this wave's survivors, (row offset << 8) | u offset within the item
 };
 
@@ -132,7 +137,7 @@ struct Staged {
 __device__ __forceinline__ void staged_add(const MatchArgs& m, const Staged& S, uint32_t t0, uint32_t tile, uint32_t g,
                                            uint32_t n) {
     const uint32_t slot = tile - t0;
-    if (slot < (uint32_t)kTileSlots) atomicAdd(&S.tilecnt[slot * m.ng + g - 1], n);
+    if (slot < (uint32_t)kTileSlots) atomicAdd(&S.tilecnt[slot * S.stride + g - 1], n);
     else atomicAdd(&m.counters[(uint64_t)tile * (m.ng + 3) + g], (uint64_t)n);
 }
 
@@ -227,51 +232,67 @@ __device__ __forceinline__ void piece_ends(const MatchArgs& m, uint32_t p, uint6
     last = m.vs[j1 - 1];
 }
 
-// Chunks [c_lo, c_hi] that the rows of a piece reach (none when c_lo > c_hi):
-// rows are sorted by v, so from the u bounds of the piece's first and last
-// rows (coordinates v_first, v_last), computed exactly as the row builder does.
-__device__ __forceinline__ void piece_chunks(const MatchArgs& m, uint64_t v_first, uint64_t v_last,
-                                             const uint64_t* dir, uint32_t ndir, uint32_t stride, uint32_t& c_lo,
-                                             uint32_t& c_hi) {
+// The u indices [left0, right1) that the rows of a piece reach: rows are sorted
+// by v, so from the exact u bounds of the piece's first row's lower end and its
+// last row's upper end (coordinates v_first, v_last), computed as the row
+// builder computes them.  Both ends are nondecreasing over the pieces.
+__device__ __forceinline__ void piece_reach(const MatchArgs& m, uint64_t v_first, uint64_t v_last,
+                                            const uint64_t* dir, uint32_t ndir, uint32_t stride, uint32_t& left0,
+                                            uint32_t& right1) {
     const U128 lower = add_wide(mul_wide(v_first, m.A), m.lo);
     const U128 upper = add_wide(mul_wide(v_last, m.A), m.hi);
-    uint32_t left0, right1;
     uint64_t unused;
     first_above_pair<false>(m, lower, upper, dir, ndir, stride, left0, right1, unused);
-    c_lo = 1;
-    c_hi = 0;
-    if (right1 > left0) {
-        c_lo = (left0 >> 6) / kStageWords;
-        c_hi = min(((right1 - 1) >> 6) / kStageWords, m.nstage - 1);
-    }
 }
 
-// Work list of the staged match: every (piece, chunk) pair whose rows reach
-// the chunk, piece-major, at most items_cap of them; nitems receives the full
-// count.  One 256-thread block: thread t takes a run of consecutive pieces, a
-// block scan turns the runs' counts into offsets.  With at most 256 pieces
-// (one per thread) the block then writes the items together, coalesced.
+// First index in [0, n) whose bound (every second word of bounds, from `at`) is at
+// least key; the bounds are nondecreasing.  n when there is none.
+__device__ __forceinline__ uint32_t first_at_least(const uint32_t* bounds, uint32_t at, uint32_t n, uint32_t key) {
+    uint32_t l = 0, h = n;
+    while (l < h) {
+        const uint32_t mid = (l + h) >> 1;
+        if (bounds[2 * mid + at] < key) l = mid + 1; else h = mid;
+    }
+    return l;
+}
+
+// Work list of the staged match, chunk-major: for every stage chunk c (u indices
+// [256c, 256c + 256)), the pieces whose rows reach it, in piece order; at most
+// items_cap items, nitems receives the full count.  With piece p reaching
+// [left0_p, right1_p), both nondecreasing in p, the pieces reaching chunk c are
+// the run from the first with right1 > 256c to the last with left0 < 256c + 256.
+// (A piece whose rows reach nothing, right1 <= left0, can lie inside a run: its
+// item has no work.)  Chunk-major order lets a block walking consecutive items
+// keep one chunk image in LDS while only the row headers change.
+// One 256-thread block: the pieces' reach goes to `reach` (two words per piece),
+// then thread t takes a run of consecutive chunks and a block scan turns the
+// runs' item counts into offsets.
 __device__ __forceinline__ void staged_plan_block(const MatchArgs& m, uint32_t npieces, uint2* items,
-                                                  uint32_t items_cap, uint32_t* nitems, uint64_t* dir) {
+                                                  uint32_t items_cap, uint32_t* nitems, uint32_t* reach,
+                                                  uint64_t* dir) {
     __shared__ uint32_t wave_total[256 / kWave];
-    __shared__ uint32_t piece_at[256], piece_lo[256];   // first item and first chunk of each piece
     const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    const uint32_t per = (npieces + 255) / 256;
-    const uint32_t p0 = min(npieces, threadIdx.x * per), p1 = min(npieces, p0 + per);
-    // the first piece's end rows, issued before the directory's loads
-    uint64_t v_first = 0, v_last = 0;
-    if (p0 < p1) piece_ends(m, p0, v_first, v_last);
     uint32_t ndir, stride;
     load_directory(m, dir, ndir, stride);
     const uint64_t* d = ndir ? dir : nullptr;
-    uint32_t count = 0, lo1 = 1, hi1 = 0;   // the range of a one-piece run is kept
-    for (uint32_t p = p0; p < p1; ++p) {
-        if (p != p0) piece_ends(m, p, v_first, v_last);
-        uint32_t lo, hi;
-        piece_chunks(m, v_first, v_last, d, ndir, stride, lo, hi);
-        if (lo <= hi) count += hi - lo + 1;
-        lo1 = lo;
-        hi1 = hi;
+    for (uint32_t p = threadIdx.x; p < npieces; p += 256) {
+        uint64_t v_first, v_last;
+        piece_ends(m, p, v_first, v_last);
+        uint32_t left0, right1;
+        piece_reach(m, v_first, v_last, d, ndir, stride, left0, right1);
+        reach[2 * p] = left0;
+        reach[2 * p + 1] = right1;
+    }
+    __threadfence_block();
+    __syncthreads();
+    constexpr uint32_t kChunkU = kStageWords * 64;
+    const uint32_t per = (m.nstage + 255) / 256;
+    const uint32_t c0 = min(m.nstage, threadIdx.x * per), c1 = min(m.nstage, c0 + per);
+    uint32_t count = 0;
+    for (uint32_t c = c0; c < c1; ++c) {
+        const uint32_t a = first_at_least(reach, 1, npieces, c * kChunkU + 1);   // right1 > 256c
+        const uint32_t b = first_at_least(reach, 0, npieces, (c + 1) * kChunkU);  // left0 >= 256c + 256
+        if (b > a) count += b - a;
     }
     uint32_t inc = count;
 #pragma unroll
@@ -283,35 +304,13 @@ __device__ __forceinline__ void staged_plan_block(const MatchArgs& m, uint32_t n
     __syncthreads();
     uint32_t at = inc - count;
     for (uint32_t w = 0; w < wave; ++w) at += wave_total[w];
-    if (per == 1) {
-        const uint32_t total = wave_total[0] + wave_total[1] + wave_total[2] + wave_total[3];
-        piece_at[threadIdx.x] = at;
-        piece_lo[threadIdx.x] = lo1;
-        __syncthreads();
-        const uint32_t n = min(total, items_cap);
-        for (uint32_t idx = threadIdx.x; idx < n; idx += 256) {
-            // the piece holding item idx: the last p with piece_at[p] <= idx
-            // (pieces without items share the next piece's start)
-            uint32_t l = 0, h = npieces;
-            while (h - l > 1) {
-                const uint32_t mid = (l + h) >> 1;
-                if (piece_at[mid] <= idx) l = mid; else h = mid;
-            }
-            items[idx] = make_uint2(l, piece_lo[l] + (idx - piece_at[l]));
-        }
-        if (threadIdx.x == 0) *nitems = total;
-        return;
+    for (uint32_t c = c0; c < c1; ++c) {
+        const uint32_t a = first_at_least(reach, 1, npieces, c * kChunkU + 1);
+        const uint32_t b = first_at_least(reach, 0, npieces, (c + 1) * kChunkU);
+        for (uint32_t q = a; q < b; ++q, ++at)
+            if (at < items_cap) items[at] = make_uint2(q, c);
     }
-    for (uint32_t p = p0; p < p1; ++p) {
-        uint32_t lo = lo1, hi = hi1;
-        if (p1 - p0 > 1) {
-            piece_ends(m, p, v_first, v_last);
-            piece_chunks(m, v_first, v_last, d, ndir, stride, lo, hi);
-        }
-        for (uint32_t c = lo; c <= hi && lo <= hi; ++c, ++at)
-            if (at < items_cap) items[at] = make_uint2(p, c);
-    }
-    if (threadIdx.x == 255) *nitems = at;
+    if (threadIdx.x == 255) *nitems = wave_total[0] + wave_total[1] + wave_total[2] + wave_total[3];
 }
 
 // Geometry of the fused prefilter launch.
@@ -320,6 +319,7 @@ struct PrefilterArgs {
     uint32_t chunk_base[kStagedFilters];
     uint2* items;
     uint32_t* nitems;
+    uint32_t* reach;     // [npieces][2]: the u indices each piece's rows reach
 };
 
 // Everything the staged match needs, in one launch of independent 256-thread
@@ -333,7 +333,7 @@ __global__ __launch_bounds__(256) void k_prefilter_staged(MatchArgs m, Prefilter
     uint64_t* dir = prefilter_dir;
     uint32_t b = blockIdx.x;
     if (b == 0) {
-        staged_plan_block(m, pa.npieces, pa.items, pa.items_cap, pa.nitems, dir);
+        staged_plan_block(m, pa.npieces, pa.items, pa.items_cap, pa.nitems, pa.reach, dir);
         return;
     }
     b -= 1;
@@ -381,21 +381,21 @@ __device__ __forceinline__ void stage_async(const MatchArgs& m, uint32_t s, uint
     }
 }
 
-// Static LDS of the staged match (one block per CU): two mask buffers, so the
-// next item's chunk can stream in while the current one is read, plus the
-// per-subtile counters, survivor queues and the item's fill counter.  The
-// chain after the items keeps the filter tables in the first buffer.
-// Distinct arrays let the compiler see that reads of one buffer do not wait
-// for the direct-to-LDS copies into the other.
+// Static LDS of the staged match (two blocks per CU): one chunk image, the
+// per-subtile counts of groups 1 and 2, the survivor queues and the item's fill
+// counter.  The chain after the items keeps the filter tables and the per-subtile
+// counts of every group in the image buffer.
 constexpr int kStageBufferUnits = kStageLines * kStageLineBytes / 16;
 constexpr uint32_t kStagedLdsBytes =
-    2 * kStageBufferUnits * 16 + kTileSlots * (kMaxResiduals / 4) * 4 + kStagedWaves * kQueue * 4 + 16;
-static_assert(kMaxAllowWords * 4 + kMaxResiduals * sizeof(Modulus) <= kStageBufferUnits * 16,
-              "the chain's filter tables fit one mask buffer");
+    kStageBufferUnits * 16 + kTileSlots * kItemGroups * 4 + kStagedWaves * kQueue * 4 + 64;
+static_assert(kStagedBlocksPerCU * kStagedLdsBytes <= 160 * 1024, "two staged blocks share one CU's LDS");
+static_assert(kMaxAllowWords * 4 + kMaxResiduals * sizeof(Modulus) + kTileSlots * (kMaxResiduals / 4) * 4 <=
+                  kStageBufferUnits * 16,
+              "the chain's filter tables and group counts fit the mask buffer");
+static_assert(kPieceRows <= (1 << 24) && kStageWords * 64 <= 256, "survivor tags hold row and u offsets");
 
 // A block's walk over its work items: logical position k is item
-// first + k * stride (strided: block b takes b, b + grid, ...; so the blocks
-// running at one time share pieces and chunk images in L2).  Each lane holds
+// first + k * stride (the staged match walks one contiguous run, stride 1).  Each lane holds
 // one entry of a 64-item window of the block's sequence (desc = its item at
 // logical position window + lane), so a descriptor is a register read.
 struct ItemState {
@@ -462,25 +462,80 @@ __device__ __forceinline__ void flush_groups(const MatchArgs& m, const Staged& S
     }
 }
 
+// One row's intersection with the staged chunk, reading its 16 mask halves in the
+// rotated order of its lane g (see the header): slots 0-3 (group X: filters 0-3
+// for g < 8, 4-7 otherwise) then slots 4-7 (group Y, the other four), first the
+// half P = (g >> 2) & 1 of every slot, then the other half Q.  c1 counts the
+// positions passing group A (X for g < 8, Y otherwise) and c2 those passing both
+// groups; words receives the four words of survivors.  kRange: apply the row's
+// chunk-relative u range [dl, dr) (some row of the wave covers the chunk only in
+// part); without it every position of the chunk is in range.
+template <int kF, bool kLast, bool kRange>
+__device__ __forceinline__ void staged_row(const uint8_t* masks_p, const uint8_t* masks_q, int first_p, int first_q,
+                                           uint32_t g, const uint4& h1, int dl, int dr, uint32_t& c1, uint32_t& c2,
+                                           uint64_t (&words)[4]) {
+    // group 1 closes after filter 3 (or after the last one), group 2 after filter 7
+    constexpr bool kCount1 = kF >= 4 || kLast;
+    constexpr bool kCount2 = kF == 8 || (kLast && kF > 4);
+    const uint32_t offs[kStagedFilters] = {h1.x & 0xffffu, h1.x >> 16, h1.y & 0xffffu, h1.y >> 16,
+                                           h1.z & 0xffffu, h1.z >> 16, h1.w & 0xffffu, h1.w >> 16};
+    uint64_t both[2][2];
+    uint32_t acc_x = 0, acc_y = 0, acc2 = 0;
+    // half P of all eight slots, then half Q: the eight reads of a half are in
+    // flight together, and a half is reduced to its counts and survivors at once
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const uint8_t* base = h == 0 ? masks_p : masks_q;
+        const int first = h == 0 ? first_p : first_q;
+        uint4 v[kStagedFilters];
+#pragma unroll
+        for (int i = 0; i < kStagedFilters; ++i) v[i] = *(const uint4*)(base + (offs[i] << 5));
+        const uint4 x01 = and3(v[0], v[1], v[2]), y01 = and3(v[4], v[5], v[6]);
+        const uint4 xh = make_uint4(x01.x & v[3].x, x01.y & v[3].y, x01.z & v[3].z, x01.w & v[3].w);
+        const uint4 yh = make_uint4(y01.x & v[7].x, y01.y & v[7].y, y01.z & v[7].z, y01.w & v[7].w);
+#pragma unroll
+        for (int w = 0; w < 2; ++w) {
+            uint64_t x = half_word(xh, w), y = half_word(yh, w);
+            if (kRange) {
+                const uint64_t range = word_range(dl, dr, first + 64 * w);
+                x &= range;
+                y &= range;
+            }
+            both[h][w] = x & y;
+            if (kCount1) {
+                acc_x += __popcll(x);
+                acc_y += __popcll(y);
+            }
+            if (kCount2) acc2 += __popcll(both[h][w]);
+        }
+        // eight reads in flight at a time keep the registers within four waves per SIMD
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    c1 = g < 8 ? acc_x : acc_y;
+    c2 = acc2;
+    words[0] = both[0][0];
+    words[1] = both[0][1];
+    words[2] = both[1][0];
+    words[3] = both[1][1];
+}
+
 // Match one work item reading chunk buffer `cur` while the next item's chunk
 // (and, when the piece changes, its row headers) stream in.  Returns false
 // when the block has no further item.
 //
-// Per row a thread reads 16 mask halves in the rotated order of its lane
-// g = lane mod 16 (see the header): slots 0-3 (group X: filters 0-3 for
-// g < 8, 4-7 otherwise) then slots 4-7 (group Y, the other four), first the
-// half P = (g >> 2) & 1 of every slot, then the other half Q.  Group 1 counts
-// positions passing group A (X for g < 8, Y otherwise), group 2 those passing
-// both.
+// Match one work item on the block's chunk image: each thread takes its rows of
+// the piece through staged_row (a wave whose rows all miss the chunk skips it,
+// one whose rows all cover it whole skips the range masks).  Once every wave is
+// done with the image, the next item's chunk starts streaming into it while the
+// block writes this item's survivors and group counts; the next piece's row
+// headers, when the piece changes, are in flight during the whole item.
+// Returns false when the block has no further item.
 template <int kF, bool kLast>
 __device__ __forceinline__ bool staged_item(const MatchArgs& m, const uint2* items, const Staged& S, ItemState& st,
-                                            const uint4 (&cur)[kStageBufferUnits], uint4 (&nxt)[kStageBufferUnits],
-                                            uint32_t* fills, uint32_t* wave_counts, uint32_t* piece_t0, uint32_t lane,
+                                            uint4 (&image)[kStageBufferUnits], uint32_t* fills,
+                                            uint32_t* wave_counts, uint32_t* piece_t0, uint32_t lane,
                                             uint32_t wave) {
     static_assert(kRowsPerLane == 2, "a thread's rows of a piece are counted as two records");
-    // group 1 closes after filter 3 (or after the last one), group 2 after filter 7
-    constexpr bool kCount1 = kF >= 4 || kLast;
-    constexpr bool kCount2 = kF == 8 || (kLast && kF > 4);
     const uint32_t p = st.it.x, s = st.it.y;
     const uint32_t j0 = p * kPieceRows, j1 = min(m.nv, j0 + kPieceRows);
     const uint32_t item = item_index(st, st.k);
@@ -495,20 +550,20 @@ __device__ __forceinline__ bool staged_item(const MatchArgs& m, const uint2* ite
     uint32_t* region = m.cand + (uint64_t)item * m.item_cap;
     __syncthreads();
     const uint32_t t0 = *piece_t0;
-    // the next item's chunk, and its piece's rows when the piece changes, are in flight meanwhile
+    // the next piece's rows are in flight while this item is matched
     const uint32_t next = st.k + 1;
     const ItemRows cur_rows = st.rows;
     uint2 nit = st.it;
     if (next < st.count) {
         nit = item_at(items, st, next, lane);
         if (nit.x != p) load_item_rows(m, nit.x, st.rows);
-        stage_async(m, nit.y, (uint8_t*)nxt, wave, lane);
     }
     const uint32_t g = lane & 15, hp = (g >> 2) & 1;
-    const uint8_t* masks_p = (const uint8_t*)cur + 16 * hp;          // half P of a column
-    const uint8_t* masks_q = (const uint8_t*)cur + 16 * (hp ^ 1);    // half Q
-    const int first_p = 128 * (int)hp, first_q = 128 - first_p;      // chunk offset of each half's first word
-    const uint32_t base0 = (s * kStageWords) << 6;                   // first u index of the chunk
+    const uint8_t* masks_p = (const uint8_t*)image + 16 * hp;          // half P of a column
+    const uint8_t* masks_q = (const uint8_t*)image + 16 * (hp ^ 1);    // half Q
+    const int first_p = 128 * (int)hp, first_q = 128 - first_p;        // chunk offset of each half's first word
+    const uint32_t base0 = (s * kStageWords) << 6;                     // first u index of the chunk
+    constexpr int kChunkU = kStageWords * 64;
     uint32_t c1[kRowsPerLane] = {}, c2[kRowsPerLane] = {}, tiles[kRowsPerLane] = {};
     bool has[kRowsPerLane] = {};
     uint32_t queued = 0;
@@ -522,40 +577,16 @@ __device__ __forceinline__ bool staged_item(const MatchArgs& m, const uint2* ite
             tiles[t] = h0.z;
             // range of the row's u indices within the chunk
             const int dl = live ? (int)(h0.x - base0) : 0, dr = live ? (int)(h0.y - base0) : 0;
-            const uint32_t offs[kStagedFilters] = {h1.x & 0xffffu, h1.x >> 16, h1.y & 0xffffu, h1.y >> 16,
-                                                   h1.z & 0xffffu, h1.z >> 16, h1.w & 0xffffu, h1.w >> 16};
-            uint64_t both[2][2];
-            uint32_t acc_x = 0, acc_y = 0, acc2 = 0;
-            // half P of all eight slots, then half Q: the eight reads of a half are in
-            // flight together, and a half is reduced to its counts and survivors at once
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const uint8_t* base = h == 0 ? masks_p : masks_q;
-                const int first = h == 0 ? first_p : first_q;
-                uint4 v[kStagedFilters];
-#pragma unroll
-                for (int i = 0; i < kStagedFilters; ++i) v[i] = *(const uint4*)(base + (offs[i] << 5));
-                const uint4 x01 = and3(v[0], v[1], v[2]), y01 = and3(v[4], v[5], v[6]);
-                const uint4 xh = make_uint4(x01.x & v[3].x, x01.y & v[3].y, x01.z & v[3].z, x01.w & v[3].w);
-                const uint4 yh = make_uint4(y01.x & v[7].x, y01.y & v[7].y, y01.z & v[7].z, y01.w & v[7].w);
-#pragma unroll
-                for (int w = 0; w < 2; ++w) {
-                    const uint64_t range = word_range(dl, dr, first + 64 * w);
-                    const uint64_t x = range & half_word(xh, w), y = range & half_word(yh, w);
-                    both[h][w] = x & half_word(yh, w);
-                    if (kCount1) {
-                        acc_x += __popcll(x);
-                        acc_y += __popcll(y);
-                    }
-                    if (kCount2) acc2 += __popcll(both[h][w]);
-                }
-                // eight reads in flight at a time keep the registers within four waves per SIMD
-                __builtin_amdgcn_sched_barrier(0);
-            }
-            c1[t] = g < 8 ? acc_x : acc_y;
-            c2[t] = acc2;
-            // a few survivors per wave and row: all four words in one append loop
-            uint64_t words[4] = {both[0][0], both[0][1], both[1][0], both[1][1]};
+            const bool touches = live && dr > 0 && dl < kChunkU;
+            const bool full = touches && dl <= 0 && dr >= kChunkU;
+            // no row of the wave reaches the chunk: nothing to read or count
+            if (!__ballot(touches)) continue;
+            uint64_t words[4];
+            // every row of the wave covers the whole chunk (most items): no range masks
+            if (!__ballot(!full))
+                staged_row<kF, kLast, false>(masks_p, masks_q, first_p, first_q, g, h1, dl, dr, c1[t], c2[t], words);
+            else
+                staged_row<kF, kLast, true>(masks_p, masks_q, first_p, first_q, g, h1, dl, dr, c1[t], c2[t], words);
             const int firsts[4] = {first_p, first_p + 64, first_q, first_q + 64};
             staged_append(S, lane, words, firsts, (j - j0) << 8, queued, region, m.item_cap, fill);
         }
@@ -571,7 +602,9 @@ __device__ __forceinline__ bool staged_item(const MatchArgs& m, const uint2* ite
     }
     flush_groups(m, S, t0, has[0], tiles[0], c1[0], c2[0], lane);
     flush_groups(m, S, t0, has[1], tiles[1], c1[1], c2[1], lane);
-    __syncthreads();
+    __syncthreads();   // every wave is done with the image and has queued its survivors
+    // the next item of another chunk streams its image in; the same chunk keeps it
+    if (next < st.count && nit.y != s) stage_async(m, nit.y, (uint8_t*)image, wave, lane);
     // the waves' queued survivors follow the flushed ones, in wave order
     {
         const uint32_t n = lane < (uint32_t)kStagedWaves ? wave_counts[lane] : 0u;
@@ -580,10 +613,10 @@ __device__ __forceinline__ bool staged_item(const MatchArgs& m, const uint2* ite
         if (lane < queued && at + lane < m.item_cap) region[at + lane] = S.queue[lane];
         if (threadIdx.x == 0) m.item_count[item] = *fill + queued_all;
     }
-    for (uint32_t t = threadIdx.x; t < (uint32_t)kTileSlots * m.ng; t += kStagedThreads) {
+    for (uint32_t t = threadIdx.x; t < (uint32_t)(kTileSlots * kItemGroups); t += kStagedThreads) {
         const uint32_t n = S.tilecnt[t];
         if (n) {
-            atomicAdd(&m.counters[(uint64_t)(t0 + t / m.ng) * (m.ng + 3) + t % m.ng + 1], (uint64_t)n);
+            atomicAdd(&m.counters[(uint64_t)(t0 + t / kItemGroups) * (m.ng + 3) + t % kItemGroups + 1], (uint64_t)n);
             S.tilecnt[t] = 0;
         }
     }
@@ -593,55 +626,61 @@ __device__ __forceinline__ bool staged_item(const MatchArgs& m, const uint2* ite
     return true;
 }
 
-// Persistent staged match: block b takes work items b, b + grid, ...  For an
-// item (piece p, chunk s) thread l intersects the staged chunk with rows
-// j0 + l + 1024 t of the piece; the next item's chunk and row headers are in
-// flight meanwhile.  Group counts of the piece's subtiles collect in LDS and
-// go to the global counters at the end of each item.  After its last item the
-// block runs its items' survivors through the residual chain.  kF = bitset
-// filters; kLast: the last residual filter is the last bitset filter (it
-// closes a group).
+// Persistent staged match, kStagedBlocksPerCU blocks per CU, each over one
+// contiguous run of the chunk-major work list.  For an item (piece p, chunk s) thread l intersects the
+// staged chunk with rows j0 + l + 512 t of the piece.  Group 1 and 2 counts of
+// the piece's subtiles collect in LDS and go to the global counters at the end
+// of each item.  After its last item the block runs its items' survivors
+// through the residual chain, with the filter tables and the counts of the
+// later groups in the image buffer.  kF = bitset filters; kLast: the last
+// residual filter is the last bitset filter (it closes a group).  The waves-per-EU
+// bound keeps the kernel within 128 VGPRs, so both blocks of a CU stay resident.
 template <int kF, bool kLast>
-__global__ __launch_bounds__(kStagedThreads) void k_match_staged(MatchArgs m, const uint2* items, uint32_t items_cap,
-                                                                 const uint32_t* nitems) {
-    __shared__ uint4 buffer0[kStageBufferUnits];
-    __shared__ uint4 buffer1[kStageBufferUnits];
-    __shared__ uint32_t tilecnt[kTileSlots * (kMaxResiduals / 4)];
+__global__ __launch_bounds__(kStagedThreads) __attribute__((amdgpu_waves_per_eu(4)))
+void k_match_staged(MatchArgs m, const uint2* items, uint32_t items_cap, const uint32_t* nitems) {
+    __shared__ uint4 image[kStageBufferUnits];
+    __shared__ uint32_t tilecnt[kTileSlots * kItemGroups];
     __shared__ uint32_t queues[kStagedWaves * kQueue];
     __shared__ uint32_t fills[2];                     // survivors flushed during an item, by item parity
     __shared__ uint32_t wave_counts[kStagedWaves];    // survivors each wave holds at an item's end
     __shared__ uint32_t piece_t0;                     // subtile of the current piece's first row
     const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     const uint32_t total = min(*nitems, items_cap);
-    if (blockIdx.x >= total) return;
+    // each block takes one contiguous run of the chunk-major list, so consecutive
+    // items mostly share their chunk image; consecutive block ids go round robin
+    // over the 8 XCDs, so the runs are grouped by XCD, and each XCD's blocks cover
+    // one stretch of the list (its chunk images and row headers in that XCD's L2)
+    const uint32_t grid = gridDim.x;
+    const uint32_t run = grid % 8 == 0 ? (blockIdx.x % 8) * (grid / 8) + blockIdx.x / 8 : blockIdx.x;
+    const uint32_t first = (uint32_t)((uint64_t)run * total / grid);
+    const uint32_t last = (uint32_t)((uint64_t)(run + 1) * total / grid);
+    if (first >= last) return;
     ItemState st;
-    st.first = blockIdx.x;
-    st.stride = gridDim.x;
-    st.count = (total - blockIdx.x + gridDim.x - 1) / gridDim.x;
+    st.first = first;
+    st.stride = 1;
+    st.count = last - first;
     st.k = 0;
-    for (uint32_t t = threadIdx.x; t < (uint32_t)kTileSlots * m.ng; t += kStagedThreads) tilecnt[t] = 0;
-    const Staged S{nullptr, nullptr, tilecnt, queues + wave * kQueue};
+    for (uint32_t t = threadIdx.x; t < (uint32_t)(kTileSlots * kItemGroups); t += kStagedThreads) tilecnt[t] = 0;
+    const Staged S{nullptr, nullptr, tilecnt, kItemGroups, queues + wave * kQueue};
     load_window(items, st, 0, lane);
     st.it = item_at(items, st, 0, lane);
     load_item_rows(m, st.it.x, st.rows);
-    stage_async(m, st.it.y, (uint8_t*)buffer0, wave, lane);
-    for (;;) {
-        if (!staged_item<kF, kLast>(m, items, S, st, buffer0, buffer1, fills, wave_counts, &piece_t0, lane, wave))
-            break;
-        if (!staged_item<kF, kLast>(m, items, S, st, buffer1, buffer0, fills, wave_counts, &piece_t0, lane, wave))
-            break;
+    stage_async(m, st.it.y, (uint8_t*)image, wave, lane);
+    while (staged_item<kF, kLast>(m, items, S, st, image, fills, wave_counts, &piece_t0, lane, wave)) {
     }
     // the residual chain over this block's items' survivors, 64 at a time, with
-    // the filter tables in the first mask buffer; the block's waves take the
-    // 64-survivor slices of its items in turn
+    // the filter tables and the later groups' counts in the image buffer; the
+    // block's waves take the 64-survivor slices of its items in turn
     __syncthreads();   // every item's region and count is written, no chunk is read any more
-    uint32_t* allow = (uint32_t*)buffer0;
+    uint32_t* allow = (uint32_t*)image;
     Modulus* mods = (Modulus*)(allow + kMaxAllowWords);
+    uint32_t* chain_counts = (uint32_t*)(mods + kMaxResiduals);
     // allow and the Modulus table are whole 16-byte units (both buffers are allocated so)
     stage_copy<kStagedThreads, 1>((uint4*)allow, (const uint4*)m.allow, (m.allow_words + 3) / 4);
     stage_copy<kStagedThreads, 1>((uint4*)mods, (const uint4*)m.mods, m.nres * (uint32_t)sizeof(Modulus) / 16);
+    for (uint32_t t = threadIdx.x; t < (uint32_t)kTileSlots * m.ng; t += kStagedThreads) chain_counts[t] = 0;
     __syncthreads();
-    const Staged C{allow, mods, tilecnt, queues + wave * kQueue};
+    const Staged C{allow, mods, chain_counts, m.ng, queues + wave * kQueue};
     uint32_t slice = 0;
     // the survivor counts and descriptors of up to 64 items arrive in one round
     // trip, a lane each; the items are then walked in registers
@@ -671,7 +710,7 @@ __global__ __launch_bounds__(kStagedThreads) void k_match_staged(MatchArgs m, co
     }
     __syncthreads();
     for (uint32_t t = threadIdx.x; t < (uint32_t)kTileSlots * m.ng; t += kStagedThreads) {
-        const uint32_t c = tilecnt[t];
+        const uint32_t c = chain_counts[t];
         if (c) atomicAdd(&m.counters[(uint64_t)(t / m.ng) * (m.ng + 3) + t % m.ng + 1], (uint64_t)c);
     }
 }
@@ -684,9 +723,10 @@ __global__ __launch_bounds__(kStagedThreads) void k_match_staged(MatchArgs m, co
 // Every address stays below 64 KB (lines < 256).  A round costs two 3-input
 // ORs per read, so the VALU keeps pace with the LDS.
 constexpr int kProbeUnits = 4096;
-__global__ __launch_bounds__(kStagedThreads) void k_lds_probe(uint32_t pattern, uint32_t rounds, uint32_t* out) {
+constexpr int kProbeThreads = 1024;   // one block of 16 waves per CU
+__global__ __launch_bounds__(kProbeThreads) void k_lds_probe(uint32_t pattern, uint32_t rounds, uint32_t* out) {
     __shared__ uint4 buf[kProbeUnits];
-    for (uint32_t t = threadIdx.x; t < (uint32_t)kProbeUnits; t += kStagedThreads)
+    for (uint32_t t = threadIdx.x; t < (uint32_t)kProbeUnits; t += kProbeThreads)
         buf[t] = make_uint4(t, 3 * t, 5 * t, 7 * t);
     __syncthreads();
     const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave, g = lane & 15;
@@ -717,7 +757,7 @@ __global__ __launch_bounds__(kStagedThreads) void k_lds_probe(uint32_t pattern, 
             acc1 = acc1 | v[i].z | v[i].w;
         }
     }
-    out[blockIdx.x * kStagedThreads + threadIdx.x] = acc0 ^ acc1;
+    out[blockIdx.x * kProbeThreads + threadIdx.x] = acc0 ^ acc1;
 }
 
 // Integer-VALU issue probe, the calibration of the VALU peak: eight independent
