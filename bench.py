@@ -57,7 +57,7 @@ def parse() -> argparse.Namespace:
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=2000)
     ap.add_argument('--warmup', type=int, default=50)
-    ap.add_argument('--cpu-seconds', type=float, default=12.0, help='bounded CPU-baseline sample')
+    ap.add_argument('--cpu-seconds', type=float, default=8.0, help='bounded CPU-baseline sample (config 2)')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-lease', action='store_true')
     ap.add_argument('--no-batched', action='store_true')
@@ -114,41 +114,103 @@ def lds_probe() -> None:
                               bytes_per_clock_per_cu_at_2_4ghz=rate / 256 / 2.4e9)), flush=True)
 
 
-def cpu_baseline(config, seconds: float) -> dict:
-    """The oracle (C, all host threads) on whole seeded batches for about `seconds`."""
+def row_digest(row: dict) -> dict:
+    """The fields of a batch row the CPU must reproduce bit for bit: every subtile's
+    scope record (valid count, group populations, local survivors, output hash) as
+    one SHA-256, the batch's group populations and its emitted values."""
+    import hashlib
+    from psuedocubes_amd.accounting import encoded
+    return dict(scopes_sha256=hashlib.sha256(encoded(row['subtile_scopes'])).hexdigest(),
+                group_survivors=row['group_survivors'], all_values=row['all_values'],
+                valid_focused_candidates=row['valid_focused_candidates'])
+
+
+def cpu_baseline(config, seconds: float, gpu_rows: list, small: tuple | None, lease: dict | None) -> dict:
+    """The oracle (C, the host's threads) on the GPU's own batches of this run.
+
+    Config 2 (the metric's workload): the first timed steps' seeded full batches,
+    recomputed on the CPU for about `seconds`; every CPU row must equal the GPU
+    row of the same batch (scope hashes, group populations, values) before any
+    time is reported.  Beside it, on the same terms: one thread on a few of those
+    batches, config 1 (the first 1e8-v subtile, v in [0, 1e8)) and config 5 (the
+    reference's fixture lease of 256 batches, transcript and output hash checked
+    against the GPU lease and the fixture)."""
     from oracle import oracle as O
     from psuedocubes_amd.optimized import BATCH_WIDTH
     fa, fb = (O.Focus(f) for f in config.focuses())
-    rng = random.Random(619003 + 1)
     threads = O.threads()
+    mismatches: list[str] = []
+
+    def gated(k: int, gpu_row: dict, nthreads: int) -> int:
+        row = O.batch(config, fa, fb, k * BATCH_WIDTH, (k + 1) * BATCH_WIDTH, nthreads=nthreads)
+        if row_digest(row) != row_digest(gpu_row):
+            mismatches.append(f'batch {k}')
+        return row['valid_focused_candidates']
+
     positions, batches = 0, 0
     started = time.perf_counter()
-    while time.perf_counter() - started < seconds or batches == 0:
-        k = rng.randrange(config.B // BATCH_WIDTH)
-        row = O.batch(config, fa, fb, k * BATCH_WIDTH, (k + 1) * BATCH_WIDTH, nthreads=threads)
-        positions += row['valid_focused_candidates']
+    for k, gpu_row in gpu_rows:
+        if batches and time.perf_counter() - started >= seconds:
+            break
+        positions += gated(k, gpu_row, threads)
         batches += 1
     elapsed = time.perf_counter() - started
-    # the same work on one thread: about a quarter of the sample's time
     one_positions, one_batches = 0, 0
     one_started = time.perf_counter()
-    while time.perf_counter() - one_started < seconds / 4 or one_batches == 0:
-        k = rng.randrange(config.B // BATCH_WIDTH)
-        row = O.batch(config, fa, fb, k * BATCH_WIDTH, (k + 1) * BATCH_WIDTH, nthreads=1)
-        one_positions += row['valid_focused_candidates']
+    for k, gpu_row in gpu_rows[batches:]:
+        if one_batches and time.perf_counter() - one_started >= seconds / 5:
+            break
+        one_positions += gated(k, gpu_row, 1)
         one_batches += 1
     one_elapsed = time.perf_counter() - one_started
+    configs = {}
+    if small is not None:
+        (first, end), gpu_row = small
+        t0 = time.perf_counter()
+        row = O.batch(config, fa, fb, first, end, nthreads=threads)
+        small_seconds = time.perf_counter() - t0
+        if row_digest(row) != row_digest(gpu_row):
+            mismatches.append('config 1 subtile')
+        configs['config1_subtile'] = dict(range=[str(first), str(end)], seconds=small_seconds,
+                                          value=row['valid_focused_candidates'] / small_seconds,
+                                          positions=row['valid_focused_candidates'])
+    if lease is not None:
+        t0 = time.perf_counter()
+        row = O.lease(config, fa, fb, lease['first'], lease['end'])
+        lease_seconds = time.perf_counter() - t0
+        fixture = json.loads((ROOT / 'tests' / 'golden' / 'lease_575222931200000000.json').read_text())
+        expected = fixture['result']['subtile_scopes'][0]
+        if (row['transcript_sha256'], row['output_sha256']) != (lease['transcript_sha256'], lease['output_sha256']) \
+                or row['transcript_sha256'] != expected['transcript_sha256'] \
+                or row['valid_focused_candidates'] != lease['valid_focused_candidates']:
+            mismatches.append('config 5 lease')
+        configs['config5_lease'] = dict(first=str(lease['first']), batches=row['compute_batches'],
+                                        seconds=lease_seconds, value=row['valid_focused_candidates'] / lease_seconds,
+                                        transcript_sha256=row['transcript_sha256'])
     try:
         model = next(line.split(':', 1)[1].strip() for line in
                      subprocess.run(['lscpu'], capture_output=True, text=True).stdout.splitlines()
                      if line.startswith('Model name'))
     except Exception:
         model = 'unknown'
+    gate = dict(passed=not mismatches, compared=dict(config2_batches=batches + one_batches,
+                                                     config1='config1_subtile' in configs,
+                                                     config5='config5_lease' in configs),
+                fields='per-subtile scope records (SHA-256), group populations, emitted values; '
+                       'the lease transcript and output hash', mismatches=mismatches)
+    if mismatches:
+        # no time is reported for a CPU run that does not reproduce the GPU's rows
+        return dict(value=None, unit='focused candidate positions/s', cores=threads, kind='port', gate=gate)
     return dict(value=positions / elapsed, unit='focused candidate positions/s', cores=threads, kind='port',
-                sample=f'{batches} seeded full p=619 batches (windows + bitset matcher + counters), '
-                       f'{elapsed:.1f} s on {threads} threads of {os.cpu_count()} ({model})',
-                single_thread=dict(value=one_positions / one_elapsed, cores=1,
-                                   sample=f'{one_batches} more seeded batches, {one_elapsed:.1f} s on one thread'))
+                sample=f'{batches} of this run\'s GPU batches (seeded full p=619 batches: windows + bitset matcher '
+                       f'+ counters), {elapsed:.1f} s on {threads} threads; the process may run on '
+                       f'{O.available_cpus()} CPUs, os.cpu_count() = {os.cpu_count()} ({model}); the oracle uses at '
+                       f'most {O.THREAD_LIMIT} threads, the CPU share one GPU box grants a process',
+                gate=gate,
+                single_thread=dict(value=one_positions / one_elapsed if one_batches else None, cores=1,
+                                   sample=f'{one_batches} more of the run\'s GPU batches, {one_elapsed:.1f} s on '
+                                          f'one thread'),
+                configs=configs)
 
 
 def strip_timings(row: dict) -> dict:
@@ -281,8 +343,11 @@ def main() -> None:
     rng = random.Random(619003)
     nbatch = B619 // BATCH_WIDTH
 
+    last_batch = [0]
+
     def step() -> dict:
         k = rng.randrange(nbatch)
+        last_batch[0] = k
         batch = prepare(runtime, config, af, bf, k * BATCH_WIDTH, (k + 1) * BATCH_WIDTH)
         return matcher.run(batch)
 
@@ -290,6 +355,9 @@ def main() -> None:
         step()
     print(f'# warmup {args.warmup} done', file=sys.stderr, flush=True)
     positions, kernel, prefilter, intersection = 0, 0.0, 0.0, 0.0
+    # the first timed steps' rows (references only) for the CPU baseline's gate
+    kept: list = []
+    keep = 0 if args.no_cpu else 200
     started = time.perf_counter()
     for _ in range(args.steps):
         row = step()
@@ -297,6 +365,8 @@ def main() -> None:
         kernel += row['kernel_seconds']
         prefilter += row['prefilter_build_seconds']
         intersection += matcher.intersection_seconds
+        if len(kept) < keep:
+            kept.append((last_batch[0], row))
     elapsed = time.perf_counter() - started
     print(f'# timed {args.steps} steps in {elapsed:.3f} s', file=sys.stderr, flush=True)
 
@@ -347,6 +417,10 @@ def main() -> None:
         result['lease'] = dict(value=lease['valid_focused_candidates'] / t1, unit='focused candidate positions/s',
                                seconds=t1, batches=256, first=str(first),
                                transcript_sha256=lease['subtile_scopes'][0]['transcript_sha256'])
+        lease_check = dict(first=first, end=first + 256 * BATCH_WIDTH,
+                           transcript_sha256=lease['subtile_scopes'][0]['transcript_sha256'],
+                           output_sha256=lease['subtile_scopes'][0]['output_sha256'],
+                           valid_focused_candidates=lease['valid_focused_candidates'])
     if not args.no_batched:
         result['batched'] = batched_leg(runtime, af, bf, args.batched_windows, args.plain_windows)
         print('# batched leg done', file=sys.stderr, flush=True)
@@ -354,7 +428,11 @@ def main() -> None:
         result['served'] = served_leg(runtime, af, bf, args.served_requests, args.served_load)
         print('# served leg done', file=sys.stderr, flush=True)
     if not args.no_cpu:
-        result['cpu_baseline'] = cpu_baseline(config, args.cpu_seconds)
+        # config 1: the first 1e8-v subtile through the drop-in path, for the CPU's gate
+        small_range = (0, 100_000_000)
+        small_row = matcher.run(prepare(runtime, config, af, bf, *small_range))
+        result['cpu_baseline'] = cpu_baseline(config, args.cpu_seconds, kept, (small_range, small_row),
+                                              None if args.no_lease else lease_check)
     print(json.dumps(result), flush=True)
 
 

@@ -73,8 +73,21 @@ def lib() -> ctypes.CDLL:
     return _lib
 
 
+# Worker threads per process: the CPUs this process may run on, at most 16 (one GPU
+# box grants a process 16 CPUs whatever os.cpu_count() reports there).
+THREAD_LIMIT = 16
+
+
+def available_cpus() -> int:
+    """CPUs this process may run on (its affinity set), else os.cpu_count()."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        return os.cpu_count() or 1
+
+
 def threads() -> int:
-    return max(1, min(16, os.cpu_count() or 1))
+    return max(1, min(THREAD_LIMIT, available_cpus()))
 
 
 # ---------------------------------------------------------------------------
