@@ -410,7 +410,9 @@ def main() -> None:
                                 roots_per_second=(1442249570 - 1000000001 + 1) / control['kernel_seconds'])
     if not args.no_lease:
         first = 575222931200000000
-        compute_lease(matcher, af, bf, first, first + 4 * BATCH_WIDTH)  # warm the lease buffers
+        # warm the lease's buffers at full size on the next lease's range, as a worker
+        # running lease after lease has them
+        compute_lease(matcher, af, bf, first + 256 * BATCH_WIDTH, first + 512 * BATCH_WIDTH)
         t0 = time.perf_counter()
         lease = compute_lease(matcher, af, bf, first, first + 256 * BATCH_WIDTH)
         t1 = time.perf_counter() - t0

@@ -317,6 +317,15 @@ int pcube_scope_records(uint32_t ntiles, const uint64_t *bounds, const uint64_t 
                         uint32_t ng, const uint64_t *local, const char *sha, char *out, uint64_t cap,
                         uint64_t *length);
 
+/* pcube_scope_records for the consecutive batches of one device pass: batch b
+ * holds per_batch[b] subtiles, and its encoding is out[offsets[b] ..
+ * offsets[b+1]) (offsets has nbatches + 1 entries, offsets[0] = 0).  bounds,
+ * counters, local and sha as above, over every subtile of the pass in order.
+ * OVERFLOW when the encodings do not fit in cap bytes.  Host only. */
+int pcube_scope_records_many(uint32_t nbatches, const uint32_t *per_batch, const uint64_t *bounds,
+                             const uint64_t *counters, uint32_t stride, uint32_t ng, const uint64_t *local,
+                             const char *sha, char *out, uint64_t cap, uint64_t *offsets);
+
 /* --- self test: the division behind u_interval (pseudocube/focus.py:120-121) --- */
 /* The row builders find a row's u bounds as first indices with u >= floor((v*A + lo)/B) + 1
  * and u >= floor((v*A + hi)/B) + 1, dividing 128-bit values by B exactly (Barrett, with

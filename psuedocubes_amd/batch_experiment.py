@@ -11,6 +11,8 @@ from __future__ import annotations
 
 from dataclasses import dataclass
 import ctypes
+import hashlib
+import json
 import threading
 import time
 from typing import Any, NamedTuple
@@ -22,6 +24,7 @@ from .accounting import digest
 from .focus import Config, Tile
 from .gpu import DeviceArray, GPUFocus, MatchEngine, Runtime
 from .predicate import is_perfect_cube, local3, verify
+from .verification import CLAIM
 
 # output_sha256 of a subtile that emitted nothing (almost every subtile at p=619)
 EMPTY_DIGEST = digest([])
@@ -73,6 +76,39 @@ def tile_ranges(config: Config, first: int, end: int, width: int) -> list[tuple[
         if r_hi >= B:
             q_hi += 1
             r_hi -= B
+    return out
+
+
+def tile_geometry(config: Config, spans: list[tuple[int, int]], width: int) -> tuple[np.ndarray, list[int]]:
+    """The subtiles of many batches in one library call (pcube_tile_ranges_many, exact
+    128-bit arithmetic): an [n, 4] uint64 array of (v_first, v_end, u_min, u_end)
+    rows, the batches' subtiles consecutive, and each batch's subtile count."""
+    n = len(spans)
+    firsts = np.array([s for s, _ in spans], dtype=np.uint64)
+    ends = np.array([e for _, e in spans], dtype=np.uint64)
+    widths = np.full(n, width, dtype=np.uint64)
+    tiles = np.zeros(4 * 64 * max(1, n), dtype=np.uint64)
+    counts = np.zeros(max(1, n), dtype=np.uint32)
+    mask = (1 << 64) - 1
+    native.call('pcube_tile_ranges_many', config.A, config.B, config.lo & mask, config.lo >> 64, config.hi & mask,
+                config.hi >> 64, n, native.ptr64(firsts), native.ptr64(ends), native.ptr64(widths),
+                native.ptr64(tiles), 64 * max(1, n), native.ptr32(counts))
+    per_batch = counts[:n].tolist()
+    return tiles[:4 * sum(per_batch)].reshape(-1, 4), per_batch
+
+
+def tile_ranges_many(config: Config, spans: list[tuple[int, int]], width: int) -> list[list[tuple[int, int, int, int]]]:
+    """tile_ranges of many batches at once: equal to [tile_ranges(config, first, end,
+    width) for first, end in spans]."""
+    if not spans:
+        return []
+    tiles, per_batch = tile_geometry(config, spans, width)
+    flat = tiles.tolist()
+    out = []
+    at = 0
+    for count in per_batch:
+        out.append([tuple(r) for r in flat[at:at + count]])
+        at += count
     return out
 
 
@@ -211,6 +247,83 @@ class Verified(NamedTuple):
     values: list[list[str]]
     digests: list[str | None]
     hits: list[dict[str, object]]
+
+
+def check_values(config: Config, firsts: np.ndarray, ends: np.ndarray, primes: np.ndarray, kept: np.ndarray,
+                 values: np.ndarray) -> tuple[list[int], list[list[str]], list[str], list[list[dict[str, object]]]]:
+    """Every value a device pass emitted, checked in one native call
+    (pcube_batch_values): per subtile sorted, no duplicates, inside (lo, hi] of
+    config, canonical v inside [firsts, ends) of its subtile and local3 at that
+    subtile's prime; raises ArithmeticError otherwise.
+
+    firsts, ends, primes, kept: per subtile of the pass (kept = values listed);
+    values: the listed values as (lo, hi) uint64 limbs, subtile after subtile.
+    Returns, for the subtiles with values in ascending order: their indices,
+    sorted decimal strings, the digest of each subtile's strings and the
+    noncubes' witness records (the form python_witnesses gives)."""
+    counts = np.ascontiguousarray(kept, dtype=np.uint64)
+    nvalues = int(counts.sum())
+    if nvalues == 0:
+        return [], [], [], []
+    limbs = np.ascontiguousarray(values[:2 * nvalues], dtype=np.uint64)
+    firsts = np.ascontiguousarray(firsts, dtype=np.uint64)
+    ends = np.ascontiguousarray(ends, dtype=np.uint64)
+    primes = np.ascontiguousarray(primes, dtype=np.uint32)
+    cube = np.zeros(nvalues, dtype=np.uint8)
+    roots = np.zeros(nvalues, dtype=np.uint64)
+    text = ctypes.create_string_buffer(nvalues * 44 + len(counts) * 3 + 16)
+    offsets = np.zeros(len(counts) + 1, dtype=np.uint64)
+    failure = ctypes.c_uint32()
+    mask = (1 << 64) - 1
+    native.call('pcube_batch_values', nvalues, native.ptr64(limbs), len(counts), native.ptr64(counts),
+                native.ptr64(firsts), native.ptr64(ends), native.ptr32(primes), config.A, config.B, config.lo & mask,
+                config.lo >> 64, config.hi & mask, config.hi >> 64, cube.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                native.ptr64(roots), text, len(text), native.ptr64(offsets), ctypes.byref(failure))
+    raw_text = text.raw
+    index = np.flatnonzero(counts).tolist()
+    number = counts[index].tolist()
+    starts = offsets[index].tolist()
+    stops = offsets[np.array(index) + 1].tolist()
+    prime_of = primes[index].tolist()
+    cubes = cube.tolist()
+    root_list = roots.tolist()
+    strings_of, digests, hits_of = [], [], []
+    v = 0
+    for n, a, b, p in zip(number, starts, stops, prime_of):
+        chunk = raw_text[a:b]
+        strings = json.loads(chunk)
+        strings_of.append(strings)
+        digests.append(hashlib.sha256(chunk).hexdigest())
+        hits_of.append([{'n': value, 'p': p, 'cube_root_floor': str(root), 'perfect_cube': False,
+                         'failing_primes': [], 'valid_pseudocube': True, 'claim': CLAIM}
+                        for value, is_cube, root in zip(strings, cubes[v:v + n], root_list[v:v + n]) if not is_cube])
+        v += n
+    return index, strings_of, digests, hits_of
+
+
+def verify_pass(config: Config, firsts: np.ndarray, ends: np.ndarray, primes: np.ndarray, kept: np.ndarray,
+                groups: list[int], values: np.ndarray) -> list[Verified]:
+    """check_values over a pass whose batches hold groups[b] consecutive subtiles:
+    per batch its Verified (decimal strings and digest per subtile, the noncubes'
+    witness records in value order)."""
+    return verify_pass_from(check_values(config, firsts, ends, primes, kept, values), groups)
+
+
+def verify_pass_from(checked: tuple[list[int], list[list[str]], list[str], list[list[dict[str, object]]]],
+                     groups: list[int]) -> list[Verified]:
+    """verify_pass's per-batch Verified from check_values's output."""
+    out = [Verified([[] for _ in range(n)], [None] * n, []) for n in groups]
+    index, strings_of, digests, hits_of = checked
+    if not index:
+        return out
+    heads = np.cumsum([0] + list(groups))
+    batch_of = (np.searchsorted(heads, index, side='right') - 1).tolist()
+    for s, b, strings, text_digest, hits in zip(index, batch_of, strings_of, digests, hits_of):
+        k = s - int(heads[b])
+        out[b].values[k] = strings
+        out[b].digests[k] = text_digest
+        out[b].hits.extend(hits)
+    return out
 
 
 def batch_row(config: Config, filter_groups: list[list[int]], ranges: list[tuple[int, int, int, int]],

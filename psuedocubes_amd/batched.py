@@ -37,7 +37,7 @@ from typing import Any, Iterable
 import numpy as np
 
 from . import native
-from .batch_experiment import Verified, batch_row, tile_ranges
+from .batch_experiment import Verified, batch_row, tile_ranges, verify_pass
 from .focus import Config
 from .gpu import GPUFocus, MatchEngine, Runtime
 from .verification import CLAIM
@@ -247,52 +247,10 @@ class BatchedEngine:
         subtile, local3 at its window's p), then per window its subtiles' decimal
         strings, digests and the noncubes' witness records."""
         kept_all = np.concatenate(kept) if kept else np.zeros(0, dtype=np.uint64)
-        nvalues = int(kept_all.sum())
-        if nvalues == 0:
-            return [Verified([[] for _ in k], [None] * len(k), []) for k in kept]
         firsts = np.array([r[0] for ranges, _ in plans for r in ranges], dtype=np.uint64)
         ends = np.array([r[1] for ranges, _ in plans for r in ranges], dtype=np.uint64)
         primes = np.repeat(np.array([w.p for w in windows], dtype=np.uint32), [len(k) for k in kept])
-        counts = np.ascontiguousarray(kept_all, dtype=np.uint64)
-        limbs = np.ascontiguousarray(values[:2 * nvalues])
-        cube = np.zeros(nvalues, dtype=np.uint8)
-        roots = np.zeros(nvalues, dtype=np.uint64)
-        text = ctypes.create_string_buffer(nvalues * 44 + len(counts) * 3 + 16)
-        offsets = np.zeros(len(counts) + 1, dtype=np.uint64)
-        failure = ctypes.c_uint32()
-        base = self.base
-        mask = (1 << 64) - 1
-        native.call('pcube_batch_values', nvalues, native.ptr64(limbs), len(counts), native.ptr64(counts),
-                    native.ptr64(firsts), native.ptr64(ends), native.ptr32(primes), base.A, base.B, base.lo & mask,
-                    base.lo >> 64, base.hi & mask, base.hi >> 64, cube.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
-                    native.ptr64(roots), text, len(text), native.ptr64(offsets), ctypes.byref(failure))
-        raw_text = text.raw
-        offs = offsets.tolist()
-        cubes = cube.tolist()
-        root_list = roots.tolist()
-        out: list[Verified] = []
-        s = v = 0
-        for window, k in zip(windows, kept):
-            tile_values: list[list[str]] = []
-            digests: list[str | None] = []
-            hits: list[dict[str, object]] = []
-            for n in k.tolist():
-                if n:
-                    chunk = raw_text[offs[s]:offs[s + 1]]
-                    strings = json.loads(chunk)
-                    tile_values.append(strings)
-                    digests.append(hashlib.sha256(chunk).hexdigest())
-                    p = window.p
-                    hits += [{'n': value, 'p': p, 'cube_root_floor': str(root), 'perfect_cube': False,
-                              'failing_primes': [], 'valid_pseudocube': True, 'claim': CLAIM}
-                             for value, is_cube, root in zip(strings, cubes[v:v + n], root_list[v:v + n]) if not is_cube]
-                    v += n
-                else:
-                    tile_values.append([])
-                    digests.append(None)
-                s += 1
-            out.append(Verified(tile_values, digests, hits))
-        return out
+        return verify_pass(self.base, firsts, ends, primes, kept_all, [len(k) for k in kept], values)
 
     def __del__(self) -> None:
         if native._lib is not None and hasattr(self, 'handle'):

@@ -281,6 +281,7 @@ struct Pinned {
     T* as() const { return (T*)p; }
 };
 Pinned g_up_host, g_down_host;
+Pinned g_lease_up, g_lease_down;   // a lease pass's subtile queries; its counters and status words
 // Coherent pinned memory that kernels write results into directly: the
 // stores reach host memory before the kernel's completion is signalled, so a
 // stream synchronisation makes them visible without a copy.
@@ -1007,6 +1008,41 @@ __global__ void k_searchsorted_seg(const uint64_t* __restrict__ data, const uint
 // remaining range [l, h) and a ballot cuts the range 64-fold, so a million
 // entries take four dependent loads.  Invariant: the first index with
 // a[i] >= k lies in [l, h] (h = the size when every entry is smaller).
+// The same search for a lease pass: nq_u queries into the u window, then nq_v
+// into the v window, read from device memory; results as k_prepare_bounds
+// (positions, then both sizes and overflow flags at nq_u + nq_v).
+__global__ __launch_bounds__(256) void k_lease_bounds(const uint64_t* __restrict__ us,
+                                                      const uint64_t* __restrict__ uinfo, uint64_t us_cap,
+                                                      const uint64_t* __restrict__ vs,
+                                                      const uint64_t* __restrict__ vinfo, uint64_t vs_cap,
+                                                      const uint64_t* __restrict__ queries, uint64_t nq_u,
+                                                      uint64_t nq_v, uint64_t* __restrict__ out) {
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint64_t q = blockIdx.x * (uint64_t)(blockDim.x / kWave) + threadIdx.x / kWave;
+    const uint64_t nq = nq_u + nq_v;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        out[nq] = uinfo[1];
+        out[nq + 1] = vinfo[1];
+        out[nq + 2] = uinfo[2];
+        out[nq + 3] = vinfo[2];
+    }
+    if (q >= nq) return;
+    const bool on_u = q < nq_u;
+    const uint64_t* a = on_u ? us : vs;
+    uint64_t l = 0, h = on_u ? min(uinfo[1], us_cap) : min(vinfo[1], vs_cap);
+    const uint64_t k = queries[q];
+    while (l < h) {
+        const uint64_t step = (h - l + kWave - 1) / kWave;
+        const uint64_t at = l + lane * step;
+        const uint32_t below = (uint32_t)__popcll(__ballot(at < h && a[at] < k));
+        if (below == 0) break;   // a[l] >= k
+        const uint64_t top = l + below * step;
+        l += (below - 1) * step + 1;
+        h = min(h, top);
+    }
+    if (lane == 0) out[q] = l;
+}
+
 // The queries of a prepared batch, passed with the kernel arguments: (u_min, u_end)
 // per subtile, then (v_first, v_end) per subtile, at most 64 subtiles.
 constexpr uint32_t kMaxPrepareTiles = 64;
@@ -1875,36 +1911,89 @@ int pcube_lease(int matcher, int focus_u, int focus_v, uint32_t nbatches, const 
     for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
     HIP_TRY(hipEventRecord(ev[0], g.stream));
     uint64_t totals[2] = {0, 0};
-    uint64_t offs[2] = {0, 0};
-    if (u_high > u_low) {
-        TRY(lease_window(*fu, u_low, u_high, lus.as<uint64_t>(), window_cap, offs));
-        totals[0] = offs[1];
-    }
-    TRY(lease_window(*fv, v_low, v_high, lvs.as<uint64_t>(), window_cap, offs));
-    totals[1] = offs[1];
-    HIP_TRY(hipStreamSynchronize(g.stream));
-    // subtile bounds inside the lease arrays
     // (u_min, u_end) queries into the u window, (v_first, v_end) into the v window
-    std::vector<uint64_t> uq(ntot * 2), vq(ntot * 2);
+    const uint64_t nq = 4 * ntot;
+    TRY(g_lease_up.ensure(nq * 8));
+    uint64_t* q = g_lease_up.as<uint64_t>();
     for (uint64_t t = 0; t < ntot; ++t) {
-        uq[2 * t] = tiles[t * 4 + 2];
-        uq[2 * t + 1] = tiles[t * 4 + 3];
-        vq[2 * t] = tiles[t * 4 + 0];
-        vq[2 * t + 1] = tiles[t * 4 + 1];
+        q[2 * t] = tiles[t * 4 + 2];
+        q[2 * t + 1] = tiles[t * 4 + 3];
+        q[2 * ntot + 2 * t] = tiles[t * 4 + 0];
+        q[2 * ntot + 2 * t + 1] = tiles[t * 4 + 1];
     }
-    TRY(queries.ensure(ntot * 32));
-    TRY(positions.ensure(ntot * 32));
-    HIP_TRY(hipMemcpyAsync(queries.p, uq.data(), ntot * 16, hipMemcpyHostToDevice, g.stream));
-    HIP_TRY(hipMemcpyAsync(queries.as<uint64_t>() + 2 * ntot, vq.data(), ntot * 16, hipMemcpyHostToDevice, g.stream));
-    hipLaunchKernelGGL(k_searchsorted, dim3(blocks_for(2 * ntot, 256)), dim3(256), 0, g.stream, lus.as<uint64_t>(),
-                       (uint64_t)totals[0], queries.as<uint64_t>(), 2 * ntot, positions.as<uint64_t>());
-    hipLaunchKernelGGL(k_searchsorted, dim3(blocks_for(2 * ntot, 256)), dim3(256), 0, g.stream, lvs.as<uint64_t>(),
-                       (uint64_t)totals[1], queries.as<uint64_t>() + 2 * ntot, 2 * ntot,
-                       positions.as<uint64_t>() + 2 * ntot);
-    std::vector<uint64_t> up(ntot * 2), vp(ntot * 2);
-    HIP_TRY(hipMemcpyAsync(up.data(), positions.p, ntot * 16, hipMemcpyDeviceToHost, g.stream));
-    HIP_TRY(hipMemcpyAsync(vp.data(), positions.as<uint64_t>() + 2 * ntot, ntot * 16, hipMemcpyDeviceToHost, g.stream));
-    HIP_TRY(hipStreamSynchronize(g.stream));
+    TRY(queries.ensure(nq * 8));
+    const uint64_t* up;   // subtile bounds: (u_min, u_end) positions per subtile, then (v_first, v_end)
+    std::vector<uint64_t> fallback;
+    if (fu->wheel.ready && fv->wheel.ready && fu != fv) {
+        // both windows in one paired wheel launch, the bound searches behind them
+        // reading the window sizes on the device, and one synchronisation: the
+        // results arrive in coherent pinned memory
+        HIP_TRY(hipMemcpyAsync(queries.p, q, nq * 8, hipMemcpyHostToDevice, g.stream));
+        std::vector<WheelWindow> pu, pv;
+        std::vector<uint64_t> su, sv;
+        const uint64_t u_end = std::max(u_low, u_high);
+        const uint64_t nseg_u = wheel_plan(fu->wheel, 1, &u_low, &u_end, pu, su);
+        const uint64_t nseg_v = wheel_plan(fv->wheel, 1, &v_low, &v_high, pv, sv);
+        TRY(wheel_check(nseg_u));
+        TRY(wheel_check(nseg_v));
+        WheelInputs in_u{}, in_v{};
+        in_u.flag = fu->wheel.flag.as<uint32_t>();
+        in_u.n = 1;
+        in_u.nseg = nseg_u;
+        in_u.win0 = pu[0];
+        in_v.flag = fv->wheel.flag.as<uint32_t>();
+        in_v.n = 1;
+        in_v.nseg = nseg_v;
+        in_v.win0 = pv[0];
+        uint32_t cap_u = wheel_slot_cap(fu->wheel), cap_v = wheel_slot_cap(fv->wheel);
+        TRY(g_result_host.ensure((nq + 4) * 8));
+        const uint64_t* r = g_result_host.as<uint64_t>();
+        for (;;) {
+            TRY(wheel_enqueue_pair(fu->wheel, in_u, lus.as<uint64_t>(), window_cap, cap_u, fv->wheel, in_v,
+                                   lvs.as<uint64_t>(), window_cap, cap_v));
+            hipLaunchKernelGGL(k_lease_bounds, dim3(blocks_for(nq, 256 / kWave)), dim3(256), 0, g.stream,
+                               (const uint64_t*)lus.as<uint64_t>(), (const uint64_t*)fu->wheel.win_info.as<uint64_t>(),
+                               window_cap, (const uint64_t*)lvs.as<uint64_t>(),
+                               (const uint64_t*)fv->wheel.win_info.as<uint64_t>(), window_cap,
+                               (const uint64_t*)queries.as<uint64_t>(), 2 * ntot, 2 * ntot,
+                               g_result_host.as<uint64_t>());
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipStreamSynchronize(g.stream));
+            const bool again_u = r[nq + 2] && cap_u < (uint32_t)kWheelWaveSpan;
+            const bool again_v = r[nq + 3] && cap_v < (uint32_t)kWheelWaveSpan;
+            if (!again_u && !again_v) break;
+            // a slot overflowed: once more with room for every candidate
+            if (again_u) { cap_u = kWheelWaveSpan; TRY(fill32(in_u.flag, 1, 0)); }
+            if (again_v) { cap_v = kWheelWaveSpan; TRY(fill32(in_v.flag, 1, 0)); }
+        }
+        totals[0] = r[nq];
+        totals[1] = r[nq + 1];
+        if (totals[0] > window_cap || totals[1] > window_cap)
+            return fail(PCUBE_OVERFLOW, "lease focus window overflow (%llu, %llu > %llu)",
+                        (unsigned long long)totals[0], (unsigned long long)totals[1], (unsigned long long)window_cap);
+        up = r;
+    } else {
+        uint64_t offs[2] = {0, 0};
+        if (u_high > u_low) {
+            TRY(lease_window(*fu, u_low, u_high, lus.as<uint64_t>(), window_cap, offs));
+            totals[0] = offs[1];
+        }
+        TRY(lease_window(*fv, v_low, v_high, lvs.as<uint64_t>(), window_cap, offs));
+        totals[1] = offs[1];
+        TRY(positions.ensure(nq * 8));
+        HIP_TRY(hipMemcpyAsync(queries.p, q, nq * 8, hipMemcpyHostToDevice, g.stream));
+        hipLaunchKernelGGL(k_searchsorted, dim3(blocks_for(2 * ntot, 256)), dim3(256), 0, g.stream,
+                           lus.as<uint64_t>(), (uint64_t)totals[0], queries.as<uint64_t>(), 2 * ntot,
+                           positions.as<uint64_t>());
+        hipLaunchKernelGGL(k_searchsorted, dim3(blocks_for(2 * ntot, 256)), dim3(256), 0, g.stream,
+                           lvs.as<uint64_t>(), (uint64_t)totals[1], queries.as<uint64_t>() + 2 * ntot, 2 * ntot,
+                           positions.as<uint64_t>() + 2 * ntot);
+        fallback.resize(nq);
+        HIP_TRY(hipMemcpyAsync(fallback.data(), positions.p, nq * 8, hipMemcpyDeviceToHost, g.stream));
+        HIP_TRY(hipStreamSynchronize(g.stream));
+        up = fallback.data();
+    }
+    const uint64_t* vp = up + 2 * ntot;
     HIP_TRY(hipEventRecord(ev[1], g.stream));
     // per batch geometry
     const uint64_t stride = mt->ng + 3;
@@ -1915,8 +2004,9 @@ int pcube_lease(int matcher, int focus_u, int focus_v, uint32_t nbatches, const 
     // every batch of the lease, then one download; a bitset-survivor list that
     // overflowed in any batch is sized from the largest count and the lease
     // runs again
-    std::vector<uint64_t> host(ntot * stride);
-    uint64_t st[4] = {0, 0, 0, 0};
+    TRY(g_lease_down.ensure((ntot * stride + 4) * 8));
+    uint64_t* host = g_lease_down.as<uint64_t>();
+    uint64_t* st = host + ntot * stride;   // status words after the counters
     int geometry_rc = PCUBE_OK;
     int engine = 0;
     for (int attempt = 0; attempt < 2; ++attempt) {
@@ -1947,7 +2037,7 @@ int pcube_lease(int matcher, int focus_u, int focus_v, uint32_t nbatches, const 
             t0 = t1;
         }
         HIP_TRY(hipEventRecord(ev[2], g.stream));
-        HIP_TRY(hipMemcpyAsync(host.data(), mt->counters.p, ntot * stride * 8, hipMemcpyDeviceToHost, g.stream));
+        HIP_TRY(hipMemcpyAsync(host, mt->counters.p, ntot * stride * 8, hipMemcpyDeviceToHost, g.stream));
         HIP_TRY(hipMemcpyAsync(st, mt->status.p, 4 * 8, hipMemcpyDeviceToHost, g.stream));
         HIP_TRY(hipEventRecord(ev[3], g.stream));
         HIP_TRY(hipStreamSynchronize(g.stream));
@@ -1957,7 +2047,7 @@ int pcube_lease(int matcher, int focus_u, int focus_v, uint32_t nbatches, const 
     }
     const uint32_t err = (uint32_t)st[1];
     const uint64_t listed = st[0];
-    memcpy(counters, host.data(), ntot * stride * 8);
+    memcpy(counters, host, ntot * stride * 8);
     if (seconds) {
         float a = 0, m = 0;
         HIP_TRY(hipEventElapsedTime(&a, ev[0], ev[1]));
@@ -1969,10 +2059,10 @@ int pcube_lease(int matcher, int focus_u, int focus_v, uint32_t nbatches, const 
     for (auto& e : ev) (void)hipEventDestroy(e);
     lus.release(); lvs.release(); queries.release(); positions.release();
     if (geometry_rc) return geometry_rc;
-    int rc = batch_status(*mt, host.data(), ntot, cap, err);
+    int rc = batch_status(*mt, counters, ntot, cap, err);
     if (rc) return rc;
     int status = PCUBE_OK;
-    TRY(collect_values(*mt, ntot, host.data(), cap, list_cap, values, values_cap, &status, listed));
+    TRY(collect_values(*mt, ntot, counters, cap, list_cap, values, values_cap, &status, listed));
     if (seconds) seconds[3] = now_seconds() - t_start;
     return status;
 }
@@ -2711,6 +2801,26 @@ int pcube_scope_records(uint32_t ntiles, const uint64_t* bounds, const uint64_t*
     t.put("]\n", 2);
     if (!t.ok) return fail(PCUBE_OVERFLOW, "scope records exceed %llu bytes", (unsigned long long)cap);
     *length = t.n;
+    return PCUBE_OK;
+}
+
+int pcube_scope_records_many(uint32_t nbatches, const uint32_t* per_batch, const uint64_t* bounds,
+                             const uint64_t* counters, uint32_t stride, uint32_t ng, const uint64_t* local,
+                             const char* sha, char* out, uint64_t cap, uint64_t* offsets) {
+    if ((!per_batch && nbatches) || !offsets || !out) return fail(PCUBE_INVALID, "null batch layout");
+    offsets[0] = 0;
+    uint64_t k = 0;
+    for (uint32_t b = 0; b < nbatches; ++b) {
+        uint64_t length = 0;
+        const uint32_t nt = per_batch[b];
+        const int rc = pcube_scope_records(nt, bounds ? bounds + 2 * k : nullptr,
+                                           counters ? counters + k * stride : nullptr, stride, ng,
+                                           local ? local + k : nullptr, sha ? sha + 64 * k : nullptr,
+                                           out + offsets[b], cap - offsets[b], &length);
+        if (rc) return rc;
+        offsets[b + 1] = offsets[b] + length;
+        k += nt;
+    }
     return PCUBE_OK;
 }
 

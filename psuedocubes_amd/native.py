@@ -65,6 +65,8 @@ SIGNATURES: dict[str, list[Any]] = {
                            ctypes.POINTER(ctypes.c_uint8), P_u64, ctypes.c_void_p, u64, P_u64, P_u32],
     'pcube_floor_div': [P_u64, P_u64, u64],
     'pcube_scope_records': [u32, P_u64, P_u64, u32, u32, P_u64, ctypes.c_char_p, ctypes.c_void_p, u64, P_u64],
+    'pcube_scope_records_many': [u32, P_u32, P_u64, P_u64, u32, u32, P_u64, ctypes.c_char_p, ctypes.c_void_p, u64,
+                                 P_u64],
     'pcube_arithmetic': [P_u64, P_u64, u64],
     'pcube_copy_probe': [u64, P_dbl],
     'pcube_lds_probe': [u32, u32, P_dbl, P_dbl],

@@ -26,7 +26,8 @@ import numpy as np
 
 from . import native
 from .accounting import digest, encoded
-from .batch_experiment import EMPTY_DIGEST, MaskedMatcher, batch_row, prepare, tile_ranges
+from .batch_experiment import (EMPTY_DIGEST, OUTPUT_CAP, MaskedMatcher, batch_row, check_values, prepare,
+                               tile_geometry, verify_pass_from)
 from .gpu import GPUFocus
 
 PROFILE: dict[str, Any] = dict(engine='residue-bitset-batch-v1', tile_width=100_000_000,
@@ -77,11 +78,12 @@ def validate_result(first: int, end: int, result: dict[str, Any]) -> None:
         raise ValueError('unknown receipt format')
 
 
-class QuietBatch(NamedTuple):
-    """A batch whose subtiles emitted nothing, by the totals its row would hold:
-    focus_u, focus_v and cartesian_opportunities from its bounds and window,
-    valid_focused_candidates and group_survivors summed over its counters;
-    every survivor count 0 and no values or hits."""
+class BatchTotals(NamedTuple):
+    """A batch by the totals its row would hold: focus_u, focus_v and
+    cartesian_opportunities from its bounds and window, valid_focused_candidates
+    and group_survivors summed over its counters, and its verified values
+    (decimal strings, subtile after subtile, each subtile's sorted) with the
+    noncubes' witness records; a batch that emitted nothing has neither."""
     subtiles: int
     span: tuple[int, int]        # (first subtile's first, last subtile's end)
     focus_u: int
@@ -91,6 +93,8 @@ class QuietBatch(NamedTuple):
     groups: list[int]
     filter_groups: list[list[int]]
     timings: dict[str, float]
+    values: list[str] = []
+    hits: list[dict[str, Any]] = []
 
 
 class Accumulator:
@@ -142,9 +146,9 @@ class Accumulator:
         self.subtiles += len(actual)
         self.cursor = end
 
-    def add_quiet(self, first: int, end: int, batch: QuietBatch, encoded_scopes: bytes) -> None:
-        """Fold a batch that emitted nothing from its totals: the checks and sums
-        of add() on the row batch_row builds for it, without building the row."""
+    def add_totals(self, first: int, end: int, batch: BatchTotals, encoded_scopes: bytes) -> None:
+        """Fold a batch from its totals: the checks and sums of add() on the row
+        batch_row builds for it, without building the row."""
         if first != self.cursor or end != min(first + BATCH_WIDTH, self.end):
             raise ValueError('gap, overlap or changed compute batch geometry')
         width = PROFILE['tile_width']
@@ -160,6 +164,13 @@ class Accumulator:
         counts['focus_v'] += batch.focus_v
         counts['cartesian_opportunities'] += batch.opportunities
         counts['valid_focused_candidates'] += batch.valid
+        if batch.values:
+            n = len(batch.values)
+            counts['local_survivors'] += n
+            counts['cube_checks'] += n
+            counts['genuine_cubes'] += n - len(batch.hits)
+            self.values.extend(batch.values)
+            self.hits.extend(batch.hits)
         for key in TIMINGS:
             self.timings[key] += batch.timings.get(key, 0.0)
         self.groups = [a + b for a, b in zip(self.groups, batch.groups)]
@@ -208,7 +219,53 @@ def scope_record_bytes(bounds: np.ndarray, counters: np.ndarray, ng: int, local:
     return buf.raw[:_records.length.value]
 
 
-LEASE_CHUNK = 32   # batches per device pass of a pipelined lease
+def pass_record_bytes(per_batch: list[int], bounds: np.ndarray, counters: np.ndarray, ng: int, local: np.ndarray,
+                      shas: bytes) -> list[bytes]:
+    """scope_record_bytes of every batch of a pass in one library call
+    (`pcube_scope_records_many`): batch b holds per_batch[b] consecutive subtiles
+    of bounds, counters, local and shas (each over the whole pass)."""
+    nt, stride = counters.shape
+    need = 256 * len(per_batch) + nt * (400 + 24 * ng)
+    buf = getattr(_records, 'many', None)
+    if buf is None or len(buf) < need:
+        buf = _records.many = ctypes.create_string_buffer(need)
+    sizes = np.array(per_batch, dtype=np.uint32)
+    offsets = np.zeros(len(per_batch) + 1, dtype=np.uint64)
+    native.check(native.lib().pcube_scope_records_many(len(per_batch), native.ptr32(sizes), native.ptr64(bounds),
+                                                       native.ptr64(counters), stride, ng,
+                                                       native.ptr64(np.ascontiguousarray(local, dtype=np.uint64)),
+                                                       shas, buf, len(buf), native.ptr64(offsets)),
+                 'pcube_scope_records_many')
+    ends = offsets.tolist()
+    raw = ctypes.string_at(buf, ends[-1])
+    return [raw[a:b] for a, b in zip(ends[:-1], ends[1:])]
+
+
+LEASE_CHUNK = 64   # most batches per device pass of a pipelined lease
+LEASE_EDGE = 8     # batches of its first and last passes
+
+
+def pass_sizes(n: int) -> list[int]:
+    """Batches per device pass for a lease of n batches: a short first pass, so the
+    host starts folding rows early; passes of LEASE_CHUNK (few pass boundaries, at
+    each of which the device waits for the next call); and a tail that halves
+    down to the edge size, so that folding one pass takes less time than the
+    device takes for the next and little folding is left once the device is done."""
+    edge = max(1, min(LEASE_EDGE, LEASE_CHUNK))
+    if n <= 2 * edge:
+        return [n] if n <= edge else [edge, n - edge]
+    rest = n - edge
+    tail = []
+    size = edge
+    while rest > 0 and size < LEASE_CHUNK:
+        tail.append(min(size, rest))
+        rest -= tail[-1]
+        size *= 2
+    middle = []
+    while rest > 0:
+        middle.append(min(LEASE_CHUNK, rest))
+        rest -= middle[-1]
+    return [edge] + middle + tail[::-1]
 
 
 def pass_totals(per_batch: list[int], bounds: np.ndarray, counters: np.ndarray,
@@ -231,19 +288,20 @@ def pass_totals(per_batch: list[int], bounds: np.ndarray, counters: np.ndarray,
 
 
 def _pass_inputs(matcher: MaskedMatcher, af: GPUFocus, bf: GPUFocus, batches: list[tuple[int, int]],
-                 ranges: list[list[tuple[int, int, int, int]]], window_cap: int | None) -> tuple[Any, ...]:
+                 geometry: np.ndarray, counts: list[int], window_cap: int | None) -> tuple[Any, ...]:
     """The host arrays of one device pass over consecutive batches and the
     arguments of its `pcube_lease` call (their windows, bounds, matches and
-    counters, with one host synchronisation)."""
+    counters, with one host synchronisation).  geometry: the pass's subtiles as
+    [n, 4] (v_first, v_end, u_min, u_end) rows; counts: subtiles per batch."""
     width = PROFILE['tile_width']
     first, end = batches[0][0], batches[-1][1]
-    tiles = np.array([x for r in ranges for t in r for x in t], dtype=np.uint64)
-    per_batch = np.array([len(r) for r in ranges], dtype=np.uint32)
+    tiles = np.ascontiguousarray(geometry, dtype=np.uint64).reshape(-1)
+    per_batch = np.array(counts, dtype=np.uint32)
     ntot = int(per_batch.sum())
     ng = len(matcher.filter_groups)
     if window_cap is None:
         # expected coordinates in each window of the pass, with margin
-        u_span = ranges[-1][-1][3] - ranges[0][0][2]
+        u_span = int(geometry[-1, 3]) - int(geometry[0, 2])
         expect = max(u_span * af.focus.count / af.focus.modulus, (end - first) * bf.focus.count / bf.focus.modulus)
         window_cap = int(expect * 1.1) + 1_000_000
     bounds = np.zeros(ntot * 4, dtype=np.uint64)
@@ -273,26 +331,27 @@ def _run_pass(check: Callable[[], None], call: Callable[..., int], args: tuple[A
 def lease_rows(matcher: MaskedMatcher, af: GPUFocus, bf: GPUFocus, first: int, end: int,
                batch_width: int = BATCH_WIDTH,
                window_cap: int | None = None,
-               quiet: bool = False) -> Iterator[tuple[int, int, dict[str, Any] | QuietBatch, bytes]]:
+               quiet: bool = False) -> Iterator[tuple[int, int, dict[str, Any] | BatchTotals, bytes]]:
     """Every batch row of [first, end), from pipelined device passes.
 
-    The batches go to the device LEASE_CHUNK at a time, each group in one
+    The batches go to the device in groups (pass_sizes), each group in one
     pass (`pcube_lease`) on a worker thread; the caller folds one group's rows
     while the device computes the next.  Each row equals
     `MaskedMatcher.run(prepare(..., start, stop))` for its batch in every
     field except the timings, which are its pass's phases divided evenly over
     the pass's batches.  With each row come the bytes of
     encoded(row['subtile_scopes']), formatted by the library from the same
-    counters and bounds.  With quiet, a batch whose subtiles emitted nothing
-    (and saw no underflow) comes as its QuietBatch totals instead of a row.
+    counters and bounds.  With quiet, a batch comes as its BatchTotals instead
+    of a row (pass_rows).
     """
     config = matcher.config
     width = PROFILE['tile_width']
-    ng = len(matcher.filter_groups)
-    cap = 8192
     starts = list(range(first, end, batch_width))
-    groups = [[(s, min(s + batch_width, end)) for s in starts[i:i + LEASE_CHUNK]]
-              for i in range(0, len(starts), LEASE_CHUNK)]
+    groups = []
+    at = 0
+    for size in pass_sizes(len(starts)):
+        groups.append([(s, min(s + batch_width, end)) for s in starts[at:at + size]])
+        at += size
     pool = ThreadPoolExecutor(max_workers=1)
     call = native.lib().pcube_lease
     # the worker takes the interpreter lock only between passes, to return from one
@@ -302,62 +361,134 @@ def lease_rows(matcher: MaskedMatcher, af: GPUFocus, bf: GPUFocus, first: int, e
     sys.setswitchinterval(min(interval, 1e-4))
     try:
         pending = []
+        # the whole lease's subtile geometry in one library call (an array, no Python
+        # tuples), so every pass is submitted at once and the worker never waits for
+        # the next one's inputs
+        geometry, per_batch = tile_geometry(config, [span for batches in groups for span in batches], width)
+        at = row = 0
         for batches in groups:
-            # the geometry of later groups is computed while the device runs the first
-            ranges = [tile_ranges(config, s, e, width) for s, e in batches]
-            arrays, args = _pass_inputs(matcher, af, bf, batches, ranges, window_cap)
-            pending.append((batches, ranges, arrays, pool.submit(_run_pass, matcher.runtime.check, call, args)))
-        for batches, ranges, arrays, future in pending:
-            device_seconds = future.result()
-            tiles, _, bounds, sizes, counters, values, seconds = arrays
-            n = len(batches)
-            timings = dict(kernel_seconds=seconds[2] / n, focus_enumeration_seconds=seconds[0] / n,
-                           transfer_seconds=max(0.0, device_seconds - seconds[0] - seconds[2]) / n)
-            t0 = 0
-            offset = 0
-            tile_ends = tiles.reshape(-1, 4)[:, :2].copy()   # (v_first, v_end) per subtile, contiguous
-            emitted = [min(k, cap) for k in counters[:, ng + 1].tolist()]
-            limbs = values[:2 * sum(emitted)].tolist()
-            bound_rows = [tuple(r) for r in bounds.tolist()]
-            if quiet:
-                quiet_of, focus_of, pairs_of, sums_of = pass_totals([len(r) for r in ranges], bounds, counters, ng)
-                shares = dict(timings, verification_seconds=0.0, prefilter_build_seconds=seconds[1] / n)
-                filters = [list(group) for group in matcher.filter_groups]
-            for b, (s, e) in enumerate(batches):
-                nt = len(ranges[b])
-                block = counters[t0:t0 + nt]
-                if quiet and quiet_of[b]:
-                    totals = QuietBatch(nt, (ranges[b][0][0], ranges[b][-1][1]), focus_of[b], int(sizes[2 * b + 1]),
-                                        pairs_of[b], sums_of[b][0], sums_of[b][1:], filters, shares)
-                    yield s, e, totals, scope_record_bytes(tile_ends[t0:t0 + nt], block, ng, None,
-                                                           EMPTY_DIGEST.encode() * nt)
-                    t0 += nt
-                    continue
-                per_tile = []
-                for k in emitted[t0:t0 + nt]:
-                    per_tile.append([limbs[2 * i] + (limbs[2 * i + 1] << 64) for i in range(offset, offset + k)])
-                    offset += k
-                bnd = bound_rows[t0:t0 + nt]
-                row_started = time.perf_counter()
-                row = batch_row(config, matcher.filter_groups, ranges[b], bnd, int(sizes[2 * b + 1]), block,
-                                per_tile, cap, timings, row_started)
-                row['kernel_registers'] = int(matcher.kernel['registers'])
-                row['kernel_local_bytes'] = int(matcher.kernel['local_bytes'])
-                row['matching_layout'] = matcher.layout
-                row['prefilter_build_seconds'] = seconds[1] / n
-                scopes = row['subtile_scopes']
-                if row['local_survivors']:
-                    local = np.array([sc['local_survivors'] for sc in scopes], dtype=np.uint64)
-                    shas = ''.join(sc['output_sha256'] for sc in scopes).encode()
-                else:   # nothing emitted: every record holds 0 and the digest of []
-                    local, shas = None, EMPTY_DIGEST.encode() * nt
-                records = scope_record_bytes(tile_ends[t0:t0 + nt], block, ng, local, shas)
-                yield s, e, row, records
-                t0 += nt
+            counts = per_batch[at:at + len(batches)]
+            nt = sum(counts)
+            arrays, args = _pass_inputs(matcher, af, bf, batches, geometry[row:row + nt], counts, window_cap)
+            pending.append((batches, counts, arrays, pool.submit(_run_pass, matcher.runtime.check, call, args)))
+            at += len(batches)
+            row += nt
+        for batches, counts, arrays, future in pending:
+            yield from pass_rows(matcher, batches, counts, arrays, future.result(), quiet)
     finally:
         # a caller that stops early (an error in its fold) leaves no pass behind
         pool.shutdown(wait=True, cancel_futures=True)
         sys.setswitchinterval(interval)
+
+
+def pass_rows(matcher: Any, batches: list[tuple[int, int]], counts: list[int], arrays: tuple[Any, ...],
+              device_seconds: float,
+              quiet: bool) -> Iterator[tuple[int, int, dict[str, Any] | BatchTotals, bytes]]:
+    """The rows of one finished device pass, batch by batch, with each row's
+    encoded(subtile_scopes).
+
+    matcher: config, filter_groups, kernel, layout (a MaskedMatcher's);
+    arrays: the pass's host arrays (_pass_inputs); counts: subtiles per batch.
+
+    With quiet, a pass that saw no underflow and no overflow has every emitted
+    value checked natively in one call (check_values), its records formatted in
+    one call (pcube_scope_records_many) and its totals summed over the counters
+    (pass_totals), and each batch comes as its BatchTotals.  Otherwise, and for a
+    pass whose values fail a check, each batch's row is built by batch_row with
+    the Python predicate, so that the first failing batch raises as the
+    reference's loop does (a batch that emitted nothing still comes as its
+    totals with quiet).
+    """
+    config = matcher.config
+    ng = len(matcher.filter_groups)
+    cap = OUTPUT_CAP
+    tiles, _, bounds, sizes, counters, values, seconds = arrays
+    n = len(batches)
+    timings = dict(kernel_seconds=seconds[2] / n, focus_enumeration_seconds=seconds[0] / n,
+                   transfer_seconds=max(0.0, device_seconds - seconds[0] - seconds[2]) / n)
+    tile_ends = tiles.reshape(-1, 4)[:, :2].copy()   # (v_first, v_end) per subtile, contiguous
+    ntot = len(tile_ends)
+    listed = counters[:, ng + 1]
+    kept = np.minimum(listed, np.uint64(cap))
+    ordered = bool((bounds[:, 1] >= bounds[:, 0]).all() and (bounds[:, 3] >= bounds[:, 2]).all())
+    clean = ordered and not counters[:, ng + 2].any() and int(listed.max(initial=0)) <= cap
+    if quiet:
+        quiet_of, focus_of, pairs_of, sums_of = pass_totals(counts, bounds, counters, ng)
+        shares = dict(timings, verification_seconds=0.0, prefilter_build_seconds=seconds[1] / n)
+        filters = [list(group) for group in matcher.filter_groups]
+    checked = None
+    if clean and (quiet or kept.any()):
+        try:
+            checked = check_values(config, tile_ends[:, 0], tile_ends[:, 1], np.full(ntot, config.p, dtype=np.uint32),
+                                   kept, values)
+        except ArithmeticError:
+            checked = None   # the per-batch path below raises at the first failing batch
+    if quiet and checked is not None:
+        index, strings_of, digests, hits_of = checked
+        shas = bytearray(EMPTY_DIGEST.encode() * ntot)
+        for s, text_digest in zip(index, digests):
+            shas[64 * s:64 * s + 64] = text_digest.encode()
+        records = pass_record_bytes(counts, tile_ends, counters, ng, kept, bytes(shas))
+        heads = np.cumsum([0] + counts).tolist()
+        batch_of = (np.searchsorted(heads, index, side='right') - 1).tolist() if index else []
+        found = 0
+        t0 = 0
+        for b, (s, e) in enumerate(batches):
+            nt = counts[b]
+            batch_values: list[str] = []
+            batch_hits: list[dict[str, Any]] = []
+            while found < len(index) and batch_of[found] == b:
+                batch_values += strings_of[found]
+                batch_hits += hits_of[found]
+                found += 1
+            totals = BatchTotals(nt, (int(tile_ends[t0, 0]), int(tile_ends[t0 + nt - 1, 1])), focus_of[b],
+                                 int(sizes[2 * b + 1]), pairs_of[b], sums_of[b][0], sums_of[b][1:], filters, shares,
+                                 batch_values, batch_hits)
+            yield s, e, totals, records[b]
+            t0 += nt
+        return
+    verified = None
+    if checked is not None:
+        verified = verify_pass_from(checked, counts)
+    emitted = kept.tolist()
+    limbs: list[int] | None = None
+    t0 = 0
+    offset = 0
+    for b, (s, e) in enumerate(batches):
+        nt = counts[b]
+        block = counters[t0:t0 + nt]
+        if quiet and quiet_of[b]:
+            totals = BatchTotals(nt, (int(tile_ends[t0, 0]), int(tile_ends[t0 + nt - 1, 1])), focus_of[b],
+                                 int(sizes[2 * b + 1]), pairs_of[b], sums_of[b][0], sums_of[b][1:], filters, shares)
+            yield s, e, totals, scope_record_bytes(tile_ends[t0:t0 + nt], block, ng, None, EMPTY_DIGEST.encode() * nt)
+            t0 += nt
+            continue
+        per_tile = None
+        if verified is None:
+            if limbs is None:
+                limbs = values[:2 * sum(emitted)].tolist()
+            per_tile = []
+            for k in emitted[t0:t0 + nt]:
+                per_tile.append([limbs[2 * i] + (limbs[2 * i + 1] << 64) for i in range(offset, offset + k)])
+                offset += k
+        bnd = [tuple(r) for r in bounds[t0:t0 + nt].tolist()]
+        row_started = time.perf_counter()
+        ranges = [tuple(r) for r in tiles.reshape(-1, 4)[t0:t0 + nt].tolist()]
+        row = batch_row(config, matcher.filter_groups, ranges, bnd, int(sizes[2 * b + 1]), block,
+                        per_tile, cap, timings, row_started,
+                        verified=verified[b] if verified is not None else None)
+        row['kernel_registers'] = int(matcher.kernel['registers'])
+        row['kernel_local_bytes'] = int(matcher.kernel['local_bytes'])
+        row['matching_layout'] = matcher.layout
+        row['prefilter_build_seconds'] = seconds[1] / n
+        scopes = row['subtile_scopes']
+        if row['local_survivors']:
+            local = np.array([sc['local_survivors'] for sc in scopes], dtype=np.uint64)
+            shas_of = ''.join(sc['output_sha256'] for sc in scopes).encode()
+        else:   # nothing emitted: every record holds 0 and the digest of []
+            local, shas_of = None, EMPTY_DIGEST.encode() * nt
+        yield s, e, row, scope_record_bytes(tile_ends[t0:t0 + nt], block, ng, local, shas_of)
+        t0 += nt
 
 
 def compute_lease(matcher: MaskedMatcher, af: GPUFocus, bf: GPUFocus, first: int, end: int,
@@ -371,8 +502,8 @@ def compute_lease(matcher: MaskedMatcher, af: GPUFocus, bf: GPUFocus, first: int
     acc = Accumulator(first, end)
     if pipelined:
         for start, stop, row, records in lease_rows(matcher, af, bf, first, end, quiet=True):
-            if isinstance(row, QuietBatch):
-                acc.add_quiet(start, stop, row, records)
+            if isinstance(row, BatchTotals):
+                acc.add_totals(start, stop, row, records)
             else:
                 acc.add(start, stop, row, records)
             heartbeat()

@@ -269,9 +269,9 @@ def test_pipelined_lease_equals_per_batch_loop(runtime, config619, focuses619, m
 
 
 def test_pipelined_lease_over_several_passes(runtime, config619, focuses619, masked, monkeypatch) -> None:
-    """Passes of three batches: a lease of seven (the last ragged) spans three
-    device passes, the last with one batch, and folds to the per-batch loop's
-    receipt; a fold stopped after its first row leaves no pass behind."""
+    """Passes of at most three batches: a lease of seven (the last ragged) spans
+    three device passes (3, 3, 1) and folds to the per-batch loop's receipt; a
+    fold stopped after its first row leaves no pass behind."""
     from psuedocubes_amd import optimized
     monkeypatch.setattr(optimized, 'LEASE_CHUNK', 3)
     first = LEASE_FIRST + 40 * BATCH_WIDTH

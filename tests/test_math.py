@@ -278,3 +278,16 @@ def test_enumerate_cpu_complete_small_sets_and_partitions() -> None:
         if local3(n, 7):
             assert n in list(enumerate_cpu(small_config(7, 0, n), Tile(0, 9)))
             assert n not in list(enumerate_cpu(small_config(7, n, n + 100), Tile(0, 9)))
+
+
+def test_library_tile_ranges_many_equal_python(config619: Config) -> None:
+    """The lease's one-call geometry (pcube_tile_ranges_many) equals tile_ranges batch by batch:
+    full batches, a ragged last batch, the tail ending at B, single-v batches."""
+    from psuedocubes_amd.batch_experiment import tile_ranges, tile_ranges_many
+    first = 575222931200000000
+    spans = [(first + k * 6_400_000_000, first + (k + 1) * 6_400_000_000) for k in range(40)]
+    spans += [(first, first + 250_000_001), (B619 - 6_400_000_000, B619), (B619 - 1, B619), (0, 1), (5, 7)]
+    got = tile_ranges_many(config619, spans, 100_000_000)
+    assert got == [tile_ranges(config619, s, e, 100_000_000) for s, e in spans]
+    with pytest.raises(ValueError):
+        tile_ranges_many(config619, [(0, 6_500_000_000)], 100_000_000)
