@@ -12,7 +12,6 @@ from __future__ import annotations
 from dataclasses import dataclass
 import ctypes
 import hashlib
-import json
 import threading
 import time
 from typing import Any, NamedTuple
@@ -289,14 +288,24 @@ def check_values(config: Config, firsts: np.ndarray, ends: np.ndarray, primes: n
     root_list = roots.tolist()
     strings_of, digests, hits_of = [], [], []
     v = 0
+    sha256 = hashlib.sha256
     for n, a, b, p in zip(number, starts, stops, prime_of):
         chunk = raw_text[a:b]
-        strings = json.loads(chunk)
+        # the library's text of a subtile is the canonical encoding ["d1","d2",...]\n of
+        # its sorted decimal strings, digits only
+        strings = chunk[2:-3].decode('ascii').split('","')
         strings_of.append(strings)
-        digests.append(hashlib.sha256(chunk).hexdigest())
-        hits_of.append([{'n': value, 'p': p, 'cube_root_floor': str(root), 'perfect_cube': False,
-                         'failing_primes': [], 'valid_pseudocube': True, 'claim': CLAIM}
-                        for value, is_cube, root in zip(strings, cubes[v:v + n], root_list[v:v + n]) if not is_cube])
+        digests.append(sha256(chunk).hexdigest())
+        flags = cubes[v:v + n]
+        roots_of = root_list[v:v + n]
+        if 1 in flags:
+            hits_of.append([{'n': value, 'p': p, 'cube_root_floor': str(root), 'perfect_cube': False,
+                             'failing_primes': [], 'valid_pseudocube': True, 'claim': CLAIM}
+                            for value, is_cube, root in zip(strings, flags, roots_of) if not is_cube])
+        else:
+            hits_of.append([{'n': value, 'p': p, 'cube_root_floor': str(root), 'perfect_cube': False,
+                             'failing_primes': [], 'valid_pseudocube': True, 'claim': CLAIM}
+                            for value, root in zip(strings, roots_of)])
         v += n
     return index, strings_of, digests, hits_of
 

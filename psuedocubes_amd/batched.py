@@ -27,7 +27,6 @@ from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass
 import ctypes
 import gc
-import hashlib
 import json
 from pathlib import Path
 import sys
@@ -37,14 +36,15 @@ from typing import Any, Iterable
 import numpy as np
 
 from . import native
-from .batch_experiment import Verified, batch_row, tile_ranges, verify_pass
+from .batch_experiment import EMPTY_DIGEST, Verified, check_values, tile_ranges, verify_pass
 from .focus import Config
 from .gpu import GPUFocus, MatchEngine, Runtime
-from .verification import CLAIM
+from .optimized import tapered_sizes
 
 TUNING = Path(__file__).resolve().parent / 'tuning.json'
 DEFAULT_TUNING = dict(rows_per_chunk=8, blocks_per_cu=4, direct_below_rows=96)
 GROUP_WINDOWS = 1024   # windows per device pass of run(): one pass runs while the previous is assembled
+GROUP_EDGE = 128       # windows of run()'s first pass and of its last, tapered from GROUP_WINDOWS
 
 
 def load_tuning() -> dict[str, int]:
@@ -125,31 +125,43 @@ class BatchedEngine:
 
     def footprint(self, window: Window) -> int:
         """Estimated device bytes of one window in a group."""
-        u_span = (window.end - window.first) * self.base.A / self.base.B + (self.base.hi - self.base.lo) / self.base.B
-        nu = u_span * self.af.focus.count / self.af.focus.modulus
-        nv = (window.end - window.first) * self.bf.focus.count / self.bf.focus.modulus
+        return self.footprints([window])[0]
+
+    def footprints(self, windows: list[Window]) -> list[int]:
+        """footprint() of many windows at once: u and v coordinates, rows, residue
+        rows and the bitset words of the window's mask rows, from its v-length."""
+        length = np.array([w.end - w.first for w in windows], dtype=np.float64)
+        u_span = length * (self.base.A / self.base.B) + (self.base.hi - self.base.lo) / self.base.B
+        nu = u_span * (self.af.focus.count / self.af.focus.modulus)
+        nv = length * (self.bf.focus.count / self.bf.focus.modulus)
         words = nu / 64 + 1
-        return int(8 * nu + 8 * nv + 32 * nv + 16 * nu + 8 * 1300 * words) + 4096
+        return (np.floor(24 * nu + 40 * nv + 8 * 1300 * words).astype(np.int64) + 4096).tolist()
 
     def run(self, windows: list[Window], budget: int | None = None) -> list[dict[str, Any]]:
         """One result row per window, in input order.
 
-        Windows go to the device in groups of at most GROUP_WINDOWS within the
-        memory budget, each one device pass (launch) on a worker thread, in
-        order; this thread assembles a group's rows while the device runs the
-        next.  The cyclic garbage collector is paused meanwhile (the rows hold
+        Windows go to the device in groups (tapered_sizes: a first group of
+        GROUP_EDGE, groups of GROUP_WINDOWS, a tail halving down to GROUP_EDGE)
+        within the memory budget, each one device pass (launch) on a worker
+        thread, in order; this thread assembles a group's rows while the device
+        runs the next.  The cyclic garbage collector is paused meanwhile (the rows hold
         no reference cycles, and full collections over thousands of fresh rows
         only add pauses), and Python's switch interval is 0.1 ms, so the
-        worker gets the interpreter lock between passes at once."""
+        worker gets the interpreter lock between passes at once.  On success the
+        fresh objects (hundreds of thousands of row, scope and witness containers)
+        go straight to the collector's oldest generation before it resumes, unless
+        the process has frozen objects of its own: the young collection that would
+        otherwise traverse them all at once, to find no garbage, is skipped."""
         if budget is None:
             budget = max(256 << 20, self.runtime.cap // 4)
         rows: list[dict[str, Any] | None] = [None] * len(windows)
         groups: list[list[int]] = []
         group: list[int] = []
         used = 0
-        for index, window in enumerate(windows):
-            need = self.footprint(window)
-            if group and (used + need > budget or len(group) >= GROUP_WINDOWS):
+        limits = tapered_sizes(len(windows), GROUP_EDGE, GROUP_WINDOWS)
+        for index, need in enumerate(self.footprints(windows) if windows else []):
+            limit = limits[len(groups)] if len(groups) < len(limits) else GROUP_WINDOWS
+            if group and (used + need > budget or len(group) >= limit):
                 groups.append(group)
                 group, used = [], 0
             group.append(index)
@@ -161,16 +173,21 @@ class BatchedEngine:
         interval = sys.getswitchinterval()
         sys.setswitchinterval(min(interval, 1e-4))
         pool = ThreadPoolExecutor(max_workers=1)
+        finished = False
         try:
             passes = [pool.submit(self.launch, [windows[i] for i in g]) for g in groups]
             for g, launched in zip(groups, passes):
                 for index, row in zip(g, self.assemble(launched.result())):
                     rows[index] = row
+            finished = True
         finally:
             # an error leaves no pass behind
             pool.shutdown(wait=True, cancel_futures=True)
             sys.setswitchinterval(interval)
             if collecting:
+                if finished and gc.get_freeze_count() == 0:
+                    gc.freeze()
+                    gc.unfreeze()
                 gc.enable()
         return rows  # type: ignore[return-value]
 
@@ -206,38 +223,117 @@ class BatchedEngine:
         n = len(group)
         timings = dict(kernel_seconds=seconds[2] / n, focus_enumeration_seconds=seconds[0] / n,
                        transfer_seconds=max(0.0, device - seconds[0] - seconds[2]) / n)
-        return dict(windows=windows, plans=plans, bounds=bounds.reshape(ntot, 4), sizes=sizes, counters=counters,
-                    values=values, timings=timings)
+        return dict(windows=windows, plans=plans, tiles=tiles.reshape(ntot, 4), bounds=bounds.reshape(ntot, 4),
+                    sizes=sizes, counters=counters, values=values, timings=timings)
 
     def assemble(self, raw: dict[str, Any]) -> list[dict[str, Any]]:
-        """Verified result rows of a launched group, in its window order."""
+        """Verified result rows of a launched group, in its window order.
+
+        Each row is field for field batch_row's for its window (the checks and
+        their order too: the pass's values first, then per window an underflow
+        and an output overflow), built from pass-wide arrays: every counter
+        column gathered once per prime bound, every emitted value checked and
+        formatted in one native call (check_values)."""
+        started = time.perf_counter()
         windows, plans, bounds, sizes = raw['windows'], raw['plans'], raw['bounds'], raw['sizes']
         counters, values, timings = raw['counters'], raw['values'], raw['timings']
-        blocks, kept = [], []
-        c0 = 0
-        for window, (ranges, _) in zip(windows, plans):
-            ng = self.engines[window.p].ng
-            nt = len(ranges)
-            block = counters[c0:c0 + nt * (ng + 3)].reshape(nt, ng + 3)
-            blocks.append(block)
-            kept.append(np.minimum(block[:, ng + 1], self.cap))
-            c0 += nt * (ng + 3)
-        verified = self.verify_values(windows, plans, kept, values)
-        # each window's focus_u and cartesian opportunities, summed over its subtiles' bounds at once
-        starts = np.cumsum([0] + [len(ranges) for ranges, _ in plans[:-1]])
+        nw = len(windows)
+        if nw == 0:
+            return []
+        cap = self.cap
+        nts = [len(ranges) for ranges, _ in plans]
+        ngs = [self.engines[w.p].ng for w in windows]
+        nt_of = np.array(nts, dtype=np.int64)
+        heads = np.zeros(nw, dtype=np.int64)
+        np.cumsum(nt_of[:-1], out=heads[1:])
+        stride = np.repeat(np.array(ngs, dtype=np.int64) + 3, nt_of)
+        row_at = np.cumsum(stride) - stride            # each subtile's counter row
+        ng_of = stride - 3
+        emitted = counters[row_at + ng_of + 1]
+        kept = np.minimum(emitted, np.uint64(cap))
+        tiles = raw.get('tiles')
+        if tiles is None:
+            tiles = np.array([r for ranges, _ in plans for r in ranges], dtype=np.uint64).reshape(-1, 4)
+        primes = np.repeat(np.array([w.p for w in windows], dtype=np.uint32), nt_of)
+        index, strings_of, digests, hits_of = check_values(self.base, tiles[:, 0], tiles[:, 1], primes, kept, values)
+        underflow = np.add.reduceat(counters[row_at + ng_of + 2] != 0, heads)
+        overflow = np.add.reduceat(emitted > np.uint64(cap), heads)
+        if underflow.any() or overflow.any():
+            first = int(np.flatnonzero((underflow != 0) | (overflow != 0))[0])
+            if underflow[first]:
+                raise ArithmeticError('unsigned underflow')
+            raise OverflowError('fused tile output overflow; no truncated coverage')
+        valid_t = counters[row_at].tolist()
+        valid_w = np.add.reduceat(counters[row_at], heads).tolist()
+        # per prime bound: every subtile's group counts, and their sums per window
+        groups_t: dict[int, list[list[int]]] = {}
+        groups_w: dict[int, list[list[int]]] = {}
+        window_p = np.array([w.p for w in windows])
+        for p in set(window_p.tolist()):
+            ng = self.engines[p].ng
+            mine = window_p == p
+            rows_p = row_at[np.repeat(mine, nt_of)]
+            block = counters[rows_p[:, None] + 1 + np.arange(ng)] if ng else np.zeros((len(rows_p), 0), np.uint64)
+            groups_t[p] = block.tolist()
+            starts = np.zeros(int(mine.sum()), dtype=np.int64)
+            np.cumsum(nt_of[mine][:-1], out=starts[1:])
+            groups_w[p] = np.add.reduceat(block, starts, axis=0).tolist() if ng else [[] for _ in starts]
+        at_t = dict.fromkeys(groups_t, 0)
+        at_w = dict.fromkeys(groups_w, 0)
         du = (bounds[:, 1] - bounds[:, 0]).astype(np.int64)
         dv = (bounds[:, 3] - bounds[:, 2]).astype(np.int64)
-        focus_u = np.add.reduceat(du, starts).tolist() if len(du) else []
-        pairs = np.add.reduceat(du * dv, starts).tolist() if len(du) else []
+        focus_u = np.add.reduceat(du, heads).tolist()
+        pairs = np.add.reduceat(du * dv, heads).tolist()
         nvs = sizes[1::2].tolist()
+        kept_t = kept.tolist()
+        window_of = np.repeat(np.arange(nw), nt_of)[index].tolist() if index else []
+        verification = time.perf_counter() - started
+        share = verification / nw
         rows: list[dict[str, Any]] = []
-        for k, (window, (ranges, mode), block, checked) in enumerate(zip(windows, plans, blocks, verified)):
-            engine = self.engines[window.p]
-            row = batch_row(self.configs[window.p], engine.filter_groups, ranges, [], nvs[k], block, None, self.cap,
-                            timings, time.perf_counter(), verified=checked, totals=(focus_u[k], pairs[k]))
-            row['matching_layout'] = 'batched-direct' if mode else 'batched-residue-bitset'
-            row['prefilter_build_seconds'] = 0.0
-            rows.append(row)
+        found = 0
+        t0 = 0
+        for k in range(nw):
+            window = windows[k]
+            ranges, mode = plans[k]
+            p = window.p
+            nt = nts[k]
+            lists = groups_t[p]
+            g0 = at_t[p]
+            at_t[p] = g0 + nt
+            scopes = []
+            n = len(ranges)
+            for j in range(n):
+                first, end = ranges[j][0], ranges[j][1]
+                scopes.append({'first': str(first), 'end': str(end), 'valid_focused_candidates': valid_t[t0 + j],
+                               'group_survivors': lists[g0 + j], 'local_survivors': 0,
+                               'output_sha256': EMPTY_DIGEST})
+            all_text: list[str] = []
+            hits: list[dict[str, Any]] = []
+            while found < len(index) and window_of[found] == k:
+                scope = scopes[index[found] - t0]
+                scope['local_survivors'] = kept_t[index[found]]
+                scope['output_sha256'] = digests[found]
+                all_text += strings_of[found]
+                hits += hits_of[found]
+                found += 1
+            engine = self.engines[p]
+            wg = at_w[p]
+            at_w[p] = wg + 1
+            nvalues = len(all_text)
+            rows.append(dict(status='PASS', host_verified=True, focus_u=focus_u[k], focus_v=nvs[k],
+                             cartesian_opportunities=pairs[k], valid_focused_candidates=valid_w[k],
+                             genuine_cubes=nvalues - len(hits), local_survivors=nvalues, cube_checks=nvalues,
+                             kernel_seconds=timings.get('kernel_seconds', 0.0),
+                             transfer_seconds=timings.get('transfer_seconds', 0.0), verification_seconds=share,
+                             focus_enumeration_seconds=timings.get('focus_enumeration_seconds', 0.0),
+                             subtile_scopes=scopes, filter_groups=engine.filter_groups,
+                             group_survivors=groups_w[p][wg], all_values=all_text, cpu_confirmed_hits=hits,
+                             overflow_splits=[],
+                             elapsed_seconds=share + timings.get('focus_enumeration_seconds', 0.0),
+                             claim='Isolated validation only; no production coverage.',
+                             matching_layout='batched-direct' if mode else 'batched-residue-bitset',
+                             prefilter_build_seconds=0.0))
+            t0 += nt
         return rows
 
     def verify_values(self, windows: list[Window], plans: list, kept: list[np.ndarray],

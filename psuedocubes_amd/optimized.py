@@ -245,27 +245,33 @@ LEASE_CHUNK = 64   # most batches per device pass of a pipelined lease
 LEASE_EDGE = 8     # batches of its first and last passes
 
 
-def pass_sizes(n: int) -> list[int]:
-    """Batches per device pass for a lease of n batches: a short first pass, so the
-    host starts folding rows early; passes of LEASE_CHUNK (few pass boundaries, at
-    each of which the device waits for the next call); and a tail that halves
-    down to the edge size, so that folding one pass takes less time than the
-    device takes for the next and little folding is left once the device is done."""
-    edge = max(1, min(LEASE_EDGE, LEASE_CHUNK))
+def tapered_sizes(n: int, edge: int, chunk: int) -> list[int]:
+    """Sizes of consecutive device passes over n items, each at most chunk: a first
+    pass of edge items, so the host starts on results early; passes of chunk (few
+    pass boundaries, at each of which the device waits for the next call); and a
+    tail halving down to edge, so that the host's work on one pass takes less time
+    than the device takes for the next and little is left once the device is done."""
+    edge = max(1, min(edge, chunk))
     if n <= 2 * edge:
         return [n] if n <= edge else [edge, n - edge]
     rest = n - edge
     tail = []
     size = edge
-    while rest > 0 and size < LEASE_CHUNK:
+    while rest > 0 and size < chunk:
         tail.append(min(size, rest))
         rest -= tail[-1]
         size *= 2
     middle = []
     while rest > 0:
-        middle.append(min(LEASE_CHUNK, rest))
+        middle.append(min(chunk, rest))
         rest -= middle[-1]
     return [edge] + middle + tail[::-1]
+
+
+def pass_sizes(n: int) -> list[int]:
+    """Batches per device pass for a lease of n batches (tapered_sizes with
+    LEASE_EDGE and LEASE_CHUNK)."""
+    return tapered_sizes(n, LEASE_EDGE, LEASE_CHUNK)
 
 
 def pass_totals(per_batch: list[int], bounds: np.ndarray, counters: np.ndarray,
