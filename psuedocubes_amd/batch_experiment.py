@@ -359,7 +359,7 @@ def batch_row(config: Config, filter_groups: list[list[int]], ranges: list[tuple
         raise OverflowError('fused tile output overflow; no truncated coverage')
     # every counted value must have come back, and nothing else
     tiles = verified.values if verified is not None else per_tile
-    if tiles is None or len(tiles) != nrows or any(len(values) != n for values, n in zip(tiles, emitted)):
+    if tiles is None or len(tiles) != nrows or list(map(len, tiles)) != emitted:
         raise ArithmeticError('emitted values disagree with the subtile counters')
     scopes = skeleton if skeleton is not None else scope_skeleton(ranges)
     if verified is not None:
@@ -397,10 +397,10 @@ def batch_row(config: Config, filter_groups: list[list[int]], ranges: list[tuple
                     scope['output_sha256'] = digest([str(n) for n in values])
                     position += number
         all_text = [str(n) for n in all_values]
-    # only the columns the records hold, each subtile's group counts already a list
-    for valid, groups_of, scope in zip(counters[:, 0].tolist(), counters[:, 1:ng + 1].tolist(), scopes):
-        scope['valid_focused_candidates'] = valid
-        scope['group_survivors'] = groups_of
+    # only the columns the records hold: [valid, group_1..group_ng] of each subtile
+    for head, scope in zip(counters[:, :ng + 1].tolist(), scopes):
+        scope['valid_focused_candidates'] = head[0]
+        scope['group_survivors'] = head[1:]
     verification_seconds = time.perf_counter() - verification_started
     sums = counters[:, :ng + 1].sum(axis=0, dtype=np.uint64).tolist() if nrows else [0] * (ng + 1)
     groups = sums[1:]

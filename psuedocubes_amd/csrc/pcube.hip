@@ -1181,6 +1181,31 @@ double now_seconds() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// Wait for an event by polling it: a blocking wait sleeps and costs a wake-up of
+// tens of microseconds, twice per batch on the plain path.  After kPollSeconds the
+// wait blocks as hipEventSynchronize does, so long work does not hold a core.
+constexpr double kPollSeconds = 0.02;
+int event_wait(hipEvent_t e) {
+    const double t0 = now_seconds();
+    for (uint32_t k = 1;; ++k) {
+        const hipError_t rc = hipEventQuery(e);
+        if (rc == hipSuccess) return PCUBE_OK;
+        if (rc != hipErrorNotReady) return fail(PCUBE_DEVICE, "hipEventQuery: %s", hipGetErrorString(rc));
+        if (k % 64 == 0 && now_seconds() - t0 > kPollSeconds) break;
+        __builtin_ia32_pause();
+    }
+    HIP_TRY(hipEventSynchronize(e));
+    return PCUBE_OK;
+}
+
+// event_wait on everything queued on the library's stream so far.
+int stream_wait() {
+    static hipEvent_t done = nullptr;
+    if (!done) HIP_TRY(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+    HIP_TRY(hipEventRecord(done, g.stream));
+    return event_wait(done);
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -1556,7 +1581,7 @@ int pcube_prepare(int focus_u, int focus_v, uint32_t ntiles, const uint64_t* til
                            (const uint64_t*)us_out, (const uint64_t*)fu->wheel.win_info.as<uint64_t>(), us_cap,
                            (const uint64_t*)vs_out, (const uint64_t*)fv->wheel.win_info.as<uint64_t>(), vs_cap,
                            queries, ntiles, g_result_host.as<uint64_t>());
-        HIP_TRY(hipStreamSynchronize(g.stream));
+        TRY(stream_wait());
         const bool again_u = r[nq + 2] && cap_u < (uint32_t)kWheelWaveSpan;
         const bool again_v = r[nq + 3] && cap_v < (uint32_t)kWheelWaveSpan;
         if (!again_u && !again_v) break;
@@ -1843,13 +1868,13 @@ int pcube_match_finish(int matcher, uint64_t* counters, uint64_t* values, uint64
         return fail(PCUBE_INVALID, "a lease on this matcher reused the batch buffers before the batch was collected");
     const uint64_t nc = (uint64_t)pd.ntiles * (mt->ng + 3);
     uint64_t* hd = mt->down.as<uint64_t>();
-    HIP_TRY(hipEventSynchronize(mt->ev[5]));
+    TRY(event_wait(mt->ev[5]));
     if (hd[3] > mt->cand_cap) {
         // a bitset-survivor region overflowed: size the regions from its count and run again
         if (hd[3] > kMaxItemCandidates) pd.engine = kUnstagedBitsets;
         else mt->cand_cap = (hd[3] + 63) / 64 * 64;
         TRY(match_launch(*mt));
-        HIP_TRY(hipEventSynchronize(mt->ev[5]));
+        TRY(event_wait(mt->ev[5]));
     }
     const uint64_t listed = hd[0];
     const uint32_t err = (uint32_t)hd[1];
@@ -1958,7 +1983,7 @@ int pcube_lease(int matcher, int focus_u, int focus_v, uint32_t nbatches, const 
                                (const uint64_t*)queries.as<uint64_t>(), 2 * ntot, 2 * ntot,
                                g_result_host.as<uint64_t>());
             HIP_TRY(hipGetLastError());
-            HIP_TRY(hipStreamSynchronize(g.stream));
+            TRY(stream_wait());
             const bool again_u = r[nq + 2] && cap_u < (uint32_t)kWheelWaveSpan;
             const bool again_v = r[nq + 3] && cap_v < (uint32_t)kWheelWaveSpan;
             if (!again_u && !again_v) break;
@@ -1990,7 +2015,7 @@ int pcube_lease(int matcher, int focus_u, int focus_v, uint32_t nbatches, const 
                            positions.as<uint64_t>() + 2 * ntot);
         fallback.resize(nq);
         HIP_TRY(hipMemcpyAsync(fallback.data(), positions.p, nq * 8, hipMemcpyDeviceToHost, g.stream));
-        HIP_TRY(hipStreamSynchronize(g.stream));
+        TRY(stream_wait());
         up = fallback.data();
     }
     const uint64_t* vp = up + 2 * ntot;
@@ -2040,7 +2065,7 @@ int pcube_lease(int matcher, int focus_u, int focus_v, uint32_t nbatches, const 
         HIP_TRY(hipMemcpyAsync(host, mt->counters.p, ntot * stride * 8, hipMemcpyDeviceToHost, g.stream));
         HIP_TRY(hipMemcpyAsync(st, mt->status.p, 4 * 8, hipMemcpyDeviceToHost, g.stream));
         HIP_TRY(hipEventRecord(ev[3], g.stream));
-        HIP_TRY(hipStreamSynchronize(g.stream));
+        TRY(stream_wait());
         if (geometry_rc || st[3] <= mt->cand_cap) break;
         if (st[3] > kMaxItemCandidates) engine = kUnstagedBitsets;
         else mt->cand_cap = (st[3] + 63) / 64 * 64;
