@@ -302,8 +302,28 @@ def served_leg(runtime, af, bf, requests: int, load: float) -> dict:
     makespan = time.perf_counter() - t0
     served = sum(c.row['valid_focused_candidates'] for c in done)
     lat = latency_summary(done)
+    # the same stream again, every completed row committed to a coverage ledger (SQLite,
+    # WAL, synchronous=FULL, compact gzip receipts in the database) by a writer process
+    # before the request is reported complete
+    import tempfile
+    from psuedocubes_amd.accounting import Ledger, LedgerWriter
+    with tempfile.TemporaryDirectory(prefix='served-ledger-') as directory:
+        ledger_config = dict(served='p=619', lo=str(config.lo), hi=str(config.hi))
+        with LedgerWriter(Path(directory), ledger_config, 'bench') as writer:
+            gc.collect()
+            t0 = time.perf_counter()
+            with_ledger = Server(engine, ledger=writer).serve(request_stream(config, requests, 619003, rate=rate))
+            ledger_makespan = time.perf_counter() - t0
+        ledger = Ledger(Path(directory), ledger_config, 'bench')
+        audit = ledger.audit()
+        receipt_bytes = ledger.db.execute('SELECT coalesce(sum(length(data)),0) FROM receipt_blobs').fetchone()[0]
+        ledger.db.close()
+    ledger_leg = dict(value=sum(c.row['valid_focused_candidates'] for c in with_ledger) / ledger_makespan,
+                      makespan_seconds=ledger_makespan, latency_seconds=latency_summary(with_ledger),
+                      committed=audit['states'].get('committed', 0), replays=len(audit['validations']),
+                      receipt_bytes=receipt_bytes)
     return dict(workload=f'{requests} Poisson range requests, L log-uniform 1e7..6.4e9 v, p=619, seed 619003',
-                value=served / makespan, unit='focused candidate positions/s', capacity=capacity,
+                value=served / makespan, unit='focused candidate positions/s', capacity=capacity, ledger=ledger_leg,
                 saturated_capacity=saturated_capacity, capacity_probes=probes,
                 offered_load=load, rate_per_second=rate, makespan_seconds=makespan, requests=len(done),
                 latency_seconds=lat, steps=server.steps, mean_requests_per_step=statistics.fmean(server.step_sizes),

@@ -14,8 +14,13 @@ device computes the next; they are field for field those of `prepare` +
 """
 from __future__ import annotations
 
+from contextlib import closing
 import ctypes
 import hashlib
+import json
+import os
+from pathlib import Path
+import sqlite3
 import sys
 import threading
 import time
@@ -76,6 +81,34 @@ def validate_result(first: int, end: int, result: dict[str, Any]) -> None:
             raise ValueError('compact output hash mismatch')
     elif 'subtiles_executed' in result or 'receipt_format' in result:
         raise ValueError('unknown receipt format')
+
+
+def allocated_bytes(directory: Path) -> int:
+    """Disk blocks allocated under a ledger directory, in bytes (`pseudocube/optimized.py:93-106`).
+    A file that vanishes during the scan is tolerated only if it is a temporary
+    receipt (`.tmp`), which an atomic rename may remove at any time."""
+    total = directory.stat().st_blocks * 512
+    with os.scandir(directory) as entries:
+        for entry in entries:
+            try:
+                if entry.is_dir(follow_symlinks=False):
+                    total += allocated_bytes(Path(entry.path))
+                else:
+                    total += entry.stat(follow_symlinks=False).st_blocks * 512
+            except FileNotFoundError:
+                if not entry.name.endswith('.tmp'):
+                    raise
+    return total
+
+
+def ledger_settings(directory: Path) -> tuple[str, int, dict[str, Any]]:
+    """(active build, batch width, runtime profile) of a shared ledger, read only
+    (`pseudocube/optimized.py:109-113`)."""
+    uri = (directory / 'coverage.sqlite3').as_uri() + '?mode=ro'
+    with closing(sqlite3.connect(uri, uri=True)) as db:
+        state = dict(db.execute('SELECT key,value FROM meta').fetchall())
+    return (state.get('active_build', state['frozen_build']), int(state['batch_width']),
+            json.loads(state.get('runtime_profile', '{}')))
 
 
 class BatchTotals(NamedTuple):

@@ -17,7 +17,13 @@ filter tables resident and loops:
   * a request completes when all its pieces are back.  Its row merges its
     pieces' rows and equals the plain drop-in's row for the whole range
     (`prepare` + `MaskedMatcher.run`, pseudocube/batch_experiment.py:104-167):
-    per-subtile scopes and values do not depend on what shares a launch.
+    per-subtile scopes and values do not depend on what shares a launch;
+  * with a ledger, the requests that complete in one step are committed as
+    coverage of their scopes in one transaction (`Ledger.commit_batch`:
+    compact gzip receipts in the database, a repeated scope recorded as a
+    replay) before any of them is reported complete.  A `LedgerWriter` does
+    that in a child process: the loop hands each step's rows over and goes on,
+    and reports the requests when the writer acknowledges their commit.
 
 Latency is completion time minus arrival time.
 
@@ -39,6 +45,7 @@ import sys
 import time
 from typing import Any, Callable
 
+from .accounting import Ledger, LedgerWriter
 from .batched import BatchedEngine, Window
 from .focus import Config, Tile
 
@@ -64,6 +71,7 @@ class Completed:
     admitted: float
     finished: float
     error: str | None = None   # set for a request rejected at admission; row is then its rejection record
+    receipt: str | None = None   # the ledger attempt token of the committed row, with a ledger
 
     @property
     def latency(self) -> float:
@@ -95,8 +103,9 @@ def merge_rows(rows: list[dict[str, Any]]) -> dict[str, Any]:
 
 class Server:
     def __init__(self, engine: BatchedEngine, budget: int | None = None, max_slots: int = 256,
-                 piece_tiles: int = 8) -> None:
+                 piece_tiles: int = 8, ledger: Ledger | LedgerWriter | None = None) -> None:
         self.engine = engine
+        self.ledger = ledger
         self.budget = budget if budget is not None else max(256 << 20, engine.runtime.cap // 4)
         self.max_slots = max_slots
         self.piece_tiles = piece_tiles
@@ -116,6 +125,12 @@ class Server:
             return str(error)
         return None
 
+    @staticmethod
+    def scope(request: Request, width: int = 100_000_000) -> dict[str, str]:
+        """The ledger scope of a request: its prime bound, v range and subtile width."""
+        return {'kind': 'served-range', 'p': str(request.p), 'first': str(request.first), 'end': str(request.end),
+                'tile_width': str(width)}
+
     def pieces(self, request: Request, width: int = 100_000_000) -> deque:
         span = self.piece_tiles * width
         return deque(Window(s, min(s + span, request.end), request.p, width)
@@ -134,6 +149,8 @@ class Server:
         used = 0
         done: list[Completed] = []
         inflight: tuple[Future, list[tuple[int, Window]]] | None = None
+        committing: dict[int, list] = {}   # writer group id -> its finished requests
+        writer = isinstance(self.ledger, LedgerWriter)
         pool = ThreadPoolExecutor(max_workers=1)
         collecting = gc.isenabled()
         gc.disable()
@@ -187,21 +204,43 @@ class Server:
         def finish(raw: dict[str, Any], step: list[tuple[int, Window]]) -> None:
             nonlocal used
             rows = self.engine.assemble(raw)
+            finished = []
             for (rid, _), row in zip(step, rows):
                 state = active[rid]
                 state.rows.append(row)
                 state.outstanding -= 1
                 if state.outstanding == 0:
-                    completed = Completed(state.request, merge_rows(state.rows), state.admitted, now())
-                    done.append(completed)
-                    if on_complete:
-                        on_complete(completed)
+                    finished.append((state.request, merge_rows(state.rows), state.admitted))
                     used -= state.reserved
                     del active[rid]
                     order.remove(rid)
+            if not finished:
+                return
+            if self.ledger is None:
+                report(finished, [None] * len(finished))
+                return
+            entries = [(self.scope(request), row) for request, row, _ in finished]
+            if writer:
+                committing[self.ledger.submit(entries)] = finished
+            else:
+                report(finished, self.ledger.commit_batch(entries))
+
+        def report(finished: list, tokens: list) -> None:
+            t = now()
+            for (request, row, admitted), token in zip(finished, tokens):
+                completed = Completed(request, row, admitted, t, receipt=token)
+                done.append(completed)
+                if on_complete:
+                    on_complete(completed)
+
+        def acknowledged(wait: bool = False) -> None:
+            for ident, tokens in self.ledger.acknowledged(wait):
+                report(committing.pop(ident), tokens)
 
         try:
             while pending or waiting or active or inflight:
+                if writer and committing:
+                    acknowledged()
                 admit()
                 if inflight is None:
                     step = next_step()
@@ -226,6 +265,8 @@ class Server:
                 finish(raw, step)
                 if inflight is not None:
                     self.overlap_seconds += now() - t0
+            while committing:
+                acknowledged(wait=True)
         finally:
             pool.shutdown(wait=True)
             sys.setswitchinterval(interval)

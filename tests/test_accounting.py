@@ -19,7 +19,7 @@ import random
 import pytest
 
 from oracle import oracle as O
-from psuedocubes_amd.accounting import digest, encoded, interoperable
+from psuedocubes_amd.accounting import digest, encoded, encoded_str_keys, interoperable
 
 GOLDEN = Path(__file__).parent / 'golden'
 LIMIT = 1 << 53
@@ -66,6 +66,43 @@ def test_fast_path_equals_element_rule() -> None:
         got = interoperable(value)
         assert got == expected
         assert json.dumps(got, sort_keys=True) == json.dumps(expected, sort_keys=True)
+
+
+def str_keyed(rng: random.Random, depth: int = 0):
+    """Random JSON values with str keys only: integers at and around +-2^53 (as
+    numbers and inside strings next to separators), digit strings, floats."""
+    kind = rng.randrange(9 if depth < 4 else 5)
+    if kind == 0:
+        return rng.choice([0, 1, -1, LIMIT - 1, LIMIT, -LIMIT, -LIMIT + 1, 10**15, 10**16 - 1, 10**16, -10**17,
+                           rng.getrandbits(70) * rng.choice([1, -1]), rng.getrandbits(52)])
+    if kind == 1:
+        return rng.choice(['', 'x', ':1234567890123456', ',-9007199254740993', '[' + '9' * 20, str(rng.getrandbits(90)),
+                           'a"b', 'tab\tnew\nline', '\u00e9'])
+    if kind == 2:
+        return rng.choice([True, False, None, 0.5, 1e17, -2.5e-300, 123456789012345678.0])
+    if kind in (3, 4):
+        return rng.getrandbits(rng.choice([10, 52, 53, 54, 64])) * rng.choice([1, -1])
+    if kind in (5, 6):
+        return [str_keyed(rng, depth + 1) for _ in range(rng.randrange(6))]
+    if kind == 7:
+        return tuple(str_keyed(rng, depth + 1) for _ in range(rng.randrange(4)))
+    return {rng.choice(['a', 'b', 'n', 'first', 'end', '9', '10', ':x']): str_keyed(rng, depth + 1)
+            for _ in range(rng.randrange(5))}
+
+
+def test_str_keyed_encoding_equals_encoded() -> None:
+    """encoded_str_keys, the receipts' encoder, gives encoded()'s bytes on str-keyed
+    values: big integers become strings whether they sit first in a list, after a
+    key or after a comma, and digits inside strings change nothing."""
+    rng = random.Random(6190)
+    fallbacks = 0
+    for _ in range(20000):
+        value = str_keyed(rng)
+        assert encoded_str_keys(value) == encoded(value), value
+        fallbacks += encoded(value) != (json.dumps(value, sort_keys=True, separators=(',', ':')) + '\n').encode()
+    assert fallbacks > 1000   # the big-integer rule was exercised
+    for value in (LIMIT, -LIMIT, [LIMIT], {'a': -LIMIT}, [1, LIMIT], LIMIT - 1, [-LIMIT + 1]):
+        assert encoded_str_keys(value) == encoded(value)
 
 
 def test_boundary_bytes() -> None:

@@ -64,3 +64,28 @@ def test_merge_rows_of_pieces_equals_whole(runtime, config619, focuses619) -> No
     parts = [plain.run(prepare(runtime, config619, *focuses619, s, s + 400_000_000))
              for s in range(first, first + 1_200_000_000, 400_000_000)]
     assert strip(merge_rows(parts)) == strip(whole)
+
+
+@pytest.mark.parametrize('writer', [False, True])
+def test_served_rows_are_committed_to_the_ledger(runtime, config619, engine, tmp_path, writer) -> None:
+    """Each completed request's row reaches the ledger as coverage of its scope before
+    it is reported, in the serving process or through a writer process, and the
+    receipt decodes to the same canonical row."""
+    from psuedocubes_amd.accounting import Ledger, LedgerWriter, encoded
+    requests = request_stream(config619, 16, 619023, rate=0.0)
+    identity = {'served': config619.p, 'lo': str(config619.lo), 'hi': str(config619.hi)}
+    if writer:
+        with LedgerWriter(tmp_path, identity, 'test-build') as child:
+            done = Server(engine, piece_tiles=4, ledger=child).serve(requests, realtime=False)
+        ledger = Ledger(tmp_path, identity, 'test-build')
+    else:
+        ledger = Ledger(tmp_path, identity, 'test-build')
+        done = Server(engine, piece_tiles=4, ledger=ledger).serve(requests, realtime=False)
+    audit = ledger.audit()
+    assert audit['states'] == {'committed': len(requests)}
+    by_token = {}
+    for token, sha in ledger.db.execute("SELECT token,sha FROM work WHERE state='committed'"):
+        by_token[token] = ledger.open_receipt('sqlite:' + token, token, sha)['result']
+    for completed in done:
+        assert encoded(by_token[completed.receipt]) == encoded(completed.row)
+    ledger.db.close()

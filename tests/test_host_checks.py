@@ -88,7 +88,8 @@ class _StandInEngine:
         rows = []
         for w in windows:
             row = {k: 0 for k in MERGED_COUNTS + MERGED_TIMES}
-            row.update(focus_v=w.end - w.first, subtile_scopes=[{'first': str(w.first), 'end': str(w.end)}],
+            row.update(status='PASS', host_verified=True, focus_v=w.end - w.first,
+                       subtile_scopes=[{'first': str(w.first), 'end': str(w.end)}],
                        all_values=[], cpu_confirmed_hits=[], overflow_splits=[], group_survivors=[0] * 24)
             rows.append(row)
         return rows
@@ -110,3 +111,52 @@ def test_served_loop_answers_malformed_requests_and_goes_on() -> None:
         assert completed.error is None
         assert completed.row['focus_v'] == request.end - request.first
         assert [s['first'] for s in completed.row['subtile_scopes']][0] == str(request.first)
+
+
+def test_served_loop_commits_completed_rows_to_the_ledger(tmp_path) -> None:
+    """With a ledger, every completed request is coverage of its scope before it is
+    reported (one transaction per device step); a rejected request is not; a
+    repeated range is recorded as a replay, not as a second unit of coverage."""
+    from psuedocubes_amd.accounting import Ledger
+    good = [Request(i, 10**15 + i * 10**10, 10**15 + i * 10**10 + 1_700_000_000, 0.0) for i in range(5)]
+    repeat = Request(50, good[1].first, good[1].end, 0.0)
+    bad = [Request(100, 50, 50, 0.0)]
+    ledger = Ledger(tmp_path, {'served': 'p=619 stand-in'}, 'build')
+    server = Server(_StandInEngine(), piece_tiles=8, ledger=ledger)
+    committed_when_seen = []
+    done = server.serve(good + bad + [repeat], realtime=False,
+                        on_complete=lambda c: committed_when_seen.append(
+                            c.receipt is None or ledger.db.execute('SELECT count(*) FROM receipt_blobs WHERE token=?',
+                                                                   (c.receipt,)).fetchone()[0] == 1))
+    assert all(committed_when_seen)
+    by_id = {c.request.id: c for c in done}
+    assert by_id[100].receipt is None and all(by_id[r.id].receipt for r in good + [repeat])
+    audit = ledger.audit()
+    assert audit['states'] == {'committed': 5}
+    assert sorted(r['focus_v'] for r in audit['committed_results']) == [1_700_000_000] * 5
+    assert [v['purpose'] for v in audit['validations']] == ['served repeat']
+    scopes = sorted(s for (s,) in ledger.db.execute('SELECT scope FROM work'))
+    assert [json.loads(s) for s in scopes] == sorted((Server.scope(r) for r in good), key=lambda s: s['first'])
+    ledger.db.close()
+
+
+def test_served_loop_with_a_writer_process_reports_after_commit(tmp_path) -> None:
+    """The same through a LedgerWriter: the loop hands each step's rows to the child
+    process and reports a request only once the child acknowledged its commit;
+    reopening the ledger shows every row as coverage."""
+    from psuedocubes_amd.accounting import Ledger, LedgerWriter
+    good = [Request(i, 10**15 + i * 10**10, 10**15 + i * 10**10 + 1_700_000_000, 0.0) for i in range(40)]
+    config = {'served': 'p=619 stand-in'}
+    with LedgerWriter(tmp_path, config, 'build') as writer:
+        done = Server(_StandInEngine(), piece_tiles=8, ledger=writer).serve(good, realtime=False)
+        assert not writer.pending
+    assert sorted(c.request.id for c in done) == list(range(40)) and all(c.receipt for c in done)
+    ledger = Ledger(tmp_path, config, 'build')
+    audit = ledger.audit()
+    assert audit['states'] == {'committed': 40}
+    tokens = {row[0] for row in ledger.db.execute("SELECT token FROM work WHERE state='committed'")}
+    assert tokens == {c.receipt for c in done}
+    ledger.db.close()
+    # a writer that cannot open its ledger fails at once
+    with pytest.raises(ValueError):
+        LedgerWriter(tmp_path, {'another': 'configuration'}, 'build')
