@@ -346,3 +346,28 @@ class SharedFrontier(Ledger):
                     allocated_v_end=str(allocated), totals=totals, worker_commits=workers,
                     unfinished_leases=unfinished, no_overlap=True, full_interval_complete=contiguous == self.end,
                     independent_arithmetic_replay=False)
+
+
+def lease_worker(frontier: SharedFrontier, owner: str, matcher: Any, af: Any, bf: Any, leases: int = 1,
+                 compute: Any = None) -> list[dict[str, Any]]:
+    """One owner's loop (the reference's GPU worker): take a lease, compute it
+    (`optimized.compute_lease`, pipelined on the device), commit its compact
+    receipt; a lease whose computation fails goes back to pending with the
+    error as its attempt's reason, and the error propagates.  Stops after
+    `leases` leases or when the frontier has nothing left; returns the
+    committed results."""
+    if compute is None:
+        from .optimized import compute_lease as compute
+    done = []
+    for _ in range(leases):
+        job = frontier.next_batch(owner)
+        if job is None:
+            break
+        try:
+            result = compute(matcher, af, bf, job[2], job[3])
+        except BaseException as error:
+            frontier.fail_lease(owner, job, f'{type(error).__name__}: {error}')
+            raise
+        frontier.complete(owner, job, result)
+        done.append(result)
+    return done

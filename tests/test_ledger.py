@@ -418,3 +418,28 @@ def test_storage_scan_tolerates_only_vanishing_temporary_files(tmp_path: Path) -
         entry.name = 'committed.json.gz'
         with pytest.raises(FileNotFoundError):
             allocated_bytes(tmp_path)
+
+
+def test_lease_worker_commits_and_returns_failed_leases(tmp_path: Path) -> None:
+    """The owner loop over a stand-in computation: two leases committed; a computation
+    that raises sends its lease back to pending, and the next call retries the same range."""
+    from psuedocubes_amd.multi import lease_worker
+    db = shared(tmp_path, BASE, end=40)
+    calls = []
+
+    def compute(matcher, af, bf, first, end):
+        calls.append((first, end))
+        if len(calls) == 3:
+            raise ArithmeticError('injected verification failure')
+        return lease_row(first, end)
+
+    assert len(lease_worker(db, 'a', None, None, None, leases=2, compute=compute)) == 2
+    with pytest.raises(ArithmeticError):
+        lease_worker(db, 'a', None, None, None, leases=1, compute=compute)
+    reason = db.db.execute("SELECT reason FROM lease_attempts WHERE state='failed'").fetchone()[0]
+    assert 'injected' in reason
+    assert len(lease_worker(db, 'a', None, None, None, leases=5, compute=compute)) == 2
+    assert calls == [(0, 10), (10, 20), (20, 30), (20, 30), (30, 40)]
+    audit = db.audit_shared()
+    assert audit['full_interval_complete'] and audit['totals']['commits'] == 4
+    db.db.close()
