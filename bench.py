@@ -57,6 +57,8 @@ def parse() -> argparse.Namespace:
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=2000)
     ap.add_argument('--warmup', type=int, default=50)
+    ap.add_argument('--prime', type=int, default=256,
+                    help='untimed batches from a separate seeded stream before the warmup steps')
     ap.add_argument('--cpu-seconds', type=float, default=8.0, help='bounded CPU-baseline sample (config 2)')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-lease', action='store_true')
@@ -371,9 +373,18 @@ def main() -> None:
         batch = prepare(runtime, config, af, bf, k * BATCH_WIDTH, (k + 1) * BATCH_WIDTH)
         return matcher.run(batch)
 
+    # priming, before the warmup: seeded batches from a stream of their own (the timed
+    # steps draw the same batches with or without it) bring the device to its steady
+    # clocks and the matcher's grow-only buffers to the largest batch sizes
+    prime_rng = random.Random(619011)
+    primed = time.perf_counter()
+    for _ in range(args.prime):
+        k = prime_rng.randrange(nbatch)
+        matcher.run(prepare(runtime, config, af, bf, k * BATCH_WIDTH, (k + 1) * BATCH_WIDTH))
+    primed = time.perf_counter() - primed
     for _ in range(args.warmup):
         step()
-    print(f'# warmup {args.warmup} done', file=sys.stderr, flush=True)
+    print(f'# priming {args.prime} batches ({primed:.3f} s), warmup {args.warmup} done', file=sys.stderr, flush=True)
     positions, kernel, prefilter, intersection = 0, 0.0, 0.0, 0.0
     # the first timed steps' rows (references only) for the CPU baseline's gate
     kept: list = []
@@ -410,7 +421,8 @@ def main() -> None:
                                 kernel='k_match_staged<8>', kernel_ms=1000 * intersection_seconds,
                                 ops_per_position=OPS_PER_POSITION, positions_per_launch=per_launch),
                   lds=dict(achieved=lds_rate / 1e12, peak=LDS_PEAK / 1e12, unit='TB/s', frac=lds_rate / LDS_PEAK,
-                           bytes_per_position=LDS_BYTES_PER_POSITION))
+                           bytes_per_position=LDS_BYTES_PER_POSITION),
+                  priming=dict(batches=args.prime, seed=619011, seconds=primed))
     # executed integer-VALU lane operations and HBM bytes of the same kernel, per
     # launch from the committed counter summary, over this run's kernel time
     valu = measured('valu_instructions_per_launch')
