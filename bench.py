@@ -10,9 +10,17 @@ build, match kernel, device-to-host copy, exact host verification and the
 per-subtile output hashes.  The focus wheels and filter tables are resident
 in HBM before timing starts, as in the reference's long-running worker.
 
-Prints one JSON line.  Extra fields: kernel-only rate, the pipelined lease
-rate (256 batches in 32-batch device passes overlapped with the host fold), the roofline of the match kernel, and
-the CPU baseline (this repository's C restatement on the host cores).
+Before the warmup steps the device is primed with --prime batches drawn from a
+separate seeded stream (random.Random(619011)); the timed batches are the same
+with or without it.
+
+Prints one JSON line.  Extra fields: kernel-only rate, the pipelined lease rate
+(256 batches in seven device passes overlapped with the host fold), the batched
+stream (4096 uneven windows of mixed prime bound), the served form (Poisson range
+requests, p50/p99, and the same stream with every row committed to a coverage
+ledger), the roofline of the match kernel, and the CPU baseline (this
+repository's C restatement on the host cores, reported only after its rows equal
+the run's GPU rows).
 """
 from __future__ import annotations
 

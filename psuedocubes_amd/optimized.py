@@ -4,13 +4,19 @@ Drop-in for the reference's `pseudocube/optimized.py:15-90` (`PROFILE`,
 `BATCH_WIDTH`, `LEASE_WIDTH`, `Accumulator`, `compute_lease`) and the
 receipt check `validate_result` of `pseudocube/multi.py:36-67`.
 
-`compute_lease` hands the lease to libpcube in passes of LEASE_CHUNK
-consecutive batches, each one call on a worker thread: one focus window per
-wheel over the pass, every batch's bitset build and match queued on the
-device stream without host synchronisation, and one copy of the pass's
-counters at the end.  The caller's thread assembles one pass's rows while the
-device computes the next; they are field for field those of `prepare` +
-`MaskedMatcher.run`, so the transcript is unchanged.
+`compute_lease` hands the lease to libpcube in device passes of consecutive
+batches (8, then 64 at a time, the last passes halving down to 8), each one
+call on a worker thread: both focus windows of the pass in one paired
+launch, every batch's subtile bounds in one launch, one host synchronisation,
+then every batch's bitset build and match queued on the device stream and one
+copy of the pass's counters at the end.  The caller's thread folds one pass's
+batches into the receipt while the device computes the next: each from its
+totals, its values checked natively in one call per pass and its scope
+records formatted in one call.  The receipt is field for field that of
+`prepare` + `MaskedMatcher.run` per batch, so the transcript is unchanged.
+
+Also here: `allocated_bytes` and `ledger_settings`, the storage helpers of the
+reference's lease runner (`pseudocube/optimized.py:93-113`).
 """
 from __future__ import annotations
 
