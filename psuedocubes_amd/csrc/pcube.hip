@@ -720,9 +720,9 @@ struct Matcher {
     std::vector<Modulus> mods;
     Buffer d_mods, d_allow, d_pstr;
     // status words: [0] survivor list length, [1] error bits, [2] unused, [3] most bitset
-    // survivors of any staged work item that overflowed its region since the last clear
+    // survivors of any wave of the staged match that overflowed its region since the last clear
     Buffer rows, masks, rho, counters, list, status, cand, items;
-    uint64_t cand_cap = 512;                            // bitset-survivor entries per staged work item
+    uint64_t cand_cap = 512;                            // bitset-survivor entries per wave of the staged match
     // start, bitsets built, match done, intersection kernel start and end, download done
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     Pending pending;       // the batch started by pcube_match_start
@@ -780,15 +780,17 @@ int staged_enqueue(Matcher& mt, MatchArgs m, hipEvent_t ev_pre, hipEvent_t ev_ma
     m.rho = mt.rho.as<uint16_t>();
     m.cand_max = mt.status.as<uint64_t>() + 3;
     m.item_cap = (uint32_t)mt.cand_cap;
-    // the work list of (piece, chunk) items, its survivor regions and their counts
+    // the work list of (piece, chunk) items and each piece's reach
     const uint32_t npieces = (m.nv + kPieceRows - 1) / kPieceRows;
     const uint32_t items_cap = npieces * m.nstage;   // staged_fits bounds it
-    // items, their count, each item's survivor count, each piece's reach
-    TRY(mt.items.ensure(((uint64_t)items_cap + 1) * sizeof(uint2) + (uint64_t)items_cap * 4 + (uint64_t)npieces * 8));
+    TRY(mt.items.ensure(((uint64_t)items_cap + 1) * sizeof(uint2) + (uint64_t)npieces * 8));
     uint2* items = mt.items.as<uint2>();
     uint32_t* nitems = (uint32_t*)(items + items_cap);
-    m.item_count = nitems + 2;
-    TRY(mt.cand.ensure(std::max<uint64_t>(4, (uint64_t)items_cap * mt.cand_cap * sizeof(uint32_t))));
+    // kStagedBlocksPerCU persistent blocks per CU, no more than there are work units
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>((uint64_t)kStagedBlocksPerCU * g.compute_units,
+                                                         (uint64_t)items_cap * kUnitsPerItem);
+    // each wave's survivor region: cand_cap (row, u index) pairs
+    TRY(mt.cand.ensure(std::max<uint64_t>(8, (uint64_t)blocks * kStagedWaves * mt.cand_cap * sizeof(uint2))));
     m.cand = mt.cand.as<uint32_t>();
     // one launch: the item plan, the rows, the bitsets and the residue cache
     PrefilterArgs pa;
@@ -802,7 +804,7 @@ int staged_enqueue(Matcher& mt, MatchArgs m, hipEvent_t ev_pre, hipEvent_t ev_ma
     for (int k = 0; k < kStagedFilters; ++k) pa.chunk_base[k] = mt.s_chunk_base[k];
     pa.items = items;
     pa.nitems = nitems;
-    pa.reach = m.item_count + items_cap;
+    pa.reach = nitems + 2;
     uint32_t dir_stride, dir_entries;
     directory_shape(m.nu, dir_stride, dir_entries);
     hipLaunchKernelGGL(k_prefilter_staged, dim3(1 + pa.rows_blocks + pa.mask_blocks + pa.rho_blocks), dim3(256),
@@ -812,13 +814,12 @@ int staged_enqueue(Matcher& mt, MatchArgs m, hipEvent_t ev_pre, hipEvent_t ev_ma
     // them and stands for its own timing events
     mt.kernel_span = ev_pre && ev_match;
     if (m.nu) {
-        // kStagedBlocksPerCU persistent blocks per CU over the work list and the residual chain
+        // persistent blocks over the work units and the residual chain
         const uint2* items_arg = items;
         const uint32_t* nitems_arg = nitems;
         void* args[] = {&m, &items_arg, (void*)&items_cap, &nitems_arg};
         if (ev_k0 && !mt.kernel_span) HIP_TRY(hipEventRecord(ev_k0, g.stream));
-        HIP_TRY(hipLaunchKernel(staged_kernel(mt.sbits, mt.nres),
-                                dim3((uint32_t)(kStagedBlocksPerCU * g.compute_units)), dim3(kStagedThreads), args, 0,
+        HIP_TRY(hipLaunchKernel(staged_kernel(mt.sbits, mt.nres), dim3(blocks), dim3(kStagedThreads), args, 0,
                                 g.stream));
         if (ev_k1 && !mt.kernel_span) HIP_TRY(hipEventRecord(ev_k1, g.stream));
     }
@@ -836,9 +837,9 @@ bool staged_fits(const Matcher& mt, uint64_t nu, uint64_t nv) {
 
 // Internal engine code: the residue-bitset engine without LDS staging.
 constexpr int kUnstagedBitsets = 2;
-// Bitset survivors one staged work item may hold before the unstaged engine is
-// used instead (an item covers kPieceRows = 1024 v rows x 256 u coordinates).
-constexpr uint64_t kMaxItemCandidates = 1ull << 16;
+// Bitset survivors one wave of the staged match may hold before the unstaged
+// engine is used instead (a wave's region is item_cap pairs; 4096 waves at most).
+constexpr uint64_t kMaxWaveCandidates = 1ull << 13;
 
 // Enqueue the kernels of one batch; counters must be zeroed by the caller.
 int match_enqueue(Matcher& mt, const uint64_t* us, uint64_t nu, const uint64_t* vs, uint64_t nv, uint64_t v_first,
@@ -1871,7 +1872,7 @@ int pcube_match_finish(int matcher, uint64_t* counters, uint64_t* values, uint64
     TRY(event_wait(mt->ev[5]));
     if (hd[3] > mt->cand_cap) {
         // a bitset-survivor region overflowed: size the regions from its count and run again
-        if (hd[3] > kMaxItemCandidates) pd.engine = kUnstagedBitsets;
+        if (hd[3] > kMaxWaveCandidates) pd.engine = kUnstagedBitsets;
         else mt->cand_cap = (hd[3] + 63) / 64 * 64;
         TRY(match_launch(*mt));
         TRY(event_wait(mt->ev[5]));
@@ -2067,7 +2068,7 @@ int pcube_lease(int matcher, int focus_u, int focus_v, uint32_t nbatches, const 
         HIP_TRY(hipEventRecord(ev[3], g.stream));
         TRY(stream_wait());
         if (geometry_rc || st[3] <= mt->cand_cap) break;
-        if (st[3] > kMaxItemCandidates) engine = kUnstagedBitsets;
+        if (st[3] > kMaxWaveCandidates) engine = kUnstagedBitsets;
         else mt->cand_cap = (st[3] + 63) / 64 * 64;
     }
     const uint32_t err = (uint32_t)st[1];

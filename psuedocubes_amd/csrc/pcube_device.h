@@ -330,9 +330,8 @@ struct MatchArgs {
     uint32_t staged, stage_lines, nstage;
     uint32_t nbeta[kStagedFilters];    // (-B * A^-1) mod q per bitset filter
     uint32_t pstr_off[kStagedFilters];
-    uint32_t* cand;            // bitset survivors, item_cap entries per work item of the staged engine
-    uint32_t* item_count;      // survivors of each work item (entries beyond item_cap are dropped)
-    uint64_t* cand_max;        // largest item_count above item_cap seen since the last clear
+    uint32_t* cand;            // bitset survivors as (row, u index) pairs, item_cap per wave of the staged match
+    uint64_t* cand_max;        // largest wave survivor count above item_cap seen since the last clear
     uint32_t item_cap;
 };
 

@@ -18,17 +18,18 @@
 // column offsets in the order its lane reads them (the row's lane is fixed by
 // its index).
 //
-// Work item = (stage chunk s, piece of kPieceRows consecutive v rows).  Two
-// persistent blocks per CU walk lists of items; for each, a block intersects the
-// chunk, staged in its LDS, with every row of the piece: a thread takes whole
-// rows and ANDs, for each of the chunk's words, the kStagedFilters words of its
-// row's residue classes.  Each block holds one chunk image: the next item's
-// chunk streams in (global_load_lds) once every wave has finished with the
-// current one, while the block writes the item's survivors and counts, and the
-// other block of the CU keeps its SIMDs busy meanwhile.  Positions
-// surviving all bitset filters (about 1 in 6561 at p = 619) are written to the
-// item's region of a candidate list, and the block runs them through the rest
-// of the filter chain after its items, exactly as in the residue-bitset engine: cached
+// Work item = (stage chunk s, piece of kPieceRows consecutive v rows), planned
+// chunk-major; work unit = 64 rows of an item's piece (one row per lane).  Two
+// persistent blocks per CU each take a contiguous run of units.  A block stages
+// the chunk image of its run's current segment (the units of one chunk) in its
+// LDS, and its waves take the segment's units in turn: a lane takes one row and
+// ANDs, for each of the chunk's words, the kStagedFilters words of its row's
+// residue classes.  Waves run independently within a segment; the block
+// synchronises only when the chunk changes (then the next image streams in with
+// global_load_lds while the other block of the CU keeps its SIMDs busy).
+// Positions surviving all bitset filters (about 1 in 6561 at p = 619) go to the
+// wave's own survivor list, and the block runs them through the rest of the
+// filter chain after its run, exactly as in the residue-bitset engine: cached
 // residues (u*B mod q) + sigma_v for the next kCachedResidues filters, then
 // exact 128-bit x = u*B - v*A reduced limb by limb.
 //
@@ -47,7 +48,6 @@ constexpr int kPieceRows = 1024;      // v rows per work item
 constexpr int kStagedThreads = 512;   // threads of a persistent block
 constexpr int kStagedBlocksPerCU = 2; // persistent blocks per CU (4 waves per SIMD, 128 VGPRs)
 constexpr int kStagedWaves = kStagedThreads / kWave;
-constexpr int kRowsPerLane = kPieceRows / kStagedThreads;   // rows of a piece each thread takes
 constexpr int kTileSlots = 64;        // subtiles whose group counts a block keeps in LDS
 constexpr int kItemGroups = 2;        // groups counted by the intersection itself (popcounts)
 constexpr int kStageLines = 280;      // lines of a chunk image: the largest staged q
@@ -123,21 +123,21 @@ __device__ __forceinline__ void stage_copy(uint4* __restrict__ dst, const uint4*
 }
 
 // The LDS a staged block's helpers use (static arrays of k_match_staged).
-// The filter tables are set only for the residual chain after the items.
+// The filter tables are set only for the residual chain after the units.
 struct Staged {
     const uint32_t* allow;
     const Modulus* mods;
-    uint32_t* tilecnt;       // [kTileSlots][stride]: groups 1..stride of subtiles t0..t0+63
-    uint32_t stride;         // groups per subtile: kItemGroups for the items, ng for the chain
-    uint32_t* queue;         // this wave's survivors, (row offset << 8) | u offset within the item
+    uint32_t* tilecnt;       // [kTileSlots][stride]: groups 1..stride of subtiles 0..kTileSlots-1
+    uint32_t stride;         // groups per subtile: kItemGroups for the units, ng for the chain
+    uint2* queue;            // this wave's survivors: (row index j, u index i)
 };
 
-// Add n to group g (1-based) of a subtile: LDS when the subtile is within the
-// block's window, else the global counters.
-__device__ __forceinline__ void staged_add(const MatchArgs& m, const Staged& S, uint32_t t0, uint32_t tile, uint32_t g,
+// Add n to group g (1-based) of a subtile: LDS for the first kTileSlots
+// subtiles of the batch (all of them for a batch of at most 64), else the
+// global counters.
+__device__ __forceinline__ void staged_add(const MatchArgs& m, const Staged& S, uint32_t tile, uint32_t g,
                                            uint32_t n) {
-    const uint32_t slot = tile - t0;
-    if (slot < (uint32_t)kTileSlots) atomicAdd(&S.tilecnt[slot * S.stride + g - 1], n);
+    if (tile < (uint32_t)kTileSlots) atomicAdd(&S.tilecnt[tile * S.stride + g - 1], n);
     else atomicAdd(&m.counters[(uint64_t)tile * (m.ng + 3) + g], (uint64_t)n);
 }
 
@@ -161,7 +161,7 @@ __device__ __forceinline__ void staged_chain(const MatchArgs& m, const Staged& S
             r = r >= mod.q ? r - mod.q : r;
             alive = allowed_bit(S.allow, mod.bit + r);
         }
-        if (((k & 3) == 3 || k + 1 == m.nres) && alive) staged_add(m, S, 0, row.tile, k / 4 + 1, 1);
+        if (((k & 3) == 3 || k + 1 == m.nres) && alive) staged_add(m, S, row.tile, k / 4 + 1, 1);
         if (!__ballot(alive)) return;
         ++k;
     }
@@ -170,7 +170,7 @@ __device__ __forceinline__ void staged_chain(const MatchArgs& m, const Staged& S
     for (; k < m.nres; ++k) {
         const Modulus mod = S.mods[k];
         if (alive) alive = allowed_bit(S.allow, mod.bit + mod128(x, mod));
-        if (((k & 3) == 3 || k + 1 == m.nres) && alive) staged_add(m, S, 0, row.tile, k / 4 + 1, 1);
+        if (((k & 3) == 3 || k + 1 == m.nres) && alive) staged_add(m, S, row.tile, k / 4 + 1, 1);
         if (!__ballot(alive)) return;
     }
     if (alive) {
@@ -179,22 +179,14 @@ __device__ __forceinline__ void staged_chain(const MatchArgs& m, const Staged& S
     }
 }
 
-// Append count (<= 64) queued survivors to the work item's region of the
-// candidate list; the item's fill counter lives in LDS.
-__device__ __forceinline__ void cand_flush(const Staged& S, uint32_t lane, uint32_t count, uint32_t* region,
-                                           uint32_t cap, uint32_t* fill) {
-    uint32_t at = 0;
-    if (lane == 0) at = atomicAdd(fill, count);
-    at = __shfl(at, 0, kWave);
-    if (lane < count && at + lane < cap) region[at + lane] = S.queue[lane];
-}
-
-// Append the set bits of a lane's four survivor words (word i at u offset
-// first[i] of the chunk) to the wave's queue as (row offset << 8) | u offset,
-// one bit per lane per round; 64 queued entries go to the item's region.
+// Append the set bits of a lane's four survivor words (word w at chunk offset
+// first[w], the chunk's first u index base) to the wave's queue as (row j,
+// u index), one bit per lane per round.  Whenever 64 are queued they go to the
+// wave's own region of the survivor list (fill entries written so far; entries
+// past cap are counted, not written).
 __device__ __forceinline__ void staged_append(const Staged& S, uint32_t lane, uint64_t (&b)[4], const int (&first)[4],
-                                              uint32_t tag, uint32_t& queued, uint32_t* region, uint32_t cap,
-                                              uint32_t* fill) {
+                                              uint32_t j, uint32_t base, uint32_t& queued, uint2* region,
+                                              uint32_t cap, uint32_t& fill) {
     for (;;) {
         const bool has = (b[0] | b[1] | b[2] | b[3]) != 0;
         const uint64_t ballot = __ballot(has);
@@ -202,7 +194,7 @@ __device__ __forceinline__ void staged_append(const Staged& S, uint32_t lane, ui
         if (has) {
             const int w = b[0] ? 0 : b[1] ? 1 : b[2] ? 2 : 3;
             const uint64_t x = w == 0 ? b[0] : w == 1 ? b[1] : w == 2 ? b[2] : b[3];
-            S.queue[queued + lanes_below(ballot)] = tag + (uint32_t)(first[w] + __builtin_ctzll(x));
+            S.queue[queued + lanes_below(ballot)] = make_uint2(j, base + (uint32_t)(first[w] + __builtin_ctzll(x)));
             const uint64_t rest = x & (x - 1);
             b[0] = w == 0 ? rest : b[0];
             b[1] = w == 1 ? rest : b[1];
@@ -212,10 +204,11 @@ __device__ __forceinline__ void staged_append(const Staged& S, uint32_t lane, ui
         queued += __popcll(ballot);
         if (queued >= kWave) {
             wave_sync();
-            cand_flush(S, lane, kWave, region, cap, fill);
+            if (fill + lane < cap) region[fill + lane] = S.queue[lane];
+            fill += kWave;
             wave_sync();
             const uint32_t left = queued - kWave;
-            uint32_t moved = 0;
+            uint2 moved = make_uint2(0, 0);
             if (lane < left) moved = S.queue[kWave + lane];
             wave_sync();
             if (lane < left) S.queue[lane] = moved;
@@ -351,21 +344,21 @@ __global__ __launch_bounds__(256) void k_prefilter_staged(MatchArgs m, Prefilter
     if (t < (uint64_t)m.nu * kCachedResidues) build_rho(m, t);
 }
 
-// Row headers of one work item's rows for this thread: (left, right, tile,
-// flags) and the offsets (8 * line + column) of the mask rows it reads.
-struct ItemRows {
-    uint4 h0[kRowsPerLane], h1[kRowsPerLane];
+// Row header of one row of a work unit: (left, right, tile, flags) and the
+// offsets (8 * line + column) of the mask rows it reads.
+struct UnitRow {
+    uint4 h0, h1;
 };
 
-__device__ __forceinline__ void load_item_rows(const MatchArgs& m, uint32_t p, ItemRows& R) {
+// Unit s of piece p is rows j0 + 64 s .. j0 + 64 s + 63 of the piece (j0 =
+// 1024 p), lane l taking row j0 + 64 s + l; lanes past the piece's end read
+// its last row (their results are discarded).
+__device__ __forceinline__ void load_unit_row(const MatchArgs& m, uint32_t p, uint32_t s, uint32_t lane, UnitRow& R) {
     const uint32_t j0 = p * kPieceRows, j1 = min(m.nv, j0 + kPieceRows);
-#pragma unroll
-    for (int t = 0; t < kRowsPerLane; ++t) {
-        const uint32_t j = j0 + threadIdx.x + kStagedThreads * t;
-        const uint4* rp = (const uint4*)&m.rows[j < j1 ? j : j1 - 1];
-        R.h0[t] = rp[0];
-        R.h1[t] = rp[1];
-    }
+    const uint32_t j = j0 + s * kWave + lane;
+    const uint4* rp = (const uint4*)&m.rows[j < j1 ? j : j1 - 1];
+    R.h0 = rp[0];
+    R.h1 = rp[1];
 }
 
 // Start the copy of stage chunk s into an LDS buffer: each wave moves 64
@@ -382,43 +375,55 @@ __device__ __forceinline__ void stage_async(const MatchArgs& m, uint32_t s, uint
 }
 
 // Static LDS of the staged match (two blocks per CU): one chunk image, the
-// per-subtile counts of groups 1 and 2, the survivor queues and the item's fill
-// counter.  The chain after the items keeps the filter tables and the per-subtile
-// counts of every group in the image buffer.
+// per-subtile counts of groups 1 and 2 and the waves' survivor queues.  The
+// chain after the units keeps the filter tables and the per-subtile counts of
+// every group in the image buffer.
 constexpr int kStageBufferUnits = kStageLines * kStageLineBytes / 16;
 constexpr uint32_t kStagedLdsBytes =
-    kStageBufferUnits * 16 + kTileSlots * kItemGroups * 4 + kStagedWaves * kQueue * 4 + 64;
+    kStageBufferUnits * 16 + kTileSlots * kItemGroups * 4 + kStagedWaves * kQueue * (uint32_t)sizeof(uint2);
 static_assert(kStagedBlocksPerCU * kStagedLdsBytes <= 160 * 1024, "two staged blocks share one CU's LDS");
 static_assert(kMaxAllowWords * 4 + kMaxResiduals * sizeof(Modulus) + kTileSlots * (kMaxResiduals / 4) * 4 <=
                   kStageBufferUnits * 16,
               "the chain's filter tables and group counts fit the mask buffer");
-static_assert(kPieceRows <= (1 << 24) && kStageWords * 64 <= 256, "survivor tags hold row and u offsets");
 
-// A block's walk over its work items: logical position k is item
-// first + k * stride (the staged match walks one contiguous run, stride 1).  Each lane holds
-// one entry of a 64-item window of the block's sequence (desc = its item at
-// logical position window + lane), so a descriptor is a register read.
-struct ItemState {
-    uint32_t k, count;       // logical position of the current item, length of the sequence
-    uint32_t first, stride;
-    uint32_t window;
+// Work units: unit x is rows 64 (x mod 16) .. 64 (x mod 16) + 63 of the piece of
+// item x / 16 against that item's chunk, so a block's contiguous run of units
+// follows the chunk-major item list.
+constexpr int kUnitsPerItem = kPieceRows / kWave;
+
+// A wave's view of the item list: lane l holds the descriptor (piece, chunk) of
+// item window + l, so a descriptor is a register read.
+struct ItemWindow {
+    uint32_t window, count;
     uint2 desc;
-    uint2 it;                // the current item (piece, chunk)
-    ItemRows rows;           // row headers of the current item's piece
 };
 
-__device__ __forceinline__ uint32_t item_index(const ItemState& st, uint32_t k) { return st.first + k * st.stride; }
-
-__device__ __forceinline__ void load_window(const uint2* items, ItemState& st, uint32_t k0, uint32_t lane) {
-    st.window = k0;
-    st.desc = items[item_index(st, min(k0 + lane, st.count - 1))];
+__device__ __forceinline__ void load_window(const uint2* items, ItemWindow& w, uint32_t k0, uint32_t lane) {
+    w.window = k0;
+    w.desc = items[min(k0 + lane, w.count - 1)];
 }
 
-// Descriptor of logical item k (k >= the window's first position).
-__device__ __forceinline__ uint2 item_at(const uint2* items, ItemState& st, uint32_t k, uint32_t lane) {
-    if (k - st.window >= (uint32_t)kWave) load_window(items, st, k, lane);
-    const int at = (int)(k - st.window);
-    return make_uint2(__builtin_amdgcn_readlane(st.desc.x, at), __builtin_amdgcn_readlane(st.desc.y, at));
+// Descriptor of item k (k < count).
+__device__ __forceinline__ uint2 item_at(const uint2* items, ItemWindow& w, uint32_t k, uint32_t lane) {
+    if (k - w.window >= (uint32_t)kWave) load_window(items, w, k, lane);
+    const int at = (int)(k - w.window);
+    return make_uint2(__builtin_amdgcn_readlane(w.desc.x, at), __builtin_amdgcn_readlane(w.desc.y, at));
+}
+
+// The end of the segment of the run that starts at unit u: the first unit of
+// an item after u's whose chunk differs from `chunk`, or the run's end u_last.
+__device__ __forceinline__ uint32_t segment_end(const uint2* items, ItemWindow& w, uint32_t u, uint32_t u_last,
+                                                uint32_t chunk, uint32_t lane) {
+    uint32_t k = u / kUnitsPerItem;
+    const uint32_t k_last = (u_last - 1) / kUnitsPerItem;
+    for (;;) {
+        if (k - w.window >= (uint32_t)kWave) load_window(items, w, k, lane);
+        const uint32_t idx = w.window + lane;
+        const uint64_t differ = __ballot(idx >= k && idx <= k_last && w.desc.y != chunk);
+        if (differ) return min(u_last, (w.window + (uint32_t)__builtin_ctzll(differ)) * kUnitsPerItem);
+        k = w.window + kWave;
+        if (k > k_last) return u_last;
+    }
 }
 
 // The two 64-bit words of a 16-byte mask half.
@@ -443,22 +448,23 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
     return x;
 }
 
-// Group 1 and 2 counts of the lanes that have them (has) for their subtile:
-// one add per group when the wave's lanes share the subtile, else one per lane.
-__device__ __forceinline__ void flush_groups(const MatchArgs& m, const Staged& S, uint32_t t0, bool has,
-                                             uint32_t tile, uint32_t c1, uint32_t c2, uint32_t lane) {
+// Group 1 and 2 counts of a unit's rows for their subtiles: one add per group
+// when the wave's counting rows share a subtile (the two counts, each at most
+// 256 per row, summed packed in one 32-bit word), else one per row.
+__device__ __forceinline__ void flush_groups(const MatchArgs& m, const Staged& S, bool has, uint32_t tile,
+                                             uint32_t c1, uint32_t c2, uint32_t lane) {
     const uint64_t active = __ballot(has && (c1 | c2));
     if (!active) return;
     const uint32_t lead = __shfl(tile, __builtin_ctzll(active), kWave);
     if (!__ballot(has && (c1 | c2) && tile != lead)) {
-        const uint32_t s1 = wave_sum(has ? c1 : 0u), s2 = wave_sum(has ? c2 : 0u);
+        const uint32_t s = wave_sum(has ? c1 | (c2 << 16) : 0u);
         if (lane == 0) {
-            if (s1) staged_add(m, S, t0, lead, 1, s1);
-            if (s2) staged_add(m, S, t0, lead, 2, s2);
+            if (s & 0xffffu) staged_add(m, S, lead, 1, s & 0xffffu);
+            if (s >> 16) staged_add(m, S, lead, 2, s >> 16);
         }
     } else if (has) {
-        if (c1) staged_add(m, S, t0, tile, 1, c1);
-        if (c2) staged_add(m, S, t0, tile, 2, c2);
+        if (c1) staged_add(m, S, tile, 1, c1);
+        if (c2) staged_add(m, S, tile, 2, c2);
     }
 }
 
@@ -519,159 +525,129 @@ __device__ __forceinline__ void staged_row(const uint8_t* masks_p, const uint8_t
     words[3] = both[1][1];
 }
 
-// Match one work item reading chunk buffer `cur` while the next item's chunk
-// (and, when the piece changes, its row headers) stream in.  Returns false
-// when the block has no further item.
-//
-// Match one work item on the block's chunk image: each thread takes its rows of
-// the piece through staged_row (a wave whose rows all miss the chunk skips it,
-// one whose rows all cover it whole skips the range masks).  Once every wave is
-// done with the image, the next item's chunk starts streaming into it while the
-// block writes this item's survivors and group counts; the next piece's row
-// headers, when the piece changes, are in flight during the whole item.
-// Returns false when the block has no further item.
+// One work unit: the rows of unit s of piece p (row j0 + 64 s + lane for this
+// lane) against the staged image of chunk `chunk`, through staged_row (skipped
+// when no row of the wave reaches the chunk; without range masks when every row
+// covers it whole).  Survivors go to the wave's queue, the group 1 and 2 counts
+// to the block's LDS counts.
 template <int kF, bool kLast>
-__device__ __forceinline__ bool staged_item(const MatchArgs& m, const uint2* items, const Staged& S, ItemState& st,
-                                            uint4 (&image)[kStageBufferUnits], uint32_t* fills,
-                                            uint32_t* wave_counts, uint32_t* piece_t0, uint32_t lane,
-                                            uint32_t wave) {
-    static_assert(kRowsPerLane == 2, "a thread's rows of a piece are counted as two records");
-    const uint32_t p = st.it.x, s = st.it.y;
+__device__ __forceinline__ void staged_unit(const MatchArgs& m, const Staged& S, uint32_t p, uint32_t s,
+                                            uint32_t chunk, const UnitRow& R, const uint4* image, uint32_t lane,
+                                            uint32_t& queued, uint2* region, uint32_t& fill) {
     const uint32_t j0 = p * kPieceRows, j1 = min(m.nv, j0 + kPieceRows);
-    const uint32_t item = item_index(st, st.k);
-    // survivors flushed during the item; the counter alternates with the item's
-    // parity, so resetting it here cannot race with the last item's readers
-    uint32_t* fill = fills + (st.k & 1);
-    __builtin_amdgcn_s_waitcnt(0);   // this wave's part of the chunk and its row headers have arrived
-    if (threadIdx.x == 0) {
-        *fill = 0;
-        *piece_t0 = st.rows.h0[0].z;   // thread 0's first row is the piece's first
-    }
-    uint32_t* region = m.cand + (uint64_t)item * m.item_cap;
-    __syncthreads();
-    const uint32_t t0 = *piece_t0;
-    // the next piece's rows are in flight while this item is matched
-    const uint32_t next = st.k + 1;
-    const ItemRows cur_rows = st.rows;
-    uint2 nit = st.it;
-    if (next < st.count) {
-        nit = item_at(items, st, next, lane);
-        if (nit.x != p) load_item_rows(m, nit.x, st.rows);
-    }
+    const uint32_t jr = j0 + s * kWave;
+    if (jr >= j1) return;   // the piece ends before this unit
+    const uint32_t j = jr + lane;
+    const bool live = j < j1;
+    constexpr int kChunkU = kStageWords * 64;
+    const uint32_t base0 = chunk * kChunkU;   // first u index of the chunk
+    // range of the row's u indices within the chunk
+    const int dl = live ? (int)(R.h0.x - base0) : 0, dr = live ? (int)(R.h0.y - base0) : 0;
+    const bool touches = live && dr > 0 && dl < kChunkU;
+    const bool full = touches && dl <= 0 && dr >= kChunkU;
+    if (!__ballot(touches)) return;
     const uint32_t g = lane & 15, hp = (g >> 2) & 1;
     const uint8_t* masks_p = (const uint8_t*)image + 16 * hp;          // half P of a column
     const uint8_t* masks_q = (const uint8_t*)image + 16 * (hp ^ 1);    // half Q
     const int first_p = 128 * (int)hp, first_q = 128 - first_p;        // chunk offset of each half's first word
-    const uint32_t base0 = (s * kStageWords) << 6;                     // first u index of the chunk
-    constexpr int kChunkU = kStageWords * 64;
-    uint32_t c1[kRowsPerLane] = {}, c2[kRowsPerLane] = {}, tiles[kRowsPerLane] = {};
-    bool has[kRowsPerLane] = {};
-    uint32_t queued = 0;
-#pragma unroll
-    for (int t = 0; t < kRowsPerLane; ++t) {
-        const uint32_t j = j0 + threadIdx.x + kStagedThreads * t;
-        if (j0 + (threadIdx.x - lane) + kStagedThreads * t < j1) {   // the wave has rows in the piece
-            const bool live = j < j1;
-            const uint4 h0 = cur_rows.h0[t], h1 = cur_rows.h1[t];
-            has[t] = live;
-            tiles[t] = h0.z;
-            // range of the row's u indices within the chunk
-            const int dl = live ? (int)(h0.x - base0) : 0, dr = live ? (int)(h0.y - base0) : 0;
-            const bool touches = live && dr > 0 && dl < kChunkU;
-            const bool full = touches && dl <= 0 && dr >= kChunkU;
-            // no row of the wave reaches the chunk: nothing to read or count
-            if (!__ballot(touches)) continue;
-            uint64_t words[4];
-            // every row of the wave covers the whole chunk (most items): no range masks
-            if (!__ballot(!full))
-                staged_row<kF, kLast, false>(masks_p, masks_q, first_p, first_q, g, h1, dl, dr, c1[t], c2[t], words);
-            else
-                staged_row<kF, kLast, true>(masks_p, masks_q, first_p, first_q, g, h1, dl, dr, c1[t], c2[t], words);
-            const int firsts[4] = {first_p, first_p + 64, first_q, first_q + 64};
-            staged_append(S, lane, words, firsts, (j - j0) << 8, queued, region, m.item_cap, fill);
-        }
-    }
-    if (lane == 0) wave_counts[wave] = queued;
-    // a thread's two rows usually share a subtile
-    if (has[1] && (!has[0] || tiles[1] == tiles[0])) {
-        c1[0] += c1[1];
-        c2[0] += c2[1];
-        tiles[0] = tiles[1];
-        has[0] = true;
-        has[1] = false;
-    }
-    flush_groups(m, S, t0, has[0], tiles[0], c1[0], c2[0], lane);
-    flush_groups(m, S, t0, has[1], tiles[1], c1[1], c2[1], lane);
-    __syncthreads();   // every wave is done with the image and has queued its survivors
-    // the next item of another chunk streams its image in; the same chunk keeps it
-    if (next < st.count && nit.y != s) stage_async(m, nit.y, (uint8_t*)image, wave, lane);
-    // the waves' queued survivors follow the flushed ones, in wave order
-    {
-        const uint32_t n = lane < (uint32_t)kStagedWaves ? wave_counts[lane] : 0u;
-        const uint32_t below = wave_sum(lane < wave ? n : 0u), queued_all = wave_sum(n);
-        const uint32_t at = *fill + below;
-        if (lane < queued && at + lane < m.item_cap) region[at + lane] = S.queue[lane];
-        if (threadIdx.x == 0) m.item_count[item] = *fill + queued_all;
-    }
-    for (uint32_t t = threadIdx.x; t < (uint32_t)(kTileSlots * kItemGroups); t += kStagedThreads) {
-        const uint32_t n = S.tilecnt[t];
-        if (n) {
-            atomicAdd(&m.counters[(uint64_t)(t0 + t / kItemGroups) * (m.ng + 3) + t % kItemGroups + 1], (uint64_t)n);
-            S.tilecnt[t] = 0;
-        }
-    }
-    if (next >= st.count) return false;
-    st.k = next;
-    st.it = nit;
-    return true;
+    uint32_t c1 = 0, c2 = 0;
+    uint64_t words[4];
+    if (!__ballot(!full))
+        staged_row<kF, kLast, false>(masks_p, masks_q, first_p, first_q, g, R.h1, dl, dr, c1, c2, words);
+    else
+        staged_row<kF, kLast, true>(masks_p, masks_q, first_p, first_q, g, R.h1, dl, dr, c1, c2, words);
+    const int firsts[4] = {first_p, first_p + 64, first_q, first_q + 64};
+    staged_append(S, lane, words, firsts, j, base0, queued, region, m.item_cap, fill);
+    flush_groups(m, S, touches, R.h0.z, c1, c2, lane);
 }
 
-// Persistent staged match, kStagedBlocksPerCU blocks per CU, each over one
-// contiguous run of the chunk-major work list.  For an item (piece p, chunk s) thread l intersects the
-// staged chunk with rows j0 + l + 512 t of the piece.  Group 1 and 2 counts of
-// the piece's subtiles collect in LDS and go to the global counters at the end
-// of each item.  After its last item the block runs its items' survivors
-// through the residual chain, with the filter tables and the counts of the
-// later groups in the image buffer.  kF = bitset filters; kLast: the last
-// residual filter is the last bitset filter (it closes a group).  The waves-per-EU
-// bound keeps the kernel within 128 VGPRs, so both blocks of a CU stay resident.
+// Persistent staged match, kStagedBlocksPerCU blocks per CU.  The work units
+// (16 per item of the chunk-major list) are split into one contiguous run per
+// block; the runs of one XCD's blocks are adjacent, so an L2 holds the chunk
+// images its blocks stage.  A run is walked in segments of one chunk: the
+// block stages the chunk's image once and its waves take the segment's units in
+// turn (wave w: units w, w + 8, ... of the segment), each wave on its own,
+// with the next unit's row headers in flight; the block synchronises only when
+// the chunk changes.  Group 1 and 2 counts collect in LDS for the whole run;
+// each wave keeps its survivors in its LDS queue and, 64 at a time, in its own
+// region of the survivor list (item_cap entries).  After the run the block
+// runs its survivors through the residual chain, with the filter tables and the
+// counts of the later groups in the image buffer.  kF = bitset filters; kLast:
+// the last residual filter is the last bitset filter (it closes a group).  The
+// waves-per-EU bound keeps the kernel within 128 VGPRs, so both blocks of a CU
+// stay resident.
 template <int kF, bool kLast>
 __global__ __launch_bounds__(kStagedThreads) __attribute__((amdgpu_waves_per_eu(4)))
 void k_match_staged(MatchArgs m, const uint2* items, uint32_t items_cap, const uint32_t* nitems) {
     __shared__ uint4 image[kStageBufferUnits];
     __shared__ uint32_t tilecnt[kTileSlots * kItemGroups];
-    __shared__ uint32_t queues[kStagedWaves * kQueue];
-    __shared__ uint32_t fills[2];                     // survivors flushed during an item, by item parity
-    __shared__ uint32_t wave_counts[kStagedWaves];    // survivors each wave holds at an item's end
-    __shared__ uint32_t piece_t0;                     // subtile of the current piece's first row
+    __shared__ uint2 queues[kStagedWaves * kQueue];
     const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    const uint32_t total = min(*nitems, items_cap);
-    // each block takes one contiguous run of the chunk-major list, so consecutive
-    // items mostly share their chunk image; consecutive block ids go round robin
-    // over the 8 XCDs, so the runs are grouped by XCD, and each XCD's blocks cover
-    // one stretch of the list (its chunk images and row headers in that XCD's L2)
+    const uint32_t total_items = min(*nitems, items_cap);
+    const uint32_t total = total_items * kUnitsPerItem;
+    // consecutive block ids go round robin over the 8 XCDs: group the runs by XCD
     const uint32_t grid = gridDim.x;
     const uint32_t run = grid % 8 == 0 ? (blockIdx.x % 8) * (grid / 8) + blockIdx.x / 8 : blockIdx.x;
-    const uint32_t first = (uint32_t)((uint64_t)run * total / grid);
-    const uint32_t last = (uint32_t)((uint64_t)(run + 1) * total / grid);
-    if (first >= last) return;
-    ItemState st;
-    st.first = first;
-    st.stride = 1;
-    st.count = last - first;
-    st.k = 0;
+    const uint32_t u_first = (uint32_t)((uint64_t)run * total / grid);
+    const uint32_t u_last = (uint32_t)((uint64_t)(run + 1) * total / grid);
+    if (u_first >= u_last) return;
+    ItemWindow iw;
+    iw.count = total_items;
+    load_window(items, iw, u_first / kUnitsPerItem, lane);
+    uint32_t chunk = item_at(items, iw, u_first / kUnitsPerItem, lane).y;
+    stage_async(m, chunk, (uint8_t*)image, wave, lane);
     for (uint32_t t = threadIdx.x; t < (uint32_t)(kTileSlots * kItemGroups); t += kStagedThreads) tilecnt[t] = 0;
     const Staged S{nullptr, nullptr, tilecnt, kItemGroups, queues + wave * kQueue};
-    load_window(items, st, 0, lane);
-    st.it = item_at(items, st, 0, lane);
-    load_item_rows(m, st.it.x, st.rows);
-    stage_async(m, st.it.y, (uint8_t*)image, wave, lane);
-    while (staged_item<kF, kLast>(m, items, S, st, image, fills, wave_counts, &piece_t0, lane, wave)) {
+    uint2* region = (uint2*)m.cand + ((uint64_t)blockIdx.x * kStagedWaves + wave) * m.item_cap;
+    uint32_t queued = 0, fill = 0;
+    uint32_t seg_end = segment_end(items, iw, u_first, u_last, chunk, lane);
+    uint32_t x = u_first + wave;   // this wave's next unit
+    UnitRow R;
+    uint2 it = make_uint2(0, 0);
+    if (x < seg_end) {
+        it = item_at(items, iw, x / kUnitsPerItem, lane);
+        load_unit_row(m, it.x, x % kUnitsPerItem, lane, R);
     }
-    // the residual chain over this block's items' survivors, 64 at a time, with
-    // the filter tables and the later groups' counts in the image buffer; the
-    // block's waves take the 64-survivor slices of its items in turn
-    __syncthreads();   // every item's region and count is written, no chunk is read any more
+    __builtin_amdgcn_s_waitcnt(0);   // this wave's part of the image has arrived
+    __syncthreads();
+    for (;;) {
+        while (x < seg_end) {
+            const UnitRow cur = R;
+            const uint32_t p = it.x, s = x % kUnitsPerItem;
+            x += kStagedWaves;
+            if (x < seg_end) {
+                it = item_at(items, iw, x / kUnitsPerItem, lane);
+                load_unit_row(m, it.x, x % kUnitsPerItem, lane, R);
+            }
+            staged_unit<kF, kLast>(m, S, p, s, chunk, cur, image, lane, queued, region, fill);
+        }
+        if (seg_end >= u_last) break;
+        // the next segment: its first unit's rows are in flight while the block
+        // waits for every wave to finish with the image
+        const uint32_t u = seg_end;
+        chunk = item_at(items, iw, u / kUnitsPerItem, lane).y;
+        seg_end = segment_end(items, iw, u, u_last, chunk, lane);
+        x = u + wave;
+        if (x < seg_end) {
+            it = item_at(items, iw, x / kUnitsPerItem, lane);
+            load_unit_row(m, it.x, x % kUnitsPerItem, lane, R);
+        }
+        __syncthreads();
+        stage_async(m, chunk, (uint8_t*)image, wave, lane);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+    }
+    // survivors past the region's capacity: the host sizes the regions from
+    // the largest count and runs the batch again
+    if (fill > m.item_cap && lane == 0) atomicMax((unsigned long long*)m.cand_max, (unsigned long long)(fill + queued));
+    const uint32_t listed = min(fill, m.item_cap);
+    __syncthreads();   // every wave is done with the image and its unit counts
+    for (uint32_t t = threadIdx.x; t < (uint32_t)(kTileSlots * kItemGroups); t += kStagedThreads) {
+        const uint32_t n = tilecnt[t];
+        if (n) atomicAdd(&m.counters[(uint64_t)(t / kItemGroups) * (m.ng + 3) + t % kItemGroups + 1], (uint64_t)n);
+    }
+    // the residual chain over each wave's survivors, 64 at a time, with the
+    // filter tables and the later groups' counts in the image buffer
     uint32_t* allow = (uint32_t*)image;
     Modulus* mods = (Modulus*)(allow + kMaxAllowWords);
     uint32_t* chain_counts = (uint32_t*)(mods + kMaxResiduals);
@@ -680,33 +656,15 @@ void k_match_staged(MatchArgs m, const uint2* items, uint32_t items_cap, const u
     stage_copy<kStagedThreads, 1>((uint4*)mods, (const uint4*)m.mods, m.nres * (uint32_t)sizeof(Modulus) / 16);
     for (uint32_t t = threadIdx.x; t < (uint32_t)kTileSlots * m.ng; t += kStagedThreads) chain_counts[t] = 0;
     __syncthreads();
-    const Staged C{allow, mods, chain_counts, m.ng, queues + wave * kQueue};
-    uint32_t slice = 0;
-    // the survivor counts and descriptors of up to 64 items arrive in one round
-    // trip, a lane each; the items are then walked in registers
-    for (uint32_t k0 = 0; k0 < st.count; k0 += kWave) {
-        const uint32_t mine = item_index(st, min(k0 + lane, st.count - 1));
-        const uint32_t n_lane = m.item_count[mine];
-        const uint2 it_lane = items[mine];
-        const uint32_t kn = min(st.count - k0, (uint32_t)kWave);
-        for (uint32_t k = 0; k < kn; ++k) {
-            const uint32_t item = item_index(st, k0 + k);
-            uint32_t n = __builtin_amdgcn_readlane(n_lane, (int)k);
-            if (n > m.item_cap) {
-                if (threadIdx.x == 0) atomicMax((unsigned long long*)m.cand_max, (unsigned long long)n);
-                n = m.item_cap;
-            }
-            const uint32_t piece = __builtin_amdgcn_readlane(it_lane.x, (int)k);
-            const uint32_t chunk = __builtin_amdgcn_readlane(it_lane.y, (int)k);
-            const uint32_t j0 = piece * kPieceRows, u0 = (chunk * kStageWords) << 6;
-            const uint32_t* region = m.cand + (uint64_t)item * m.item_cap;
-            for (uint32_t first = 0; first < n; first += kWave, ++slice) {
-                if (slice % kStagedWaves != wave) continue;
-                const uint32_t count = min(n - first, (uint32_t)kWave);
-                const uint32_t e = lane < count ? region[first + lane] : 0u;
-                staged_chain(m, C, lane, count, j0 + (e >> 8), u0 + (e & 0xffu));
-            }
-        }
+    const Staged C{allow, mods, chain_counts, m.ng, S.queue};
+    for (uint32_t first = 0; first < listed; first += kWave) {
+        const uint32_t count = min(listed - first, (uint32_t)kWave);
+        const uint2 e = lane < count ? region[first + lane] : make_uint2(0, 0);
+        staged_chain(m, C, lane, count, e.x, e.y);
+    }
+    if (queued) {
+        const uint2 e = lane < queued ? C.queue[lane] : make_uint2(0, 0);
+        staged_chain(m, C, lane, queued, e.x, e.y);
     }
     __syncthreads();
     for (uint32_t t = threadIdx.x; t < (uint32_t)kTileSlots * m.ng; t += kStagedThreads) {
