@@ -805,10 +805,12 @@ int staged_enqueue(Matcher& mt, MatchArgs m, hipEvent_t ev_pre, hipEvent_t ev_ma
     pa.items = items;
     pa.nitems = nitems;
     pa.reach = nitems + 2;
+    pa.reach_lds = npieces <= kPlanLdsPieces;
     uint32_t dir_stride, dir_entries;
     directory_shape(m.nu, dir_stride, dir_entries);
+    const uint32_t dynamic_lds = (std::max<uint32_t>(1, dir_entries) + (pa.reach_lds ? npieces : 0)) * 8;
     hipLaunchKernelGGL(k_prefilter_staged, dim3(1 + pa.rows_blocks + pa.mask_blocks + pa.rho_blocks), dim3(256),
-                       std::max<uint32_t>(1, dir_entries) * 8, g.stream, m, pa);
+                       dynamic_lds, g.stream, m, pa);
     if (ev_pre) HIP_TRY(hipEventRecord(ev_pre, g.stream));
     // with both of these events recorded, the match kernel alone lies between
     // them and stands for its own timing events
@@ -817,7 +819,9 @@ int staged_enqueue(Matcher& mt, MatchArgs m, hipEvent_t ev_pre, hipEvent_t ev_ma
         // persistent blocks over the work units and the residual chain
         const uint2* items_arg = items;
         const uint32_t* nitems_arg = nitems;
-        void* args[] = {&m, &items_arg, (void*)&items_cap, &nitems_arg};
+        // with two blocks on every CU, the first-round blocks take the larger share
+        uint32_t first64 = blocks == (uint32_t)(kStagedBlocksPerCU * g.compute_units) ? kFirstRoundWeight : 64u;
+        void* args[] = {&m, &items_arg, (void*)&items_cap, &nitems_arg, &first64};
         if (ev_k0 && !mt.kernel_span) HIP_TRY(hipEventRecord(ev_k0, g.stream));
         HIP_TRY(hipLaunchKernel(staged_kernel(mt.sbits, mt.nres), dim3(blocks), dim3(kStagedThreads), args, 0,
                                 g.stream));

@@ -449,8 +449,8 @@ __device__ __forceinline__ void first_above_pair(const MatchArgs& m, const U128&
 // subtile, v mod q.  With a directory (kDir) the search also yields us[left];
 // without one, us[left] is loaded.
 template <bool kDir>
-__device__ __forceinline__ void build_row(const MatchArgs& m, uint32_t j, uint64_t v, const uint64_t* dir = nullptr,
-                                          uint32_t ndir = 0, uint32_t stride = 0) {
+__device__ __forceinline__ Row build_row(const MatchArgs& m, uint32_t j, uint64_t v, const uint64_t* dir = nullptr,
+                                         uint32_t ndir = 0, uint32_t stride = 0) {
     Row row;
     row.v = v;
     row.flags = 0;
@@ -477,17 +477,32 @@ __device__ __forceinline__ void build_row(const MatchArgs& m, uint32_t j, uint64
         atomicAdd(&m.counters[(uint64_t)row.tile * (m.ng + 3) + m.ng + 2], 1ull);
     // the staged engine keeps instead, per offset slot, 8 * line + column of
     // the mask row its lane reads there: line v mod q of the slot's filter
-    // (line 0, all ones, for an absent filter)
-    for (uint32_t k = 0; k < kStagedFilters; ++k) {
-        uint32_t r;
-        if (m.staged) {
-            const uint32_t f = stage_column(k, j);
-            r = (f < m.nbits ? mod64(v, m.mods[f]) : 0u) * kStagedFilters + f;
-        } else {
-            r = k < m.nbits ? mod64(v, m.mods[k]) : 0u;
-        }
-        row.r[k] = uint16_t(r);
+    // (line 0, all ones, for an absent filter).  Slot k of row j reads column
+    // stage_column(k, j) = k ^ x with x = (g & 3) | (g >> 3) << 2, g = j mod 16:
+    // the eight entries are formed in column order (the moduli are the same
+    // for every lane) and permuted by three conditional swaps.
+    uint32_t res[kStagedFilters];
+#pragma unroll
+    for (uint32_t f = 0; f < (uint32_t)kStagedFilters; ++f) {
+        const uint32_t r = f < m.nbits ? mod64(v, m.mods[f]) : 0u;
+        res[f] = m.staged ? r * kStagedFilters + f : r;
     }
+    if (m.staged) {
+        const uint32_t g = j & 15, x = (g & 3) | ((g >> 3) << 2);
+#pragma unroll
+        for (uint32_t bit = 1; bit < (uint32_t)kStagedFilters; bit <<= 1) {
+            const bool swap = (x & bit) != 0;
+#pragma unroll
+            for (uint32_t k = 0; k < (uint32_t)kStagedFilters; ++k) {
+                if (k & bit) continue;
+                const uint32_t a = res[k], b = res[k | bit];
+                res[k] = swap ? b : a;
+                res[k | bit] = swap ? a : b;
+            }
+        }
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < (uint32_t)kStagedFilters; ++k) row.r[k] = uint16_t(res[k]);
     for (uint32_t c = 0; c < kCachedResidues; ++c) {
         uint16_t s = 0;
         if (m.staged && c < m.ncache) {
@@ -499,6 +514,7 @@ __device__ __forceinline__ void build_row(const MatchArgs& m, uint32_t j, uint64
     }
     row.pad = 0;
     ((Row*)m.rows)[j] = row;
+    return row;
 }
 
 __global__ void k_rows(MatchArgs m) {
@@ -552,8 +568,7 @@ __device__ __forceinline__ void staged_rows_block(const MatchArgs& m, uint32_t b
     uint32_t tile = 0xffffffffu;
     uint64_t count = 0;
     if (j < m.nv) {
-        build_row<true>(m, j, v, ndir ? dir : nullptr, ndir, stride);
-        const Row& row = m.rows[j];
+        const Row row = build_row<true>(m, j, v, ndir ? dir : nullptr, ndir, stride);
         if (!row.flags) {
             tile = row.tile;
             count = row.right - row.left;
@@ -752,20 +767,36 @@ __device__ void run_chain(const MatchArgs& m, const Lds& L, uint32_t lane, uint3
         if (!live) return;
         ++k;
     }
+    // the filters on exact x: each surviving lane's x in turn is tested by the
+    // whole wave, lane l taking filter k + l, 64 filters per round; the first
+    // failing filter ends it, and each group closed before it gains one (the
+    // group ends of one round are distinct counters of this wave)
     U128 x = {0, 0};
     if (alive) x = sub_wide(mul_wide(m.us[i], m.B), shift);
-    for (; k < m.nres; ++k) {
-        const Modulus mod = L.mods[k];
-        if (alive) alive = allowed_bit(L.allow, mod.bit + mod128(x, mod));
-        const uint64_t live = __ballot(alive);
-        if ((k & 3) == 3 || k + 1 == m.nres) {
-            if (lane == 0 && live) L.groups[k / 4 + 1] += __popcll(live);
+    for (uint64_t pending = __ballot(alive); pending; pending &= pending - 1) {
+        const int src = __builtin_ctzll(pending);
+        const U128 xs = {readlane64(x.lo, src), readlane64(x.hi, src)};
+        bool passed = true;
+        for (uint32_t k0 = k; k0 < m.nres; k0 += kWave) {
+            const uint32_t f = k0 + lane;
+            bool pass = true;
+            if (f < m.nres) {
+                const Modulus mod = L.mods[f];
+                pass = allowed_bit(L.allow, mod.bit + mod128(xs, mod));
+            }
+            const uint64_t fails = __ballot(!pass);
+            const uint32_t upto = fails ? (uint32_t)__builtin_ctzll(fails) : (uint32_t)kWave;
+            if (lane < upto && f < m.nres && ((f & 3) == 3 || f + 1 == m.nres)) L.groups[f / 4 + 1] += 1;
+            wave_sync();
+            if (fails) {
+                passed = false;
+                break;
+            }
         }
-        if (!live) return;
-    }
-    if (alive) {
-        if (!less_wide(m.lo, x) || less_wide(m.hi, x)) atomicOr(m.error, 2u);
-        emit(m, row.tile, x);
+        if (passed && lane == 0) {
+            if (!less_wide(m.lo, xs) || less_wide(m.hi, xs)) atomicOr(m.error, 2u);
+            emit(m, row.tile, xs);
+        }
     }
 }
 

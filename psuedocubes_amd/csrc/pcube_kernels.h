@@ -106,6 +106,12 @@ __device__ __forceinline__ void wave_sync() {
 __device__ __forceinline__ uint32_t lane_id() {
     return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
 }
+// A 64-bit value of lane src, in every lane.
+__device__ __forceinline__ uint64_t readlane64(uint64_t x, int src) {
+    return ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(x >> 32), src) << 32) |
+           (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, src);
+}
+
 // Number of set bits of `mask` at lanes below this one.
 __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));

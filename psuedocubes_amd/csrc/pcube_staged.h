@@ -47,6 +47,8 @@ constexpr int kStageWords = 4;        // mask words (256 u coordinates) per stag
 constexpr int kPieceRows = 1024;      // v rows per work item
 constexpr int kStagedThreads = 512;   // threads of a persistent block
 constexpr int kStagedBlocksPerCU = 2; // persistent blocks per CU (4 waves per SIMD, 128 VGPRs)
+constexpr uint32_t kFirstRoundWeight = 78;   // a first-round block's share of units, in 1/64 of an average run (measured)
+constexpr uint32_t kPlanLdsPieces = 1024;    // pieces whose reach the planner keeps in LDS
 constexpr int kStagedWaves = kStagedThreads / kWave;
 constexpr int kTileSlots = 64;        // subtiles whose group counts a block keeps in LDS
 constexpr int kItemGroups = 2;        // groups counted by the intersection itself (popcounts)
@@ -165,17 +167,36 @@ __device__ __forceinline__ void staged_chain(const MatchArgs& m, const Staged& S
         if (!__ballot(alive)) return;
         ++k;
     }
+    // the filters on exact x = u*B - v*A: few lanes get here (about 1 in 81 of
+    // the bitset survivors at p = 619), so each survivor in turn is tested by the
+    // whole wave, lane l taking filter k + l, 64 filters per round; the first
+    // failing filter ends it, and the groups closed before it are counted
     U128 x = {0, 0};
     if (alive) x = sub_wide(mul_wide(m.us[i], m.B), mul_wide(row.v, m.A));
-    for (; k < m.nres; ++k) {
-        const Modulus mod = S.mods[k];
-        if (alive) alive = allowed_bit(S.allow, mod.bit + mod128(x, mod));
-        if (((k & 3) == 3 || k + 1 == m.nres) && alive) staged_add(m, S, row.tile, k / 4 + 1, 1);
-        if (!__ballot(alive)) return;
-    }
-    if (alive) {
-        if (!less_wide(m.lo, x) || less_wide(m.hi, x)) atomicOr(m.error, 2u);
-        emit(m, row.tile, x);
+    for (uint64_t pending = __ballot(alive); pending; pending &= pending - 1) {
+        const int src = __builtin_ctzll(pending);
+        const U128 xs = {readlane64(x.lo, src), readlane64(x.hi, src)};
+        const uint32_t tile = __builtin_amdgcn_readlane(row.tile, src);
+        bool passed = true;
+        for (uint32_t k0 = k; k0 < m.nres; k0 += kWave) {
+            const uint32_t f = k0 + lane;
+            bool pass = true;
+            if (f < m.nres) {
+                const Modulus mod = S.mods[f];
+                pass = allowed_bit(S.allow, mod.bit + mod128(xs, mod));
+            }
+            const uint64_t fails = __ballot(!pass);
+            const uint32_t upto = fails ? (uint32_t)__builtin_ctzll(fails) : (uint32_t)kWave;
+            if (lane < upto && f < m.nres && ((f & 3) == 3 || f + 1 == m.nres)) staged_add(m, S, tile, f / 4 + 1, 1);
+            if (fails) {
+                passed = false;
+                break;
+            }
+        }
+        if (passed && lane == 0) {
+            if (!less_wide(m.lo, xs) || less_wide(m.hi, xs)) atomicOr(m.error, 2u);
+            emit(m, tile, xs);
+        }
     }
 }
 
@@ -240,7 +261,8 @@ __device__ __forceinline__ void piece_reach(const MatchArgs& m, uint64_t v_first
 
 // First index in [0, n) whose bound (every second word of bounds, from `at`) is at
 // least key; the bounds are nondecreasing.  n when there is none.
-__device__ __forceinline__ uint32_t first_at_least(const uint32_t* bounds, uint32_t at, uint32_t n, uint32_t key) {
+template <typename P>
+__device__ __forceinline__ uint32_t first_at_least(P bounds, uint32_t at, uint32_t n, uint32_t key) {
     uint32_t l = 0, h = n;
     while (l < h) {
         const uint32_t mid = (l + h) >> 1;
@@ -257,16 +279,19 @@ __device__ __forceinline__ uint32_t first_at_least(const uint32_t* bounds, uint3
 // (A piece whose rows reach nothing, right1 <= left0, can lie inside a run: its
 // item has no work.)  Chunk-major order lets a block walking consecutive items
 // keep one chunk image in LDS while only the row headers change.
-// One 256-thread block: the pieces' reach goes to `reach` (two words per piece),
-// then thread t takes a run of consecutive chunks and a block scan turns the
-// runs' item counts into offsets.
+// One 256-thread block: the pieces' reach goes to `reach` (two words per piece;
+// in LDS after the directory for kLds, else global), then thread t takes a run
+// of consecutive chunks and a block scan turns the runs' item counts into
+// offsets.
+template <bool kLds>
 __device__ __forceinline__ void staged_plan_block(const MatchArgs& m, uint32_t npieces, uint2* items,
-                                                  uint32_t items_cap, uint32_t* nitems, uint32_t* reach,
+                                                  uint32_t items_cap, uint32_t* nitems, uint32_t* reach_global,
                                                   uint64_t* dir) {
     __shared__ uint32_t wave_total[256 / kWave];
     const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     uint32_t ndir, stride;
     load_directory(m, dir, ndir, stride);
+    uint32_t* reach = kLds ? (uint32_t*)(dir + (ndir ? ndir : 1)) : reach_global;
     const uint64_t* d = ndir ? dir : nullptr;
     for (uint32_t p = threadIdx.x; p < npieces; p += 256) {
         uint64_t v_first, v_last;
@@ -309,10 +334,11 @@ __device__ __forceinline__ void staged_plan_block(const MatchArgs& m, uint32_t n
 // Geometry of the fused prefilter launch.
 struct PrefilterArgs {
     uint32_t rows_blocks, mask_blocks, rho_blocks, chunks_total, npieces, items_cap;
+    uint32_t reach_lds;  // the pieces' reach fits the dynamic LDS after the directory
     uint32_t chunk_base[kStagedFilters];
     uint2* items;
     uint32_t* nitems;
-    uint32_t* reach;     // [npieces][2]: the u indices each piece's rows reach
+    uint32_t* reach;     // [npieces][2]: the u indices each piece's rows reach (when not in LDS)
 };
 
 // Everything the staged match needs, in one launch of independent 256-thread
@@ -320,13 +346,15 @@ struct PrefilterArgs {
 // mask row offsets, sigmas, valid counts), the mask waves and the residue
 // cache.
 // The u directory lives in dynamic LDS sized to the batch (directory_shape of
-// nu, 8 bytes an entry), so the blocks that never read it stay small.
+// nu, 8 bytes an entry, then the planner's piece reach when it fits), so the
+// blocks that never read it stay small.
 __global__ __launch_bounds__(256) void k_prefilter_staged(MatchArgs m, PrefilterArgs pa) {
     extern __shared__ uint64_t prefilter_dir[];
     uint64_t* dir = prefilter_dir;
     uint32_t b = blockIdx.x;
     if (b == 0) {
-        staged_plan_block(m, pa.npieces, pa.items, pa.items_cap, pa.nitems, pa.reach, dir);
+        if (pa.reach_lds) staged_plan_block<true>(m, pa.npieces, pa.items, pa.items_cap, pa.nitems, pa.reach, dir);
+        else staged_plan_block<false>(m, pa.npieces, pa.items, pa.items_cap, pa.nitems, pa.reach, dir);
         return;
     }
     b -= 1;
@@ -561,10 +589,24 @@ __device__ __forceinline__ void staged_unit(const MatchArgs& m, const Staged& S,
     flush_groups(m, S, touches, R.h0.z, c1, c2, lane);
 }
 
+// Units before run r, in 1/64 units of an average run, when the first-round
+// blocks (block id < grid / 2, the first to reach each CU) take first64 / 64 of
+// an average run and the others (128 - first64) / 64.  Runs are grouped by XCD
+// (run = (b mod 8) * grid / 8 + b / 8), so within each XCD's runs the first half
+// belongs to first-round blocks.  The two blocks of a CU then end together
+// although the older one's waves win the CU's issue arbitration.
+__device__ __forceinline__ uint64_t run_weight_prefix(uint32_t r, uint32_t grid, uint32_t first64) {
+    const uint32_t per = grid / 8;
+    if (grid % 16 != 0 || first64 == 64) return (uint64_t)r * 64;
+    const uint32_t x = r / per, k = r % per, half = per / 2;
+    const uint64_t within = k < half ? (uint64_t)k * first64 : (uint64_t)half * first64 + (uint64_t)(k - half) * (128 - first64);
+    return (uint64_t)x * per * 64 + within;
+}
+
 // Persistent staged match, kStagedBlocksPerCU blocks per CU.  The work units
 // (16 per item of the chunk-major list) are split into one contiguous run per
-// block; the runs of one XCD's blocks are adjacent, so an L2 holds the chunk
-// images its blocks stage.  A run is walked in segments of one chunk: the
+// block (weighted by dispatch round, run_weight_prefix); the runs of one XCD's
+// blocks are adjacent, so an L2 holds the chunk images its blocks stage.  A run is walked in segments of one chunk: the
 // block stages the chunk's image once and its waves take the segment's units in
 // turn (wave w: units w, w + 8, ... of the segment), each wave on its own,
 // with the next unit's row headers in flight; the block synchronises only when
@@ -578,7 +620,7 @@ __device__ __forceinline__ void staged_unit(const MatchArgs& m, const Staged& S,
 // stay resident.
 template <int kF, bool kLast>
 __global__ __launch_bounds__(kStagedThreads) __attribute__((amdgpu_waves_per_eu(4)))
-void k_match_staged(MatchArgs m, const uint2* items, uint32_t items_cap, const uint32_t* nitems) {
+void k_match_staged(MatchArgs m, const uint2* items, uint32_t items_cap, const uint32_t* nitems, uint32_t first64) {
     __shared__ uint4 image[kStageBufferUnits];
     __shared__ uint32_t tilecnt[kTileSlots * kItemGroups];
     __shared__ uint2 queues[kStagedWaves * kQueue];
@@ -588,8 +630,8 @@ void k_match_staged(MatchArgs m, const uint2* items, uint32_t items_cap, const u
     // consecutive block ids go round robin over the 8 XCDs: group the runs by XCD
     const uint32_t grid = gridDim.x;
     const uint32_t run = grid % 8 == 0 ? (blockIdx.x % 8) * (grid / 8) + blockIdx.x / 8 : blockIdx.x;
-    const uint32_t u_first = (uint32_t)((uint64_t)run * total / grid);
-    const uint32_t u_last = (uint32_t)((uint64_t)(run + 1) * total / grid);
+    const uint32_t u_first = (uint32_t)(total * run_weight_prefix(run, grid, first64) / ((uint64_t)grid * 64));
+    const uint32_t u_last = (uint32_t)(total * run_weight_prefix(run + 1, grid, first64) / ((uint64_t)grid * 64));
     if (u_first >= u_last) return;
     ItemWindow iw;
     iw.count = total_items;
@@ -743,8 +785,8 @@ __global__ __launch_bounds__(256) void k_valu_probe(uint32_t rounds, uint32_t* o
 }
 
 #define PCUBE_STAGED(F)                                                                               \
-    template __global__ void k_match_staged<F, false>(MatchArgs, const uint2*, uint32_t, const uint32_t*); \
-    template __global__ void k_match_staged<F, true>(MatchArgs, const uint2*, uint32_t, const uint32_t*);
+    template __global__ void k_match_staged<F, false>(MatchArgs, const uint2*, uint32_t, const uint32_t*, uint32_t); \
+    template __global__ void k_match_staged<F, true>(MatchArgs, const uint2*, uint32_t, const uint32_t*, uint32_t);
 PCUBE_STAGED(1)
 PCUBE_STAGED(2)
 PCUBE_STAGED(3)
