@@ -378,6 +378,15 @@ struct UnitRow {
     uint4 h0, h1;
 };
 
+// Makes the compiler wait for a unit's row header here (an empty statement that
+// reads its registers), so that the next unit's headers, loaded after this, may
+// stay in flight while this unit is matched: the vector memory counter waits
+// in order, so without it the first use of this header would also wait for
+// those loads.
+__device__ __forceinline__ void unit_row_ready(const UnitRow& R) {
+    asm volatile("" ::"v"(R.h0.x), "v"(R.h0.y), "v"(R.h0.z), "v"(R.h1.x), "v"(R.h1.y), "v"(R.h1.z), "v"(R.h1.w));
+}
+
 // Unit s of piece p is rows j0 + 64 s .. j0 + 64 s + 63 of the piece (j0 =
 // 1024 p), lane l taking row j0 + 64 s + l; lanes past the piece's end read
 // its last row (their results are discarded).
@@ -624,7 +633,9 @@ void k_match_staged(MatchArgs m, const uint2* items, uint32_t items_cap, const u
     __shared__ uint4 image[kStageBufferUnits];
     __shared__ uint32_t tilecnt[kTileSlots * kItemGroups];
     __shared__ uint2 queues[kStagedWaves * kQueue];
-    const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    // the wave index is the same in every lane: held in a scalar register, the
+    // unit loop's control is scalar
+    const uint32_t lane = threadIdx.x & (kWave - 1), wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const uint32_t total_items = min(*nitems, items_cap);
     const uint32_t total = total_items * kUnitsPerItem;
     // consecutive block ids go round robin over the 8 XCDs: group the runs by XCD
@@ -653,15 +664,27 @@ void k_match_staged(MatchArgs m, const uint2* items, uint32_t items_cap, const u
     __builtin_amdgcn_s_waitcnt(0);   // this wave's part of the image has arrived
     __syncthreads();
     for (;;) {
+        // unrolled by two: rows of unit x in R while those of x + 8 load into R2,
+        // then the other way round
+        UnitRow R2;
         while (x < seg_end) {
-            const UnitRow cur = R;
             const uint32_t p = it.x, s = x % kUnitsPerItem;
             x += kStagedWaves;
+            unit_row_ready(R);
+            if (x < seg_end) {
+                it = item_at(items, iw, x / kUnitsPerItem, lane);
+                load_unit_row(m, it.x, x % kUnitsPerItem, lane, R2);
+            }
+            staged_unit<kF, kLast>(m, S, p, s, chunk, R, image, lane, queued, region, fill);
+            if (x >= seg_end) break;
+            const uint32_t p2 = it.x, s2 = x % kUnitsPerItem;
+            x += kStagedWaves;
+            unit_row_ready(R2);
             if (x < seg_end) {
                 it = item_at(items, iw, x / kUnitsPerItem, lane);
                 load_unit_row(m, it.x, x % kUnitsPerItem, lane, R);
             }
-            staged_unit<kF, kLast>(m, S, p, s, chunk, cur, image, lane, queued, region, fill);
+            staged_unit<kF, kLast>(m, S, p2, s2, chunk, R2, image, lane, queued, region, fill);
         }
         if (seg_end >= u_last) break;
         // the next segment: its first unit's rows are in flight while the block
