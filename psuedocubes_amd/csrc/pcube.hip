@@ -719,10 +719,10 @@ struct Matcher {
     uint32_t s_pstr_off[kStagedFilters] = {}, s_nbeta[kStagedFilters] = {};
     std::vector<Modulus> mods;
     Buffer d_mods, d_allow, d_pstr;
-    // status words: [0] survivor list length, [1] error bits, [2] unused, [3] most bitset
-    // survivors of any wave of the staged match that overflowed its region since the last clear
+    // status words: [0] survivor list length, [1] error bits, [2] unused, [3] most survivor
+    // records of any wave of the staged match that overflowed its region since the last clear
     Buffer rows, masks, rho, counters, list, status, cand, items;
-    uint64_t cand_cap = 512;                            // bitset-survivor entries per wave of the staged match
+    uint64_t cand_cap = 256;                            // bitset-survivor records per wave of the staged match
     // start, bitsets built, match done, intersection kernel start and end, download done
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     Pending pending;       // the batch started by pcube_match_start
@@ -789,8 +789,8 @@ int staged_enqueue(Matcher& mt, MatchArgs m, hipEvent_t ev_pre, hipEvent_t ev_ma
     // kStagedBlocksPerCU persistent blocks per CU, no more than there are work units
     const uint32_t blocks = (uint32_t)std::min<uint64_t>((uint64_t)kStagedBlocksPerCU * g.compute_units,
                                                          (uint64_t)items_cap * kUnitsPerItem);
-    // each wave's survivor region: cand_cap (row, u index) pairs
-    TRY(mt.cand.ensure(std::max<uint64_t>(8, (uint64_t)blocks * kStagedWaves * mt.cand_cap * sizeof(uint2))));
+    // each wave's survivor region: cand_cap survivor records
+    TRY(mt.cand.ensure(std::max<uint64_t>(16, (uint64_t)blocks * kStagedWaves * mt.cand_cap * kRecordUnits * 16)));
     m.cand = mt.cand.as<uint32_t>();
     // one launch: the item plan, the rows, the bitsets and the residue cache
     PrefilterArgs pa;
@@ -841,9 +841,10 @@ bool staged_fits(const Matcher& mt, uint64_t nu, uint64_t nv) {
 
 // Internal engine code: the residue-bitset engine without LDS staging.
 constexpr int kUnstagedBitsets = 2;
-// Bitset survivors one wave of the staged match may hold before the unstaged
-// engine is used instead (a wave's region is item_cap pairs; 4096 waves at most).
-constexpr uint64_t kMaxWaveCandidates = 1ull << 13;
+// Survivor records (rows with bitset survivors in one chunk) one wave of the
+// staged match may hold before the unstaged engine is used instead (a wave's
+// region is item_cap records of 48 bytes; 4096 waves at most).
+constexpr uint64_t kMaxWaveCandidates = 1ull << 11;
 
 // Enqueue the kernels of one batch; counters must be zeroed by the caller.
 int match_enqueue(Matcher& mt, const uint64_t* us, uint64_t nu, const uint64_t* vs, uint64_t nv, uint64_t v_first,

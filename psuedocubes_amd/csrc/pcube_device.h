@@ -330,8 +330,8 @@ struct MatchArgs {
     uint32_t staged, stage_lines, nstage;
     uint32_t nbeta[kStagedFilters];    // (-B * A^-1) mod q per bitset filter
     uint32_t pstr_off[kStagedFilters];
-    uint32_t* cand;            // bitset survivors as (row, u index) pairs, item_cap per wave of the staged match
-    uint64_t* cand_max;        // largest wave survivor count above item_cap seen since the last clear
+    uint32_t* cand;            // bitset survivor records, item_cap per wave of the staged match
+    uint64_t* cand_max;        // largest wave record count above item_cap seen since the last clear
     uint32_t item_cap;
 };
 

@@ -200,42 +200,86 @@ __device__ __forceinline__ void staged_chain(const MatchArgs& m, const Staged& S
     }
 }
 
-// Append the set bits of a lane's four survivor words (word w at chunk offset
-// first[w], the chunk's first u index base) to the wave's queue as (row j,
-// u index), one bit per lane per round.  Whenever 64 are queued they go to the
-// wave's own region of the survivor list (fill entries written so far; entries
-// past cap are counted, not written).
-__device__ __forceinline__ void staged_append(const Staged& S, uint32_t lane, uint64_t (&b)[4], const int (&first)[4],
-                                              uint32_t j, uint32_t base, uint32_t& queued, uint2* region,
-                                              uint32_t cap, uint32_t& fill) {
-    for (;;) {
-        const bool has = (b[0] | b[1] | b[2] | b[3]) != 0;
-        const uint64_t ballot = __ballot(has);
-        if (!ballot) return;
-        if (has) {
-            const int w = b[0] ? 0 : b[1] ? 1 : b[2] ? 2 : 3;
-            const uint64_t x = w == 0 ? b[0] : w == 1 ? b[1] : w == 2 ? b[2] : b[3];
-            S.queue[queued + lanes_below(ballot)] = make_uint2(j, base + (uint32_t)(first[w] + __builtin_ctzll(x)));
-            const uint64_t rest = x & (x - 1);
-            b[0] = w == 0 ? rest : b[0];
-            b[1] = w == 1 ? rest : b[1];
-            b[2] = w == 2 ? rest : b[2];
-            b[3] = w == 3 ? rest : b[3];
+// A survivor record: the bitset survivors of one row in one chunk, 48 bytes:
+// (row j, u index of word 0, u index of word 2, 0), then the four 64-bit
+// survivor words (words 0-1 cover u from the first index on, words 2-3 from
+// the second).
+constexpr int kRecordUnits = 3;   // 16-byte units of a record
+
+// Write the record of each lane with survivors to the wave's region (fill
+// records written so far; those past cap are counted, not written).
+__device__ __forceinline__ void write_records(const uint64_t (&w)[4], uint32_t j, uint32_t up, uint32_t uq,
+                                              uint4* region, uint32_t cap, uint32_t& fill) {
+    const bool has = (w[0] | w[1] | w[2] | w[3]) != 0;
+    const uint64_t ballot = __ballot(has);
+    if (!ballot) return;
+    if (has) {
+        const uint32_t at = fill + lanes_below(ballot);
+        if (at < cap) {
+            uint4* rec = region + (uint64_t)at * kRecordUnits;
+            rec[0] = make_uint4(j, up, uq, 0);
+            rec[1] = make_uint4(uint32_t(w[0]), uint32_t(w[0] >> 32), uint32_t(w[1]), uint32_t(w[1] >> 32));
+            rec[2] = make_uint4(uint32_t(w[2]), uint32_t(w[2] >> 32), uint32_t(w[3]), uint32_t(w[3] >> 32));
         }
-        queued += __popcll(ballot);
-        if (queued >= kWave) {
-            wave_sync();
-            if (fill + lane < cap) region[fill + lane] = S.queue[lane];
-            fill += kWave;
-            wave_sync();
-            const uint32_t left = queued - kWave;
-            uint2 moved = make_uint2(0, 0);
-            if (lane < left) moved = S.queue[kWave + lane];
-            wave_sync();
-            if (lane < left) S.queue[lane] = moved;
-            queued = left;
-            wave_sync();
+    }
+    fill += (uint32_t)__popcll(ballot);
+}
+
+// Run a wave's n survivor records through the residual chain: lane l expands
+// record l of each group of 64 into (row, u index) pairs, one bit per lane per
+// round, into the wave's queue; each 64 queued pairs take one chain call.
+__device__ __forceinline__ void chain_records(const MatchArgs& m, const Staged& C, uint32_t lane,
+                                              const uint4* region, uint32_t n) {
+    uint32_t queued = 0;
+    for (uint32_t r0 = 0; r0 < n; r0 += kWave) {
+        uint64_t b[4] = {0, 0, 0, 0};
+        uint32_t j = 0, up = 0, uq = 0;
+        if (r0 + lane < n) {
+            const uint4* rec = region + (uint64_t)(r0 + lane) * kRecordUnits;
+            const uint4 h = rec[0], x = rec[1], y = rec[2];
+            j = h.x;
+            up = h.y;
+            uq = h.z;
+            b[0] = ((uint64_t)x.y << 32) | x.x;
+            b[1] = ((uint64_t)x.w << 32) | x.z;
+            b[2] = ((uint64_t)y.y << 32) | y.x;
+            b[3] = ((uint64_t)y.w << 32) | y.z;
         }
+        for (;;) {
+            const bool has = (b[0] | b[1] | b[2] | b[3]) != 0;
+            const uint64_t ballot = __ballot(has);
+            if (!ballot) break;
+            if (has) {
+                const int w = b[0] ? 0 : b[1] ? 1 : b[2] ? 2 : 3;
+                const uint64_t x = w == 0 ? b[0] : w == 1 ? b[1] : w == 2 ? b[2] : b[3];
+                const uint32_t u0 = w < 2 ? up + 64 * w : uq + 64 * (w - 2);
+                C.queue[queued + lanes_below(ballot)] = make_uint2(j, u0 + (uint32_t)__builtin_ctzll(x));
+                const uint64_t rest = x & (x - 1);
+                b[0] = w == 0 ? rest : b[0];
+                b[1] = w == 1 ? rest : b[1];
+                b[2] = w == 2 ? rest : b[2];
+                b[3] = w == 3 ? rest : b[3];
+            }
+            queued += (uint32_t)__popcll(ballot);
+            if (queued >= kWave) {
+                wave_sync();
+                const uint2 e = C.queue[lane];
+                staged_chain(m, C, lane, kWave, e.x, e.y);
+                wave_sync();
+                const uint32_t left = queued - kWave;
+                uint2 moved = make_uint2(0, 0);
+                if (lane < left) moved = C.queue[kWave + lane];
+                wave_sync();
+                if (lane < left) C.queue[lane] = moved;
+                queued = left;
+                wave_sync();
+            }
+        }
+    }
+    if (queued) {
+        wave_sync();
+        const uint2 e = lane < queued ? C.queue[lane] : make_uint2(0, 0);
+        staged_chain(m, C, lane, queued, e.x, e.y);
     }
 }
 
@@ -565,12 +609,14 @@ __device__ __forceinline__ void staged_row(const uint8_t* masks_p, const uint8_t
 // One work unit: the rows of unit s of piece p (row j0 + 64 s + lane for this
 // lane) against the staged image of chunk `chunk`, through staged_row (skipped
 // when no row of the wave reaches the chunk; without range masks when every row
-// covers it whole).  Survivors go to the wave's queue, the group 1 and 2 counts
-// to the block's LDS counts.
+// covers it whole).  Survivor records go to the wave's region; the lane's
+// group 1 and 2 counts and its row's subtile are returned (zero counts when
+// the wave had nothing to do).
 template <int kF, bool kLast>
-__device__ __forceinline__ void staged_unit(const MatchArgs& m, const Staged& S, uint32_t p, uint32_t s,
-                                            uint32_t chunk, const UnitRow& R, const uint4* image, uint32_t lane,
-                                            uint32_t& queued, uint2* region, uint32_t& fill) {
+__device__ __forceinline__ void staged_unit(const MatchArgs& m, uint32_t p, uint32_t s, uint32_t chunk,
+                                            const UnitRow& R, const uint4* image, uint32_t lane, uint4* region,
+                                            uint32_t& fill, uint32_t& c1, uint32_t& c2) {
+    c1 = c2 = 0;
     const uint32_t j0 = p * kPieceRows, j1 = min(m.nv, j0 + kPieceRows);
     const uint32_t jr = j0 + s * kWave;
     if (jr >= j1) return;   // the piece ends before this unit
@@ -587,15 +633,50 @@ __device__ __forceinline__ void staged_unit(const MatchArgs& m, const Staged& S,
     const uint8_t* masks_p = (const uint8_t*)image + 16 * hp;          // half P of a column
     const uint8_t* masks_q = (const uint8_t*)image + 16 * (hp ^ 1);    // half Q
     const int first_p = 128 * (int)hp, first_q = 128 - first_p;        // chunk offset of each half's first word
-    uint32_t c1 = 0, c2 = 0;
     uint64_t words[4];
     if (!__ballot(!full))
         staged_row<kF, kLast, false>(masks_p, masks_q, first_p, first_q, g, R.h1, dl, dr, c1, c2, words);
     else
         staged_row<kF, kLast, true>(masks_p, masks_q, first_p, first_q, g, R.h1, dl, dr, c1, c2, words);
-    const int firsts[4] = {first_p, first_p + 64, first_q, first_q + 64};
-    staged_append(S, lane, words, firsts, j, base0, queued, region, m.item_cap, fill);
-    flush_groups(m, S, touches, R.h0.z, c1, c2, lane);
+    write_records(words, j, base0 + first_p, base0 + first_q, region, m.item_cap, fill);
+}
+
+// Group 1 and 2 counts of a wave's consecutive units, kept per lane while the
+// counting rows of its units lie in one subtile and added to the block's
+// counts when the subtile changes; a unit whose counting rows span two
+// subtiles adds its own counts row by row.
+struct GroupAcc {
+    uint32_t tile;   // subtile of the held counts (wave-uniform), ~0u when none
+    uint32_t c1, c2;
+};
+
+__device__ __forceinline__ void acc_flush(const MatchArgs& m, const Staged& S, GroupAcc& A, uint32_t lane) {
+    if (A.tile == ~0u) return;
+    const uint32_t s1 = wave_sum(A.c1), s2 = wave_sum(A.c2);
+    if (lane == 0) {
+        if (s1) staged_add(m, S, A.tile, 1, s1);
+        if (s2) staged_add(m, S, A.tile, 2, s2);
+    }
+    A.tile = ~0u;
+    A.c1 = A.c2 = 0;
+}
+
+__device__ __forceinline__ void acc_add(const MatchArgs& m, const Staged& S, GroupAcc& A, uint32_t tile, uint32_t c1,
+                                        uint32_t c2, uint32_t lane) {
+    const bool counted = (c1 | c2) != 0;
+    const uint64_t active = __ballot(counted);
+    if (!active) return;
+    const uint32_t t0 = __builtin_amdgcn_readlane(tile, __builtin_ctzll(active));
+    if (__ballot(counted && tile != t0)) {
+        flush_groups(m, S, true, tile, c1, c2, lane);
+        return;
+    }
+    if (t0 != A.tile) {
+        acc_flush(m, S, A, lane);
+        A.tile = t0;
+    }
+    A.c1 += c1;
+    A.c2 += c2;
 }
 
 // Units before run r, in 1/64 units of an average run, when the first-round
@@ -651,49 +732,58 @@ void k_match_staged(MatchArgs m, const uint2* items, uint32_t items_cap, const u
     stage_async(m, chunk, (uint8_t*)image, wave, lane);
     for (uint32_t t = threadIdx.x; t < (uint32_t)(kTileSlots * kItemGroups); t += kStagedThreads) tilecnt[t] = 0;
     const Staged S{nullptr, nullptr, tilecnt, kItemGroups, queues + wave * kQueue};
-    uint2* region = (uint2*)m.cand + ((uint64_t)blockIdx.x * kStagedWaves + wave) * m.item_cap;
-    uint32_t queued = 0, fill = 0;
-    uint32_t seg_end = segment_end(items, iw, u_first, u_last, chunk, lane);
-    uint32_t x = u_first + wave;   // this wave's next unit
+    uint4* region = (uint4*)m.cand + ((uint64_t)blockIdx.x * kStagedWaves + wave) * m.item_cap * kRecordUnits;
+    uint32_t fill = 0;
+    GroupAcc A{~0u, 0, 0};
+    // a segment [u, seg_end) of n units gives wave w the contiguous units
+    // [u + n w / 8, u + n (w + 1) / 8), so its consecutive units mostly share a subtile
+    uint32_t u = u_first;
+    uint32_t seg_end = segment_end(items, iw, u, u_last, chunk, lane);
+    uint32_t x = u + (seg_end - u) * wave / kStagedWaves;
+    uint32_t x_end = u + (seg_end - u) * (wave + 1) / kStagedWaves;
     UnitRow R;
     uint2 it = make_uint2(0, 0);
-    if (x < seg_end) {
+    if (x < x_end) {
         it = item_at(items, iw, x / kUnitsPerItem, lane);
         load_unit_row(m, it.x, x % kUnitsPerItem, lane, R);
     }
     __builtin_amdgcn_s_waitcnt(0);   // this wave's part of the image has arrived
     __syncthreads();
     for (;;) {
-        // unrolled by two: rows of unit x in R while those of x + 8 load into R2,
+        // unrolled by two: rows of unit x in R while those of x + 1 load into R2,
         // then the other way round
         UnitRow R2;
-        while (x < seg_end) {
+        uint32_t c1, c2;
+        while (x < x_end) {
             const uint32_t p = it.x, s = x % kUnitsPerItem;
-            x += kStagedWaves;
+            ++x;
             unit_row_ready(R);
-            if (x < seg_end) {
+            if (x < x_end) {
                 it = item_at(items, iw, x / kUnitsPerItem, lane);
                 load_unit_row(m, it.x, x % kUnitsPerItem, lane, R2);
             }
-            staged_unit<kF, kLast>(m, S, p, s, chunk, R, image, lane, queued, region, fill);
-            if (x >= seg_end) break;
+            staged_unit<kF, kLast>(m, p, s, chunk, R, image, lane, region, fill, c1, c2);
+            acc_add(m, S, A, R.h0.z, c1, c2, lane);
+            if (x >= x_end) break;
             const uint32_t p2 = it.x, s2 = x % kUnitsPerItem;
-            x += kStagedWaves;
+            ++x;
             unit_row_ready(R2);
-            if (x < seg_end) {
+            if (x < x_end) {
                 it = item_at(items, iw, x / kUnitsPerItem, lane);
                 load_unit_row(m, it.x, x % kUnitsPerItem, lane, R);
             }
-            staged_unit<kF, kLast>(m, S, p2, s2, chunk, R2, image, lane, queued, region, fill);
+            staged_unit<kF, kLast>(m, p2, s2, chunk, R2, image, lane, region, fill, c1, c2);
+            acc_add(m, S, A, R2.h0.z, c1, c2, lane);
         }
         if (seg_end >= u_last) break;
         // the next segment: its first unit's rows are in flight while the block
         // waits for every wave to finish with the image
-        const uint32_t u = seg_end;
+        u = seg_end;
         chunk = item_at(items, iw, u / kUnitsPerItem, lane).y;
         seg_end = segment_end(items, iw, u, u_last, chunk, lane);
-        x = u + wave;
-        if (x < seg_end) {
+        x = u + (seg_end - u) * wave / kStagedWaves;
+        x_end = u + (seg_end - u) * (wave + 1) / kStagedWaves;
+        if (x < x_end) {
             it = item_at(items, iw, x / kUnitsPerItem, lane);
             load_unit_row(m, it.x, x % kUnitsPerItem, lane, R);
         }
@@ -702,9 +792,10 @@ void k_match_staged(MatchArgs m, const uint2* items, uint32_t items_cap, const u
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
     }
-    // survivors past the region's capacity: the host sizes the regions from
-    // the largest count and runs the batch again
-    if (fill > m.item_cap && lane == 0) atomicMax((unsigned long long*)m.cand_max, (unsigned long long)(fill + queued));
+    acc_flush(m, S, A, lane);
+    // records past the region's capacity: the host sizes the regions from the
+    // largest count and runs the batch again
+    if (fill > m.item_cap && lane == 0) atomicMax((unsigned long long*)m.cand_max, (unsigned long long)fill);
     const uint32_t listed = min(fill, m.item_cap);
     __syncthreads();   // every wave is done with the image and its unit counts
     for (uint32_t t = threadIdx.x; t < (uint32_t)(kTileSlots * kItemGroups); t += kStagedThreads) {
@@ -722,15 +813,7 @@ void k_match_staged(MatchArgs m, const uint2* items, uint32_t items_cap, const u
     for (uint32_t t = threadIdx.x; t < (uint32_t)kTileSlots * m.ng; t += kStagedThreads) chain_counts[t] = 0;
     __syncthreads();
     const Staged C{allow, mods, chain_counts, m.ng, S.queue};
-    for (uint32_t first = 0; first < listed; first += kWave) {
-        const uint32_t count = min(listed - first, (uint32_t)kWave);
-        const uint2 e = lane < count ? region[first + lane] : make_uint2(0, 0);
-        staged_chain(m, C, lane, count, e.x, e.y);
-    }
-    if (queued) {
-        const uint2 e = lane < queued ? C.queue[lane] : make_uint2(0, 0);
-        staged_chain(m, C, lane, queued, e.x, e.y);
-    }
+    chain_records(m, C, lane, region, listed);
     __syncthreads();
     for (uint32_t t = threadIdx.x; t < (uint32_t)kTileSlots * m.ng; t += kStagedThreads) {
         const uint32_t c = chain_counts[t];
