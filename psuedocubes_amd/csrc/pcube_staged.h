@@ -47,7 +47,7 @@ constexpr int kStageWords = 4;        // mask words (256 u coordinates) per stag
 constexpr int kPieceRows = 1024;      // v rows per work item
 constexpr int kStagedThreads = 512;   // threads of a persistent block
 constexpr int kStagedBlocksPerCU = 2; // persistent blocks per CU (4 waves per SIMD, 128 VGPRs)
-constexpr uint32_t kFirstRoundWeight = 78;   // a first-round block's share of units, in 1/64 of an average run (measured)
+constexpr uint32_t kFirstRoundWeight = 72;   // a first-round block's share of units, in 1/64 of an average run (measured)
 constexpr uint32_t kPlanLdsPieces = 1024;    // pieces whose reach the planner keeps in LDS
 constexpr int kStagedWaves = kStagedThreads / kWave;
 constexpr int kTileSlots = 64;        // subtiles whose group counts a block keeps in LDS
