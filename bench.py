@@ -74,6 +74,8 @@ def parse() -> argparse.Namespace:
     ap.add_argument('--batched-windows', type=int, default=4096)
     ap.add_argument('--plain-windows', type=int, default=256, help='prefix of the stream timed on the plain path')
     ap.add_argument('--no-served', action='store_true')
+    ap.add_argument('--no-staging', action='store_true')
+    ap.add_argument('--staging-jobs', type=int, default=64, help='batches through each producer/consumer mode')
     ap.add_argument('--served-requests', type=int, default=20000)
     ap.add_argument('--served-load', type=float, default=0.8, help='offered load as a fraction of capacity')
     ap.add_argument('--copy-probe', type=int, default=0, metavar='BYTES',
@@ -340,6 +342,33 @@ def served_leg(runtime, af, bf, requests: int, load: float) -> dict:
                 max_requests_per_step=max(server.step_sizes), host_overlap_seconds=server.overlap_seconds)
 
 
+def staging_leg(runtime, config, af, bf, jobs: int) -> dict:
+    """Producer/consumer staging (tools/batch_window_experiment.py:256-404): one producer (device
+    wheel or host producer) and one matcher consumer through page-locked slots, over `jobs`
+    consecutive full batches from the reference's positive lease; the direct-engine modes over an
+    eighth of them.  Every mode's rows must carry the same signatures."""
+    from psuedocubes_amd.staging import fleet
+    first_job = 575222931200000000 // 6_400_000_000
+    legs, signatures = {}, {}
+    for mode in ('masked-gpu-producer', 'masked-cpu-producer', 'gpu-producer', 'cpu-producer'):
+        n = jobs if mode.startswith('masked-') else max(2, jobs // 8)
+        fleet(runtime, config, af, bf, mode, list(range(first_job + n, first_job + n + 2)))   # warm
+        result = fleet(runtime, config, af, bf, mode, list(range(first_job, first_job + n)))
+        for r in result['rows']:
+            if signatures.setdefault(r['job'], r['signature']) != r['signature']:
+                raise ArithmeticError(f'staging mode {mode} disagrees on job {r["job"]}')
+        producer, consumer = result['workers'][0], result['workers'][1]
+        legs[mode] = dict(jobs=n, value=result['candidates_per_second'], seconds=result['elapsed_seconds'],
+                          producer_generation_seconds=producer['generation_seconds'],
+                          producer_staging_seconds=producer['staging_seconds'],
+                          consumer_waiting_seconds=consumer['waiting_seconds'],
+                          consumer_upload_seconds=consumer['upload_seconds'],
+                          consumer_filter_seconds=consumer['filter_seconds'])
+    return dict(workload=f'{jobs} consecutive full p=619 batches from the positive lease (direct-engine modes: '
+                         f'{max(2, jobs // 8)}), one producer and one consumer thread, 2 page-locked slots',
+                unit='focused candidate positions/s', modes=legs, rows_agree=True)
+
+
 def main() -> None:
     args = parse()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) > 1:
@@ -469,6 +498,9 @@ def main() -> None:
     if not args.no_served:
         result['served'] = served_leg(runtime, af, bf, args.served_requests, args.served_load)
         print('# served leg done', file=sys.stderr, flush=True)
+    if not args.no_staging:
+        result['staging'] = staging_leg(runtime, config, af, bf, args.staging_jobs)
+        print('# staging leg done', file=sys.stderr, flush=True)
     if not args.no_cpu:
         # config 1: the first 1e8-v subtile through the drop-in path, for the CPU's gate
         small_range = (0, 100_000_000)

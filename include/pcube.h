@@ -72,6 +72,28 @@ int pcube_free(uint64_t address);
 int pcube_upload(uint64_t dst, const void *src, uint64_t bytes);
 int pcube_download(void *dst, uint64_t src, uint64_t bytes);
 
+/* --- producer/consumer staging: replaces the slots and the CPU producer of
+ * tools/batch_window_experiment.py:52-82, 256-336 -------------------------- */
+
+/* Page-locked host memory for the staging slots between a window producer and
+ * a matcher consumer (the reference's shared-memory slots of 2 x 1e6 uint64,
+ * tools/batch_window_experiment.py:314-318); copies to and from it run as DMA.
+ * pcube_pinned_free waits for the library stream before it frees. */
+int pcube_pinned_alloc(uint64_t bytes, void **host);
+int pcube_pinned_free(void *host);
+
+/* Host window producer with the arguments and codes of the reference's CPU
+ * producer focus_window (review/focus_window_cpu.cpp:4-6, bound at
+ * tools/batch_window_experiment.py:58-64): the focus coordinates in
+ * [lo, hi) of one period (0 <= lo <= hi <= M < 2^63), offset by base, from the
+ * ascending half arrays C and D, written in ascending order (the reference
+ * sorts after the call, tools/batch_window_experiment.py:82).  Returns 0, 1
+ * with *count = cap + 1 when more than cap exist (as the reference: written + 1),
+ * or 2. */
+int pcube_host_focus_window(const uint64_t *C, uint64_t nc, const uint64_t *D, uint64_t nd, uint64_t M,
+                            uint64_t lo, uint64_t hi, uint64_t base, uint64_t *out, uint64_t cap,
+                            uint64_t *count);
+
 /* --- focus wheels: replaces pseudocube/gpu.py:70-114 (GPUFocus) ----------- */
 
 /* Build the two sorted half arrays C and D of one focus on the device

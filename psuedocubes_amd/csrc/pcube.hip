@@ -1333,6 +1333,61 @@ int pcube_download(void* dst, uint64_t src, uint64_t bytes) {
     return PCUBE_OK;
 }
 
+int pcube_pinned_alloc(uint64_t bytes, void** host) {
+    Entry lock;
+    if (!host || !bytes) return fail(PCUBE_INVALID, "pinned allocation needs a size and an output pointer");
+    if (g.device < 0) return fail(PCUBE_INVALID, "pcube_init has not been called");
+    *host = nullptr;
+    if (hipHostMalloc(host, bytes, hipHostMallocDefault) != hipSuccess || !*host) {
+        *host = nullptr;
+        return fail(PCUBE_NOMEM, "pinned host allocation failed");
+    }
+    return PCUBE_OK;
+}
+
+int pcube_pinned_free(void* host) {
+    Entry lock;
+    if (!host) return PCUBE_OK;
+    // the stream may still copy from or into the block
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    HIP_TRY(hipHostFree(host));
+    return PCUBE_OK;
+}
+
+// Host window producer: every y = c + d - w*M + base with c in C, d in D (both
+// ascending, values in [0, M)), w in {0, 1} and lo <= c + d - w*M < hi, written
+// in ascending order.  The walk goes up C: for each c the admissible d form
+// the range [lo + w*M - c, hi + w*M - c), whose ends only move down as c
+// grows, so two cursors into D walk it once per w.
+int pcube_host_focus_window(const uint64_t* C, uint64_t nc, const uint64_t* D, uint64_t nd, uint64_t M, uint64_t lo,
+                            uint64_t hi, uint64_t base, uint64_t* out, uint64_t cap, uint64_t* count) {
+    if (!count || (nc && !C) || (nd && !D) || (cap && !out)) return fail(PCUBE_INVALID, "null host window buffer");
+    *count = 0;
+    if (lo > hi || hi > M || M >= (1ull << 63)) return fail(PCUBE_INVALID, "host window outside [0, M]");
+    uint64_t n = 0;
+    for (int w = 0; w < 2; ++w) {
+        const uint64_t a = lo + (w ? M : 0), b = hi + (w ? M : 0);
+        uint64_t dl = nd, dh = nd;   // first index of D at or above a - c, and b - c
+        for (uint64_t i = 0; i < nc; ++i) {
+            const uint64_t c = C[i];
+            if (c >= b) break;
+            const uint64_t la = a > c ? a - c : 0, lb = b - c;
+            while (dh > 0 && D[dh - 1] >= lb) --dh;
+            while (dl > 0 && D[dl - 1] >= la) --dl;
+            for (uint64_t k = dl; k < dh; ++k) {
+                if (n >= cap) {
+                    *count = n + 1;
+                    return fail(PCUBE_OVERFLOW, "host window output overflow");
+                }
+                out[n++] = c + D[k] - (w ? M : 0) + base;
+            }
+        }
+    }
+    std::sort(out, out + n);
+    *count = n;
+    return PCUBE_OK;
+}
+
 int pcube_focus_create(uint64_t modulus, const uint64_t* terms, const uint32_t* sizes, uint32_t nrings_c,
                        uint32_t nrings_d, int* focus, uint64_t* nc, uint64_t* nd) {
     Entry lock;

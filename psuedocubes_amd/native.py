@@ -34,6 +34,9 @@ SIGNATURES: dict[str, list[Any]] = {
     'pcube_free': [u64],
     'pcube_upload': [u64, ctypes.c_void_p, u64],
     'pcube_download': [ctypes.c_void_p, u64, u64],
+    'pcube_pinned_alloc': [u64, ctypes.POINTER(ctypes.c_void_p)],
+    'pcube_pinned_free': [ctypes.c_void_p],
+    'pcube_host_focus_window': [P_u64, u64, P_u64, u64, u64, u64, u64, u64, P_u64, u64, P_u64],
     'pcube_focus_create': [u64, P_u64, P_u32, u32, u32, P_int, P_u64, P_u64],
     'pcube_focus_arrays': [c_int, P_u64, P_u64],
     'pcube_focus_destroy': [c_int],
