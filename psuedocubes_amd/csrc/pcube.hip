@@ -798,7 +798,7 @@ int staged_enqueue(Matcher& mt, MatchArgs m, hipEvent_t ev_pre, hipEvent_t ev_ma
     pa.rows_blocks = (uint32_t)blocks_for(m.nv, 256);
     pa.chunks_total = mt.s_chunk_base[mt.sbits];
     pa.mask_blocks = m.nu ? (uint32_t)blocks_for((uint64_t)pa.chunks_total * m.nstage * kWave, 256) : 0;
-    pa.rho_blocks = m.nu && m.ncache ? (uint32_t)blocks_for((uint64_t)m.nu * kCachedResidues, 256) : 0;
+    pa.rho_blocks = m.nu && m.ncache ? (uint32_t)blocks_for((uint64_t)m.nu, 256) : 0;
     pa.npieces = npieces;
     pa.items_cap = items_cap;
     for (int k = 0; k < kStagedFilters; ++k) pa.chunk_base[k] = mt.s_chunk_base[k];
@@ -895,7 +895,7 @@ int match_enqueue(Matcher& mt, const uint64_t* us, uint64_t nu, const uint64_t* 
                            chunk_base[1], mt.nbits > 2 ? chunk_base[2] : 0xffffffffu,
                            mt.nbits > 3 ? chunk_base[3] : 0xffffffffu);
         if (mt.ncache)
-            hipLaunchKernelGGL(k_rho, dim3(blocks_for(nu * kCachedResidues, 256)), dim3(256), 0, g.stream, m);
+            hipLaunchKernelGGL(k_rho, dim3(blocks_for(nu, 256)), dim3(256), 0, g.stream, m);
     }
     if (ev_pre) HIP_TRY(hipEventRecord(ev_pre, g.stream));
     if (ev_k0) HIP_TRY(hipEventRecord(ev_k0, g.stream));

@@ -288,12 +288,12 @@ __global__ void k_searchsorted(const uint64_t* __restrict__ sorted, uint64_t n, 
 // Matching
 
 // Per-v row data computed once per batch.  The first 32 bytes are what the
-// bitset stage reads; sigma and v serve the residual chain.
+// bitset stage reads; v serves the residual chain.
 struct Row {
     uint32_t left, right;   // u index range [left, right) with lo < u*B - v*A <= hi
     uint32_t tile, flags;   // subtile index; bit0: a v outside [v_first, v_first + ntiles*width)
     uint16_t r[kStagedFilters];       // v mod q for the bitset filters (staged engine: 8 * line + column per read slot)
-    uint16_t sigma[kCachedResidues];  // (-v*A) mod q for the cached chain filters (staged engine)
+    uint16_t unused[kCachedResidues];
     uint64_t v;
     uint64_t pad;
 };
@@ -405,11 +405,10 @@ __device__ __forceinline__ void above_round(const MatchArgs& m, AboveSearch& s, 
 // final round over the last eight.  kTrack also returns u0 = us[r0] (when
 // r0 < nu) from the probes, at the cost of the registers that follow it.
 template <bool kTrack>
-__device__ __forceinline__ void first_above_pair(const MatchArgs& m, const U128& b0, const U128& b1,
+__device__ __forceinline__ void first_above_keys(const MatchArgs& m, const AboveKey& k0, const AboveKey& k1,
                                                  const uint64_t* dir, uint32_t ndir, uint32_t stride, uint32_t& r0,
                                                  uint32_t& r1, uint64_t& u0) {
-    AboveSearch s[2] = {{above_key(b0, m.B, m.B_mu), 0, m.nu, false, 0, 0},
-                        {above_key(b1, m.B, m.B_mu), 0, m.nu, false, 0, 0}};
+    AboveSearch s[2] = {{k0, 0, m.nu, false, 0, 0}, {k1, 0, m.nu, false, 0, 0}};
     if (dir) {
         uint32_t a[2] = {0, 0}, e[2] = {ndir, ndir};
         while (a[0] < e[0] || a[1] < e[1]) {
@@ -445,13 +444,26 @@ __device__ __forceinline__ void first_above_pair(const MatchArgs& m, const U128&
     if (kTrack) u0 = s[0].val;   // us[r0] when r0 < nu
 }
 
-// Exact u bounds of row j (coordinate v = vs[j]) by 128-bit binary search,
-// subtile, v mod q.  With a directory (kDir) the search also yields us[left];
-// without one, us[left] is loaded.
-template <bool kDir>
-__device__ __forceinline__ Row build_row(const MatchArgs& m, uint32_t j, uint64_t v, const uint64_t* dir = nullptr,
-                                         uint32_t ndir = 0, uint32_t stride = 0) {
+template <bool kTrack>
+__device__ __forceinline__ void first_above_pair(const MatchArgs& m, const U128& b0, const U128& b1,
+                                                 const uint64_t* dir, uint32_t ndir, uint32_t stride, uint32_t& r0,
+                                                 uint32_t& r1, uint64_t& u0) {
+    first_above_keys<kTrack>(m, above_key(b0, m.B, m.B_mu), above_key(b1, m.B, m.B_mu), dir, ndir, stride, r0, r1,
+                             u0);
+}
+
+// The parts of row j (coordinate v) that depend on v alone: flags and
+// subtile, the bitset offsets, v*A and the search keys of the row's two u
+// bounds.
+struct RowPrep {
     Row row;
+    U128 shift;
+    AboveKey lower, upper;
+};
+
+__device__ __forceinline__ RowPrep row_prep(const MatchArgs& m, uint32_t j, uint64_t v) {
+    RowPrep p;
+    Row& row = p.row;
     row.v = v;
     row.flags = 0;
     uint64_t tile = 0;
@@ -461,23 +473,12 @@ __device__ __forceinline__ Row build_row(const MatchArgs& m, uint32_t j, uint64_
         atomicOr(m.error, 1u);
     }
     row.tile = uint32_t(tile);
-    const U128 shift = mul_wide(v, m.A);
-    const U128 lower = add_wide(shift, m.lo), upper = add_wide(shift, m.hi);
-    // first index with u*B > bound, for the lower and the upper bound
-    uint32_t left, right;
-    uint64_t u_left = 0;
-    first_above_pair<kDir>(m, lower, upper, dir, ndir, stride, left, right, u_left);
-    row.left = left;
-    row.right = right;
-    if (!kDir && left < right) u_left = m.us[left];
-    if (row.flags) row.right = row.left;  // an invalid row contributes nothing
-    // u*B >= v*A holds for every u in range because u*B > v*A + lo; a violation
-    // (impossible for sorted input) is reported as the underflow counter.
-    if (row.left < row.right && less_wide(mul_wide(u_left, m.B), shift))
-        atomicAdd(&m.counters[(uint64_t)row.tile * (m.ng + 3) + m.ng + 2], 1ull);
-    // the staged engine keeps instead, per offset slot, 8 * line + column of
-    // the mask row its lane reads there: line v mod q of the slot's filter
-    // (line 0, all ones, for an absent filter).  Slot k of row j reads column
+    p.shift = mul_wide(v, m.A);
+    p.lower = above_key(add_wide(p.shift, m.lo), m.B, m.B_mu);
+    p.upper = above_key(add_wide(p.shift, m.hi), m.B, m.B_mu);
+    // the staged engine keeps, per offset slot, 8 * line + column of the mask
+    // row its lane reads there: line v mod q of the slot's filter (line 0, all
+    // ones, for an absent filter).  Slot k of row j reads column
     // stage_column(k, j) = k ^ x with x = (g & 3) | (g >> 3) << 2, g = j mod 16:
     // the eight entries are formed in column order (the moduli are the same
     // for every lane) and permuted by three conditional swaps.
@@ -503,18 +504,41 @@ __device__ __forceinline__ Row build_row(const MatchArgs& m, uint32_t j, uint64_
     }
 #pragma unroll
     for (uint32_t k = 0; k < (uint32_t)kStagedFilters; ++k) row.r[k] = uint16_t(res[k]);
-    for (uint32_t c = 0; c < kCachedResidues; ++c) {
-        uint16_t s = 0;
-        if (m.staged && c < m.ncache) {
-            const Modulus mod = m.mods[m.nbits + c];
-            const uint32_t r = mod32(mod64(v, mod) * mod.a, mod);
-            s = uint16_t(r ? mod.q - r : 0);
-        }
-        row.sigma[c] = s;
-    }
+#pragma unroll
+    for (uint32_t c = 0; c < (uint32_t)kCachedResidues; ++c) row.unused[c] = 0;
     row.pad = 0;
+    return p;
+}
+
+// The rest of row j: exact u bounds by 128-bit binary search, the underflow
+// check, and the record's store.  With a directory (kDir) the search also
+// yields us[left]; without one, us[left] is loaded.
+template <bool kDir>
+__device__ __forceinline__ Row row_finish(const MatchArgs& m, uint32_t j, RowPrep& p, const uint64_t* dir,
+                                          uint32_t ndir, uint32_t stride) {
+    Row& row = p.row;
+    // first index with u*B > bound, for the lower and the upper bound
+    uint32_t left, right;
+    uint64_t u_left = 0;
+    first_above_keys<kDir>(m, p.lower, p.upper, dir, ndir, stride, left, right, u_left);
+    row.left = left;
+    row.right = right;
+    if (!kDir && left < right) u_left = m.us[left];
+    if (row.flags) row.right = row.left;  // an invalid row contributes nothing
+    // u*B >= v*A holds for every u in range because u*B > v*A + lo; a violation
+    // (impossible for sorted input) is reported as the underflow counter.
+    if (row.left < row.right && less_wide(mul_wide(u_left, m.B), p.shift))
+        atomicAdd(&m.counters[(uint64_t)row.tile * (m.ng + 3) + m.ng + 2], 1ull);
     ((Row*)m.rows)[j] = row;
     return row;
+}
+
+// Row j (coordinate v = vs[j]): exact u bounds, subtile, v mod q.
+template <bool kDir>
+__device__ __forceinline__ Row build_row(const MatchArgs& m, uint32_t j, uint64_t v, const uint64_t* dir = nullptr,
+                                         uint32_t ndir = 0, uint32_t stride = 0) {
+    RowPrep p = row_prep(m, j, v);
+    return row_finish<kDir>(m, j, p, dir, ndir, stride);
 }
 
 __global__ void k_rows(MatchArgs m) {
@@ -561,14 +585,30 @@ __device__ __forceinline__ void load_directory(const MatchArgs& m, uint64_t* dir
 // v, so a wave's rows span few subtiles: one atomic per (wave, subtile).
 __device__ __forceinline__ void staged_rows_block(const MatchArgs& m, uint32_t block, uint64_t* dir) {
     const uint32_t j = block * 256 + threadIdx.x;
-    // issued before the directory's loads, so it arrives with them
+    // v first, then the directory's loads; the parts of the row that need v
+    // alone are computed while the directory is in flight
     const uint64_t v = j < m.nv ? m.vs[j] : 0ull;
     uint32_t ndir, stride;
-    load_directory(m, dir, ndir, stride);
+    directory_shape(m.nu, stride, ndir);
+    static_assert(kRowDirectory == 256 * 16, "one round of sixteen loads per thread fills the directory");
+    uint64_t d[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const uint32_t t = 256 * k + threadIdx.x;
+        d[k] = ndir ? m.us[(t < ndir ? t : ndir - 1) * stride] : 0ull;
+    }
+    RowPrep p;
+    if (j < m.nv) p = row_prep(m, j, v);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const uint32_t t = 256 * k + threadIdx.x;
+        if (t < ndir) dir[t] = d[k];
+    }
+    __syncthreads();
     uint32_t tile = 0xffffffffu;
     uint64_t count = 0;
     if (j < m.nv) {
-        const Row row = build_row<true>(m, j, v, ndir ? dir : nullptr, ndir, stride);
+        const Row row = row_finish<true>(m, j, p, ndir ? dir : nullptr, ndir, stride);
         if (!row.flags) {
             tile = row.tile;
             count = row.right - row.left;
@@ -647,20 +687,24 @@ __global__ __launch_bounds__(256) void k_masks(MatchArgs m, uint32_t chunks_tota
 
 // rho[i][c] = (u_i * B) mod q for the cached chain filters c = 0..ncache-1.
 // Rows are kCachedResidues entries wide (16 bytes) whatever ncache is, so the
-// chain reads one u coordinate's residues with a single vector load.
-__device__ __forceinline__ void build_rho(const MatchArgs& m, uint64_t t) {
-    const uint32_t i = uint32_t(t / kCachedResidues), c = uint32_t(t % kCachedResidues);
-    uint16_t r = 0;
-    if (c < m.ncache) {
-        const Modulus mod = m.mods[m.nbits + c];
-        r = uint16_t(mod32(mod64(m.us[i], mod) * mod.b, mod));
+// chain reads one u coordinate's residues with a single vector load.  One
+// thread builds u_i's whole row (the moduli are the same for every lane).
+__device__ __forceinline__ void build_rho(const MatchArgs& m, uint32_t i) {
+    const uint64_t u = m.us[i];
+    uint32_t packed[kCachedResidues / 2] = {};
+#pragma unroll
+    for (uint32_t c = 0; c < (uint32_t)kCachedResidues; ++c) {
+        if (c < m.ncache) {
+            const Modulus mod = m.mods[m.nbits + c];
+            packed[c / 2] |= mod32(mod64(u, mod) * mod.b, mod) << (16 * (c & 1));
+        }
     }
-    ((uint16_t*)m.rho)[t] = r;
+    ((uint4*)m.rho)[i] = make_uint4(packed[0], packed[1], packed[2], packed[3]);
 }
 
 __global__ void k_rho(MatchArgs m) {
-    uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (t < (uint64_t)m.nu * kCachedResidues) build_rho(m, t);
+    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i < m.nu) build_rho(m, (uint32_t)i);
 }
 
 constexpr int kMatchThreads = 256;

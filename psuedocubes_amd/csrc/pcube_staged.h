@@ -159,7 +159,10 @@ __device__ __forceinline__ void staged_chain(const MatchArgs& m, const Staged& S
         if ((uint32_t)c >= m.ncache) break;
         const Modulus mod = S.mods[k];
         if (alive) {
-            uint32_t r = ((words[c >> 1] >> (16 * (c & 1))) & 0xffffu) + row.sigma[c];
+            // (u*B - v*A) mod q = rho + sigma, sigma = (-v*A) mod q, formed here for
+            // the lanes still alive (the row records do not carry it)
+            const uint32_t va = mod32(mod64(row.v, mod) * mod.a, mod);
+            uint32_t r = ((words[c >> 1] >> (16 * (c & 1))) & 0xffffu) + (va ? mod.q - va : 0u);
             r = r >= mod.q ? r - mod.q : r;
             alive = allowed_bit(S.allow, mod.bit + r);
         }
@@ -290,17 +293,38 @@ __device__ __forceinline__ void piece_ends(const MatchArgs& m, uint32_t p, uint6
     last = m.vs[j1 - 1];
 }
 
-// The u indices [left0, right1) that the rows of a piece reach: rows are sorted
-// by v, so from the exact u bounds of the piece's first row's lower end and its
-// last row's upper end (coordinates v_first, v_last), computed as the row
-// builder computes them.  Both ends are nondecreasing over the pieces.
+// A range [left0, right1) of u indices holding every index the rows of a piece
+// reach: rows are sorted by v, so it holds the exact u bounds of the piece's
+// first row's lower end and its last row's upper end (coordinates v_first,
+// v_last).  With a directory (dir[k] = us[k * stride] in LDS) the ends come
+// from it alone, widened to the directory interval that holds each exact end
+// (no global probe); without one, they are exact.  Both ends are
+// nondecreasing over the pieces, and a piece's items may include a chunk
+// beyond each end that none of its rows touches (the match skips such units).
 __device__ __forceinline__ void piece_reach(const MatchArgs& m, uint64_t v_first, uint64_t v_last,
                                             const uint64_t* dir, uint32_t ndir, uint32_t stride, uint32_t& left0,
                                             uint32_t& right1) {
     const U128 lower = add_wide(mul_wide(v_first, m.A), m.lo);
     const U128 upper = add_wide(mul_wide(v_last, m.A), m.hi);
-    uint64_t unused;
-    first_above_pair<false>(m, lower, upper, dir, ndir, stride, left0, right1, unused);
+    if (!dir) {
+        uint64_t unused;
+        first_above_pair<false>(m, lower, upper, dir, ndir, stride, left0, right1, unused);
+        return;
+    }
+    // first directory entry above each bound: the exact end lies in
+    // ((a - 1) * stride, a * stride] (0 when a = 0)
+    const AboveKey kl = above_key(lower, m.B, m.B_mu), ku = above_key(upper, m.B, m.B_mu);
+    uint32_t al = 0, el = ndir, au = 0, eu = ndir;
+    while (al < el) {
+        const uint32_t mid = al + ((el - al) >> 1);
+        if (!below_key(dir[mid], kl)) el = mid; else al = mid + 1;
+    }
+    while (au < eu) {
+        const uint32_t mid = au + ((eu - au) >> 1);
+        if (!below_key(dir[mid], ku)) eu = mid; else au = mid + 1;
+    }
+    left0 = al == 0 ? 0u : (al - 1) * stride + 1;
+    right1 = (uint32_t)min((uint64_t)au * stride, (uint64_t)m.nu);
 }
 
 // First index in [0, n) whose bound (every second word of bounds, from `at`) is at
@@ -412,8 +436,8 @@ __global__ __launch_bounds__(256) void k_prefilter_staged(MatchArgs m, Prefilter
         return;
     }
     b -= pa.mask_blocks;
-    const uint64_t t = (uint64_t)b * 256 + threadIdx.x;
-    if (t < (uint64_t)m.nu * kCachedResidues) build_rho(m, t);
+    const uint64_t i = (uint64_t)b * 256 + threadIdx.x;
+    if (i < m.nu) build_rho(m, (uint32_t)i);
 }
 
 // Row header of one row of a work unit: (left, right, tile, flags) and the
