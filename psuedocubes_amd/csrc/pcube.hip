@@ -715,7 +715,6 @@ struct Matcher {
     uint32_t mask_rows = 0;
     // staged engine: bitset filters, their rows (sum of q), cached chain filters, LDS bytes
     uint32_t sbits = 0, slines = 0, sncache = 0;   // staged filters, lines of a chunk image (largest q)
-    uint32_t s_chunk_base[kStagedFilters + 1] = {};
     uint32_t s_pstr_off[kStagedFilters] = {}, s_nbeta[kStagedFilters] = {};
     std::vector<Modulus> mods;
     Buffer d_mods, d_allow, d_pstr;
@@ -796,12 +795,11 @@ int staged_enqueue(Matcher& mt, MatchArgs m, hipEvent_t ev_pre, hipEvent_t ev_ma
     PrefilterArgs pa;
     memset(&pa, 0, sizeof pa);
     pa.rows_blocks = (uint32_t)blocks_for(m.nv, 256);
-    pa.chunks_total = mt.s_chunk_base[mt.sbits];
-    pa.mask_blocks = m.nu ? (uint32_t)blocks_for((uint64_t)pa.chunks_total * m.nstage * kWave, 256) : 0;
+    // one wave per (stage chunk, bitset filter)
+    pa.mask_blocks = m.nu ? (uint32_t)blocks_for((uint64_t)mt.sbits * m.nstage * kWave, 256) : 0;
     pa.rho_blocks = m.nu && m.ncache ? (uint32_t)blocks_for((uint64_t)m.nu, 256) : 0;
     pa.npieces = npieces;
     pa.items_cap = items_cap;
-    for (int k = 0; k < kStagedFilters; ++k) pa.chunk_base[k] = mt.s_chunk_base[k];
     pa.items = items;
     pa.nitems = nitems;
     pa.reach = nitems + 2;
@@ -1849,11 +1847,9 @@ int pcube_matcher_create(uint64_t A, uint64_t B, uint64_t lo0, uint64_t lo1, uin
             const uint32_t res = (uint32_t)((md.q - (uint64_t)(j % md.q) * md.a % md.q) % md.q);
             if (allowed_residue(md.q, res)) pstr[o + j / 32] |= 1u << (j % 32);
         }
-        mt->s_chunk_base[k + 1] = mt->s_chunk_base[k] + (md.q + kWave - 1) / kWave;
         mt->slines = std::max(mt->slines, md.q);
         ++mt->sbits;
     }
-    for (uint32_t k = mt->sbits; k < (uint32_t)kStagedFilters; ++k) mt->s_chunk_base[k + 1] = mt->s_chunk_base[k];
     mt->sncache = std::min<uint32_t>(nres - mt->sbits, kCachedResidues);
     if (mt->sbits) {
         TRY(mt->d_pstr.ensure(pstr.size() * 4));

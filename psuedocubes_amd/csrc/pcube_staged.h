@@ -56,48 +56,64 @@ constexpr int kStageLines = 280;      // lines of a chunk image: the largest sta
 constexpr int kStageLineBytes = kStagedFilters * kStageWords * 8;   // 8 columns of 32 bytes
 static_assert(kStagedFilters == 8 && kStageLineBytes == 256, "the read rotation assumes 8 columns of 32 bytes");
 
-// Staged residue bitsets.  Wave (s, c) builds rows r0..r0+63 of one bitset
-// filter (lines r0..r0+63 of its column) for the kStageWords words of stage
-// chunk s.  Lane t takes the u at
-// index 64w + t.  With P[k] = allowed_q[(-k*A) mod q] (stored repeated, q + 64
-// bits) and k0 = (r0 + (u mod q) * nbeta) mod q, bit j of the 64-bit window
+static_assert((kStageLines + 64 + 31) / 32 + 2 <= kWave, "a permuted string takes one load per lane");
+
+// Staged residue bitsets.  Wave (s, k) builds every row of bitset filter k
+// (its column of the chunk image) for the kStageWords words of stage chunk s,
+// 64 rows at a time.  Lane t takes the u at index 64w + t and reduces it once:
+// with P[j] = allowed_q[(-j*A) mod q] (stored repeated, q + 64 bits) and
+// k0 = (r0 + (u mod q) * nbeta) mod q, bit j of the 64-bit window
 // P[k0 .. k0 + 63] is allowed_q[(u*B - (r0 + j)*A) mod q]: the column of this
 // u over rows r0..r0+63.  A wave transpose turns the 64 columns into the 64
-// row words.  chunk_base[k] = first 64-row chunk of filter k.
-__device__ __forceinline__ void staged_mask_wave(const MatchArgs& m, uint64_t g, uint32_t chunks_total,
-                                                 const uint32_t* base) {
+// row words.
+// Words of a staged filter's permuted string: q + 64 bits with q <= kStageLines,
+// plus the two words of margin the matcher stores after each.
+constexpr int kStrWords = (kStageLines + 64 + 31) / 32 + 2;
+
+__device__ __forceinline__ void staged_mask_wave(const MatchArgs& m, uint64_t g) {
+    // the filter's permuted string, read as 64-bit windows, per wave in LDS
+    __shared__ uint32_t strings[256 / kWave][kStrWords];
     const uint32_t lane = threadIdx.x & (kWave - 1);
-    if (g >= (uint64_t)chunks_total * m.nstage) return;
-    const uint32_t s = uint32_t(g / chunks_total), c = uint32_t(g % chunks_total);
-    uint32_t k = 0;
-    while (k + 1 < m.nbits && c >= base[k + 1]) ++k;
+    if (g >= (uint64_t)m.nbits * m.nstage) return;
+    const uint32_t s = uint32_t(g / m.nbits), k = uint32_t(g % m.nbits);
     const Modulus mod = m.mods[k];
-    const uint32_t r0 = (c - base[k]) * kWave;
-    const uint32_t rows = mod.q - r0 < (uint32_t)kWave ? mod.q - r0 : (uint32_t)kWave;
-    const uint32_t* P = m.pstr + m.pstr_off[k];
-    uint64_t out[kStageWords];
+    uint32_t* P = strings[threadIdx.x / kWave];
+    if (lane < (mod.q + 64 + 31) / 32 + 2) P[lane] = m.pstr[m.pstr_off[k] + lane];
+    wave_sync();
+    // (u mod q) * nbeta mod q of this lane's u in each word, once for all rows
+    uint32_t base[kStageWords];
+    bool valid[kStageWords];
 #pragma unroll
     for (int ww = 0; ww < kStageWords; ++ww) {
         const uint32_t i = (s * kStageWords + ww) * kWave + lane;
-        uint64_t col = 0;
-        if (i < m.nu) {
-            uint32_t k0 = mod32(mod64(m.us[i], mod) * m.nbeta[k], mod) + r0;
-            k0 = k0 >= mod.q ? k0 - mod.q : k0;
-            const uint32_t w0 = k0 >> 5, sh = k0 & 31;
-            const uint64_t low = ((uint64_t)P[w0 + 1] << 32) | P[w0];
-            col = (low >> sh) | (sh ? (uint64_t)P[w0 + 2] << (64 - sh) : 0ull);
-        }
-        out[ww] = wave_transpose(col, lane);
+        valid[ww] = i < m.nu;
+        base[ww] = valid[ww] ? mod32(mod64(m.us[i], mod) * m.nbeta[k], mod) : 0u;
     }
     uint64_t* image = (uint64_t*)m.masks + (uint64_t)s * m.stage_lines * (kStageLineBytes / 8);
-    if (lane < rows) {
-        uint4* dst = (uint4*)(image + (r0 + lane) * (kStageLineBytes / 8) + k * kStageWords);
-        dst[0] = make_uint4(uint32_t(out[0]), uint32_t(out[0] >> 32), uint32_t(out[1]), uint32_t(out[1] >> 32));
-        dst[1] = make_uint4(uint32_t(out[2]), uint32_t(out[2] >> 32), uint32_t(out[3]), uint32_t(out[3] >> 32));
+    for (uint32_t r0 = 0; r0 < mod.q; r0 += kWave) {
+        const uint32_t rows = mod.q - r0 < (uint32_t)kWave ? mod.q - r0 : (uint32_t)kWave;
+        uint64_t out[kStageWords];
+#pragma unroll
+        for (int ww = 0; ww < kStageWords; ++ww) {
+            uint64_t col = 0;
+            if (valid[ww]) {
+                uint32_t k0 = base[ww] + r0;
+                k0 = k0 >= mod.q ? k0 - mod.q : k0;
+                const uint32_t w0 = k0 >> 5, sh = k0 & 31;
+                const uint64_t low = ((uint64_t)P[w0 + 1] << 32) | P[w0];
+                col = (low >> sh) | (sh ? (uint64_t)P[w0 + 2] << (64 - sh) : 0ull);
+            }
+            out[ww] = wave_transpose(col, lane);
+        }
+        if (lane < rows) {
+            uint4* dst = (uint4*)(image + (r0 + lane) * (kStageLineBytes / 8) + k * kStageWords);
+            dst[0] = make_uint4(uint32_t(out[0]), uint32_t(out[0] >> 32), uint32_t(out[1]), uint32_t(out[1] >> 32));
+            dst[1] = make_uint4(uint32_t(out[2]), uint32_t(out[2] >> 32), uint32_t(out[3]), uint32_t(out[3] >> 32));
+        }
     }
     // the columns of absent filters (fewer than kStagedFilters staged) keep one
     // all-ones line 0, which their rows read
-    if (c == 0 && lane < kStagedFilters - m.nbits) {
+    if (k == 0 && lane < kStagedFilters - m.nbits) {
         uint4* dst = (uint4*)(image + (m.nbits + lane) * kStageWords);
         dst[0] = dst[1] = make_uint4(~0u, ~0u, ~0u, ~0u);
     }
@@ -401,9 +417,8 @@ __device__ __forceinline__ void staged_plan_block(const MatchArgs& m, uint32_t n
 
 // Geometry of the fused prefilter launch.
 struct PrefilterArgs {
-    uint32_t rows_blocks, mask_blocks, rho_blocks, chunks_total, npieces, items_cap;
+    uint32_t rows_blocks, mask_blocks, rho_blocks, npieces, items_cap;
     uint32_t reach_lds;  // the pieces' reach fits the dynamic LDS after the directory
-    uint32_t chunk_base[kStagedFilters];
     uint2* items;
     uint32_t* nitems;
     uint32_t* reach;     // [npieces][2]: the u indices each piece's rows reach (when not in LDS)
@@ -432,7 +447,7 @@ __global__ __launch_bounds__(256) void k_prefilter_staged(MatchArgs m, Prefilter
     }
     b -= pa.rows_blocks;
     if (b < pa.mask_blocks) {
-        staged_mask_wave(m, (uint64_t)b * (256 / kWave) + threadIdx.x / kWave, pa.chunks_total, pa.chunk_base);
+        staged_mask_wave(m, (uint64_t)b * (256 / kWave) + threadIdx.x / kWave);
         return;
     }
     b -= pa.mask_blocks;
