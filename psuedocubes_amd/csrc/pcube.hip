@@ -779,15 +779,18 @@ int staged_enqueue(Matcher& mt, MatchArgs m, hipEvent_t ev_pre, hipEvent_t ev_ma
     m.rho = mt.rho.as<uint16_t>();
     m.cand_max = mt.status.as<uint64_t>() + 3;
     m.item_cap = (uint32_t)mt.cand_cap;
-    // the work list of (piece, chunk) items and each piece's reach
+    // kStagedBlocksPerCU persistent blocks per CU, no more than there can be work units
     const uint32_t npieces = (m.nv + kPieceRows - 1) / kPieceRows;
     const uint32_t items_cap = npieces * m.nstage;   // staged_fits bounds it
-    TRY(mt.items.ensure(((uint64_t)items_cap + 1) * sizeof(uint2) + (uint64_t)npieces * 8));
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>((uint64_t)kStagedBlocksPerCU * g.compute_units,
+                                                         std::max<uint64_t>(1, (uint64_t)items_cap * kUnitsPerItem));
+    // the work list of (piece, chunk) items, its count, each piece's reach and the
+    // match blocks' runs
+    TRY(mt.items.ensure(((uint64_t)items_cap + 1) * sizeof(uint2) + (uint64_t)npieces * 8 +
+                        ((uint64_t)blocks + 1) * sizeof(uint2)));
     uint2* items = mt.items.as<uint2>();
     uint32_t* nitems = (uint32_t*)(items + items_cap);
-    // kStagedBlocksPerCU persistent blocks per CU, no more than there are work units
-    const uint32_t blocks = (uint32_t)std::min<uint64_t>((uint64_t)kStagedBlocksPerCU * g.compute_units,
-                                                         (uint64_t)items_cap * kUnitsPerItem);
+    uint2* runs = (uint2*)(nitems + 2 + 2 * (uint64_t)npieces);
     // each wave's survivor region: cand_cap survivor records
     TRY(mt.cand.ensure(std::max<uint64_t>(16, (uint64_t)blocks * kStagedWaves * mt.cand_cap * kRecordUnits * 16)));
     m.cand = mt.cand.as<uint32_t>();
@@ -804,6 +807,10 @@ int staged_enqueue(Matcher& mt, MatchArgs m, hipEvent_t ev_pre, hipEvent_t ev_ma
     pa.nitems = nitems;
     pa.reach = nitems + 2;
     pa.reach_lds = npieces <= kPlanLdsPieces;
+    pa.grid = blocks;
+    // with two blocks on every CU, the first-round blocks take the larger share
+    pa.first64 = blocks == (uint32_t)(kStagedBlocksPerCU * g.compute_units) ? kFirstRoundWeight : 64u;
+    pa.runs = runs;
     uint32_t dir_stride, dir_entries;
     directory_shape(m.nu, dir_stride, dir_entries);
     const uint32_t dynamic_lds = (std::max<uint32_t>(1, dir_entries) + (pa.reach_lds ? npieces : 0)) * 8;
@@ -814,12 +821,10 @@ int staged_enqueue(Matcher& mt, MatchArgs m, hipEvent_t ev_pre, hipEvent_t ev_ma
     // them and stands for its own timing events
     mt.kernel_span = ev_pre && ev_match;
     if (m.nu) {
-        // persistent blocks over the work units and the residual chain
+        // persistent blocks over the runs of work units the plan assigned, then the residual chain
         const uint2* items_arg = items;
-        const uint32_t* nitems_arg = nitems;
-        // with two blocks on every CU, the first-round blocks take the larger share
-        uint32_t first64 = blocks == (uint32_t)(kStagedBlocksPerCU * g.compute_units) ? kFirstRoundWeight : 64u;
-        void* args[] = {&m, &items_arg, (void*)&items_cap, &nitems_arg, &first64};
+        const uint2* runs_arg = runs;
+        void* args[] = {&m, &items_arg, &runs_arg};
         if (ev_k0 && !mt.kernel_span) HIP_TRY(hipEventRecord(ev_k0, g.stream));
         HIP_TRY(hipLaunchKernel(staged_kernel(mt.sbits, mt.nres), dim3(blocks), dim3(kStagedThreads), args, 0,
                                 g.stream));

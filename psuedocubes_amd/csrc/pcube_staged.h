@@ -45,9 +45,13 @@ namespace pcube {
 
 constexpr int kStageWords = 4;        // mask words (256 u coordinates) per stage chunk
 constexpr int kPieceRows = 1024;      // v rows per work item
+// Work units: unit x is rows 64 (x mod 16) .. 64 (x mod 16) + 63 of the piece of
+// item x / 16 against that item's chunk, so a block's contiguous run of units
+// follows the chunk-major item list.
+constexpr uint32_t kUnitsPerItem = kPieceRows / 64;
 constexpr int kStagedThreads = 512;   // threads of a persistent block
 constexpr int kStagedBlocksPerCU = 2; // persistent blocks per CU (4 waves per SIMD, 128 VGPRs)
-constexpr uint32_t kFirstRoundWeight = 72;   // a first-round block's share of units, in 1/64 of an average run (measured)
+constexpr uint32_t kFirstRoundWeight = 80;   // a first-round block's share of units, in 1/64 of an average run (measured)
 constexpr uint32_t kPlanLdsPieces = 1024;    // pieces whose reach the planner keeps in LDS
 constexpr int kStagedWaves = kStagedThreads / kWave;
 constexpr int kTileSlots = 64;        // subtiles whose group counts a block keeps in LDS
@@ -355,6 +359,20 @@ __device__ __forceinline__ uint32_t first_at_least(P bounds, uint32_t at, uint32
     return l;
 }
 
+// Share of the work before run r, in 1/64 of an average run, when the first-round
+// blocks (block id < grid / 2, the first to reach each CU) take first64 / 64 of
+// an average run and the others (128 - first64) / 64.  Runs are grouped by XCD
+// (run = (b mod 8) * grid / 8 + b / 8), so within each XCD's runs the first half
+// belongs to first-round blocks.  The two blocks of a CU then end together
+// although the older one's waves win the CU's issue arbitration.
+__device__ __forceinline__ uint64_t run_weight_prefix(uint32_t r, uint32_t grid, uint32_t first64) {
+    const uint32_t per = grid / 8;
+    if (grid % 16 != 0 || first64 == 64) return (uint64_t)r * 64;
+    const uint32_t x = r / per, k = r % per, half = per / 2;
+    const uint64_t within = k < half ? (uint64_t)k * first64 : (uint64_t)half * first64 + (uint64_t)(k - half) * (128 - first64);
+    return (uint64_t)x * per * 64 + within;
+}
+
 // Work list of the staged match, chunk-major: for every stage chunk c (u indices
 // [256c, 256c + 256)), the pieces whose rows reach it, in piece order; at most
 // items_cap items, nitems receives the full count.  With piece p reaching
@@ -367,15 +385,32 @@ __device__ __forceinline__ uint32_t first_at_least(P bounds, uint32_t at, uint32
 // in LDS after the directory for kLds, else global), then thread t takes a run
 // of consecutive chunks and a block scan turns the runs' item counts into
 // offsets.
+// Geometry of the fused prefilter launch and of the match that follows it.
+struct PrefilterArgs {
+    uint32_t rows_blocks, mask_blocks, rho_blocks, npieces, items_cap;
+    uint32_t reach_lds;  // the pieces' reach fits the dynamic LDS after the directory
+    uint32_t grid;       // blocks of the staged match
+    uint32_t first64;    // a first-round match block's share of the work (1/64 of an average run)
+    uint2* items;
+    uint32_t* nitems;
+    uint32_t* reach;     // [npieces][2]: the u indices each piece's rows reach (when not in LDS)
+    uint2* runs;         // [grid + 1]: first work unit of each match block's run and that unit's chunk
+};
+
+// Cost of a chunk change inside a match block's run, in work units: the block
+// waits for the next chunk image about as long as its waves take for this
+// many units (measured).
+constexpr uint32_t kChunkChangeUnits = 16;
+
 template <bool kLds>
-__device__ __forceinline__ void staged_plan_block(const MatchArgs& m, uint32_t npieces, uint2* items,
-                                                  uint32_t items_cap, uint32_t* nitems, uint32_t* reach_global,
-                                                  uint64_t* dir) {
-    __shared__ uint32_t wave_total[256 / kWave];
+__device__ __forceinline__ void staged_plan_block(const MatchArgs& m, const PrefilterArgs& pa, uint64_t* dir) {
+    __shared__ uint32_t wave_items[256 / kWave], wave_cost[256 / kWave];
+    __shared__ uint32_t items_before[257], cost_before[257];   // per thread's chunk run; [256]: the totals
     const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const uint32_t npieces = pa.npieces;
     uint32_t ndir, stride;
     load_directory(m, dir, ndir, stride);
-    uint32_t* reach = kLds ? (uint32_t*)(dir + (ndir ? ndir : 1)) : reach_global;
+    uint32_t* reach = kLds ? (uint32_t*)(dir + (ndir ? ndir : 1)) : pa.reach;
     const uint64_t* d = ndir ? dir : nullptr;
     for (uint32_t p = threadIdx.x; p < npieces; p += 256) {
         uint64_t v_first, v_last;
@@ -390,39 +425,88 @@ __device__ __forceinline__ void staged_plan_block(const MatchArgs& m, uint32_t n
     constexpr uint32_t kChunkU = kStageWords * 64;
     const uint32_t per = (m.nstage + 255) / 256;
     const uint32_t c0 = min(m.nstage, threadIdx.x * per), c1 = min(m.nstage, c0 + per);
-    uint32_t count = 0;
+    // items of this thread's chunks, and their cost: each nonempty chunk's units
+    // plus one chunk change
+    uint32_t count = 0, cost = 0;
     for (uint32_t c = c0; c < c1; ++c) {
         const uint32_t a = first_at_least(reach, 1, npieces, c * kChunkU + 1);   // right1 > 256c
         const uint32_t b = first_at_least(reach, 0, npieces, (c + 1) * kChunkU);  // left0 >= 256c + 256
-        if (b > a) count += b - a;
+        if (b > a) {
+            count += b - a;
+            cost += kChunkChangeUnits + kUnitsPerItem * (b - a);
+        }
     }
-    uint32_t inc = count;
+    uint32_t inc = count, inc_cost = cost;
 #pragma unroll
     for (int dd = 1; dd < kWave; dd <<= 1) {
-        const uint32_t y = __shfl_up(inc, dd, kWave);
-        if (lane >= (uint32_t)dd) inc += y;
+        const uint32_t y = __shfl_up(inc, dd, kWave), z = __shfl_up(inc_cost, dd, kWave);
+        if (lane >= (uint32_t)dd) {
+            inc += y;
+            inc_cost += z;
+        }
     }
-    if (lane == kWave - 1) wave_total[wave] = inc;
+    if (lane == kWave - 1) {
+        wave_items[wave] = inc;
+        wave_cost[wave] = inc_cost;
+    }
     __syncthreads();
-    uint32_t at = inc - count;
-    for (uint32_t w = 0; w < wave; ++w) at += wave_total[w];
+    uint32_t at = inc - count, kat = inc_cost - cost;
+    for (uint32_t w = 0; w < wave; ++w) {
+        at += wave_items[w];
+        kat += wave_cost[w];
+    }
+    items_before[threadIdx.x] = at;
+    cost_before[threadIdx.x] = kat;
+    if (threadIdx.x == 255) {
+        items_before[256] = at + count;
+        cost_before[256] = kat + cost;
+        *pa.nitems = at + count;
+    }
     for (uint32_t c = c0; c < c1; ++c) {
         const uint32_t a = first_at_least(reach, 1, npieces, c * kChunkU + 1);
         const uint32_t b = first_at_least(reach, 0, npieces, (c + 1) * kChunkU);
         for (uint32_t q = a; q < b; ++q, ++at)
-            if (at < items_cap) items[at] = make_uint2(q, c);
+            if (at < pa.items_cap) pa.items[at] = make_uint2(q, c);
     }
-    if (threadIdx.x == 255) *nitems = wave_total[0] + wave_total[1] + wave_total[2] + wave_total[3];
+    __syncthreads();
+    // the match blocks' runs: run r starts at the unit where the cost reaches its
+    // share (run_weight_prefix) of the total; a chunk change costs
+    // kChunkChangeUnits before the chunk's first unit
+    const uint32_t total_units = min(items_before[256], pa.items_cap) * kUnitsPerItem;
+    const uint64_t total_cost = cost_before[256];
+    for (uint32_t r = threadIdx.x; r <= pa.grid; r += 256) {
+        uint32_t u = total_units, chunk = 0;
+        if (r < pa.grid && total_cost) {
+            const uint64_t target = total_cost * run_weight_prefix(r, pa.grid, pa.first64) / ((uint64_t)pa.grid * 64);
+            // the last thread whose chunks' cost starts at or below the target
+            uint32_t lo = 0, hi = 256;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1) >> 1;
+                if (cost_before[mid] <= target) lo = mid; else hi = mid - 1;
+            }
+            const uint32_t t0 = min(m.nstage, lo * per), t1 = min(m.nstage, t0 + per);
+            uint64_t k = cost_before[lo];
+            uint32_t unit = items_before[lo] * kUnitsPerItem;
+            for (uint32_t c = t0; c < t1; ++c) {
+                const uint32_t a = first_at_least(reach, 1, npieces, c * kChunkU + 1);
+                const uint32_t b = first_at_least(reach, 0, npieces, (c + 1) * kChunkU);
+                if (b <= a) continue;
+                const uint32_t units = kUnitsPerItem * (b - a);
+                if (target < k + kChunkChangeUnits + units) {
+                    const uint64_t off = target - k;
+                    u = unit + (off < kChunkChangeUnits ? 0u : (uint32_t)(off - kChunkChangeUnits));
+                    chunk = c;
+                    break;
+                }
+                k += kChunkChangeUnits + units;
+                unit += units;
+            }
+            u = min(u, total_units);
+        }
+        pa.runs[r] = make_uint2(u, chunk);
+    }
 }
 
-// Geometry of the fused prefilter launch.
-struct PrefilterArgs {
-    uint32_t rows_blocks, mask_blocks, rho_blocks, npieces, items_cap;
-    uint32_t reach_lds;  // the pieces' reach fits the dynamic LDS after the directory
-    uint2* items;
-    uint32_t* nitems;
-    uint32_t* reach;     // [npieces][2]: the u indices each piece's rows reach (when not in LDS)
-};
 
 // Everything the staged match needs, in one launch of independent 256-thread
 // blocks: block 0 plans the work items; then the row blocks (exact u bounds,
@@ -436,8 +520,8 @@ __global__ __launch_bounds__(256) void k_prefilter_staged(MatchArgs m, Prefilter
     uint64_t* dir = prefilter_dir;
     uint32_t b = blockIdx.x;
     if (b == 0) {
-        if (pa.reach_lds) staged_plan_block<true>(m, pa.npieces, pa.items, pa.items_cap, pa.nitems, pa.reach, dir);
-        else staged_plan_block<false>(m, pa.npieces, pa.items, pa.items_cap, pa.nitems, pa.reach, dir);
+        if (pa.reach_lds) staged_plan_block<true>(m, pa, dir);
+        else staged_plan_block<false>(m, pa, dir);
         return;
     }
     b -= 1;
@@ -506,10 +590,6 @@ static_assert(kMaxAllowWords * 4 + kMaxResiduals * sizeof(Modulus) + kTileSlots 
                   kStageBufferUnits * 16,
               "the chain's filter tables and group counts fit the mask buffer");
 
-// Work units: unit x is rows 64 (x mod 16) .. 64 (x mod 16) + 63 of the piece of
-// item x / 16 against that item's chunk, so a block's contiguous run of units
-// follows the chunk-major item list.
-constexpr int kUnitsPerItem = kPieceRows / kWave;
 
 // A wave's view of the item list: lane l holds the descriptor (piece, chunk) of
 // item window + l, so a descriptor is a register read.
@@ -718,20 +798,6 @@ __device__ __forceinline__ void acc_add(const MatchArgs& m, const Staged& S, Gro
     A.c2 += c2;
 }
 
-// Units before run r, in 1/64 units of an average run, when the first-round
-// blocks (block id < grid / 2, the first to reach each CU) take first64 / 64 of
-// an average run and the others (128 - first64) / 64.  Runs are grouped by XCD
-// (run = (b mod 8) * grid / 8 + b / 8), so within each XCD's runs the first half
-// belongs to first-round blocks.  The two blocks of a CU then end together
-// although the older one's waves win the CU's issue arbitration.
-__device__ __forceinline__ uint64_t run_weight_prefix(uint32_t r, uint32_t grid, uint32_t first64) {
-    const uint32_t per = grid / 8;
-    if (grid % 16 != 0 || first64 == 64) return (uint64_t)r * 64;
-    const uint32_t x = r / per, k = r % per, half = per / 2;
-    const uint64_t within = k < half ? (uint64_t)k * first64 : (uint64_t)half * first64 + (uint64_t)(k - half) * (128 - first64);
-    return (uint64_t)x * per * 64 + within;
-}
-
 // Persistent staged match, kStagedBlocksPerCU blocks per CU.  The work units
 // (16 per item of the chunk-major list) are split into one contiguous run per
 // block (weighted by dispatch round, run_weight_prefix); the runs of one XCD's
@@ -749,26 +815,25 @@ __device__ __forceinline__ uint64_t run_weight_prefix(uint32_t r, uint32_t grid,
 // stay resident.
 template <int kF, bool kLast>
 __global__ __launch_bounds__(kStagedThreads) __attribute__((amdgpu_waves_per_eu(4)))
-void k_match_staged(MatchArgs m, const uint2* items, uint32_t items_cap, const uint32_t* nitems, uint32_t first64) {
+void k_match_staged(MatchArgs m, const uint2* items, const uint2* runs) {
     __shared__ uint4 image[kStageBufferUnits];
     __shared__ uint32_t tilecnt[kTileSlots * kItemGroups];
     __shared__ uint2 queues[kStagedWaves * kQueue];
     // the wave index is the same in every lane: held in a scalar register, the
     // unit loop's control is scalar
     const uint32_t lane = threadIdx.x & (kWave - 1), wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const uint32_t total_items = min(*nitems, items_cap);
-    const uint32_t total = total_items * kUnitsPerItem;
     // consecutive block ids go round robin over the 8 XCDs: group the runs by XCD
     const uint32_t grid = gridDim.x;
     const uint32_t run = grid % 8 == 0 ? (blockIdx.x % 8) * (grid / 8) + blockIdx.x / 8 : blockIdx.x;
-    const uint32_t u_first = (uint32_t)(total * run_weight_prefix(run, grid, first64) / ((uint64_t)grid * 64));
-    const uint32_t u_last = (uint32_t)(total * run_weight_prefix(run + 1, grid, first64) / ((uint64_t)grid * 64));
+    const uint2 mine = runs[run], next = runs[run + 1];
+    const uint32_t u_first = mine.x, u_last = next.x;
     if (u_first >= u_last) return;
-    ItemWindow iw;
-    iw.count = total_items;
-    load_window(items, iw, u_first / kUnitsPerItem, lane);
-    uint32_t chunk = item_at(items, iw, u_first / kUnitsPerItem, lane).y;
+    uint32_t chunk = mine.y;
+    // the first chunk image streams in while the item descriptors load
     stage_async(m, chunk, (uint8_t*)image, wave, lane);
+    ItemWindow iw;
+    iw.count = runs[grid].x / kUnitsPerItem;
+    load_window(items, iw, u_first / kUnitsPerItem, lane);
     for (uint32_t t = threadIdx.x; t < (uint32_t)(kTileSlots * kItemGroups); t += kStagedThreads) tilecnt[t] = 0;
     const Staged S{nullptr, nullptr, tilecnt, kItemGroups, queues + wave * kQueue};
     uint4* region = (uint4*)m.cand + ((uint64_t)blockIdx.x * kStagedWaves + wave) * m.item_cap * kRecordUnits;
@@ -930,8 +995,8 @@ __global__ __launch_bounds__(256) void k_valu_probe(uint32_t rounds, uint32_t* o
 }
 
 #define PCUBE_STAGED(F)                                                                               \
-    template __global__ void k_match_staged<F, false>(MatchArgs, const uint2*, uint32_t, const uint32_t*, uint32_t); \
-    template __global__ void k_match_staged<F, true>(MatchArgs, const uint2*, uint32_t, const uint32_t*, uint32_t);
+    template __global__ void k_match_staged<F, false>(MatchArgs, const uint2*, const uint2*); \
+    template __global__ void k_match_staged<F, true>(MatchArgs, const uint2*, const uint2*);
 PCUBE_STAGED(1)
 PCUBE_STAGED(2)
 PCUBE_STAGED(3)
