@@ -292,8 +292,7 @@ __global__ void k_searchsorted(const uint64_t* __restrict__ sorted, uint64_t n, 
 struct Row {
     uint32_t left, right;   // u index range [left, right) with lo < u*B - v*A <= hi
     uint32_t tile, flags;   // subtile index; bit0: a v outside [v_first, v_first + ntiles*width)
-    uint16_t r[kStagedFilters];       // v mod q for the bitset filters (staged engine: 8 * line + column per read slot)
-    uint16_t unused[kCachedResidues];
+    uint32_t r[kStagedFilters];       // v mod q for the bitset filters (staged engine: LDS byte offset per read slot)
     uint64_t v;
     uint64_t pad;
 };
@@ -476,12 +475,14 @@ __device__ __forceinline__ RowPrep row_prep(const MatchArgs& m, uint32_t j, uint
     p.shift = mul_wide(v, m.A);
     p.lower = above_key(add_wide(p.shift, m.lo), m.B, m.B_mu);
     p.upper = above_key(add_wide(p.shift, m.hi), m.B, m.B_mu);
-    // the staged engine keeps, per offset slot, 8 * line + column of the mask
-    // row its lane reads there: line v mod q of the slot's filter (line 0, all
-    // ones, for an absent filter).  Slot k of row j reads column
-    // stage_column(k, j) = k ^ x with x = (g & 3) | (g >> 3) << 2, g = j mod 16:
-    // the eight entries are formed in column order (the moduli are the same
-    // for every lane) and permuted by three conditional swaps.
+    // the staged engine keeps, per offset slot, the chunk-image byte offset of
+    // the mask half its lane reads first there: 256 * line + 32 * column + 16 *
+    // P, with line v mod q of the slot's filter (line 0, all ones, for an
+    // absent filter) and P = (g >> 2) & 1 the lane's first half; the other
+    // half is the neighbouring 16 bytes (offset xor 16).  Slot k of row j reads
+    // column stage_column(k, j) = k ^ x with x = (g & 3) | (g >> 3) << 2,
+    // g = j mod 16: the eight entries are formed in column order (the moduli
+    // are the same for every lane) and permuted by three conditional swaps.
     uint32_t res[kStagedFilters];
 #pragma unroll
     for (uint32_t f = 0; f < (uint32_t)kStagedFilters; ++f) {
@@ -489,7 +490,7 @@ __device__ __forceinline__ RowPrep row_prep(const MatchArgs& m, uint32_t j, uint
         res[f] = m.staged ? r * kStagedFilters + f : r;
     }
     if (m.staged) {
-        const uint32_t g = j & 15, x = (g & 3) | ((g >> 3) << 2);
+        const uint32_t g = j & 15, x = (g & 3) | ((g >> 3) << 2), half = (g >> 2) & 1;
 #pragma unroll
         for (uint32_t bit = 1; bit < (uint32_t)kStagedFilters; bit <<= 1) {
             const bool swap = (x & bit) != 0;
@@ -501,11 +502,11 @@ __device__ __forceinline__ RowPrep row_prep(const MatchArgs& m, uint32_t j, uint
                 res[k | bit] = swap ? a : b;
             }
         }
+#pragma unroll
+        for (uint32_t k = 0; k < (uint32_t)kStagedFilters; ++k) res[k] = 32 * res[k] + 16 * half;
     }
 #pragma unroll
-    for (uint32_t k = 0; k < (uint32_t)kStagedFilters; ++k) row.r[k] = uint16_t(res[k]);
-#pragma unroll
-    for (uint32_t c = 0; c < (uint32_t)kCachedResidues; ++c) row.unused[c] = 0;
+    for (uint32_t k = 0; k < (uint32_t)kStagedFilters; ++k) row.r[k] = res[k];
     row.pad = 0;
     return p;
 }

@@ -539,10 +539,11 @@ __global__ __launch_bounds__(256) void k_prefilter_staged(MatchArgs m, Prefilter
     if (i < m.nu) build_rho(m, (uint32_t)i);
 }
 
-// Row header of one row of a work unit: (left, right, tile, flags) and the
-// offsets (8 * line + column) of the mask rows it reads.
+// Row header of one row of a work unit: (left, right, tile) and the LDS byte
+// offsets of the mask halves it reads first (h1: slots 0-3, h2: slots 4-7).
 struct UnitRow {
-    uint4 h0, h1;
+    uint3 h0;
+    uint4 h1, h2;
 };
 
 // Makes the compiler wait for a unit's row header here (an empty statement that
@@ -551,7 +552,8 @@ struct UnitRow {
 // in order, so without it the first use of this header would also wait for
 // those loads.
 __device__ __forceinline__ void unit_row_ready(const UnitRow& R) {
-    asm volatile("" ::"v"(R.h0.x), "v"(R.h0.y), "v"(R.h0.z), "v"(R.h1.x), "v"(R.h1.y), "v"(R.h1.z), "v"(R.h1.w));
+    asm volatile("" ::"v"(R.h0.x), "v"(R.h0.y), "v"(R.h0.z), "v"(R.h1.x), "v"(R.h1.y), "v"(R.h1.z), "v"(R.h1.w),
+                 "v"(R.h2.x), "v"(R.h2.y), "v"(R.h2.z), "v"(R.h2.w));
 }
 
 // Unit s of piece p is rows j0 + 64 s .. j0 + 64 s + 63 of the piece (j0 =
@@ -561,8 +563,9 @@ __device__ __forceinline__ void load_unit_row(const MatchArgs& m, uint32_t p, ui
     const uint32_t j0 = p * kPieceRows, j1 = min(m.nv, j0 + kPieceRows);
     const uint32_t j = j0 + s * kWave + lane;
     const uint4* rp = (const uint4*)&m.rows[j < j1 ? j : j1 - 1];
-    R.h0 = rp[0];
+    R.h0 = *(const uint3*)rp;
     R.h1 = rp[1];
+    R.h2 = rp[2];
 }
 
 // Start the copy of stage chunk s into an LDS buffer: each wave moves 64
@@ -677,25 +680,23 @@ __device__ __forceinline__ void flush_groups(const MatchArgs& m, const Staged& S
 // chunk-relative u range [dl, dr) (some row of the wave covers the chunk only in
 // part); without it every position of the chunk is in range.
 template <int kF, bool kLast, bool kRange>
-__device__ __forceinline__ void staged_row(const uint8_t* masks_p, const uint8_t* masks_q, int first_p, int first_q,
-                                           uint32_t g, const uint4& h1, int dl, int dr, uint32_t& c1, uint32_t& c2,
-                                           uint64_t (&words)[4]) {
+__device__ __forceinline__ void staged_row(const uint8_t* image, int first_p, int first_q, uint32_t g, const UnitRow& R,
+                                           int dl, int dr, uint32_t& c1, uint32_t& c2, uint64_t (&words)[4]) {
     // group 1 closes after filter 3 (or after the last one), group 2 after filter 7
     constexpr bool kCount1 = kF >= 4 || kLast;
     constexpr bool kCount2 = kF == 8 || (kLast && kF > 4);
-    const uint32_t offs[kStagedFilters] = {h1.x & 0xffffu, h1.x >> 16, h1.y & 0xffffu, h1.y >> 16,
-                                           h1.z & 0xffffu, h1.z >> 16, h1.w & 0xffffu, h1.w >> 16};
+    // byte offsets of half P of each slot; half Q is the other 16 bytes of the column
+    const uint32_t offs[kStagedFilters] = {R.h1.x, R.h1.y, R.h1.z, R.h1.w, R.h2.x, R.h2.y, R.h2.z, R.h2.w};
     uint64_t both[2][2];
     uint32_t acc_x = 0, acc_y = 0, acc2 = 0;
     // half P of all eight slots, then half Q: the eight reads of a half are in
     // flight together, and a half is reduced to its counts and survivors at once
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-        const uint8_t* base = h == 0 ? masks_p : masks_q;
         const int first = h == 0 ? first_p : first_q;
         uint4 v[kStagedFilters];
 #pragma unroll
-        for (int i = 0; i < kStagedFilters; ++i) v[i] = *(const uint4*)(base + (offs[i] << 5));
+        for (int i = 0; i < kStagedFilters; ++i) v[i] = *(const uint4*)(image + (h == 0 ? offs[i] : offs[i] ^ 16u));
         const uint4 x01 = and3(v[0], v[1], v[2]), y01 = and3(v[4], v[5], v[6]);
         const uint4 xh = make_uint4(x01.x & v[3].x, x01.y & v[3].y, x01.z & v[3].z, x01.w & v[3].w);
         const uint4 yh = make_uint4(y01.x & v[7].x, y01.y & v[7].y, y01.z & v[7].z, y01.w & v[7].w);
@@ -748,15 +749,13 @@ __device__ __forceinline__ void staged_unit(const MatchArgs& m, uint32_t p, uint
     const bool touches = live && dr > 0 && dl < kChunkU;
     const bool full = touches && dl <= 0 && dr >= kChunkU;
     if (!__ballot(touches)) return;
-    const uint32_t g = lane & 15, hp = (g >> 2) & 1;
-    const uint8_t* masks_p = (const uint8_t*)image + 16 * hp;          // half P of a column
-    const uint8_t* masks_q = (const uint8_t*)image + 16 * (hp ^ 1);    // half Q
+    const uint32_t g = lane & 15, hp = (g >> 2) & 1;                  // hp: the lane's half P of a column
     const int first_p = 128 * (int)hp, first_q = 128 - first_p;        // chunk offset of each half's first word
     uint64_t words[4];
     if (!__ballot(!full))
-        staged_row<kF, kLast, false>(masks_p, masks_q, first_p, first_q, g, R.h1, dl, dr, c1, c2, words);
+        staged_row<kF, kLast, false>((const uint8_t*)image, first_p, first_q, g, R, dl, dr, c1, c2, words);
     else
-        staged_row<kF, kLast, true>(masks_p, masks_q, first_p, first_q, g, R.h1, dl, dr, c1, c2, words);
+        staged_row<kF, kLast, true>((const uint8_t*)image, first_p, first_q, g, R, dl, dr, c1, c2, words);
     write_records(words, j, base0 + first_p, base0 + first_q, region, m.item_cap, fill);
 }
 
