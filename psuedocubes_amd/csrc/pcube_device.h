@@ -628,23 +628,70 @@ __device__ __forceinline__ void staged_rows_block(const MatchArgs& m, uint32_t b
     }
 }
 
-// One butterfly stage of a 64 x 64 bit transpose across a wave.
-template <int kB, uint64_t kMask>
-__device__ __forceinline__ uint64_t transpose_step(uint64_t x, uint32_t lane) {
-    const uint64_t y = __shfl_xor(x, kB, kWave);
-    return (lane & kB) ? ((x & kMask) | ((y & kMask) >> kB)) : ((x & ~kMask) | ((y & ~kMask) << kB));
+// Lane-exchange helpers of the 64 x 64 bit wave transpose.  DPP moves within
+// 16-lane rows (every source lane exists for these patterns):
+//   row_ror:8 -> lane ^ 8; row_half_mirror then quad_perm [3,2,1,0] -> lane ^ 4;
+//   quad_perm [2,3,0,1] -> lane ^ 2; quad_perm [1,0,3,2] -> lane ^ 1.
+template <int kCtrl>
+__device__ __forceinline__ uint32_t dpp_move(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, kCtrl, 0xf, 0xf, true);
+}
+
+template <int kB>
+__device__ __forceinline__ uint32_t lane_xor(uint32_t x) {
+    if constexpr (kB == 8) return dpp_move<0x128>(x);
+    else if constexpr (kB == 4) return dpp_move<0x1b>(dpp_move<0x141>(x));
+    else if constexpr (kB == 2) return dpp_move<0x4e>(x);
+    else return dpp_move<0xb1>(x);
+}
+
+// Transpose stage kB < 16 on one 32-bit half (its 2kB-bit groups lie inside
+// the half): y is the half of lane ^ kB.  A lane with bit kB clear keeps the
+// low kB bits of each group and takes the partner's low bits into the high
+// ones; a lane with it set keeps the high bits and takes the partner's high
+// bits into the low ones.  One rotation of y and one bit select.
+template <int kB, uint32_t kHigh>
+__device__ __forceinline__ uint32_t transpose_merge(uint32_t x, uint32_t lane) {
+    const uint32_t y = lane_xor<kB>(x);
+    const bool up = (lane & kB) == 0;
+    const uint32_t s = __builtin_amdgcn_alignbit(y, y, up ? 32 - kB : kB);   // y rotated toward the kept blocks
+    const uint32_t take = up ? kHigh : ~kHigh;                               // bits that come from the partner
+    return (s & take) | (x & ~take);
 }
 
 // Lane t holds column t of a 64 x 64 bit matrix (bit k = element (k, t));
-// returns row `lane` (bit t = element (lane, t)).
+// returns row `lane` (bit t = element (lane, t)).  Each stage exchanges one
+// bit of the row index with the same bit of the lane index: the 32-bit
+// halves across lanes 32 apart (v_permlane32_swap: lanes 32-63 of the first
+// operand trade with lanes 0-31 of the second), the 16-bit quarters across
+// lanes 16 apart (v_permlane16_swap: odd 16-lane rows of the first operand
+// trade with even rows of the second, then a byte permute), the smaller
+// blocks by DPP.  No LDS traffic and no branches.
 __device__ __forceinline__ uint64_t wave_transpose(uint64_t x, uint32_t lane) {
-    x = transpose_step<32, 0xffffffff00000000ull>(x, lane);
-    x = transpose_step<16, 0xffff0000ffff0000ull>(x, lane);
-    x = transpose_step<8, 0xff00ff00ff00ff00ull>(x, lane);
-    x = transpose_step<4, 0xf0f0f0f0f0f0f0f0ull>(x, lane);
-    x = transpose_step<2, 0xccccccccccccccccull>(x, lane);
-    x = transpose_step<1, 0xaaaaaaaaaaaaaaaaull>(x, lane);
-    return x;
+    uint32_t lo = uint32_t(x), hi = uint32_t(x >> 32);
+    {
+        const auto r = __builtin_amdgcn_permlane32_swap(lo, hi, false, false);
+        lo = r[0];
+        hi = r[1];
+    }
+    {
+        // per half: the lane with bit 16 clear ends with its own low quarter
+        // and the partner's low quarter above it, the other lane with both
+        // high quarters, low first; the rotated copy supplies the latter
+        const auto a = __builtin_amdgcn_permlane16_swap(lo, __builtin_amdgcn_alignbit(lo, lo, 16), false, false);
+        const auto b = __builtin_amdgcn_permlane16_swap(hi, __builtin_amdgcn_alignbit(hi, hi, 16), false, false);
+        lo = __builtin_amdgcn_perm(a[1], a[0], 0x05040100u);
+        hi = __builtin_amdgcn_perm(b[1], b[0], 0x05040100u);
+    }
+    lo = transpose_merge<8, 0xff00ff00u>(lo, lane);
+    hi = transpose_merge<8, 0xff00ff00u>(hi, lane);
+    lo = transpose_merge<4, 0xf0f0f0f0u>(lo, lane);
+    hi = transpose_merge<4, 0xf0f0f0f0u>(hi, lane);
+    lo = transpose_merge<2, 0xccccccccu>(lo, lane);
+    hi = transpose_merge<2, 0xccccccccu>(hi, lane);
+    lo = transpose_merge<1, 0xaaaaaaaau>(lo, lane);
+    hi = transpose_merge<1, 0xaaaaaaaau>(hi, lane);
+    return ((uint64_t)hi << 32) | lo;
 }
 
 // One 64-bit word of the residue bitsets: wave builds 64 rows (residues
@@ -1050,15 +1097,15 @@ __device__ __forceinline__ void build_mask_filter_run(const MatchArgs& m, const 
         uint64_t row[kMaskRun];
 #pragma unroll
         for (uint32_t ww = 0; ww < kMaskRun; ++ww) {
-            uint64_t col = 0;
-            if (live[ww]) {
-                uint32_t k0 = beta[ww] + r0;
-                k0 = k0 >= mod.q ? k0 - mod.q : k0;
-                const uint32_t a = k0 >> 5, sh = k0 & 31;
-                const uint64_t low = ((uint64_t)P[a + 1] << 32) | P[a];
-                col = (low >> sh) | (sh ? (uint64_t)P[a + 2] << (64 - sh) : 0ull);
-            }
-            row[ww] = wave_transpose(col, lane);
+            // bits k0 .. k0 + 63 of P by two funnel shifts (k0 < q: the three
+            // words lie in the string and its margin)
+            uint32_t k0 = beta[ww] + r0;
+            k0 = k0 >= mod.q ? k0 - mod.q : k0;
+            const uint32_t a = k0 >> 5, sh = k0 & 31;
+            const uint32_t p0 = P[a], p1 = P[a + 1], p2 = P[a + 2];
+            const uint64_t col = ((uint64_t)__builtin_amdgcn_alignbit(p2, p1, sh) << 32) |
+                                 __builtin_amdgcn_alignbit(p1, p0, sh);
+            row[ww] = wave_transpose(live[ww] ? col : 0ull, lane);
         }
         if (r0 + lane < mod.q) {
             uint64_t* out = (uint64_t*)m.masks + (uint64_t)(m.row_base[k] + r0 + lane) * m.words + w0;

@@ -99,15 +99,15 @@ __device__ __forceinline__ void staged_mask_wave(const MatchArgs& m, uint64_t g)
         uint64_t out[kStageWords];
 #pragma unroll
         for (int ww = 0; ww < kStageWords; ++ww) {
-            uint64_t col = 0;
-            if (valid[ww]) {
-                uint32_t k0 = base[ww] + r0;
-                k0 = k0 >= mod.q ? k0 - mod.q : k0;
-                const uint32_t w0 = k0 >> 5, sh = k0 & 31;
-                const uint64_t low = ((uint64_t)P[w0 + 1] << 32) | P[w0];
-                col = (low >> sh) | (sh ? (uint64_t)P[w0 + 2] << (64 - sh) : 0ull);
-            }
-            out[ww] = wave_transpose(col, lane);
+            // bits k0 .. k0 + 63 of P by two funnel shifts (k0 < q, so the
+            // three words lie in the loaded string)
+            uint32_t k0 = base[ww] + r0;
+            k0 = k0 >= mod.q ? k0 - mod.q : k0;
+            const uint32_t w0 = k0 >> 5, sh = k0 & 31;
+            const uint32_t p0 = P[w0], p1 = P[w0 + 1], p2 = P[w0 + 2];
+            const uint64_t col = ((uint64_t)__builtin_amdgcn_alignbit(p2, p1, sh) << 32) |
+                                 __builtin_amdgcn_alignbit(p1, p0, sh);
+            out[ww] = wave_transpose(valid[ww] ? col : 0ull, lane);
         }
         if (lane < rows) {
             uint4* dst = (uint4*)(image + (r0 + lane) * (kStageLineBytes / 8) + k * kStageWords);
