@@ -355,6 +355,14 @@ int pcube_scope_records_many(uint32_t nbatches, const uint32_t *per_batch, const
  * x = in[3i] + 2^64 in[3i+1] and B = in[3i+2]; host arrays.  INVALID when a B is 0. */
 int pcube_floor_div(const uint64_t *in, uint64_t *out, uint64_t count);
 
+/* --- self test: the transpose behind the residue bitsets (pseudocube/batch_experiment.py:184-194) --- */
+/* The mask builders form each 64-row block of a filter's bitset as 64 columns (one u
+ * coordinate per lane, its bits over 64 residues of v) and transpose it across the
+ * wave.  out[64i + r] = row r of matrix i, whose column t is in[64i + t] (bit k of
+ * column t = bit t of row k); host arrays of 64 * count words.  INVALID above 2^24
+ * matrices. */
+int pcube_bit_transpose(const uint64_t *in, uint64_t *out, uint64_t count);
+
 /* --- full-domain control: replaces review/cube_count.c (main 12-38) -------- */
 /* Number of roots r in [first, last] coprime to every prime q <= p, counted
  * on the device (the reference's control: 38,322,775 for [1000000001,

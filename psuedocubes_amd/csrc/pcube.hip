@@ -781,16 +781,16 @@ int staged_enqueue(Matcher& mt, MatchArgs m, hipEvent_t ev_pre, hipEvent_t ev_ma
     m.item_cap = (uint32_t)mt.cand_cap;
     // kStagedBlocksPerCU persistent blocks per CU, no more than there can be work units
     const uint32_t npieces = (m.nv + kPieceRows - 1) / kPieceRows;
-    const uint32_t items_cap = npieces * m.nstage;   // staged_fits bounds it
+    const uint64_t items_max = (uint64_t)npieces * m.nstage;   // staged_fits bounds it
     const uint32_t blocks = (uint32_t)std::min<uint64_t>((uint64_t)kStagedBlocksPerCU * g.compute_units,
-                                                         std::max<uint64_t>(1, (uint64_t)items_cap * kUnitsPerItem));
-    // the work list of (piece, chunk) items, its count, each piece's reach and the
+                                                         std::max<uint64_t>(1, items_max * kUnitsPerItem));
+    // the chunk table of the (piece, chunk) work list, each piece's reach and the
     // match blocks' runs
-    TRY(mt.items.ensure(((uint64_t)items_cap + 1) * sizeof(uint2) + (uint64_t)npieces * 8 +
+    TRY(mt.items.ensure(((uint64_t)m.nstage + 1) * sizeof(uint2) + (uint64_t)npieces * 8 +
                         ((uint64_t)blocks + 1) * sizeof(uint2)));
-    uint2* items = mt.items.as<uint2>();
-    uint32_t* nitems = (uint32_t*)(items + items_cap);
-    uint2* runs = (uint2*)(nitems + 2 + 2 * (uint64_t)npieces);
+    uint2* chunks = mt.items.as<uint2>();
+    uint32_t* reach = (uint32_t*)(chunks + m.nstage + 1);
+    uint2* runs = (uint2*)(reach + 2 * (uint64_t)npieces);
     // each wave's survivor region: cand_cap survivor records
     TRY(mt.cand.ensure(std::max<uint64_t>(16, (uint64_t)blocks * kStagedWaves * mt.cand_cap * kRecordUnits * 16)));
     m.cand = mt.cand.as<uint32_t>();
@@ -802,10 +802,8 @@ int staged_enqueue(Matcher& mt, MatchArgs m, hipEvent_t ev_pre, hipEvent_t ev_ma
     pa.mask_blocks = m.nu ? (uint32_t)blocks_for((uint64_t)mt.sbits * m.nstage * kWave, 256) : 0;
     pa.rho_blocks = m.nu && m.ncache ? (uint32_t)blocks_for((uint64_t)m.nu, 256) : 0;
     pa.npieces = npieces;
-    pa.items_cap = items_cap;
-    pa.items = items;
-    pa.nitems = nitems;
-    pa.reach = nitems + 2;
+    pa.chunks = chunks;
+    pa.reach = reach;
     pa.reach_lds = npieces <= kPlanLdsPieces;
     pa.grid = blocks;
     // with two blocks on every CU, the first-round blocks take the larger share
@@ -822,9 +820,9 @@ int staged_enqueue(Matcher& mt, MatchArgs m, hipEvent_t ev_pre, hipEvent_t ev_ma
     mt.kernel_span = ev_pre && ev_match;
     if (m.nu) {
         // persistent blocks over the runs of work units the plan assigned, then the residual chain
-        const uint2* items_arg = items;
+        const uint2* chunks_arg = chunks;
         const uint2* runs_arg = runs;
-        void* args[] = {&m, &items_arg, &runs_arg};
+        void* args[] = {&m, &chunks_arg, &runs_arg};
         if (ev_k0 && !mt.kernel_span) HIP_TRY(hipEventRecord(ev_k0, g.stream));
         HIP_TRY(hipLaunchKernel(staged_kernel(mt.sbits, mt.nres), dim3(blocks), dim3(kStagedThreads), args, 0,
                                 g.stream));
@@ -2933,6 +2931,25 @@ int pcube_floor_div(const uint64_t* in, uint64_t* out, uint64_t count) {
     hipLaunchKernelGGL(k_floor_div, dim3(blocks_for(count, 256)), dim3(256), 0, g.stream, (const uint64_t*)din,
                        (uint64_t*)dout, count);
     HIP_TRY(hipMemcpyAsync(out, dout, count * 16, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    dev_free(din);
+    dev_free(dout);
+    return PCUBE_OK;
+}
+
+int pcube_bit_transpose(const uint64_t* in, uint64_t* out, uint64_t count) {
+    Entry lock;
+    if (!count) return PCUBE_OK;
+    if (count > (1ull << 24)) return fail(PCUBE_INVALID, "at most 2^24 matrices per call");
+    const uint64_t bytes = count * kWave * 8;
+    void *din = nullptr, *dout = nullptr;
+    TRY(dev_alloc(bytes, &din));
+    int rc = dev_alloc(bytes, &dout);
+    if (rc) { dev_free(din); return rc; }
+    HIP_TRY(hipMemcpyAsync(din, in, bytes, hipMemcpyHostToDevice, g.stream));
+    hipLaunchKernelGGL(k_bit_transpose, dim3(blocks_for(count * kWave, 256)), dim3(256), 0, g.stream,
+                       (const uint64_t*)din, (uint64_t*)dout, count);
+    HIP_TRY(hipMemcpyAsync(out, dout, bytes, hipMemcpyDeviceToHost, g.stream));
     HIP_TRY(hipStreamSynchronize(g.stream));
     dev_free(din);
     dev_free(dout);

@@ -12,6 +12,7 @@
 //   k_match_direct   per-position chain without bitsets      pseudocube/gpu.py:147-181
 //   k_arithmetic     wide-limb self test                     kernels/pseudocube.cuh:6-11
 //   k_floor_div      the u-interval division, self test      pseudocube/focus.py:120-121
+//   k_bit_transpose  the mask builders' wave transpose, self test
 #include "pcube_kernels.h"
 
 namespace pcube {
@@ -1303,6 +1304,14 @@ __global__ void k_floor_div(const uint64_t* __restrict__ in, uint64_t* __restric
     const U128 q = floor_div_wide(U128{in[5 * i], in[5 * i + 1]}, in[5 * i + 2], U128{in[5 * i + 3], in[5 * i + 4]});
     out[2 * i] = q.lo;
     out[2 * i + 1] = q.hi;
+}
+
+// One 64 x 64 bit matrix per wave: in[64i + t] is column t of matrix i,
+// out[64i + r] its row r (the transpose the bitset mask builders apply).
+__global__ void k_bit_transpose(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, uint64_t count) {
+    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i / kWave >= count) return;   // whole waves: count matrices are count * 64 lanes
+    out[i] = wave_transpose(in[i], (uint32_t)(i & (kWave - 1)));
 }
 
 __global__ void k_arithmetic(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, uint64_t count) {

@@ -306,57 +306,41 @@ __device__ __forceinline__ void chain_records(const MatchArgs& m, const Staged& 
     }
 }
 
-// v coordinates of the first and last rows of piece p.
-__device__ __forceinline__ void piece_ends(const MatchArgs& m, uint32_t p, uint64_t& first, uint64_t& last) {
-    const uint32_t j0 = p * kPieceRows, j1 = min(m.nv, j0 + kPieceRows);
-    first = m.vs[j0];
-    last = m.vs[j1 - 1];
+// End i of the pieces' reach (i = 2p: the lower end of piece p's first row;
+// i = 2p + 1: the upper end of its last row): the row's v coordinate, the search
+// key of its bound, and the u index of the end.  Rows are sorted by v, so
+// [left0, right1) = [end 2p, end 2p + 1) holds every u index the piece's rows
+// reach.  With a directory (dir[k] = us[k * stride] in LDS) an end comes from it
+// alone, widened to the directory interval that holds the exact end (no global
+// probe); without one, it is exact.  Both ends are nondecreasing over the
+// pieces, and a piece's items may include a chunk beyond each end that none of
+// its rows touches (the match skips such units).
+__device__ __forceinline__ uint64_t piece_end(const MatchArgs& m, uint32_t i) {
+    const uint32_t j0 = (i / 2) * kPieceRows;
+    return m.vs[i & 1 ? min(m.nv, j0 + kPieceRows) - 1 : j0];
 }
 
-// A range [left0, right1) of u indices holding every index the rows of a piece
-// reach: rows are sorted by v, so it holds the exact u bounds of the piece's
-// first row's lower end and its last row's upper end (coordinates v_first,
-// v_last).  With a directory (dir[k] = us[k * stride] in LDS) the ends come
-// from it alone, widened to the directory interval that holds each exact end
-// (no global probe); without one, they are exact.  Both ends are
-// nondecreasing over the pieces, and a piece's items may include a chunk
-// beyond each end that none of its rows touches (the match skips such units).
-__device__ __forceinline__ void piece_reach(const MatchArgs& m, uint64_t v_first, uint64_t v_last,
-                                            const uint64_t* dir, uint32_t ndir, uint32_t stride, uint32_t& left0,
-                                            uint32_t& right1) {
-    const U128 lower = add_wide(mul_wide(v_first, m.A), m.lo);
-    const U128 upper = add_wide(mul_wide(v_last, m.A), m.hi);
+__device__ __forceinline__ AboveKey piece_end_key(const MatchArgs& m, uint32_t i, uint64_t v) {
+    return above_key(add_wide(mul_wide(v, m.A), i & 1 ? m.hi : m.lo), m.B, m.B_mu);
+}
+
+__device__ __forceinline__ uint32_t piece_end_index(const MatchArgs& m, uint32_t i, const AboveKey& key,
+                                                    const uint64_t* dir, uint32_t ndir, uint32_t stride) {
     if (!dir) {
+        uint32_t r0, r1;
         uint64_t unused;
-        first_above_pair<false>(m, lower, upper, dir, ndir, stride, left0, right1, unused);
-        return;
+        first_above_keys<false>(m, key, key, dir, ndir, stride, r0, r1, unused);
+        return r0;
     }
-    // first directory entry above each bound: the exact end lies in
+    // the first directory entry a above the bound: the exact end lies in
     // ((a - 1) * stride, a * stride] (0 when a = 0)
-    const AboveKey kl = above_key(lower, m.B, m.B_mu), ku = above_key(upper, m.B, m.B_mu);
-    uint32_t al = 0, el = ndir, au = 0, eu = ndir;
-    while (al < el) {
-        const uint32_t mid = al + ((el - al) >> 1);
-        if (!below_key(dir[mid], kl)) el = mid; else al = mid + 1;
+    uint32_t a = 0, e = ndir;
+    while (a < e) {
+        const uint32_t mid = a + ((e - a) >> 1);
+        if (!below_key(dir[mid], key)) e = mid; else a = mid + 1;
     }
-    while (au < eu) {
-        const uint32_t mid = au + ((eu - au) >> 1);
-        if (!below_key(dir[mid], ku)) eu = mid; else au = mid + 1;
-    }
-    left0 = al == 0 ? 0u : (al - 1) * stride + 1;
-    right1 = (uint32_t)min((uint64_t)au * stride, (uint64_t)m.nu);
-}
-
-// First index in [0, n) whose bound (every second word of bounds, from `at`) is at
-// least key; the bounds are nondecreasing.  n when there is none.
-template <typename P>
-__device__ __forceinline__ uint32_t first_at_least(P bounds, uint32_t at, uint32_t n, uint32_t key) {
-    uint32_t l = 0, h = n;
-    while (l < h) {
-        const uint32_t mid = (l + h) >> 1;
-        if (bounds[2 * mid + at] < key) l = mid + 1; else h = mid;
-    }
-    return l;
+    if (i & 1) return (uint32_t)min((uint64_t)a * stride, (uint64_t)m.nu);
+    return a == 0 ? 0u : (a - 1) * stride + 1;
 }
 
 // Share of the work before run r, in 1/64 of an average run, when the first-round
@@ -373,26 +357,24 @@ __device__ __forceinline__ uint64_t run_weight_prefix(uint32_t r, uint32_t grid,
     return (uint64_t)x * per * 64 + within;
 }
 
-// Work list of the staged match, chunk-major: for every stage chunk c (u indices
-// [256c, 256c + 256)), the pieces whose rows reach it, in piece order; at most
-// items_cap items, nitems receives the full count.  With piece p reaching
-// [left0_p, right1_p), both nondecreasing in p, the pieces reaching chunk c are
-// the run from the first with right1 > 256c to the last with left0 < 256c + 256.
-// (A piece whose rows reach nothing, right1 <= left0, can lie inside a run: its
-// item has no work.)  Chunk-major order lets a block walking consecutive items
-// keep one chunk image in LDS while only the row headers change.
-// One 256-thread block: the pieces' reach goes to `reach` (two words per piece;
-// in LDS after the directory for kLds, else global), then thread t takes a run
-// of consecutive chunks and a block scan turns the runs' item counts into
-// offsets.
+// Work of the staged match, chunk-major: for every stage chunk c (u indices
+// [256c, 256c + 256)) the pieces whose rows reach it, in piece order; item k
+// of this list is one (piece, chunk) pair of 16 work units.  With piece p
+// reaching [left0_p, right1_p), both nondecreasing in p, the pieces reaching
+// chunk c are the run [a_c, b_c): a_c the first with right1 > 256c, b_c the
+// first with left0 >= 256c + 256.  (A piece whose rows reach nothing, right1 <=
+// left0, can lie inside a run: its item has no work.)  The list is kept as its
+// chunk table: chunks[c] = (a_c, first item of chunk c) for c <= nstage, so item
+// k of chunk c is piece a_c + k - chunks[c].y.  Chunk-major order lets a block
+// walking consecutive items keep one chunk image in LDS while only the row
+// headers change.
 // Geometry of the fused prefilter launch and of the match that follows it.
 struct PrefilterArgs {
-    uint32_t rows_blocks, mask_blocks, rho_blocks, npieces, items_cap;
+    uint32_t rows_blocks, mask_blocks, rho_blocks, npieces;
     uint32_t reach_lds;  // the pieces' reach fits the dynamic LDS after the directory
     uint32_t grid;       // blocks of the staged match
     uint32_t first64;    // a first-round match block's share of the work (1/64 of an average run)
-    uint2* items;
-    uint32_t* nitems;
+    uint2* chunks;       // [nstage + 1]: first piece and first item of each chunk
     uint32_t* reach;     // [npieces][2]: the u indices each piece's rows reach (when not in LDS)
     uint2* runs;         // [grid + 1]: first work unit of each match block's run and that unit's chunk
 };
@@ -402,38 +384,94 @@ struct PrefilterArgs {
 // many units (measured).
 constexpr uint32_t kChunkChangeUnits = 16;
 
+// The planner (one 256-thread block, at raised wave priority: the match waits
+// for it).  One thread per end of each piece's reach: the row coordinate loads
+// with the u directory, the search key is formed while the directory arrives,
+// and the end goes to `reach` (two words per piece; in LDS after the directory
+// for kLds, else global).  Then thread t takes a run of consecutive chunks,
+// finds each chunk's piece run [a_c, b_c) and keeps it in LDS over the
+// directory, no longer needed, when that holds nstage entries (else the runs
+// are searched again where needed); a block scan of the runs' item counts
+// gives the chunk table, and each match block's run starts where the cost
+// (units plus kChunkChangeUnits per nonempty chunk) reaches its share.
 template <bool kLds>
 __device__ __forceinline__ void staged_plan_block(const MatchArgs& m, const PrefilterArgs& pa, uint64_t* dir) {
     __shared__ uint32_t wave_items[256 / kWave], wave_cost[256 / kWave];
     __shared__ uint32_t items_before[257], cost_before[257];   // per thread's chunk run; [256]: the totals
     const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    const uint32_t npieces = pa.npieces;
+    const uint32_t npieces = pa.npieces, nstage = m.nstage;
+    // the other blocks of this CU (rows, bitsets) yield issue slots to the planner
+    __builtin_amdgcn_s_setprio(3);
     uint32_t ndir, stride;
-    load_directory(m, dir, ndir, stride);
-    uint32_t* reach = kLds ? (uint32_t*)(dir + (ndir ? ndir : 1)) : pa.reach;
-    const uint64_t* d = ndir ? dir : nullptr;
-    for (uint32_t p = threadIdx.x; p < npieces; p += 256) {
-        uint64_t v_first, v_last;
-        piece_ends(m, p, v_first, v_last);
-        uint32_t left0, right1;
-        piece_reach(m, v_first, v_last, d, ndir, stride, left0, right1);
-        reach[2 * p] = left0;
-        reach[2 * p + 1] = right1;
+    directory_shape(m.nu, stride, ndir);
+    // thread i < 2 npieces takes one end of piece i / 2 (even i: the lower end of
+    // its first row, odd i: the upper end of its last): the coordinate loads
+    // with the directory's, the search key is formed before the directory is
+    // stored, and a binary search over the directory gives the end
+    static_assert(kRowDirectory == 256 * 16, "one round of sixteen loads per thread fills the directory");
+    const uint32_t ends = 2 * npieces;
+    uint64_t v_end = 0;
+    if (threadIdx.x < ends) v_end = piece_end(m, threadIdx.x);
+    uint64_t d[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const uint32_t t = 256 * k + threadIdx.x;
+        d[k] = ndir ? m.us[(t < ndir ? t : ndir - 1) * stride] : 0ull;
+    }
+    AboveKey key0{};
+    if (threadIdx.x < ends) key0 = piece_end_key(m, threadIdx.x, v_end);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const uint32_t t = 256 * k + threadIdx.x;
+        if (t < ndir) dir[t] = d[k];
+    }
+    __syncthreads();
+    {
+        uint32_t* reach = kLds ? (uint32_t*)(dir + (ndir ? ndir : 1)) : pa.reach;
+        for (uint32_t i = threadIdx.x; i < ends; i += 256) {
+            const AboveKey key = i == threadIdx.x ? key0 : piece_end_key(m, i, piece_end(m, i));
+            reach[i] = piece_end_index(m, i, key, ndir ? dir : nullptr, ndir, stride);
+        }
     }
     __threadfence_block();
     __syncthreads();
+    const uint32_t* reach = kLds ? (const uint32_t*)(dir + (ndir ? ndir : 1)) : pa.reach;
     constexpr uint32_t kChunkU = kStageWords * 64;
-    const uint32_t per = (m.nstage + 255) / 256;
-    const uint32_t c0 = min(m.nstage, threadIdx.x * per), c1 = min(m.nstage, c0 + per);
+    const uint32_t per = (nstage + 255) / 256;
+    const uint32_t c0 = min(nstage, threadIdx.x * per), c1 = min(nstage, c0 + per);
+    // each chunk's piece run [a_c, b_c): a_c the first piece with right1 > 256c,
+    // b_c the first with left0 >= 256c + 256 (two interleaved searches); kept in
+    // LDS over the directory, no longer read, when it holds nstage entries
+    uint2* span = (uint2*)dir;
+    const bool tabled = nstage <= ndir;
+    auto search_span = [&](uint32_t c) -> uint2 {
+        uint32_t al = 0, ah = npieces, bl = 0, bh = npieces;
+        const uint32_t ka = c * kChunkU + 1, kb = (c + 1) * kChunkU;
+        while (al < ah || bl < bh) {
+            if (al < ah) {
+                const uint32_t mid = (al + ah) >> 1;
+                if (reach[2 * mid + 1] < ka) al = mid + 1; else ah = mid;
+            }
+            if (bl < bh) {
+                const uint32_t mid = (bl + bh) >> 1;
+                if (reach[2 * mid] < kb) bl = mid + 1; else bh = mid;
+            }
+        }
+        return make_uint2(al, bl);
+    };
+    if (tabled) {
+        for (uint32_t c = c0; c < c1; ++c) span[c] = search_span(c);
+        __syncthreads();
+    }
+    auto chunk_span = [&](uint32_t c) -> uint2 { return tabled ? span[c] : search_span(c); };
     // items of this thread's chunks, and their cost: each nonempty chunk's units
     // plus one chunk change
     uint32_t count = 0, cost = 0;
     for (uint32_t c = c0; c < c1; ++c) {
-        const uint32_t a = first_at_least(reach, 1, npieces, c * kChunkU + 1);   // right1 > 256c
-        const uint32_t b = first_at_least(reach, 0, npieces, (c + 1) * kChunkU);  // left0 >= 256c + 256
-        if (b > a) {
-            count += b - a;
-            cost += kChunkChangeUnits + kUnitsPerItem * (b - a);
+        const uint2 ab = chunk_span(c);
+        if (ab.y > ab.x) {
+            count += ab.y - ab.x;
+            cost += kChunkChangeUnits + kUnitsPerItem * (ab.y - ab.x);
         }
     }
     uint32_t inc = count, inc_cost = cost;
@@ -460,19 +498,18 @@ __device__ __forceinline__ void staged_plan_block(const MatchArgs& m, const Pref
     if (threadIdx.x == 255) {
         items_before[256] = at + count;
         cost_before[256] = kat + cost;
-        *pa.nitems = at + count;
+        pa.chunks[nstage] = make_uint2(npieces, at + count);
     }
     for (uint32_t c = c0; c < c1; ++c) {
-        const uint32_t a = first_at_least(reach, 1, npieces, c * kChunkU + 1);
-        const uint32_t b = first_at_least(reach, 0, npieces, (c + 1) * kChunkU);
-        for (uint32_t q = a; q < b; ++q, ++at)
-            if (at < pa.items_cap) pa.items[at] = make_uint2(q, c);
+        const uint2 ab = chunk_span(c);
+        pa.chunks[c] = make_uint2(ab.x, at);
+        if (ab.y > ab.x) at += ab.y - ab.x;
     }
     __syncthreads();
     // the match blocks' runs: run r starts at the unit where the cost reaches its
     // share (run_weight_prefix) of the total; a chunk change costs
     // kChunkChangeUnits before the chunk's first unit
-    const uint32_t total_units = min(items_before[256], pa.items_cap) * kUnitsPerItem;
+    const uint32_t total_units = items_before[256] * kUnitsPerItem;
     const uint64_t total_cost = cost_before[256];
     for (uint32_t r = threadIdx.x; r <= pa.grid; r += 256) {
         uint32_t u = total_units, chunk = 0;
@@ -484,14 +521,13 @@ __device__ __forceinline__ void staged_plan_block(const MatchArgs& m, const Pref
                 const uint32_t mid = (lo + hi + 1) >> 1;
                 if (cost_before[mid] <= target) lo = mid; else hi = mid - 1;
             }
-            const uint32_t t0 = min(m.nstage, lo * per), t1 = min(m.nstage, t0 + per);
+            const uint32_t t0 = min(nstage, lo * per), t1 = min(nstage, t0 + per);
             uint64_t k = cost_before[lo];
             uint32_t unit = items_before[lo] * kUnitsPerItem;
             for (uint32_t c = t0; c < t1; ++c) {
-                const uint32_t a = first_at_least(reach, 1, npieces, c * kChunkU + 1);
-                const uint32_t b = first_at_least(reach, 0, npieces, (c + 1) * kChunkU);
-                if (b <= a) continue;
-                const uint32_t units = kUnitsPerItem * (b - a);
+                const uint2 ab = chunk_span(c);
+                if (ab.y <= ab.x) continue;
+                const uint32_t units = kUnitsPerItem * (ab.y - ab.x);
                 if (target < k + kChunkChangeUnits + units) {
                     const uint64_t off = target - k;
                     u = unit + (off < kChunkChangeUnits ? 0u : (uint32_t)(off - kChunkChangeUnits));
@@ -594,39 +630,20 @@ static_assert(kMaxAllowWords * 4 + kMaxResiduals * sizeof(Modulus) + kTileSlots 
               "the chain's filter tables and group counts fit the mask buffer");
 
 
-// A wave's view of the item list: lane l holds the descriptor (piece, chunk) of
-// item window + l, so a descriptor is a register read.
-struct ItemWindow {
-    uint32_t window, count;
-    uint2 desc;
+// One segment of a match block's run: the items of one chunk, [item0,
+// item_end) of the chunk-major list, item k being piece piece0 + k - item0.
+struct Segment {
+    uint32_t chunk, piece0, item0, item_end;
 };
 
-__device__ __forceinline__ void load_window(const uint2* items, ItemWindow& w, uint32_t k0, uint32_t lane) {
-    w.window = k0;
-    w.desc = items[min(k0 + lane, w.count - 1)];
+__device__ __forceinline__ Segment chunk_segment(const uint2* chunks, uint32_t c) {
+    const uint2 d = chunks[c], e = chunks[c + 1];
+    return Segment{c, d.x, d.y, e.y};
 }
 
-// Descriptor of item k (k < count).
-__device__ __forceinline__ uint2 item_at(const uint2* items, ItemWindow& w, uint32_t k, uint32_t lane) {
-    if (k - w.window >= (uint32_t)kWave) load_window(items, w, k, lane);
-    const int at = (int)(k - w.window);
-    return make_uint2(__builtin_amdgcn_readlane(w.desc.x, at), __builtin_amdgcn_readlane(w.desc.y, at));
-}
-
-// The end of the segment of the run that starts at unit u: the first unit of
-// an item after u's whose chunk differs from `chunk`, or the run's end u_last.
-__device__ __forceinline__ uint32_t segment_end(const uint2* items, ItemWindow& w, uint32_t u, uint32_t u_last,
-                                                uint32_t chunk, uint32_t lane) {
-    uint32_t k = u / kUnitsPerItem;
-    const uint32_t k_last = (u_last - 1) / kUnitsPerItem;
-    for (;;) {
-        if (k - w.window >= (uint32_t)kWave) load_window(items, w, k, lane);
-        const uint32_t idx = w.window + lane;
-        const uint64_t differ = __ballot(idx >= k && idx <= k_last && w.desc.y != chunk);
-        if (differ) return min(u_last, (w.window + (uint32_t)__builtin_ctzll(differ)) * kUnitsPerItem);
-        k = w.window + kWave;
-        if (k > k_last) return u_last;
-    }
+// Piece of work unit x (x in the segment).
+__device__ __forceinline__ uint32_t unit_piece(const Segment& sg, uint32_t x) {
+    return sg.piece0 + x / kUnitsPerItem - sg.item0;
 }
 
 // The two 64-bit words of a 16-byte mask half.
@@ -814,7 +831,7 @@ __device__ __forceinline__ void acc_add(const MatchArgs& m, const Staged& S, Gro
 // stay resident.
 template <int kF, bool kLast>
 __global__ __launch_bounds__(kStagedThreads) __attribute__((amdgpu_waves_per_eu(4)))
-void k_match_staged(MatchArgs m, const uint2* items, const uint2* runs) {
+void k_match_staged(MatchArgs m, const uint2* chunks, const uint2* runs) {
     __shared__ uint4 image[kStageBufferUnits];
     __shared__ uint32_t tilecnt[kTileSlots * kItemGroups];
     __shared__ uint2 queues[kStagedWaves * kQueue];
@@ -827,12 +844,9 @@ void k_match_staged(MatchArgs m, const uint2* items, const uint2* runs) {
     const uint2 mine = runs[run], next = runs[run + 1];
     const uint32_t u_first = mine.x, u_last = next.x;
     if (u_first >= u_last) return;
-    uint32_t chunk = mine.y;
-    // the first chunk image streams in while the item descriptors load
-    stage_async(m, chunk, (uint8_t*)image, wave, lane);
-    ItemWindow iw;
-    iw.count = runs[grid].x / kUnitsPerItem;
-    load_window(items, iw, u_first / kUnitsPerItem, lane);
+    // the first chunk image streams in while its items' descriptor loads
+    stage_async(m, mine.y, (uint8_t*)image, wave, lane);
+    Segment sg = chunk_segment(chunks, mine.y);
     for (uint32_t t = threadIdx.x; t < (uint32_t)(kTileSlots * kItemGroups); t += kStagedThreads) tilecnt[t] = 0;
     const Staged S{nullptr, nullptr, tilecnt, kItemGroups, queues + wave * kQueue};
     uint4* region = (uint4*)m.cand + ((uint64_t)blockIdx.x * kStagedWaves + wave) * m.item_cap * kRecordUnits;
@@ -841,15 +855,11 @@ void k_match_staged(MatchArgs m, const uint2* items, const uint2* runs) {
     // a segment [u, seg_end) of n units gives wave w the contiguous units
     // [u + n w / 8, u + n (w + 1) / 8), so its consecutive units mostly share a subtile
     uint32_t u = u_first;
-    uint32_t seg_end = segment_end(items, iw, u, u_last, chunk, lane);
+    uint32_t seg_end = min(u_last, sg.item_end * kUnitsPerItem);
     uint32_t x = u + (seg_end - u) * wave / kStagedWaves;
     uint32_t x_end = u + (seg_end - u) * (wave + 1) / kStagedWaves;
     UnitRow R;
-    uint2 it = make_uint2(0, 0);
-    if (x < x_end) {
-        it = item_at(items, iw, x / kUnitsPerItem, lane);
-        load_unit_row(m, it.x, x % kUnitsPerItem, lane, R);
-    }
+    if (x < x_end) load_unit_row(m, unit_piece(sg, x), x % kUnitsPerItem, lane, R);
     __builtin_amdgcn_s_waitcnt(0);   // this wave's part of the image has arrived
     __syncthreads();
     for (;;) {
@@ -858,40 +868,35 @@ void k_match_staged(MatchArgs m, const uint2* items, const uint2* runs) {
         UnitRow R2;
         uint32_t c1, c2;
         while (x < x_end) {
-            const uint32_t p = it.x, s = x % kUnitsPerItem;
+            const uint32_t p = unit_piece(sg, x), s = x % kUnitsPerItem;
             ++x;
             unit_row_ready(R);
-            if (x < x_end) {
-                it = item_at(items, iw, x / kUnitsPerItem, lane);
-                load_unit_row(m, it.x, x % kUnitsPerItem, lane, R2);
-            }
-            staged_unit<kF, kLast>(m, p, s, chunk, R, image, lane, region, fill, c1, c2);
+            if (x < x_end) load_unit_row(m, unit_piece(sg, x), x % kUnitsPerItem, lane, R2);
+            staged_unit<kF, kLast>(m, p, s, sg.chunk, R, image, lane, region, fill, c1, c2);
             acc_add(m, S, A, R.h0.z, c1, c2, lane);
             if (x >= x_end) break;
-            const uint32_t p2 = it.x, s2 = x % kUnitsPerItem;
+            const uint32_t p2 = unit_piece(sg, x), s2 = x % kUnitsPerItem;
             ++x;
             unit_row_ready(R2);
-            if (x < x_end) {
-                it = item_at(items, iw, x / kUnitsPerItem, lane);
-                load_unit_row(m, it.x, x % kUnitsPerItem, lane, R);
-            }
-            staged_unit<kF, kLast>(m, p2, s2, chunk, R2, image, lane, region, fill, c1, c2);
+            if (x < x_end) load_unit_row(m, unit_piece(sg, x), x % kUnitsPerItem, lane, R);
+            staged_unit<kF, kLast>(m, p2, s2, sg.chunk, R2, image, lane, region, fill, c1, c2);
             acc_add(m, S, A, R2.h0.z, c1, c2, lane);
         }
         if (seg_end >= u_last) break;
-        // the next segment: its first unit's rows are in flight while the block
-        // waits for every wave to finish with the image
+        // the next segment: the next chunk with items (the last entry of the
+        // table holds the item count, above every item of the run); its first
+        // unit's rows are in flight while the block waits for every wave to
+        // finish with the image
         u = seg_end;
-        chunk = item_at(items, iw, u / kUnitsPerItem, lane).y;
-        seg_end = segment_end(items, iw, u, u_last, chunk, lane);
+        uint32_t c = sg.chunk + 1;
+        while (chunks[c + 1].y <= u / kUnitsPerItem) ++c;
+        sg = chunk_segment(chunks, c);
+        seg_end = min(u_last, sg.item_end * kUnitsPerItem);
         x = u + (seg_end - u) * wave / kStagedWaves;
         x_end = u + (seg_end - u) * (wave + 1) / kStagedWaves;
-        if (x < x_end) {
-            it = item_at(items, iw, x / kUnitsPerItem, lane);
-            load_unit_row(m, it.x, x % kUnitsPerItem, lane, R);
-        }
+        if (x < x_end) load_unit_row(m, unit_piece(sg, x), x % kUnitsPerItem, lane, R);
         __syncthreads();
-        stage_async(m, chunk, (uint8_t*)image, wave, lane);
+        stage_async(m, sg.chunk, (uint8_t*)image, wave, lane);
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
     }

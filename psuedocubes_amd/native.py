@@ -67,6 +67,7 @@ SIGNATURES: dict[str, list[Any]] = {
     'pcube_batch_values': [u64, P_u64, u32, P_u64, P_u64, P_u64, P_u32, u64, u64, u64, u64, u64, u64,
                            ctypes.POINTER(ctypes.c_uint8), P_u64, ctypes.c_void_p, u64, P_u64, P_u32],
     'pcube_floor_div': [P_u64, P_u64, u64],
+    'pcube_bit_transpose': [P_u64, P_u64, u64],
     'pcube_scope_records': [u32, P_u64, P_u64, u32, u32, P_u64, ctypes.c_char_p, ctypes.c_void_p, u64, P_u64],
     'pcube_scope_records_many': [u32, P_u32, P_u64, P_u64, u32, u32, P_u64, ctypes.c_char_p, ctypes.c_void_p, u64,
                                  P_u64],

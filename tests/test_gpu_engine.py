@@ -104,6 +104,30 @@ def test_floor_div_kernel(runtime) -> None:
         native.call('pcube_floor_div', native.ptr64(bad), native.ptr64(out), 1)
 
 
+def test_bit_transpose_kernel(runtime) -> None:
+    """The 64 x 64 bit wave transpose the residue-bitset builders apply
+    (pseudocube/batch_experiment.py:184-194 builds the same rows one test at a
+    time): row r of the result holds bit r of every column, against numpy's
+    bit unpacking on zero, full, identity, single-bit and seeded matrices."""
+    rng = np.random.default_rng(619184)
+    mats = [np.zeros(64, dtype=np.uint64), np.full(64, 2**64 - 1, dtype=np.uint64),
+            np.array([1 << t for t in range(64)], dtype=np.uint64),
+            np.array([1 << (63 - t) for t in range(64)], dtype=np.uint64)]
+    for k in range(64):   # one set bit at (k, 63 - k) and at (k, k)
+        m = np.zeros(64, dtype=np.uint64)
+        m[63 - k] = np.uint64(1) << np.uint64(k)
+        m[k] |= np.uint64(1) << np.uint64(k)
+        mats.append(m)
+    mats += list(rng.integers(0, 2**64, size=(700, 64), dtype=np.uint64, endpoint=False))
+    cols = np.ascontiguousarray(np.stack(mats))
+    out = np.zeros_like(cols)
+    native.call('pcube_bit_transpose', native.ptr64(cols.ravel()), native.ptr64(out.ravel()), cols.shape[0])
+    bits = np.unpackbits(cols.view(np.uint8).reshape(-1, 64, 8), axis=2, bitorder='little')   # [m, t, k]
+    expect = np.packbits(bits.transpose(0, 2, 1), axis=2, bitorder='little').reshape(-1, 64, 8)
+    assert np.array_equal(out.view(np.uint8).reshape(-1, 64, 8), expect)
+    assert np.array_equal(out[2], cols[2]) and np.array_equal(out[0], cols[0])   # identity and zero are fixed
+
+
 def test_focus_halves_equal_oracle(focuses619, oracle_focuses619) -> None:
     for gpu, cpu in zip(focuses619, oracle_focuses619):
         assert np.array_equal(gpu.C.get(), cpu.C)
