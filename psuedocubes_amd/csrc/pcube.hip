@@ -727,7 +727,7 @@ struct Matcher {
     Pending pending;       // the batch started by pcube_match_start
     Pinned down;           // its download: [status words, counters]
     bool kernel_span = false;   // the last batch's match kernel was timed by ev[1] and ev[2]
-    uint64_t head_zero = 0;   // leading bytes of status known to be zero (cleared after each download)
+    uint64_t head_zero = 0;   // leading bytes of status known to be zero (cleared by a prepare's bounds kernel)
     uint64_t launches = 0;    // enqueues that used the batch buffers (status, counters, list, rows, masks)
     uint64_t* list_count() const { return status.as<uint64_t>(); }
     uint32_t* error() const { return (uint32_t*)(status.as<uint64_t>() + 1); }
@@ -1062,10 +1062,15 @@ __global__ __launch_bounds__(256) void k_prepare_bounds(const uint64_t* __restri
                                                         const uint64_t* __restrict__ vs,
                                                         const uint64_t* __restrict__ vinfo, uint64_t vs_cap,
                                                         const PrepareQueries queries, uint32_t ntiles,
-                                                        uint64_t* __restrict__ out) {
+                                                        uint64_t* __restrict__ out, uint64_t* __restrict__ clear,
+                                                        uint64_t nclear) {
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint64_t q = blockIdx.x * (uint64_t)(blockDim.x / kWave) + threadIdx.x / kWave;
     const uint64_t nq = 4 * (uint64_t)ntiles;
+    // the matcher's status words and counters, cleared for the match this
+    // batch will start (stream-ordered after the last batch's download)
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nclear; i += (uint64_t)gridDim.x * blockDim.x)
+        clear[i] = 0;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         out[nq] = uinfo[1];
         out[nq + 1] = vinfo[1];
@@ -1597,9 +1602,11 @@ int pcube_focus_windows(int focus, uint32_t n, const uint64_t* lows, const uint6
     return PCUBE_OK;
 }
 
-int pcube_prepare(int focus_u, int focus_v, uint32_t ntiles, const uint64_t* tiles, uint64_t us_out, uint64_t us_cap,
-                  uint64_t vs_out, uint64_t vs_cap, uint64_t* nu, uint64_t* nv, uint64_t* bounds) {
-    Entry lock;
+// pcube_prepare, clearing nclear words at clear in the bounds kernel (the
+// matcher's status and counters when the match follows in the same call).
+static int prepare_windows(int focus_u, int focus_v, uint32_t ntiles, const uint64_t* tiles, uint64_t us_out,
+                           uint64_t us_cap, uint64_t vs_out, uint64_t vs_cap, uint64_t* nu, uint64_t* nv,
+                           uint64_t* bounds, uint64_t* clear, uint64_t nclear) {
     Focus* fu = focus_at(focus_u);
     Focus* fv = focus_at(focus_v);
     if (!fu || !fv || fu == fv || !fu->wheel.ready || !fv->wheel.ready || ntiles == 0 || ntiles > kMaxPrepareTiles)
@@ -1642,7 +1649,7 @@ int pcube_prepare(int focus_u, int focus_v, uint32_t ntiles, const uint64_t* til
         hipLaunchKernelGGL(k_prepare_bounds, dim3(blocks_for(nq, 256 / kWave)), dim3(256), 0, g.stream,
                            (const uint64_t*)us_out, (const uint64_t*)fu->wheel.win_info.as<uint64_t>(), us_cap,
                            (const uint64_t*)vs_out, (const uint64_t*)fv->wheel.win_info.as<uint64_t>(), vs_cap,
-                           queries, ntiles, g_result_host.as<uint64_t>());
+                           queries, ntiles, g_result_host.as<uint64_t>(), clear, nclear);
         TRY(stream_wait());
         const bool again_u = r[nq + 2] && cap_u < (uint32_t)kWheelWaveSpan;
         const bool again_v = r[nq + 3] && cap_v < (uint32_t)kWheelWaveSpan;
@@ -1663,6 +1670,13 @@ int pcube_prepare(int focus_u, int focus_v, uint32_t ntiles, const uint64_t* til
         bounds[4 * t + 3] = r[2 * ntiles + 2 * t + 1];
     }
     return PCUBE_OK;
+}
+
+int pcube_prepare(int focus_u, int focus_v, uint32_t ntiles, const uint64_t* tiles, uint64_t us_out, uint64_t us_cap,
+                  uint64_t vs_out, uint64_t vs_cap, uint64_t* nu, uint64_t* nv, uint64_t* bounds) {
+    Entry lock;
+    return prepare_windows(focus_u, focus_v, ntiles, tiles, us_out, us_cap, vs_out, vs_cap, nu, nv, bounds, nullptr,
+                           0);
 }
 
 int pcube_tile_ranges(uint64_t A, uint64_t B, uint64_t lo0, uint64_t lo1, uint64_t hi0, uint64_t hi1, uint64_t first,
@@ -1707,11 +1721,11 @@ int pcube_tile_ranges_many(uint64_t A, uint64_t B, uint64_t lo0, uint64_t lo1, u
     return PCUBE_OK;
 }
 
-int pcube_prepare_range(int focus_u, int focus_v, uint64_t A, uint64_t B, uint64_t lo0, uint64_t lo1, uint64_t hi0,
-                        uint64_t hi1, uint64_t first, uint64_t end, uint64_t width, uint64_t* tiles, uint32_t* ntiles,
-                        uint64_t us_out, uint64_t us_cap, uint64_t vs_out, uint64_t vs_cap, uint64_t* nu, uint64_t* nv,
-                        uint64_t* bounds, uint64_t* totals) {
-    Entry lock;
+// pcube_prepare_range, clearing nclear words at clear as prepare_windows does.
+static int prepare_range(int focus_u, int focus_v, uint64_t A, uint64_t B, uint64_t lo0, uint64_t lo1, uint64_t hi0,
+                         uint64_t hi1, uint64_t first, uint64_t end, uint64_t width, uint64_t* tiles, uint32_t* ntiles,
+                         uint64_t us_out, uint64_t us_cap, uint64_t vs_out, uint64_t vs_cap, uint64_t* nu, uint64_t* nv,
+                         uint64_t* bounds, uint64_t* totals, uint64_t* clear, uint64_t nclear) {
     Focus* fu = focus_at(focus_u);
     Focus* fv = focus_at(focus_v);
     if (!fu || !fv) return fail(PCUBE_INVALID, "unknown focus");
@@ -1723,7 +1737,7 @@ int pcube_prepare_range(int focus_u, int focus_v, uint64_t A, uint64_t B, uint64
         return fail(PCUBE_RANGE, "focus window exceeds bounded period count");
     if ((end - 1) / fv->M - first / fv->M + 1 > 4096)
         return fail(PCUBE_RANGE, "focus window exceeds bounded period count");
-    TRY(pcube_prepare(focus_u, focus_v, n, tiles, us_out, us_cap, vs_out, vs_cap, nu, nv, bounds));
+    TRY(prepare_windows(focus_u, focus_v, n, tiles, us_out, us_cap, vs_out, vs_cap, nu, nv, bounds, clear, nclear));
     if (totals) {
         // bounds hold (u_first, u_end, v_first, v_end) index pairs < 2^32, so every product fits
         uint64_t focus = 0, opportunities = 0, largest = 0;
@@ -1740,6 +1754,15 @@ int pcube_prepare_range(int focus_u, int focus_v, uint64_t A, uint64_t B, uint64
     return PCUBE_OK;
 }
 
+int pcube_prepare_range(int focus_u, int focus_v, uint64_t A, uint64_t B, uint64_t lo0, uint64_t lo1, uint64_t hi0,
+                        uint64_t hi1, uint64_t first, uint64_t end, uint64_t width, uint64_t* tiles, uint32_t* ntiles,
+                        uint64_t us_out, uint64_t us_cap, uint64_t vs_out, uint64_t vs_cap, uint64_t* nu, uint64_t* nv,
+                        uint64_t* bounds, uint64_t* totals) {
+    Entry lock;
+    return prepare_range(focus_u, focus_v, A, B, lo0, lo1, hi0, hi1, first, end, width, tiles, ntiles, us_out, us_cap,
+                         vs_out, vs_cap, nu, nv, bounds, totals, nullptr, 0);
+}
+
 int pcube_prepare_match(int focus_u, int focus_v, uint64_t A, uint64_t B, uint64_t lo0, uint64_t lo1, uint64_t hi0,
                         uint64_t hi1, uint64_t first, uint64_t end, uint64_t width, uint64_t* tiles, uint32_t* ntiles,
                         uint64_t us_out, uint64_t us_cap, uint64_t vs_out, uint64_t vs_cap, uint64_t* nu, uint64_t* nv,
@@ -1748,9 +1771,14 @@ int pcube_prepare_match(int focus_u, int focus_v, uint64_t A, uint64_t B, uint64
     Entry lock;
     if (!totals || !launch) return fail(PCUBE_INVALID, "null totals or launch");
     *launch = 0;
-    TRY(pcube_prepare_range(focus_u, focus_v, A, B, lo0, lo1, hi0, hi1, first, end, width, tiles, ntiles, us_out,
-                            us_cap, vs_out, vs_cap, nu, nv, bounds, totals));
+    // a matcher with no batch to collect gets its status words and counters
+    // cleared by this batch's bounds kernel, so its match needs no fill launch
     Matcher* mt = matcher_at(matcher);
+    const bool clears = mt && !mt->pending.active && mt->status.bytes;
+    TRY(prepare_range(focus_u, focus_v, A, B, lo0, lo1, hi0, hi1, first, end, width, tiles, ntiles, us_out, us_cap,
+                      vs_out, vs_cap, nu, nv, bounds, totals, clears ? mt->status.as<uint64_t>() : nullptr,
+                      clears ? mt->status.bytes / 8 : 0));
+    if (clears) mt->head_zero = mt->status.bytes;
     // a batch the caller will reject, or a matcher with a batch still to collect, is left alone
     if (!mt || mt->pending.active || totals[2] > max_opportunities) return PCUBE_OK;
     // a start that fails here is left for pcube_match_start to report when the batch is run
@@ -1955,11 +1983,8 @@ int pcube_match_finish(int matcher, uint64_t* counters, uint64_t* values, uint64
     int rc = batch_status(*mt, host, pd.ntiles, pd.cap, err);
     int status = PCUBE_OK;
     if (!rc) rc = collect_values(*mt, pd.ntiles, host, pd.cap, pd.list_cap, values, values_cap, &status, listed);
-    // clear status and counters for the next batch now, off its critical path
-    const uint64_t words = mt->status.bytes / 8;
-    hipLaunchKernelGGL(k_fill2_u64, dim3(std::min<uint64_t>(blocks_for(words, 256), 4096)), dim3(256), 0, g.stream,
-                       mt->status.as<uint64_t>(), words, mt->status.as<uint64_t>(), (uint64_t)0);
-    if (hipGetLastError() == hipSuccess) mt->head_zero = words * 8;
+    // status and counters are cleared before the next match: by the next
+    // prepare's bounds kernel, or by the fill the next launch adds itself
     return rc ? rc : status;
 }
 
