@@ -588,8 +588,13 @@ int wheel_enqueue(Wheel& wh, const WheelInputs& in, uint64_t* out, uint64_t cap,
 // The passes of two different wheels (a batch's u and v windows) as three
 // paired launches, so their latency chains overlap; two single passes when
 // the wheels use different instantiations or a window is empty.
+// With prepared (one window per wheel), the compaction also answers its subtile
+// queries and clears prepared->clear; *fused tells whether it did (else the
+// caller searches the bounds itself).
 int wheel_enqueue_pair(Wheel& wa, const WheelInputs& ia, uint64_t* out_a, uint64_t cap_a, uint32_t slot_a, Wheel& wb,
-                       const WheelInputs& ib, uint64_t* out_b, uint64_t cap_b, uint32_t slot_b) {
+                       const WheelInputs& ib, uint64_t* out_b, uint64_t cap_b, uint32_t slot_b,
+                       WheelPrepared* prepared = nullptr, bool* fused = nullptr) {
+    if (fused) *fused = false;
     if (&wa == &wb || ia.nseg == 0 || ib.nseg == 0 || wa.packed != wb.packed || wa.nouter != wb.nouter ||
         ia.nseg + ib.nseg > (1ull << 31)) {
         TRY(wheel_enqueue(wa, ia, out_a, cap_a, slot_a));
@@ -620,6 +625,13 @@ int wheel_enqueue_pair(Wheel& wa, const WheelInputs& ia, uint64_t* out_a, uint64
                       wa.win_info.as<uint64_t>(), direct};
     const WheelOut ob{wb.seg_excl.as<uint32_t>(), wb.chunk_total.as<uint64_t>(), out_b, cap_b,
                       wb.win_info.as<uint64_t>(), direct};
+    if (prepared && ia.n == 1 && ib.n == 1) {
+        prepared->oa = oa;
+        prepared->ob = ob;
+        hipLaunchKernelGGL(k_wheel_compact_prepared, dim3(blocks), dim3(kWheelThreads), 0, g.stream, a, b, *prepared);
+        if (fused) *fused = true;
+        return PCUBE_OK;
+    }
     hipLaunchKernelGGL(k_wheel_compact_pair, dim3(blocks), dim3(kWheelThreads), 0, g.stream, a, oa, b, ob);
     return PCUBE_OK;
 }
@@ -704,6 +716,10 @@ struct Pending {
     uint32_t ntiles = 0;
     int engine = 0;
     uint64_t launch = 0;   // the matcher's launch count right after this batch was enqueued
+    // enqueued before the windows' sizes were known: nu and nv are estimates,
+    // geo the device words that receive the true counts (staged engine only)
+    const uint64_t* geo = nullptr;
+    uint64_t nu_real = 0, nv_real = 0;   // with geo: the true sizes, once the host has seen them
 };
 
 struct Matcher {
@@ -721,6 +737,8 @@ struct Matcher {
     // status words: [0] survivor list length, [1] error bits, [2] unused, [3] most survivor
     // records of any wave of the staged match that overflowed its region since the last clear
     Buffer rows, masks, rho, counters, list, status, cand, items;
+    Buffer geo;                            // true u and v counts of a speculatively enqueued batch
+    uint64_t last_nu = 0, last_nv = 0;     // window sizes of the last prepared batch (the next estimate)
     uint64_t cand_cap = 256;                            // bitset-survivor records per wave of the staged match
     // start, bitsets built, match done, intersection kernel start and end, download done
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -762,7 +780,7 @@ bool allowed_residue(uint32_t q, uint32_t a) {
 // rows with per-subtile valid counts, staged bitsets, cached residues, then
 // one block per (stage chunk, row piece).
 int staged_enqueue(Matcher& mt, MatchArgs m, hipEvent_t ev_pre, hipEvent_t ev_match, hipEvent_t ev_k0,
-                   hipEvent_t ev_k1) {
+                   hipEvent_t ev_k1, const uint64_t* geo = nullptr) {
     m.staged = 1;
     m.nbits = mt.sbits;
     m.ncache = mt.sncache;
@@ -809,6 +827,7 @@ int staged_enqueue(Matcher& mt, MatchArgs m, hipEvent_t ev_pre, hipEvent_t ev_ma
     // with two blocks on every CU, the first-round blocks take the larger share
     pa.first64 = blocks == (uint32_t)(kStagedBlocksPerCU * g.compute_units) ? kFirstRoundWeight : 64u;
     pa.runs = runs;
+    pa.geo = geo;
     uint32_t dir_stride, dir_entries;
     directory_shape(m.nu, dir_stride, dir_entries);
     const uint32_t dynamic_lds = (std::max<uint32_t>(1, dir_entries) + (pa.reach_lds ? npieces : 0)) * 8;
@@ -851,8 +870,10 @@ constexpr uint64_t kMaxWaveCandidates = 1ull << 11;
 int match_enqueue(Matcher& mt, const uint64_t* us, uint64_t nu, const uint64_t* vs, uint64_t nv, uint64_t v_first,
                   uint64_t tile_width, uint32_t ntiles, uint64_t cap, int engine, uint64_t* counters,
                   uint64_t tile_base, uint64_t list_cap, hipEvent_t ev_pre, hipEvent_t ev_match,
-                  hipEvent_t ev_k0 = nullptr, hipEvent_t ev_k1 = nullptr) {
+                  hipEvent_t ev_k0 = nullptr, hipEvent_t ev_k1 = nullptr, const uint64_t* geo = nullptr) {
     if (nu >= (1ull << 32) - 64 || nv >= (1ull << 32)) return fail(PCUBE_INVALID, "batch too large");
+    if (geo && !(engine == 0 && staged_fits(mt, nu, nv)))
+        return fail(PCUBE_INVALID, "a batch of unknown size runs on the staged engine only");
     const uint32_t words = (uint32_t)((nu + 63) / 64);
     TRY(mt.rows.ensure(std::max<uint64_t>(1, nv) * sizeof(Row)));
     if (engine == 0) {
@@ -879,7 +900,7 @@ int match_enqueue(Matcher& mt, const uint64_t* us, uint64_t nu, const uint64_t* 
     m.ng = mt.ng; m.ntiles = ntiles; m.allow_words = mt.allow_words;
     for (int k = 0; k < kMaxBitsetFilters; ++k) m.row_base[k] = mt.row_base[k];
     if (nv == 0) return PCUBE_OK;
-    if (engine == 0 && staged_fits(mt, nu, nv)) return staged_enqueue(mt, m, ev_pre, ev_match, ev_k0, ev_k1);
+    if (engine == 0 && staged_fits(mt, nu, nv)) return staged_enqueue(mt, m, ev_pre, ev_match, ev_k0, ev_k1, geo);
     mt.kernel_span = false;
     if (engine == kUnstagedBitsets) engine = 0;
     // k_rows always computes v mod q for the bitset filters of the matcher
@@ -932,7 +953,7 @@ int match_launch(Matcher& mt) {
     mt.head_zero = 0;
     HIP_TRY(hipEventRecord(e0, g.stream));
     TRY(match_enqueue(mt, (const uint64_t*)pd.us, pd.nu, (const uint64_t*)pd.vs, pd.nv, pd.v_first, pd.tile_width,
-                      pd.ntiles, pd.cap, pd.engine, head + 4, 0, pd.list_cap, e1, e2, k0, k1));
+                      pd.ntiles, pd.cap, pd.engine, head + 4, 0, pd.list_cap, e1, e2, k0, k1, pd.geo));
     if (pd.nv == 0)
         for (hipEvent_t e : {e1, k0, k1, e2}) HIP_TRY(hipEventRecord(e, g.stream));
     HIP_TRY(hipMemcpyAsync(mt.down.p, head, (nc + 4) * 8, hipMemcpyDeviceToHost, g.stream));
@@ -1602,11 +1623,28 @@ int pcube_focus_windows(int focus, uint32_t n, const uint64_t* lows, const uint6
     return PCUBE_OK;
 }
 
+static int match_start(int matcher, uint64_t us, uint64_t nu, uint64_t vs, uint64_t nv, uint64_t v_first,
+                       uint64_t tile_width, uint32_t ntiles, uint64_t cap, int engine, const uint64_t* geo);
+
+// A match enqueued right behind a batch's windows, before their sizes are
+// known: nu and nv are estimates (the launch's sizes; the u window is padded to
+// nu), geo receives the true counts on the device, and launched tells whether
+// the match was enqueued.
+struct Speculation {
+    int matcher;
+    uint64_t first, width, cap;
+    int engine;
+    uint64_t nu, nv;
+    uint64_t* geo;
+    bool launched;
+};
+
 // pcube_prepare, clearing nclear words at clear in the bounds kernel (the
-// matcher's status and counters when the match follows in the same call).
+// matcher's status and counters when the match follows in the same call);
+// with spec, the match is enqueued behind the windows before the wait.
 static int prepare_windows(int focus_u, int focus_v, uint32_t ntiles, const uint64_t* tiles, uint64_t us_out,
                            uint64_t us_cap, uint64_t vs_out, uint64_t vs_cap, uint64_t* nu, uint64_t* nv,
-                           uint64_t* bounds, uint64_t* clear, uint64_t nclear) {
+                           uint64_t* bounds, uint64_t* clear, uint64_t nclear, Speculation* spec = nullptr) {
     Focus* fu = focus_at(focus_u);
     Focus* fv = focus_at(focus_v);
     if (!fu || !fv || fu == fv || !fu->wheel.ready || !fv->wheel.ready || ntiles == 0 || ntiles > kMaxPrepareTiles)
@@ -1643,14 +1681,58 @@ static int prepare_windows(int focus_u, int focus_v, uint32_t ntiles, const uint
     uint32_t cap_u = wheel_slot_cap(fu->wheel), cap_v = wheel_slot_cap(fv->wheel);
     // the bounds kernel writes its results straight into coherent pinned memory
     const uint64_t* r = g_result_host.as<uint64_t>();
+    // the compaction answers the subtile queries itself (one window per wheel):
+    // u queries to r[0, 2n), v queries to r[2n, 4n), the totals and overflow
+    // flags to r[nq .. nq + 3]
+    uint64_t* rw = g_result_host.as<uint64_t>();
+    WheelPrepared wp;
+    memset(&wp, 0, sizeof wp);
+    static_assert(2 * kMaxPrepareTiles <= kMaxWindowQueries, "a wheel answers two queries per subtile");
+    for (uint32_t k = 0; k < 2 * ntiles; ++k) {
+        wp.qa.q[k] = queries.q[k];
+        wp.qb.q[k] = queries.q[2 * ntiles + k];
+    }
+    wp.qa.nq = wp.qb.nq = 2 * ntiles;
+    wp.qa.out = rw;
+    wp.qb.out = rw + 2 * ntiles;
+    wp.qa.total = rw + nq;
+    wp.qb.total = rw + nq + 1;
+    wp.qa.flag = rw + nq + 2;
+    wp.qb.flag = rw + nq + 3;
+    wp.clear = clear;
+    wp.nclear = nclear;
+    if (spec) {
+        wp.qa.dev_total = spec->geo;
+        wp.qb.dev_total = spec->geo + 1;
+        wp.qa.pad_to = std::min(spec->nu, us_cap);
+    }
     for (;;) {
+        bool fused = false;
+        if (spec && spec->launched) {
+            // the windows are being made again (a slot overflowed): so is the match
+            matcher_at(spec->matcher)->pending.active = false;
+            matcher_at(spec->matcher)->head_zero = 0;
+            spec->launched = false;
+        }
         TRY(wheel_enqueue_pair(fu->wheel, in_u, (uint64_t*)us_out, us_cap, cap_u, fv->wheel, in_v, (uint64_t*)vs_out,
-                               vs_cap, cap_v));
-        hipLaunchKernelGGL(k_prepare_bounds, dim3(blocks_for(nq, 256 / kWave)), dim3(256), 0, g.stream,
-                           (const uint64_t*)us_out, (const uint64_t*)fu->wheel.win_info.as<uint64_t>(), us_cap,
-                           (const uint64_t*)vs_out, (const uint64_t*)fv->wheel.win_info.as<uint64_t>(), vs_cap,
-                           queries, ntiles, g_result_host.as<uint64_t>(), clear, nclear);
-        TRY(stream_wait());
+                               vs_cap, cap_v, &wp, &fused));
+        if (!fused)
+            hipLaunchKernelGGL(k_prepare_bounds, dim3(blocks_for(nq, 256 / kWave)), dim3(256), 0, g.stream,
+                               (const uint64_t*)us_out, (const uint64_t*)fu->wheel.win_info.as<uint64_t>(), us_cap,
+                               (const uint64_t*)vs_out, (const uint64_t*)fv->wheel.win_info.as<uint64_t>(), vs_cap,
+                               queries, ntiles, rw, clear, nclear);
+        // the host waits for the bounds only, not for a match queued behind them
+        static hipEvent_t bounds_done = nullptr;
+        if (!bounds_done) HIP_TRY(hipEventCreateWithFlags(&bounds_done, hipEventDisableTiming));
+        HIP_TRY(hipEventRecord(bounds_done, g.stream));
+        if (spec && fused) {
+            // the compaction clears the matcher's status and counters ahead of its match
+            Matcher* smt = matcher_at(spec->matcher);
+            if (clear && clear == smt->status.as<uint64_t>()) smt->head_zero = nclear * 8;
+            spec->launched = match_start(spec->matcher, us_out, spec->nu, vs_out, spec->nv, spec->first, spec->width,
+                                         ntiles, spec->cap, spec->engine, spec->geo) == PCUBE_OK;
+        }
+        TRY(event_wait(bounds_done));
         const bool again_u = r[nq + 2] && cap_u < (uint32_t)kWheelWaveSpan;
         const bool again_v = r[nq + 3] && cap_v < (uint32_t)kWheelWaveSpan;
         if (!again_u && !again_v) break;
@@ -1725,7 +1807,8 @@ int pcube_tile_ranges_many(uint64_t A, uint64_t B, uint64_t lo0, uint64_t lo1, u
 static int prepare_range(int focus_u, int focus_v, uint64_t A, uint64_t B, uint64_t lo0, uint64_t lo1, uint64_t hi0,
                          uint64_t hi1, uint64_t first, uint64_t end, uint64_t width, uint64_t* tiles, uint32_t* ntiles,
                          uint64_t us_out, uint64_t us_cap, uint64_t vs_out, uint64_t vs_cap, uint64_t* nu, uint64_t* nv,
-                         uint64_t* bounds, uint64_t* totals, uint64_t* clear, uint64_t nclear) {
+                         uint64_t* bounds, uint64_t* totals, uint64_t* clear, uint64_t nclear,
+                         Speculation* spec = nullptr) {
     Focus* fu = focus_at(focus_u);
     Focus* fv = focus_at(focus_v);
     if (!fu || !fv) return fail(PCUBE_INVALID, "unknown focus");
@@ -1737,7 +1820,8 @@ static int prepare_range(int focus_u, int focus_v, uint64_t A, uint64_t B, uint6
         return fail(PCUBE_RANGE, "focus window exceeds bounded period count");
     if ((end - 1) / fv->M - first / fv->M + 1 > 4096)
         return fail(PCUBE_RANGE, "focus window exceeds bounded period count");
-    TRY(prepare_windows(focus_u, focus_v, n, tiles, us_out, us_cap, vs_out, vs_cap, nu, nv, bounds, clear, nclear));
+    TRY(prepare_windows(focus_u, focus_v, n, tiles, us_out, us_cap, vs_out, vs_cap, nu, nv, bounds, clear, nclear,
+                        spec));
     if (totals) {
         // bounds hold (u_first, u_end, v_first, v_end) index pairs < 2^32, so every product fits
         uint64_t focus = 0, opportunities = 0, largest = 0;
@@ -1772,17 +1856,57 @@ int pcube_prepare_match(int focus_u, int focus_v, uint64_t A, uint64_t B, uint64
     if (!totals || !launch) return fail(PCUBE_INVALID, "null totals or launch");
     *launch = 0;
     // a matcher with no batch to collect gets its status words and counters
-    // cleared by this batch's bounds kernel, so its match needs no fill launch
+    // cleared by this batch's bounds kernel, so its match needs no fill launch;
+    // once it has seen a batch, the match is enqueued behind the windows sized
+    // from the last batch's windows plus a margin, before their sizes are known
     Matcher* mt = matcher_at(matcher);
-    const bool clears = mt && !mt->pending.active && mt->status.bytes;
-    TRY(prepare_range(focus_u, focus_v, A, B, lo0, lo1, hi0, hi1, first, end, width, tiles, ntiles, us_out, us_cap,
-                      vs_out, vs_cap, nu, nv, bounds, totals, clears ? mt->status.as<uint64_t>() : nullptr,
-                      clears ? mt->status.bytes / 8 : 0));
-    if (clears) mt->head_zero = mt->status.bytes;
+    const bool idle = mt && !mt->pending.active;
+    const bool clears = idle && mt->status.bytes;
+    Speculation spec{};
+    Speculation* sp = nullptr;
+    if (idle && engine == 0 && mt->sbits && mt->last_nu && mt->last_nv && mt->geo.ensure(16) == PCUBE_OK) {
+        spec.matcher = matcher;
+        spec.first = first;
+        spec.width = width;
+        spec.cap = cap;
+        spec.engine = engine;
+        spec.nu = std::min(us_cap, mt->last_nu + mt->last_nu / 32 + 256);
+        spec.nv = std::min(vs_cap, mt->last_nv + mt->last_nv / 32 + 256);
+        spec.geo = mt->geo.as<uint64_t>();
+        sp = &spec;
+    }
+    const int rc = prepare_range(focus_u, focus_v, A, B, lo0, lo1, hi0, hi1, first, end, width, tiles, ntiles, us_out,
+                                 us_cap, vs_out, vs_cap, nu, nv, bounds, totals,
+                                 clears ? mt->status.as<uint64_t>() : nullptr, clears ? mt->status.bytes / 8 : 0, sp);
+    if (rc != PCUBE_OK) {
+        if (spec.launched) {   // a failed prepare leaves no batch to collect
+            mt->pending.active = false;
+            mt->head_zero = 0;
+        }
+        return rc;
+    }
+    if (clears && !spec.launched) mt->head_zero = mt->status.bytes;
+    if (idle) {
+        mt->last_nu = *nu;
+        mt->last_nv = *nv;
+    }
+    if (spec.launched) {
+        Pending& pd = mt->pending;
+        if (*nu <= spec.nu && *nv <= spec.nv && totals[2] <= max_opportunities) {
+            pd.nu_real = *nu;
+            pd.nv_real = *nv;
+            *launch = mt->launches;
+            return PCUBE_OK;
+        }
+        // the estimate fell short, or the batch will be rejected: the enqueued
+        // match runs to its end unread, having written status and counters
+        pd.active = false;
+        mt->head_zero = 0;
+    }
     // a batch the caller will reject, or a matcher with a batch still to collect, is left alone
     if (!mt || mt->pending.active || totals[2] > max_opportunities) return PCUBE_OK;
     // a start that fails here is left for pcube_match_start to report when the batch is run
-    if (pcube_match_start(matcher, us_out, *nu, vs_out, *nv, first, width, *ntiles, cap, engine) == PCUBE_OK)
+    if (match_start(matcher, us_out, *nu, vs_out, *nv, first, width, *ntiles, cap, engine, nullptr) == PCUBE_OK)
         *launch = mt->launches;
     return PCUBE_OK;
 }
@@ -1920,9 +2044,10 @@ int pcube_matcher_destroy(int matcher) {
     return PCUBE_OK;
 }
 
-int pcube_match_start(int matcher, uint64_t us, uint64_t nu, uint64_t vs, uint64_t nv, uint64_t v_first,
-                      uint64_t tile_width, uint32_t ntiles, uint64_t cap, int engine) {
-    Entry lock;
+// pcube_match_start; with geo, nu and nv are upper bounds and geo receives the
+// true counts on the device (Pending::geo).
+static int match_start(int matcher, uint64_t us, uint64_t nu, uint64_t vs, uint64_t nv, uint64_t v_first,
+                       uint64_t tile_width, uint32_t ntiles, uint64_t cap, int engine, const uint64_t* geo) {
     Matcher* mt = matcher_at(matcher);
     if (!mt) return fail(PCUBE_INVALID, "unknown matcher");
     if (ntiles == 0 || ntiles > 65536 || tile_width == 0 || (engine != 0 && engine != 1))
@@ -1930,7 +2055,7 @@ int pcube_match_start(int matcher, uint64_t us, uint64_t nu, uint64_t vs, uint64
     Pending& pd = mt->pending;
     pd = Pending{};
     pd.us = us; pd.nu = nu; pd.vs = vs; pd.nv = nv; pd.v_first = v_first; pd.tile_width = tile_width;
-    pd.ntiles = ntiles; pd.cap = cap; pd.engine = engine;
+    pd.ntiles = ntiles; pd.cap = cap; pd.engine = engine; pd.geo = geo;
     const uint64_t nc = (uint64_t)ntiles * (mt->ng + 3);
     pd.list_cap = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)ntiles * cap, 1ull << 22));
     if (mt->status.bytes < (nc + 4) * 8) mt->head_zero = 0;   // a new buffer holds no zeros yet
@@ -1943,6 +2068,12 @@ int pcube_match_start(int matcher, uint64_t us, uint64_t nu, uint64_t vs, uint64
     pd.launch = mt->launches;
     pd.active = true;
     return PCUBE_OK;
+}
+
+int pcube_match_start(int matcher, uint64_t us, uint64_t nu, uint64_t vs, uint64_t nv, uint64_t v_first,
+                      uint64_t tile_width, uint32_t ntiles, uint64_t cap, int engine) {
+    Entry lock;
+    return match_start(matcher, us, nu, vs, nv, v_first, tile_width, ntiles, cap, engine, nullptr);
 }
 
 int pcube_match_finish(int matcher, uint64_t* counters, uint64_t* values, uint64_t values_cap, double* seconds) {
@@ -1959,8 +2090,16 @@ int pcube_match_finish(int matcher, uint64_t* counters, uint64_t* values, uint64
     TRY(event_wait(mt->ev[5]));
     if (hd[3] > mt->cand_cap) {
         // a bitset-survivor region overflowed: size the regions from its count and run again
-        if (hd[3] > kMaxWaveCandidates) pd.engine = kUnstagedBitsets;
-        else mt->cand_cap = (hd[3] + 63) / 64 * 64;
+        if (hd[3] > kMaxWaveCandidates) {
+            pd.engine = kUnstagedBitsets;
+            if (pd.geo) {   // the unstaged engine takes exact sizes
+                pd.nu = pd.nu_real;
+                pd.nv = pd.nv_real;
+                pd.geo = nullptr;
+            }
+        } else {
+            mt->cand_cap = (hd[3] + 63) / 64 * 64;
+        }
         TRY(match_launch(*mt));
         TRY(event_wait(mt->ev[5]));
     }

@@ -585,11 +585,18 @@ __device__ __forceinline__ void load_directory(const MatchArgs& m, uint64_t* dir
 // Staged engine rows for one 256-row block: as k_rows, plus each subtile's
 // valid-position count, sum of right - left over its rows.  Rows are sorted by
 // v, so a wave's rows span few subtiles: one atomic per (wave, subtile).
-__device__ __forceinline__ void staged_rows_block(const MatchArgs& m, uint32_t block, uint64_t* dir) {
+// Rows from nv_real on (up to m.nv) are padding: an empty u range, no counts.
+__device__ __forceinline__ void staged_rows_block(const MatchArgs& m, uint32_t block, uint64_t* dir,
+                                                  uint32_t nv_real) {
     const uint32_t j = block * 256 + threadIdx.x;
+    if (j >= nv_real && j < m.nv) {
+        Row pad{};
+        pad.v = ~0ull;
+        ((Row*)m.rows)[j] = pad;
+    }
     // v first, then the directory's loads; the parts of the row that need v
     // alone are computed while the directory is in flight
-    const uint64_t v = j < m.nv ? m.vs[j] : 0ull;
+    const uint64_t v = j < nv_real ? m.vs[j] : 0ull;
     uint32_t ndir, stride;
     directory_shape(m.nu, stride, ndir);
     static_assert(kRowDirectory == 256 * 16, "one round of sixteen loads per thread fills the directory");
@@ -600,7 +607,7 @@ __device__ __forceinline__ void staged_rows_block(const MatchArgs& m, uint32_t b
         d[k] = ndir ? m.us[(t < ndir ? t : ndir - 1) * stride] : 0ull;
     }
     RowPrep p;
-    if (j < m.nv) p = row_prep(m, j, v);
+    if (j < nv_real) p = row_prep(m, j, v);
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
         const uint32_t t = 256 * k + threadIdx.x;
@@ -609,7 +616,7 @@ __device__ __forceinline__ void staged_rows_block(const MatchArgs& m, uint32_t b
     __syncthreads();
     uint32_t tile = 0xffffffffu;
     uint64_t count = 0;
-    if (j < m.nv) {
+    if (j < nv_real) {
         const Row row = row_finish<true>(m, j, p, ndir ? dir : nullptr, ndir, stride);
         if (!row.flags) {
             tile = row.tile;

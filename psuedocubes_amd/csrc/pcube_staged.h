@@ -315,9 +315,9 @@ __device__ __forceinline__ void chain_records(const MatchArgs& m, const Staged& 
 // probe); without one, it is exact.  Both ends are nondecreasing over the
 // pieces, and a piece's items may include a chunk beyond each end that none of
 // its rows touches (the match skips such units).
-__device__ __forceinline__ uint64_t piece_end(const MatchArgs& m, uint32_t i) {
+__device__ __forceinline__ uint64_t piece_end(const MatchArgs& m, uint32_t i, uint32_t nv) {
     const uint32_t j0 = (i / 2) * kPieceRows;
-    return m.vs[i & 1 ? min(m.nv, j0 + kPieceRows) - 1 : j0];
+    return j0 < nv ? m.vs[i & 1 ? min(nv, j0 + kPieceRows) - 1 : j0] : 0ull;
 }
 
 __device__ __forceinline__ AboveKey piece_end_key(const MatchArgs& m, uint32_t i, uint64_t v) {
@@ -377,7 +377,16 @@ struct PrefilterArgs {
     uint2* chunks;       // [nstage + 1]: first piece and first item of each chunk
     uint32_t* reach;     // [npieces][2]: the u indices each piece's rows reach (when not in LDS)
     uint2* runs;         // [grid + 1]: first work unit of each match block's run and that unit's chunk
+    // a launch sized before the windows were known: geo[0], geo[1] = the true u
+    // and v counts (at most nu and nv); rows from geo[1] on are padding, and
+    // us holds 2^64 - 1 from geo[0] on.  Null: nu and nv are exact.
+    const uint64_t* geo;
 };
+
+// Rows of the batch that are real (the rest, up to nv, are padding).
+__device__ __forceinline__ uint32_t real_rows(const MatchArgs& m, const PrefilterArgs& pa) {
+    return pa.geo ? (uint32_t)min(pa.geo[1], (uint64_t)m.nv) : m.nv;
+}
 
 // Cost of a chunk change inside a match block's run, in work units: the block
 // waits for the next chunk image about as long as its waves take for this
@@ -410,8 +419,11 @@ __device__ __forceinline__ void staged_plan_block(const MatchArgs& m, const Pref
     // stored, and a binary search over the directory gives the end
     static_assert(kRowDirectory == 256 * 16, "one round of sixteen loads per thread fills the directory");
     const uint32_t ends = 2 * npieces;
+    // pieces past the real rows (padding) reach [nu, nu): nothing, and the ends
+    // stay nondecreasing
+    const uint32_t nv_real = real_rows(m, pa);
     uint64_t v_end = 0;
-    if (threadIdx.x < ends) v_end = piece_end(m, threadIdx.x);
+    if (threadIdx.x < ends) v_end = piece_end(m, threadIdx.x, nv_real);
     uint64_t d[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
@@ -429,7 +441,11 @@ __device__ __forceinline__ void staged_plan_block(const MatchArgs& m, const Pref
     {
         uint32_t* reach = kLds ? (uint32_t*)(dir + (ndir ? ndir : 1)) : pa.reach;
         for (uint32_t i = threadIdx.x; i < ends; i += 256) {
-            const AboveKey key = i == threadIdx.x ? key0 : piece_end_key(m, i, piece_end(m, i));
+            if ((i / 2) * kPieceRows >= nv_real) {
+                reach[i] = m.nu;
+                continue;
+            }
+            const AboveKey key = i == threadIdx.x ? key0 : piece_end_key(m, i, piece_end(m, i, nv_real));
             reach[i] = piece_end_index(m, i, key, ndir ? dir : nullptr, ndir, stride);
         }
     }
@@ -562,7 +578,7 @@ __global__ __launch_bounds__(256) void k_prefilter_staged(MatchArgs m, Prefilter
     }
     b -= 1;
     if (b < pa.rows_blocks) {
-        staged_rows_block(m, b, dir);
+        staged_rows_block(m, b, dir, real_rows(m, pa));
         return;
     }
     b -= pa.rows_blocks;

@@ -262,6 +262,25 @@ __global__ __launch_bounds__(256) void k_wheel_seg_scan_pair(WheelScan a, WheelS
                          second ? blockIdx.x - a.nchunks : blockIdx.x);
 }
 
+// Subtile queries of a one-window compaction (a prepared batch's bounds),
+// answered by the segments themselves: result k = the number of window
+// coordinates below q[k] (searchsorted, side left), written to out[k]; the
+// last segment writes the window's total to *total and the overflow flag to
+// *flag (out, total and flag may be host memory), the total also to
+// *dev_total when set, and fills positions [total, pad_to) of the window with
+// 2^64 - 1, above every coordinate, so a match sized before the total was
+// known sees a sorted window of pad_to coordinates.
+constexpr uint32_t kMaxWindowQueries = 128;
+struct WheelQueries {
+    uint64_t q[kMaxWindowQueries];
+    uint32_t nq;
+    uint64_t* out;
+    uint64_t* total;
+    uint64_t* flag;
+    uint64_t* dev_total;
+    uint64_t pad_to;
+};
+
 // Compaction: one block per segment.  Output offset of the segment = the
 // chunk totals before its chunk + its exclusive offset within the chunk; each
 // wave copies its slot to out, dropping positions at or beyond cap.  With
@@ -272,8 +291,10 @@ __global__ __launch_bounds__(256) void k_wheel_seg_scan_pair(WheelScan a, WheelS
 __device__ __forceinline__ void wheel_compact_block(const WheelArgs& a, const uint32_t* __restrict__ seg_excl,
                                                     const uint64_t* __restrict__ chunk_total,
                                                     uint64_t* __restrict__ out, uint64_t cap,
-                                                    uint64_t* __restrict__ win_info, uint64_t seg, bool direct) {
+                                                    uint64_t* __restrict__ win_info, uint64_t seg, bool direct,
+                                                    const WheelQueries* queries = nullptr) {
     __shared__ uint64_t partial[kWheelThreads / kWave];
+    __shared__ uint32_t below[kWheelThreads / kWave];
     __shared__ uint32_t s_window;
     const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     uint64_t acc = 0;
@@ -313,6 +334,47 @@ __device__ __forceinline__ void wheel_compact_block(const WheelArgs& a, const ui
             const uint64_t total = prefix + cw[0] + cw[1] + cw[2] + cw[3];
             for (uint32_t v = s_window + 1; v <= a.nwin; ++v) win_info[v] = total;
             win_info[a.nwin + 1] = *a.flag;
+            if (queries) {
+                *queries->total = total;
+                *queries->flag = *a.flag;
+                if (queries->dev_total) *queries->dev_total = total;
+            }
+        }
+    }
+    if (!queries) return;
+    if (seg + 1 == a.nseg) {
+        const uint64_t total = prefix + cw[0] + cw[1] + cw[2] + cw[3];
+        for (uint64_t o = total + threadIdx.x; o < min(queries->pad_to, cap); o += kWheelThreads) out[o] = ~0ull;
+    }
+    // the segment's coordinates lie in [lo, hi): from its first candidate to the
+    // next segment's first (the first segment from 0, the last to 2^64 - 1), so
+    // each query falls in exactly one segment, which counts its own survivors
+    // below it (every wave's slot is ascending)
+    const uint64_t lo = seg == 0 ? 0ull : ybase + (a.R1[begin] & kResidueMask);
+    uint64_t hi = ~0ull;
+    if (seg + 1 < a.nseg) {
+        uint64_t t2;
+        uint32_t b2, e2;
+        wheel_segment(w, a.n1, uint32_t(seg + 1 - w.seg_first), t2, b2, e2);
+        hi = t2 * a.M1 + (a.R1[b2] & kResidueMask);
+    }
+    // 64 queries tested at a time, one per lane; every wave finds the same hits
+    for (uint32_t k0 = 0; k0 < queries->nq; k0 += kWave) {
+        const uint32_t k = k0 + lane;
+        const uint64_t mine = k < queries->nq ? queries->q[k] : 0ull;
+        uint64_t hits = __ballot(k < queries->nq && mine >= lo && mine < hi);
+        while (hits) {
+            const uint32_t j = (uint32_t)__builtin_ctzll(hits);
+            hits &= hits - 1;
+            const uint64_t q = readlane64(mine, (int)j);
+            uint32_t c = 0;
+            for (uint32_t i = lane; i < n; i += kWave) c += ybase + slot[i] < q ? 1u : 0u;
+#pragma unroll
+            for (int d = kWave / 2; d >= 1; d >>= 1) c += __shfl_xor(c, d, kWave);
+            if (lane == 0) below[wave] = c;
+            __syncthreads();
+            if (threadIdx.x == 0) queries->out[k0 + j] = prefix + below[0] + below[1] + below[2] + below[3];
+            __syncthreads();
         }
     }
 }
@@ -334,6 +396,15 @@ struct WheelOut {
     uint32_t direct;   // offsets summed by the compaction itself (no scan pass)
 };
 
+// A prepared batch's compaction: both wheels' outputs, their subtile queries,
+// and words to clear on the way (a matcher's status and counters, or none).
+struct WheelPrepared {
+    WheelOut oa, ob;
+    WheelQueries qa, qb;
+    uint64_t* clear;
+    uint64_t nclear;
+};
+
 // Segments per wheel up to which a paired launch skips the scan pass and each
 // compaction block sums the earlier segments' counts (at most 8 loads a thread).
 constexpr uint64_t kDirectPrefixSegments = 2048;
@@ -346,6 +417,21 @@ __global__ __launch_bounds__(kWheelThreads) void k_wheel_compact_pair(WheelArgs 
                             ob.direct != 0);
     else
         wheel_compact_block(a, oa.seg_excl, oa.chunk_total, oa.out, oa.cap, oa.win_info, blockIdx.x, oa.direct != 0);
+}
+
+// k_wheel_compact_pair with the subtile queries answered by the segments
+// (one window per wheel): the separate bounds search of a prepared batch.
+__global__ __launch_bounds__(kWheelThreads) void k_wheel_compact_prepared(WheelArgs a, WheelArgs b,
+                                                                          const WheelPrepared p) {
+    for (uint64_t i = blockIdx.x * (uint64_t)kWheelThreads + threadIdx.x; i < p.nclear;
+         i += (uint64_t)gridDim.x * kWheelThreads)
+        p.clear[i] = 0;
+    if (blockIdx.x >= a.nseg)
+        wheel_compact_block(b, p.ob.seg_excl, p.ob.chunk_total, p.ob.out, p.ob.cap, p.ob.win_info,
+                            blockIdx.x - a.nseg, p.ob.direct != 0, &p.qb);
+    else
+        wheel_compact_block(a, p.oa.seg_excl, p.oa.chunk_total, p.oa.out, p.oa.cap, p.oa.win_info, blockIdx.x,
+                            p.oa.direct != 0, &p.qa);
 }
 
 #define PCUBE_WHEEL(K)                                              \
