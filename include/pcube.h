@@ -54,6 +54,11 @@ int pcube_init(int device, uint64_t memory_cap, double fraction, uint64_t *cap, 
 /* Reserved and peak device bytes of the capped pool (pseudocube/gpu.py:49-52). */
 int pcube_memory(uint64_t *reserved, uint64_t *peak, uint64_t *cap);
 
+/* Address of the pool's peak device bytes, for a caller that checks the cap
+ * before every step without a call (pseudocube/gpu.py:49-52 reads the pool's
+ * total each time); the word is updated as the pool grows. */
+const uint64_t *pcube_memory_peak(void);
+
 /* Device name and compute-unit count. */
 int pcube_device_info(char *name, size_t name_len, int *compute_units);
 

@@ -179,7 +179,7 @@ def prepare(runtime: Runtime, config: Config, af: GPUFocus, bf: GPUFocus,
     if width <= 0 or (end - first + width - 1) // width > 64:
         raise ValueError('experiment allows at most 64 bounded subtiles')
     cap = 1000000
-    us, vs = DeviceArray.empty(cap), DeviceArray.empty(cap)
+    us, vs = DeviceArray.pooled(cap), DeviceArray.pooled(cap)
     sc = _prepare_scratch()
     runtime.check()
     mask = (1 << 64) - 1

@@ -1308,6 +1308,8 @@ int pcube_memory(uint64_t* reserved, uint64_t* peak, uint64_t* cap) {
     return PCUBE_OK;
 }
 
+const uint64_t* pcube_memory_peak(void) { return &g.peak; }
+
 int pcube_device_info(char* name, size_t name_len, int* compute_units) {
     Entry lock;
     if (g.device < 0) return fail(PCUBE_INVALID, "pcube_init has not been called");

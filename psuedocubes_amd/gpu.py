@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import ctypes
 import hashlib
+import threading
 import time
 from math import prod
 import weakref
@@ -21,6 +22,12 @@ from .focus import Config, Focus, Tile
 from .predicate import is_perfect_cube, local3
 
 MASK = (1 << 64) - 1
+
+
+# Free lists of DeviceArray.pooled: at most _POOL_DEPTH buffers per size.
+_POOL: dict[int, list[int]] = {}
+_POOL_LOCK = threading.Lock()
+_POOL_DEPTH = 4
 
 
 class DeviceArray:
@@ -38,6 +45,21 @@ class DeviceArray:
         address = ctypes.c_uint64()
         native.call('pcube_alloc', max(1, int(size)) * 8, ctypes.byref(address))
         return cls(address.value, size, True, size)
+
+    @classmethod
+    def pooled(cls, size: int) -> 'DeviceArray':
+        """An owning array of size words whose memory, once the array is dropped, goes back to a
+        small per-size free list instead of the library (prepare()'s per-batch windows).  Reuse
+        is safe because every device use of a buffer is ordered on the library's one stream."""
+        with _POOL_LOCK:
+            free = _POOL.get(size)
+            address = free.pop() if free else None
+        if address is None:
+            out = cls.empty(size)
+        else:
+            out = cls(address, size, True, size)
+        out.pooled_size = size
+        return out
 
     @classmethod
     def from_host(cls, values: Any) -> 'DeviceArray':
@@ -70,6 +92,14 @@ class DeviceArray:
 
     def __del__(self) -> None:
         if getattr(self, 'owner', False) and native._lib is not None:
+            size = getattr(self, 'pooled_size', None)
+            if size is not None:
+                with _POOL_LOCK:
+                    free = _POOL.setdefault(size, [])
+                    if len(free) < _POOL_DEPTH:
+                        free.append(self.address)
+                        self.owner = False
+                        return
             try:
                 native.lib().pcube_free(self.address)
             except Exception:
@@ -132,14 +162,12 @@ class Runtime:
     def check(self) -> None:
         if time.monotonic() >= self.deadline - 3:
             raise TimeoutError('BUDGET_EXHAUSTED')
-        # called several times per batch: the pool's peak through one prebuilt ctypes call
+        # called several times per batch: the pool's peak read in place (the
+        # library keeps it in one word), with no call
         try:
-            memory_call = self._memory_call
+            peak = self._peak_word
         except AttributeError:
-            peak = ctypes.c_uint64()
-            memory_call = self._memory_call = (native.lib().pcube_memory, (None, ctypes.byref(peak), None), peak)
-        fn, args, peak = memory_call
-        native.check(fn(*args))
+            peak = self._peak_word = ctypes.c_uint64.from_address(native.lib().pcube_memory_peak())
         if peak.value > self.peak_device:
             self.peak_device = peak.value
         if self.peak_device > self.cap:
@@ -446,6 +474,21 @@ class MatchEngine:
         self._pending = (ntiles, cap)
 
     def finish(self, overflow_ok: bool = False) -> tuple[np.ndarray, list[list[int]], list[float]]:
+        counters, limbs, kept, seconds = self.finish_raw(overflow_ok)
+        if kept is None or not any(kept):
+            return counters, [[] for _ in range(len(counters))], seconds
+        words = limbs.tolist()
+        per_tile: list[list[int]] = []
+        offset = 0
+        for n in kept:
+            per_tile.append([words[2 * i] + (words[2 * i + 1] << 64) for i in range(offset, offset + n)])
+            offset += n
+        return counters, per_tile, seconds
+
+    def finish_raw(self, overflow_ok: bool = False) -> tuple[np.ndarray, np.ndarray, list[int] | None, list[float]]:
+        """finish() without Python integers: counters, the emitted values as (low, high) uint64
+        limbs subtile after subtile, the number listed per subtile (None when a subtile
+        overflowed and overflow_ok holds), and the timings."""
         ntiles, cap = self._pending
         stride = self.ng + 3
         if self._counters.size < ntiles * stride:
@@ -463,22 +506,16 @@ class MatchEngine:
         counters = self._counters[:ntiles * stride].reshape(ntiles, stride).copy()
         # overflow_ok covers a subtile that emitted more than cap, which the caller
         # reports itself; any other overflow (or a list/counter mismatch) raises here
+        timings = [seconds[0], seconds[1], seconds[2]]
         if code == native.OVERFLOW and overflow_ok and bool((counters[:, self.ng + 1] > cap).any()):
-            return counters, [[] for _ in range(ntiles)], [seconds[0], seconds[1], seconds[2]]
+            return counters, values[:0], None, timings
         native.check(code, 'pcube_match')
         self.runtime.check()
-        emitted = counters[:, self.ng + 1].tolist()
-        if not any(emitted):   # the usual full p = 619 batch
-            return counters, [[] for _ in emitted], [seconds[0], seconds[1], seconds[2]]
-        per_tile: list[list[int]] = []
-        total = sum(min(n, cap) for n in emitted)
-        limbs = values[:2 * total].tolist()
-        offset = 0
-        for n in emitted:
-            n = min(n, cap)
-            per_tile.append([limbs[2 * i] + (limbs[2 * i + 1] << 64) for i in range(offset, offset + n)])
-            offset += n
-        return counters, per_tile, [seconds[0], seconds[1], seconds[2]]
+        emitted = counters[:, self.ng + 1]
+        if not emitted.any():   # the usual full p = 619 batch
+            return counters, values[:0], [0] * ntiles, timings
+        kept = np.minimum(emitted, cap).tolist()
+        return counters, values[:2 * sum(kept)].copy(), kept, timings
 
     def __del__(self) -> None:
         if native._lib is not None and hasattr(self, 'handle'):

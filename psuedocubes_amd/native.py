@@ -28,6 +28,7 @@ SIGNATURES: dict[str, list[Any]] = {
     'pcube_device_count': [P_int],
     'pcube_init': [c_int, u64, ctypes.c_double, P_u64, P_u64, P_u64],
     'pcube_memory': [P_u64, P_u64, P_u64],
+    'pcube_memory_peak': [],
     'pcube_device_info': [ctypes.c_char_p, ctypes.c_size_t, P_int],
     'pcube_kernel_info': [c_int, ctypes.c_char_p, ctypes.c_size_t, P_int, P_int, P_int],
     'pcube_alloc': [u64, P_u64],
@@ -96,6 +97,7 @@ def lib() -> ctypes.CDLL:
             fn.restype = ctypes.c_int
         handle.pcube_last_error.argtypes = []
         handle.pcube_last_error.restype = ctypes.c_char_p
+        handle.pcube_memory_peak.restype = ctypes.c_void_p
         _lib = handle
     return _lib
 
