@@ -149,8 +149,8 @@ __device__ __forceinline__ void stage_copy(uint4* __restrict__ dst, const uint4*
 struct Staged {
     const uint32_t* allow;
     const Modulus* mods;
-    uint32_t* tilecnt;       // [kTileSlots][stride]: groups 1..stride of subtiles 0..kTileSlots-1
-    uint32_t stride;         // groups per subtile: kItemGroups for the units, ng for the chain
+    uint32_t* tilecnt;       // [kTileSlots][stride]: groups 1..stride of subtiles 0..kTileSlots-1 (units only)
+    uint32_t stride;         // groups per subtile kept in tilecnt
     uint2* queue;            // this wave's survivors: (row index j, u index i)
 };
 
@@ -163,8 +163,23 @@ __device__ __forceinline__ void staged_add(const MatchArgs& m, const Staged& S, 
     else atomicAdd(&m.counters[(uint64_t)tile * (m.ng + 3) + g], (uint64_t)n);
 }
 
+// Add 1 to group g (1-based) of its row's subtile for each lane where pred
+// holds, in the global counters: one atomic for the wave when those lanes share
+// a subtile (the chain's counts are few: a few thousand per batch).
+__device__ __forceinline__ void chain_count(const MatchArgs& m, bool pred, uint32_t tile, uint32_t g, uint32_t lane) {
+    const uint64_t active = __ballot(pred);
+    if (!active) return;
+    const uint32_t lead = __builtin_amdgcn_readlane(tile, __builtin_ctzll(active));
+    if (!__ballot(pred && tile != lead)) {
+        if (lane == 0) atomicAdd(&m.counters[(uint64_t)lead * (m.ng + 3) + g], (uint64_t)__popcll(active));
+    } else if (pred) {
+        atomicAdd(&m.counters[(uint64_t)tile * (m.ng + 3) + g], 1ull);
+    }
+}
+
 // Run up to 64 bitset survivors (lane l takes row j, u index i, live when
-// l < count) through the filters after the bitsets.
+// l < count) through the filters after the bitsets; the filter tables are
+// S.allow and S.mods, the later groups' counts go to the global counters.
 __device__ __forceinline__ void staged_chain(const MatchArgs& m, const Staged& S, uint32_t lane, uint32_t count,
                                              uint32_t j, uint32_t i) {
     bool alive = lane < count;
@@ -186,7 +201,7 @@ __device__ __forceinline__ void staged_chain(const MatchArgs& m, const Staged& S
             r = r >= mod.q ? r - mod.q : r;
             alive = allowed_bit(S.allow, mod.bit + r);
         }
-        if (((k & 3) == 3 || k + 1 == m.nres) && alive) staged_add(m, S, row.tile, k / 4 + 1, 1);
+        if ((k & 3) == 3 || k + 1 == m.nres) chain_count(m, alive, row.tile, k / 4 + 1, lane);
         if (!__ballot(alive)) return;
         ++k;
     }
@@ -210,7 +225,8 @@ __device__ __forceinline__ void staged_chain(const MatchArgs& m, const Staged& S
             }
             const uint64_t fails = __ballot(!pass);
             const uint32_t upto = fails ? (uint32_t)__builtin_ctzll(fails) : (uint32_t)kWave;
-            if (lane < upto && f < m.nres && ((f & 3) == 3 || f + 1 == m.nres)) staged_add(m, S, tile, f / 4 + 1, 1);
+            if (lane < upto && f < m.nres && ((f & 3) == 3 || f + 1 == m.nres))
+                atomicAdd(&m.counters[(uint64_t)tile * (m.ng + 3) + f / 4 + 1], 1ull);
             if (fails) {
                 passed = false;
                 break;
@@ -832,16 +848,16 @@ __device__ __forceinline__ void acc_add(const MatchArgs& m, const Staged& S, Gro
 
 // Persistent staged match, kStagedBlocksPerCU blocks per CU.  The work units
 // (16 per item of the chunk-major list) are split into one contiguous run per
-// block (weighted by dispatch round, run_weight_prefix); the runs of one XCD's
-// blocks are adjacent, so an L2 holds the chunk images its blocks stage.  A run is walked in segments of one chunk: the
-// block stages the chunk's image once and its waves take the segment's units in
-// turn (wave w: units w, w + 8, ... of the segment), each wave on its own,
-// with the next unit's row headers in flight; the block synchronises only when
-// the chunk changes.  Group 1 and 2 counts collect in LDS for the whole run;
-// each wave keeps its survivors in its LDS queue and, 64 at a time, in its own
-// region of the survivor list (item_cap entries).  After the run the block
-// runs its survivors through the residual chain, with the filter tables and the
-// counts of the later groups in the image buffer.  kF = bitset filters; kLast:
+// block (planned by cost, run_weight_prefix); the runs of one XCD's blocks are
+// adjacent, so an L2 holds the chunk images its blocks stage.  A run is walked
+// in segments of one chunk: the block stages the chunk's image once and each
+// wave takes a contiguous share of the segment's units on its own, with the
+// next unit's row headers in flight; the block synchronises only when the
+// chunk changes.  Group 1 and 2 counts collect in LDS for the whole run; each
+// wave writes its survivor records to its own region (item_cap records) and,
+// when its units are done, runs them through the residual chain (filter tables
+// from global memory, later groups' counts to the global counters) while the
+// other waves may still be matching.  kF = bitset filters; kLast:
 // the last residual filter is the last bitset filter (it closes a group).  The
 // waves-per-EU bound keeps the kernel within 128 VGPRs, so both blocks of a CU
 // stay resident.
@@ -921,27 +937,15 @@ void k_match_staged(MatchArgs m, const uint2* chunks, const uint2* runs) {
     // largest count and runs the batch again
     if (fill > m.item_cap && lane == 0) atomicMax((unsigned long long*)m.cand_max, (unsigned long long)fill);
     const uint32_t listed = min(fill, m.item_cap);
-    __syncthreads();   // every wave is done with the image and its unit counts
+    // the residual chain over this wave's survivors, 64 at a time, as soon as the
+    // wave's units are done (the other waves may still be matching): the filter
+    // tables read from global memory, where they stay in the caches
+    const Staged C{m.allow, m.mods, nullptr, m.ng, S.queue};
+    chain_records(m, C, lane, region, listed);
+    __syncthreads();   // every wave is done with its unit counts
     for (uint32_t t = threadIdx.x; t < (uint32_t)(kTileSlots * kItemGroups); t += kStagedThreads) {
         const uint32_t n = tilecnt[t];
         if (n) atomicAdd(&m.counters[(uint64_t)(t / kItemGroups) * (m.ng + 3) + t % kItemGroups + 1], (uint64_t)n);
-    }
-    // the residual chain over each wave's survivors, 64 at a time, with the
-    // filter tables and the later groups' counts in the image buffer
-    uint32_t* allow = (uint32_t*)image;
-    Modulus* mods = (Modulus*)(allow + kMaxAllowWords);
-    uint32_t* chain_counts = (uint32_t*)(mods + kMaxResiduals);
-    // allow and the Modulus table are whole 16-byte units (both buffers are allocated so)
-    stage_copy<kStagedThreads, 1>((uint4*)allow, (const uint4*)m.allow, (m.allow_words + 3) / 4);
-    stage_copy<kStagedThreads, 1>((uint4*)mods, (const uint4*)m.mods, m.nres * (uint32_t)sizeof(Modulus) / 16);
-    for (uint32_t t = threadIdx.x; t < (uint32_t)kTileSlots * m.ng; t += kStagedThreads) chain_counts[t] = 0;
-    __syncthreads();
-    const Staged C{allow, mods, chain_counts, m.ng, S.queue};
-    chain_records(m, C, lane, region, listed);
-    __syncthreads();
-    for (uint32_t t = threadIdx.x; t < (uint32_t)kTileSlots * m.ng; t += kStagedThreads) {
-        const uint32_t c = chain_counts[t];
-        if (c) atomicAdd(&m.counters[(uint64_t)(t / m.ng) * (m.ng + 3) + t % m.ng + 1], (uint64_t)c);
     }
 }
 
