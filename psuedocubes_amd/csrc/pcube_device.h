@@ -604,7 +604,7 @@ __device__ __forceinline__ void staged_rows_block(const MatchArgs& m, uint32_t b
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
         const uint32_t t = 256 * k + threadIdx.x;
-        d[k] = ndir ? m.us[(t < ndir ? t : ndir - 1) * stride] : 0ull;
+        d[k] = t < ndir ? m.us[t * stride] : 0ull;   // only the directory's own entries are loaded
     }
     RowPrep p;
     if (j < nv_real) p = row_prep(m, j, v);

@@ -444,7 +444,7 @@ __device__ __forceinline__ void staged_plan_block(const MatchArgs& m, const Pref
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
         const uint32_t t = 256 * k + threadIdx.x;
-        d[k] = ndir ? m.us[(t < ndir ? t : ndir - 1) * stride] : 0ull;
+        d[k] = t < ndir ? m.us[t * stride] : 0ull;
     }
     AboveKey key0{};
     if (threadIdx.x < ends) key0 = piece_end_key(m, threadIdx.x, v_end);
