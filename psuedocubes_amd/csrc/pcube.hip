@@ -1124,7 +1124,7 @@ struct BatchEngine {
     uint32_t blocks_match = 1024;
     bool use_graph = true;
     BatchArgs args;
-    Buffer win, totals, rows, counters, list, list_count, error, cursor, masks, rho, lus, lvs;
+    Buffer win, totals, rows, counters, list, list_count, error, cursor, masks, rho, lus, lvs, cfg;
     Buffer q_begin, q_len, q_key, q_out;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -1134,7 +1134,7 @@ struct BatchEngine {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         for (Buffer* b : {&win, &totals, &rows, &counters, &list, &list_count, &error, &cursor, &masks, &rho, &lus,
-                          &lvs, &q_begin, &q_len, &q_key, &q_out})
+                          &lvs, &cfg, &q_begin, &q_len, &q_key, &q_out})
             b->release();
     }
 };
@@ -2364,6 +2364,10 @@ int pcube_batch_create(const int* matchers, uint32_t ncfg, int focus_u, int focu
     e->args.list_cap = e->list_cap;
     e->blocks_match = std::max<uint32_t>(1, blocks_per_cu) * (uint32_t)std::max(1, g.compute_units);
     e->use_graph = use_graph != 0;
+    TRY(e->cfg.ensure(sizeof(BatchConfig) * kMaxConfigs));
+    HIP_TRY(hipMemcpyAsync(e->cfg.p, e->args.cfg, sizeof(BatchConfig) * ncfg, hipMemcpyHostToDevice, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    e->args.dcfg = e->cfg.as<BatchConfig>();
     TRY(e->totals.ensure(sizeof(BatchTotals)));
     TRY(e->list.ensure(e->list_cap * 24));
     TRY(e->list_count.ensure(8));

@@ -706,8 +706,8 @@ __device__ __forceinline__ uint64_t wave_transpose(uint64_t x, uint32_t lane) {
 // r0..r0+63 of v mod q) of word w; lane t tests u index 64*w + t and a wave
 // ballot packs the 64 tests.  Row r, bit t: allowed_q[(u*B - r*A) mod q].
 // chunk c of the k-th bitset prime starts at chunk_base[k].
-__device__ __forceinline__ void build_mask_word(const MatchArgs& m, const uint32_t* chunk_base, uint32_t c,
-                                                uint32_t w, uint32_t lane) {
+__device__ __forceinline__ void build_mask_word(const MatchArgs& m, const uint32_t* chunk_base,
+                                                const uint32_t* row_base, uint32_t c, uint32_t w, uint32_t lane) {
     uint32_t k = 0;
     while (k + 1 < m.nbits && c >= chunk_base[k + 1]) ++k;
     const Modulus mod = m.mods[k];
@@ -729,7 +729,7 @@ __device__ __forceinline__ void build_mask_word(const MatchArgs& m, const uint32
         if (lane == rr) mine = word;
         res = res >= mod.a ? res - mod.a : res + mod.q - mod.a;
     }
-    if (lane < rows) ((uint64_t*)m.masks)[(uint64_t)(m.row_base[k] + r0 + lane) * m.words + w] = mine;
+    if (lane < rows) ((uint64_t*)m.masks)[(uint64_t)(row_base[k] + r0 + lane) * m.words + w] = mine;
 }
 
 __global__ __launch_bounds__(256) void k_masks(MatchArgs m, uint32_t chunks_total, uint32_t chunk_base1,
@@ -738,7 +738,7 @@ __global__ __launch_bounds__(256) void k_masks(MatchArgs m, uint32_t chunks_tota
     const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / kWave;
     if (wave >= (uint64_t)chunks_total * m.words) return;
     const uint32_t base[kMaxBitsetFilters] = {0, chunk_base1, chunk_base2, chunk_base3};
-    build_mask_word(m, base, uint32_t(wave / m.words), uint32_t(wave % m.words), lane);
+    build_mask_word(m, base, m.row_base, uint32_t(wave / m.words), uint32_t(wave % m.words), lane);
 }
 
 // rho[i][c] = (u_i * B) mod q for the cached chain filters c = 0..ncache-1.
@@ -1116,14 +1116,18 @@ __device__ __forceinline__ void build_mask_filter_run(const MatchArgs& m, const 
             row[ww] = wave_transpose(live[ww] ? col : 0ull, lane);
         }
         if (r0 + lane < mod.q) {
-            uint64_t* out = (uint64_t*)m.masks + (uint64_t)(m.row_base[k] + r0 + lane) * m.words + w0;
+            // row_base from the configuration (device memory) and constant indices into
+            // row: a run-time index would put the arguments or the row in scratch memory
+            uint64_t* out = (uint64_t*)m.masks + (uint64_t)(c.row_base[k] + r0 + lane) * m.words + w0;
             if (nwords == kMaskRun && ((uintptr_t)out & 15) == 0) {
 #pragma unroll
                 for (uint32_t ww = 0; ww < kMaskRun; ww += 2)
                     *(uint4*)(out + ww) = make_uint4(uint32_t(row[ww]), uint32_t(row[ww] >> 32), uint32_t(row[ww + 1]),
                                                      uint32_t(row[ww + 1] >> 32));
             } else {
-                for (uint32_t ww = 0; ww < nwords; ++ww) out[ww] = row[ww];
+#pragma unroll
+                for (uint32_t ww = 0; ww < kMaskRun; ++ww)
+                    if (ww < nwords) out[ww] = row[ww];
             }
         }
     }
@@ -1161,7 +1165,9 @@ struct BatchArgs {
     U128 B_mu;                 // floor((2^128 - 1) / B)
     U128 lo, hi;
     uint32_t ncfg;
-    BatchConfig cfg[kMaxConfigs];
+    BatchConfig cfg[kMaxConfigs];   // host copy (the kernels index dcfg: a run-time index into
+                                    // kernel arguments would copy them to scratch memory)
+    const BatchConfig* dcfg;        // the same configurations in device memory
 };
 
 // last window w with field(w) <= key (fields ascending over windows)
@@ -1176,7 +1182,7 @@ __device__ __forceinline__ uint32_t find_window(const BatchWindow* win, uint32_t
 }
 
 __device__ __forceinline__ MatchArgs window_args(const BatchArgs& b, const BatchWindow& w) {
-    const BatchConfig& c = b.cfg[w.cfg];
+    const BatchConfig& c = b.dcfg[w.cfg];
     MatchArgs m{};   // every field the batched engine does not set is zero (staged = 0: plain row records)
     m.us = w.us; m.vs = w.vs;
     m.rows = b.rows + w.row_begin;
@@ -1234,13 +1240,14 @@ __global__ __launch_bounds__(256) void k_batch_masks(BatchArgs b) {
         const BatchWindow win = b.win[w];
         const uint64_t next = w + 1 < t.nwin ? b.win[w + 1].mask_begin : t.mask_waves;
         const MatchArgs m = window_args(b, win);
-        const BatchConfig& c = b.cfg[win.cfg];
+        const BatchConfig& c = b.dcfg[win.cfg];
         for (; g < stop && g < next; ++g) {
             const uint64_t local = g - win.mask_begin;
             if (c.transpose)
                 build_mask_filter_run(m, c, uint32_t(local % m.nbits), uint32_t(local / m.nbits) * kMaskRun, lane);
             else
-                build_mask_word(m, c.chunk_base, uint32_t(local / win.words), uint32_t(local % win.words), lane);
+                build_mask_word(m, c.chunk_base, c.row_base, uint32_t(local / win.words), uint32_t(local % win.words),
+                                lane);
         }
         ++w;
     }
@@ -1254,7 +1261,7 @@ __global__ void k_batch_rho(BatchArgs b) {
          g += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t w = find_window(b.win, t.nwin, g, [](const BatchWindow& x) { return x.rho_begin; });
         const BatchWindow win = b.win[w];
-        const BatchConfig& c = b.cfg[win.cfg];
+        const BatchConfig& c = b.dcfg[win.cfg];
         const uint64_t i = g - win.rho_begin;
         const uint64_t u = win.us[i];
         uint32_t packed[kCachedResidues / 2] = {};
@@ -1274,7 +1281,7 @@ __global__ void k_batch_rho(BatchArgs b) {
 __global__ __launch_bounds__(kMatchThreads) void k_batch_match(BatchArgs b) {
     __shared__ MatchSharedT<kMaxConfigs> sh;
     for (uint32_t c = 0; c < b.ncfg; ++c)
-        load_tables(sh, c, b.cfg[c].allow, b.cfg[c].allow_words, b.cfg[c].mods, b.cfg[c].nres);
+        load_tables(sh, c, b.dcfg[c].allow, b.dcfg[c].allow_words, b.dcfg[c].mods, b.dcfg[c].nres);
     for (uint32_t t = threadIdx.x; t < kMatchWaves * kGroupSlots; t += kMatchThreads) (&sh.groups[0][0])[t] = 0;
     __syncthreads();
     const BatchTotals t = *b.totals;
