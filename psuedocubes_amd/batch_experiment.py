@@ -398,9 +398,9 @@ def batch_row(config: Config, filter_groups: list[list[int]], ranges: list[tuple
                     position += number
         all_text = [str(n) for n in all_values]
     # only the columns the records hold: [valid, group_1..group_ng] of each subtile
-    for head, scope in zip(counters[:, :ng + 1].tolist(), scopes):
-        scope['valid_focused_candidates'] = head[0]
-        scope['group_survivors'] = head[1:]
+    for valid, group, scope in zip(counters[:, 0].tolist(), counters[:, 1:ng + 1].tolist(), scopes):
+        scope['valid_focused_candidates'] = valid
+        scope['group_survivors'] = group
     verification_seconds = time.perf_counter() - verification_started
     sums = counters[:, :ng + 1].sum(axis=0, dtype=np.uint64).tolist() if nrows else [0] * (ng + 1)
     groups = sums[1:]
