@@ -183,7 +183,7 @@ def cpu_baseline(config, seconds: float, gpu_rows: list, small: tuple | None, le
         small_seconds = time.perf_counter() - t0
         if row_digest(row) != row_digest(gpu_row):
             mismatches.append('config 1 subtile')
-        configs['config1_subtile'] = dict(range=[str(first), str(end)], seconds=small_seconds,
+        configs['subtile_1e8_v'] = dict(range=[str(first), str(end)], seconds=small_seconds,
                                           value=row['valid_focused_candidates'] / small_seconds,
                                           positions=row['valid_focused_candidates'])
     if lease is not None:
@@ -196,7 +196,7 @@ def cpu_baseline(config, seconds: float, gpu_rows: list, small: tuple | None, le
                 or row['transcript_sha256'] != expected['transcript_sha256'] \
                 or row['valid_focused_candidates'] != lease['valid_focused_candidates']:
             mismatches.append('config 5 lease')
-        configs['config5_lease'] = dict(first=str(lease['first']), batches=row['compute_batches'],
+        configs['lease_256_batches'] = dict(first=str(lease['first']), batches=row['compute_batches'],
                                         seconds=lease_seconds, value=row['valid_focused_candidates'] / lease_seconds,
                                         transcript_sha256=row['transcript_sha256'])
     try:
@@ -205,9 +205,9 @@ def cpu_baseline(config, seconds: float, gpu_rows: list, small: tuple | None, le
                      if line.startswith('Model name'))
     except Exception:
         model = 'unknown'
-    gate = dict(passed=not mismatches, compared=dict(config2_batches=batches + one_batches,
-                                                     config1='config1_subtile' in configs,
-                                                     config5='config5_lease' in configs),
+    gate = dict(passed=not mismatches, compared=dict(full_batches=batches + one_batches,
+                                                     subtile_1e8_v='subtile_1e8_v' in configs,
+                                                     lease_256_batches='lease_256_batches' in configs),
                 fields='per-subtile scope records (SHA-256), group populations, emitted values; '
                        'the lease transcript and output hash', mismatches=mismatches)
     if mismatches:
@@ -217,7 +217,7 @@ def cpu_baseline(config, seconds: float, gpu_rows: list, small: tuple | None, le
                 sample=f'{batches} of this run\'s GPU batches (seeded full p=619 batches: windows + bitset matcher '
                        f'+ counters), {elapsed:.1f} s on {threads} threads; the process may run on '
                        f'{O.available_cpus()} CPUs, os.cpu_count() = {os.cpu_count()} ({model}); the oracle uses at '
-                       f'most {O.THREAD_LIMIT} threads, the CPU share one GPU box grants a process',
+                       f'most {O.THREAD_LIMIT} threads, the CPU share granted to the process that drives one GPU',
                 gate=gate,
                 single_thread=dict(value=one_positions / one_elapsed if one_batches else None, cores=1,
                                    sample=f'{one_batches} more of the run\'s GPU batches, {one_elapsed:.1f} s on '
